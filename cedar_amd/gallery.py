@@ -1,0 +1,162 @@
+"""Test operators and seeded synthetic problems, as numpy arrays in the solver's HBM layout.
+
+Layout: a grid function with nx*ny interior points is a C-contiguous (ny+2, nx+2) float64 array
+(the bytes of the Fortran array Q(nx+2, ny+2)); a 2D stencil is (nstncl, ny+2, nx+2) with planes
+{c, w, s} (5-point) or {c, w, s, sw, nw} (9-point).  Off-diagonal couplings are stored negated,
+couplings to the Dirichlet boundary are zero and all ghost entries are zero.
+
+`poisson`, `diag_diffusion` and `fe` restate the reference gallery (src/2d/gallery.cc:7-113).
+The diffusion problems below are defined here (the reference gallery has no heterogeneous
+operator); the random stream is SplitMix64 so that any language can regenerate it.
+"""
+import numpy as np
+
+C, W, S, SW, NW = 0, 1, 2, 3, 4
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(seed, n):
+    """n successive SplitMix64 outputs from `seed` (uint64 array)."""
+    with np.errstate(over="ignore"):
+        k = np.arange(1, n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + k * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform01(seed, n):
+    """Doubles in [0, 1) from the top 53 bits of SplitMix64."""
+    return (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+
+
+def uniform_pm1(seed, n):
+    return 2.0 * uniform01(seed, n) - 1.0
+
+
+def normal(seed, n):
+    """Standard normals by Box-Muller on pairs of uniforms (first output of each pair)."""
+    u = uniform01(seed, 2 * n).reshape(n, 2)
+    u1 = 1.0 - u[:, 0]  # (0, 1]
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u[:, 1])
+
+
+# ---------------------------------------------------------------------------------------------
+# reference gallery (src/2d/gallery.cc)
+# ---------------------------------------------------------------------------------------------
+def diag_diffusion(nx, ny, dx=1.0, dy=1.0):
+    """5-point -dx u_xx - dy u_yy scaled by h^2 (src/2d/gallery.cc:42-74)."""
+    so = np.zeros((3, ny + 2, nx + 2))
+    hx = 1.0 / (nx + 1)
+    hy = 1.0 / (ny + 1)
+    xh = hy / hx
+    yh = hx / hy
+    # C++ index k  <->  array index k (ghost at 0): s for j in [2, ny], i in [1, nx]
+    so[S, 2:ny + 1, 1:nx + 1] = dy * yh
+    so[W, 1:ny + 1, 2:nx + 1] = dx * xh
+    so[C, 1:ny + 1, 1:nx + 1] = 2 * dx * xh + 2 * dy * yh
+    return so
+
+
+def poisson(nx, ny):
+    """5-point Poisson (src/2d/gallery.cc:7-39)."""
+    so = np.zeros((3, ny + 2, nx + 2))
+    hx = 1.0 / (nx + 1)
+    hy = 1.0 / (ny + 1)
+    xh = hy / hx
+    yh = hx / hy
+    so[S, 2:ny + 1, 1:nx + 1] = 1.0 * yh
+    so[W, 1:ny + 1, 2:nx + 1] = 1.0 * xh
+    so[C, 1:ny + 1, 1:nx + 1] = 2 * xh + 2 * yh
+    return so
+
+
+def fe(nx, ny):
+    """All-ones 9-point finite-element stencil (src/2d/gallery.cc:77-113)."""
+    so = np.zeros((5, ny + 2, nx + 2))
+    so[S, 2:ny + 1, 1:nx + 1] = 1.0
+    so[W, 1:ny + 1, 2:nx + 1] = 1.0
+    so[SW, 2:ny + 1, 2:nx + 1] = 1.0
+    so[NW, 2:ny + 1, 2:nx + 1] = 1.0
+    so[C, 1:ny + 1, 1:nx + 1] = 8.0
+    return so
+
+
+# ---------------------------------------------------------------------------------------------
+# heterogeneous diffusion problems
+# ---------------------------------------------------------------------------------------------
+def block_lognormal(ncx, ncy, block, sigma, seed, contrast=1e4):
+    """Cell coefficients k (shape (ncy, ncx)), constant on block x block cells, log k ~ N(0, sigma^2)
+    clipped to +-ln(contrast)/2 so that max k / min k <= contrast."""
+    nbx = -(-ncx // block)
+    nby = -(-ncy // block)
+    g = sigma * normal(seed, nbx * nby).reshape(nby, nbx)
+    lim = 0.5 * np.log(contrast)
+    g = np.clip(g, -lim, lim)
+    kb = np.exp(g)
+    return np.repeat(np.repeat(kb, block, axis=0), block, axis=1)[:ncy, :ncx].copy()
+
+
+def q1_diffusion(kcell):
+    """9-point bilinear (Q1) finite-element stiffness of -div(k grad u) on a uniform grid.
+
+    kcell has shape (ny+1, nx+1): cell (cj, ci) has corners at array indices
+    (cj, ci), (cj, ci+1), (cj+1, ci), (cj+1, ci+1) of the (ny+2, nx+2) node grid (row/col 0 and
+    ny+1/nx+1 are the Dirichlet boundary).  Element matrix k/6 [4 -1 -2 -1; ...]: an edge gets
+    k/6 from each of its two cells, a diagonal pair k/3, the centre 2k/3 per cell.
+    """
+    ncy, ncx = kcell.shape
+    ny, nx = ncy - 1, ncx - 1
+    so = np.zeros((5, ny + 2, nx + 2))
+    k = kcell
+    # node (j, i) interior: 1..n.  Cells touching node (j,i): (j-1,i-1),(j-1,i),(j,i-1),(j,i)
+    so[C, 1:ny + 1, 1:nx + 1] = (2.0 / 3.0) * (k[:-1, :-1] + k[:-1, 1:] + k[1:, :-1] + k[1:, 1:])
+    # w at node (j,i): edge to (j, i-1), cells (j-1, i-1) and (j, i-1); zero when i-1 is boundary
+    w = (k[:-1, :-1] + k[1:, :-1]) / 6.0           # indexed by node (j,i) for j,i in 1..n
+    so[W, 1:ny + 1, 2:nx + 1] = w[:, 1:]
+    # s at node (j,i): edge to (j-1, i), cells (j-1, i-1) and (j-1, i); zero when j-1 is boundary
+    s = (k[:-1, :-1] + k[:-1, 1:]) / 6.0
+    so[S, 2:ny + 1, 1:nx + 1] = s[1:, :]
+    # sw at node (j,i): pair (j,i)-(j-1,i-1) in cell (j-1,i-1); nw at node (j,i): pair
+    # (j-1,i)-(j,i-1) in cell (j-1,i-1).  Both zero when either end is on the boundary.
+    d = k[:-1, :-1] / 3.0
+    so[SW, 2:ny + 1, 2:nx + 1] = d[1:, 1:]
+    so[NW, 2:ny + 1, 2:nx + 1] = d[1:, 1:]
+    # nw at (j,i) links (j,i-1) and (j-1,i): (j-1,i) is interior for j>=2, (j,i-1) for i>=2;
+    # also both must be interior rows/cols, already ensured by the slices above.
+    return so
+
+
+def rhs_uniform(nx, ny, seed):
+    b = np.zeros((ny + 2, nx + 2))
+    b[1:ny + 1, 1:nx + 1] = uniform_pm1(seed, nx * ny).reshape(ny, nx)
+    return b
+
+
+def heterogeneous_9pt(n, seed, block=16, sigma=2.0, contrast=1e4):
+    """-div(k grad u) = f on n x n interior nodes: lognormal k on block x block cells, f ~ U[-1,1]."""
+    k = block_lognormal(n + 1, n + 1, block, sigma, seed, contrast)
+    so = q1_diffusion(k)
+    b = rhs_uniform(n, n, seed + 1000003)
+    return so, b, k
+
+
+def poisson_uniform_rhs(n, seed=1):
+    """poisson(n, n) with f ~ U[-1, 1] (seeded)."""
+    return poisson(n, n), rhs_uniform(n, n, seed)
+
+
+def readme_problem(n=400):
+    """Cedar's example problem: poisson(n, n), b = 8 pi^2 sin(2 pi x) sin(2 pi y) h^2
+    (examples/basic-2d-ser/poisson.cc:15-37); returns (so, b, exact)."""
+    so = poisson(n, n)
+    h = 1.0 / (n + 1)
+    idx = np.arange(n + 2) * h
+    X, Y = np.meshgrid(idx, idx)  # X[j, i] = i*h, Y[j, i] = j*h
+    b = np.zeros((n + 2, n + 2))
+    rhs = 8 * (np.pi * np.pi) * np.sin(2 * np.pi * X) * np.sin(2 * np.pi * Y)
+    b[1:n + 1, 1:n + 1] = (rhs * (h * h))[1:n + 1, 1:n + 1]
+    exact = np.sin(2 * np.pi * X) * np.sin(2 * np.pi * Y)
+    return so, b, exact

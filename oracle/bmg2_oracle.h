@@ -1,0 +1,92 @@
+/*
+ * FP64 CPU oracle for the 2D BoxMG solver path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of Cedar's serial 2D kernels and of
+ * its multilevel driver, written to follow the reference's floating-point operation order.  It
+ * is used by tests/ (as the parity checker), by __graft_entry__.smoke() (as the checker) and by
+ * bench.py's cpu_baseline leg (timed as the CPU baseline).  Nothing in cedar_amd/ may link,
+ * load or call it.
+ *
+ * Conventions (same as the reference):
+ *   - arrays are Fortran order, first index fastest, one ghost layer: ii = nx+2, jj = ny+2;
+ *   - a stencil is nstncl contiguous planes of ii*jj values: 3 planes {c,w,s} (5-point) or
+ *     5 planes {c,w,s,sw,nw} (9-point) (include/cedar/2d/base_types.h:4-14);
+ *   - off-diagonal couplings are stored negated (positive for an M-matrix);
+ *   - the prolongation CI has 8 planes {L,R,A,B,SW,NW,NE,SE} on the coarse grid
+ *     (src/2d/ftn/BMG_stencils_f90.h:35-36).
+ *
+ * Parity status of each function (how it is pinned) is listed in oracle/README.md.
+ */
+#ifndef CEDAR_ORACLE_BMG2_H
+#define CEDAR_ORACLE_BMG2_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_DOWN = 0, ORC_UP = 1 };
+enum { ORC_RELAX_POINT = 0, ORC_RELAX_LINE_X = 1, ORC_RELAX_LINE_Y = 2, ORC_RELAX_LINE_XY = 3 };
+
+/* ---- kernels (2D, FP64, Dirichlet / BMG_BCs_definite) ---- */
+void orc2_residual(const double *so, int nstncl, const double *qf, const double *q,
+                   double *res, int ii, int jj);
+void orc2_setup_recip(const double *so, double *sor, int ii, int jj);
+void orc2_relax_gs(const double *so, int nstncl, const double *qf, double *q,
+                   const double *sor, int ii, int jj, int updown);
+int  orc2_setup_lines_x(const double *so, double *sor, int ii, int jj);
+int  orc2_setup_lines_y(const double *so, double *sor, int ii, int jj);
+void orc2_relax_lines_x(const double *so, int nstncl, const double *qf, double *q,
+                        const double *sor, int ii, int jj, int updown);
+void orc2_relax_lines_y(const double *so, int nstncl, const double *qf, double *q,
+                        const double *sor, double *b, int ii, int jj, int updown);
+void orc2_restrict(const double *ci, const double *q, double *qc,
+                   int iif, int jjf, int iic, int jjc);
+void orc2_interp_add(double *q, const double *qc, double *res, const double *so, int nstncl,
+                     const double *ci, int iic, int jjc, int iif, int jjf);
+void orc2_setup_interp(const double *so, int nstncl, double *ci,
+                       int iif, int jjf, int iic, int jjc);
+void orc2_setup_galerkin(const double *so, int nstncl, const double *ci, double *soc,
+                         int iif, int jjf, int iic, int jjc);
+int  orc2_setup_cg(const double *so, int nstncl, double *abd, int ii, int jj);
+int  orc2_solve_cg(double *q, const double *qf, const double *abd, double *bbd, int ii, int jj);
+double orc2_norm(const double *x, int ii, int jj);
+
+/* ---- textbook LAPACK restatements (reference LAPACK 3.x algorithms) ---- */
+int  orc_dpttrf(int n, double *d, double *e);
+void orc_dpttrs(int n, const double *d, const double *e, double *b);
+int  orc_dpbtrf_u(int n, int kd, double *ab, int ldab);
+void orc_dpbtrs_u(int n, int kd, const double *ab, int ldab, double *b);
+
+/* ---- multilevel driver ---- */
+typedef struct {
+    int relax;        /* ORC_RELAX_* */
+    int nrelax_pre;   /* solver.cycle.nrelax-pre, default 2 */
+    int nrelax_post;  /* solver.cycle.nrelax-post, default 1 */
+    int num_levels;   /* solver.num-levels, -1 = computed */
+    int maxiter;      /* solver.max-iter */
+    double tol;       /* solver.tol */
+    int min_coarse;   /* solver.min_coarse, default 3 */
+} orc2_settings;
+
+typedef struct orc2_solver orc2_solver;
+
+int  orc2_num_levels(int nx, int ny, int min_coarse);
+orc2_solver *orc2_solver_create(const double *so_fine, int nstncl, int nx, int ny,
+                                const orc2_settings *s);
+int  orc2_solver_nlevels(const orc2_solver *slv);
+void orc2_solver_level_dims(const orc2_solver *slv, int lvl, int *nx, int *ny);
+/* Copies of the level data built by setup (coarse operator, CI, relaxation factors). */
+void orc2_solver_get_level(const orc2_solver *slv, int lvl, double *so, double *ci,
+                           double *sor0, double *sor1);
+const double *orc2_solver_abd(const orc2_solver *slv, int *nabd1, int *nabd2);
+/* One V-cycle on (x, b) at the finest level. */
+void orc2_solver_vcycle(orc2_solver *slv, double *x, const double *b);
+/* Full solve from the x given (x0 = 0 for Cedar's solve(b)); writes rho_i into hist,
+   returns the number of cycles run. */
+int  orc2_solver_solve(orc2_solver *slv, const double *b, double *x, double *hist, int maxhist);
+void orc2_solver_destroy(orc2_solver *slv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
