@@ -1,0 +1,17 @@
+"""cedar_amd: Cedar's BoxMG solver path on one MI355X (gfx950).
+
+The numerical work runs in hand-written HIP kernels behind the C ABI in include/cedar_hip.h
+(libcedar_hip.so); Python only allocates device memory through torch and passes pointers.
+Importing the package loads the library and fails if it is missing -- there is no CPU path.
+"""
+from . import _abi
+from ._abi import CedarHipError, RELAX, PRECISION  # noqa: F401
+
+_abi.load()
+
+__version__ = "0.1.0"
+
+from .cdr2 import (  # noqa: E402,F401
+    GridFunc, StencilOp, Solver, residual, residual_norm, setup_recip, relax_gs, setup_lines,
+    relax_lines, restrict, interp_add, setup_interp, setup_galerkin, setup_cg, solve_cg,
+)

@@ -1,0 +1,128 @@
+"""ctypes binding of the C ABI in include/cedar_hip.h (libcedar_hip.so, built in-tree).
+
+Importing this module loads the HIP library and raises if it is missing: there is no CPU
+fallback anywhere in cedar_amd.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libcedar_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "cedar_hip.h")
+
+CDR_OK = 0
+CDR_DOWN, CDR_UP = 0, 1
+RELAX = {"point": 0, "line-x": 1, "line-y": 2, "line-xy": 3}
+PRECISION = {"f64": 0, "mixed": 1}
+
+_i = ctypes.c_int
+_d = ctypes.c_double
+_vp = ctypes.c_void_p
+_sz = ctypes.c_size_t
+
+
+class Settings(ctypes.Structure):
+    """cdr_solver_settings"""
+    _fields_ = [("relaxation", _i), ("nrelax_pre", _i), ("nrelax_post", _i), ("num_levels", _i),
+                ("maxiter", _i), ("tol", _d), ("min_coarse", _i), ("precision", _i)]
+
+
+# name -> (restype, argtypes); pointers to device memory are passed as c_void_p
+_P = _vp
+PROTOTYPES = {
+    "cdr_hip_status_string": (ctypes.c_char_p, [_i]),
+    "cdr_hip_version": (ctypes.c_char_p, []),
+    "cdr_hip_norm_work2d": (_sz, [_i, _i]),
+    "cdr_hip_residual2d_f64": (_i, [_P, _i, _P, _P, _P, _i, _i, _vp]),
+    "cdr_hip_residual2d_f32": (_i, [_P, _i, _P, _P, _P, _i, _i, _vp]),
+    "cdr_hip_residual_norm2d_f64": (_i, [_P, _i, _P, _P, _P, _i, _i, _P, _P, _vp]),
+    "cdr_hip_sumsq2d_f64": (_i, [_P, _i, _i, _P, _P, _vp]),
+    "cdr_hip_setup_recip2d_f64": (_i, [_P, _P, _i, _i, _i, _vp]),
+    "cdr_hip_relax_gs2d_f64": (_i, [_P, _i, _P, _P, _P, _i, _i, _i, _vp]),
+    "cdr_hip_relax_gs2d_f32": (_i, [_P, _i, _P, _P, _P, _i, _i, _i, _vp]),
+    "cdr_hip_setup_lines_x2d_f64": (_i, [_P, _P, _i, _i, _i, _P, _vp]),
+    "cdr_hip_setup_lines_y2d_f64": (_i, [_P, _P, _i, _i, _i, _P, _vp]),
+    "cdr_hip_relax_lines_x2d_f64": (_i, [_P, _i, _P, _P, _P, _P, _i, _i, _i, _vp]),
+    "cdr_hip_relax_lines_y2d_f64": (_i, [_P, _i, _P, _P, _P, _P, _i, _i, _i, _vp]),
+    "cdr_hip_relax_lines_x2d_f32": (_i, [_P, _i, _P, _P, _P, _P, _i, _i, _i, _vp]),
+    "cdr_hip_relax_lines_y2d_f32": (_i, [_P, _i, _P, _P, _P, _P, _i, _i, _i, _vp]),
+    "cdr_hip_restrict2d_f64": (_i, [_P, _P, _P, _i, _i, _i, _i, _vp]),
+    "cdr_hip_restrict2d_f32": (_i, [_P, _P, _P, _i, _i, _i, _i, _vp]),
+    "cdr_hip_interp_add2d_f64": (_i, [_P, _P, _P, _P, _i, _P, _i, _i, _i, _i, _vp]),
+    "cdr_hip_interp_add2d_f32": (_i, [_P, _P, _P, _P, _i, _P, _i, _i, _i, _i, _vp]),
+    "cdr_hip_setup_interp2d_f64": (_i, [_P, _i, _P, _i, _i, _i, _i, _vp]),
+    "cdr_hip_setup_galerkin2d_f64": (_i, [_P, _i, _P, _P, _i, _i, _i, _i, _vp]),
+    "cdr_hip_setup_cg2d_f64": (_i, [_P, _i, _P, _i, _i, _P, _vp]),
+    "cdr_hip_solve_cg2d_f64": (_i, [_P, _P, _P, _P, _i, _i, _vp]),
+    "cdr_hip_solve_cg2d_f32": (_i, [_P, _P, _P, _P, _i, _i, _vp]),
+    "cdr_hip_default_settings": (None, [ctypes.POINTER(Settings)]),
+    "cdr_hip_solver2d_create": (_i, [_P, _i, _i, _i, ctypes.POINTER(Settings), _vp,
+                                     ctypes.POINTER(_vp)]),
+    "cdr_hip_solver2d_solve": (_i, [_vp, _P, _P, ctypes.POINTER(_d), _i, ctypes.POINTER(_i)]),
+    "cdr_hip_solver2d_vcycle": (_i, [_vp, _P, _P]),
+    "cdr_hip_solver2d_nlevels": (_i, [_vp]),
+    "cdr_hip_solver2d_level": (_i, [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i),
+                                    ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                    ctypes.POINTER(_vp)]),
+    "cdr_hip_solver2d_times": (_i, [_vp, ctypes.POINTER(_d), ctypes.POINTER(_d)]),
+    "cdr_hip_solver2d_destroy": (None, [_vp]),
+}
+
+
+class CedarHipError(RuntimeError):
+    def __init__(self, status, what):
+        self.status = status
+        super().__init__(f"{what}: {status_string(status)} (status {status})")
+
+
+_lib = None
+
+
+def _hip_runtimes():
+    """Distinct libamdhip64 files mapped into this process."""
+    paths = set()
+    try:
+        with open("/proc/self/maps") as f:
+            for ln in f:
+                if "libamdhip64" in ln:
+                    paths.add(os.path.realpath(ln.split()[-1]))
+    except OSError:
+        pass
+    return paths
+
+
+def load():
+    """Load libcedar_hip.so (raises if it has not been built).
+
+    torch is imported first so that the library binds to the HIP runtime torch already uses
+    (same soname): device pointers, streams and events are then shared by one runtime.
+    """
+    global _lib
+    if _lib is None:
+        import torch  # noqa: F401  (loads the process's HIP runtime before ours)
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"cedar_amd: {LIB_PATH} is missing; build it with "
+                              "`python -m cedar_amd.build` (no CPU fallback exists)")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in PROTOTYPES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        rts = _hip_runtimes()
+        if len(rts) > 1:
+            raise ImportError(f"cedar_amd: two HIP runtimes are loaded ({sorted(rts)}); import "
+                              "torch before any other HIP library")
+        _lib = lib
+    return _lib
+
+
+def status_string(status):
+    return load().cdr_hip_status_string(status).decode()
+
+
+def call(name, *args):
+    """Call an int-returning entry point and raise CedarHipError on a non-zero status."""
+    st = getattr(load(), name)(*args)
+    if st != CDR_OK:
+        raise CedarHipError(st, name)
+    return st

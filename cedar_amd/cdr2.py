@@ -1,0 +1,298 @@
+"""Python face of the 2D path: Cedar's cdr2 objects on device memory and the kernel calls.
+
+`StencilOp` / `GridFunc` hold torch tensors on the GPU in the solver's layout (a grid function
+with nx*ny interior points is a (ny+2, nx+2) tensor = Fortran Q(nx+2, ny+2); a stencil is
+(nstncl, ny+2, nx+2)).  Every function below is one C-ABI call on torch's current stream.
+`Solver` mirrors cdr2::solver<fsten>: setup in the constructor, `solve(b)` from x0 = 0
+(include/cedar/2d/solver.h:55-61, include/cedar/multilevel.h:268-298).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _abi
+
+DOWN, UP = _abi.CDR_DOWN, _abi.CDR_UP
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _p(t):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("cedar_amd works on device tensors only")
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _sfx(t):
+    if t.dtype == torch.float64:
+        return "f64"
+    if t.dtype == torch.float32:
+        return "f32"
+    raise TypeError(f"unsupported dtype {t.dtype}")
+
+
+class GridFunc:
+    """cdr2::grid_func on the device: (ny+2, nx+2) tensor with one ghost layer."""
+
+    def __init__(self, data):
+        self.data = data
+
+    @classmethod
+    def zeros(cls, nx, ny, dtype=torch.float64, device="cuda"):
+        return cls(torch.zeros((ny + 2, nx + 2), dtype=dtype, device=device))
+
+    @classmethod
+    def from_numpy(cls, a, dtype=torch.float64, device="cuda"):
+        return cls(torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(device))
+
+    @property
+    def ii(self):
+        return self.data.shape[1]
+
+    @property
+    def jj(self):
+        return self.data.shape[0]
+
+    def numpy(self):
+        return self.data.detach().cpu().numpy()
+
+
+class StencilOp:
+    """cdr2::stencil_op<five_pt|nine_pt> on the device: (nstncl, ny+2, nx+2)."""
+
+    def __init__(self, data):
+        if data.dim() != 3 or data.shape[0] not in (3, 5):
+            raise ValueError("stencil must be (3|5, ny+2, nx+2)")
+        self.data = data
+
+    @classmethod
+    def from_numpy(cls, a, dtype=torch.float64, device="cuda"):
+        return cls(torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(device))
+
+    @property
+    def nstncl(self):
+        return self.data.shape[0]
+
+    @property
+    def ii(self):
+        return self.data.shape[2]
+
+    @property
+    def jj(self):
+        return self.data.shape[1]
+
+
+def _t(x):
+    return x.data if isinstance(x, (GridFunc, StencilOp)) else x
+
+
+# ---------------------------------------------------------------------------------------------
+# kernels (arrays are torch tensors or GridFunc/StencilOp)
+# ---------------------------------------------------------------------------------------------
+def residual(so, qf, q, res):
+    so, qf, q, res = map(_t, (so, qf, q, res))
+    _abi.call(f"cdr_hip_residual2d_{_sfx(q)}", _p(so), so.shape[0], _p(qf), _p(q), _p(res),
+              q.shape[1], q.shape[0], _stream())
+    return res
+
+
+def residual_norm(so, qf, q, res):
+    """Residual into res; returns ||r||_2 over the interior (synchronises)."""
+    so, qf, q, res = map(_t, (so, qf, q, res))
+    ii, jj = q.shape[1], q.shape[0]
+    work = torch.empty(max(_abi.load().cdr_hip_norm_work2d(ii, jj), 1), dtype=torch.float64,
+                       device=q.device)
+    out = torch.empty(1, dtype=torch.float64, device=q.device)
+    _abi.call("cdr_hip_residual_norm2d_f64", _p(so), so.shape[0], _p(qf), _p(q), _p(res), ii, jj,
+              _p(work), _p(out), _stream())
+    return float(out.sqrt().item())
+
+
+def setup_recip(so, sor):
+    so, sor = _t(so), _t(sor)
+    _abi.call("cdr_hip_setup_recip2d_f64", _p(so), _p(sor), so.shape[2], so.shape[1],
+              so.shape[0], _stream())
+    return sor
+
+
+def relax_gs(so, qf, q, sor, updown):
+    so, qf, q, sor = map(_t, (so, qf, q, sor))
+    _abi.call(f"cdr_hip_relax_gs2d_{_sfx(q)}", _p(so), so.shape[0], _p(qf), _p(q), _p(sor),
+              q.shape[1], q.shape[0], int(updown), _stream())
+    return q
+
+
+def setup_lines(so, sor, direction):
+    so, sor = _t(so), _t(sor)
+    info = torch.zeros(1, dtype=torch.int32, device=so.device)
+    _abi.call(f"cdr_hip_setup_lines_{direction}2d_f64", _p(so), _p(sor), so.shape[2],
+              so.shape[1], so.shape[0], _p(info), _stream())
+    return sor
+
+
+def relax_lines(so, qf, q, sor, updown, direction):
+    so, qf, q, sor = map(_t, (so, qf, q, sor))
+    _abi.call(f"cdr_hip_relax_lines_{direction}2d_{_sfx(q)}", _p(so), so.shape[0], _p(qf), _p(q),
+              _p(sor), None, q.shape[1], q.shape[0], int(updown), _stream())
+    return q
+
+
+def restrict(ci, q, qc):
+    ci, q, qc = map(_t, (ci, q, qc))
+    _abi.call(f"cdr_hip_restrict2d_{_sfx(q)}", _p(ci), _p(q), _p(qc), q.shape[1], q.shape[0],
+              qc.shape[1], qc.shape[0], _stream())
+    return qc
+
+
+def interp_add(q, qc, res, so, ci):
+    q, qc, res, so, ci = map(_t, (q, qc, res, so, ci))
+    _abi.call(f"cdr_hip_interp_add2d_{_sfx(q)}", _p(q), _p(qc), _p(res), _p(so), so.shape[0],
+              _p(ci), qc.shape[1], qc.shape[0], q.shape[1], q.shape[0], _stream())
+    return q
+
+
+def setup_interp(so, ci):
+    so, ci = _t(so), _t(ci)
+    _abi.call("cdr_hip_setup_interp2d_f64", _p(so), so.shape[0], _p(ci), so.shape[2], so.shape[1],
+              ci.shape[2], ci.shape[1], _stream())
+    return ci
+
+
+def setup_galerkin(so, ci, soc):
+    so, ci, soc = map(_t, (so, ci, soc))
+    _abi.call("cdr_hip_setup_galerkin2d_f64", _p(so), so.shape[0], _p(ci), _p(soc), so.shape[2],
+              so.shape[1], ci.shape[2], ci.shape[1], _stream())
+    return soc
+
+
+def setup_cg(so):
+    """Returns the band factor ABD as an (nx*ny, ii) device tensor (Fortran ABD(ii, n))."""
+    so = _t(so)
+    ii, jj = so.shape[2], so.shape[1]
+    abd = torch.zeros(((ii - 2) * (jj - 2), ii), dtype=torch.float64, device=so.device)
+    info = torch.zeros(1, dtype=torch.int32, device=so.device)
+    _abi.call("cdr_hip_setup_cg2d_f64", _p(so), so.shape[0], _p(abd), ii, jj, _p(info), _stream())
+    return abd
+
+
+def solve_cg(abd, qf, q):
+    qf, q = _t(qf), _t(q)
+    bbd = torch.zeros(abd.shape[0], dtype=torch.float64, device=q.device)
+    _abi.call(f"cdr_hip_solve_cg2d_{_sfx(q)}", _p(q), _p(qf), _p(abd), _p(bbd), q.shape[1],
+              q.shape[0], _stream())
+    return q
+
+
+# ---------------------------------------------------------------------------------------------
+# solver
+# ---------------------------------------------------------------------------------------------
+class Solver:
+    """cdr2::solver<fsten>(so, conf) on the GPU.
+
+    Settings use Cedar's config keys and defaults (src/multilevel_settings.cc:15-61):
+    relaxation ('point' | 'line-x' | 'line-y' | 'line-xy'), nrelax_pre=2, nrelax_post=1,
+    num_levels=-1, maxiter=10, tol=1e-8, min_coarse=3; precision 'f64' (Cedar's algorithm) or
+    'mixed' (FP32 V-cycle preconditioning FP64 iterative refinement).
+    """
+
+    def __init__(self, so, relaxation="point", nrelax_pre=2, nrelax_post=1, num_levels=-1,
+                 maxiter=10, tol=1e-8, min_coarse=3, precision="f64"):
+        so = _t(so) if not isinstance(so, np.ndarray) else torch.as_tensor(so).cuda()
+        if so.dtype != torch.float64:
+            so = so.to(torch.float64)
+        self.so = so.contiguous()  # held by reference for the solver's lifetime
+        self.maxiter = maxiter
+        s = _abi.Settings(_abi.RELAX[relaxation], nrelax_pre, nrelax_post, num_levels, maxiter,
+                          tol, min_coarse, _abi.PRECISION[precision])
+        h = ctypes.c_void_p()
+        _abi.call("cdr_hip_solver2d_create", _p(self.so), self.so.shape[0], self.so.shape[2] - 2,
+                  self.so.shape[1] - 2, ctypes.byref(s), _stream(), ctypes.byref(h))
+        self._h = h
+
+    @property
+    def nlevels(self):
+        return _abi.load().cdr_hip_solver2d_nlevels(self._h)
+
+    def level(self, lvl):
+        """(nx, ny, so_ptr, ci_ptr, sor0_ptr, sor1_ptr) of a level (device pointers)."""
+        nx, ny = ctypes.c_int(), ctypes.c_int()
+        ptrs = [ctypes.c_void_p() for _ in range(4)]
+        _abi.call("cdr_hip_solver2d_level", self._h, lvl, ctypes.byref(nx), ctypes.byref(ny),
+                  *[ctypes.byref(p) for p in ptrs])
+        return (nx.value, ny.value) + tuple(p.value for p in ptrs)
+
+    def level_arrays(self, lvl):
+        """Host copies (so, ci, sor0, sor1) of a level's FP64 data."""
+        nx, ny, so, ci, s0, s1 = self.level(lvl)
+        ii, jj = nx + 2, ny + 2
+        ns = self.so.shape[0] if lvl == 0 else 5
+
+        def grab(ptr, planes):
+            if not ptr:
+                return None
+            out = np.empty((planes, jj, ii))
+            torch.cuda.synchronize()
+            _memcpy_d2h(out, ptr)
+            return out
+        return grab(so, ns), grab(ci, 8), grab(s0, 2), grab(s1, 2)
+
+    def times(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _abi.call("cdr_hip_solver2d_times", self._h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def vcycle(self, x, b):
+        x, b = _t(x), _t(b)
+        _abi.call("cdr_hip_solver2d_vcycle", self._h, _p(b), _p(x))
+        return x
+
+    def solve(self, b, x=None):
+        """Returns (x, rho history as a numpy array)."""
+        b = _t(b)
+        if x is None:
+            x = torch.zeros_like(b)
+        x = _t(x)
+        hist = (ctypes.c_double * max(self.maxiter, 1))()
+        nit = ctypes.c_int(0)
+        _abi.call("cdr_hip_solver2d_solve", self._h, _p(b), _p(x), hist, len(hist),
+                  ctypes.byref(nit))
+        return x, np.array(hist[:nit.value])
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _abi.load().cdr_hip_solver2d_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _memcpy_d2h(out, ptr):
+    """Copy out.nbytes from device pointer ptr into the numpy array out (synchronous)."""
+    hip = _hip()
+    st = hip.hipMemcpy(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(ptr),
+                       ctypes.c_size_t(out.nbytes), 2)  # hipMemcpyDeviceToHost
+    if st != 0:
+        raise RuntimeError(f"hipMemcpy failed ({st})")
+
+
+_hiprt = None
+
+
+def _hip():
+    global _hiprt
+    if _hiprt is None:
+        _hiprt = ctypes.CDLL("libamdhip64.so.7")  # the runtime already loaded (by soname)
+        _hiprt.hipMemcpy.restype = ctypes.c_int
+        _hiprt.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    return _hiprt

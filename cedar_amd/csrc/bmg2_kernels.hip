@@ -1,0 +1,939 @@
+// 2D BoxMG kernels for CDNA4 (gfx950).
+//
+// Layout: Fortran order with one ghost layer; array index (i, j) with 0 <= i < ii, 0 <= j < jj,
+// interior 1..ii-2 x 1..jj-2, point p = j*ii + i.  Stencil plane k of a level lives at
+// so[k*ii*jj + p].  The off-diagonal couplings are stored negated; so[KNW] at (a, b) couples
+// (a-1, b) with (a, b-1) and so[KSW] at (a, b) couples (a, b) with (a-1, b-1).
+//
+// Numerics: the FP64 kernels for residual, Gauss-Seidel, restriction, interpolation and
+// operator-induced interpolation evaluate every point with the same operation order as Cedar's
+// serial kernels and are compiled with -ffp-contract=off, so they reproduce them bit for bit.
+// Line relaxation replaces the sequential LDL^T sweeps by a wavefront-wide affine scan and the
+// Galerkin product is formed generically as P^T A P, so those two round differently (ulp level).
+//
+// All kernels are stencils on HBM-resident arrays (~0.2-0.3 flop/byte): no MFMA, coalesced
+// accesses along i, 64-wide wavefronts, deterministic fixed-order reductions.
+#include "bmg2_launch.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace cdr {
+namespace k2 {
+
+namespace {
+
+constexpr int WAVE = 64;
+
+__host__ __device__ inline std::size_t pidx(int i, int j, int ii) {
+    return (std::size_t)j * (std::size_t)ii + (std::size_t)i;
+}
+
+inline dim3 grid2(int nx, int ny, int bx, int by) {
+    return dim3((unsigned)((nx + bx - 1) / bx), (unsigned)((ny + by - 1) / by), 1);
+}
+
+// ------------------------------------------------------------------------------------------
+// deterministic block reduction of one double per thread (256 threads = 4 waves)
+// ------------------------------------------------------------------------------------------
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+__device__ inline void block_sum_store(double v, double *dst) {
+    __shared__ double part[4];
+    v = wave_sum(v);
+    int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
+    if (lane == 0) part[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) *dst = ((part[0] + part[1]) + part[2]) + part[3];
+}
+
+// one block: sum of n partials in a fixed order -> *out
+__global__ __launch_bounds__(256) void finalize_sum_k(const double *__restrict__ partials,
+                                                      std::size_t n, double *__restrict__ out) {
+    double v = 0.0;
+    for (std::size_t k = threadIdx.x; k < n; k += blockDim.x) v += partials[k];
+    block_sum_store(v, out);
+}
+
+// ------------------------------------------------------------------------------------------
+// residual  r = f + sum(couplings * q) - c q   (src/2d/ftn/BMG2_SymStd_residual.f90:85-119)
+// block: 64 (i) x 4 (j)
+// ------------------------------------------------------------------------------------------
+template <typename T, bool NINE>
+__device__ inline T apply_offdiag(const T *__restrict__ so, std::size_t P, const T *__restrict__ q,
+                                  std::size_t p, int ii, T acc) {
+    acc = acc + so[P * KW + p] * q[p - 1];
+    acc = acc + so[P * KW + p + 1] * q[p + 1];
+    acc = acc + so[P * KS + p] * q[p - ii];
+    acc = acc + so[P * KS + p + ii] * q[p + ii];
+    if (NINE) {
+        acc = acc + so[P * KSW + p] * q[p - ii - 1];
+        acc = acc + so[P * KNW + p + 1] * q[p - ii + 1];
+        acc = acc + so[P * KNW + p + ii] * q[p + ii - 1];
+        acc = acc + so[P * KSW + p + ii + 1] * q[p + ii + 1];
+    }
+    return acc;
+}
+
+template <typename T, bool NINE, bool NORM>
+__global__ __launch_bounds__(256) void residual_k(const T *__restrict__ so, const T *__restrict__ qf,
+                                                  const T *__restrict__ q, T *__restrict__ res,
+                                                  int ii, int jj, double *__restrict__ partials) {
+    const int i = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
+    const int j = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
+    const std::size_t P = (std::size_t)ii * jj;
+    double sq = 0.0;
+    if (i <= ii - 2 && j <= jj - 2) {
+        const std::size_t p = pidx(i, j, ii);
+        T r = apply_offdiag<T, NINE>(so, P, q, p, ii, qf[p]);
+        r = r - so[p] * q[p];
+        res[p] = r;
+        if (NORM) sq = (double)r * (double)r;
+    }
+    if (NORM) block_sum_store(sq, &partials[blockIdx.y * gridDim.x + blockIdx.x]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void sumsq_k(const T *__restrict__ x, int ii, int jj,
+                                               double *__restrict__ partials) {
+    const int i = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
+    const int j = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
+    double sq = 0.0;
+    if (i <= ii - 2 && j <= jj - 2) {
+        double v = (double)x[pidx(i, j, ii)];
+        sq = v * v;
+    }
+    block_sum_store(sq, &partials[blockIdx.y * gridDim.x + blockIdx.x]);
+}
+
+// mixed precision outer step: xnew = x + e, r = f - A xnew (FP64), r32 = (float) r
+template <bool NINE, bool HAS_E>
+__global__ __launch_bounds__(256) void residual_ir_k(const double *__restrict__ so,
+                                                     const double *__restrict__ qf,
+                                                     const double *__restrict__ x,
+                                                     const float *__restrict__ e,
+                                                     double *__restrict__ xnew,
+                                                     float *__restrict__ r32, int ii, int jj,
+                                                     double *__restrict__ partials) {
+    const int i = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
+    const int j = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
+    const std::size_t P = (std::size_t)ii * jj;
+    double sq = 0.0;
+    if (i <= ii - 2 && j <= jj - 2) {
+        const std::size_t p = pidx(i, j, ii);
+        auto X = [&](std::size_t k) -> double {
+            return HAS_E ? x[k] + (double)e[k] : x[k];
+        };
+        double r = qf[p];
+        r = r + so[P * KW + p] * X(p - 1);
+        r = r + so[P * KW + p + 1] * X(p + 1);
+        r = r + so[P * KS + p] * X(p - ii);
+        r = r + so[P * KS + p + ii] * X(p + ii);
+        if (NINE) {
+            r = r + so[P * KSW + p] * X(p - ii - 1);
+            r = r + so[P * KNW + p + 1] * X(p - ii + 1);
+            r = r + so[P * KNW + p + ii] * X(p + ii - 1);
+            r = r + so[P * KSW + p + ii + 1] * X(p + ii + 1);
+        }
+        const double xc = X(p);
+        r = r - so[p] * xc;
+        xnew[p] = xc;
+        r32[p] = (float)r;
+        sq = r * r;
+    }
+    block_sum_store(sq, &partials[blockIdx.y * gridDim.x + blockIdx.x]);
+}
+
+// ------------------------------------------------------------------------------------------
+// point Gauss-Seidel, one launch per colour (src/2d/ftn/BMG2_SymStd_relax_GS.f90:78-137)
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void recip_k(const double *__restrict__ so,
+                                               double *__restrict__ sor, int ii, int jj) {
+    const int i = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
+    const int j = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
+    if (i <= ii - 2 && j <= jj - 2) {
+        const std::size_t p = pidx(i, j, ii), P = (std::size_t)ii * jj;
+        sor[P + p] = 1.0 / so[p];
+    }
+}
+
+// 5-point: colour = parity of (i + j); 9-point: colour = (i parity pi, j parity pj)
+template <typename T, bool NINE>
+__global__ __launch_bounds__(256) void gs_colour_k(const T *__restrict__ so, const T *__restrict__ qf,
+                                                   T *__restrict__ q, const T *__restrict__ sor,
+                                                   int ii, int jj, int pi, int pj) {
+    const int k = blockIdx.x * 64 + (threadIdx.x % 64);
+    const int m = blockIdx.y * 4 + (threadIdx.x / 64);
+    int i, j;
+    if (NINE) {
+        i = 2 * k + pi;
+        j = 2 * m + pj;
+    } else {
+        j = m + 1;
+        i = 2 * k + ((pi + j) & 1);  // (i + j) % 2 == pi
+    }
+    if (i < 1 || i > ii - 2 || j < 1 || j > jj - 2) return;
+    const std::size_t P = (std::size_t)ii * jj;
+    const std::size_t p = pidx(i, j, ii);
+    T v = apply_offdiag<T, NINE>(so, P, q, p, ii, qf[p]);
+    q[p] = v * sor[P + p];
+}
+
+// ------------------------------------------------------------------------------------------
+// line relaxation
+//
+// setup (src/2d/ftn/BMG2_SymStd_SETUP_lines_{x,y}.f90): diag = c, off = -w (x) or -s (y), then
+// LAPACK DPTTRF per line; one thread walks one line sequentially (setup-time only).
+// x factors: sor[0][p] = D, sor[1][p] = E at (i, j) as in the reference; y factors stored
+// transposed: plane k element (j, i) at k*ii*jj + i*jj + j.
+// ------------------------------------------------------------------------------------------
+__global__ void setup_lines_x_k(const double *__restrict__ so, double *__restrict__ sor, int ii,
+                                int jj, int *__restrict__ info) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (j > jj - 2) return;
+    const std::size_t P = (std::size_t)ii * jj;
+    const int n = ii - 2;
+    double *D = sor + pidx(1, j, ii);      // D[k] at i = k + 1
+    double *E = sor + P + pidx(2, j, ii);  // E[k] at i = k + 2
+    for (int i = 1; i <= n; ++i) {
+        const std::size_t p = pidx(i, j, ii);
+        sor[P + p] = -so[P * KW + p];
+        sor[p] = so[p];
+    }
+    for (int k = 0; k < n - 1; ++k) {
+        const double d = D[k];
+        if (!(d > 0.0)) {
+            atomicMin(info, k + 1);
+            return;
+        }
+        const double ei = E[k];
+        E[k] = ei / d;
+        D[k + 1] = D[k + 1] - E[k] * ei;
+    }
+    if (!(D[n - 1] > 0.0)) atomicMin(info, n);
+}
+
+__global__ void setup_lines_y_k(const double *__restrict__ so, double *__restrict__ sor, int ii,
+                                int jj, int *__restrict__ info) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (i > ii - 2) return;
+    const std::size_t P = (std::size_t)ii * jj;
+    const int n = jj - 2;
+    double *D = sor + (std::size_t)i * jj + 1;
+    double *E = sor + P + (std::size_t)i * jj + 2;
+    for (int j = 1; j <= n; ++j) {
+        const std::size_t p = pidx(i, j, ii);
+        sor[P + (std::size_t)i * jj + j] = -so[P * KS + p];
+        sor[(std::size_t)i * jj + j] = so[p];
+    }
+    for (int k = 0; k < n - 1; ++k) {
+        const double d = D[k];
+        if (!(d > 0.0)) {
+            atomicMin(info, k + 1);
+            return;
+        }
+        const double ei = E[k];
+        E[k] = ei / d;
+        D[k + 1] = D[k + 1] - E[k] * ei;
+    }
+    if (!(D[n - 1] > 0.0)) atomicMin(info, n);
+}
+
+// Inclusive scans of affine maps y -> a*y + b across the 64 lanes of a wavefront.
+// up:   lane l gets F_l o F_{l-1} o ... o F_0        (forward substitution)
+// down: lane l gets G_l o G_{l+1} o ... o G_63       (backward substitution)
+template <typename T>
+__device__ inline void scan_affine_up(T &a, T &b, int lane) {
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const T ap = __shfl_up(a, d, WAVE);
+        const T bp = __shfl_up(b, d, WAVE);
+        if (lane >= d) {
+            b = a * bp + b;
+            a = a * ap;
+        }
+    }
+}
+
+template <typename T>
+__device__ inline void scan_affine_down(T &a, T &b, int lane) {
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const T an = __shfl_down(a, d, WAVE);
+        const T bn = __shfl_down(b, d, WAVE);
+        if (lane + d < WAVE) {
+            b = a * bn + b;
+            a = a * an;
+        }
+    }
+}
+
+// One wavefront solves one line: L D L^T x = rhs with the stored factors.  Forward pass in
+// 64-element chunks: y_k = rhs_k - E_{k-1} y_{k-1}, stores c_k = y_k / D_k in place of q;
+// backward pass: x_k = c_k - E_k x_{k+1}.  A line of one colour only reads lines of the other
+// colour, so the in-place staging is race free.
+//
+// DIR 0: x-lines (row j = first + 2*line, elements along i, unit stride)
+// DIR 1: y-lines (column i = first + 2*line, elements along j, stride ii; factors transposed)
+template <typename T, bool NINE, int DIR>
+__global__ __launch_bounds__(256) void relax_lines_k(const T *__restrict__ so, const T *__restrict__ qf,
+                                                     T *__restrict__ q, const T *__restrict__ sor,
+                                                     int ii, int jj, int first, int nlines) {
+    const int lane = threadIdx.x % WAVE;
+    const int line = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (line >= nlines) return;  // whole wavefront leaves together; no block barrier below
+    const std::size_t P = (std::size_t)ii * jj;
+    const int fixed = first + 2 * line;
+    const int n = (DIR == 0) ? ii - 2 : jj - 2;
+    const std::size_t stride = (DIR == 0) ? 1 : (std::size_t)ii;
+    const std::size_t base = (DIR == 0) ? pidx(1, fixed, ii) : pidx(fixed, 1, ii);  // element 0
+    const T *D = (DIR == 0) ? sor + pidx(1, fixed, ii) : sor + (std::size_t)fixed * jj + 1;
+    const T *E = (DIR == 0) ? sor + P + pidx(2, fixed, ii) : sor + P + (std::size_t)fixed * jj + 2;
+    const int nch = (n + WAVE - 1) / WAVE;
+
+    T carry = T(0);
+    for (int c = 0; c < nch; ++c) {
+        const int k = c * WAVE + lane;
+        const bool act = k < n;
+        T a = T(0), b = T(0);
+        if (act) {
+            const std::size_t p = base + (std::size_t)k * stride;
+            T rhs = qf[p];
+            if (DIR == 0) {
+                rhs = rhs + so[P * KS + p] * q[p - ii];
+                rhs = rhs + so[P * KS + p + ii] * q[p + ii];
+            } else {
+                rhs = rhs + so[P * KW + p] * q[p - 1];
+                rhs = rhs + so[P * KW + p + 1] * q[p + 1];
+            }
+            if (NINE) {
+                rhs = rhs + so[P * KSW + p] * q[p - ii - 1];
+                rhs = rhs + so[P * KNW + p + 1] * q[p - ii + 1];
+                rhs = rhs + so[P * KNW + p + ii] * q[p + ii - 1];
+                rhs = rhs + so[P * KSW + p + ii + 1] * q[p + ii + 1];
+            }
+            b = rhs;
+            a = (k > 0) ? -E[k - 1] : T(0);
+        }
+        scan_affine_up(a, b, lane);
+        const T y = a * carry + b;
+        carry = __shfl(y, WAVE - 1, WAVE);
+        if (act) q[base + (std::size_t)k * stride] = y / D[k];
+    }
+    carry = T(0);
+    for (int c = nch - 1; c >= 0; --c) {
+        const int k = c * WAVE + lane;
+        const bool act = k < n;
+        T a = T(1), b = T(0);
+        if (act) {
+            b = q[base + (std::size_t)k * stride];
+            a = (k < n - 1) ? -E[k] : T(0);
+        }
+        scan_affine_down(a, b, lane);
+        const T x = a * carry + b;
+        carry = __shfl(x, 0, WAVE);
+        if (act) q[base + (std::size_t)k * stride] = x;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// restriction qc = P^T q (src/2d/ftn/BMG2_SymStd_restrict.f90:76-92); coarse (ic, jc) sits on
+// fine (2ic-1, 2jc-1)
+// ------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void restrict_k(const T *__restrict__ ci, const T *__restrict__ q,
+                                                  T *__restrict__ qc, int iif, int jjf, int iic,
+                                                  int jjc) {
+    const int ic = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
+    const int jc = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
+    if (ic > iic - 2 || jc > jjc - 2) return;
+    const std::size_t Pc = (std::size_t)iic * jjc;
+    const int i = 2 * ic - 1, j = 2 * jc - 1;
+    auto CI = [&](int a, int b, int k) { return ci[Pc * k + pidx(a, b, iic)]; };
+    auto Q = [&](int a, int b) { return q[pidx(a, b, iif)]; };
+    T v = CI(ic, jc, LNE) * Q(i - 1, j - 1);
+    v = v + CI(ic, jc, LA) * Q(i, j - 1);
+    v = v + CI(ic + 1, jc, LNW) * Q(i + 1, j - 1);
+    v = v + CI(ic, jc, LR) * Q(i - 1, j);
+    v = v + Q(i, j);
+    v = v + CI(ic + 1, jc, LL) * Q(i + 1, j);
+    v = v + CI(ic, jc + 1, LSE) * Q(i - 1, j + 1);
+    v = v + CI(ic, jc + 1, LB) * Q(i, j + 1);
+    v = v + CI(ic + 1, jc + 1, LSW) * Q(i + 1, j + 1);
+    qc[pidx(ic, jc, iic)] = v;
+}
+
+// ------------------------------------------------------------------------------------------
+// interp_add q += P qc + res / c at F-points (src/2d/ftn/BMG2_SymStd_interp_add.f90:101-137);
+// one thread per fine point.  res is only read (the reference divides it in place; nothing
+// reads it afterwards).  For an even fine extent the reference's loops also reach the last
+// fine ghost column / row (coarse index IICF1 = IIC); those points are updated the same way,
+// with the raw res value there since the reference divides res only in the interior.
+// ------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void interp_add_k(T *__restrict__ q, const T *__restrict__ qc,
+                                                    const T *__restrict__ res,
+                                                    const T *__restrict__ so,
+                                                    const T *__restrict__ ci, int iic, int jjc,
+                                                    int iif, int jjf) {
+    const int i = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
+    const int j = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
+    const int imax = iif - 2 + ((iif - 2) % 2 == 0 ? 1 : 0);
+    const int jmax = jjf - 2 + ((jjf - 2) % 2 == 0 ? 1 : 0);
+    if (i > imax || j > jmax) return;
+    const std::size_t Pc = (std::size_t)iic * jjc;
+    auto CI = [&](int a, int b, int k) { return ci[Pc * k + pidx(a, b, iic)]; };
+    auto QC = [&](int a, int b) { return qc[pidx(a, b, iic)]; };
+    const std::size_t p = pidx(i, j, iif);
+    const bool ci_col = (i & 1), ci_row = (j & 1);
+    if (ci_col && ci_row) {
+        q[p] = q[p] + QC((i + 1) / 2, (j + 1) / 2);
+        return;
+    }
+    const bool ghost = i > iif - 2 || j > jjf - 2;
+    const T r = ghost ? res[p] : res[p] / so[p];
+    T a;
+    if (!ci_col && ci_row) {
+        const int icr = i / 2 + 1, jc = (j + 1) / 2;
+        a = CI(icr, jc, LR) * QC(icr, jc) + CI(icr, jc, LL) * QC(icr - 1, jc);
+    } else if (ci_col && !ci_row) {
+        const int ic = (i + 1) / 2, jca = j / 2 + 1;
+        a = CI(ic, jca, LA) * QC(ic, jca) + CI(ic, jca, LB) * QC(ic, jca - 1);
+    } else {
+        const int icr = i / 2 + 1, jca = j / 2 + 1;
+        a = CI(icr, jca, LSW) * QC(icr - 1, jca - 1);
+        a = a + CI(icr, jca, LNW) * QC(icr - 1, jca);
+        a = a + CI(icr, jca, LNE) * QC(icr, jca);
+        a = a + CI(icr, jca, LSE) * QC(icr, jca - 1);
+    }
+    q[p] = q[p] + a + r;
+}
+
+// ------------------------------------------------------------------------------------------
+// operator-induced interpolation (src/2d/ftn/BMG2_SymStd_SETUP_interp_OI.f90:109-257)
+// ------------------------------------------------------------------------------------------
+__device__ inline double lumped(double c, double sum, double ep) {
+    const double emach = 0x1.0p-53;  // D1MACH(3)
+    return (c - sum) * fmax(c - (1.0 + ep) * sum, 0.0) / (fabs(c - (1.0 + ep) * sum) + emach);
+}
+
+// edge weights: x-edges (LR, LL) for ic in [2, nxf/2+1], jc in [1, ncy]; y-edges (LA, LB) for
+// ic in [1, ncx], jc in [2, nyf/2+1].  Threads cover the union of both ranges.
+template <bool NINE>
+__global__ __launch_bounds__(256) void oi_edges_k(const double *__restrict__ so, double *__restrict__ ci,
+                                                  int iif, int jjf, int iic, int jjc) {
+    const int ic = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
+    const int jc = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
+    const std::size_t P = (std::size_t)iif * jjf, Pc = (std::size_t)iic * jjc;
+    auto S = [&](int a, int b, int k) { return so[P * k + pidx(a, b, iif)]; };
+    const int icmax = (iif - 2) / 2 + 1, jcmax = (jjf - 2) / 2 + 1;
+    const int i = 2 * ic - 1, j = 2 * jc - 1;
+    const std::size_t pc = pidx(ic, jc, iic);
+    if (ic >= 2 && ic <= icmax && jc <= jjc - 2) {
+        double a, b;
+        if (NINE) {
+            a = S(i, j, KW) + S(i, j, KNW) + S(i, j + 1, KSW);
+            b = S(i - 1, j, KW) + S(i - 1, j, KSW) + S(i - 1, j + 1, KNW);
+        } else {
+            a = S(i, j, KW);
+            b = S(i - 1, j, KW);
+        }
+        const double c = S(i - 1, j, KO);
+        const double ep = fmin(fabs(a / c), fabs(b / c));
+        double sum = a + b + S(i - 1, j, KS) + S(i - 1, j + 1, KS);
+        sum = a + b + lumped(c, sum, ep);
+        sum = 1.0 / sum;
+        ci[Pc * LR + pc] = a * sum;
+        ci[Pc * LL + pc] = b * sum;
+    }
+    if (jc >= 2 && jc <= jcmax && ic <= iic - 2) {
+        double a, b;
+        if (NINE) {
+            a = S(i, j, KS) + S(i, j, KNW) + S(i + 1, j, KSW);
+            b = S(i, j - 1, KS) + S(i, j - 1, KSW) + S(i + 1, j - 1, KNW);
+        } else {
+            a = S(i, j, KS);
+            b = S(i, j - 1, KS);
+        }
+        const double c = S(i, j - 1, KO);
+        const double ep = fmin(fabs(a / c), fabs(b / c));
+        double sum = a + b + S(i, j - 1, KW) + S(i + 1, j - 1, KW);
+        sum = a + b + lumped(c, sum, ep);
+        sum = 1.0 / sum;
+        ci[Pc * LA + pc] = a * sum;
+        ci[Pc * LB + pc] = b * sum;
+    }
+}
+
+// corner weights for ic in [2, nxf/2+1], jc in [2, nyf/2+1], from the edge weights
+template <bool NINE>
+__global__ __launch_bounds__(256) void oi_corners_k(const double *__restrict__ so, double *__restrict__ ci,
+                                                    int iif, int jjf, int iic, int jjc) {
+    const int ic = blockIdx.x * 64 + (threadIdx.x % 64) + 2;
+    const int jc = blockIdx.y * 4 + (threadIdx.x / 64) + 2;
+    const int icmax = (iif - 2) / 2 + 1, jcmax = (jjf - 2) / 2 + 1;
+    if (ic > icmax || jc > jcmax) return;
+    const std::size_t P = (std::size_t)iif * jjf, Pc = (std::size_t)iic * jjc;
+    auto S = [&](int a, int b, int k) { return so[P * k + pidx(a, b, iif)]; };
+    auto CI = [&](int a, int b, int k) { return ci[Pc * k + pidx(a, b, iic)]; };
+    const int i = 2 * ic - 1, j = 2 * jc - 1;
+    const double c = S(i - 1, j - 1, KO);
+    double sum, ep;
+    if (NINE) {
+        sum = S(i - 1, j - 1, KW) + S(i - 1, j, KNW) + S(i - 1, j, KS) + S(i, j, KSW)
+            + S(i, j - 1, KW) + S(i, j - 1, KNW) + S(i - 1, j - 1, KS) + S(i - 1, j - 1, KSW);
+        const double e1 = fabs((S(i - 1, j - 1, KSW) + S(i - 1, j - 1, KW) + S(i - 1, j, KNW)) / c);
+        const double e2 = fabs((S(i - 1, j, KNW) + S(i - 1, j, KS) + S(i, j, KSW)) / c);
+        const double e3 = fabs((S(i, j, KSW) + S(i, j - 1, KW) + S(i, j - 1, KNW)) / c);
+        const double e4 = fabs((S(i, j - 1, KNW) + S(i - 1, j - 1, KS) + S(i - 1, j - 1, KSW)) / c);
+        ep = fmin(fmin(fmin(e1, e2), e3), e4);
+    } else {
+        sum = S(i - 1, j - 1, KW) + S(i - 1, j, KS) + S(i, j - 1, KW) + S(i - 1, j - 1, KS);
+        ep = fmin(fmin(fmin(fabs(S(i - 1, j - 1, KW) / c), fabs(S(i - 1, j, KS) / c)),
+                       fabs(S(i, j - 1, KW) / c)),
+                  fabs(S(i - 1, j - 1, KS) / c));
+    }
+    sum = sum + lumped(c, sum, ep);
+    const double s = 1.0 / sum;
+    const std::size_t pc = pidx(ic, jc, iic);
+    double sw = S(i - 1, j - 1, KS) * CI(ic, jc - 1, LL) + S(i - 1, j - 1, KW) * CI(ic - 1, jc, LB);
+    double se = S(i - 1, j - 1, KS) * CI(ic, jc - 1, LR) + S(i, j - 1, KW) * CI(ic, jc, LB);
+    double nw = S(i - 1, j - 1, KW) * CI(ic - 1, jc, LA) + S(i - 1, j, KS) * CI(ic, jc, LL);
+    double ne = S(i - 1, j, KS) * CI(ic, jc, LR) + S(i, j - 1, KW) * CI(ic, jc, LA);
+    if (NINE) {
+        sw = sw + S(i - 1, j - 1, KSW);
+        se = se + S(i, j - 1, KNW);
+        nw = nw + S(i - 1, j, KNW);
+        ne = ne + S(i, j, KSW);
+    }
+    ci[Pc * LSW + pc] = sw * s;
+    ci[Pc * LSE + pc] = se * s;
+    ci[Pc * LNW + pc] = nw * s;
+    ci[Pc * LNE + pc] = ne * s;
+}
+
+// ------------------------------------------------------------------------------------------
+// Galerkin coarse operator, formed generically: K(X, Y) = sum_f sum_g P(f,X) A(f,g) P(g,Y)
+// over the 3x3 fine supports of the two coarse points.  Coarse entries (lower half):
+// c = K(C,C), w = -K(C,C-e_x), s = -K(C,C-e_y), sw = -K(C,C-e_x-e_y),
+// nw(C) = -K(C-e_x, C-e_y).  Couplings to coarse ghosts are zero (Dirichlet).
+// ------------------------------------------------------------------------------------------
+// weight of fine point centre(ic,jc) + (di,dj) in the column of coarse point (ic, jc)
+__device__ inline double pweight(const double *__restrict__ ci, std::size_t Pc, int iic, int ic,
+                                 int jc, int di, int dj) {
+    auto CI = [&](int a, int b, int k) { return ci[Pc * k + pidx(a, b, iic)]; };
+    if (di == 0 && dj == 0) return 1.0;
+    if (di == -1 && dj == 0) return CI(ic, jc, LR);
+    if (di == 1 && dj == 0) return CI(ic + 1, jc, LL);
+    if (di == 0 && dj == -1) return CI(ic, jc, LA);
+    if (di == 0 && dj == 1) return CI(ic, jc + 1, LB);
+    if (di == -1 && dj == -1) return CI(ic, jc, LNE);
+    if (di == 1 && dj == -1) return CI(ic + 1, jc, LNW);
+    if (di == -1 && dj == 1) return CI(ic, jc + 1, LSE);
+    return CI(ic + 1, jc + 1, LSW);
+}
+
+// A(f, f + (dx, dy)) of the fine operator
+template <bool NINE>
+__device__ inline double aentry(const double *__restrict__ so, std::size_t P, int ii, int x, int y,
+                                int dx, int dy) {
+    const std::size_t p = pidx(x, y, ii);
+    if (dx == 0 && dy == 0) return so[p];
+    if (dx == -1 && dy == 0) return -so[P * KW + p];
+    if (dx == 1 && dy == 0) return -so[P * KW + p + 1];
+    if (dx == 0 && dy == -1) return -so[P * KS + p];
+    if (dx == 0 && dy == 1) return -so[P * KS + p + ii];
+    if (!NINE) return 0.0;
+    if (dx == -1 && dy == -1) return -so[P * KSW + p];
+    if (dx == 1 && dy == 1) return -so[P * KSW + p + ii + 1];
+    if (dx == 1 && dy == -1) return -so[P * KNW + p + 1];
+    return -so[P * KNW + p + ii];
+}
+
+template <bool NINE, int DXC, int DYC>
+__device__ inline double kpair(const double *__restrict__ so, const double *__restrict__ ci,
+                               int iif, int jjf, int iic, int jjc, int icx, int jcx) {
+    const std::size_t P = (std::size_t)iif * jjf, Pc = (std::size_t)iic * jjc;
+    const int nxf = iif - 2, nyf = jjf - 2;
+    const int X0 = 2 * icx - 1, Y0 = 2 * jcx - 1;
+    const int icy = icx + DXC, jcy = jcx + DYC;
+    double sum = 0.0;
+#pragma unroll
+    for (int fy = -1; fy <= 1; ++fy) {
+#pragma unroll
+        for (int fx = -1; fx <= 1; ++fx) {
+            const int x = X0 + fx, y = Y0 + fy;
+            if (x < 1 || x > nxf || y < 1 || y > nyf) continue;
+            double inner = 0.0;
+            bool any = false;
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy) {
+#pragma unroll
+                for (int dx = -1; dx <= 1; ++dx) {
+                    if (!NINE && dx != 0 && dy != 0) continue;
+                    const int gx = fx + dx - 2 * DXC, gy = fy + dy - 2 * DYC;
+                    if (gx < -1 || gx > 1 || gy < -1 || gy > 1) continue;
+                    const int xg = x + dx, yg = y + dy;
+                    if (xg < 1 || xg > nxf || yg < 1 || yg > nyf) continue;
+                    inner += aentry<NINE>(so, P, iif, x, y, dx, dy) *
+                             pweight(ci, Pc, iic, icy, jcy, gx, gy);
+                    any = true;
+                }
+            }
+            if (any) sum += pweight(ci, Pc, iic, icx, jcx, fx, fy) * inner;
+        }
+    }
+    return sum;
+}
+
+template <bool NINE>
+__global__ __launch_bounds__(256) void galerkin_k(const double *__restrict__ so,
+                                                  const double *__restrict__ ci,
+                                                  double *__restrict__ soc, int iif, int jjf,
+                                                  int iic, int jjc) {
+    const int ic = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
+    const int jc = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
+    if (ic > iic - 2 || jc > jjc - 2) return;
+    const std::size_t Pc = (std::size_t)iic * jjc, pc = pidx(ic, jc, iic);
+    soc[pc] = kpair<NINE, 0, 0>(so, ci, iif, jjf, iic, jjc, ic, jc);
+    soc[Pc * KW + pc] = (ic >= 2) ? -kpair<NINE, -1, 0>(so, ci, iif, jjf, iic, jjc, ic, jc) : 0.0;
+    soc[Pc * KS + pc] = (jc >= 2) ? -kpair<NINE, 0, -1>(so, ci, iif, jjf, iic, jjc, ic, jc) : 0.0;
+    const bool corner = ic >= 2 && jc >= 2;
+    soc[Pc * KSW + pc] = corner ? -kpair<NINE, -1, -1>(so, ci, iif, jjf, iic, jjc, ic, jc) : 0.0;
+    soc[Pc * KNW + pc] = corner ? -kpair<NINE, 1, -1>(so, ci, iif, jjf, iic, jjc, ic - 1, jc) : 0.0;
+}
+
+// ------------------------------------------------------------------------------------------
+// coarsest grid: banded Cholesky (DPBTF2 upper) and solve (DPBTRS = two DTBSV), FP64, one
+// workgroup.  Band layout ABD(ii, n), diagonal in row kd+1 = ii, kd = ii-1
+// (src/2d/ftn/BMG2_SymStd_SETUP_cg_LU.f90:92-118, src/2d/ftn/BMG2_SymStd_SOLVE_cg.f90:93-123).
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void setup_cg_k(const double *__restrict__ so, int nstncl,
+                                                  double *__restrict__ abd, int ii, int jj,
+                                                  int *__restrict__ info) {
+    const int nx = ii - 2, ny = jj - 2, n = nx * ny, kd = ii - 1, ld = ii;
+    const std::size_t P = (std::size_t)ii * jj;
+    auto AB = [&](int r, int c) -> double & { return abd[(std::size_t)(c - 1) * ld + (r - 1)]; };
+    for (int kk = threadIdx.x; kk < n; kk += blockDim.x) {
+        const int i = kk % nx + 1, j = kk / nx + 1;
+        const std::size_t p = pidx(i, j, ii);
+        const int c = kk + 1;
+        for (int r = 1; r <= ld; ++r) AB(r, c) = 0.0;
+        AB(ii, c) = so[p];
+        AB(ii - 1, c) = -so[P * KW + p];
+        AB(3, c) = (nstncl == 5) ? -so[P * KNW + p + 1] : 0.0;
+        AB(2, c) = -so[P * KS + p];
+        AB(1, c) = (nstncl == 5) ? -so[P * KSW + p] : 0.0;
+    }
+    __syncthreads();
+    __shared__ int bad;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    for (int j = 1; j <= n; ++j) {
+        if (threadIdx.x == 0) {
+            double ajj = AB(kd + 1, j);
+            if (!(ajj > 0.0)) {
+                bad = j;
+            } else {
+                ajj = sqrt(ajj);
+                AB(kd + 1, j) = ajj;
+                const int kn = (kd < n - j) ? kd : n - j;
+                const double r = 1.0 / ajj;
+                for (int t = 1; t <= kn; ++t) AB(kd + 1 - t, j + t) *= r;
+            }
+        }
+        __syncthreads();
+        if (bad) break;
+        const int kn = (kd < n - j) ? kd : n - j;
+        // DSYR upper: M(p,q) += x(p) * (-x(q)), p <= q, for q = 1..kn
+        const int npairs = kn * (kn + 1) / 2;
+        for (int t = threadIdx.x; t < npairs; t += blockDim.x) {
+            int qq = 1, rem = t;
+            while (rem >= qq) {
+                rem -= qq;
+                ++qq;
+            }
+            const int pp = rem + 1;
+            const double xq = AB(kd + 1 - qq, j + qq);
+            if (xq != 0.0) {
+                const double temp = -1.0 * xq;
+                AB(kd + 1 + pp - qq, j + qq) = AB(kd + 1 + pp - qq, j + qq) + AB(kd + 1 - pp, j + pp) * temp;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *info = bad;
+}
+
+template <typename T>
+__global__ void solve_cg_k(T *__restrict__ q, const T *__restrict__ qf, const double *__restrict__ abd,
+                           double *__restrict__ x, int ii, int jj) {
+    if (threadIdx.x != 0) return;
+    const int nx = ii - 2, ny = jj - 2, n = nx * ny, kd = ii - 1, ld = ii, kp1 = kd + 1;
+    auto AB = [&](int r, int c) { return abd[(std::size_t)(c - 1) * ld + (r - 1)]; };
+    for (int kk = 0; kk < n; ++kk) x[kk] = (double)qf[pidx(kk % nx + 1, kk / nx + 1, ii)];
+    for (int j = 1; j <= n; ++j) {
+        double temp = x[j - 1];
+        const int l = kp1 - j;
+        const int i0 = (j - kd > 1) ? j - kd : 1;
+        for (int i = i0; i <= j - 1; ++i) temp = temp - AB(l + i, j) * x[i - 1];
+        x[j - 1] = temp / AB(kp1, j);
+    }
+    for (int j = n; j >= 1; --j) {
+        if (x[j - 1] != 0.0) {
+            const int l = kp1 - j;
+            x[j - 1] = x[j - 1] / AB(kp1, j);
+            const double temp = x[j - 1];
+            const int i0 = (j - kd > 1) ? j - kd : 1;
+            for (int i = j - 1; i >= i0; --i) x[i - 1] = x[i - 1] - temp * AB(l + i, j);
+        }
+    }
+    for (int kk = 0; kk < n; ++kk) q[pidx(kk % nx + 1, kk / nx + 1, ii)] = (T)x[kk];
+}
+
+// ------------------------------------------------------------------------------------------
+// elementwise helpers
+// ------------------------------------------------------------------------------------------
+template <typename Tin, typename Tout>
+__global__ __launch_bounds__(256) void convert_k(const Tin *__restrict__ in, Tout *__restrict__ out,
+                                                 std::size_t n) {
+    for (std::size_t k = (std::size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (std::size_t)gridDim.x * blockDim.x)
+        out[k] = (Tout)in[k];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void fill_k(T *__restrict__ x, T v, std::size_t n) {
+    for (std::size_t k = (std::size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (std::size_t)gridDim.x * blockDim.x)
+        x[k] = v;
+}
+
+__global__ __launch_bounds__(256) void add_correction_k(double *__restrict__ x, const float *__restrict__ e,
+                                                        int ii, int jj) {
+    const int i = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
+    const int j = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
+    if (i > ii - 2 || j > jj - 2) return;
+    const std::size_t p = pidx(i, j, ii);
+    x[p] = x[p] + (double)e[p];
+}
+
+inline unsigned grid1(std::size_t n) {
+    std::size_t b = (n + 255) / 256;
+    if (b > 65536) b = 65536;
+    return (unsigned)(b ? b : 1);
+}
+
+}  // namespace
+
+// ==========================================================================================
+// launchers
+// ==========================================================================================
+std::size_t norm_partials(int ii, int jj) {
+    const dim3 g = grid2(ii - 2, jj - 2, 64, 4);
+    return (std::size_t)g.x * g.y;
+}
+
+template <typename T>
+void residual(const T *so, int nstncl, const T *qf, const T *q, T *res, int ii, int jj,
+              hipStream_t s) {
+    const dim3 g = grid2(ii - 2, jj - 2, 64, 4);
+    if (nstncl == 5)
+        residual_k<T, true, false><<<g, 256, 0, s>>>(so, qf, q, res, ii, jj, nullptr);
+    else
+        residual_k<T, false, false><<<g, 256, 0, s>>>(so, qf, q, res, ii, jj, nullptr);
+}
+
+template <typename T>
+void residual_norm(const T *so, int nstncl, const T *qf, const T *q, T *res, int ii, int jj,
+                   double *partials, double *sumsq_out, hipStream_t s) {
+    const dim3 g = grid2(ii - 2, jj - 2, 64, 4);
+    if (nstncl == 5)
+        residual_k<T, true, true><<<g, 256, 0, s>>>(so, qf, q, res, ii, jj, partials);
+    else
+        residual_k<T, false, true><<<g, 256, 0, s>>>(so, qf, q, res, ii, jj, partials);
+    finalize_sum_k<<<1, 256, 0, s>>>(partials, (std::size_t)g.x * g.y, sumsq_out);
+}
+
+template <typename T>
+void sumsq(const T *x, int ii, int jj, double *partials, double *out, hipStream_t s) {
+    const dim3 g = grid2(ii - 2, jj - 2, 64, 4);
+    sumsq_k<T><<<g, 256, 0, s>>>(x, ii, jj, partials);
+    finalize_sum_k<<<1, 256, 0, s>>>(partials, (std::size_t)g.x * g.y, out);
+}
+
+void residual_ir(const double *so, int nstncl, const double *qf, const double *x, const float *e,
+                 double *xnew, float *r32, int ii, int jj, double *partials, double *sumsq_out,
+                 hipStream_t s) {
+    const dim3 g = grid2(ii - 2, jj - 2, 64, 4);
+    if (nstncl == 5) {
+        if (e)
+            residual_ir_k<true, true><<<g, 256, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, partials);
+        else
+            residual_ir_k<true, false><<<g, 256, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, partials);
+    } else {
+        if (e)
+            residual_ir_k<false, true><<<g, 256, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, partials);
+        else
+            residual_ir_k<false, false><<<g, 256, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, partials);
+    }
+    finalize_sum_k<<<1, 256, 0, s>>>(partials, (std::size_t)g.x * g.y, sumsq_out);
+}
+
+void setup_recip(const double *so, double *sor, int ii, int jj, hipStream_t s) {
+    recip_k<<<grid2(ii - 2, jj - 2, 64, 4), 256, 0, s>>>(so, sor, ii, jj);
+}
+
+template <typename T>
+void relax_gs(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
+              int updown, hipStream_t s) {
+    if (nstncl == 5) {
+        // DOWN: (i odd, j odd), (even, odd), (odd, even), (even, even); UP: reverse
+        static const int order[4][2] = {{1, 1}, {0, 1}, {1, 0}, {0, 0}};
+        const dim3 g = grid2(ii / 2 + 1, jj / 2 + 1, 64, 4);
+        for (int t = 0; t < 4; ++t) {
+            const int c = updown == 0 ? t : 3 - t;
+            gs_colour_k<T, true><<<g, 256, 0, s>>>(so, qf, q, sor, ii, jj, order[c][0], order[c][1]);
+        }
+    } else {
+        // DOWN: (i + j) even first; UP: odd first
+        const dim3 g = grid2(ii / 2 + 1, jj - 2, 64, 4);
+        for (int t = 0; t < 2; ++t) {
+            const int parity = updown == 0 ? t : 1 - t;
+            gs_colour_k<T, false><<<g, 256, 0, s>>>(so, qf, q, sor, ii, jj, parity, 0);
+        }
+    }
+}
+
+void setup_lines_x(const double *so, double *sor, int ii, int jj, int *info, hipStream_t s) {
+    const int n = jj - 2;
+    setup_lines_x_k<<<(n + 63) / 64, 64, 0, s>>>(so, sor, ii, jj, info);
+}
+
+void setup_lines_y(const double *so, double *sor, int ii, int jj, int *info, hipStream_t s) {
+    const int n = ii - 2;
+    setup_lines_y_k<<<(n + 63) / 64, 64, 0, s>>>(so, sor, ii, jj, info);
+}
+
+template <typename T, int DIR>
+static void relax_lines(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
+                        int updown, hipStream_t s) {
+    // lines of one colour: DOWN relaxes the lines at even array index (odd Cedar index) first,
+    // UP the odd ones first (src/2d/ftn/BMG2_SymStd_relax_lines_x.f90:82-97)
+    const int nfix = (DIR == 0) ? jj - 2 : ii - 2;
+    for (int t = 0; t < 2; ++t) {
+        const int first = (updown == 0) ? (t == 0 ? 2 : 1) : (t == 0 ? 1 : 2);
+        if (first > nfix) continue;
+        const int nlines = (nfix - first) / 2 + 1;
+        const unsigned blocks = (unsigned)((nlines + 3) / 4);
+        if (nstncl == 5)
+            relax_lines_k<T, true, DIR><<<blocks, 256, 0, s>>>(so, qf, q, sor, ii, jj, first, nlines);
+        else
+            relax_lines_k<T, false, DIR><<<blocks, 256, 0, s>>>(so, qf, q, sor, ii, jj, first, nlines);
+    }
+}
+
+template <typename T>
+void relax_lines_x(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
+                   int updown, hipStream_t s) {
+    relax_lines<T, 0>(so, nstncl, qf, q, sor, ii, jj, updown, s);
+}
+
+template <typename T>
+void relax_lines_y(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
+                   int updown, hipStream_t s) {
+    relax_lines<T, 1>(so, nstncl, qf, q, sor, ii, jj, updown, s);
+}
+
+template <typename T>
+void restrict_(const T *ci, const T *q, T *qc, int iif, int jjf, int iic, int jjc, hipStream_t s) {
+    restrict_k<T><<<grid2(iic - 2, jjc - 2, 64, 4), 256, 0, s>>>(ci, q, qc, iif, jjf, iic, jjc);
+}
+
+template <typename T>
+void interp_add(T *q, const T *qc, const T *res, const T *so, int nstncl, const T *ci, int iic,
+                int jjc, int iif, int jjf, hipStream_t s) {
+    (void)nstncl;
+    interp_add_k<T><<<grid2(iif - 1, jjf - 1, 64, 4), 256, 0, s>>>(q, qc, res, so, ci, iic, jjc,
+                                                                   iif, jjf);
+}
+
+void setup_interp(const double *so, int nstncl, double *ci, int iif, int jjf, int iic, int jjc,
+                  hipStream_t s) {
+    const int icmax = (iif - 2) / 2 + 1, jcmax = (jjf - 2) / 2 + 1;
+    const int ne = icmax > iic - 2 ? icmax : iic - 2;
+    const int me = jcmax > jjc - 2 ? jcmax : jjc - 2;
+    const dim3 ge = grid2(ne, me, 64, 4);
+    const dim3 gc = grid2(icmax - 1, jcmax - 1, 64, 4);
+    if (nstncl == 5) {
+        oi_edges_k<true><<<ge, 256, 0, s>>>(so, ci, iif, jjf, iic, jjc);
+        oi_corners_k<true><<<gc, 256, 0, s>>>(so, ci, iif, jjf, iic, jjc);
+    } else {
+        oi_edges_k<false><<<ge, 256, 0, s>>>(so, ci, iif, jjf, iic, jjc);
+        oi_corners_k<false><<<gc, 256, 0, s>>>(so, ci, iif, jjf, iic, jjc);
+    }
+}
+
+void setup_galerkin(const double *so, int nstncl, const double *ci, double *soc, int iif, int jjf,
+                    int iic, int jjc, hipStream_t s) {
+    const dim3 g = grid2(iic - 2, jjc - 2, 64, 4);
+    if (nstncl == 5)
+        galerkin_k<true><<<g, 256, 0, s>>>(so, ci, soc, iif, jjf, iic, jjc);
+    else
+        galerkin_k<false><<<g, 256, 0, s>>>(so, ci, soc, iif, jjf, iic, jjc);
+}
+
+void setup_cg(const double *so, int nstncl, double *abd, int ii, int jj, int *info, hipStream_t s) {
+    setup_cg_k<<<1, 256, 0, s>>>(so, nstncl, abd, ii, jj, info);
+}
+
+template <typename T>
+void solve_cg(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj, hipStream_t s) {
+    solve_cg_k<T><<<1, 64, 0, s>>>(q, qf, abd, bbd, ii, jj);
+}
+
+template <typename Tin, typename Tout>
+void convert(const Tin *in, Tout *out, std::size_t n, hipStream_t s) {
+    convert_k<Tin, Tout><<<grid1(n), 256, 0, s>>>(in, out, n);
+}
+
+template <typename T>
+void fill(T *x, T v, std::size_t n, hipStream_t s) {
+    fill_k<T><<<grid1(n), 256, 0, s>>>(x, v, n);
+}
+
+void add_correction(double *x, const float *e, int ii, int jj, hipStream_t s) {
+    add_correction_k<<<grid2(ii - 2, jj - 2, 64, 4), 256, 0, s>>>(x, e, ii, jj);
+}
+
+// explicit instantiations
+#define CDR_INST(T)                                                                               \
+    template void residual<T>(const T *, int, const T *, const T *, T *, int, int, hipStream_t);  \
+    template void residual_norm<T>(const T *, int, const T *, const T *, T *, int, int, double *, \
+                                   double *, hipStream_t);                                        \
+    template void sumsq<T>(const T *, int, int, double *, double *, hipStream_t);                 \
+    template void relax_gs<T>(const T *, int, const T *, T *, const T *, int, int, int,           \
+                              hipStream_t);                                                       \
+    template void relax_lines_x<T>(const T *, int, const T *, T *, const T *, int, int, int,      \
+                                   hipStream_t);                                                  \
+    template void relax_lines_y<T>(const T *, int, const T *, T *, const T *, int, int, int,      \
+                                   hipStream_t);                                                  \
+    template void restrict_<T>(const T *, const T *, T *, int, int, int, int, hipStream_t);       \
+    template void interp_add<T>(T *, const T *, const T *, const T *, int, const T *, int, int,   \
+                                int, int, hipStream_t);                                           \
+    template void solve_cg<T>(T *, const T *, const double *, double *, int, int, hipStream_t);   \
+    template void fill<T>(T *, T, std::size_t, hipStream_t);
+CDR_INST(double)
+CDR_INST(float)
+#undef CDR_INST
+template void fill<int>(int *, int, std::size_t, hipStream_t);
+template void convert<double, float>(const double *, float *, std::size_t, hipStream_t);
+template void convert<float, double>(const float *, double *, std::size_t, hipStream_t);
+
+}  // namespace k2
+}  // namespace cdr

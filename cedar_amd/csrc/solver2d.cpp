@@ -1,0 +1,550 @@
+// cdr2 solver on the GPU: "hip" kernel implementations, the level hierarchy, setup, V-cycle and
+// the outer solve loop (FP64, or an FP32 V-cycle preconditioning FP64 iterative refinement),
+// plus the solver half of the C ABI.
+//
+// Control flow follows Cedar: multilevel::setup (include/cedar/multilevel.h:243-265),
+// multilevel::solve (include/cedar/multilevel.h:278-298), vcycle::ncycle_helper
+// (include/cedar/cycle/vcycle.h:57-115), smoother sequencing (include/cedar/multilevel.h:165-222),
+// level sizes (include/cedar/2d/solver.h:63-123).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <type_traits>
+#include <vector>
+
+#include "bmg2_launch.hpp"
+#include "cedar_hip.hpp"
+
+namespace cedar {
+namespace hip {
+
+namespace k2 = cdr::k2;
+
+namespace {
+
+template <typename T>
+constexpr bool is_f64 = std::is_same<T, double>::value;
+
+// ------------------------------------------------------------------------------------------
+// "hip" implementations: thin forwards to the C ABI
+// ------------------------------------------------------------------------------------------
+template <typename T>
+struct hip_residual : kernels::residual<T> {
+    void run(const stencil_op<T> &so, const grid_func<T> &x, const grid_func<T> &b,
+             grid_func<T> &r) override {
+        if constexpr (is_f64<T>)
+            check(cdr_hip_residual2d_f64(so.p, so.nstncl, b.p, x.p, r.p, so.ext.ii, so.ext.jj, this->stream), "residual");
+        else
+            check(cdr_hip_residual2d_f32(so.p, so.nstncl, b.p, x.p, r.p, so.ext.ii, so.ext.jj, this->stream), "residual");
+    }
+};
+
+template <typename T>
+struct hip_rbgs : kernels::point_relax<T> {
+    void setup(const stencil_op<double> &so, relax_stencil<double> &sor) override {
+        check(cdr_hip_setup_recip2d_f64(so.p, sor.p, so.ext.ii, so.ext.jj, so.nstncl, this->stream), "setup_recip");
+    }
+    void run(const stencil_op<T> &so, grid_func<T> &x, const grid_func<T> &b,
+             const relax_stencil<T> &sor, Dir d) override {
+        if constexpr (is_f64<T>)
+            check(cdr_hip_relax_gs2d_f64(so.p, so.nstncl, b.p, x.p, sor.p, so.ext.ii, so.ext.jj, (int)d, this->stream), "relax_gs");
+        else
+            check(cdr_hip_relax_gs2d_f32(so.p, so.nstncl, b.p, x.p, sor.p, so.ext.ii, so.ext.jj, (int)d, this->stream), "relax_gs");
+    }
+};
+
+template <typename T, relax_dir RD>
+struct hip_lines : kernels::line_relax<T, RD> {
+    explicit hip_lines(int *info) : info_(info) {}
+    void setup(const stencil_op<double> &so, relax_stencil<double> &sor) override {
+        if (RD == relax_dir::x)
+            check(cdr_hip_setup_lines_x2d_f64(so.p, sor.p, so.ext.ii, so.ext.jj, so.nstncl, info_, this->stream), "setup_lines_x");
+        else
+            check(cdr_hip_setup_lines_y2d_f64(so.p, sor.p, so.ext.ii, so.ext.jj, so.nstncl, info_, this->stream), "setup_lines_y");
+    }
+    void run(const stencil_op<T> &so, grid_func<T> &x, const grid_func<T> &b,
+             const relax_stencil<T> &sor, grid_func<T> &res, Dir d) override {
+        const int ii = so.ext.ii, jj = so.ext.jj;
+        int st;
+        if constexpr (is_f64<T>)
+            st = RD == relax_dir::x
+                     ? cdr_hip_relax_lines_x2d_f64(so.p, so.nstncl, b.p, x.p, sor.p, res.p, ii, jj, (int)d, this->stream)
+                     : cdr_hip_relax_lines_y2d_f64(so.p, so.nstncl, b.p, x.p, sor.p, res.p, ii, jj, (int)d, this->stream);
+        else
+            st = RD == relax_dir::x
+                     ? cdr_hip_relax_lines_x2d_f32(so.p, so.nstncl, b.p, x.p, sor.p, res.p, ii, jj, (int)d, this->stream)
+                     : cdr_hip_relax_lines_y2d_f32(so.p, so.nstncl, b.p, x.p, sor.p, res.p, ii, jj, (int)d, this->stream);
+        check(st, "relax_lines");
+    }
+
+private:
+    int *info_;
+};
+
+struct hip_setup_interp : kernels::setup_interp {
+    void run(const stencil_op<double> &fop, const stencil_op<double> &cop,
+             prolong_op<double> &P) override {
+        P.fine_op = &fop;
+        check(cdr_hip_setup_interp2d_f64(fop.p, fop.nstncl, P.p, fop.ext.ii, fop.ext.jj, cop.ext.ii, cop.ext.jj, this->stream), "setup_interp");
+    }
+};
+
+struct hip_galerkin : kernels::coarsen_op {
+    void run(const prolong_op<double> &P, const stencil_op<double> &fop,
+             stencil_op<double> &cop) override {
+        check(cdr_hip_setup_galerkin2d_f64(fop.p, fop.nstncl, P.p, cop.p, fop.ext.ii, fop.ext.jj, cop.ext.ii, cop.ext.jj, this->stream), "galerkin");
+    }
+};
+
+template <typename T>
+struct hip_restrict : kernels::restriction<T> {
+    void run(const prolong_op<T> &R, const grid_func<T> &fine, grid_func<T> &coarse) override {
+        if constexpr (is_f64<T>)
+            check(cdr_hip_restrict2d_f64(R.p, fine.p, coarse.p, fine.ext.ii, fine.ext.jj, coarse.ext.ii, coarse.ext.jj, this->stream), "restrict");
+        else
+            check(cdr_hip_restrict2d_f32(R.p, fine.p, coarse.p, fine.ext.ii, fine.ext.jj, coarse.ext.ii, coarse.ext.jj, this->stream), "restrict");
+    }
+};
+
+template <typename T>
+struct hip_interp : kernels::interp_add<T> {
+    void run(const prolong_op<T> &P, const grid_func<T> &coarse, const grid_func<T> &residual,
+             grid_func<T> &fine) override {
+        const stencil_op<T> &fop = *P.fine_op;
+        if constexpr (is_f64<T>)
+            check(cdr_hip_interp_add2d_f64(fine.p, coarse.p, residual.p, fop.p, fop.nstncl, P.p, coarse.ext.ii, coarse.ext.jj, fine.ext.ii, fine.ext.jj, this->stream), "interp_add");
+        else
+            check(cdr_hip_interp_add2d_f32(fine.p, coarse.p, residual.p, fop.p, fop.nstncl, P.p, coarse.ext.ii, coarse.ext.jj, fine.ext.ii, fine.ext.jj, this->stream), "interp_add");
+    }
+};
+
+template <typename T>
+struct hip_solve_cg : kernels::solve_cg<T> {
+    explicit hip_solve_cg(int *info) : info_(info) {}
+    void setup(const stencil_op<double> &so, grid_func<double> &abd) override {
+        check(cdr_hip_setup_cg2d_f64(so.p, so.nstncl, abd.p, so.ext.ii, so.ext.jj, info_, this->stream), "setup_cg");
+    }
+    void run(grid_func<T> &x, const grid_func<T> &b, const grid_func<double> &abd,
+             double *bbd) override {
+        (void)abd;
+        if constexpr (is_f64<T>)
+            check(cdr_hip_solve_cg2d_f64(x.p, b.p, abd.p, bbd, x.ext.ii, x.ext.jj, this->stream), "solve_cg");
+        else
+            check(cdr_hip_solve_cg2d_f32(x.p, b.p, abd.p, bbd, x.ext.ii, x.ext.jj, this->stream), "solve_cg");
+    }
+
+private:
+    int *info_;
+};
+
+}  // namespace
+
+void register_hip_kernels(kernel_manager &km, int *info) {
+    km.add<kernels::residual<double>, hip_residual<double>>("hip");
+    km.add<kernels::residual<float>, hip_residual<float>>("hip");
+    km.add<kernels::point_relax<double>, hip_rbgs<double>>("hip");
+    km.add<kernels::point_relax<float>, hip_rbgs<float>>("hip");
+    km.add<kernels::line_relax<double, relax_dir::x>, hip_lines<double, relax_dir::x>>("hip", info);
+    km.add<kernels::line_relax<double, relax_dir::y>, hip_lines<double, relax_dir::y>>("hip", info);
+    km.add<kernels::line_relax<float, relax_dir::x>, hip_lines<float, relax_dir::x>>("hip", info);
+    km.add<kernels::line_relax<float, relax_dir::y>, hip_lines<float, relax_dir::y>>("hip", info);
+    km.add<kernels::setup_interp, hip_setup_interp>("hip");
+    km.add<kernels::coarsen_op, hip_galerkin>("hip");
+    km.add<kernels::restriction<double>, hip_restrict<double>>("hip");
+    km.add<kernels::restriction<float>, hip_restrict<float>>("hip");
+    km.add<kernels::interp_add<double>, hip_interp<double>>("hip");
+    km.add<kernels::interp_add<float>, hip_interp<float>>("hip");
+    km.add<kernels::solve_cg<double>, hip_solve_cg<double>>("hip", info);
+    km.add<kernels::solve_cg<float>, hip_solve_cg<float>>("hip", info);
+}
+
+// ------------------------------------------------------------------------------------------
+// level storage (one precision)
+// ------------------------------------------------------------------------------------------
+template <typename T>
+struct level_data {
+    extents ext;
+    dbuf<T> so_buf, ci_buf, x_buf, b_buf, res_buf, sor_buf[2];
+    stencil_op<T> A;
+    prolong_op<T> P;  // stored on this (coarse) level; unused on level 0
+    grid_func<T> x, b, res;
+    relax_stencil<T> SOR[2];
+};
+
+class solver2d {
+public:
+    solver2d(const double *so_dev, int nstncl, int nx, int ny, const cdr_solver_settings &s,
+             hipStream_t stream)
+        : set_(s), stream_(stream), km_(stream) {
+        check_hip(hipEventCreate(&ev0_), "hipEventCreate");
+        check_hip(hipEventCreate(&ev1_), "hipEventCreate");
+        info_ = dbuf<int>(1, stream_);
+        register_hip_kernels(km_, info_.data());
+        build(so_dev, nstncl, nx, ny);
+    }
+
+    ~solver2d() {
+        (void)hipEventDestroy(ev0_);
+        (void)hipEventDestroy(ev1_);
+        if (host_norm_) (void)hipHostFree(host_norm_);
+    }
+
+    int nlevels() const { return (int)L64_.size(); }
+    const level_data<double> &level(int l) const { return L64_[l]; }
+    double setup_seconds() const { return setup_s_; }
+    double solve_seconds() const { return solve_s_; }
+    kernel_manager &kernels() { return km_; }
+
+    // include/cedar/2d/solver.h:63-79: float coarse sizes, unsigned division
+    static int compute_num_levels(int nx, int ny, int min_coarse) {
+        float nxc, nyc;
+        int ng = 0;
+        do {
+            ng++;
+            nxc = (float)((unsigned)(nx - 1) / (unsigned)(1 << ng) + 1);
+            nyc = (float)((unsigned)(ny - 1) / (unsigned)(1 << ng) + 1);
+        } while ((nxc < nyc ? nxc : nyc) >= (float)min_coarse);
+        return ng;
+    }
+
+    void vcycle64(const double *b_dev, double *x_dev) {
+        grid_func<double> x{x_dev, L64_[0].ext}, b{const_cast<double *>(b_dev), L64_[0].ext};
+        run_cycle<double>(L64_, x, b);
+    }
+
+    int solve(const double *b_dev, double *x_dev, double *hist, int maxhist, int *niter) {
+        const extents e = L64_[0].ext;
+        const stencil_op<double> &A = L64_[0].A;
+        check_hip(hipEventRecord(ev0_, stream_), "hipEventRecord");
+        int it = 0, status = CDR_OK;
+        if (set_.precision == CDR_PREC_F64) {
+            grid_func<double> x{x_dev, e}, b{const_cast<double *>(b_dev), e};
+            check(cdr_hip_residual_norm2d_f64(A.p, A.nstncl, b.p, x.p, L64_[0].res.p, e.ii, e.jj,
+                                              work_.data(), sumsq_.data(), stream_), "residual_norm");
+            const double res0 = std::sqrt(read_norm2());
+            for (int i = 0; i < set_.maxiter; ++i) {
+                run_cycle<double>(L64_, x, b);
+                check(cdr_hip_residual_norm2d_f64(A.p, A.nstncl, b.p, x.p, L64_[0].res.p, e.ii,
+                                                  e.jj, work_.data(), sumsq_.data(), stream_), "residual_norm");
+                const double rel = std::sqrt(read_norm2()) / res0;
+                if (i < maxhist && hist) hist[i] = rel;
+                it = i + 1;
+                if (!std::isfinite(rel)) { status = CDR_ERR_NONFINITE; break; }
+                if (rel < set_.tol) break;
+            }
+        } else {
+            // FP64 iterative refinement around an FP32 V-cycle:
+            //   r = b - A x (FP64), e = Vcycle32(r), x <- x + e, fused into one residual pass.
+            double *xa = xa_.data(), *xb = xb_.data();
+            level_data<float> &F = L32_[0];
+            k2::residual_ir(A.p, A.nstncl, b_dev, x_dev, nullptr, xa, F.b.p, e.ii, e.jj,
+                            work_.data(), sumsq_.data(), stream_);
+            check_hip(hipGetLastError(), "residual_ir");
+            const double res0 = std::sqrt(read_norm2());
+            for (int i = 0; i < set_.maxiter; ++i) {
+                k2::fill<float>(F.x.p, 0.0f, e.points(), stream_);
+                run_cycle<float>(L32_, F.x, F.b);
+                k2::residual_ir(A.p, A.nstncl, b_dev, xa, F.x.p, xb, F.b.p, e.ii, e.jj,
+                                work_.data(), sumsq_.data(), stream_);
+                check_hip(hipGetLastError(), "residual_ir");
+                std::swap(xa, xb);
+                const double rel = std::sqrt(read_norm2()) / res0;
+                if (i < maxhist && hist) hist[i] = rel;
+                it = i + 1;
+                if (!std::isfinite(rel)) { status = CDR_ERR_NONFINITE; break; }
+                if (rel < set_.tol) break;
+            }
+            check_hip(hipMemcpyAsync(x_dev, xa, e.points() * sizeof(double),
+                                     hipMemcpyDeviceToDevice, stream_), "copy x");
+        }
+        check_hip(hipEventRecord(ev1_, stream_), "hipEventRecord");
+        check_hip(hipEventSynchronize(ev1_), "hipEventSynchronize");
+        float ms = 0.f;
+        check_hip(hipEventElapsedTime(&ms, ev0_, ev1_), "hipEventElapsedTime");
+        solve_s_ = ms * 1e-3;
+        if (niter) *niter = it;
+        return status;
+    }
+
+private:
+    cdr_solver_settings set_;
+    hipStream_t stream_;
+    kernel_manager km_;
+    std::vector<level_data<double>> L64_;
+    std::vector<level_data<float>> L32_;
+    dbuf<double> abd_, bbd_, work_, sumsq_, xa_, xb_;
+    dbuf<int> info_;
+    grid_func<double> abd_g_;
+    double *host_norm_ = nullptr;
+    hipEvent_t ev0_{}, ev1_{};
+    double setup_s_ = 0, solve_s_ = 0;
+
+    double read_norm2() {
+        check_hip(hipMemcpyAsync(host_norm_, sumsq_.data(), sizeof(double), hipMemcpyDeviceToHost,
+                                 stream_), "read norm");
+        check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+        return *host_norm_;
+    }
+
+    template <typename T>
+    static void alloc_level(level_data<T> &L, extents e, int nstncl, bool own_op, bool need_ci,
+                            bool need_xb, hipStream_t s) {
+        L.ext = e;
+        const std::size_t np = e.points();
+        if (own_op) {
+            L.so_buf = dbuf<T>(np * (std::size_t)nstncl, s);
+            L.A = stencil_op<T>{L.so_buf.data(), e, nstncl};
+        }
+        if (need_ci) {
+            L.ci_buf = dbuf<T>(np * 8, s);
+            L.P = prolong_op<T>{L.ci_buf.data(), e, nullptr};
+        }
+        if (need_xb) {
+            L.x_buf = dbuf<T>(np, s);
+            L.b_buf = dbuf<T>(np, s);
+            L.x = grid_func<T>{L.x_buf.data(), e};
+            L.b = grid_func<T>{L.b_buf.data(), e};
+        }
+        L.res_buf = dbuf<T>(np, s);
+        L.res = grid_func<T>{L.res_buf.data(), e};
+        for (int k = 0; k < 2; ++k) {
+            L.sor_buf[k] = dbuf<T>(np * 2, s);
+            L.SOR[k] = relax_stencil<T>{L.sor_buf[k].data(), e};
+        }
+    }
+
+    void build(const double *so_dev, int nstncl, int nx, int ny) {
+        if (!so_dev || (nstncl != 3 && nstncl != 5) || nx < 1 || ny < 1)
+            throw error(CDR_ERR_ARG, "invalid fine operator");
+        int nl = compute_num_levels(nx, ny, set_.min_coarse);
+        if (set_.num_levels > 0) {
+            if (set_.num_levels > nl) throw error(CDR_ERR_LEVELS, "too many levels specified");
+            nl = set_.num_levels;
+        }
+        // setup_space (include/cedar/2d/solver.h:81-123)
+        L64_.resize(nl);
+        int cx = nx, cy = ny;
+        for (int l = 0; l < nl; ++l) {
+            extents e{cx + 2, cy + 2};
+            if (l == 0) {
+                alloc_level(L64_[0], e, nstncl, false, false, false, stream_);
+                L64_[0].A = stencil_op<double>{const_cast<double *>(so_dev), e, nstncl};
+            } else {
+                alloc_level(L64_[l], e, 5, true, true, true, stream_);
+            }
+            cx = (int)((cx - 1) / 2.0 + 1);
+            cy = (int)((cy - 1) / 2.0 + 1);
+        }
+        const extents ec = L64_[nl - 1].ext;
+        abd_ = dbuf<double>((std::size_t)ec.ii * ec.nx() * ec.ny(), stream_);
+        bbd_ = dbuf<double>((std::size_t)ec.nx() * ec.ny(), stream_);
+        work_ = dbuf<double>(cdr_hip_norm_work2d(L64_[0].ext.ii, L64_[0].ext.jj), stream_);
+        sumsq_ = dbuf<double>(1, stream_);
+        check_hip(hipHostMalloc(&host_norm_, sizeof(double)), "hipHostMalloc");
+
+        check_hip(hipEventRecord(ev0_, stream_), "hipEventRecord");
+        // setup: interp -> Galerkin -> relaxation per level, then the coarse factorisation
+        for (int l = 0; l < nl - 1; ++l) {
+            auto &F = L64_[l];
+            auto &C = L64_[l + 1];
+            km_.run<kernels::setup_interp>(F.A, C.A, C.P);
+            km_.run<kernels::coarsen_op>(C.P, F.A, C.A);
+            setup_relax(F);
+        }
+        km_.setup<kernels::solve_cg<double>>(L64_[nl - 1].A, abd_grid());
+
+        if (set_.precision == CDR_PREC_MIXED) build_f32();
+        check_hip(hipEventRecord(ev1_, stream_), "hipEventRecord");
+        check_hip(hipEventSynchronize(ev1_), "hipEventSynchronize");
+        float ms = 0.f;
+        check_hip(hipEventElapsedTime(&ms, ev0_, ev1_), "hipEventElapsedTime");
+        setup_s_ = ms * 1e-3;
+    }
+
+    grid_func<double> &abd_grid() {
+        const extents ec = L64_.back().ext;
+        abd_g_ = grid_func<double>{abd_.data(), extents{ec.ii, ec.nx() * ec.ny()}};
+        return abd_g_;
+    }
+
+    void setup_relax(level_data<double> &F) {
+        switch (set_.relaxation) {
+        case CDR_RELAX_POINT: km_.setup<kernels::point_relax<double>>(F.A, F.SOR[0]); break;
+        case CDR_RELAX_LINE_X: km_.setup<kernels::line_relax<double, relax_dir::x>>(F.A, F.SOR[0]); break;
+        case CDR_RELAX_LINE_Y: km_.setup<kernels::line_relax<double, relax_dir::y>>(F.A, F.SOR[0]); break;
+        case CDR_RELAX_LINE_XY:
+            km_.setup<kernels::line_relax<double, relax_dir::x>>(F.A, F.SOR[0]);
+            km_.setup<kernels::line_relax<double, relax_dir::y>>(F.A, F.SOR[1]);
+            break;
+        default: throw error(CDR_ERR_ARG, "invalid relaxation");
+        }
+    }
+
+    // FP32 copy of the hierarchy (operators, interpolation, smoother factors)
+    void build_f32() {
+        const int nl = nlevels();
+        L32_.resize(nl);
+        xa_ = dbuf<double>(L64_[0].ext.points(), stream_);
+        xb_ = dbuf<double>(L64_[0].ext.points(), stream_);
+        for (int l = 0; l < nl; ++l) {
+            auto &D = L64_[l];
+            auto &F = L32_[l];
+            alloc_level(F, D.ext, D.A.nstncl, true, l > 0, true, stream_);
+            const std::size_t np = D.ext.points();
+            k2::convert<double, float>(D.A.p, F.A.p, np * D.A.nstncl, stream_);
+            if (l > 0) k2::convert<double, float>(D.P.p, F.P.p, np * 8, stream_);
+            for (int k = 0; k < 2; ++k) k2::convert<double, float>(D.SOR[k].p, F.SOR[k].p, np * 2, stream_);
+        }
+        for (int l = 1; l < nl; ++l) L32_[l].P.fine_op = &L32_[l - 1].A;
+        check_hip(hipGetLastError(), "convert");
+    }
+
+    template <typename T>
+    void smooth(level_data<T> &L, grid_func<T> &x, const grid_func<T> &b, Dir d) {
+        const int nrep = d == Dir::DOWN ? set_.nrelax_pre : set_.nrelax_post;
+        for (int r = 0; r < nrep; ++r) {
+            switch (set_.relaxation) {
+            case CDR_RELAX_POINT: km_.run<kernels::point_relax<T>>(L.A, x, b, L.SOR[0], d); break;
+            case CDR_RELAX_LINE_X: km_.run<kernels::line_relax<T, relax_dir::x>>(L.A, x, b, L.SOR[0], L.res, d); break;
+            case CDR_RELAX_LINE_Y: km_.run<kernels::line_relax<T, relax_dir::y>>(L.A, x, b, L.SOR[0], L.res, d); break;
+            default:
+                if (d == Dir::DOWN) {
+                    km_.run<kernels::line_relax<T, relax_dir::x>>(L.A, x, b, L.SOR[0], L.res, d);
+                    km_.run<kernels::line_relax<T, relax_dir::y>>(L.A, x, b, L.SOR[1], L.res, d);
+                } else {
+                    km_.run<kernels::line_relax<T, relax_dir::y>>(L.A, x, b, L.SOR[1], L.res, d);
+                    km_.run<kernels::line_relax<T, relax_dir::x>>(L.A, x, b, L.SOR[0], L.res, d);
+                }
+            }
+        }
+    }
+
+    template <typename T>
+    void run_cycle(std::vector<level_data<T>> &L, grid_func<T> &x, const grid_func<T> &b) {
+        if (L.size() == 1) {
+            km_.run<kernels::solve_cg<T>>(x, b, abd_grid(), bbd_.data());
+            return;
+        }
+        ncycle<T>(L, 0, x, b);
+    }
+
+    template <typename T>
+    void ncycle(std::vector<level_data<T>> &L, int lvl, grid_func<T> &x, const grid_func<T> &b) {
+        auto &F = L[lvl];
+        auto &C = L[lvl + 1];
+        smooth<T>(F, x, b, Dir::DOWN);
+        km_.run<kernels::residual<T>>(F.A, x, b, F.res);
+        km_.run<kernels::restriction<T>>(C.P, F.res, C.b);
+        k2::fill<T>(C.x.p, T(0), C.ext.points(), stream_);
+        if (lvl + 1 == (int)L.size() - 1)
+            km_.run<kernels::solve_cg<T>>(C.x, C.b, abd_grid(), bbd_.data());
+        else
+            ncycle<T>(L, lvl + 1, C.x, C.b);
+        km_.run<kernels::interp_add<T>>(C.P, C.x, F.res, x);
+        smooth<T>(F, x, b, Dir::UP);
+    }
+};
+
+}  // namespace hip
+}  // namespace cedar
+
+// ==========================================================================================
+// C ABI: solver handle
+// ==========================================================================================
+struct cdr_hip_solver2d {
+    std::unique_ptr<cedar::hip::solver2d> impl;
+};
+
+namespace {
+template <typename F>
+int guarded(F f) {
+    try {
+        return f();
+    } catch (const cedar::hip::error &e) {
+        std::fprintf(stderr, "cedar_hip: %s\n", e.what());
+        return e.status;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "cedar_hip: %s\n", e.what());
+        return CDR_ERR_HIP;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+void cdr_hip_default_settings(cdr_solver_settings *s) {
+    if (!s) return;
+    s->relaxation = CDR_RELAX_POINT;
+    s->nrelax_pre = 2;
+    s->nrelax_post = 1;
+    s->num_levels = -1;
+    s->maxiter = 10;
+    s->tol = 1e-8;
+    s->min_coarse = 3;
+    s->precision = CDR_PREC_F64;
+}
+
+int cdr_hip_solver2d_create(const double *so_dev, int nstncl, int nx, int ny,
+                            const cdr_solver_settings *s, void *stream, cdr_hip_solver2d **out) {
+    if (!so_dev || !s || !out || nx < 1 || ny < 1 || (nstncl != 3 && nstncl != 5) ||
+        s->relaxation < CDR_RELAX_POINT || s->relaxation > CDR_RELAX_LINE_XY ||
+        s->nrelax_pre < 0 || s->nrelax_post < 0 || s->maxiter < 0 || s->min_coarse < 2 ||
+        (s->precision != CDR_PREC_F64 && s->precision != CDR_PREC_MIXED))
+        return CDR_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CDR_ERR_NO_DEVICE;
+    return guarded([&] {
+        auto h = std::make_unique<cdr_hip_solver2d>();
+        h->impl = std::make_unique<cedar::hip::solver2d>(so_dev, nstncl, nx, ny, *s,
+                                                         reinterpret_cast<hipStream_t>(stream));
+        *out = h.release();
+        return (int)CDR_OK;
+    });
+}
+
+int cdr_hip_solver2d_solve(cdr_hip_solver2d *slv, const double *b_dev, double *x_dev,
+                           double *hist, int maxhist, int *niter) {
+    if (!slv || !b_dev || !x_dev || maxhist < 0) return CDR_ERR_ARG;
+    return guarded([&] { return slv->impl->solve(b_dev, x_dev, hist, maxhist, niter); });
+}
+
+int cdr_hip_solver2d_vcycle(cdr_hip_solver2d *slv, const double *b_dev, double *x_dev) {
+    if (!slv || !b_dev || !x_dev) return CDR_ERR_ARG;
+    return guarded([&] {
+        slv->impl->vcycle64(b_dev, x_dev);
+        return (int)CDR_OK;
+    });
+}
+
+int cdr_hip_solver2d_nlevels(const cdr_hip_solver2d *slv) {
+    return slv ? slv->impl->nlevels() : 0;
+}
+
+int cdr_hip_solver2d_level(const cdr_hip_solver2d *slv, int lvl, int *nx, int *ny,
+                           const double **so, const double **ci, const double **sor0,
+                           const double **sor1) {
+    if (!slv || lvl < 0 || lvl >= slv->impl->nlevels()) return CDR_ERR_ARG;
+    const auto &L = slv->impl->level(lvl);
+    if (nx) *nx = L.ext.nx();
+    if (ny) *ny = L.ext.ny();
+    if (so) *so = L.A.p;
+    if (ci) *ci = lvl > 0 ? L.P.p : nullptr;
+    if (sor0) *sor0 = L.SOR[0].p;
+    if (sor1) *sor1 = L.SOR[1].p;
+    return CDR_OK;
+}
+
+int cdr_hip_solver2d_times(const cdr_hip_solver2d *slv, double *setup_s, double *solve_s) {
+    if (!slv) return CDR_ERR_ARG;
+    if (setup_s) *setup_s = slv->impl->setup_seconds();
+    if (solve_s) *solve_s = slv->impl->solve_seconds();
+    return CDR_OK;
+}
+
+void cdr_hip_solver2d_destroy(cdr_hip_solver2d *slv) { delete slv; }
+
+}  // extern "C"

@@ -1,0 +1,197 @@
+/*
+ * cedar_hip.h -- C ABI of the MI355X (gfx950) BoxMG kernels and solver.
+ *
+ * Drop-in for Cedar's serial 2D kernel set: every kernel entry point below replaces one
+ * BIND(C) Fortran subroutine that Cedar's C++ adapters call (cited per function), with the same
+ * data layout and argument meaning, but operating on DEVICE pointers and enqueued on a HIP
+ * stream.  Layout (unchanged from the reference):
+ *   - grid functions: Fortran order, one ghost layer, ghost-inclusive extents ii = nx+2, jj = ny+2;
+ *   - stencils: nstncl contiguous planes of ii*jj values, 3 planes {c,w,s} (5-point) or
+ *     5 planes {c,w,s,sw,nw} (9-point), off-diagonals stored negated;
+ *   - prolongation CI: 8 planes {L,R,A,B,SW,NW,NE,SE} on the coarse grid (iic x jjc);
+ *   - relaxation stencil SOR: 2 planes of ii*jj; point GS keeps 1/c in plane 2 (msor); x-line
+ *     factors (D in plane 1, L multipliers in plane 2) as in the reference, y-line factors
+ *     stored transposed (jj x ii per plane).
+ * Every function returns a cdr_status; none throws.  Kernel entry points do not allocate and
+ * do not synchronise the stream unless stated ("synchronises").  `stream` is a hipStream_t
+ * (NULL = the default stream).  Suffix _f64 computes in double, _f32 in float.
+ *
+ * Boundary conditions: Dirichlet (Cedar's BMG_BCs_definite, per_mask = 0).
+ */
+#ifndef CEDAR_HIP_H
+#define CEDAR_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    CDR_OK = 0,
+    CDR_ERR_ARG = 1,         /* bad pointer, extent, stencil width or option */
+    CDR_ERR_HIP = 2,         /* a HIP runtime call or kernel launch failed */
+    CDR_ERR_NOT_SPD = 3,     /* a factorisation met a non-positive pivot (LAPACK INFO > 0) */
+    CDR_ERR_NONFINITE = 4,   /* NaN or Inf in a residual norm */
+    CDR_ERR_NO_DEVICE = 5,   /* no usable GPU */
+    CDR_ERR_LEVELS = 6,      /* num_levels larger than the grid allows */
+    CDR_ERR_STALLED = 7      /* the solve stopped reducing the residual */
+} cdr_status;
+
+enum { CDR_DOWN = 0, CDR_UP = 1 };  /* cycle::Dir, BMG_DOWN / BMG_UP */
+
+const char *cdr_hip_status_string(int status);
+/* Library version string ("cedar_amd <version> gfx950"). */
+const char *cdr_hip_version(void);
+
+/* ---------------------------------------------------------------------------------------
+ * 2D kernels (FP64 and FP32 variants where the V-cycle needs them)
+ * ------------------------------------------------------------------------------------- */
+
+/* r = f + sum(off-diagonal couplings * q) - c q over the interior.
+ * Replaces BMG2_SymStd_residual (include/cedar/2d/residual.h:9-10, src/2d/ftn/BMG2_SymStd_residual.f90:1-119). */
+int cdr_hip_residual2d_f64(const double *so, int nstncl, const double *qf, const double *q,
+                           double *res, int ii, int jj, void *stream);
+int cdr_hip_residual2d_f32(const float *so, int nstncl, const float *qf, const float *q,
+                           float *res, int ii, int jj, void *stream);
+
+/* Residual fused with the squared l2 norm of r over the interior: *sumsq_dev (device double)
+ * receives sum(r^2), reduced in a fixed order (deterministic).  work needs
+ * cdr_hip_norm_work2d(ii, jj) doubles of device memory.
+ * Replaces residual + grid_func::lp_norm<2> (include/cedar/2d/grid_func.h:43-54). */
+size_t cdr_hip_norm_work2d(int ii, int jj);
+int cdr_hip_residual_norm2d_f64(const double *so, int nstncl, const double *qf, const double *q,
+                                double *res, int ii, int jj, double *work, double *sumsq_dev,
+                                void *stream);
+/* sum of squares of the interior of x into *sumsq_dev (device). */
+int cdr_hip_sumsq2d_f64(const double *x, int ii, int jj, double *work, double *sumsq_dev,
+                        void *stream);
+
+/* SOR(:,:,2) = 1/c.  Replaces BMG2_SymStd_SETUP_recip (src/2d/relax.cc:8, src/2d/ftn/BMG2_SymStd_SETUP_recip.f90). */
+int cdr_hip_setup_recip2d_f64(const double *so, double *sor, int ii, int jj, int nstncl,
+                              void *stream);
+
+/* Red-black (5-point) or four-colour (9-point) point Gauss-Seidel, colours forward on DOWN and
+ * reversed on UP.  Replaces BMG2_SymStd_relax_GS (src/2d/relax.cc:10-11). */
+int cdr_hip_relax_gs2d_f64(const double *so, int nstncl, const double *qf, double *q,
+                           const double *sor, int ii, int jj, int updown, void *stream);
+int cdr_hip_relax_gs2d_f32(const float *so, int nstncl, const float *qf, float *q,
+                           const float *sor, int ii, int jj, int updown, void *stream);
+
+/* LDL^T factors of every x-line (y-line) of the operator.  Synchronises the stream to report a
+ * non-SPD line as CDR_ERR_NOT_SPD.  info_dev: one device int of scratch.
+ * Replace BMG2_SymStd_SETUP_lines_x / _y (src/2d/relax.cc:8-9). */
+int cdr_hip_setup_lines_x2d_f64(const double *so, double *sor, int ii, int jj, int nstncl,
+                                int *info_dev, void *stream);
+int cdr_hip_setup_lines_y2d_f64(const double *so, double *sor, int ii, int jj, int nstncl,
+                                int *info_dev, void *stream);
+
+/* Zebra line relaxation with the stored factors; one wavefront per line.  The reference's B
+ * scratch argument (level.res) is not needed and may be NULL.
+ * Replace BMG2_SymStd_relax_lines_x / _y (src/2d/relax.cc:12-17). */
+int cdr_hip_relax_lines_x2d_f64(const double *so, int nstncl, const double *qf, double *q,
+                                const double *sor, double *b, int ii, int jj, int updown,
+                                void *stream);
+int cdr_hip_relax_lines_y2d_f64(const double *so, int nstncl, const double *qf, double *q,
+                                const double *sor, double *b, int ii, int jj, int updown,
+                                void *stream);
+int cdr_hip_relax_lines_x2d_f32(const float *so, int nstncl, const float *qf, float *q,
+                                const float *sor, float *b, int ii, int jj, int updown,
+                                void *stream);
+int cdr_hip_relax_lines_y2d_f32(const float *so, int nstncl, const float *qf, float *q,
+                                const float *sor, float *b, int ii, int jj, int updown,
+                                void *stream);
+
+/* qc = P^T q.  Replaces BMG2_SymStd_restrict (src/2d/restrict.cc:6-7). */
+int cdr_hip_restrict2d_f64(const double *ci, const double *q, double *qc, int iif, int jjf,
+                           int iic, int jjc, void *stream);
+int cdr_hip_restrict2d_f32(const float *ci, const float *q, float *qc, int iif, int jjf,
+                           int iic, int jjc, void *stream);
+
+/* q += P qc, plus res/c at the fine-only points.  Replaces BMG2_SymStd_interp_add
+ * (src/2d/interp.cc:8-9).  Difference: the reference divides res by c in place; here res is
+ * only read (Cedar's V-cycle never reads res after this call, include/cedar/cycle/vcycle.h:103-105). */
+int cdr_hip_interp_add2d_f64(double *q, const double *qc, const double *res, const double *so,
+                             int nstncl, const double *ci, int iic, int jjc, int iif, int jjf,
+                             void *stream);
+int cdr_hip_interp_add2d_f32(float *q, const float *qc, const float *res, const float *so,
+                             int nstncl, const float *ci, int iic, int jjc, int iif, int jjf,
+                             void *stream);
+
+/* Operator-induced interpolation weights CI from the fine operator.
+ * Replaces BMG2_SymStd_SETUP_interp_OI (src/2d/interp.cc:10-12). */
+int cdr_hip_setup_interp2d_f64(const double *so, int nstncl, double *ci, int iif, int jjf,
+                               int iic, int jjc, void *stream);
+
+/* Galerkin coarse operator soc = P^T A P (9-point, 5 planes).
+ * Replaces BMG2_SymStd_SETUP_ITLI_ex (include/cedar/2d/coarsen.h:10-12). */
+int cdr_hip_setup_galerkin2d_f64(const double *so, int nstncl, const double *ci, double *soc,
+                                 int iif, int jjf, int iic, int jjc, void *stream);
+
+/* Coarsest grid: banded Cholesky of the operator into abd (ii x nx*ny doubles, kd = ii-1).
+ * Synchronises the stream; CDR_ERR_NOT_SPD if a pivot is not positive.
+ * Replaces BMG2_SymStd_SETUP_cg_LU (include/cedar/2d/solve_cg.h:10). */
+int cdr_hip_setup_cg2d_f64(const double *so, int nstncl, double *abd, int ii, int jj,
+                           int *info_dev, void *stream);
+/* q = A^{-1} qf on the coarsest grid with the factors (FP64 arithmetic for both variants);
+ * bbd: nx*ny doubles of device scratch.  Replaces BMG2_SymStd_SOLVE_cg (src/2d/solve_cg.cc:6). */
+int cdr_hip_solve_cg2d_f64(double *q, const double *qf, const double *abd, double *bbd, int ii,
+                           int jj, void *stream);
+int cdr_hip_solve_cg2d_f32(float *q, const float *qf, const double *abd, double *bbd, int ii,
+                           int jj, void *stream);
+
+/* ---------------------------------------------------------------------------------------
+ * 2D solver (cdr2::solver<five_pt|nine_pt>: setup in create, solve(b, x))
+ * ------------------------------------------------------------------------------------- */
+enum {
+    CDR_RELAX_POINT = 0,
+    CDR_RELAX_LINE_X = 1,
+    CDR_RELAX_LINE_Y = 2,
+    CDR_RELAX_LINE_XY = 3
+};
+enum {
+    CDR_PREC_F64 = 0,    /* plain FP64 V-cycles (Cedar's solve) */
+    CDR_PREC_MIXED = 1   /* FP32 V-cycle preconditioning FP64 iterative refinement */
+};
+
+typedef struct {
+    int relaxation;   /* solver.relaxation: CDR_RELAX_* (default point) */
+    int nrelax_pre;   /* solver.cycle.nrelax-pre (default 2) */
+    int nrelax_post;  /* solver.cycle.nrelax-post (default 1) */
+    int num_levels;   /* solver.num-levels (default -1: computed) */
+    int maxiter;      /* solver.max-iter (default 10) */
+    double tol;       /* solver.tol (default 1e-8) */
+    int min_coarse;   /* solver.min_coarse (default 3) */
+    int precision;    /* CDR_PREC_* (default CDR_PREC_F64) */
+} cdr_solver_settings;
+
+typedef struct cdr_hip_solver2d cdr_hip_solver2d;
+
+/* Cedar's defaults (src/multilevel_settings.cc:15-61). */
+void cdr_hip_default_settings(cdr_solver_settings *s);
+
+/* Builds the hierarchy on the GPU from the fine operator so_dev (device, held by reference as
+ * Cedar holds the fine operator, include/cedar/level.h:25,30) of nx x ny interior points.
+ * Synchronises.  Replaces cdr2::solver<fsten>(so, conf) (include/cedar/2d/solver.h:55-61). */
+int cdr_hip_solver2d_create(const double *so_dev, int nstncl, int nx, int ny,
+                            const cdr_solver_settings *s, void *stream, cdr_hip_solver2d **out);
+/* solve(b, x) from the x given (x0 = 0 gives Cedar's solve(b)); rho_i = ||r_i|| / ||r_0|| go to
+ * hist (host, up to maxhist), *niter gets the cycles run.  Synchronises once per cycle to read
+ * the norm.  Replaces multilevel::solve (include/cedar/multilevel.h:278-298). */
+int cdr_hip_solver2d_solve(cdr_hip_solver2d *slv, const double *b_dev, double *x_dev,
+                           double *hist, int maxhist, int *niter);
+/* One FP64 V-cycle on (x, b) (include/cedar/cycle/vcycle.h:35-54). */
+int cdr_hip_solver2d_vcycle(cdr_hip_solver2d *slv, const double *b_dev, double *x_dev);
+int cdr_hip_solver2d_nlevels(const cdr_hip_solver2d *slv);
+/* Extents of level lvl (0 = finest) and device pointers to its FP64 data (NULL where absent). */
+int cdr_hip_solver2d_level(const cdr_hip_solver2d *slv, int lvl, int *nx, int *ny,
+                           const double **so, const double **ci, const double **sor0,
+                           const double **sor1);
+/* Wall time (s, HIP events) of the last create (setup) and solve. */
+int cdr_hip_solver2d_times(const cdr_hip_solver2d *slv, double *setup_s, double *solve_s);
+void cdr_hip_solver2d_destroy(cdr_hip_solver2d *slv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CEDAR_HIP_H */

@@ -1,0 +1,210 @@
+"""Parity of every 2D HIP kernel (through the C ABI) with the FP64 oracle.
+
+Bitwise: residual, 1/c, Gauss-Seidel, restriction, interpolation, operator-induced
+interpolation, line factorisation, coarse Cholesky (same operation order, no FMA contraction).
+Within 1e-13 (relative to the largest entry): Galerkin coarse operator (formed as P^T A P) and
+line relaxation (wavefront affine scans instead of the sequential LDL^T substitution).
+FP32 kernels: within 1e-5 relative of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.golden.make_golden import rand_so
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+CASES = ["7x9_3", "7x9_5", "8x6_3", "8x6_5", "16x11_3", "16x11_5", "15x15_3", "15x15_5"]
+SIZES = [(3, 3), (4, 4), (5, 8), (33, 17), (64, 65), (130, 129), (257, 200)]
+
+
+@pytest.fixture(scope="module")
+def C():
+    import cedar_amd
+    return cedar_amd
+
+
+def dev(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.double().cpu().numpy()
+
+
+def case(golden, tag):
+    return {k.split("/", 1)[1]: golden[k] for k in golden.files if k.startswith(tag + "/")}
+
+
+def random_case(nx, ny, ns, seed):
+    rng = np.random.default_rng(seed)
+    so = rand_so(rng, nx, ny, ns)
+    q = rng.standard_normal((ny + 2, nx + 2))
+    qf = rng.standard_normal((ny + 2, nx + 2))
+    return so, q, qf, rng
+
+
+def assert_close(got, want, rtol):
+    scale = max(np.abs(want).max(), 1e-300)
+    err = np.abs(got - want).max() / scale
+    assert err <= rtol, f"max rel err {err:.3e} > {rtol:.1e}"
+
+
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("tag", CASES)
+def test_golden_fixtures_bitwise(C, golden2d, tag):
+    g = case(golden2d, tag)
+    so, q, qf, res, qc = (dev(g[k]) for k in ("so", "q", "qf", "res", "qc"))
+    r = torch.zeros_like(q)
+    C.residual(so, qf, q, r)
+    assert np.array_equal(host(r), g["residual"])
+    sor = torch.zeros((2,) + tuple(q.shape), dtype=torch.float64, device="cuda")
+    C.setup_recip(so, sor)
+    assert np.array_equal(host(sor)[1, 1:-1, 1:-1], g["recip"][1, 1:-1, 1:-1])
+    for ud, key in ((0, "gs_down"), (1, "gs_up")):
+        qq = q.clone()
+        C.relax_gs(so, qf, qq, sor, ud)
+        assert np.array_equal(host(qq), g[key]), key
+    ci = torch.zeros(tuple(g["ci"].shape), dtype=torch.float64, device="cuda")
+    C.setup_interp(so, ci)
+    assert np.array_equal(host(ci), g["ci"])
+    soc = torch.zeros(tuple(g["soc"].shape), dtype=torch.float64, device="cuda")
+    C.setup_galerkin(so, ci, soc)
+    assert_close(host(soc), g["soc"], 1e-13)
+    qcr = torch.zeros_like(qc)
+    C.restrict(ci, q, qcr)
+    assert np.array_equal(host(qcr), g["restrict"])
+    qi = q.clone()
+    C.interp_add(qi, qc, res, so, ci)
+    assert np.array_equal(host(qi), g["interp_q"])
+
+
+@pytest.mark.parametrize("nx,ny", SIZES)
+@pytest.mark.parametrize("ns", [3, 5])
+def test_pointwise_kernels_bitwise(C, nx, ny, ns):
+    so, q, qf, rng = random_case(nx, ny, ns, 10 * nx + ny + ns)
+    dso, dq, dqf = dev(so), dev(q), dev(qf)
+    r = torch.zeros_like(dq)
+    C.residual(dso, dqf, dq, r)
+    assert np.array_equal(host(r), O.residual(so, qf, q))
+    nrm = C.residual_norm(dso, dqf, dq, r)
+    np.testing.assert_allclose(nrm, O.norm(O.residual(so, qf, q)), rtol=1e-13)
+    sor = O.setup_recip(so)
+    dsor = torch.zeros_like(dev(sor))
+    C.setup_recip(dso, dsor)
+    assert np.array_equal(host(dsor), sor)
+    for ud in (0, 1):
+        qq = dq.clone()
+        C.relax_gs(dso, dqf, qq, dsor, ud)
+        assert np.array_equal(host(qq), O.relax_gs(so, qf, q, sor, ud))
+    if min(nx, ny) >= 3:
+        ci = O.setup_interp(so)
+        dci = torch.zeros_like(dev(ci))
+        C.setup_interp(dso, dci)
+        assert np.array_equal(host(dci), ci)
+        soc = O.setup_galerkin(so, ci)
+        dsoc = torch.zeros_like(dev(soc))
+        C.setup_galerkin(dso, dci, dsoc)
+        assert_close(host(dsoc), soc, 1e-13)
+        qc = np.zeros(ci.shape[1:])
+        dqc = torch.zeros_like(dev(qc))
+        C.restrict(dci, dq, dqc)
+        assert np.array_equal(host(dqc), O.restrict(ci, q))
+        qc = rng.standard_normal(ci.shape[1:])
+        res = rng.standard_normal(q.shape)
+        qq = dq.clone()
+        C.interp_add(qq, dev(qc), dev(res), dso, dci)
+        assert np.array_equal(host(qq), O.interp_add(q, qc, res, so, ci)[0])
+
+
+@pytest.mark.parametrize("nx,ny", SIZES + [(1000, 7), (7, 1000)])
+@pytest.mark.parametrize("ns", [3, 5])
+@pytest.mark.parametrize("direction", ["x", "y"])
+def test_line_relaxation(C, nx, ny, ns, direction):
+    so, q, qf, _ = random_case(nx, ny, ns, 7 * nx + ny + ns + (direction == "y"))
+    sor, info = O.setup_lines(so, direction)
+    assert info == 0
+    dso = dev(so)
+    dsor = torch.zeros_like(dev(sor))
+    C.setup_lines(dso, dsor, direction)
+    assert np.array_equal(host(dsor), sor)
+    for ud in (0, 1):
+        qq = dev(q)
+        C.relax_lines(dso, dev(qf), qq, dsor, ud, direction)
+        assert_close(host(qq), O.relax_lines(so, qf, q, sor, ud, direction), 1e-13)
+
+
+def test_line_factorisation_reports_non_spd(C):
+    so = np.zeros((3, 7, 7))
+    so[0, 1:-1, 1:-1] = 1.0
+    so[1, 1:-1, 2:-1] = 2.0  # |off-diagonal| > diagonal: not positive definite
+    dsor = torch.zeros((2, 7, 7), dtype=torch.float64, device="cuda")
+    with pytest.raises(C.CedarHipError) as e:
+        C.setup_lines(dev(so), dsor, "x")
+    assert e.value.status == 3
+
+
+@pytest.mark.parametrize("nx,ny,ns", [(3, 3, 5), (4, 4, 5), (4, 4, 3), (5, 3, 5), (9, 7, 5)])
+def test_coarse_cholesky(C, nx, ny, ns):
+    so, q, qf, _ = random_case(nx, ny, ns, nx * 31 + ny)
+    abd, info = O.setup_cg(so)
+    dabd = C.setup_cg(dev(so))
+    assert np.array_equal(host(dabd), abd)
+    qf[0, :] = qf[-1, :] = 0.0
+    qf[:, 0] = qf[:, -1] = 0.0
+    dq = torch.zeros_like(dev(qf))
+    C.solve_cg(dabd, dev(qf), dq)
+    assert np.array_equal(host(dq), O.solve_cg(abd, qf))
+
+
+def test_coarse_cholesky_reports_non_spd(C):
+    so = np.zeros((5, 6, 6))
+    so[0, 1:-1, 1:-1] = -1.0
+    with pytest.raises(C.CedarHipError) as e:
+        C.setup_cg(dev(so))
+    assert e.value.status == 3
+
+
+@pytest.mark.parametrize("ns", [3, 5])
+def test_fp32_kernels_close_to_fp64_oracle(C, ns):
+    nx, ny = 129, 96
+    so, q, qf, rng = random_case(nx, ny, ns, 99 + ns)
+    f = lambda a: dev(a, torch.float32)
+    r = torch.zeros_like(f(q))
+    C.residual(f(so), f(qf), f(q), r)
+    assert_close(host(r), O.residual(so, qf, q), 2e-6)
+    sor = O.setup_recip(so)
+    qq = f(q)
+    C.relax_gs(f(so), f(qf), qq, f(sor), 0)
+    assert_close(host(qq), O.relax_gs(so, qf, q, sor, 0), 2e-6)
+    for d in ("x", "y"):
+        sl, _ = O.setup_lines(so, d)
+        qq = f(q)
+        C.relax_lines(f(so), f(qf), qq, f(sl), 1, d)
+        assert_close(host(qq), O.relax_lines(so, qf, q, sl, 1, d), 1e-5)
+    ci = O.setup_interp(so)
+    qc = torch.zeros(ci.shape[1:], dtype=torch.float32, device="cuda")
+    C.restrict(f(ci), f(q), qc)
+    assert_close(host(qc), O.restrict(ci, q), 2e-6)
+    qcn = rng.standard_normal(ci.shape[1:])
+    res = rng.standard_normal(q.shape)
+    qq = f(q)
+    C.interp_add(qq, f(qcn), f(res), f(so), f(ci))
+    assert_close(host(qq), O.interp_add(q, qcn, res, so, ci)[0], 2e-6)
+
+
+def test_bad_arguments_are_rejected(C):
+    from cedar_amd import _abi
+    z = torch.zeros((5, 5), dtype=torch.float64, device="cuda")
+    so = torch.zeros((4, 5, 5), dtype=torch.float64, device="cuda")  # 4 planes: invalid
+    st = _abi.load().cdr_hip_residual2d_f64(so.data_ptr(), 4, z.data_ptr(), z.data_ptr(),
+                                            z.data_ptr(), 5, 5, None)
+    assert st == 1
+    st = _abi.load().cdr_hip_relax_gs2d_f64(so.data_ptr(), 5, z.data_ptr(), z.data_ptr(),
+                                            z.data_ptr(), 5, 5, 7, None)
+    assert st == 1
+    st = _abi.load().cdr_hip_restrict2d_f64(z.data_ptr(), z.data_ptr(), z.data_ptr(), 5, 5, 9, 9,
+                                            None)
+    assert st == 1

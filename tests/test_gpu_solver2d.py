@@ -1,0 +1,138 @@
+"""Whole-solve parity of the GPU solver (C ABI) with the FP64 oracle and Cedar's known answer.
+
+Bar (north star): FP64 GPU residual history within 1e-12 of the oracle's per iteration
+(|rho_gpu - rho_cpu|, rho normalised by ||r_0||), the same iteration count, and the solution
+within 1e-10 relative l2.  The mixed-precision solve must stop at the same iteration count
+with its FP64 residual below the tolerance.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from cedar_amd import gallery as G
+from oracle import oracle as O
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def C():
+    import cedar_amd
+    return cedar_amd
+
+
+def dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64).cuda()
+
+
+def rel_l2(a, b):
+    return np.linalg.norm((a - b)[1:-1, 1:-1]) / np.linalg.norm(b[1:-1, 1:-1])
+
+
+def compare(C, so, b, **kw):
+    s_cpu = O.Solver(so, **{k: v for k, v in kw.items() if k != "precision"})
+    x_cpu, h_cpu = s_cpu.solve(b)
+    s_gpu = C.Solver(dev(so), **kw)
+    assert s_gpu.nlevels == s_cpu.nlevels
+    x_gpu, h_gpu = s_gpu.solve(dev(b))
+    x_gpu = x_gpu.cpu().numpy()
+    return x_cpu, h_cpu, x_gpu, h_gpu, s_gpu
+
+
+def test_readme_known_answer(C):
+    with open(os.path.join(HERE, "golden", "readme_history.json")) as f:
+        g = json.load(f)
+    so, b, exact = G.readme_problem(400)
+    s = C.Solver(dev(so), relaxation="point", nrelax_pre=1, nrelax_post=1, maxiter=10, tol=1e-10,
+                 num_levels=g["levels"])
+    x, hist = s.solve(dev(b))
+    assert [float("%.6g" % h) for h in hist] == g["rho"]
+    d = (exact - x.cpu().numpy())[1:-1, 1:-1].ravel()
+    assert float("%.6g" % abs(d[np.argmax(np.abs(d))])) == g["solution_norm"]
+
+
+@pytest.mark.parametrize("relax,n,nine", [("point", 511, False), ("point", 200, True),
+                                          ("line-xy", 255, True), ("line-x", 130, True),
+                                          ("line-y", 97, False), ("line-xy", 256, False)])
+def test_fp64_solve_matches_oracle(C, relax, n, nine):
+    if nine:
+        so, b, _ = G.heterogeneous_9pt(n, seed=3, block=8, sigma=1.5)
+    else:
+        so, b = G.poisson_uniform_rhs(n, seed=1)
+    x_cpu, h_cpu, x_gpu, h_gpu, _ = compare(C, so, b, relaxation=relax, nrelax_pre=1,
+                                            nrelax_post=1, maxiter=40, tol=1e-10)
+    assert len(h_gpu) == len(h_cpu)
+    assert np.abs(h_gpu - h_cpu).max() <= 1e-12
+    assert rel_l2(x_gpu, x_cpu) <= 1e-10
+    assert h_gpu[-1] < 1e-10
+
+
+def test_fp64_point_solve_is_bitwise_for_pointwise_kernels(C):
+    """Point relaxation uses only bitwise-exact kernels except the Galerkin setup and the norm
+    reduction; with a 2-level hierarchy on a 5-point operator the coarse operator comes from the
+    5-to-9 product, so allow rounding only there."""
+    so, b = G.poisson_uniform_rhs(63, seed=5)
+    x_cpu, h_cpu, x_gpu, h_gpu, _ = compare(C, so, b, relaxation="point", maxiter=12, tol=1e-14)
+    assert np.abs(h_gpu - h_cpu).max() <= 1e-13
+    assert rel_l2(x_gpu, x_cpu) <= 1e-12
+
+
+@pytest.mark.parametrize("relax,n", [("line-xy", 255), ("point", 255)])
+def test_mixed_precision_same_iterations(C, relax, n):
+    so, b, _ = G.heterogeneous_9pt(n, seed=2, block=16, sigma=2.0)
+    kw = dict(relaxation=relax, nrelax_pre=1, nrelax_post=1, maxiter=60, tol=1e-10)
+    s64 = C.Solver(dev(so), **kw)
+    x64, h64 = s64.solve(dev(b))
+    s32 = C.Solver(dev(so), precision="mixed", **kw)
+    x32, h32 = s32.solve(dev(b))
+    assert h32[-1] < 1e-10
+    assert len(h32) == len(h64)
+    assert rel_l2(x32.cpu().numpy(), x64.cpu().numpy()) < 1e-9
+
+
+@pytest.mark.parametrize("nx,ny", [(3, 3), (4, 4), (5, 17), (17, 5), (2, 9)])
+def test_small_and_ragged_grids(C, nx, ny):
+    so = G.poisson(nx, ny)
+    b = G.rhs_uniform(nx, ny, seed=nx * 100 + ny)
+    x_cpu, h_cpu, x_gpu, h_gpu, s = compare(C, so, b, relaxation="point", maxiter=20, tol=1e-12)
+    assert len(h_gpu) == len(h_cpu)
+    np.testing.assert_allclose(h_gpu, h_cpu, rtol=0, atol=1e-12)
+    assert rel_l2(x_gpu, x_cpu) <= 1e-10
+
+
+def test_setup_hierarchy_matches_oracle(C):
+    so, b, _ = G.heterogeneous_9pt(127, seed=9, block=4)
+    o = O.Solver(so, relaxation="line-xy")
+    s = C.Solver(dev(so), relaxation="line-xy")
+    for lvl in range(o.nlevels):
+        oso, oci, os0, os1 = o.level(lvl)
+        gso, gci, gs0, gs1 = s.level_arrays(lvl)
+        scale = np.abs(oso).max()
+        assert np.abs(gso - oso).max() <= 1e-12 * scale, lvl
+        if lvl > 0:
+            assert np.abs(gci - oci).max() <= 1e-12, lvl
+        if lvl < o.nlevels - 1:
+            assert np.abs(gs0 - os0).max() <= 1e-12 * max(1.0, np.abs(os0).max()), lvl
+            assert np.abs(gs1 - os1).max() <= 1e-12 * max(1.0, np.abs(os1).max()), lvl
+
+
+def test_errors_become_status_codes(C):
+    so = G.poisson(31, 31)
+    with pytest.raises(C.CedarHipError) as e:
+        C.Solver(dev(so), num_levels=9)
+    assert e.value.status == 6
+    bad = so.copy()
+    bad[0, 1:-1, 1:-1] = -1.0
+    with pytest.raises(C.CedarHipError) as e:
+        C.Solver(dev(bad), relaxation="line-x")
+    assert e.value.status == 3
+    s = C.Solver(dev(so))
+    b = G.rhs_uniform(31, 31, 1)
+    b[5, 5] = np.nan
+    with pytest.raises(C.CedarHipError) as e:
+        s.solve(dev(b))
+    assert e.value.status == 4
