@@ -21,7 +21,7 @@ ARCH = os.environ.get("CEDAR_AMD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
-         "-mcode-object-version=5",
+         "-mcode-object-version=5", "-mllvm", "-pragma-unroll-threshold=1000000",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument"]
 

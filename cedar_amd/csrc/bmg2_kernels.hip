@@ -43,21 +43,36 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
+template <int NWAVES = 4>
 __device__ inline void block_sum_store(double v, double *dst) {
-    __shared__ double part[4];
+    __shared__ double part[NWAVES];
     v = wave_sum(v);
     int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
     if (lane == 0) part[wid] = v;
     __syncthreads();
-    if (threadIdx.x == 0) *dst = ((part[0] + part[1]) + part[2]) + part[3];
+    if (threadIdx.x == 0) {
+        double t = part[0];
+        for (int w = 1; w < NWAVES; ++w) t += part[w];
+        *dst = t;
+    }
 }
 
-// one block: sum of n partials in a fixed order -> *out
-__global__ __launch_bounds__(256) void finalize_sum_k(const double *__restrict__ partials,
-                                                      std::size_t n, double *__restrict__ out) {
+// one block of 1024 threads: sum of n partials in a fixed order -> *out.  Thread t adds
+// partials t, t+1024, t+2048, ... in that order; loads are batched so they overlap.
+__global__ __launch_bounds__(1024) void finalize_sum_k(const double *__restrict__ partials,
+                                                       std::size_t n, double *__restrict__ out) {
+    constexpr int T = 1024, U = 8;
     double v = 0.0;
-    for (std::size_t k = threadIdx.x; k < n; k += blockDim.x) v += partials[k];
-    block_sum_store(v, out);
+    std::size_t k = threadIdx.x;
+    for (; k + (U - 1) * T < n; k += U * T) {
+        double t[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) t[u] = partials[k + (std::size_t)u * T];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v += t[u];
+    }
+    for (; k < n; k += T) v += partials[k];
+    block_sum_store<16>(v, out);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -273,6 +288,55 @@ __device__ inline void scan_affine_down(T &a, T &b, int lane) {
     }
 }
 
+// Same inclusive scan with DPP lane moves (row_shr 1/2/4/8 inside 16-lane rows, then
+// row_bcast 15 and 31 across rows) instead of LDS permutes: a few cycles per step, which keeps
+// the carry chain of a line short.  Lanes whose source is out of range keep the identity (1, 0).
+template <int CTRL, int RMASK>
+__device__ inline float dpp_move(float v, float ident) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, ident),
+                                                                 __builtin_bit_cast(int, v), CTRL,
+                                                                 RMASK, 0xF, false));
+}
+
+template <int CTRL, int RMASK>
+__device__ inline double dpp_move(double v, double ident) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned long long o = __builtin_bit_cast(unsigned long long, ident);
+    const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)u, CTRL, RMASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(u >> 32), CTRL,
+                                               RMASK, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+template <int CTRL, int RMASK, typename T>
+__device__ inline void dpp_scan_step(T &a, T &b) {
+    const T ap = dpp_move<CTRL, RMASK>(a, T(1));
+    const T bp = dpp_move<CTRL, RMASK>(b, T(0));
+    b = a * bp + b;
+    a = a * ap;
+}
+
+template <typename T>
+__device__ inline void scan_affine_dpp(T &a, T &b) {
+    dpp_scan_step<0x111, 0xF>(a, b);  // row_shr:1
+    dpp_scan_step<0x112, 0xF>(a, b);  // row_shr:2
+    dpp_scan_step<0x114, 0xF>(a, b);  // row_shr:4
+    dpp_scan_step<0x118, 0xF>(a, b);  // row_shr:8
+    dpp_scan_step<0x142, 0xA>(a, b);  // row_bcast:15 -> rows 1 and 3
+    dpp_scan_step<0x143, 0xC>(a, b);  // row_bcast:31 -> rows 2 and 3
+}
+
+// value of lane 63, broadcast (v_readlane: no LDS round trip)
+__device__ inline float last_lane(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), WAVE - 1));
+}
+__device__ inline double last_lane(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, WAVE - 1);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), WAVE - 1);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 // One wavefront solves one line: L D L^T x = rhs with the stored factors.  Forward pass in
 // 64-element chunks: y_k = rhs_k - E_{k-1} y_{k-1}, stores c_k = y_k / D_k in place of q;
 // backward pass: x_k = c_k - E_k x_{k+1}.  A line of one colour only reads lines of the other
@@ -338,6 +402,251 @@ __global__ __launch_bounds__(256) void relax_lines_k(const T *__restrict__ so, c
         const T x = a * carry + b;
         carry = __shfl(x, 0, WAVE);
         if (act) q[base + (std::size_t)k * stride] = x;
+    }
+}
+
+// Workgroup index remapped so that consecutive logical tiles run on the same XCD (workgroups
+// are dealt round-robin over the 8 XCDs): neighbouring lines share rows of q and of the
+// operator, which then stay in one XCD's L2.  Launch 8 * ceil(ntiles / 8) workgroups.
+__device__ inline int xcd_tile(int ntiles) {
+    const int per = (ntiles + 7) / 8;
+    return (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+}
+
+// Forward-substitution results of a line stay in registers between the two passes (NCH chunks
+// of 64, loops fully unrolled so the array is register-allocated; the library is built with a
+// raised unroll threshold for this).  The raw inputs of chunk c+LA are loaded while chunk c is
+// processed (an LA-deep software pipeline; no arithmetic touches a prefetched value before its
+// chunk is due, so the loads stay in flight).  A chunk's loads never depend on the scan carry.
+template <typename T>
+constexpr int line_lookahead() { return sizeof(T) == 8 ? 2 : 4; }
+
+// raw operands of one point of a line: right-hand side terms, factor entries
+template <typename T>
+struct line_raw {
+    T qf, s0, q0, s1, q1, sw0, q0m, nw0, q0p, nw1, q1m, sw1, q1p, e, d;
+};
+
+// Right-hand side of one point in the reference's order.  X-lines: s0/q0 = coupling to the row
+// below, s1/q1 = to the row above; y-lines: the column to the left / right.
+template <typename T, bool NINE>
+__device__ inline T line_rhs(const line_raw<T> &r) {
+    T rhs = r.qf;
+    rhs = rhs + r.s0 * r.q0;
+    rhs = rhs + r.s1 * r.q1;
+    if (NINE) {
+        rhs = rhs + r.sw0 * r.q0m;
+        rhs = rhs + r.nw0 * r.q0p;
+        rhs = rhs + r.nw1 * r.q1m;
+        rhs = rhs + r.sw1 * r.q1p;
+    }
+    return rhs;
+}
+
+// loads the operands of point p (array index) of an x-line (DIR 0) or y-line (DIR 1)
+template <typename T, bool NINE, int DIR>
+__device__ inline void line_load(line_raw<T> &r, const T *__restrict__ so, const T *__restrict__ qf,
+                                 const T *__restrict__ q, std::size_t P, int ii, std::size_t p) {
+    r.qf = qf[p];
+    if (DIR == 0) {
+        r.s0 = so[P * KS + p];
+        r.q0 = q[p - ii];
+        r.s1 = so[P * KS + p + ii];
+        r.q1 = q[p + ii];
+    } else {
+        r.s0 = so[P * KW + p];
+        r.q0 = q[p - 1];
+        r.s1 = so[P * KW + p + 1];
+        r.q1 = q[p + 1];
+    }
+    if (NINE) {
+        r.sw0 = so[P * KSW + p];
+        r.q0m = q[p - ii - 1];
+        r.nw0 = so[P * KNW + p + 1];
+        r.q0p = q[p - ii + 1];
+        r.nw1 = so[P * KNW + p + ii];
+        r.q1m = q[p + ii - 1];
+        r.sw1 = so[P * KSW + p + ii + 1];
+        r.q1p = q[p + ii + 1];
+    }
+}
+
+template <typename T>
+__device__ inline void line_zero(line_raw<T> &r) {
+    r.qf = r.s0 = r.q0 = r.s1 = r.q1 = r.sw0 = r.q0m = r.nw0 = r.q0p = r.nw1 = r.q1m = r.sw1 =
+        r.q1p = r.e = T(0);
+    r.d = T(1);
+}
+
+// barrier that orders LDS only: prefetched global loads stay in flight across it
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// x-lines: one wavefront per line, four lines per workgroup.
+template <typename T, bool NINE, int NCH>
+__global__ __launch_bounds__(256) void relax_xlines_reg_k(const T *__restrict__ so,
+                                                          const T *__restrict__ qf,
+                                                          T *__restrict__ q,
+                                                          const T *__restrict__ sor, int ii,
+                                                          int jj, int first, int nlines,
+                                                          int ntiles) {
+    constexpr int WPB = 4, LA = line_lookahead<T>();
+    const int tile = xcd_tile(ntiles);
+    if (tile >= ntiles) return;
+    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    const int line = tile * WPB + wave;
+    if (line >= nlines) return;  // whole wavefront; no workgroup barrier in this kernel
+    const std::size_t P = (std::size_t)ii * jj;
+    const int j = first + 2 * line;
+    const int n = ii - 2;
+    const int nact = (n + WAVE - 1) / WAVE;
+    const std::size_t row = (std::size_t)j * ii;
+    const T *D = sor + row + 1;      // D[k]
+    const T *E = sor + P + row + 2;  // E[k], k < n-1
+    // Loads are branch-free: indices past the end of the line are clamped onto its last point
+    // (valid memory) and those lanes are neutralised where their values are used, so no
+    // exec-masked region splits the prefetch from its use.
+    auto fetch = [&](int c, line_raw<T> &r) {
+        const int k = min(c * WAVE + lane, n - 1);
+        line_load<T, NINE, 0>(r, so, qf, q, P, ii, row + k + 1);
+        r.e = E[max(k - 1, 0)];
+        r.d = D[k];
+    };
+    line_raw<T> pf[LA];
+#pragma unroll
+    for (int c = 0; c < LA; ++c)
+        if (c < NCH) fetch(c, pf[c]);
+    T cst[NCH];
+    T carry = T(0);
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        if (c < nact) {
+            T b = line_rhs<T, NINE>(pf[c % LA]);
+            T a = (c * WAVE + lane > 0) ? -pf[c % LA].e : T(0);
+            const T d = pf[c % LA].d;
+            if (c + LA < NCH) fetch(c + LA, pf[c % LA]);
+            scan_affine_dpp(a, b);
+            const T y = a * carry + b;
+            carry = last_lane(y);
+            cst[c] = y / d;
+        }
+    }
+    // backward pass in reversed lane order (lane l holds element c*64 + 63 - l), so that the
+    // suffix recurrence x_k = c_k - E_k x_{k+1} is again a prefix scan over lanes
+    const int rl = WAVE - 1 - lane;
+    auto fetch_e = [&](int c) { return E[min(c * WAVE + rl, max(n - 2, 0))]; };
+    T pe[LA];
+    carry = T(0);
+#pragma unroll
+    for (int c = NCH - 1; c >= 0; --c) {
+        if (c < nact) {
+            const T e = (c + LA >= nact) ? fetch_e(c) : pe[c % LA];
+            if (c - LA >= 0) pe[c % LA] = fetch_e(c - LA);
+            const int k = c * WAVE + rl;
+            T b = __shfl(cst[c], rl, WAVE);
+            T a = (k < n - 1) ? -e : T(0);
+            a = (k < n) ? a : T(1);
+            b = (k < n) ? b : T(0);
+            scan_affine_dpp(a, b);
+            const T x = a * carry + b;
+            carry = last_lane(x);
+            if (k < n) q[row + k + 1] = x;
+        }
+    }
+}
+
+// y-lines: a workgroup of NW wavefronts relaxes NW neighbouring columns of one colour.  The
+// right-hand sides of each 64-row chunk are formed cooperatively with row-contiguous (coalesced)
+// loads and handed to the column owners through LDS; each wavefront then scans its column
+// (lanes along j; the transposed factors are contiguous in j).  Results go back through LDS so
+// that the stores are row-contiguous too.
+template <typename T, bool NINE, int NCH, int NW>
+__global__ __launch_bounds__(NW * 64) void relax_ylines_reg_k(const T *__restrict__ so,
+                                                              const T *__restrict__ qf,
+                                                              T *__restrict__ q,
+                                                              const T *__restrict__ sor, int ii,
+                                                              int jj, int first, int ncols,
+                                                              int ntiles) {
+    constexpr int LA = line_lookahead<T>();
+    __shared__ T tile_s[NW][WAVE + 1];
+    const int tile = xcd_tile(ntiles);
+    if (tile >= ntiles) return;  // uniform over the workgroup
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int col0 = tile * NW;
+    const bool colok = col0 + wave < ncols;
+    const int ime = first + 2 * (col0 + wave);
+    const int tc = threadIdx.x % NW, tr = threadIdx.x / NW;
+    const bool tok = col0 + tc < ncols;
+    const int ti = first + 2 * (col0 + tc);
+    const std::size_t P = (std::size_t)ii * jj;
+    const int n = jj - 2;
+    const int nact = (n + WAVE - 1) / WAVE;
+    const T *D = sor + (std::size_t)ime * jj + 1;
+    const T *E = sor + P + (std::size_t)ime * jj + 2;
+    // this thread's tile entry of chunk c (column ti, row c*64 + tr) plus its wavefront's own
+    // factor entries (column ime, row c*64 + lane)
+    // branch-free loads: row and column indices are clamped onto valid points, unused lanes are
+    // neutralised where the values are consumed (see the x-line kernel)
+    const int tic = first + 2 * min(col0 + tc, ncols - 1);
+    const T *Dc = colok ? D : sor + (std::size_t)(first + 2 * (ncols - 1)) * jj + 1;
+    const T *Ec = colok ? E : sor + P + (std::size_t)(first + 2 * (ncols - 1)) * jj + 2;
+    auto fetch = [&](int c, line_raw<T> &r) {
+        const int k = min(c * WAVE + tr, n - 1);
+        line_load<T, NINE, 1>(r, so, qf, q, P, ii, pidx(tic, k + 1, ii));
+        const int kw = min(c * WAVE + lane, n - 1);
+        r.e = Ec[max(kw - 1, 0)];
+        r.d = Dc[kw];
+    };
+    line_raw<T> pf[LA];
+#pragma unroll
+    for (int c = 0; c < LA; ++c)
+        if (c < NCH) fetch(c, pf[c]);
+    T cst[NCH];
+    T carry = T(0);
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        if (c < nact) {
+            tile_s[tc][tr] = line_rhs<T, NINE>(pf[c % LA]);
+            T a = (c * WAVE + lane > 0) ? -pf[c % LA].e : T(0);
+            const T d = pf[c % LA].d;
+            if (c + LA < NCH) fetch(c + LA, pf[c % LA]);
+            lds_barrier();
+            T b = tile_s[wave][lane];
+            lds_barrier();
+            scan_affine_dpp(a, b);
+            const T y = a * carry + b;
+            carry = last_lane(y);
+            cst[c] = y / d;
+        }
+    }
+    // backward pass in reversed lane order (see the x-line kernel)
+    const int rl = WAVE - 1 - lane;
+    auto fetch_e = [&](int c) { return Ec[min(c * WAVE + rl, max(n - 2, 0))]; };
+    T pe[LA];
+    carry = T(0);
+#pragma unroll
+    for (int c = NCH - 1; c >= 0; --c) {
+        if (c < nact) {
+            const T e = (c + LA >= nact) ? fetch_e(c) : pe[c % LA];
+            if (c - LA >= 0) pe[c % LA] = fetch_e(c - LA);
+            const int k = c * WAVE + rl;
+            T b = __shfl(cst[c], rl, WAVE);
+            const bool act = colok && k < n;
+            T a = (k < n - 1) ? -e : T(0);
+            a = act ? a : T(1);
+            b = act ? b : T(0);
+            scan_affine_dpp(a, b);
+            const T x = a * carry + b;
+            carry = last_lane(x);
+            tile_s[wave][rl] = x;
+            lds_barrier();
+            const int kk = c * WAVE + tr;
+            if (tok && kk < n) q[pidx(ti, kk + 1, ii)] = tile_s[tc][tr];
+            lds_barrier();
+        }
     }
 }
 
@@ -757,14 +1066,14 @@ void residual_norm(const T *so, int nstncl, const T *qf, const T *q, T *res, int
         residual_k<T, true, true><<<g, 256, 0, s>>>(so, qf, q, res, ii, jj, partials);
     else
         residual_k<T, false, true><<<g, 256, 0, s>>>(so, qf, q, res, ii, jj, partials);
-    finalize_sum_k<<<1, 256, 0, s>>>(partials, (std::size_t)g.x * g.y, sumsq_out);
+    finalize_sum_k<<<1, 1024, 0, s>>>(partials, (std::size_t)g.x * g.y, sumsq_out);
 }
 
 template <typename T>
 void sumsq(const T *x, int ii, int jj, double *partials, double *out, hipStream_t s) {
     const dim3 g = grid2(ii - 2, jj - 2, 64, 4);
     sumsq_k<T><<<g, 256, 0, s>>>(x, ii, jj, partials);
-    finalize_sum_k<<<1, 256, 0, s>>>(partials, (std::size_t)g.x * g.y, out);
+    finalize_sum_k<<<1, 1024, 0, s>>>(partials, (std::size_t)g.x * g.y, out);
 }
 
 void residual_ir(const double *so, int nstncl, const double *qf, const double *x, const float *e,
@@ -782,7 +1091,7 @@ void residual_ir(const double *so, int nstncl, const double *qf, const double *x
         else
             residual_ir_k<false, false><<<g, 256, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, partials);
     }
-    finalize_sum_k<<<1, 256, 0, s>>>(partials, (std::size_t)g.x * g.y, sumsq_out);
+    finalize_sum_k<<<1, 1024, 0, s>>>(partials, (std::size_t)g.x * g.y, sumsq_out);
 }
 
 void setup_recip(const double *so, double *sor, int ii, int jj, hipStream_t s) {
@@ -820,6 +1129,40 @@ void setup_lines_y(const double *so, double *sor, int ii, int jj, int *info, hip
     setup_lines_y_k<<<(n + 63) / 64, 64, 0, s>>>(so, sor, ii, jj, info);
 }
 
+template <typename T, bool NINE, int DIR, int NCH>
+static void launch_lines_reg(const T *so, const T *qf, T *q, const T *sor, int ii, int jj,
+                             int first, int nlines, hipStream_t s) {
+    constexpr int NW = 8;  // y-lines: columns per workgroup
+    const int per_tile = (DIR == 0) ? 4 : NW;
+    const int ntiles = (nlines + per_tile - 1) / per_tile;
+    const unsigned blocks = (unsigned)(8 * ((ntiles + 7) / 8));
+    if (DIR == 0)
+        relax_xlines_reg_k<T, NINE, NCH><<<blocks, 256, 0, s>>>(so, qf, q, sor, ii, jj, first,
+                                                               nlines, ntiles);
+    else
+        relax_ylines_reg_k<T, NINE, NCH, NW><<<blocks, NW * 64, 0, s>>>(so, qf, q, sor, ii, jj,
+                                                                       first, nlines, ntiles);
+}
+
+template <typename T, bool NINE, int DIR>
+static void launch_lines(const T *so, const T *qf, T *q, const T *sor, int ii, int jj, int first,
+                         int nlines, hipStream_t s) {
+    const int n = (DIR == 0) ? ii - 2 : jj - 2;
+    const int nch = (n + WAVE - 1) / WAVE;
+#define CDR_L(NC) launch_lines_reg<T, NINE, DIR, NC>(so, qf, q, sor, ii, jj, first, nlines, s)
+    if (nch <= 1) CDR_L(1);
+    else if (nch <= 2) CDR_L(2);
+    else if (nch <= 4) CDR_L(4);
+    else if (nch <= 8) CDR_L(8);
+    else if (nch <= 16) CDR_L(16);
+    else if (nch <= 32) CDR_L(32);
+    else if (nch <= 64) CDR_L(64);
+    else  // lines longer than 4096: stage the forward results in q itself
+        relax_lines_k<T, NINE, DIR><<<(unsigned)((nlines + 3) / 4), 256, 0, s>>>(so, qf, q, sor, ii,
+                                                                                jj, first, nlines);
+#undef CDR_L
+}
+
 template <typename T, int DIR>
 static void relax_lines(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
                         int updown, hipStream_t s) {
@@ -830,11 +1173,10 @@ static void relax_lines(const T *so, int nstncl, const T *qf, T *q, const T *sor
         const int first = (updown == 0) ? (t == 0 ? 2 : 1) : (t == 0 ? 1 : 2);
         if (first > nfix) continue;
         const int nlines = (nfix - first) / 2 + 1;
-        const unsigned blocks = (unsigned)((nlines + 3) / 4);
         if (nstncl == 5)
-            relax_lines_k<T, true, DIR><<<blocks, 256, 0, s>>>(so, qf, q, sor, ii, jj, first, nlines);
+            launch_lines<T, true, DIR>(so, qf, q, sor, ii, jj, first, nlines, s);
         else
-            relax_lines_k<T, false, DIR><<<blocks, 256, 0, s>>>(so, qf, q, sor, ii, jj, first, nlines);
+            launch_lines<T, false, DIR>(so, qf, q, sor, ii, jj, first, nlines, s);
     }
 }
 
