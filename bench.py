@@ -52,6 +52,53 @@ def time_kernel(fn, reps, torch):
     return e0.elapsed_time(e1) / 1e3 / reps
 
 
+def measured_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed counter summary
+    (two rocprofv3 --pmc runs, calibrated against a copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "relax_ylines_f32_4095_counters.md")
+    try:
+        with open(path) as f:
+            for ln in f:
+                if ln.startswith("traffic_bytes_per_launch:"):
+                    return int(ln.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def kernel_only(args, so, b):
+    """Dominant kernel alone (5 sweeps = 10 launches) plus a copy of known size whose counters
+    calibrate FETCH_SIZE / WRITE_SIZE."""
+    import torch
+
+    import cedar_amd
+    from cedar_amd import _abi
+    s = cedar_amd.Solver(so, precision="mixed", relaxation="line-xy", nrelax_pre=1,
+                         nrelax_post=1)
+    ii, jj = so.shape[2], so.shape[1]
+    _, _, _, _, _, sor1 = s.level(0)
+    sor_y = torch.empty((2, jj, ii), dtype=torch.float64, device="cuda")
+    cedar_amd.cdr2._hip().hipMemcpy(ctypes.c_void_p(sor_y.data_ptr()), ctypes.c_void_p(sor1),
+                                    2 * ii * jj * 8, 3)
+    A = so.to(torch.float32).contiguous()
+    sy = sor_y.to(torch.float32).contiguous()
+    f = b.to(torch.float32).contiguous()
+    q = torch.zeros_like(f)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    fy = _abi.load().cdr_hip_relax_lines_y2d_f32
+    torch.cuda.synchronize()
+    for _ in range(5):
+        fy(A.data_ptr(), so.shape[0], f.data_ptr(), q.data_ptr(), sy.data_ptr(), None, ii, jj, 0,
+           stream)
+    src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")  # 256 MiB
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"kernel_only": True, "copy_bytes": src.numel() * 4,
+                      "launches": 10, "points": (ii - 2) * (jj - 2)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,6 +108,9 @@ def main():
     ap.add_argument("--cpu-cycles", type=int, default=2,
                     help="V-cycles of the CPU oracle to time (0 disables the CPU baseline)")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="run only the dominant kernel (fine-level FP32 y-line sweep) and a plain "
+                         "256 MiB device copy, for hardware-counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
@@ -80,6 +130,10 @@ def main():
     so = torch.as_tensor(so_np).cuda()
     b = torch.as_tensor(b_np).cuda()
     kw = dict(relaxation="line-xy", nrelax_pre=1, nrelax_post=1, maxiter=100, tol=1e-10)
+
+    if args.kernel_only:
+        kernel_only(args, so, b)
+        return
 
     solvers = {}
     for prec in ("f64", "mixed"):
@@ -204,7 +258,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic() if n == 4095 else None,
             "algorithmic_bytes_per_launch": km["bytes_sweep"] / 2,
             "avg_launch_s": dom[1] / 2,
         },

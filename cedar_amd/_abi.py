@@ -24,7 +24,7 @@ _sz = ctypes.c_size_t
 class Settings(ctypes.Structure):
     """cdr_solver_settings"""
     _fields_ = [("relaxation", _i), ("nrelax_pre", _i), ("nrelax_post", _i), ("num_levels", _i),
-                ("maxiter", _i), ("tol", _d), ("min_coarse", _i), ("precision", _i)]
+                ("maxiter", _i), ("tol", _d), ("min_coarse", _i), ("precision", _i), ("graph", _i)]
 
 
 # name -> (restype, argtypes); pointers to device memory are passed as c_void_p

@@ -199,18 +199,19 @@ class Solver:
     Settings use Cedar's config keys and defaults (src/multilevel_settings.cc:15-61):
     relaxation ('point' | 'line-x' | 'line-y' | 'line-xy'), nrelax_pre=2, nrelax_post=1,
     num_levels=-1, maxiter=10, tol=1e-8, min_coarse=3; precision 'f64' (Cedar's algorithm) or
-    'mixed' (FP32 V-cycle preconditioning FP64 iterative refinement).
+    'mixed' (FP32 V-cycle preconditioning FP64 iterative refinement); graph=True replays each
+    V-cycle from a captured hipGraph.
     """
 
     def __init__(self, so, relaxation="point", nrelax_pre=2, nrelax_post=1, num_levels=-1,
-                 maxiter=10, tol=1e-8, min_coarse=3, precision="f64"):
+                 maxiter=10, tol=1e-8, min_coarse=3, precision="f64", graph=True):
         so = _t(so) if not isinstance(so, np.ndarray) else torch.as_tensor(so).cuda()
         if so.dtype != torch.float64:
             so = so.to(torch.float64)
         self.so = so.contiguous()  # held by reference for the solver's lifetime
         self.maxiter = maxiter
         s = _abi.Settings(_abi.RELAX[relaxation], nrelax_pre, nrelax_post, num_levels, maxiter,
-                          tol, min_coarse, _abi.PRECISION[precision])
+                          tol, min_coarse, _abi.PRECISION[precision], int(bool(graph)))
         h = ctypes.c_void_p()
         _abi.call("cdr_hip_solver2d_create", _p(self.so), self.so.shape[0], self.so.shape[2] - 2,
                   self.so.shape[1] - 2, ctypes.byref(s), _stream(), ctypes.byref(h))

@@ -415,9 +415,12 @@ __device__ inline int xcd_tile(int ntiles) {
 
 // Forward-substitution results of a line stay in registers between the two passes (NCH chunks
 // of 64, loops fully unrolled so the array is register-allocated; the library is built with a
-// raised unroll threshold for this).  The raw inputs of chunk c+LA are loaded while chunk c is
-// processed (an LA-deep software pipeline; no arithmetic touches a prefetched value before its
-// chunk is due, so the loads stay in flight).  A chunk's loads never depend on the scan carry.
+// raised unroll threshold for this).  Chunks are handled in groups of G: the G scans of a group
+// are independent (only the final carry application is sequential), which gives the scheduler
+// instruction-level parallelism and, for y-lines, one LDS exchange per group.  The raw inputs of
+// the group LA chunks ahead are loaded while the current group is processed.
+template <typename T>
+constexpr int line_group() { return sizeof(T) == 8 ? 2 : 4; }
 template <typename T>
 constexpr int line_lookahead() { return sizeof(T) == 8 ? 2 : 4; }
 
@@ -515,6 +518,7 @@ __global__ __launch_bounds__(256) void relax_xlines_reg_k(const T *__restrict__ 
         r.e = E[max(k - 1, 0)];
         r.d = D[k];
     };
+    constexpr int G = NCH < line_group<T>() ? NCH : line_group<T>();
     line_raw<T> pf[LA];
 #pragma unroll
     for (int c = 0; c < LA; ++c)
@@ -522,38 +526,61 @@ __global__ __launch_bounds__(256) void relax_xlines_reg_k(const T *__restrict__ 
     T cst[NCH];
     T carry = T(0);
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) {
-        if (c < nact) {
-            T b = line_rhs<T, NINE>(pf[c % LA]);
-            T a = (c * WAVE + lane > 0) ? -pf[c % LA].e : T(0);
-            const T d = pf[c % LA].d;
-            if (c + LA < NCH) fetch(c + LA, pf[c % LA]);
-            scan_affine_dpp(a, b);
-            const T y = a * carry + b;
-            carry = last_lane(y);
-            cst[c] = y / d;
+    for (int g = 0; g < NCH; g += G) {
+        if (g < nact) {
+            T a[G], b[G], d[G];
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                const int c = g + u;
+                b[u] = line_rhs<T, NINE>(pf[c % LA]);
+                a[u] = (c * WAVE + lane > 0) ? -pf[c % LA].e : T(0);
+                d[u] = pf[c % LA].d;
+                if (c + LA < NCH) fetch(c + LA, pf[c % LA]);
+            }
+#pragma unroll
+            for (int u = 0; u < G; ++u) scan_affine_dpp(a[u], b[u]);
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                const T y = a[u] * carry + b[u];
+                carry = last_lane(y);
+                cst[g + u] = y / d[u];
+            }
         }
     }
     // backward pass in reversed lane order (lane l holds element c*64 + 63 - l), so that the
     // suffix recurrence x_k = c_k - E_k x_{k+1} is again a prefix scan over lanes
     const int rl = WAVE - 1 - lane;
     auto fetch_e = [&](int c) { return E[min(c * WAVE + rl, max(n - 2, 0))]; };
+    // chunks at or above nactg belong to groups that are skipped, so a chunk whose prefetch
+    // would have come from such a group loads its entry directly
+    const int nactg = min(NCH, ((nact + G - 1) / G) * G);
     T pe[LA];
     carry = T(0);
 #pragma unroll
-    for (int c = NCH - 1; c >= 0; --c) {
-        if (c < nact) {
-            const T e = (c + LA >= nact) ? fetch_e(c) : pe[c % LA];
-            if (c - LA >= 0) pe[c % LA] = fetch_e(c - LA);
-            const int k = c * WAVE + rl;
-            T b = __shfl(cst[c], rl, WAVE);
-            T a = (k < n - 1) ? -e : T(0);
-            a = (k < n) ? a : T(1);
-            b = (k < n) ? b : T(0);
-            scan_affine_dpp(a, b);
-            const T x = a * carry + b;
-            carry = last_lane(x);
-            if (k < n) q[row + k + 1] = x;
+    for (int g = NCH - G; g >= 0; g -= G) {
+        if (g < nact) {
+            T a[G], b[G];
+#pragma unroll
+            for (int u = G - 1; u >= 0; --u) {
+                const int c = g + u;
+                const T e = (c + LA >= nactg) ? fetch_e(c) : pe[c % LA];
+                if (c - LA >= 0) pe[c % LA] = fetch_e(c - LA);
+                const int k = c * WAVE + rl;
+                b[u] = __shfl(cst[c], rl, WAVE);
+                a[u] = (k < n - 1) ? -e : T(0);
+                a[u] = (k < n) ? a[u] : T(1);
+                b[u] = (k < n) ? b[u] : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < G; ++u) scan_affine_dpp(a[u], b[u]);
+#pragma unroll
+            for (int u = G - 1; u >= 0; --u) {
+                const int c = g + u;
+                const T x = a[u] * carry + b[u];
+                carry = last_lane(x);
+                const int k = c * WAVE + rl;
+                if (k < n) q[row + k + 1] = x;
+            }
         }
     }
 }
@@ -571,7 +598,8 @@ __global__ __launch_bounds__(NW * 64) void relax_ylines_reg_k(const T *__restric
                                                               int jj, int first, int ncols,
                                                               int ntiles) {
     constexpr int LA = line_lookahead<T>();
-    __shared__ T tile_s[NW][WAVE + 1];
+    constexpr int G = NCH < line_group<T>() ? NCH : line_group<T>();
+    __shared__ T tile_s[G][NW][WAVE + 1];
     const int tile = xcd_tile(ntiles);
     if (tile >= ntiles) return;  // uniform over the workgroup
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
@@ -607,44 +635,69 @@ __global__ __launch_bounds__(NW * 64) void relax_ylines_reg_k(const T *__restric
     T cst[NCH];
     T carry = T(0);
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) {
-        if (c < nact) {
-            tile_s[tc][tr] = line_rhs<T, NINE>(pf[c % LA]);
-            T a = (c * WAVE + lane > 0) ? -pf[c % LA].e : T(0);
-            const T d = pf[c % LA].d;
-            if (c + LA < NCH) fetch(c + LA, pf[c % LA]);
+    for (int g = 0; g < NCH; g += G) {
+        if (g < nact) {
+            T a[G], b[G], d[G];
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                const int c = g + u;
+                tile_s[u][tc][tr] = line_rhs<T, NINE>(pf[c % LA]);
+                a[u] = (c * WAVE + lane > 0) ? -pf[c % LA].e : T(0);
+                d[u] = pf[c % LA].d;
+                if (c + LA < NCH) fetch(c + LA, pf[c % LA]);
+            }
             lds_barrier();
-            T b = tile_s[wave][lane];
+#pragma unroll
+            for (int u = 0; u < G; ++u) b[u] = tile_s[u][wave][lane];
             lds_barrier();
-            scan_affine_dpp(a, b);
-            const T y = a * carry + b;
-            carry = last_lane(y);
-            cst[c] = y / d;
+#pragma unroll
+            for (int u = 0; u < G; ++u) scan_affine_dpp(a[u], b[u]);
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                const T y = a[u] * carry + b[u];
+                carry = last_lane(y);
+                cst[g + u] = y / d[u];
+            }
         }
     }
     // backward pass in reversed lane order (see the x-line kernel)
     const int rl = WAVE - 1 - lane;
     auto fetch_e = [&](int c) { return Ec[min(c * WAVE + rl, max(n - 2, 0))]; };
+    // chunks at or above nactg belong to groups that are skipped, so a chunk whose prefetch
+    // would have come from such a group loads its entry directly
+    const int nactg = min(NCH, ((nact + G - 1) / G) * G);
     T pe[LA];
     carry = T(0);
 #pragma unroll
-    for (int c = NCH - 1; c >= 0; --c) {
-        if (c < nact) {
-            const T e = (c + LA >= nact) ? fetch_e(c) : pe[c % LA];
-            if (c - LA >= 0) pe[c % LA] = fetch_e(c - LA);
-            const int k = c * WAVE + rl;
-            T b = __shfl(cst[c], rl, WAVE);
-            const bool act = colok && k < n;
-            T a = (k < n - 1) ? -e : T(0);
-            a = act ? a : T(1);
-            b = act ? b : T(0);
-            scan_affine_dpp(a, b);
-            const T x = a * carry + b;
-            carry = last_lane(x);
-            tile_s[wave][rl] = x;
+    for (int g = NCH - G; g >= 0; g -= G) {
+        if (g < nact) {
+            T a[G], b[G];
+#pragma unroll
+            for (int u = G - 1; u >= 0; --u) {
+                const int c = g + u;
+                const T e = (c + LA >= nactg) ? fetch_e(c) : pe[c % LA];
+                if (c - LA >= 0) pe[c % LA] = fetch_e(c - LA);
+                const int k = c * WAVE + rl;
+                const bool act = colok && k < n;
+                b[u] = __shfl(cst[c], rl, WAVE);
+                a[u] = (k < n - 1) ? -e : T(0);
+                a[u] = act ? a[u] : T(1);
+                b[u] = act ? b[u] : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < G; ++u) scan_affine_dpp(a[u], b[u]);
+#pragma unroll
+            for (int u = G - 1; u >= 0; --u) {
+                const T x = a[u] * carry + b[u];
+                carry = last_lane(x);
+                tile_s[u][wave][rl] = x;
+            }
             lds_barrier();
-            const int kk = c * WAVE + tr;
-            if (tok && kk < n) q[pidx(ti, kk + 1, ii)] = tile_s[tc][tr];
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                const int kk = (g + u) * WAVE + tr;
+                if (tok && kk < n) q[pidx(ti, kk + 1, ii)] = tile_s[u][tc][tr];
+            }
             lds_barrier();
         }
     }

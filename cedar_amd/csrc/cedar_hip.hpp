@@ -235,6 +235,13 @@ public:
 
     hipStream_t stream() const { return stream_; }
 
+    // points every registered implementation at stream s (used while capturing a graph)
+    void set_stream(hipStream_t s) {
+        stream_ = s;
+        for (auto &t : table_)
+            for (auto &kv : t.second.impls) kv.second->stream = s;
+    }
+
 private:
     struct slot_t {
         std::map<std::string, std::shared_ptr<kernel_base>> impls;

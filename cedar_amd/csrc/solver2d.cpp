@@ -178,7 +178,7 @@ class solver2d {
 public:
     solver2d(const double *so_dev, int nstncl, int nx, int ny, const cdr_solver_settings &s,
              hipStream_t stream)
-        : set_(s), stream_(stream), km_(stream) {
+        : set_(s), stream_(stream), cur_(stream), km_(stream) {
         check_hip(hipEventCreate(&ev0_), "hipEventCreate");
         check_hip(hipEventCreate(&ev1_), "hipEventCreate");
         info_ = dbuf<int>(1, stream_);
@@ -187,6 +187,9 @@ public:
     }
 
     ~solver2d() {
+        for (auto *g : {&g64_, &g32_})
+            if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
         (void)hipEventDestroy(ev0_);
         (void)hipEventDestroy(ev1_);
         if (host_norm_) (void)hipHostFree(host_norm_);
@@ -226,7 +229,7 @@ public:
                                               work_.data(), sumsq_.data(), stream_), "residual_norm");
             const double res0 = std::sqrt(read_norm2());
             for (int i = 0; i < set_.maxiter; ++i) {
-                run_cycle<double>(L64_, x, b);
+                run_maybe_graph(g64_, x.p, b.p, [&] { run_cycle<double>(L64_, x, b); });
                 check(cdr_hip_residual_norm2d_f64(A.p, A.nstncl, b.p, x.p, L64_[0].res.p, e.ii,
                                                   e.jj, work_.data(), sumsq_.data(), stream_), "residual_norm");
                 const double rel = std::sqrt(read_norm2()) / res0;
@@ -245,8 +248,10 @@ public:
             check_hip(hipGetLastError(), "residual_ir");
             const double res0 = std::sqrt(read_norm2());
             for (int i = 0; i < set_.maxiter; ++i) {
-                k2::fill<float>(F.x.p, 0.0f, e.points(), stream_);
-                run_cycle<float>(L32_, F.x, F.b);
+                run_maybe_graph(g32_, F.x.p, F.b.p, [&] {
+                    k2::fill<float>(F.x.p, 0.0f, e.points(), cur_);
+                    run_cycle<float>(L32_, F.x, F.b);
+                });
                 k2::residual_ir(A.p, A.nstncl, b_dev, xa, F.x.p, xb, F.b.p, e.ii, e.jj,
                                 work_.data(), sumsq_.data(), stream_);
                 check_hip(hipGetLastError(), "residual_ir");
@@ -272,7 +277,13 @@ public:
 private:
     cdr_solver_settings set_;
     hipStream_t stream_;
+    hipStream_t cur_;                  // stream the cycle kernels go to (capture stream while capturing)
+    hipStream_t cap_stream_ = nullptr;
     kernel_manager km_;
+    struct graph_t {
+        hipGraphExec_t exec = nullptr;
+        const void *k1 = nullptr, *k2 = nullptr;  // the x / b buffers the graph was captured on
+    } g64_, g32_;
     std::vector<level_data<double>> L64_;
     std::vector<level_data<float>> L32_;
     dbuf<double> abd_, bbd_, work_, sumsq_, xa_, xb_;
@@ -281,6 +292,51 @@ private:
     double *host_norm_ = nullptr;
     hipEvent_t ev0_{}, ev1_{};
     double setup_s_ = 0, solve_s_ = 0;
+
+    // Enqueues body() on stream_, either directly or by replaying a graph captured from it.  The
+    // graph is captured on a private stream (capture is not allowed on the null stream) and
+    // re-captured when the buffers it was captured on change.  Captured cycles contain kernel
+    // nodes only (fills are kernels, no memset nodes).
+    template <typename F>
+    void run_maybe_graph(graph_t &g, const void *k1, const void *k2, F body) {
+        if (!set_.graph) {
+            body();
+            return;
+        }
+        if (!g.exec || g.k1 != k1 || g.k2 != k2) {
+            if (g.exec) {
+                check_hip(hipGraphExecDestroy(g.exec), "hipGraphExecDestroy");
+                g.exec = nullptr;
+            }
+            if (!cap_stream_)
+                check_hip(hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking),
+                          "hipStreamCreateWithFlags");
+            check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+            km_.set_stream(cap_stream_);
+            cur_ = cap_stream_;
+            check_hip(hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeThreadLocal),
+                      "hipStreamBeginCapture");
+            hipGraph_t graph = nullptr;
+            try {
+                body();
+            } catch (...) {
+                (void)hipStreamEndCapture(cap_stream_, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                km_.set_stream(stream_);
+                cur_ = stream_;
+                throw;
+            }
+            check_hip(hipStreamEndCapture(cap_stream_, &graph), "hipStreamEndCapture");
+            km_.set_stream(stream_);
+            cur_ = stream_;
+            const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            check_hip(e, "hipGraphInstantiate");
+            g.k1 = k1;
+            g.k2 = k2;
+        }
+        check_hip(hipGraphLaunch(g.exec, stream_), "hipGraphLaunch");
+    }
 
     double read_norm2() {
         check_hip(hipMemcpyAsync(host_norm_, sumsq_.data(), sizeof(double), hipMemcpyDeviceToHost,
@@ -438,7 +494,7 @@ private:
         smooth<T>(F, x, b, Dir::DOWN);
         km_.run<kernels::residual<T>>(F.A, x, b, F.res);
         km_.run<kernels::restriction<T>>(C.P, F.res, C.b);
-        k2::fill<T>(C.x.p, T(0), C.ext.points(), stream_);
+        k2::fill<T>(C.x.p, T(0), C.ext.points(), cur_);
         if (lvl + 1 == (int)L.size() - 1)
             km_.run<kernels::solve_cg<T>>(C.x, C.b, abd_grid(), bbd_.data());
         else
@@ -485,6 +541,7 @@ void cdr_hip_default_settings(cdr_solver_settings *s) {
     s->tol = 1e-8;
     s->min_coarse = 3;
     s->precision = CDR_PREC_F64;
+    s->graph = 1;
 }
 
 int cdr_hip_solver2d_create(const double *so_dev, int nstncl, int nx, int ny,
@@ -492,7 +549,8 @@ int cdr_hip_solver2d_create(const double *so_dev, int nstncl, int nx, int ny,
     if (!so_dev || !s || !out || nx < 1 || ny < 1 || (nstncl != 3 && nstncl != 5) ||
         s->relaxation < CDR_RELAX_POINT || s->relaxation > CDR_RELAX_LINE_XY ||
         s->nrelax_pre < 0 || s->nrelax_post < 0 || s->maxiter < 0 || s->min_coarse < 2 ||
-        (s->precision != CDR_PREC_F64 && s->precision != CDR_PREC_MIXED))
+        (s->precision != CDR_PREC_F64 && s->precision != CDR_PREC_MIXED) ||
+        (s->graph != 0 && s->graph != 1))
         return CDR_ERR_ARG;
     *out = nullptr;
     int ndev = 0;

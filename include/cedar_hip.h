@@ -163,6 +163,7 @@ typedef struct {
     double tol;       /* solver.tol (default 1e-8) */
     int min_coarse;   /* solver.min_coarse (default 3) */
     int precision;    /* CDR_PREC_* (default CDR_PREC_F64) */
+    int graph;        /* 1: each V-cycle is replayed from a captured hipGraph (default), 0: eager */
 } cdr_solver_settings;
 
 typedef struct cdr_hip_solver2d cdr_hip_solver2d;

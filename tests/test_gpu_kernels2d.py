@@ -119,7 +119,10 @@ def test_pointwise_kernels_bitwise(C, nx, ny, ns):
         assert np.array_equal(host(qq), O.interp_add(q, qc, res, so, ci)[0])
 
 
-@pytest.mark.parametrize("nx,ny", SIZES + [(1000, 7), (7, 1000)])
+LINE_SIZES = SIZES + [(1000, 7), (7, 1000), (700, 33), (33, 700), (300, 5000), (4095, 9)]
+
+
+@pytest.mark.parametrize("nx,ny", LINE_SIZES)
 @pytest.mark.parametrize("ns", [3, 5])
 @pytest.mark.parametrize("direction", ["x", "y"])
 def test_line_relaxation(C, nx, ny, ns, direction):
@@ -134,6 +137,18 @@ def test_line_relaxation(C, nx, ny, ns, direction):
         qq = dev(q)
         C.relax_lines(dso, dev(qf), qq, dsor, ud, direction)
         assert_close(host(qq), O.relax_lines(so, qf, q, sor, ud, direction), 1e-13)
+
+
+@pytest.mark.parametrize("nx,ny", [(700, 33), (33, 700), (4095, 5), (5, 4095), (130, 129)])
+@pytest.mark.parametrize("direction", ["x", "y"])
+def test_line_relaxation_fp32(C, nx, ny, direction):
+    so, q, qf, _ = random_case(nx, ny, 5, 3 * nx + ny + (direction == "y"))
+    sor, info = O.setup_lines(so, direction)
+    f = lambda a: dev(a, torch.float32)
+    for ud in (0, 1):
+        qq = f(q)
+        C.relax_lines(f(so), f(qf), qq, f(sor), ud, direction)
+        assert_close(host(qq), O.relax_lines(so, qf, q, sor, ud, direction), 2e-5)
 
 
 def test_line_factorisation_reports_non_spd(C):
