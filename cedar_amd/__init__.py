@@ -2,12 +2,12 @@
 
 The numerical work runs in hand-written HIP kernels behind the C ABI in include/cedar_hip.h
 (libcedar_hip.so); Python only allocates device memory through torch and passes pointers.
-Importing the package loads the library and fails if it is missing -- there is no CPU path.
+The library is loaded on first use (or by `cedar_amd.load()`), which raises if it is missing --
+there is no CPU path.  Importing the package alone does not load it, so that
+`python -m cedar_amd.build` can rebuild a stale library.
 """
 from . import _abi
-from ._abi import CedarHipError, RELAX, PRECISION  # noqa: F401
-
-_abi.load()
+from ._abi import CedarHipError, RELAX, PRECISION, load  # noqa: F401
 
 __version__ = "0.1.0"
 

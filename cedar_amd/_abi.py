@@ -58,6 +58,9 @@ PROTOTYPES = {
     "cdr_hip_default_settings": (None, [ctypes.POINTER(Settings)]),
     "cdr_hip_solver2d_create": (_i, [_P, _i, _i, _i, ctypes.POINTER(Settings), _vp,
                                      ctypes.POINTER(_vp)]),
+    "cdr_hip_solver2d_create_batched": (_i, [_P, _i, _i, _i, _i, ctypes.POINTER(Settings), _vp,
+                                             ctypes.POINTER(_vp)]),
+    "cdr_hip_solver2d_nbatch": (_i, [_vp]),
     "cdr_hip_solver2d_solve": (_i, [_vp, _P, _P, ctypes.POINTER(_d), _i, ctypes.POINTER(_i)]),
     "cdr_hip_solver2d_vcycle": (_i, [_vp, _P, _P]),
     "cdr_hip_solver2d_nlevels": (_i, [_vp]),

@@ -201,6 +201,11 @@ class Solver:
     num_levels=-1, maxiter=10, tol=1e-8, min_coarse=3; precision 'f64' (Cedar's algorithm) or
     'mixed' (FP32 V-cycle preconditioning FP64 iterative refinement); graph=True replays each
     V-cycle from a captured hipGraph.
+
+    A 4-D operator (nb, nstncl, ny+2, nx+2) makes a batched solver: nb independent problems of
+    the same extents solved together (every kernel launch covers the batch); solve() then takes
+    and returns (nb, ny+2, nx+2) grids and a list of nb residual histories, each the history of
+    solving that problem alone.
     """
 
     def __init__(self, so, relaxation="point", nrelax_pre=2, nrelax_post=1, num_levels=-1,
@@ -210,11 +215,15 @@ class Solver:
             so = so.to(torch.float64)
         self.so = so.contiguous()  # held by reference for the solver's lifetime
         self.maxiter = maxiter
+        self.batched = self.so.dim() == 4
+        nb = self.so.shape[0] if self.batched else 1
+        self.nb = nb
+        ns, jj, ii = self.so.shape[-3:]
         s = _abi.Settings(_abi.RELAX[relaxation], nrelax_pre, nrelax_post, num_levels, maxiter,
                           tol, min_coarse, _abi.PRECISION[precision], int(bool(graph)))
         h = ctypes.c_void_p()
-        _abi.call("cdr_hip_solver2d_create", _p(self.so), self.so.shape[0], self.so.shape[2] - 2,
-                  self.so.shape[1] - 2, ctypes.byref(s), _stream(), ctypes.byref(h))
+        _abi.call("cdr_hip_solver2d_create_batched", _p(self.so), ns, ii - 2, jj - 2, nb,
+                  ctypes.byref(s), _stream(), ctypes.byref(h))
         self._h = h
 
     @property
@@ -233,12 +242,13 @@ class Solver:
         """Host copies (so, ci, sor0, sor1) of a level's FP64 data."""
         nx, ny, so, ci, s0, s1 = self.level(lvl)
         ii, jj = nx + 2, ny + 2
-        ns = self.so.shape[0] if lvl == 0 else 5
+        ns = self.so.shape[-3] if lvl == 0 else 5
+        lead = (self.nb,) if self.batched else ()
 
         def grab(ptr, planes):
             if not ptr:
                 return None
-            out = np.empty((planes, jj, ii))
+            out = np.empty(lead + (planes, jj, ii))
             torch.cuda.synchronize()
             _memcpy_d2h(out, ptr)
             return out
@@ -260,11 +270,12 @@ class Solver:
         if x is None:
             x = torch.zeros_like(b)
         x = _t(x)
-        hist = (ctypes.c_double * max(self.maxiter, 1))()
-        nit = ctypes.c_int(0)
-        _abi.call("cdr_hip_solver2d_solve", self._h, _p(b), _p(x), hist, len(hist),
-                  ctypes.byref(nit))
-        return x, np.array(hist[:nit.value])
+        m = max(self.maxiter, 1)
+        hist = (ctypes.c_double * (m * self.nb))()
+        nit = (ctypes.c_int * self.nb)()
+        _abi.call("cdr_hip_solver2d_solve", self._h, _p(b), _p(x), hist, m, nit)
+        hs = [np.array(hist[k * m:k * m + nit[k]]) for k in range(self.nb)]
+        return x, (hs if self.batched else hs[0])
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:

@@ -57,11 +57,17 @@ __device__ inline void block_sum_store(double v, double *dst) {
     }
 }
 
+// Batched problems: every kernel below takes the problem index from the last grid dimension
+// (blockIdx.z for tiled kernels, blockIdx.y for line kernels, blockIdx.x for one-workgroup
+// kernels); the problems of a batch are stored back to back, each with the layout above.
+//
 // one block of 1024 threads: sum of n partials in a fixed order -> *out.  Thread t adds
 // partials t, t+1024, t+2048, ... in that order; loads are batched so they overlap.
 __global__ __launch_bounds__(1024) void finalize_sum_k(const double *__restrict__ partials,
                                                        std::size_t n, double *__restrict__ out) {
     constexpr int T = 1024, U = 8;
+    partials += (std::size_t)blockIdx.x * n;  // one workgroup per problem of a batch
+    out += blockIdx.x;
     double v = 0.0;
     std::size_t k = threadIdx.x;
     for (; k + (U - 1) * T < n; k += U * T) {
@@ -101,7 +107,12 @@ __global__ __launch_bounds__(256) void residual_k(const T *__restrict__ so, cons
                                                   int ii, int jj, double *__restrict__ partials) {
     const int i = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
     const int j = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
-    const std::size_t P = (std::size_t)ii * jj;
+    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.z;
+    so += bz * (NINE ? 5 : 3) * P;
+    qf += bz * P;
+    q += bz * P;
+    res += bz * P;
+    if (NORM) partials += bz * gridDim.x * gridDim.y;
     double sq = 0.0;
     if (i <= ii - 2 && j <= jj - 2) {
         const std::size_t p = pidx(i, j, ii);
@@ -118,6 +129,9 @@ __global__ __launch_bounds__(256) void sumsq_k(const T *__restrict__ x, int ii, 
                                                double *__restrict__ partials) {
     const int i = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
     const int j = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
+    const std::size_t bz = blockIdx.z;
+    x += bz * (std::size_t)ii * jj;
+    partials += bz * gridDim.x * gridDim.y;
     double sq = 0.0;
     if (i <= ii - 2 && j <= jj - 2) {
         double v = (double)x[pidx(i, j, ii)];
@@ -134,15 +148,25 @@ __global__ __launch_bounds__(256) void residual_ir_k(const double *__restrict__ 
                                                      const float *__restrict__ e,
                                                      double *__restrict__ xnew,
                                                      float *__restrict__ r32, int ii, int jj,
-                                                     double *__restrict__ partials) {
+                                                     double *__restrict__ partials,
+                                                     const int *__restrict__ active) {
     const int i = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
     const int j = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
-    const std::size_t P = (std::size_t)ii * jj;
+    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.z;
+    so += bz * (NINE ? 5 : 3) * P;
+    qf += bz * P;
+    x += bz * P;
+    if (HAS_E) e += bz * P;
+    xnew += bz * P;
+    r32 += bz * P;
+    partials += bz * gridDim.x * gridDim.y;
+    // a problem of a batch that has already converged keeps its iterate (active[b] == 0)
+    const bool upd = HAS_E && (active == nullptr || active[bz] != 0);
     double sq = 0.0;
     if (i <= ii - 2 && j <= jj - 2) {
         const std::size_t p = pidx(i, j, ii);
         auto X = [&](std::size_t k) -> double {
-            return HAS_E ? x[k] + (double)e[k] : x[k];
+            return upd ? x[k] + (double)e[k] : x[k];
         };
         double r = qf[p];
         r = r + so[P * KW + p] * X(p - 1);
@@ -168,11 +192,14 @@ __global__ __launch_bounds__(256) void residual_ir_k(const double *__restrict__ 
 // point Gauss-Seidel, one launch per colour (src/2d/ftn/BMG2_SymStd_relax_GS.f90:78-137)
 // ------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void recip_k(const double *__restrict__ so,
-                                               double *__restrict__ sor, int ii, int jj) {
+                                               double *__restrict__ sor, int ii, int jj, int ns) {
     const int i = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
     const int j = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
+    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.z;
+    so += bz * ns * P;
+    sor += bz * 2 * P;
     if (i <= ii - 2 && j <= jj - 2) {
-        const std::size_t p = pidx(i, j, ii), P = (std::size_t)ii * jj;
+        const std::size_t p = pidx(i, j, ii);
         sor[P + p] = 1.0 / so[p];
     }
 }
@@ -193,7 +220,11 @@ __global__ __launch_bounds__(256) void gs_colour_k(const T *__restrict__ so, con
         i = 2 * k + ((pi + j) & 1);  // (i + j) % 2 == pi
     }
     if (i < 1 || i > ii - 2 || j < 1 || j > jj - 2) return;
-    const std::size_t P = (std::size_t)ii * jj;
+    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.z;
+    so += bz * (NINE ? 5 : 3) * P;
+    qf += bz * P;
+    q += bz * P;
+    sor += bz * 2 * P;
     const std::size_t p = pidx(i, j, ii);
     T v = apply_offdiag<T, NINE>(so, P, q, p, ii, qf[p]);
     q[p] = v * sor[P + p];
@@ -208,10 +239,12 @@ __global__ __launch_bounds__(256) void gs_colour_k(const T *__restrict__ so, con
 // transposed: plane k element (j, i) at k*ii*jj + i*jj + j.
 // ------------------------------------------------------------------------------------------
 __global__ void setup_lines_x_k(const double *__restrict__ so, double *__restrict__ sor, int ii,
-                                int jj, int *__restrict__ info) {
+                                int jj, int ns, int *__restrict__ info) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
     if (j > jj - 2) return;
     const std::size_t P = (std::size_t)ii * jj;
+    so += (std::size_t)blockIdx.y * ns * P;
+    sor += (std::size_t)blockIdx.y * 2 * P;
     const int n = ii - 2;
     double *D = sor + pidx(1, j, ii);      // D[k] at i = k + 1
     double *E = sor + P + pidx(2, j, ii);  // E[k] at i = k + 2
@@ -223,21 +256,23 @@ __global__ void setup_lines_x_k(const double *__restrict__ so, double *__restric
     for (int k = 0; k < n - 1; ++k) {
         const double d = D[k];
         if (!(d > 0.0)) {
-            atomicMin(info, k + 1);
+            atomicMax(info, k + 1);
             return;
         }
         const double ei = E[k];
         E[k] = ei / d;
         D[k + 1] = D[k + 1] - E[k] * ei;
     }
-    if (!(D[n - 1] > 0.0)) atomicMin(info, n);
+    if (!(D[n - 1] > 0.0)) atomicMax(info, n);
 }
 
 __global__ void setup_lines_y_k(const double *__restrict__ so, double *__restrict__ sor, int ii,
-                                int jj, int *__restrict__ info) {
+                                int jj, int ns, int *__restrict__ info) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     if (i > ii - 2) return;
     const std::size_t P = (std::size_t)ii * jj;
+    so += (std::size_t)blockIdx.y * ns * P;
+    sor += (std::size_t)blockIdx.y * 2 * P;
     const int n = jj - 2;
     double *D = sor + (std::size_t)i * jj + 1;
     double *E = sor + P + (std::size_t)i * jj + 2;
@@ -249,14 +284,14 @@ __global__ void setup_lines_y_k(const double *__restrict__ so, double *__restric
     for (int k = 0; k < n - 1; ++k) {
         const double d = D[k];
         if (!(d > 0.0)) {
-            atomicMin(info, k + 1);
+            atomicMax(info, k + 1);
             return;
         }
         const double ei = E[k];
         E[k] = ei / d;
         D[k + 1] = D[k + 1] - E[k] * ei;
     }
-    if (!(D[n - 1] > 0.0)) atomicMin(info, n);
+    if (!(D[n - 1] > 0.0)) atomicMax(info, n);
 }
 
 // Inclusive scans of affine maps y -> a*y + b across the 64 lanes of a wavefront.
@@ -351,7 +386,11 @@ __global__ __launch_bounds__(256) void relax_lines_k(const T *__restrict__ so, c
     const int lane = threadIdx.x % WAVE;
     const int line = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     if (line >= nlines) return;  // whole wavefront leaves together; no block barrier below
-    const std::size_t P = (std::size_t)ii * jj;
+    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.y;
+    so += bz * (NINE ? 5 : 3) * P;
+    qf += bz * P;
+    q += bz * P;
+    sor += bz * 2 * P;
     const int fixed = first + 2 * line;
     const int n = (DIR == 0) ? ii - 2 : jj - 2;
     const std::size_t stride = (DIR == 0) ? 1 : (std::size_t)ii;
@@ -502,7 +541,11 @@ __global__ __launch_bounds__(256) void relax_xlines_reg_k(const T *__restrict__ 
     const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
     const int line = tile * WPB + wave;
     if (line >= nlines) return;  // whole wavefront; no workgroup barrier in this kernel
-    const std::size_t P = (std::size_t)ii * jj;
+    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.y;
+    so += bz * (NINE ? 5 : 3) * P;
+    qf += bz * P;
+    q += bz * P;
+    sor += bz * 2 * P;
     const int j = first + 2 * line;
     const int n = ii - 2;
     const int nact = (n + WAVE - 1) / WAVE;
@@ -609,7 +652,11 @@ __global__ __launch_bounds__(NW * 64) void relax_ylines_reg_k(const T *__restric
     const int tc = threadIdx.x % NW, tr = threadIdx.x / NW;
     const bool tok = col0 + tc < ncols;
     const int ti = first + 2 * (col0 + tc);
-    const std::size_t P = (std::size_t)ii * jj;
+    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.y;
+    so += bz * (NINE ? 5 : 3) * P;
+    qf += bz * P;
+    q += bz * P;
+    sor += bz * 2 * P;
     const int n = jj - 2;
     const int nact = (n + WAVE - 1) / WAVE;
     const T *D = sor + (std::size_t)ime * jj + 1;
@@ -714,7 +761,10 @@ __global__ __launch_bounds__(256) void restrict_k(const T *__restrict__ ci, cons
     const int ic = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
     const int jc = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
     if (ic > iic - 2 || jc > jjc - 2) return;
-    const std::size_t Pc = (std::size_t)iic * jjc;
+    const std::size_t Pc = (std::size_t)iic * jjc, bz = blockIdx.z;
+    ci += bz * 8 * Pc;
+    q += bz * iif * (std::size_t)jjf;
+    qc += bz * Pc;
     const int i = 2 * ic - 1, j = 2 * jc - 1;
     auto CI = [&](int a, int b, int k) { return ci[Pc * k + pidx(a, b, iic)]; };
     auto Q = [&](int a, int b) { return q[pidx(a, b, iif)]; };
@@ -742,13 +792,18 @@ __global__ __launch_bounds__(256) void interp_add_k(T *__restrict__ q, const T *
                                                     const T *__restrict__ res,
                                                     const T *__restrict__ so,
                                                     const T *__restrict__ ci, int iic, int jjc,
-                                                    int iif, int jjf) {
+                                                    int iif, int jjf, int ns) {
     const int i = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
     const int j = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
     const int imax = iif - 2 + ((iif - 2) % 2 == 0 ? 1 : 0);
     const int jmax = jjf - 2 + ((jjf - 2) % 2 == 0 ? 1 : 0);
     if (i > imax || j > jmax) return;
-    const std::size_t Pc = (std::size_t)iic * jjc;
+    const std::size_t Pc = (std::size_t)iic * jjc, Pf = (std::size_t)iif * jjf, bz = blockIdx.z;
+    q += bz * Pf;
+    qc += bz * Pc;
+    res += bz * Pf;
+    so += bz * ns * Pf;
+    ci += bz * 8 * Pc;
     auto CI = [&](int a, int b, int k) { return ci[Pc * k + pidx(a, b, iic)]; };
     auto QC = [&](int a, int b) { return qc[pidx(a, b, iic)]; };
     const std::size_t p = pidx(i, j, iif);
@@ -791,7 +846,9 @@ __global__ __launch_bounds__(256) void oi_edges_k(const double *__restrict__ so,
                                                   int iif, int jjf, int iic, int jjc) {
     const int ic = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
     const int jc = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
-    const std::size_t P = (std::size_t)iif * jjf, Pc = (std::size_t)iic * jjc;
+    const std::size_t P = (std::size_t)iif * jjf, Pc = (std::size_t)iic * jjc, bz = blockIdx.z;
+    so += bz * (NINE ? 5 : 3) * P;
+    ci += bz * 8 * Pc;
     auto S = [&](int a, int b, int k) { return so[P * k + pidx(a, b, iif)]; };
     const int icmax = (iif - 2) / 2 + 1, jcmax = (jjf - 2) / 2 + 1;
     const int i = 2 * ic - 1, j = 2 * jc - 1;
@@ -840,7 +897,9 @@ __global__ __launch_bounds__(256) void oi_corners_k(const double *__restrict__ s
     const int jc = blockIdx.y * 4 + (threadIdx.x / 64) + 2;
     const int icmax = (iif - 2) / 2 + 1, jcmax = (jjf - 2) / 2 + 1;
     if (ic > icmax || jc > jcmax) return;
-    const std::size_t P = (std::size_t)iif * jjf, Pc = (std::size_t)iic * jjc;
+    const std::size_t P = (std::size_t)iif * jjf, Pc = (std::size_t)iic * jjc, bz = blockIdx.z;
+    so += bz * (NINE ? 5 : 3) * P;
+    ci += bz * 8 * Pc;
     auto S = [&](int a, int b, int k) { return so[P * k + pidx(a, b, iif)]; };
     auto CI = [&](int a, int b, int k) { return ci[Pc * k + pidx(a, b, iic)]; };
     const int i = 2 * ic - 1, j = 2 * jc - 1;
@@ -961,7 +1020,10 @@ __global__ __launch_bounds__(256) void galerkin_k(const double *__restrict__ so,
     const int ic = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
     const int jc = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
     if (ic > iic - 2 || jc > jjc - 2) return;
-    const std::size_t Pc = (std::size_t)iic * jjc, pc = pidx(ic, jc, iic);
+    const std::size_t Pc = (std::size_t)iic * jjc, pc = pidx(ic, jc, iic), bz = blockIdx.z;
+    so += bz * (NINE ? 5 : 3) * (std::size_t)iif * jjf;
+    ci += bz * 8 * Pc;
+    soc += bz * 5 * Pc;
     soc[pc] = kpair<NINE, 0, 0>(so, ci, iif, jjf, iic, jjc, ic, jc);
     soc[Pc * KW + pc] = (ic >= 2) ? -kpair<NINE, -1, 0>(so, ci, iif, jjf, iic, jjc, ic, jc) : 0.0;
     soc[Pc * KS + pc] = (jc >= 2) ? -kpair<NINE, 0, -1>(so, ci, iif, jjf, iic, jjc, ic, jc) : 0.0;
@@ -980,6 +1042,8 @@ __global__ __launch_bounds__(256) void setup_cg_k(const double *__restrict__ so,
                                                   int *__restrict__ info) {
     const int nx = ii - 2, ny = jj - 2, n = nx * ny, kd = ii - 1, ld = ii;
     const std::size_t P = (std::size_t)ii * jj;
+    so += (std::size_t)blockIdx.x * nstncl * P;  // one workgroup per problem of a batch
+    abd += (std::size_t)blockIdx.x * ld * n;
     auto AB = [&](int r, int c) -> double & { return abd[(std::size_t)(c - 1) * ld + (r - 1)]; };
     for (int kk = threadIdx.x; kk < n; kk += blockDim.x) {
         const int i = kk % nx + 1, j = kk / nx + 1;
@@ -1029,7 +1093,7 @@ __global__ __launch_bounds__(256) void setup_cg_k(const double *__restrict__ so,
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) *info = bad;
+    if (threadIdx.x == 0 && bad) atomicMax(info, bad);
 }
 
 template <typename T>
@@ -1037,6 +1101,10 @@ __global__ void solve_cg_k(T *__restrict__ q, const T *__restrict__ qf, const do
                            double *__restrict__ x, int ii, int jj) {
     if (threadIdx.x != 0) return;
     const int nx = ii - 2, ny = jj - 2, n = nx * ny, kd = ii - 1, ld = ii, kp1 = kd + 1;
+    q += (std::size_t)blockIdx.x * ii * jj;  // one workgroup per problem of a batch
+    qf += (std::size_t)blockIdx.x * ii * jj;
+    abd += (std::size_t)blockIdx.x * ld * n;
+    x += (std::size_t)blockIdx.x * n;
     auto AB = [&](int r, int c) { return abd[(std::size_t)(c - 1) * ld + (r - 1)]; };
     for (int kk = 0; kk < n; ++kk) x[kk] = (double)qf[pidx(kk % nx + 1, kk / nx + 1, ii)];
     for (int j = 1; j <= n; ++j) {
@@ -1081,8 +1149,18 @@ __global__ __launch_bounds__(256) void add_correction_k(double *__restrict__ x, 
     const int i = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
     const int j = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
     if (i > ii - 2 || j > jj - 2) return;
-    const std::size_t p = pidx(i, j, ii);
+    const std::size_t p = pidx(i, j, ii) + (std::size_t)blockIdx.z * ii * jj;
     x[p] = x[p] + (double)e[p];
+}
+
+// x[k] = xprev[k] for the problems of a batch with active[b] == 0 (whole arrays, ghosts included)
+__global__ __launch_bounds__(256) void keep_inactive_k(double *__restrict__ x,
+                                                       const double *__restrict__ xprev,
+                                                       const int *__restrict__ active,
+                                                       std::size_t per, std::size_t n) {
+    for (std::size_t k = (std::size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (std::size_t)gridDim.x * blockDim.x)
+        if (active[k / per] == 0) x[k] = xprev[k];
 }
 
 inline unsigned grid1(std::size_t n) {
@@ -1101,10 +1179,17 @@ std::size_t norm_partials(int ii, int jj) {
     return (std::size_t)g.x * g.y;
 }
 
+// tiled grid over an nx x ny index range, one z-slice per problem of a batch
+static dim3 gridb(int nx, int ny, int nb) {
+    dim3 g = grid2(nx, ny, 64, 4);
+    g.z = (unsigned)nb;
+    return g;
+}
+
 template <typename T>
 void residual(const T *so, int nstncl, const T *qf, const T *q, T *res, int ii, int jj,
-              hipStream_t s) {
-    const dim3 g = grid2(ii - 2, jj - 2, 64, 4);
+              hipStream_t s, int nb) {
+    const dim3 g = gridb(ii - 2, jj - 2, nb);
     if (nstncl == 5)
         residual_k<T, true, false><<<g, 256, 0, s>>>(so, qf, q, res, ii, jj, nullptr);
     else
@@ -1113,58 +1198,63 @@ void residual(const T *so, int nstncl, const T *qf, const T *q, T *res, int ii, 
 
 template <typename T>
 void residual_norm(const T *so, int nstncl, const T *qf, const T *q, T *res, int ii, int jj,
-                   double *partials, double *sumsq_out, hipStream_t s) {
-    const dim3 g = grid2(ii - 2, jj - 2, 64, 4);
+                   double *partials, double *sumsq_out, hipStream_t s, int nb) {
+    const dim3 g = gridb(ii - 2, jj - 2, nb);
     if (nstncl == 5)
         residual_k<T, true, true><<<g, 256, 0, s>>>(so, qf, q, res, ii, jj, partials);
     else
         residual_k<T, false, true><<<g, 256, 0, s>>>(so, qf, q, res, ii, jj, partials);
-    finalize_sum_k<<<1, 1024, 0, s>>>(partials, (std::size_t)g.x * g.y, sumsq_out);
+    finalize_sum_k<<<nb, 1024, 0, s>>>(partials, (std::size_t)g.x * g.y, sumsq_out);
 }
 
 template <typename T>
-void sumsq(const T *x, int ii, int jj, double *partials, double *out, hipStream_t s) {
-    const dim3 g = grid2(ii - 2, jj - 2, 64, 4);
+void sumsq(const T *x, int ii, int jj, double *partials, double *out, hipStream_t s, int nb) {
+    const dim3 g = gridb(ii - 2, jj - 2, nb);
     sumsq_k<T><<<g, 256, 0, s>>>(x, ii, jj, partials);
-    finalize_sum_k<<<1, 1024, 0, s>>>(partials, (std::size_t)g.x * g.y, out);
+    finalize_sum_k<<<nb, 1024, 0, s>>>(partials, (std::size_t)g.x * g.y, out);
 }
 
 void residual_ir(const double *so, int nstncl, const double *qf, const double *x, const float *e,
                  double *xnew, float *r32, int ii, int jj, double *partials, double *sumsq_out,
-                 hipStream_t s) {
-    const dim3 g = grid2(ii - 2, jj - 2, 64, 4);
+                 hipStream_t s, int nb, const int *active) {
+    const dim3 g = gridb(ii - 2, jj - 2, nb);
     if (nstncl == 5) {
         if (e)
-            residual_ir_k<true, true><<<g, 256, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, partials);
+            residual_ir_k<true, true><<<g, 256, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, partials,
+                                                               active);
         else
-            residual_ir_k<true, false><<<g, 256, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, partials);
+            residual_ir_k<true, false><<<g, 256, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, partials,
+                                                               active);
     } else {
         if (e)
-            residual_ir_k<false, true><<<g, 256, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, partials);
+            residual_ir_k<false, true><<<g, 256, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, partials,
+                                                               active);
         else
-            residual_ir_k<false, false><<<g, 256, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, partials);
+            residual_ir_k<false, false><<<g, 256, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, partials,
+                                                               active);
     }
-    finalize_sum_k<<<1, 1024, 0, s>>>(partials, (std::size_t)g.x * g.y, sumsq_out);
+    finalize_sum_k<<<nb, 1024, 0, s>>>(partials, (std::size_t)g.x * g.y, sumsq_out);
 }
 
-void setup_recip(const double *so, double *sor, int ii, int jj, hipStream_t s) {
-    recip_k<<<grid2(ii - 2, jj - 2, 64, 4), 256, 0, s>>>(so, sor, ii, jj);
+void setup_recip(const double *so, int nstncl, double *sor, int ii, int jj, hipStream_t s,
+                 int nb) {
+    recip_k<<<gridb(ii - 2, jj - 2, nb), 256, 0, s>>>(so, sor, ii, jj, nstncl);
 }
 
 template <typename T>
 void relax_gs(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
-              int updown, hipStream_t s) {
+              int updown, hipStream_t s, int nb) {
     if (nstncl == 5) {
         // DOWN: (i odd, j odd), (even, odd), (odd, even), (even, even); UP: reverse
         static const int order[4][2] = {{1, 1}, {0, 1}, {1, 0}, {0, 0}};
-        const dim3 g = grid2(ii / 2 + 1, jj / 2 + 1, 64, 4);
+        const dim3 g = gridb(ii / 2 + 1, jj / 2 + 1, nb);
         for (int t = 0; t < 4; ++t) {
             const int c = updown == 0 ? t : 3 - t;
             gs_colour_k<T, true><<<g, 256, 0, s>>>(so, qf, q, sor, ii, jj, order[c][0], order[c][1]);
         }
     } else {
         // DOWN: (i + j) even first; UP: odd first
-        const dim3 g = grid2(ii / 2 + 1, jj - 2, 64, 4);
+        const dim3 g = gridb(ii / 2 + 1, jj - 2, nb);
         for (int t = 0; t < 2; ++t) {
             const int parity = updown == 0 ? t : 1 - t;
             gs_colour_k<T, false><<<g, 256, 0, s>>>(so, qf, q, sor, ii, jj, parity, 0);
@@ -1172,23 +1262,25 @@ void relax_gs(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, 
     }
 }
 
-void setup_lines_x(const double *so, double *sor, int ii, int jj, int *info, hipStream_t s) {
+void setup_lines_x(const double *so, int nstncl, double *sor, int ii, int jj, int *info,
+                   hipStream_t s, int nb) {
     const int n = jj - 2;
-    setup_lines_x_k<<<(n + 63) / 64, 64, 0, s>>>(so, sor, ii, jj, info);
+    setup_lines_x_k<<<dim3((n + 63) / 64, nb), 64, 0, s>>>(so, sor, ii, jj, nstncl, info);
 }
 
-void setup_lines_y(const double *so, double *sor, int ii, int jj, int *info, hipStream_t s) {
+void setup_lines_y(const double *so, int nstncl, double *sor, int ii, int jj, int *info,
+                   hipStream_t s, int nb) {
     const int n = ii - 2;
-    setup_lines_y_k<<<(n + 63) / 64, 64, 0, s>>>(so, sor, ii, jj, info);
+    setup_lines_y_k<<<dim3((n + 63) / 64, nb), 64, 0, s>>>(so, sor, ii, jj, nstncl, info);
 }
 
 template <typename T, bool NINE, int DIR, int NCH>
 static void launch_lines_reg(const T *so, const T *qf, T *q, const T *sor, int ii, int jj,
-                             int first, int nlines, hipStream_t s) {
+                             int first, int nlines, hipStream_t s, int nb) {
     constexpr int NW = 8;  // y-lines: columns per workgroup
     const int per_tile = (DIR == 0) ? 4 : NW;
     const int ntiles = (nlines + per_tile - 1) / per_tile;
-    const unsigned blocks = (unsigned)(8 * ((ntiles + 7) / 8));
+    const dim3 blocks((unsigned)(8 * ((ntiles + 7) / 8)), (unsigned)nb);
     if (DIR == 0)
         relax_xlines_reg_k<T, NINE, NCH><<<blocks, 256, 0, s>>>(so, qf, q, sor, ii, jj, first,
                                                                nlines, ntiles);
@@ -1199,10 +1291,10 @@ static void launch_lines_reg(const T *so, const T *qf, T *q, const T *sor, int i
 
 template <typename T, bool NINE, int DIR>
 static void launch_lines(const T *so, const T *qf, T *q, const T *sor, int ii, int jj, int first,
-                         int nlines, hipStream_t s) {
+                         int nlines, hipStream_t s, int nb) {
     const int n = (DIR == 0) ? ii - 2 : jj - 2;
     const int nch = (n + WAVE - 1) / WAVE;
-#define CDR_L(NC) launch_lines_reg<T, NINE, DIR, NC>(so, qf, q, sor, ii, jj, first, nlines, s)
+#define CDR_L(NC) launch_lines_reg<T, NINE, DIR, NC>(so, qf, q, sor, ii, jj, first, nlines, s, nb)
     if (nch <= 1) CDR_L(1);
     else if (nch <= 2) CDR_L(2);
     else if (nch <= 4) CDR_L(4);
@@ -1211,14 +1303,14 @@ static void launch_lines(const T *so, const T *qf, T *q, const T *sor, int ii, i
     else if (nch <= 32) CDR_L(32);
     else if (nch <= 64) CDR_L(64);
     else  // lines longer than 4096: stage the forward results in q itself
-        relax_lines_k<T, NINE, DIR><<<(unsigned)((nlines + 3) / 4), 256, 0, s>>>(so, qf, q, sor, ii,
-                                                                                jj, first, nlines);
+        relax_lines_k<T, NINE, DIR><<<dim3((unsigned)((nlines + 3) / 4), (unsigned)nb), 256, 0,
+                                      s>>>(so, qf, q, sor, ii, jj, first, nlines);
 #undef CDR_L
 }
 
 template <typename T, int DIR>
 static void relax_lines(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
-                        int updown, hipStream_t s) {
+                        int updown, hipStream_t s, int nb) {
     // lines of one colour: DOWN relaxes the lines at even array index (odd Cedar index) first,
     // UP the odd ones first (src/2d/ftn/BMG2_SymStd_relax_lines_x.f90:82-97)
     const int nfix = (DIR == 0) ? jj - 2 : ii - 2;
@@ -1227,44 +1319,44 @@ static void relax_lines(const T *so, int nstncl, const T *qf, T *q, const T *sor
         if (first > nfix) continue;
         const int nlines = (nfix - first) / 2 + 1;
         if (nstncl == 5)
-            launch_lines<T, true, DIR>(so, qf, q, sor, ii, jj, first, nlines, s);
+            launch_lines<T, true, DIR>(so, qf, q, sor, ii, jj, first, nlines, s, nb);
         else
-            launch_lines<T, false, DIR>(so, qf, q, sor, ii, jj, first, nlines, s);
+            launch_lines<T, false, DIR>(so, qf, q, sor, ii, jj, first, nlines, s, nb);
     }
 }
 
 template <typename T>
 void relax_lines_x(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
-                   int updown, hipStream_t s) {
-    relax_lines<T, 0>(so, nstncl, qf, q, sor, ii, jj, updown, s);
+                   int updown, hipStream_t s, int nb) {
+    relax_lines<T, 0>(so, nstncl, qf, q, sor, ii, jj, updown, s, nb);
 }
 
 template <typename T>
 void relax_lines_y(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
-                   int updown, hipStream_t s) {
-    relax_lines<T, 1>(so, nstncl, qf, q, sor, ii, jj, updown, s);
+                   int updown, hipStream_t s, int nb) {
+    relax_lines<T, 1>(so, nstncl, qf, q, sor, ii, jj, updown, s, nb);
 }
 
 template <typename T>
-void restrict_(const T *ci, const T *q, T *qc, int iif, int jjf, int iic, int jjc, hipStream_t s) {
-    restrict_k<T><<<grid2(iic - 2, jjc - 2, 64, 4), 256, 0, s>>>(ci, q, qc, iif, jjf, iic, jjc);
+void restrict_(const T *ci, const T *q, T *qc, int iif, int jjf, int iic, int jjc, hipStream_t s,
+               int nb) {
+    restrict_k<T><<<gridb(iic - 2, jjc - 2, nb), 256, 0, s>>>(ci, q, qc, iif, jjf, iic, jjc);
 }
 
 template <typename T>
 void interp_add(T *q, const T *qc, const T *res, const T *so, int nstncl, const T *ci, int iic,
-                int jjc, int iif, int jjf, hipStream_t s) {
-    (void)nstncl;
-    interp_add_k<T><<<grid2(iif - 1, jjf - 1, 64, 4), 256, 0, s>>>(q, qc, res, so, ci, iic, jjc,
-                                                                   iif, jjf);
+                int jjc, int iif, int jjf, hipStream_t s, int nb) {
+    interp_add_k<T><<<gridb(iif - 1, jjf - 1, nb), 256, 0, s>>>(q, qc, res, so, ci, iic, jjc, iif,
+                                                                jjf, nstncl);
 }
 
 void setup_interp(const double *so, int nstncl, double *ci, int iif, int jjf, int iic, int jjc,
-                  hipStream_t s) {
+                  hipStream_t s, int nb) {
     const int icmax = (iif - 2) / 2 + 1, jcmax = (jjf - 2) / 2 + 1;
     const int ne = icmax > iic - 2 ? icmax : iic - 2;
     const int me = jcmax > jjc - 2 ? jcmax : jjc - 2;
-    const dim3 ge = grid2(ne, me, 64, 4);
-    const dim3 gc = grid2(icmax - 1, jcmax - 1, 64, 4);
+    const dim3 ge = gridb(ne, me, nb);
+    const dim3 gc = gridb(icmax - 1, jcmax - 1, nb);
     if (nstncl == 5) {
         oi_edges_k<true><<<ge, 256, 0, s>>>(so, ci, iif, jjf, iic, jjc);
         oi_corners_k<true><<<gc, 256, 0, s>>>(so, ci, iif, jjf, iic, jjc);
@@ -1275,21 +1367,23 @@ void setup_interp(const double *so, int nstncl, double *ci, int iif, int jjf, in
 }
 
 void setup_galerkin(const double *so, int nstncl, const double *ci, double *soc, int iif, int jjf,
-                    int iic, int jjc, hipStream_t s) {
-    const dim3 g = grid2(iic - 2, jjc - 2, 64, 4);
+                    int iic, int jjc, hipStream_t s, int nb) {
+    const dim3 g = gridb(iic - 2, jjc - 2, nb);
     if (nstncl == 5)
         galerkin_k<true><<<g, 256, 0, s>>>(so, ci, soc, iif, jjf, iic, jjc);
     else
         galerkin_k<false><<<g, 256, 0, s>>>(so, ci, soc, iif, jjf, iic, jjc);
 }
 
-void setup_cg(const double *so, int nstncl, double *abd, int ii, int jj, int *info, hipStream_t s) {
-    setup_cg_k<<<1, 256, 0, s>>>(so, nstncl, abd, ii, jj, info);
+void setup_cg(const double *so, int nstncl, double *abd, int ii, int jj, int *info, hipStream_t s,
+              int nb) {
+    setup_cg_k<<<nb, 256, 0, s>>>(so, nstncl, abd, ii, jj, info);
 }
 
 template <typename T>
-void solve_cg(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj, hipStream_t s) {
-    solve_cg_k<T><<<1, 64, 0, s>>>(q, qf, abd, bbd, ii, jj);
+void solve_cg(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj, hipStream_t s,
+              int nb) {
+    solve_cg_k<T><<<nb, 64, 0, s>>>(q, qf, abd, bbd, ii, jj);
 }
 
 template <typename Tin, typename Tout>
@@ -1302,26 +1396,34 @@ void fill(T *x, T v, std::size_t n, hipStream_t s) {
     fill_k<T><<<grid1(n), 256, 0, s>>>(x, v, n);
 }
 
-void add_correction(double *x, const float *e, int ii, int jj, hipStream_t s) {
-    add_correction_k<<<grid2(ii - 2, jj - 2, 64, 4), 256, 0, s>>>(x, e, ii, jj);
+void keep_inactive(double *x, const double *xprev, const int *active, std::size_t per, int nb,
+                   hipStream_t s) {
+    const std::size_t n = per * (std::size_t)nb;
+    keep_inactive_k<<<grid1(n), 256, 0, s>>>(x, xprev, active, per, n);
+}
+
+void add_correction(double *x, const float *e, int ii, int jj, hipStream_t s, int nb) {
+    add_correction_k<<<gridb(ii - 2, jj - 2, nb), 256, 0, s>>>(x, e, ii, jj);
 }
 
 // explicit instantiations
 #define CDR_INST(T)                                                                               \
-    template void residual<T>(const T *, int, const T *, const T *, T *, int, int, hipStream_t);  \
+    template void residual<T>(const T *, int, const T *, const T *, T *, int, int, hipStream_t,   \
+                              int);                                                               \
     template void residual_norm<T>(const T *, int, const T *, const T *, T *, int, int, double *, \
-                                   double *, hipStream_t);                                        \
-    template void sumsq<T>(const T *, int, int, double *, double *, hipStream_t);                 \
+                                   double *, hipStream_t, int);                                   \
+    template void sumsq<T>(const T *, int, int, double *, double *, hipStream_t, int);            \
     template void relax_gs<T>(const T *, int, const T *, T *, const T *, int, int, int,           \
-                              hipStream_t);                                                       \
+                              hipStream_t, int);                                                  \
     template void relax_lines_x<T>(const T *, int, const T *, T *, const T *, int, int, int,      \
-                                   hipStream_t);                                                  \
+                                   hipStream_t, int);                                             \
     template void relax_lines_y<T>(const T *, int, const T *, T *, const T *, int, int, int,      \
-                                   hipStream_t);                                                  \
-    template void restrict_<T>(const T *, const T *, T *, int, int, int, int, hipStream_t);       \
+                                   hipStream_t, int);                                             \
+    template void restrict_<T>(const T *, const T *, T *, int, int, int, int, hipStream_t, int);  \
     template void interp_add<T>(T *, const T *, const T *, const T *, int, const T *, int, int,   \
-                                int, int, hipStream_t);                                           \
-    template void solve_cg<T>(T *, const T *, const double *, double *, int, int, hipStream_t);   \
+                                int, int, hipStream_t, int);                                      \
+    template void solve_cg<T>(T *, const T *, const double *, double *, int, int, hipStream_t,    \
+                              int);                                                               \
     template void fill<T>(T *, T, std::size_t, hipStream_t);
 CDR_INST(double)
 CDR_INST(float)

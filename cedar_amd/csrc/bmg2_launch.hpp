@@ -4,6 +4,13 @@
 // layer, ghost-inclusive extents ii = nx+2, jj = ny+2, stencils as nstncl contiguous planes of
 // ii*jj values {c,w,s[,sw,nw]} -- and enqueues work on `stream`.  No launcher allocates or
 // synchronises; argument checking happens in the C ABI (capi2d.hip).
+//
+// The trailing `nb` runs the same call on nb problems of identical extents stored back to back
+// (each array of problem b starts at b times the size of one problem's array); per-problem
+// outputs of the reductions (sumsq) are consecutive doubles, and `partials` holds
+// nb * norm_partials(ii, jj) values.  Factorisation info words are shared by the batch: the
+// kernels raise them with atomicMax, so they must start at 0 and end at 0 when every problem is
+// SPD.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -25,52 +32,61 @@ std::size_t norm_partials(int ii, int jj);
 
 template <typename T>
 void residual(const T *so, int nstncl, const T *qf, const T *q, T *res, int ii, int jj,
-              hipStream_t s);
+              hipStream_t s, int nb = 1);
 // residual + per-block partial sums of r^2 (FP64) + final reduction into *sumsq (device)
 template <typename T>
 void residual_norm(const T *so, int nstncl, const T *qf, const T *q, T *res, int ii, int jj,
-                   double *partials, double *sumsq, hipStream_t s);
+                   double *partials, double *sumsq, hipStream_t s, int nb = 1);
 // sum of squares of the interior of x into *sumsq (device), deterministic two-stage reduction
 template <typename T>
-void sumsq(const T *x, int ii, int jj, double *partials, double *out, hipStream_t s);
+void sumsq(const T *x, int ii, int jj, double *partials, double *out, hipStream_t s, int nb = 1);
 
 // Mixed-precision outer step: xnew = x + (double)e (e may be null: xnew = x),
-// r = b - A xnew in FP64, r32 = (float) r, partial sums of r^2 into *sumsq.
+// r = b - A xnew in FP64, r32 = (float) r, partial sums of r^2 into *sumsq.  With `active`
+// (nb ints, may be null) a problem b with active[b] == 0 keeps xnew = x.
 void residual_ir(const double *so, int nstncl, const double *qf, const double *x, const float *e,
                  double *xnew, float *r32, int ii, int jj, double *partials, double *sumsq,
-                 hipStream_t s);
+                 hipStream_t s, int nb = 1, const int *active = nullptr);
+// x[k] = xprev[k] over every problem b (per values each) with active[b] == 0
+void keep_inactive(double *x, const double *xprev, const int *active, std::size_t per, int nb,
+                   hipStream_t s);
 
-void setup_recip(const double *so, double *sor, int ii, int jj, hipStream_t s);
+void setup_recip(const double *so, int nstncl, double *sor, int ii, int jj, hipStream_t s,
+                 int nb = 1);
 template <typename T>
 void relax_gs(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
-              int updown, hipStream_t s);
+              int updown, hipStream_t s, int nb = 1);
 
-// LDL^T factors of every line (x: rows, y: columns, stored transposed); info gets the first
-// failing pivot (1-based line-local index), 0 if all lines are SPD.
-void setup_lines_x(const double *so, double *sor, int ii, int jj, int *info, hipStream_t s);
-void setup_lines_y(const double *so, double *sor, int ii, int jj, int *info, hipStream_t s);
+// LDL^T factors of every line (x: rows, y: columns, stored transposed); info (zero on entry)
+// is raised to the largest failing pivot index (1-based, line-local); it stays 0 when every
+// line is SPD.
+void setup_lines_x(const double *so, int nstncl, double *sor, int ii, int jj, int *info,
+                   hipStream_t s, int nb = 1);
+void setup_lines_y(const double *so, int nstncl, double *sor, int ii, int jj, int *info,
+                   hipStream_t s, int nb = 1);
 template <typename T>
 void relax_lines_x(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
-                   int updown, hipStream_t s);
+                   int updown, hipStream_t s, int nb = 1);
 template <typename T>
 void relax_lines_y(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
-                   int updown, hipStream_t s);
+                   int updown, hipStream_t s, int nb = 1);
 
 template <typename T>
-void restrict_(const T *ci, const T *q, T *qc, int iif, int jjf, int iic, int jjc, hipStream_t s);
+void restrict_(const T *ci, const T *q, T *qc, int iif, int jjf, int iic, int jjc, hipStream_t s, int nb = 1);
 template <typename T>
 void interp_add(T *q, const T *qc, const T *res, const T *so, int nstncl, const T *ci, int iic,
-                int jjc, int iif, int jjf, hipStream_t s);
+                int jjc, int iif, int jjf, hipStream_t s, int nb = 1);
 
 void setup_interp(const double *so, int nstncl, double *ci, int iif, int jjf, int iic, int jjc,
-                  hipStream_t s);
+                  hipStream_t s, int nb = 1);
 void setup_galerkin(const double *so, int nstncl, const double *ci, double *soc, int iif, int jjf,
-                    int iic, int jjc, hipStream_t s);
+                    int iic, int jjc, hipStream_t s, int nb = 1);
 
-// coarsest grid: band ABD(ii, nx*ny) (upper, kd = ii-1), info on device
-void setup_cg(const double *so, int nstncl, double *abd, int ii, int jj, int *info, hipStream_t s);
+// coarsest grid: band ABD(ii, nx*ny) (upper, kd = ii-1); info on device, zero on entry, gets
+// the failing column when a pivot is not positive
+void setup_cg(const double *so, int nstncl, double *abd, int ii, int jj, int *info, hipStream_t s, int nb = 1);
 template <typename T>
-void solve_cg(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj, hipStream_t s);
+void solve_cg(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj, hipStream_t s, int nb = 1);
 
 // elementwise helpers
 template <typename Tin, typename Tout>
@@ -78,7 +94,7 @@ void convert(const Tin *in, Tout *out, std::size_t n, hipStream_t s);
 template <typename T>
 void fill(T *x, T v, std::size_t n, hipStream_t s);
 // x += (double) e over the interior
-void add_correction(double *x, const float *e, int ii, int jj, hipStream_t s);
+void add_correction(double *x, const float *e, int ii, int jj, hipStream_t s, int nb = 1);
 
 }  // namespace k2
 }  // namespace cdr

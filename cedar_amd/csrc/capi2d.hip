@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "bmg2_launch.hpp"
+#include "capi2d_impl.hpp"
 
 namespace {
 
@@ -22,72 +23,159 @@ inline bool coarse_ok(int iif, int jjf, int iic, int jjc) {
            jjc - 2 == (jjf - 3) / 2 + 1;
 }
 
-template <typename T>
-int residual_t(const T *so, int ns, const T *qf, const T *q, T *res, int ii, int jj, void *st) {
-    if (!so || !qf || !q || !res || !ext_ok(ii, jj) || !sten_ok(ns)) return CDR_ERR_ARG;
-    k2::residual<T>(so, ns, qf, q, res, ii, jj, S(st));
-    return launched();
-}
-
-template <typename T>
-int gs_t(const T *so, int ns, const T *qf, T *q, const T *sor, int ii, int jj, int ud, void *st) {
-    if (!so || !qf || !q || !sor || !ext_ok(ii, jj) || !sten_ok(ns) || !dir_ok(ud))
-        return CDR_ERR_ARG;
-    k2::relax_gs<T>(so, ns, qf, q, sor, ii, jj, ud, S(st));
-    return launched();
-}
-
-template <typename T, int DIR>
-int lines_t(const T *so, int ns, const T *qf, T *q, const T *sor, int ii, int jj, int ud, void *st) {
-    if (!so || !qf || !q || !sor || !ext_ok(ii, jj) || !sten_ok(ns) || !dir_ok(ud))
-        return CDR_ERR_ARG;
-    if (DIR == 0)
-        k2::relax_lines_x<T>(so, ns, qf, q, sor, ii, jj, ud, S(st));
-    else
-        k2::relax_lines_y<T>(so, ns, qf, q, sor, ii, jj, ud, S(st));
-    return launched();
-}
-
-template <typename T>
-int restrict_t(const T *ci, const T *q, T *qc, int iif, int jjf, int iic, int jjc, void *st) {
-    if (!ci || !q || !qc || !coarse_ok(iif, jjf, iic, jjc)) return CDR_ERR_ARG;
-    k2::restrict_<T>(ci, q, qc, iif, jjf, iic, jjc, S(st));
-    return launched();
-}
-
-template <typename T>
-int interp_t(T *q, const T *qc, const T *res, const T *so, int ns, const T *ci, int iic, int jjc,
-             int iif, int jjf, void *st) {
-    if (!q || !qc || !res || !so || !ci || !sten_ok(ns) || !coarse_ok(iif, jjf, iic, jjc))
-        return CDR_ERR_ARG;
-    k2::interp_add<T>(q, qc, res, so, ns, ci, iic, jjc, iif, jjf, S(st));
-    return launched();
-}
-
-template <typename T>
-int solve_cg_t(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj, void *st) {
-    if (!q || !qf || !abd || !bbd || !ext_ok(ii, jj)) return CDR_ERR_ARG;
-    k2::solve_cg<T>(q, qf, abd, bbd, ii, jj, S(st));
-    return launched();
-}
+inline bool nb_ok(int nb) { return nb >= 1 && nb <= 65535; }
 
 // runs a factorisation kernel, then reads its device info word (setup-time synchronisation)
 template <typename F>
-int with_info(int *info_dev, void *st, F launch) {
+int with_info(int *info_dev, hipStream_t st, F launch) {
     if (!info_dev) return CDR_ERR_ARG;
-    const int big = 0x7fffffff;
-    if (hipMemcpyAsync(info_dev, &big, sizeof(int), hipMemcpyHostToDevice, S(st)) != hipSuccess)
-        return CDR_ERR_HIP;
+    k2::fill<int>(info_dev, 0, 1, st);  // kernels raise it with atomicMax
     launch();
     if (hipGetLastError() != hipSuccess) return CDR_ERR_HIP;
     int info = 0;
-    if (hipMemcpyAsync(&info, info_dev, sizeof(int), hipMemcpyDeviceToHost, S(st)) != hipSuccess)
+    if (hipMemcpyAsync(&info, info_dev, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
         return CDR_ERR_HIP;
-    if (hipStreamSynchronize(S(st)) != hipSuccess) return CDR_ERR_HIP;
-    return (info != big && info != 0) ? CDR_ERR_NOT_SPD : CDR_OK;
+    if (hipStreamSynchronize(st) != hipSuccess) return CDR_ERR_HIP;
+    return info != 0 ? CDR_ERR_NOT_SPD : CDR_OK;
 }
 
 }  // namespace
+
+namespace cdr {
+namespace api2 {
+
+template <typename T>
+int residual(const T *so, int ns, const T *qf, const T *q, T *res, int ii, int jj, hipStream_t s,
+             int nb) {
+    if (!so || !qf || !q || !res || !ext_ok(ii, jj) || !sten_ok(ns) || !nb_ok(nb))
+        return CDR_ERR_ARG;
+    k2::residual<T>(so, ns, qf, q, res, ii, jj, s, nb);
+    return launched();
+}
+
+int residual_norm(const double *so, int ns, const double *qf, const double *q, double *res, int ii,
+                  int jj, double *work, double *sumsq_dev, hipStream_t s, int nb) {
+    if (!so || !qf || !q || !res || !work || !sumsq_dev || !ext_ok(ii, jj) || !sten_ok(ns) ||
+        !nb_ok(nb))
+        return CDR_ERR_ARG;
+    k2::residual_norm<double>(so, ns, qf, q, res, ii, jj, work, sumsq_dev, s, nb);
+    return launched();
+}
+
+int sumsq(const double *x, int ii, int jj, double *work, double *sumsq_dev, hipStream_t s,
+          int nb) {
+    if (!x || !work || !sumsq_dev || !ext_ok(ii, jj) || !nb_ok(nb)) return CDR_ERR_ARG;
+    k2::sumsq<double>(x, ii, jj, work, sumsq_dev, s, nb);
+    return launched();
+}
+
+int setup_recip(const double *so, double *sor, int ii, int jj, int ns, hipStream_t s, int nb) {
+    if (!so || !sor || !ext_ok(ii, jj) || !sten_ok(ns) || !nb_ok(nb)) return CDR_ERR_ARG;
+    k2::setup_recip(so, ns, sor, ii, jj, s, nb);
+    return launched();
+}
+
+template <typename T>
+int relax_gs(const T *so, int ns, const T *qf, T *q, const T *sor, int ii, int jj, int ud,
+             hipStream_t s, int nb) {
+    if (!so || !qf || !q || !sor || !ext_ok(ii, jj) || !sten_ok(ns) || !dir_ok(ud) || !nb_ok(nb))
+        return CDR_ERR_ARG;
+    k2::relax_gs<T>(so, ns, qf, q, sor, ii, jj, ud, s, nb);
+    return launched();
+}
+
+int setup_lines(int dir, const double *so, double *sor, int ii, int jj, int ns, int *info_dev,
+                hipStream_t s, int nb) {
+    if (!so || !sor || !ext_ok(ii, jj) || !sten_ok(ns) || !nb_ok(nb) || (dir != 0 && dir != 1))
+        return CDR_ERR_ARG;
+    return with_info(info_dev, s, [&] {
+        if (dir == 0)
+            k2::setup_lines_x(so, ns, sor, ii, jj, info_dev, s, nb);
+        else
+            k2::setup_lines_y(so, ns, sor, ii, jj, info_dev, s, nb);
+    });
+}
+
+template <typename T>
+int relax_lines(int dir, const T *so, int ns, const T *qf, T *q, const T *sor, int ii, int jj,
+                int ud, hipStream_t s, int nb) {
+    if (!so || !qf || !q || !sor || !ext_ok(ii, jj) || !sten_ok(ns) || !dir_ok(ud) ||
+        !nb_ok(nb) || (dir != 0 && dir != 1))
+        return CDR_ERR_ARG;
+    if (dir == 0)
+        k2::relax_lines_x<T>(so, ns, qf, q, sor, ii, jj, ud, s, nb);
+    else
+        k2::relax_lines_y<T>(so, ns, qf, q, sor, ii, jj, ud, s, nb);
+    return launched();
+}
+
+template <typename T>
+int restrict_(const T *ci, const T *q, T *qc, int iif, int jjf, int iic, int jjc, hipStream_t s,
+              int nb) {
+    if (!ci || !q || !qc || !coarse_ok(iif, jjf, iic, jjc) || !nb_ok(nb)) return CDR_ERR_ARG;
+    k2::restrict_<T>(ci, q, qc, iif, jjf, iic, jjc, s, nb);
+    return launched();
+}
+
+template <typename T>
+int interp_add(T *q, const T *qc, const T *res, const T *so, int ns, const T *ci, int iic, int jjc,
+               int iif, int jjf, hipStream_t s, int nb) {
+    if (!q || !qc || !res || !so || !ci || !sten_ok(ns) || !coarse_ok(iif, jjf, iic, jjc) ||
+        !nb_ok(nb))
+        return CDR_ERR_ARG;
+    k2::interp_add<T>(q, qc, res, so, ns, ci, iic, jjc, iif, jjf, s, nb);
+    return launched();
+}
+
+int setup_interp(const double *so, int ns, double *ci, int iif, int jjf, int iic, int jjc,
+                 hipStream_t s, int nb) {
+    if (!so || !ci || !sten_ok(ns) || !coarse_ok(iif, jjf, iic, jjc) || !nb_ok(nb))
+        return CDR_ERR_ARG;
+    k2::setup_interp(so, ns, ci, iif, jjf, iic, jjc, s, nb);
+    return launched();
+}
+
+int setup_galerkin(const double *so, int ns, const double *ci, double *soc, int iif, int jjf,
+                   int iic, int jjc, hipStream_t s, int nb) {
+    if (!so || !ci || !soc || !sten_ok(ns) || !coarse_ok(iif, jjf, iic, jjc) || !nb_ok(nb))
+        return CDR_ERR_ARG;
+    k2::setup_galerkin(so, ns, ci, soc, iif, jjf, iic, jjc, s, nb);
+    return launched();
+}
+
+int setup_cg(const double *so, int ns, double *abd, int ii, int jj, int *info_dev, hipStream_t s,
+             int nb) {
+    if (!so || !abd || !ext_ok(ii, jj) || !sten_ok(ns) || !nb_ok(nb)) return CDR_ERR_ARG;
+    return with_info(info_dev, s, [&] { k2::setup_cg(so, ns, abd, ii, jj, info_dev, s, nb); });
+}
+
+template <typename T>
+int solve_cg(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj, hipStream_t s,
+             int nb) {
+    if (!q || !qf || !abd || !bbd || !ext_ok(ii, jj) || !nb_ok(nb)) return CDR_ERR_ARG;
+    k2::solve_cg<T>(q, qf, abd, bbd, ii, jj, s, nb);
+    return launched();
+}
+
+#define CDR_API_INST(T)                                                                            \
+    template int residual<T>(const T *, int, const T *, const T *, T *, int, int, hipStream_t,     \
+                             int);                                                                 \
+    template int relax_gs<T>(const T *, int, const T *, T *, const T *, int, int, int, hipStream_t, \
+                             int);                                                                 \
+    template int relax_lines<T>(int, const T *, int, const T *, T *, const T *, int, int, int,      \
+                                hipStream_t, int);                                                 \
+    template int restrict_<T>(const T *, const T *, T *, int, int, int, int, hipStream_t, int);    \
+    template int interp_add<T>(T *, const T *, const T *, const T *, int, const T *, int, int, int, \
+                               int, hipStream_t, int);                                             \
+    template int solve_cg<T>(T *, const T *, const double *, double *, int, int, hipStream_t, int);
+CDR_API_INST(double)
+CDR_API_INST(float)
+#undef CDR_API_INST
+
+}  // namespace api2
+}  // namespace cdr
+
+namespace api2 = cdr::api2;
 
 extern "C" {
 
@@ -113,122 +201,108 @@ size_t cdr_hip_norm_work2d(int ii, int jj) {
 
 int cdr_hip_residual2d_f64(const double *so, int ns, const double *qf, const double *q,
                            double *res, int ii, int jj, void *st) {
-    return residual_t<double>(so, ns, qf, q, res, ii, jj, st);
+    return api2::residual<double>(so, ns, qf, q, res, ii, jj, S(st));
 }
 int cdr_hip_residual2d_f32(const float *so, int ns, const float *qf, const float *q, float *res,
                            int ii, int jj, void *st) {
-    return residual_t<float>(so, ns, qf, q, res, ii, jj, st);
+    return api2::residual<float>(so, ns, qf, q, res, ii, jj, S(st));
 }
 
 int cdr_hip_residual_norm2d_f64(const double *so, int ns, const double *qf, const double *q,
                                 double *res, int ii, int jj, double *work, double *sumsq_dev,
                                 void *st) {
-    if (!so || !qf || !q || !res || !work || !sumsq_dev || !ext_ok(ii, jj) || !sten_ok(ns))
-        return CDR_ERR_ARG;
-    k2::residual_norm<double>(so, ns, qf, q, res, ii, jj, work, sumsq_dev, S(st));
-    return launched();
+    return api2::residual_norm(so, ns, qf, q, res, ii, jj, work, sumsq_dev, S(st));
 }
 
 int cdr_hip_sumsq2d_f64(const double *x, int ii, int jj, double *work, double *sumsq_dev,
                         void *st) {
-    if (!x || !work || !sumsq_dev || !ext_ok(ii, jj)) return CDR_ERR_ARG;
-    k2::sumsq<double>(x, ii, jj, work, sumsq_dev, S(st));
-    return launched();
+    return api2::sumsq(x, ii, jj, work, sumsq_dev, S(st));
 }
 
 int cdr_hip_setup_recip2d_f64(const double *so, double *sor, int ii, int jj, int ns, void *st) {
-    if (!so || !sor || !ext_ok(ii, jj) || !sten_ok(ns)) return CDR_ERR_ARG;
-    k2::setup_recip(so, sor, ii, jj, S(st));
-    return launched();
+    return api2::setup_recip(so, sor, ii, jj, ns, S(st));
 }
 
 int cdr_hip_relax_gs2d_f64(const double *so, int ns, const double *qf, double *q,
                            const double *sor, int ii, int jj, int ud, void *st) {
-    return gs_t<double>(so, ns, qf, q, sor, ii, jj, ud, st);
+    return api2::relax_gs<double>(so, ns, qf, q, sor, ii, jj, ud, S(st));
 }
 int cdr_hip_relax_gs2d_f32(const float *so, int ns, const float *qf, float *q, const float *sor,
                            int ii, int jj, int ud, void *st) {
-    return gs_t<float>(so, ns, qf, q, sor, ii, jj, ud, st);
+    return api2::relax_gs<float>(so, ns, qf, q, sor, ii, jj, ud, S(st));
 }
 
 int cdr_hip_setup_lines_x2d_f64(const double *so, double *sor, int ii, int jj, int ns,
                                 int *info_dev, void *st) {
-    if (!so || !sor || !ext_ok(ii, jj) || !sten_ok(ns)) return CDR_ERR_ARG;
-    return with_info(info_dev, st, [&] { k2::setup_lines_x(so, sor, ii, jj, info_dev, S(st)); });
+    return api2::setup_lines(0, so, sor, ii, jj, ns, info_dev, S(st));
 }
 int cdr_hip_setup_lines_y2d_f64(const double *so, double *sor, int ii, int jj, int ns,
                                 int *info_dev, void *st) {
-    if (!so || !sor || !ext_ok(ii, jj) || !sten_ok(ns)) return CDR_ERR_ARG;
-    return with_info(info_dev, st, [&] { k2::setup_lines_y(so, sor, ii, jj, info_dev, S(st)); });
+    return api2::setup_lines(1, so, sor, ii, jj, ns, info_dev, S(st));
 }
 
 int cdr_hip_relax_lines_x2d_f64(const double *so, int ns, const double *qf, double *q,
                                 const double *sor, double *b, int ii, int jj, int ud, void *st) {
     (void)b;
-    return lines_t<double, 0>(so, ns, qf, q, sor, ii, jj, ud, st);
+    return api2::relax_lines<double>(0, so, ns, qf, q, sor, ii, jj, ud, S(st));
 }
 int cdr_hip_relax_lines_y2d_f64(const double *so, int ns, const double *qf, double *q,
                                 const double *sor, double *b, int ii, int jj, int ud, void *st) {
     (void)b;
-    return lines_t<double, 1>(so, ns, qf, q, sor, ii, jj, ud, st);
+    return api2::relax_lines<double>(1, so, ns, qf, q, sor, ii, jj, ud, S(st));
 }
 int cdr_hip_relax_lines_x2d_f32(const float *so, int ns, const float *qf, float *q,
                                 const float *sor, float *b, int ii, int jj, int ud, void *st) {
     (void)b;
-    return lines_t<float, 0>(so, ns, qf, q, sor, ii, jj, ud, st);
+    return api2::relax_lines<float>(0, so, ns, qf, q, sor, ii, jj, ud, S(st));
 }
 int cdr_hip_relax_lines_y2d_f32(const float *so, int ns, const float *qf, float *q,
                                 const float *sor, float *b, int ii, int jj, int ud, void *st) {
     (void)b;
-    return lines_t<float, 1>(so, ns, qf, q, sor, ii, jj, ud, st);
+    return api2::relax_lines<float>(1, so, ns, qf, q, sor, ii, jj, ud, S(st));
 }
 
 int cdr_hip_restrict2d_f64(const double *ci, const double *q, double *qc, int iif, int jjf,
                            int iic, int jjc, void *st) {
-    return restrict_t<double>(ci, q, qc, iif, jjf, iic, jjc, st);
+    return api2::restrict_<double>(ci, q, qc, iif, jjf, iic, jjc, S(st));
 }
 int cdr_hip_restrict2d_f32(const float *ci, const float *q, float *qc, int iif, int jjf, int iic,
                            int jjc, void *st) {
-    return restrict_t<float>(ci, q, qc, iif, jjf, iic, jjc, st);
+    return api2::restrict_<float>(ci, q, qc, iif, jjf, iic, jjc, S(st));
 }
 
 int cdr_hip_interp_add2d_f64(double *q, const double *qc, const double *res, const double *so,
                              int ns, const double *ci, int iic, int jjc, int iif, int jjf,
                              void *st) {
-    return interp_t<double>(q, qc, res, so, ns, ci, iic, jjc, iif, jjf, st);
+    return api2::interp_add<double>(q, qc, res, so, ns, ci, iic, jjc, iif, jjf, S(st));
 }
 int cdr_hip_interp_add2d_f32(float *q, const float *qc, const float *res, const float *so, int ns,
                              const float *ci, int iic, int jjc, int iif, int jjf, void *st) {
-    return interp_t<float>(q, qc, res, so, ns, ci, iic, jjc, iif, jjf, st);
+    return api2::interp_add<float>(q, qc, res, so, ns, ci, iic, jjc, iif, jjf, S(st));
 }
 
 int cdr_hip_setup_interp2d_f64(const double *so, int ns, double *ci, int iif, int jjf, int iic,
                                int jjc, void *st) {
-    if (!so || !ci || !sten_ok(ns) || !coarse_ok(iif, jjf, iic, jjc)) return CDR_ERR_ARG;
-    k2::setup_interp(so, ns, ci, iif, jjf, iic, jjc, S(st));
-    return launched();
+    return api2::setup_interp(so, ns, ci, iif, jjf, iic, jjc, S(st));
 }
 
 int cdr_hip_setup_galerkin2d_f64(const double *so, int ns, const double *ci, double *soc, int iif,
                                  int jjf, int iic, int jjc, void *st) {
-    if (!so || !ci || !soc || !sten_ok(ns) || !coarse_ok(iif, jjf, iic, jjc)) return CDR_ERR_ARG;
-    k2::setup_galerkin(so, ns, ci, soc, iif, jjf, iic, jjc, S(st));
-    return launched();
+    return api2::setup_galerkin(so, ns, ci, soc, iif, jjf, iic, jjc, S(st));
 }
 
 int cdr_hip_setup_cg2d_f64(const double *so, int ns, double *abd, int ii, int jj, int *info_dev,
                            void *st) {
-    if (!so || !abd || !ext_ok(ii, jj) || !sten_ok(ns)) return CDR_ERR_ARG;
-    return with_info(info_dev, st, [&] { k2::setup_cg(so, ns, abd, ii, jj, info_dev, S(st)); });
+    return api2::setup_cg(so, ns, abd, ii, jj, info_dev, S(st));
 }
 
 int cdr_hip_solve_cg2d_f64(double *q, const double *qf, const double *abd, double *bbd, int ii,
                            int jj, void *st) {
-    return solve_cg_t<double>(q, qf, abd, bbd, ii, jj, st);
+    return api2::solve_cg<double>(q, qf, abd, bbd, ii, jj, S(st));
 }
 int cdr_hip_solve_cg2d_f32(float *q, const float *qf, const double *abd, double *bbd, int ii,
                            int jj, void *st) {
-    return solve_cg_t<float>(q, qf, abd, bbd, ii, jj, st);
+    return api2::solve_cg<float>(q, qf, abd, bbd, ii, jj, S(st));
 }
 
 }  // extern "C"

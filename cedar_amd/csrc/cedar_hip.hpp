@@ -84,10 +84,13 @@ enum class relax_dir { x, y };
 struct five_pt { static constexpr int ndirs = 3; };
 struct nine_pt { static constexpr int ndirs = 5; };
 
-// ghost-inclusive extents of a 2D grid quantity
+// ghost-inclusive extents of a 2D grid quantity; nb problems of these extents are stored back
+// to back (a batch), each kernel runs on all of them
 struct extents {
     int ii = 0, jj = 0;
-    std::size_t points() const { return (std::size_t)ii * jj; }
+    int nb = 1;
+    std::size_t points() const { return (std::size_t)ii * jj; }  // one problem
+    std::size_t total() const { return points() * (std::size_t)nb; }
     int nx() const { return ii - 2; }
     int ny() const { return jj - 2; }
 };

@@ -16,12 +16,14 @@
 #include <vector>
 
 #include "bmg2_launch.hpp"
+#include "capi2d_impl.hpp"
 #include "cedar_hip.hpp"
 
 namespace cedar {
 namespace hip {
 
 namespace k2 = cdr::k2;
+namespace api2 = cdr::api2;
 
 namespace {
 
@@ -29,55 +31,39 @@ template <typename T>
 constexpr bool is_f64 = std::is_same<T, double>::value;
 
 // ------------------------------------------------------------------------------------------
-// "hip" implementations: thin forwards to the C ABI
+// "hip" implementations: forwards to the checked entry points behind the C ABI, with the batch
+// count of the operands' extents
 // ------------------------------------------------------------------------------------------
 template <typename T>
 struct hip_residual : kernels::residual<T> {
     void run(const stencil_op<T> &so, const grid_func<T> &x, const grid_func<T> &b,
              grid_func<T> &r) override {
-        if constexpr (is_f64<T>)
-            check(cdr_hip_residual2d_f64(so.p, so.nstncl, b.p, x.p, r.p, so.ext.ii, so.ext.jj, this->stream), "residual");
-        else
-            check(cdr_hip_residual2d_f32(so.p, so.nstncl, b.p, x.p, r.p, so.ext.ii, so.ext.jj, this->stream), "residual");
+        check(api2::residual<T>(so.p, so.nstncl, b.p, x.p, r.p, so.ext.ii, so.ext.jj, this->stream, so.ext.nb), "residual");
     }
 };
 
 template <typename T>
 struct hip_rbgs : kernels::point_relax<T> {
     void setup(const stencil_op<double> &so, relax_stencil<double> &sor) override {
-        check(cdr_hip_setup_recip2d_f64(so.p, sor.p, so.ext.ii, so.ext.jj, so.nstncl, this->stream), "setup_recip");
+        check(api2::setup_recip(so.p, sor.p, so.ext.ii, so.ext.jj, so.nstncl, this->stream, so.ext.nb), "setup_recip");
     }
     void run(const stencil_op<T> &so, grid_func<T> &x, const grid_func<T> &b,
              const relax_stencil<T> &sor, Dir d) override {
-        if constexpr (is_f64<T>)
-            check(cdr_hip_relax_gs2d_f64(so.p, so.nstncl, b.p, x.p, sor.p, so.ext.ii, so.ext.jj, (int)d, this->stream), "relax_gs");
-        else
-            check(cdr_hip_relax_gs2d_f32(so.p, so.nstncl, b.p, x.p, sor.p, so.ext.ii, so.ext.jj, (int)d, this->stream), "relax_gs");
+        check(api2::relax_gs<T>(so.p, so.nstncl, b.p, x.p, sor.p, so.ext.ii, so.ext.jj, (int)d, this->stream, so.ext.nb), "relax_gs");
     }
 };
 
 template <typename T, relax_dir RD>
 struct hip_lines : kernels::line_relax<T, RD> {
+    static constexpr int dir = RD == relax_dir::x ? 0 : 1;
     explicit hip_lines(int *info) : info_(info) {}
     void setup(const stencil_op<double> &so, relax_stencil<double> &sor) override {
-        if (RD == relax_dir::x)
-            check(cdr_hip_setup_lines_x2d_f64(so.p, sor.p, so.ext.ii, so.ext.jj, so.nstncl, info_, this->stream), "setup_lines_x");
-        else
-            check(cdr_hip_setup_lines_y2d_f64(so.p, sor.p, so.ext.ii, so.ext.jj, so.nstncl, info_, this->stream), "setup_lines_y");
+        check(api2::setup_lines(dir, so.p, sor.p, so.ext.ii, so.ext.jj, so.nstncl, info_, this->stream, so.ext.nb), "setup_lines");
     }
     void run(const stencil_op<T> &so, grid_func<T> &x, const grid_func<T> &b,
              const relax_stencil<T> &sor, grid_func<T> &res, Dir d) override {
-        const int ii = so.ext.ii, jj = so.ext.jj;
-        int st;
-        if constexpr (is_f64<T>)
-            st = RD == relax_dir::x
-                     ? cdr_hip_relax_lines_x2d_f64(so.p, so.nstncl, b.p, x.p, sor.p, res.p, ii, jj, (int)d, this->stream)
-                     : cdr_hip_relax_lines_y2d_f64(so.p, so.nstncl, b.p, x.p, sor.p, res.p, ii, jj, (int)d, this->stream);
-        else
-            st = RD == relax_dir::x
-                     ? cdr_hip_relax_lines_x2d_f32(so.p, so.nstncl, b.p, x.p, sor.p, res.p, ii, jj, (int)d, this->stream)
-                     : cdr_hip_relax_lines_y2d_f32(so.p, so.nstncl, b.p, x.p, sor.p, res.p, ii, jj, (int)d, this->stream);
-        check(st, "relax_lines");
+        (void)res;  // the reference's scratch; the GPU sweep keeps its intermediates on chip
+        check(api2::relax_lines<T>(dir, so.p, so.nstncl, b.p, x.p, sor.p, so.ext.ii, so.ext.jj, (int)d, this->stream, so.ext.nb), "relax_lines");
     }
 
 private:
@@ -88,24 +74,21 @@ struct hip_setup_interp : kernels::setup_interp {
     void run(const stencil_op<double> &fop, const stencil_op<double> &cop,
              prolong_op<double> &P) override {
         P.fine_op = &fop;
-        check(cdr_hip_setup_interp2d_f64(fop.p, fop.nstncl, P.p, fop.ext.ii, fop.ext.jj, cop.ext.ii, cop.ext.jj, this->stream), "setup_interp");
+        check(api2::setup_interp(fop.p, fop.nstncl, P.p, fop.ext.ii, fop.ext.jj, cop.ext.ii, cop.ext.jj, this->stream, fop.ext.nb), "setup_interp");
     }
 };
 
 struct hip_galerkin : kernels::coarsen_op {
     void run(const prolong_op<double> &P, const stencil_op<double> &fop,
              stencil_op<double> &cop) override {
-        check(cdr_hip_setup_galerkin2d_f64(fop.p, fop.nstncl, P.p, cop.p, fop.ext.ii, fop.ext.jj, cop.ext.ii, cop.ext.jj, this->stream), "galerkin");
+        check(api2::setup_galerkin(fop.p, fop.nstncl, P.p, cop.p, fop.ext.ii, fop.ext.jj, cop.ext.ii, cop.ext.jj, this->stream, fop.ext.nb), "galerkin");
     }
 };
 
 template <typename T>
 struct hip_restrict : kernels::restriction<T> {
     void run(const prolong_op<T> &R, const grid_func<T> &fine, grid_func<T> &coarse) override {
-        if constexpr (is_f64<T>)
-            check(cdr_hip_restrict2d_f64(R.p, fine.p, coarse.p, fine.ext.ii, fine.ext.jj, coarse.ext.ii, coarse.ext.jj, this->stream), "restrict");
-        else
-            check(cdr_hip_restrict2d_f32(R.p, fine.p, coarse.p, fine.ext.ii, fine.ext.jj, coarse.ext.ii, coarse.ext.jj, this->stream), "restrict");
+        check(api2::restrict_<T>(R.p, fine.p, coarse.p, fine.ext.ii, fine.ext.jj, coarse.ext.ii, coarse.ext.jj, this->stream, fine.ext.nb), "restrict");
     }
 };
 
@@ -114,10 +97,7 @@ struct hip_interp : kernels::interp_add<T> {
     void run(const prolong_op<T> &P, const grid_func<T> &coarse, const grid_func<T> &residual,
              grid_func<T> &fine) override {
         const stencil_op<T> &fop = *P.fine_op;
-        if constexpr (is_f64<T>)
-            check(cdr_hip_interp_add2d_f64(fine.p, coarse.p, residual.p, fop.p, fop.nstncl, P.p, coarse.ext.ii, coarse.ext.jj, fine.ext.ii, fine.ext.jj, this->stream), "interp_add");
-        else
-            check(cdr_hip_interp_add2d_f32(fine.p, coarse.p, residual.p, fop.p, fop.nstncl, P.p, coarse.ext.ii, coarse.ext.jj, fine.ext.ii, fine.ext.jj, this->stream), "interp_add");
+        check(api2::interp_add<T>(fine.p, coarse.p, residual.p, fop.p, fop.nstncl, P.p, coarse.ext.ii, coarse.ext.jj, fine.ext.ii, fine.ext.jj, this->stream, fine.ext.nb), "interp_add");
     }
 };
 
@@ -125,15 +105,12 @@ template <typename T>
 struct hip_solve_cg : kernels::solve_cg<T> {
     explicit hip_solve_cg(int *info) : info_(info) {}
     void setup(const stencil_op<double> &so, grid_func<double> &abd) override {
-        check(cdr_hip_setup_cg2d_f64(so.p, so.nstncl, abd.p, so.ext.ii, so.ext.jj, info_, this->stream), "setup_cg");
+        (void)abd.ext;
+        check(api2::setup_cg(so.p, so.nstncl, abd.p, so.ext.ii, so.ext.jj, info_, this->stream, so.ext.nb), "setup_cg");
     }
     void run(grid_func<T> &x, const grid_func<T> &b, const grid_func<double> &abd,
              double *bbd) override {
-        (void)abd;
-        if constexpr (is_f64<T>)
-            check(cdr_hip_solve_cg2d_f64(x.p, b.p, abd.p, bbd, x.ext.ii, x.ext.jj, this->stream), "solve_cg");
-        else
-            check(cdr_hip_solve_cg2d_f32(x.p, b.p, abd.p, bbd, x.ext.ii, x.ext.jj, this->stream), "solve_cg");
+        check(api2::solve_cg<T>(x.p, b.p, abd.p, bbd, x.ext.ii, x.ext.jj, this->stream, x.ext.nb), "solve_cg");
     }
 
 private:
@@ -176,14 +153,15 @@ struct level_data {
 
 class solver2d {
 public:
-    solver2d(const double *so_dev, int nstncl, int nx, int ny, const cdr_solver_settings &s,
-             hipStream_t stream)
+    // nb problems of the same extents: so_dev holds nb operators back to back
+    solver2d(const double *so_dev, int nstncl, int nx, int ny, int nb,
+             const cdr_solver_settings &s, hipStream_t stream)
         : set_(s), stream_(stream), cur_(stream), km_(stream) {
         check_hip(hipEventCreate(&ev0_), "hipEventCreate");
         check_hip(hipEventCreate(&ev1_), "hipEventCreate");
         info_ = dbuf<int>(1, stream_);
         register_hip_kernels(km_, info_.data());
-        build(so_dev, nstncl, nx, ny);
+        build(so_dev, nstncl, nx, ny, nb);
     }
 
     ~solver2d() {
@@ -193,9 +171,11 @@ public:
         (void)hipEventDestroy(ev0_);
         (void)hipEventDestroy(ev1_);
         if (host_norm_) (void)hipHostFree(host_norm_);
+        if (host_active_) (void)hipHostFree(host_active_);
     }
 
     int nlevels() const { return (int)L64_.size(); }
+    int nbatch() const { return L64_[0].ext.nb; }
     const level_data<double> &level(int l) const { return L64_[l]; }
     double setup_seconds() const { return setup_s_; }
     double solve_seconds() const { return solve_s_; }
@@ -213,30 +193,86 @@ public:
         return ng;
     }
 
-    void vcycle64(const double *b_dev, double *x_dev) {
-        grid_func<double> x{x_dev, L64_[0].ext}, b{const_cast<double *>(b_dev), L64_[0].ext};
-        run_cycle<double>(L64_, x, b);
+    // one V-cycle on x (all problems of the batch), in the precision of T; the FP32 hierarchy
+    // exists only for mixed-precision solvers
+    template <typename T>
+    void vcycle(const T *b_dev, T *x_dev) {
+        auto &L = levels<T>();
+        if (L.empty()) throw error(CDR_ERR_ARG, "no hierarchy in this precision");
+        grid_func<T> x{x_dev, L[0].ext}, b{const_cast<T *>(b_dev), L[0].ext};
+        run_cycle<T>(L, x, b);
     }
 
+    template <typename T>
+    std::vector<level_data<T>> &levels() {
+        if constexpr (std::is_same<T, double>::value)
+            return L64_;
+        else
+            return L32_;
+    }
+
+    // kernels go to `s` from now on (the caller's stream, or a capture stream)
+    void set_stream(hipStream_t s) {
+        km_.set_stream(s);
+        cur_ = s;
+    }
+
+    // Cedar's solve loop (include/cedar/multilevel.h:278-298) for every problem of the batch:
+    // problem b records rho_i in hist[b * maxhist + i] and its cycle count in niter[b], and stops
+    // (keeps its iterate) once rho_i < tol, exactly as if it were solved alone.
     int solve(const double *b_dev, double *x_dev, double *hist, int maxhist, int *niter) {
         const extents e = L64_[0].ext;
+        const int nb = e.nb;
         const stencil_op<double> &A = L64_[0].A;
         check_hip(hipEventRecord(ev0_, stream_), "hipEventRecord");
-        int it = 0, status = CDR_OK;
+        std::vector<double> res0(nb);
+        std::vector<int> it(nb, 0);
+        int status = CDR_OK, nactive = nb;
+        for (int p = 0; p < nb; ++p) host_active_[p] = 1;
+        bool mask_dirty = false;
+        // per-problem bookkeeping after a norm; returns false when every problem has stopped
+        auto record = [&](int i) {
+            for (int p = 0; p < nb; ++p) {
+                if (!host_active_[p]) continue;
+                const double rel = std::sqrt(host_norm_[p]) / res0[p];
+                if (i < maxhist && hist) hist[(std::size_t)p * maxhist + i] = rel;
+                it[p] = i + 1;
+                if (!std::isfinite(rel)) status = CDR_ERR_NONFINITE;
+                if (!std::isfinite(rel) || rel < set_.tol) {
+                    host_active_[p] = 0;
+                    --nactive;
+                    mask_dirty = true;
+                }
+            }
+            return nactive > 0;
+        };
+        auto push_mask = [&] {
+            if (!mask_dirty) return;
+            check_hip(hipMemcpyAsync(active_.data(), host_active_, nb * sizeof(int),
+                                     hipMemcpyHostToDevice, stream_), "copy mask");
+            mask_dirty = false;
+        };
         if (set_.precision == CDR_PREC_F64) {
             grid_func<double> x{x_dev, e}, b{const_cast<double *>(b_dev), e};
-            check(cdr_hip_residual_norm2d_f64(A.p, A.nstncl, b.p, x.p, L64_[0].res.p, e.ii, e.jj,
-                                              work_.data(), sumsq_.data(), stream_), "residual_norm");
-            const double res0 = std::sqrt(read_norm2());
+            check(api2::residual_norm(A.p, A.nstncl, b.p, x.p, L64_[0].res.p, e.ii, e.jj,
+                                      work_.data(), sumsq_.data(), stream_, nb), "residual_norm");
+            read_norms();
+            for (int p = 0; p < nb; ++p) res0[p] = std::sqrt(host_norm_[p]);
             for (int i = 0; i < set_.maxiter; ++i) {
+                const bool masked = nactive < nb;
+                if (masked) {
+                    // problems that have stopped keep their iterate through this cycle
+                    if (xprev_.size() < e.total()) xprev_ = dbuf<double>(e.total(), stream_);
+                    check_hip(hipMemcpyAsync(xprev_.data(), x.p, e.total() * sizeof(double),
+                                             hipMemcpyDeviceToDevice, stream_), "save x");
+                    push_mask();
+                }
                 run_maybe_graph(g64_, x.p, b.p, [&] { run_cycle<double>(L64_, x, b); });
-                check(cdr_hip_residual_norm2d_f64(A.p, A.nstncl, b.p, x.p, L64_[0].res.p, e.ii,
-                                                  e.jj, work_.data(), sumsq_.data(), stream_), "residual_norm");
-                const double rel = std::sqrt(read_norm2()) / res0;
-                if (i < maxhist && hist) hist[i] = rel;
-                it = i + 1;
-                if (!std::isfinite(rel)) { status = CDR_ERR_NONFINITE; break; }
-                if (rel < set_.tol) break;
+                if (masked) k2::keep_inactive(x.p, xprev_.data(), active_.data(), e.points(), nb, stream_);
+                check(api2::residual_norm(A.p, A.nstncl, b.p, x.p, L64_[0].res.p, e.ii, e.jj,
+                                          work_.data(), sumsq_.data(), stream_, nb), "residual_norm");
+                read_norms();
+                if (!record(i)) break;
             }
         } else {
             // FP64 iterative refinement around an FP32 V-cycle:
@@ -244,25 +280,25 @@ public:
             double *xa = xa_.data(), *xb = xb_.data();
             level_data<float> &F = L32_[0];
             k2::residual_ir(A.p, A.nstncl, b_dev, x_dev, nullptr, xa, F.b.p, e.ii, e.jj,
-                            work_.data(), sumsq_.data(), stream_);
+                            work_.data(), sumsq_.data(), stream_, nb);
             check_hip(hipGetLastError(), "residual_ir");
-            const double res0 = std::sqrt(read_norm2());
+            read_norms();
+            for (int p = 0; p < nb; ++p) res0[p] = std::sqrt(host_norm_[p]);
             for (int i = 0; i < set_.maxiter; ++i) {
                 run_maybe_graph(g32_, F.x.p, F.b.p, [&] {
-                    k2::fill<float>(F.x.p, 0.0f, e.points(), cur_);
+                    k2::fill<float>(F.x.p, 0.0f, e.total(), cur_);
                     run_cycle<float>(L32_, F.x, F.b);
                 });
+                push_mask();
                 k2::residual_ir(A.p, A.nstncl, b_dev, xa, F.x.p, xb, F.b.p, e.ii, e.jj,
-                                work_.data(), sumsq_.data(), stream_);
+                                work_.data(), sumsq_.data(), stream_, nb,
+                                nactive < nb ? active_.data() : nullptr);
                 check_hip(hipGetLastError(), "residual_ir");
                 std::swap(xa, xb);
-                const double rel = std::sqrt(read_norm2()) / res0;
-                if (i < maxhist && hist) hist[i] = rel;
-                it = i + 1;
-                if (!std::isfinite(rel)) { status = CDR_ERR_NONFINITE; break; }
-                if (rel < set_.tol) break;
+                read_norms();
+                if (!record(i)) break;
             }
-            check_hip(hipMemcpyAsync(x_dev, xa, e.points() * sizeof(double),
+            check_hip(hipMemcpyAsync(x_dev, xa, e.total() * sizeof(double),
                                      hipMemcpyDeviceToDevice, stream_), "copy x");
         }
         check_hip(hipEventRecord(ev1_, stream_), "hipEventRecord");
@@ -270,7 +306,8 @@ public:
         float ms = 0.f;
         check_hip(hipEventElapsedTime(&ms, ev0_, ev1_), "hipEventElapsedTime");
         solve_s_ = ms * 1e-3;
-        if (niter) *niter = it;
+        if (niter)
+            for (int p = 0; p < nb; ++p) niter[p] = it[p];
         return status;
     }
 
@@ -286,10 +323,11 @@ private:
     } g64_, g32_;
     std::vector<level_data<double>> L64_;
     std::vector<level_data<float>> L32_;
-    dbuf<double> abd_, bbd_, work_, sumsq_, xa_, xb_;
-    dbuf<int> info_;
+    dbuf<double> abd_, bbd_, work_, sumsq_, xa_, xb_, xprev_;
+    dbuf<int> info_, active_;
     grid_func<double> abd_g_;
-    double *host_norm_ = nullptr;
+    double *host_norm_ = nullptr;  // nb squared norms (pinned)
+    int *host_active_ = nullptr;   // nb flags (pinned)
     hipEvent_t ev0_{}, ev1_{};
     double setup_s_ = 0, solve_s_ = 0;
 
@@ -312,8 +350,7 @@ private:
                 check_hip(hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking),
                           "hipStreamCreateWithFlags");
             check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-            km_.set_stream(cap_stream_);
-            cur_ = cap_stream_;
+            set_stream(cap_stream_);
             check_hip(hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeThreadLocal),
                       "hipStreamBeginCapture");
             hipGraph_t graph = nullptr;
@@ -322,13 +359,11 @@ private:
             } catch (...) {
                 (void)hipStreamEndCapture(cap_stream_, &graph);
                 if (graph) (void)hipGraphDestroy(graph);
-                km_.set_stream(stream_);
-                cur_ = stream_;
+                set_stream(stream_);
                 throw;
             }
             check_hip(hipStreamEndCapture(cap_stream_, &graph), "hipStreamEndCapture");
-            km_.set_stream(stream_);
-            cur_ = stream_;
+            set_stream(stream_);
             const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
             check_hip(e, "hipGraphInstantiate");
@@ -338,18 +373,17 @@ private:
         check_hip(hipGraphLaunch(g.exec, stream_), "hipGraphLaunch");
     }
 
-    double read_norm2() {
-        check_hip(hipMemcpyAsync(host_norm_, sumsq_.data(), sizeof(double), hipMemcpyDeviceToHost,
-                                 stream_), "read norm");
+    void read_norms() {
+        check_hip(hipMemcpyAsync(host_norm_, sumsq_.data(), sumsq_.size() * sizeof(double),
+                                 hipMemcpyDeviceToHost, stream_), "read norm");
         check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-        return *host_norm_;
     }
 
     template <typename T>
     static void alloc_level(level_data<T> &L, extents e, int nstncl, bool own_op, bool need_ci,
                             bool need_xb, hipStream_t s) {
         L.ext = e;
-        const std::size_t np = e.points();
+        const std::size_t np = e.total();
         if (own_op) {
             L.so_buf = dbuf<T>(np * (std::size_t)nstncl, s);
             L.A = stencil_op<T>{L.so_buf.data(), e, nstncl};
@@ -372,8 +406,8 @@ private:
         }
     }
 
-    void build(const double *so_dev, int nstncl, int nx, int ny) {
-        if (!so_dev || (nstncl != 3 && nstncl != 5) || nx < 1 || ny < 1)
+    void build(const double *so_dev, int nstncl, int nx, int ny, int nb) {
+        if (!so_dev || (nstncl != 3 && nstncl != 5) || nx < 1 || ny < 1 || nb < 1 || nb > 65535)
             throw error(CDR_ERR_ARG, "invalid fine operator");
         int nl = compute_num_levels(nx, ny, set_.min_coarse);
         if (set_.num_levels > 0) {
@@ -384,7 +418,7 @@ private:
         L64_.resize(nl);
         int cx = nx, cy = ny;
         for (int l = 0; l < nl; ++l) {
-            extents e{cx + 2, cy + 2};
+            extents e{cx + 2, cy + 2, nb};
             if (l == 0) {
                 alloc_level(L64_[0], e, nstncl, false, false, false, stream_);
                 L64_[0].A = stencil_op<double>{const_cast<double *>(so_dev), e, nstncl};
@@ -395,11 +429,13 @@ private:
             cy = (int)((cy - 1) / 2.0 + 1);
         }
         const extents ec = L64_[nl - 1].ext;
-        abd_ = dbuf<double>((std::size_t)ec.ii * ec.nx() * ec.ny(), stream_);
-        bbd_ = dbuf<double>((std::size_t)ec.nx() * ec.ny(), stream_);
-        work_ = dbuf<double>(cdr_hip_norm_work2d(L64_[0].ext.ii, L64_[0].ext.jj), stream_);
-        sumsq_ = dbuf<double>(1, stream_);
-        check_hip(hipHostMalloc(&host_norm_, sizeof(double)), "hipHostMalloc");
+        abd_ = dbuf<double>((std::size_t)ec.ii * ec.nx() * ec.ny() * nb, stream_);
+        bbd_ = dbuf<double>((std::size_t)ec.nx() * ec.ny() * nb, stream_);
+        work_ = dbuf<double>(cdr_hip_norm_work2d(L64_[0].ext.ii, L64_[0].ext.jj) * nb, stream_);
+        sumsq_ = dbuf<double>(nb, stream_);
+        active_ = dbuf<int>(nb, stream_);
+        check_hip(hipHostMalloc(&host_norm_, nb * sizeof(double)), "hipHostMalloc");
+        check_hip(hipHostMalloc(&host_active_, nb * sizeof(int)), "hipHostMalloc");
 
         check_hip(hipEventRecord(ev0_, stream_), "hipEventRecord");
         // setup: interp -> Galerkin -> relaxation per level, then the coarse factorisation
@@ -422,7 +458,7 @@ private:
 
     grid_func<double> &abd_grid() {
         const extents ec = L64_.back().ext;
-        abd_g_ = grid_func<double>{abd_.data(), extents{ec.ii, ec.nx() * ec.ny()}};
+        abd_g_ = grid_func<double>{abd_.data(), extents{ec.ii, ec.nx() * ec.ny(), ec.nb}};
         return abd_g_;
     }
 
@@ -443,13 +479,13 @@ private:
     void build_f32() {
         const int nl = nlevels();
         L32_.resize(nl);
-        xa_ = dbuf<double>(L64_[0].ext.points(), stream_);
-        xb_ = dbuf<double>(L64_[0].ext.points(), stream_);
+        xa_ = dbuf<double>(L64_[0].ext.total(), stream_);
+        xb_ = dbuf<double>(L64_[0].ext.total(), stream_);
         for (int l = 0; l < nl; ++l) {
             auto &D = L64_[l];
             auto &F = L32_[l];
             alloc_level(F, D.ext, D.A.nstncl, true, l > 0, true, stream_);
-            const std::size_t np = D.ext.points();
+            const std::size_t np = D.ext.total();
             k2::convert<double, float>(D.A.p, F.A.p, np * D.A.nstncl, stream_);
             if (l > 0) k2::convert<double, float>(D.P.p, F.P.p, np * 8, stream_);
             for (int k = 0; k < 2; ++k) k2::convert<double, float>(D.SOR[k].p, F.SOR[k].p, np * 2, stream_);
@@ -494,7 +530,7 @@ private:
         smooth<T>(F, x, b, Dir::DOWN);
         km_.run<kernels::residual<T>>(F.A, x, b, F.res);
         km_.run<kernels::restriction<T>>(C.P, F.res, C.b);
-        k2::fill<T>(C.x.p, T(0), C.ext.points(), cur_);
+        k2::fill<T>(C.x.p, T(0), C.ext.total(), cur_);
         if (lvl + 1 == (int)L.size() - 1)
             km_.run<kernels::solve_cg<T>>(C.x, C.b, abd_grid(), bbd_.data());
         else
@@ -546,7 +582,14 @@ void cdr_hip_default_settings(cdr_solver_settings *s) {
 
 int cdr_hip_solver2d_create(const double *so_dev, int nstncl, int nx, int ny,
                             const cdr_solver_settings *s, void *stream, cdr_hip_solver2d **out) {
-    if (!so_dev || !s || !out || nx < 1 || ny < 1 || (nstncl != 3 && nstncl != 5) ||
+    return cdr_hip_solver2d_create_batched(so_dev, nstncl, nx, ny, 1, s, stream, out);
+}
+
+int cdr_hip_solver2d_create_batched(const double *so_dev, int nstncl, int nx, int ny, int nb,
+                                    const cdr_solver_settings *s, void *stream,
+                                    cdr_hip_solver2d **out) {
+    if (!so_dev || !s || !out || nx < 1 || ny < 1 || nb < 1 || nb > 65535 ||
+        (nstncl != 3 && nstncl != 5) ||
         s->relaxation < CDR_RELAX_POINT || s->relaxation > CDR_RELAX_LINE_XY ||
         s->nrelax_pre < 0 || s->nrelax_post < 0 || s->maxiter < 0 || s->min_coarse < 2 ||
         (s->precision != CDR_PREC_F64 && s->precision != CDR_PREC_MIXED) ||
@@ -557,7 +600,7 @@ int cdr_hip_solver2d_create(const double *so_dev, int nstncl, int nx, int ny,
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CDR_ERR_NO_DEVICE;
     return guarded([&] {
         auto h = std::make_unique<cdr_hip_solver2d>();
-        h->impl = std::make_unique<cedar::hip::solver2d>(so_dev, nstncl, nx, ny, *s,
+        h->impl = std::make_unique<cedar::hip::solver2d>(so_dev, nstncl, nx, ny, nb, *s,
                                                          reinterpret_cast<hipStream_t>(stream));
         *out = h.release();
         return (int)CDR_OK;
@@ -573,13 +616,17 @@ int cdr_hip_solver2d_solve(cdr_hip_solver2d *slv, const double *b_dev, double *x
 int cdr_hip_solver2d_vcycle(cdr_hip_solver2d *slv, const double *b_dev, double *x_dev) {
     if (!slv || !b_dev || !x_dev) return CDR_ERR_ARG;
     return guarded([&] {
-        slv->impl->vcycle64(b_dev, x_dev);
+        slv->impl->vcycle<double>(b_dev, x_dev);
         return (int)CDR_OK;
     });
 }
 
 int cdr_hip_solver2d_nlevels(const cdr_hip_solver2d *slv) {
     return slv ? slv->impl->nlevels() : 0;
+}
+
+int cdr_hip_solver2d_nbatch(const cdr_hip_solver2d *slv) {
+    return slv ? slv->impl->nbatch() : 0;
 }
 
 int cdr_hip_solver2d_level(const cdr_hip_solver2d *slv, int lvl, int *nx, int *ny,

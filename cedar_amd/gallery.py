@@ -160,3 +160,139 @@ def readme_problem(n=400):
     b[1:n + 1, 1:n + 1] = (rhs * (h * h))[1:n + 1, 1:n + 1]
     exact = np.sin(2 * np.pi * X) * np.sin(2 * np.pi * Y)
     return so, b, exact
+
+
+# ---------------------------------------------------------------------------------------------
+# 3D: a grid function is (nz+2, ny+2, nx+2), a stencil (4, ...) {p, pw, ps, b} (7-point) or
+# (14, ...) {p, pw, ps, b, psw, pnw, bw, bnw, bn, bne, be, bse, bs, bsw} (27-point)
+# ---------------------------------------------------------------------------------------------
+P3, PW3, PS3, B3 = 0, 1, 2, 3
+
+
+def diag_diffusion3(nx, ny, nz, dx=1.0, dy=1.0, dz=1.0):
+    """7-point -dx u_xx - dy u_yy - dz u_zz scaled by h^3 / h^2 (src/3d/gallery.cc:55-101)."""
+    so = np.zeros((4, nz + 2, ny + 2, nx + 2))
+    hx, hy, hz = 1.0 / (nx + 1), 1.0 / (ny + 1), 1.0 / (nz + 1)
+    xh, yh, zh = hy * hz / hx, hx * hz / hy, hx * hy / hz
+    so[PS3, 1:nz + 1, 2:ny + 1, 1:nx + 1] = dy * yh
+    so[PW3, 1:nz + 1, 1:ny + 1, 2:nx + 1] = dx * xh
+    so[B3, 2:nz + 1, 1:ny + 1, 1:nx + 1] = dz * zh
+    so[P3, 1:nz + 1, 1:ny + 1, 1:nx + 1] = 2.0 * dx * xh + 2.0 * dy * yh + 2.0 * dz * zh
+    return so
+
+
+def poisson3(nx, ny, nz):
+    """7-point Poisson (src/3d/gallery.cc:7-52)."""
+    so = np.zeros((4, nz + 2, ny + 2, nx + 2))
+    hx, hy, hz = 1.0 / (nx + 1), 1.0 / (ny + 1), 1.0 / (nz + 1)
+    xh, yh, zh = hy * hz / hx, hx * hz / hy, hx * hy / hz
+    so[PS3, 1:nz + 1, 2:ny + 1, 1:nx + 1] = 1.0 * yh
+    so[PW3, 1:nz + 1, 1:ny + 1, 2:nx + 1] = 1.0 * xh
+    so[B3, 2:nz + 1, 1:ny + 1, 1:nx + 1] = 1.0 * zh
+    so[P3, 1:nz + 1, 1:ny + 1, 1:nx + 1] = 2.0 * xh + 2.0 * yh + 2.0 * zh
+    return so
+
+
+def fe3(nx, ny, nz):
+    """All-ones 27-point stencil, centre 26 (src/3d/gallery.cc:104-187)."""
+    so = np.zeros((14, nz + 2, ny + 2, nx + 2))
+    I = slice(1, nx + 1)
+    J = slice(1, ny + 1)
+    K = slice(1, nz + 1)
+    I2, J2, K2 = slice(2, nx + 1), slice(2, ny + 1), slice(2, nz + 1)
+    so[1][K, J, I2] = 1.0                    # pw
+    so[2][K, J2, I] = 1.0                    # ps
+    so[3][K2, J, I] = 1.0                    # b
+    so[5][K, J2, I2] = 1.0                   # pnw
+    so[4][K, J2, I2] = 1.0                   # psw
+    so[6][K2, J, I2] = 1.0                   # bw
+    so[10][K2, J, I2] = 1.0                  # be
+    so[8][K2, J2, I] = 1.0                   # bn
+    so[12][K2, J2, I] = 1.0                  # bs
+    for d in (7, 9, 11, 13):                 # bnw, bne, bse, bsw
+        so[d][K2, J2, I2] = 1.0
+    so[0][K, J, I] = 26.0
+    return so
+
+
+def rhs_uniform3(nx, ny, nz, seed):
+    b = np.zeros((nz + 2, ny + 2, nx + 2))
+    b[1:nz + 1, 1:ny + 1, 1:nx + 1] = uniform_pm1(seed, nx * ny * nz).reshape(nz, ny, nx)
+    return b
+
+
+def block_lognormal3(ncx, ncy, ncz, block, sigma, seed, contrast=None):
+    """Cell coefficients (ncz, ncy, ncx), constant on block^3 cells, log k ~ N(0, sigma^2)
+    (clipped to +-ln(contrast)/2 when contrast is given)."""
+    nb = [-(-n // block) for n in (ncx, ncy, ncz)]
+    g = sigma * normal(seed, nb[0] * nb[1] * nb[2]).reshape(nb[2], nb[1], nb[0])
+    if contrast is not None:
+        lim = 0.5 * np.log(contrast)
+        g = np.clip(g, -lim, lim)
+    kb = np.exp(g)
+    k = np.repeat(np.repeat(np.repeat(kb, block, axis=0), block, axis=1), block, axis=2)
+    return k[:ncz, :ncy, :ncx].copy()
+
+
+def fv7_diffusion(kcell, aniso=(1.0, 1.0, 1.0)):
+    """7-point cell-centred finite-volume -div(K grad u) on n^3 unknowns with Dirichlet ghosts.
+
+    kcell has shape (nz, ny, nx), one coefficient per unknown; K = diag(aniso) * k.  A face
+    between two unknowns gets the harmonic mean of their coefficients times h^2 / h (unit cube,
+    h = 1/(n+1) per direction as in the reference gallery); couplings to the boundary are dropped
+    from the stencil but their Dirichlet faces (coefficient of the cell itself) stay in the
+    diagonal."""
+    nz, ny, nx = kcell.shape
+    hx, hy, hz = 1.0 / (nx + 1), 1.0 / (ny + 1), 1.0 / (nz + 1)
+    xh, yh, zh = hy * hz / hx, hx * hz / hy, hx * hy / hz
+    kx, ky, kz = (a * kcell for a in aniso)
+
+    def harm(a, b):
+        return 2.0 * a * b / (a + b)
+    so = np.zeros((4, nz + 2, ny + 2, nx + 2))
+    fw = xh * harm(kx[:, :, 1:], kx[:, :, :-1])      # face between i-1 and i (i = 1..nx-1)
+    fs = yh * harm(ky[:, 1:, :], ky[:, :-1, :])
+    fb = zh * harm(kz[1:, :, :], kz[:-1, :, :])
+    so[PW3, 1:nz + 1, 1:ny + 1, 2:nx + 1] = fw
+    so[PS3, 1:nz + 1, 2:ny + 1, 1:nx + 1] = fs
+    so[B3, 2:nz + 1, 1:ny + 1, 1:nx + 1] = fb
+    diag = np.zeros((nz, ny, nx))
+    diag[:, :, 1:] += fw
+    diag[:, :, :-1] += fw
+    diag[:, 1:, :] += fs
+    diag[:, :-1, :] += fs
+    diag[1:, :, :] += fb
+    diag[:-1, :, :] += fb
+    # Dirichlet faces on the domain boundary use the cell's own coefficient
+    diag[:, :, 0] += xh * kx[:, :, 0]
+    diag[:, :, -1] += xh * kx[:, :, -1]
+    diag[:, 0, :] += yh * ky[:, 0, :]
+    diag[:, -1, :] += yh * ky[:, -1, :]
+    diag[0, :, :] += zh * kz[0, :, :]
+    diag[-1, :, :] += zh * kz[-1, :, :]
+    so[P3, 1:nz + 1, 1:ny + 1, 1:nx + 1] = diag
+    return so
+
+
+def heterogeneous_7pt(n, seed=3, block=8, sigma=1.5, aniso=(1.0, 1.0, 1e-3)):
+    """n^3 7-point finite-volume problem with anisotropy (kx, ky, kz) = aniso * k, log k ~
+    N(0, sigma^2) on block^3 blocks, harmonic face averaging, f ~ U[-1, 1]."""
+    k = block_lognormal3(n, n, n, block, sigma, seed)
+    so = fv7_diffusion(k, aniso)
+    b = rhs_uniform3(n, n, n, seed + 1000003)
+    return so, b, k
+
+
+def poisson3_sine(n):
+    """poisson3(n, n, n) with b = 12 pi^2 sin(2 pi x) sin(2 pi y) sin(2 pi z) h^3 and the exact
+    continuous solution (test/3d/test_poisson.cc:14-67); returns (so, b, exact)."""
+    so = poisson3(n, n, n)
+    h = 1.0 / (n + 1)
+    t = np.arange(n + 2) * h
+    Z, Y, X = np.meshgrid(t, t, t, indexing="ij")
+    s = np.sin(2 * np.pi * X) * np.sin(2 * np.pi * Y) * np.sin(2 * np.pi * Z)
+    b = np.zeros((n + 2,) * 3)
+    b[1:-1, 1:-1, 1:-1] = (12 * np.pi ** 2 * s * h ** 3)[1:-1, 1:-1, 1:-1]
+    exact = np.zeros_like(b)
+    exact[1:-1, 1:-1, 1:-1] = s[1:-1, 1:-1, 1:-1]
+    return so, b, exact

@@ -176,15 +176,26 @@ void cdr_hip_default_settings(cdr_solver_settings *s);
  * Synchronises.  Replaces cdr2::solver<fsten>(so, conf) (include/cedar/2d/solver.h:55-61). */
 int cdr_hip_solver2d_create(const double *so_dev, int nstncl, int nx, int ny,
                             const cdr_solver_settings *s, void *stream, cdr_hip_solver2d **out);
+/* Batched form (no counterpart in the reference, which solves one problem per solver): nb
+ * independent problems of the same extents, so_dev holding their nb operators back to back
+ * (problem b at so_dev + b * nstncl * (nx+2) * (ny+2)); every kernel launch covers the batch.
+ * b_dev / x_dev of solve and vcycle then hold nb grids back to back. */
+int cdr_hip_solver2d_create_batched(const double *so_dev, int nstncl, int nx, int ny, int nb,
+                                    const cdr_solver_settings *s, void *stream,
+                                    cdr_hip_solver2d **out);
 /* solve(b, x) from the x given (x0 = 0 gives Cedar's solve(b)); rho_i = ||r_i|| / ||r_0|| go to
- * hist (host, up to maxhist), *niter gets the cycles run.  Synchronises once per cycle to read
- * the norm.  Replaces multilevel::solve (include/cedar/multilevel.h:278-298). */
+ * hist (host, up to maxhist per problem, problem b at hist[b * maxhist + i]), niter[b] gets the
+ * cycles run for problem b.  A problem stops, keeping its iterate, once its rho < tol, so each
+ * problem of a batch gets the result of solving it alone.  Synchronises once per cycle to read
+ * the norms.  Replaces multilevel::solve (include/cedar/multilevel.h:278-298). */
 int cdr_hip_solver2d_solve(cdr_hip_solver2d *slv, const double *b_dev, double *x_dev,
                            double *hist, int maxhist, int *niter);
 /* One FP64 V-cycle on (x, b) (include/cedar/cycle/vcycle.h:35-54). */
 int cdr_hip_solver2d_vcycle(cdr_hip_solver2d *slv, const double *b_dev, double *x_dev);
 int cdr_hip_solver2d_nlevels(const cdr_hip_solver2d *slv);
-/* Extents of level lvl (0 = finest) and device pointers to its FP64 data (NULL where absent). */
+int cdr_hip_solver2d_nbatch(const cdr_hip_solver2d *slv);
+/* Extents of level lvl (0 = finest) and device pointers to its FP64 data (NULL where absent);
+ * for a batch, the data of problem 0 followed by the other problems'. */
 int cdr_hip_solver2d_level(const cdr_hip_solver2d *slv, int lvl, int *nx, int *ny,
                            const double **so, const double **ci, const double **sor0,
                            const double **sor1);

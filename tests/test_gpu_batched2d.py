@@ -1,0 +1,78 @@
+"""Batched 2D solves: nb problems of the same extents in one solver, every kernel launch covering
+the batch.  Each problem of a batch must get exactly (bit for bit) the iterate and residual
+history of solving it alone, including problems that converge in fewer cycles than others
+(they stop and keep their iterate while the rest continue)."""
+import numpy as np
+import pytest
+
+from cedar_amd import gallery as G
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def C():
+    import cedar_amd
+    return cedar_amd
+
+
+def dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64).cuda()
+
+
+def problems(n, nb, nine):
+    sos, bs = [], []
+    for k in range(nb):
+        if nine:
+            # contrasts differ per problem, so the problems need different cycle counts
+            so, b, _ = G.heterogeneous_9pt(n, seed=20 + k, block=8, sigma=0.5 + 0.6 * k)
+        else:
+            so, b = G.poisson_uniform_rhs(n, seed=30 + k)
+        sos.append(so)
+        bs.append(b)
+    return np.stack(sos), np.stack(bs)
+
+
+@pytest.mark.parametrize("relax,n,nine,precision", [
+    ("line-xy", 127, True, "f64"), ("line-xy", 127, True, "mixed"),
+    ("point", 96, False, "f64"), ("line-x", 65, True, "mixed")])
+def test_batch_equals_individual_solves(C, relax, n, nine, precision):
+    nb = 4
+    so, b = problems(n, nb, nine)
+    kw = dict(relaxation=relax, nrelax_pre=1, nrelax_post=1, maxiter=40, tol=1e-10,
+              precision=precision)
+    sb = C.Solver(dev(so), **kw)
+    assert sb.nb == nb and sb.batched
+    xb, hb = sb.solve(dev(b))
+    counts = []
+    for k in range(nb):
+        s1 = C.Solver(dev(so[k]), **kw)
+        x1, h1 = s1.solve(dev(b[k]))
+        assert np.array_equal(hb[k], h1), k
+        assert torch.equal(xb[k], x1), k
+        assert h1[-1] < 1e-10
+        counts.append(len(h1))
+    if relax == "line-xy":
+        assert len(set(counts)) > 1, counts  # the stopping mask was exercised
+
+
+def test_batched_setup_matches_individual(C):
+    so, _ = problems(63, 3, True)
+    sb = C.Solver(dev(so), relaxation="line-xy")
+    for k in range(3):
+        s1 = C.Solver(dev(so[k]), relaxation="line-xy")
+        assert s1.nlevels == sb.nlevels
+        for lvl in range(s1.nlevels):
+            for a, c in zip(sb.level_arrays(lvl), s1.level_arrays(lvl)):
+                if c is None:
+                    continue
+                assert np.array_equal(a[k], c), (k, lvl)
+
+
+def test_batched_not_spd_reported(C):
+    so, _ = problems(31, 3, False)
+    so[1, 0, 1:-1, 1:-1] = -1.0
+    with pytest.raises(C.CedarHipError) as e:
+        C.Solver(dev(so), relaxation="line-y")
+    assert e.value.status == 3
