@@ -1,7 +1,7 @@
 """ctypes binding of the C ABI in include/cedar_hip.h (libcedar_hip.so, built in-tree).
 
-Importing this module loads the HIP library and raises if it is missing: there is no CPU
-fallback anywhere in cedar_amd.
+load() (called on first use) loads the HIP library and raises if it is missing: there is no
+CPU fallback anywhere in cedar_amd.
 """
 import ctypes
 import os
@@ -12,7 +12,9 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "cedar_hip.h")
 
 CDR_OK = 0
 CDR_DOWN, CDR_UP = 0, 1
-RELAX = {"point": 0, "line-x": 1, "line-y": 2, "line-xy": 3}
+RELAX = {"point": 0, "line-x": 1, "line-y": 2, "line-xy": 3,
+         "plane-xy": 4, "plane-xz": 5, "plane-yz": 6, "plane-xyz": 7}
+PLANE_FAMILY = {"xy": 0, "xz": 1, "yz": 2}
 PRECISION = {"f64": 0, "mixed": 1}
 
 _i = ctypes.c_int
@@ -25,6 +27,12 @@ class Settings(ctypes.Structure):
     """cdr_solver_settings"""
     _fields_ = [("relaxation", _i), ("nrelax_pre", _i), ("nrelax_post", _i), ("num_levels", _i),
                 ("maxiter", _i), ("tol", _d), ("min_coarse", _i), ("precision", _i), ("graph", _i)]
+
+
+class Settings3(ctypes.Structure):
+    """cdr_solver3_settings"""
+    _fields_ = [("relaxation", _i), ("nrelax_pre", _i), ("nrelax_post", _i), ("num_levels", _i),
+                ("maxiter", _i), ("tol", _d), ("min_coarse", _i), ("plane", Settings)]
 
 
 # name -> (restype, argtypes); pointers to device memory are passed as c_void_p
@@ -69,6 +77,33 @@ PROTOTYPES = {
                                     ctypes.POINTER(_vp)]),
     "cdr_hip_solver2d_times": (_i, [_vp, ctypes.POINTER(_d), ctypes.POINTER(_d)]),
     "cdr_hip_solver2d_destroy": (None, [_vp]),
+    # 3D kernels
+    "cdr_hip_norm_work3d": (_sz, [_i, _i, _i]),
+    "cdr_hip_residual3d_f64": (_i, [_P, _i, _P, _P, _P, _i, _i, _i, _vp]),
+    "cdr_hip_residual3d_f32": (_i, [_P, _i, _P, _P, _P, _i, _i, _i, _vp]),
+    "cdr_hip_residual_norm3d_f64": (_i, [_P, _i, _P, _P, _P, _i, _i, _i, _P, _P, _vp]),
+    "cdr_hip_setup_recip3d_f64": (_i, [_P, _P, _i, _i, _i, _i, _vp]),
+    "cdr_hip_relax_gs3d_f64": (_i, [_P, _i, _P, _P, _P, _i, _i, _i, _i, _vp]),
+    "cdr_hip_relax_gs3d_f32": (_i, [_P, _i, _P, _P, _P, _i, _i, _i, _i, _vp]),
+    "cdr_hip_restrict3d_f64": (_i, [_P, _P, _P, _i, _i, _i, _i, _i, _i, _vp]),
+    "cdr_hip_interp_add3d_f64": (_i, [_P, _P, _P, _P, _i, _P, _i, _i, _i, _i, _i, _i, _vp]),
+    "cdr_hip_setup_interp3d_f64": (_i, [_P, _i, _P, _i, _i, _i, _i, _i, _i, _vp]),
+    "cdr_hip_setup_galerkin3d_f64": (_i, [_P, _i, _P, _P, _i, _i, _i, _i, _i, _i, _vp]),
+    "cdr_hip_cg_bandwidth3d": (_i, [_i, _i]),
+    "cdr_hip_setup_cg3d_f64": (_i, [_P, _i, _P, _i, _i, _i, _i, _P, _vp]),
+    "cdr_hip_solve_cg3d_f64": (_i, [_P, _P, _P, _i, _P, _i, _i, _i, _vp]),
+    # 3D solver
+    "cdr_hip_default_settings3": (None, [ctypes.POINTER(Settings3)]),
+    "cdr_hip_solver3d_create": (_i, [_P, _i, _i, _i, _i, ctypes.POINTER(Settings3), _vp,
+                                     ctypes.POINTER(_vp)]),
+    "cdr_hip_solver3d_solve": (_i, [_vp, _P, _P, ctypes.POINTER(_d), _i, ctypes.POINTER(_i)]),
+    "cdr_hip_solver3d_vcycle": (_i, [_vp, _P, _P]),
+    "cdr_hip_solver3d_relax_planes": (_i, [_vp, _i, _i, _P, _P, _i]),
+    "cdr_hip_solver3d_nlevels": (_i, [_vp]),
+    "cdr_hip_solver3d_level": (_i, [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i),
+                                    ctypes.POINTER(_i), ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+    "cdr_hip_solver3d_times": (_i, [_vp, ctypes.POINTER(_d), ctypes.POINTER(_d)]),
+    "cdr_hip_solver3d_destroy": (None, [_vp]),
 }
 
 

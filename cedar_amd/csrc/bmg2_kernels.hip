@@ -13,6 +13,7 @@
 //
 // All kernels are stencils on HBM-resident arrays (~0.2-0.3 flop/byte): no MFMA, coalesced
 // accesses along i, 64-wide wavefronts, deterministic fixed-order reductions.
+#include "band_chol.hpp"
 #include "bmg2_launch.hpp"
 
 #include <hip/hip_runtime.h>
@@ -1058,41 +1059,7 @@ __global__ __launch_bounds__(256) void setup_cg_k(const double *__restrict__ so,
     }
     __syncthreads();
     __shared__ int bad;
-    if (threadIdx.x == 0) bad = 0;
-    __syncthreads();
-    for (int j = 1; j <= n; ++j) {
-        if (threadIdx.x == 0) {
-            double ajj = AB(kd + 1, j);
-            if (!(ajj > 0.0)) {
-                bad = j;
-            } else {
-                ajj = sqrt(ajj);
-                AB(kd + 1, j) = ajj;
-                const int kn = (kd < n - j) ? kd : n - j;
-                const double r = 1.0 / ajj;
-                for (int t = 1; t <= kn; ++t) AB(kd + 1 - t, j + t) *= r;
-            }
-        }
-        __syncthreads();
-        if (bad) break;
-        const int kn = (kd < n - j) ? kd : n - j;
-        // DSYR upper: M(p,q) += x(p) * (-x(q)), p <= q, for q = 1..kn
-        const int npairs = kn * (kn + 1) / 2;
-        for (int t = threadIdx.x; t < npairs; t += blockDim.x) {
-            int qq = 1, rem = t;
-            while (rem >= qq) {
-                rem -= qq;
-                ++qq;
-            }
-            const int pp = rem + 1;
-            const double xq = AB(kd + 1 - qq, j + qq);
-            if (xq != 0.0) {
-                const double temp = -1.0 * xq;
-                AB(kd + 1 + pp - qq, j + qq) = AB(kd + 1 + pp - qq, j + qq) + AB(kd + 1 - pp, j + pp) * temp;
-            }
-        }
-        __syncthreads();
-    }
+    band_factor_upper(abd, n, kd, ld, bad);
     if (threadIdx.x == 0 && bad) atomicMax(info, bad);
 }
 
@@ -1100,29 +1067,13 @@ template <typename T>
 __global__ void solve_cg_k(T *__restrict__ q, const T *__restrict__ qf, const double *__restrict__ abd,
                            double *__restrict__ x, int ii, int jj) {
     if (threadIdx.x != 0) return;
-    const int nx = ii - 2, ny = jj - 2, n = nx * ny, kd = ii - 1, ld = ii, kp1 = kd + 1;
+    const int nx = ii - 2, ny = jj - 2, n = nx * ny, kd = ii - 1, ld = ii;
     q += (std::size_t)blockIdx.x * ii * jj;  // one workgroup per problem of a batch
     qf += (std::size_t)blockIdx.x * ii * jj;
     abd += (std::size_t)blockIdx.x * ld * n;
     x += (std::size_t)blockIdx.x * n;
-    auto AB = [&](int r, int c) { return abd[(std::size_t)(c - 1) * ld + (r - 1)]; };
     for (int kk = 0; kk < n; ++kk) x[kk] = (double)qf[pidx(kk % nx + 1, kk / nx + 1, ii)];
-    for (int j = 1; j <= n; ++j) {
-        double temp = x[j - 1];
-        const int l = kp1 - j;
-        const int i0 = (j - kd > 1) ? j - kd : 1;
-        for (int i = i0; i <= j - 1; ++i) temp = temp - AB(l + i, j) * x[i - 1];
-        x[j - 1] = temp / AB(kp1, j);
-    }
-    for (int j = n; j >= 1; --j) {
-        if (x[j - 1] != 0.0) {
-            const int l = kp1 - j;
-            x[j - 1] = x[j - 1] / AB(kp1, j);
-            const double temp = x[j - 1];
-            const int i0 = (j - kd > 1) ? j - kd : 1;
-            for (int i = j - 1; i >= i0; --i) x[i - 1] = x[i - 1] - temp * AB(l + i, j);
-        }
-    }
+    band_solve_upper(abd, n, kd, ld, x);
     for (int kk = 0; kk < n; ++kk) q[pidx(kk % nx + 1, kk / nx + 1, ii)] = (T)x[kk];
 }
 
@@ -1394,6 +1345,10 @@ void convert(const Tin *in, Tout *out, std::size_t n, hipStream_t s) {
 template <typename T>
 void fill(T *x, T v, std::size_t n, hipStream_t s) {
     fill_k<T><<<grid1(n), 256, 0, s>>>(x, v, n);
+}
+
+void finalize_sum(const double *partials, std::size_t n, double *out, int nb, hipStream_t s) {
+    finalize_sum_k<<<nb, 1024, 0, s>>>(partials, n, out);
 }
 
 void keep_inactive(double *x, const double *xprev, const int *active, std::size_t per, int nb,

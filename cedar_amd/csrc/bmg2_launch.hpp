@@ -47,6 +47,8 @@ void sumsq(const T *x, int ii, int jj, double *partials, double *out, hipStream_
 void residual_ir(const double *so, int nstncl, const double *qf, const double *x, const float *e,
                  double *xnew, float *r32, int ii, int jj, double *partials, double *sumsq,
                  hipStream_t s, int nb = 1, const int *active = nullptr);
+// out[b] = sum of partials[b*n .. b*n+n-1] in a fixed order (deterministic), b < nb
+void finalize_sum(const double *partials, std::size_t n, double *out, int nb, hipStream_t s);
 // x[k] = xprev[k] over every problem b (per values each) with active[b] == 0
 void keep_inactive(double *x, const double *xprev, const int *active, std::size_t per, int nb,
                    hipStream_t s);

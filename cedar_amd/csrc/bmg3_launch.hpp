@@ -1,0 +1,71 @@
+// Host-side launchers for the 3D BoxMG kernels (bmg3_kernels.hip).
+//
+// Device pointers in the solver's HBM layout: Fortran order, one ghost layer, ghost-inclusive
+// extents ii = nx+2, jj = ny+2, kk = nz+2; a stencil is nstncl contiguous planes of ii*jj*kk
+// values, 4 for the 7-point operator {p, pw, ps, b} and 14 for the 27-point operator
+// {p, pw, ps, b, psw, pnw, bw, bnw, bn, bne, be, bse, bs, bsw}; the prolongation has 26 planes
+// on the coarse grid.  No launcher allocates or synchronises.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace cdr {
+namespace k3 {
+
+enum { XY = 0, XZ = 1, YZ = 2 };  // plane families
+
+// per-block partial sums written by residual_norm for an ii x jj x kk grid
+std::size_t norm_partials(int ii, int jj, int kk);
+
+template <typename T>
+void residual(const T *so, int nstncl, const T *qf, const T *q, T *res, int ii, int jj, int kk,
+              hipStream_t s);
+// residual + deterministic sum of r^2 over the interior into *sumsq (device)
+void residual_norm(const double *so, int nstncl, const double *qf, const double *q, double *res,
+                   int ii, int jj, int kk, double *partials, double *sumsq, hipStream_t s);
+
+void setup_recip(const double *so, int nstncl, double *sor, int ii, int jj, int kk,
+                 hipStream_t s);
+template <typename T>
+void relax_gs(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj, int kk,
+              int updown, hipStream_t s);
+
+template <typename T>
+void restrict_(const T *ci, const T *q, T *qc, int iif, int jjf, int kkf, int iic, int jjc,
+               int kkc, hipStream_t s);
+// q += P qc + res / SO_p (res is read, not modified)
+template <typename T>
+void interp_add(T *q, const T *qc, const T *res, const T *so, int nstncl, const T *ci, int iic,
+                int jjc, int kkc, int iif, int jjf, int kkf, hipStream_t s);
+
+void setup_interp(const double *so, int nstncl, double *ci, int iif, int jjf, int kkf, int iic,
+                  int jjc, int kkc, hipStream_t s);
+void setup_galerkin(const double *so, int nstncl, const double *ci, double *soc, int iif, int jjf,
+                    int kkf, int iic, int jjc, int kkc, hipStream_t s);
+
+// coarsest grid: band ABD(ldab, nx*ny*nz) with kd = cg_bandwidth(ii, jj), ldab >= kd + 1;
+// *info (zero on entry) is raised to the failing column when a pivot is not positive
+int cg_bandwidth(int ii, int jj);
+void setup_cg(const double *so, int nstncl, double *abd, int ldab, int ii, int jj, int kk,
+              int *info, hipStream_t s);
+template <typename T>
+void solve_cg(T *q, const T *qf, const double *abd, int ldab, double *bbd, int ii, int jj, int kk,
+              hipStream_t s);
+
+// Plane relaxation pieces for the planes first, first+2, ... (npl of them) of family dir; the
+// npl 2D arrays of the batch are stored back to back, plane b holding plane first + 2b.
+// Plane extents: XY (ii, jj), XZ (ii, kk), YZ (jj, kk).
+void copy_coeff(int dir, const double *so3, int nstncl3, double *so2, int first, int npl, int ii,
+                int jj, int kk, hipStream_t s);
+template <typename T>
+void copy32(int dir, const T *x, T *x2, int first, int npl, int ii, int jj, int kk, hipStream_t s);
+template <typename T>
+void copy23(int dir, const T *x2, T *x, int first, int npl, int ii, int jj, int kk, hipStream_t s);
+template <typename T>
+void copy_rhs(int dir, const T *so3, int nstncl3, const T *x, const T *b, T *b2, int first,
+              int npl, int ii, int jj, int kk, hipStream_t s);
+
+}  // namespace k3
+}  // namespace cdr

@@ -1,11 +1,5 @@
-// cdr2 solver on the GPU: "hip" kernel implementations, the level hierarchy, setup, V-cycle and
-// the outer solve loop (FP64, or an FP32 V-cycle preconditioning FP64 iterative refinement),
-// plus the solver half of the C ABI.
-//
-// Control flow follows Cedar: multilevel::setup (include/cedar/multilevel.h:243-265),
-// multilevel::solve (include/cedar/multilevel.h:278-298), vcycle::ncycle_helper
-// (include/cedar/cycle/vcycle.h:57-115), smoother sequencing (include/cedar/multilevel.h:165-222),
-// level sizes (include/cedar/2d/solver.h:63-123).
+// "hip" implementations of the 2D kernel interfaces (registered in the kernel manager) and the
+// solver half of the 2D C ABI; the solver itself is in solver2d.hpp.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -18,12 +12,10 @@
 #include "bmg2_launch.hpp"
 #include "capi2d_impl.hpp"
 #include "cedar_hip.hpp"
+#include "solver2d.hpp"
 
 namespace cedar {
 namespace hip {
-
-namespace k2 = cdr::k2;
-namespace api2 = cdr::api2;
 
 namespace {
 
@@ -137,408 +129,6 @@ void register_hip_kernels(kernel_manager &km, int *info) {
     km.add<kernels::solve_cg<double>, hip_solve_cg<double>>("hip", info);
     km.add<kernels::solve_cg<float>, hip_solve_cg<float>>("hip", info);
 }
-
-// ------------------------------------------------------------------------------------------
-// level storage (one precision)
-// ------------------------------------------------------------------------------------------
-template <typename T>
-struct level_data {
-    extents ext;
-    dbuf<T> so_buf, ci_buf, x_buf, b_buf, res_buf, sor_buf[2];
-    stencil_op<T> A;
-    prolong_op<T> P;  // stored on this (coarse) level; unused on level 0
-    grid_func<T> x, b, res;
-    relax_stencil<T> SOR[2];
-};
-
-class solver2d {
-public:
-    // nb problems of the same extents: so_dev holds nb operators back to back
-    solver2d(const double *so_dev, int nstncl, int nx, int ny, int nb,
-             const cdr_solver_settings &s, hipStream_t stream)
-        : set_(s), stream_(stream), cur_(stream), km_(stream) {
-        check_hip(hipEventCreate(&ev0_), "hipEventCreate");
-        check_hip(hipEventCreate(&ev1_), "hipEventCreate");
-        info_ = dbuf<int>(1, stream_);
-        register_hip_kernels(km_, info_.data());
-        build(so_dev, nstncl, nx, ny, nb);
-    }
-
-    ~solver2d() {
-        for (auto *g : {&g64_, &g32_})
-            if (g->exec) (void)hipGraphExecDestroy(g->exec);
-        if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
-        (void)hipEventDestroy(ev0_);
-        (void)hipEventDestroy(ev1_);
-        if (host_norm_) (void)hipHostFree(host_norm_);
-        if (host_active_) (void)hipHostFree(host_active_);
-    }
-
-    int nlevels() const { return (int)L64_.size(); }
-    int nbatch() const { return L64_[0].ext.nb; }
-    const level_data<double> &level(int l) const { return L64_[l]; }
-    double setup_seconds() const { return setup_s_; }
-    double solve_seconds() const { return solve_s_; }
-    kernel_manager &kernels() { return km_; }
-
-    // include/cedar/2d/solver.h:63-79: float coarse sizes, unsigned division
-    static int compute_num_levels(int nx, int ny, int min_coarse) {
-        float nxc, nyc;
-        int ng = 0;
-        do {
-            ng++;
-            nxc = (float)((unsigned)(nx - 1) / (unsigned)(1 << ng) + 1);
-            nyc = (float)((unsigned)(ny - 1) / (unsigned)(1 << ng) + 1);
-        } while ((nxc < nyc ? nxc : nyc) >= (float)min_coarse);
-        return ng;
-    }
-
-    // one V-cycle on x (all problems of the batch), in the precision of T; the FP32 hierarchy
-    // exists only for mixed-precision solvers
-    template <typename T>
-    void vcycle(const T *b_dev, T *x_dev) {
-        auto &L = levels<T>();
-        if (L.empty()) throw error(CDR_ERR_ARG, "no hierarchy in this precision");
-        grid_func<T> x{x_dev, L[0].ext}, b{const_cast<T *>(b_dev), L[0].ext};
-        run_cycle<T>(L, x, b);
-    }
-
-    template <typename T>
-    std::vector<level_data<T>> &levels() {
-        if constexpr (std::is_same<T, double>::value)
-            return L64_;
-        else
-            return L32_;
-    }
-
-    // kernels go to `s` from now on (the caller's stream, or a capture stream)
-    void set_stream(hipStream_t s) {
-        km_.set_stream(s);
-        cur_ = s;
-    }
-
-    // Cedar's solve loop (include/cedar/multilevel.h:278-298) for every problem of the batch:
-    // problem b records rho_i in hist[b * maxhist + i] and its cycle count in niter[b], and stops
-    // (keeps its iterate) once rho_i < tol, exactly as if it were solved alone.
-    int solve(const double *b_dev, double *x_dev, double *hist, int maxhist, int *niter) {
-        const extents e = L64_[0].ext;
-        const int nb = e.nb;
-        const stencil_op<double> &A = L64_[0].A;
-        check_hip(hipEventRecord(ev0_, stream_), "hipEventRecord");
-        std::vector<double> res0(nb);
-        std::vector<int> it(nb, 0);
-        int status = CDR_OK, nactive = nb;
-        for (int p = 0; p < nb; ++p) host_active_[p] = 1;
-        bool mask_dirty = false;
-        // per-problem bookkeeping after a norm; returns false when every problem has stopped
-        auto record = [&](int i) {
-            for (int p = 0; p < nb; ++p) {
-                if (!host_active_[p]) continue;
-                const double rel = std::sqrt(host_norm_[p]) / res0[p];
-                if (i < maxhist && hist) hist[(std::size_t)p * maxhist + i] = rel;
-                it[p] = i + 1;
-                if (!std::isfinite(rel)) status = CDR_ERR_NONFINITE;
-                if (!std::isfinite(rel) || rel < set_.tol) {
-                    host_active_[p] = 0;
-                    --nactive;
-                    mask_dirty = true;
-                }
-            }
-            return nactive > 0;
-        };
-        auto push_mask = [&] {
-            if (!mask_dirty) return;
-            check_hip(hipMemcpyAsync(active_.data(), host_active_, nb * sizeof(int),
-                                     hipMemcpyHostToDevice, stream_), "copy mask");
-            mask_dirty = false;
-        };
-        if (set_.precision == CDR_PREC_F64) {
-            grid_func<double> x{x_dev, e}, b{const_cast<double *>(b_dev), e};
-            check(api2::residual_norm(A.p, A.nstncl, b.p, x.p, L64_[0].res.p, e.ii, e.jj,
-                                      work_.data(), sumsq_.data(), stream_, nb), "residual_norm");
-            read_norms();
-            for (int p = 0; p < nb; ++p) res0[p] = std::sqrt(host_norm_[p]);
-            for (int i = 0; i < set_.maxiter; ++i) {
-                const bool masked = nactive < nb;
-                if (masked) {
-                    // problems that have stopped keep their iterate through this cycle
-                    if (xprev_.size() < e.total()) xprev_ = dbuf<double>(e.total(), stream_);
-                    check_hip(hipMemcpyAsync(xprev_.data(), x.p, e.total() * sizeof(double),
-                                             hipMemcpyDeviceToDevice, stream_), "save x");
-                    push_mask();
-                }
-                run_maybe_graph(g64_, x.p, b.p, [&] { run_cycle<double>(L64_, x, b); });
-                if (masked) k2::keep_inactive(x.p, xprev_.data(), active_.data(), e.points(), nb, stream_);
-                check(api2::residual_norm(A.p, A.nstncl, b.p, x.p, L64_[0].res.p, e.ii, e.jj,
-                                          work_.data(), sumsq_.data(), stream_, nb), "residual_norm");
-                read_norms();
-                if (!record(i)) break;
-            }
-        } else {
-            // FP64 iterative refinement around an FP32 V-cycle:
-            //   r = b - A x (FP64), e = Vcycle32(r), x <- x + e, fused into one residual pass.
-            double *xa = xa_.data(), *xb = xb_.data();
-            level_data<float> &F = L32_[0];
-            k2::residual_ir(A.p, A.nstncl, b_dev, x_dev, nullptr, xa, F.b.p, e.ii, e.jj,
-                            work_.data(), sumsq_.data(), stream_, nb);
-            check_hip(hipGetLastError(), "residual_ir");
-            read_norms();
-            for (int p = 0; p < nb; ++p) res0[p] = std::sqrt(host_norm_[p]);
-            for (int i = 0; i < set_.maxiter; ++i) {
-                run_maybe_graph(g32_, F.x.p, F.b.p, [&] {
-                    k2::fill<float>(F.x.p, 0.0f, e.total(), cur_);
-                    run_cycle<float>(L32_, F.x, F.b);
-                });
-                push_mask();
-                k2::residual_ir(A.p, A.nstncl, b_dev, xa, F.x.p, xb, F.b.p, e.ii, e.jj,
-                                work_.data(), sumsq_.data(), stream_, nb,
-                                nactive < nb ? active_.data() : nullptr);
-                check_hip(hipGetLastError(), "residual_ir");
-                std::swap(xa, xb);
-                read_norms();
-                if (!record(i)) break;
-            }
-            check_hip(hipMemcpyAsync(x_dev, xa, e.total() * sizeof(double),
-                                     hipMemcpyDeviceToDevice, stream_), "copy x");
-        }
-        check_hip(hipEventRecord(ev1_, stream_), "hipEventRecord");
-        check_hip(hipEventSynchronize(ev1_), "hipEventSynchronize");
-        float ms = 0.f;
-        check_hip(hipEventElapsedTime(&ms, ev0_, ev1_), "hipEventElapsedTime");
-        solve_s_ = ms * 1e-3;
-        if (niter)
-            for (int p = 0; p < nb; ++p) niter[p] = it[p];
-        return status;
-    }
-
-private:
-    cdr_solver_settings set_;
-    hipStream_t stream_;
-    hipStream_t cur_;                  // stream the cycle kernels go to (capture stream while capturing)
-    hipStream_t cap_stream_ = nullptr;
-    kernel_manager km_;
-    struct graph_t {
-        hipGraphExec_t exec = nullptr;
-        const void *k1 = nullptr, *k2 = nullptr;  // the x / b buffers the graph was captured on
-    } g64_, g32_;
-    std::vector<level_data<double>> L64_;
-    std::vector<level_data<float>> L32_;
-    dbuf<double> abd_, bbd_, work_, sumsq_, xa_, xb_, xprev_;
-    dbuf<int> info_, active_;
-    grid_func<double> abd_g_;
-    double *host_norm_ = nullptr;  // nb squared norms (pinned)
-    int *host_active_ = nullptr;   // nb flags (pinned)
-    hipEvent_t ev0_{}, ev1_{};
-    double setup_s_ = 0, solve_s_ = 0;
-
-    // Enqueues body() on stream_, either directly or by replaying a graph captured from it.  The
-    // graph is captured on a private stream (capture is not allowed on the null stream) and
-    // re-captured when the buffers it was captured on change.  Captured cycles contain kernel
-    // nodes only (fills are kernels, no memset nodes).
-    template <typename F>
-    void run_maybe_graph(graph_t &g, const void *k1, const void *k2, F body) {
-        if (!set_.graph) {
-            body();
-            return;
-        }
-        if (!g.exec || g.k1 != k1 || g.k2 != k2) {
-            if (g.exec) {
-                check_hip(hipGraphExecDestroy(g.exec), "hipGraphExecDestroy");
-                g.exec = nullptr;
-            }
-            if (!cap_stream_)
-                check_hip(hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking),
-                          "hipStreamCreateWithFlags");
-            check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-            set_stream(cap_stream_);
-            check_hip(hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeThreadLocal),
-                      "hipStreamBeginCapture");
-            hipGraph_t graph = nullptr;
-            try {
-                body();
-            } catch (...) {
-                (void)hipStreamEndCapture(cap_stream_, &graph);
-                if (graph) (void)hipGraphDestroy(graph);
-                set_stream(stream_);
-                throw;
-            }
-            check_hip(hipStreamEndCapture(cap_stream_, &graph), "hipStreamEndCapture");
-            set_stream(stream_);
-            const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            check_hip(e, "hipGraphInstantiate");
-            g.k1 = k1;
-            g.k2 = k2;
-        }
-        check_hip(hipGraphLaunch(g.exec, stream_), "hipGraphLaunch");
-    }
-
-    void read_norms() {
-        check_hip(hipMemcpyAsync(host_norm_, sumsq_.data(), sumsq_.size() * sizeof(double),
-                                 hipMemcpyDeviceToHost, stream_), "read norm");
-        check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-    }
-
-    template <typename T>
-    static void alloc_level(level_data<T> &L, extents e, int nstncl, bool own_op, bool need_ci,
-                            bool need_xb, hipStream_t s) {
-        L.ext = e;
-        const std::size_t np = e.total();
-        if (own_op) {
-            L.so_buf = dbuf<T>(np * (std::size_t)nstncl, s);
-            L.A = stencil_op<T>{L.so_buf.data(), e, nstncl};
-        }
-        if (need_ci) {
-            L.ci_buf = dbuf<T>(np * 8, s);
-            L.P = prolong_op<T>{L.ci_buf.data(), e, nullptr};
-        }
-        if (need_xb) {
-            L.x_buf = dbuf<T>(np, s);
-            L.b_buf = dbuf<T>(np, s);
-            L.x = grid_func<T>{L.x_buf.data(), e};
-            L.b = grid_func<T>{L.b_buf.data(), e};
-        }
-        L.res_buf = dbuf<T>(np, s);
-        L.res = grid_func<T>{L.res_buf.data(), e};
-        for (int k = 0; k < 2; ++k) {
-            L.sor_buf[k] = dbuf<T>(np * 2, s);
-            L.SOR[k] = relax_stencil<T>{L.sor_buf[k].data(), e};
-        }
-    }
-
-    void build(const double *so_dev, int nstncl, int nx, int ny, int nb) {
-        if (!so_dev || (nstncl != 3 && nstncl != 5) || nx < 1 || ny < 1 || nb < 1 || nb > 65535)
-            throw error(CDR_ERR_ARG, "invalid fine operator");
-        int nl = compute_num_levels(nx, ny, set_.min_coarse);
-        if (set_.num_levels > 0) {
-            if (set_.num_levels > nl) throw error(CDR_ERR_LEVELS, "too many levels specified");
-            nl = set_.num_levels;
-        }
-        // setup_space (include/cedar/2d/solver.h:81-123)
-        L64_.resize(nl);
-        int cx = nx, cy = ny;
-        for (int l = 0; l < nl; ++l) {
-            extents e{cx + 2, cy + 2, nb};
-            if (l == 0) {
-                alloc_level(L64_[0], e, nstncl, false, false, false, stream_);
-                L64_[0].A = stencil_op<double>{const_cast<double *>(so_dev), e, nstncl};
-            } else {
-                alloc_level(L64_[l], e, 5, true, true, true, stream_);
-            }
-            cx = (int)((cx - 1) / 2.0 + 1);
-            cy = (int)((cy - 1) / 2.0 + 1);
-        }
-        const extents ec = L64_[nl - 1].ext;
-        abd_ = dbuf<double>((std::size_t)ec.ii * ec.nx() * ec.ny() * nb, stream_);
-        bbd_ = dbuf<double>((std::size_t)ec.nx() * ec.ny() * nb, stream_);
-        work_ = dbuf<double>(cdr_hip_norm_work2d(L64_[0].ext.ii, L64_[0].ext.jj) * nb, stream_);
-        sumsq_ = dbuf<double>(nb, stream_);
-        active_ = dbuf<int>(nb, stream_);
-        check_hip(hipHostMalloc(&host_norm_, nb * sizeof(double)), "hipHostMalloc");
-        check_hip(hipHostMalloc(&host_active_, nb * sizeof(int)), "hipHostMalloc");
-
-        check_hip(hipEventRecord(ev0_, stream_), "hipEventRecord");
-        // setup: interp -> Galerkin -> relaxation per level, then the coarse factorisation
-        for (int l = 0; l < nl - 1; ++l) {
-            auto &F = L64_[l];
-            auto &C = L64_[l + 1];
-            km_.run<kernels::setup_interp>(F.A, C.A, C.P);
-            km_.run<kernels::coarsen_op>(C.P, F.A, C.A);
-            setup_relax(F);
-        }
-        km_.setup<kernels::solve_cg<double>>(L64_[nl - 1].A, abd_grid());
-
-        if (set_.precision == CDR_PREC_MIXED) build_f32();
-        check_hip(hipEventRecord(ev1_, stream_), "hipEventRecord");
-        check_hip(hipEventSynchronize(ev1_), "hipEventSynchronize");
-        float ms = 0.f;
-        check_hip(hipEventElapsedTime(&ms, ev0_, ev1_), "hipEventElapsedTime");
-        setup_s_ = ms * 1e-3;
-    }
-
-    grid_func<double> &abd_grid() {
-        const extents ec = L64_.back().ext;
-        abd_g_ = grid_func<double>{abd_.data(), extents{ec.ii, ec.nx() * ec.ny(), ec.nb}};
-        return abd_g_;
-    }
-
-    void setup_relax(level_data<double> &F) {
-        switch (set_.relaxation) {
-        case CDR_RELAX_POINT: km_.setup<kernels::point_relax<double>>(F.A, F.SOR[0]); break;
-        case CDR_RELAX_LINE_X: km_.setup<kernels::line_relax<double, relax_dir::x>>(F.A, F.SOR[0]); break;
-        case CDR_RELAX_LINE_Y: km_.setup<kernels::line_relax<double, relax_dir::y>>(F.A, F.SOR[0]); break;
-        case CDR_RELAX_LINE_XY:
-            km_.setup<kernels::line_relax<double, relax_dir::x>>(F.A, F.SOR[0]);
-            km_.setup<kernels::line_relax<double, relax_dir::y>>(F.A, F.SOR[1]);
-            break;
-        default: throw error(CDR_ERR_ARG, "invalid relaxation");
-        }
-    }
-
-    // FP32 copy of the hierarchy (operators, interpolation, smoother factors)
-    void build_f32() {
-        const int nl = nlevels();
-        L32_.resize(nl);
-        xa_ = dbuf<double>(L64_[0].ext.total(), stream_);
-        xb_ = dbuf<double>(L64_[0].ext.total(), stream_);
-        for (int l = 0; l < nl; ++l) {
-            auto &D = L64_[l];
-            auto &F = L32_[l];
-            alloc_level(F, D.ext, D.A.nstncl, true, l > 0, true, stream_);
-            const std::size_t np = D.ext.total();
-            k2::convert<double, float>(D.A.p, F.A.p, np * D.A.nstncl, stream_);
-            if (l > 0) k2::convert<double, float>(D.P.p, F.P.p, np * 8, stream_);
-            for (int k = 0; k < 2; ++k) k2::convert<double, float>(D.SOR[k].p, F.SOR[k].p, np * 2, stream_);
-        }
-        for (int l = 1; l < nl; ++l) L32_[l].P.fine_op = &L32_[l - 1].A;
-        check_hip(hipGetLastError(), "convert");
-    }
-
-    template <typename T>
-    void smooth(level_data<T> &L, grid_func<T> &x, const grid_func<T> &b, Dir d) {
-        const int nrep = d == Dir::DOWN ? set_.nrelax_pre : set_.nrelax_post;
-        for (int r = 0; r < nrep; ++r) {
-            switch (set_.relaxation) {
-            case CDR_RELAX_POINT: km_.run<kernels::point_relax<T>>(L.A, x, b, L.SOR[0], d); break;
-            case CDR_RELAX_LINE_X: km_.run<kernels::line_relax<T, relax_dir::x>>(L.A, x, b, L.SOR[0], L.res, d); break;
-            case CDR_RELAX_LINE_Y: km_.run<kernels::line_relax<T, relax_dir::y>>(L.A, x, b, L.SOR[0], L.res, d); break;
-            default:
-                if (d == Dir::DOWN) {
-                    km_.run<kernels::line_relax<T, relax_dir::x>>(L.A, x, b, L.SOR[0], L.res, d);
-                    km_.run<kernels::line_relax<T, relax_dir::y>>(L.A, x, b, L.SOR[1], L.res, d);
-                } else {
-                    km_.run<kernels::line_relax<T, relax_dir::y>>(L.A, x, b, L.SOR[1], L.res, d);
-                    km_.run<kernels::line_relax<T, relax_dir::x>>(L.A, x, b, L.SOR[0], L.res, d);
-                }
-            }
-        }
-    }
-
-    template <typename T>
-    void run_cycle(std::vector<level_data<T>> &L, grid_func<T> &x, const grid_func<T> &b) {
-        if (L.size() == 1) {
-            km_.run<kernels::solve_cg<T>>(x, b, abd_grid(), bbd_.data());
-            return;
-        }
-        ncycle<T>(L, 0, x, b);
-    }
-
-    template <typename T>
-    void ncycle(std::vector<level_data<T>> &L, int lvl, grid_func<T> &x, const grid_func<T> &b) {
-        auto &F = L[lvl];
-        auto &C = L[lvl + 1];
-        smooth<T>(F, x, b, Dir::DOWN);
-        km_.run<kernels::residual<T>>(F.A, x, b, F.res);
-        km_.run<kernels::restriction<T>>(C.P, F.res, C.b);
-        k2::fill<T>(C.x.p, T(0), C.ext.total(), cur_);
-        if (lvl + 1 == (int)L.size() - 1)
-            km_.run<kernels::solve_cg<T>>(C.x, C.b, abd_grid(), bbd_.data());
-        else
-            ncycle<T>(L, lvl + 1, C.x, C.b);
-        km_.run<kernels::interp_add<T>>(C.P, C.x, F.res, x);
-        smooth<T>(F, x, b, Dir::UP);
-    }
-};
 
 }  // namespace hip
 }  // namespace cedar

@@ -1,0 +1,468 @@
+// The cdr3 solver on the GPU (FP64): level hierarchy, setup, V-cycle with point or plane
+// relaxation, the outer solve loop, and the 3D solver half of the C ABI.
+//
+// Plane relaxation (include/cedar/3d/relax_planes.h:35-234, src/3d/relax_planes.cc) relaxes the
+// planes of one family in two colours; within a colour the planes are independent, so each
+// colour is one batched 2D V-cycle over all its planes (solver2d with nb = number of planes):
+// gather x and the plane right-hand sides (copy32, copy_rhs), one cycle, scatter (copy23).
+// Each plane keeps its own 2D hierarchy, built at setup from its own operator (copy_coeff), as
+// the reference's per-plane cdr2 solvers do.
+//
+// Control flow: include/cedar/3d/solver.h:17-128 (levels), include/cedar/multilevel.h:95-298
+// (setup order, smoother sequencing, solve loop), include/cedar/cycle/vcycle.h:57-115.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <memory>
+#include <vector>
+
+#include "bmg2_launch.hpp"
+#include "bmg3_launch.hpp"
+#include "cedar_hip.hpp"
+#include "solver2d.hpp"
+
+namespace cedar {
+namespace hip {
+
+namespace k3 = cdr::k3;
+
+struct ext3 {
+    int ii = 0, jj = 0, kk = 0;
+    std::size_t points() const { return (std::size_t)ii * jj * kk; }
+};
+
+// The planes of one family (xy, xz or yz) of one level: a batched 2D solver per colour.
+class plane_family {
+public:
+    plane_family(const double *so3, int ns3, ext3 e, int dir, const cdr_solver_settings &s2,
+                 hipStream_t st)
+        : dir_(dir), ns3_(ns3), e_(e), stream_(st) {
+        n1_ = dir == k3::YZ ? e.jj : e.ii;
+        n2_ = dir == k3::XY ? e.jj : e.kk;
+        const int nplanes = dir == k3::XY ? e.kk - 2 : dir == k3::XZ ? e.jj - 2 : e.ii - 2;
+        const int ns2 = ns3 == 14 ? 5 : 3;
+        const std::size_t np2 = (std::size_t)n1_ * n2_;
+        maxiter_ = s2.maxiter;
+        for (int c = 0; c < 2; ++c) {
+            const int first = 1 + c;
+            const int npl = first > nplanes ? 0 : (nplanes - first) / 2 + 1;
+            col_[c].npl = npl;
+            if (npl == 0) continue;
+            col_[c].so2 = dbuf<double>(np2 * ns2 * npl, st);
+            col_[c].x2 = dbuf<double>(np2 * npl, st);
+            col_[c].b2 = dbuf<double>(np2 * npl, st);
+            k3::copy_coeff(dir, so3, ns3, col_[c].so2.data(), first, npl, e.ii, e.jj, e.kk, st);
+            check_hip(hipGetLastError(), "copy_coeff");
+            col_[c].slv = std::make_unique<solver2d>(col_[c].so2.data(), ns2, n1_ - 2, n2_ - 2, npl,
+                                                     s2, st);
+        }
+    }
+
+    // relax_planes: DOWN relaxes the odd planes (1, 3, ...) first, UP the even ones first
+    void run(const double *so3, double *x, const double *b, Dir d) {
+        for (int t = 0; t < 2; ++t) {
+            const int c = d == Dir::DOWN ? t : 1 - t;
+            auto &C = col_[c];
+            if (C.npl == 0) continue;
+            const int first = 1 + c;
+            k3::copy32<double>(dir_, x, C.x2.data(), first, C.npl, e_.ii, e_.jj, e_.kk, stream_);
+            k3::copy_rhs<double>(dir_, so3, ns3_, x, b, C.b2.data(), first, C.npl, e_.ii, e_.jj,
+                                 e_.kk, stream_);
+            check_hip(hipGetLastError(), "plane gather");
+            // the nested solve: its residual norms only feed its log and stop test, which with
+            // max-iter cycles and no early stop leave x unchanged
+            for (int it = 0; it < maxiter_; ++it) C.slv->vcycle<double>(C.b2.data(), C.x2.data());
+            k3::copy23<double>(dir_, C.x2.data(), x, first, C.npl, e_.ii, e_.jj, e_.kk, stream_);
+            check_hip(hipGetLastError(), "plane scatter");
+        }
+    }
+
+    int nplanes(int colour) const { return col_[colour].npl; }
+    solver2d *solver(int colour) { return col_[colour].slv.get(); }
+
+private:
+    struct colour {
+        int npl = 0;
+        dbuf<double> so2, x2, b2;
+        std::unique_ptr<solver2d> slv;
+    };
+    int dir_, ns3_;
+    ext3 e_;
+    hipStream_t stream_;
+    int n1_ = 0, n2_ = 0, maxiter_ = 1;
+    colour col_[2];
+};
+
+struct level3 {
+    ext3 e;
+    int ns = 14;
+    dbuf<double> so_buf, ci_buf, x_buf, b_buf, res_buf, sor_buf;
+    const double *so = nullptr;
+    double *ci = nullptr, *x = nullptr, *b = nullptr, *res = nullptr, *sor = nullptr;
+    std::unique_ptr<plane_family> pl[3];
+};
+
+static bool is_plane(int r) { return r >= CDR_RELAX_PLANE_XY && r <= CDR_RELAX_PLANE_XYZ; }
+
+class solver3d {
+public:
+    solver3d(const double *so_dev, int ns, int nx, int ny, int nz, const cdr_solver3_settings &s,
+             hipStream_t stream)
+        : set_(s), stream_(stream) {
+        check_hip(hipEventCreate(&ev0_), "hipEventCreate");
+        check_hip(hipEventCreate(&ev1_), "hipEventCreate");
+        info_ = dbuf<int>(1, stream_);
+        build(so_dev, ns, nx, ny, nz);
+    }
+    ~solver3d() {
+        (void)hipEventDestroy(ev0_);
+        (void)hipEventDestroy(ev1_);
+        if (host_norm_) (void)hipHostFree(host_norm_);
+    }
+
+    int nlevels() const { return (int)L_.size(); }
+    const level3 &level(int l) const { return L_[l]; }
+    double setup_seconds() const { return setup_s_; }
+    double solve_seconds() const { return solve_s_; }
+
+    // include/cedar/3d/solver.h:59-77
+    static int compute_num_levels(int nx, int ny, int nz, int min_coarse) {
+        float nxc, nyc, nzc;
+        int ng = 0;
+        do {
+            ng++;
+            nxc = (float)((unsigned)(nx - 1) / (unsigned)(1 << ng) + 1);
+            nyc = (float)((unsigned)(ny - 1) / (unsigned)(1 << ng) + 1);
+            nzc = (float)((unsigned)(nz - 1) / (unsigned)(1 << ng) + 1);
+            float m = nxc < nyc ? nxc : nyc;
+            if (nzc < m) m = nzc;
+            if (!(m >= (float)min_coarse)) break;
+        } while (true);
+        return ng;
+    }
+
+    void vcycle(const double *b_dev, double *x_dev) {
+        if (L_.size() == 1)
+            coarse_solve(x_dev, b_dev);
+        else
+            ncycle(0, x_dev, b_dev);
+    }
+
+    void relax_planes(int lvl, int dir, double *x, const double *b, Dir d) {
+        if (lvl < 0 || lvl >= nlevels() - 1 || dir < 0 || dir > 2 || !L_[lvl].pl[dir])
+            throw error(CDR_ERR_ARG, "no planes of this family on this level");
+        L_[lvl].pl[dir]->run(L_[lvl].so, x, b, d);
+    }
+
+    int solve(const double *b_dev, double *x_dev, double *hist, int maxhist, int *niter) {
+        const level3 &F = L_[0];
+        const ext3 e = F.e;
+        check_hip(hipEventRecord(ev0_, stream_), "hipEventRecord");
+        k3::residual_norm(F.so, F.ns, b_dev, x_dev, F.res, e.ii, e.jj, e.kk, work_.data(),
+                          sumsq_.data(), stream_);
+        check_hip(hipGetLastError(), "residual_norm");
+        const double res0 = std::sqrt(read_norm2());
+        int it = 0, status = CDR_OK;
+        for (int i = 0; i < set_.maxiter; ++i) {
+            vcycle(b_dev, x_dev);
+            k3::residual_norm(F.so, F.ns, b_dev, x_dev, F.res, e.ii, e.jj, e.kk, work_.data(),
+                              sumsq_.data(), stream_);
+            check_hip(hipGetLastError(), "residual_norm");
+            const double rel = std::sqrt(read_norm2()) / res0;
+            if (hist && i < maxhist) hist[i] = rel;
+            it = i + 1;
+            if (!std::isfinite(rel)) {
+                status = CDR_ERR_NONFINITE;
+                break;
+            }
+            if (rel < set_.tol) break;
+        }
+        check_hip(hipEventRecord(ev1_, stream_), "hipEventRecord");
+        check_hip(hipEventSynchronize(ev1_), "hipEventSynchronize");
+        float ms = 0.f;
+        check_hip(hipEventElapsedTime(&ms, ev0_, ev1_), "hipEventElapsedTime");
+        solve_s_ = ms * 1e-3;
+        if (niter) *niter = it;
+        return status;
+    }
+
+private:
+    cdr_solver3_settings set_;
+    hipStream_t stream_;
+    std::vector<level3> L_;
+    dbuf<double> abd_, bbd_, work_, sumsq_;
+    dbuf<int> info_;
+    int ldab_ = 0;
+    double *host_norm_ = nullptr;
+    hipEvent_t ev0_{}, ev1_{};
+    double setup_s_ = 0, solve_s_ = 0;
+
+    double read_norm2() {
+        check_hip(hipMemcpyAsync(host_norm_, sumsq_.data(), sizeof(double), hipMemcpyDeviceToHost,
+                                 stream_), "read norm");
+        check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+        return *host_norm_;
+    }
+
+    void build(const double *so_dev, int ns, int nx, int ny, int nz) {
+        if (!so_dev || (ns != 4 && ns != 14) || nx < 1 || ny < 1 || nz < 1)
+            throw error(CDR_ERR_ARG, "invalid fine operator");
+        int nl = compute_num_levels(nx, ny, nz, set_.min_coarse);
+        if (set_.num_levels > 0) {
+            if (set_.num_levels > nl) throw error(CDR_ERR_LEVELS, "too many levels specified");
+            nl = set_.num_levels;
+        }
+        // setup_space (include/cedar/3d/solver.h:80-128)
+        L_.resize(nl);
+        int cx = nx, cy = ny, cz = nz;
+        for (int l = 0; l < nl; ++l) {
+            level3 &L = L_[l];
+            L.e = ext3{cx + 2, cy + 2, cz + 2};
+            const std::size_t np = L.e.points();
+            if (l == 0) {
+                L.ns = ns;
+                L.so = so_dev;  // held by reference
+            } else {
+                L.ns = 14;
+                L.so_buf = dbuf<double>(np * 14, stream_);
+                L.ci_buf = dbuf<double>(np * 26, stream_);
+                L.x_buf = dbuf<double>(np, stream_);
+                L.b_buf = dbuf<double>(np, stream_);
+                L.so = L.so_buf.data();
+                L.ci = L.ci_buf.data();
+                L.x = L.x_buf.data();
+                L.b = L.b_buf.data();
+            }
+            L.res_buf = dbuf<double>(np, stream_);
+            L.res = L.res_buf.data();
+            if (set_.relaxation == CDR_RELAX_POINT) {
+                L.sor_buf = dbuf<double>(np * 2, stream_);
+                L.sor = L.sor_buf.data();
+            }
+            cx = (int)((cx - 1) / 2.0 + 1);
+            cy = (int)((cy - 1) / 2.0 + 1);
+            cz = (int)((cz - 1) / 2.0 + 1);
+        }
+        const ext3 ec = L_[nl - 1].e;
+        const int kd = k3::cg_bandwidth(ec.ii, ec.jj);
+        ldab_ = kd + 1;
+        const std::size_t n = (std::size_t)(ec.ii - 2) * (ec.jj - 2) * (ec.kk - 2);
+        abd_ = dbuf<double>((std::size_t)ldab_ * n, stream_);
+        bbd_ = dbuf<double>(n, stream_);
+        work_ = dbuf<double>(k3::norm_partials(L_[0].e.ii, L_[0].e.jj, L_[0].e.kk), stream_);
+        sumsq_ = dbuf<double>(1, stream_);
+        check_hip(hipHostMalloc(&host_norm_, sizeof(double)), "hipHostMalloc");
+
+        check_hip(hipEventRecord(ev0_, stream_), "hipEventRecord");
+        for (int l = 0; l < nl - 1; ++l) {
+            level3 &F = L_[l];
+            level3 &C = L_[l + 1];
+            k3::setup_interp(F.so, F.ns, C.ci, F.e.ii, F.e.jj, F.e.kk, C.e.ii, C.e.jj, C.e.kk, stream_);
+            k3::setup_galerkin(F.so, F.ns, C.ci, C.so_buf.data(), F.e.ii, F.e.jj, F.e.kk, C.e.ii,
+                               C.e.jj, C.e.kk, stream_);
+            check_hip(hipGetLastError(), "setup interp/galerkin");
+            setup_relax(F);
+        }
+        {
+            const level3 &C = L_[nl - 1];
+            k2::fill<int>(info_.data(), 0, 1, stream_);
+            k3::setup_cg(C.so, C.ns, abd_.data(), ldab_, ec.ii, ec.jj, ec.kk, info_.data(), stream_);
+            check_hip(hipGetLastError(), "setup_cg");
+            int info = 0;
+            check_hip(hipMemcpyAsync(&info, info_.data(), sizeof(int), hipMemcpyDeviceToHost, stream_),
+                      "read info");
+            check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+            if (info != 0) throw error(CDR_ERR_NOT_SPD, "coarse-grid Cholesky failed");
+        }
+        check_hip(hipEventRecord(ev1_, stream_), "hipEventRecord");
+        check_hip(hipEventSynchronize(ev1_), "hipEventSynchronize");
+        float ms = 0.f;
+        check_hip(hipEventElapsedTime(&ms, ev0_, ev1_), "hipEventElapsedTime");
+        setup_s_ = ms * 1e-3;
+    }
+
+    // multilevel::setup_relax_helper (include/cedar/multilevel.h:135-157): plane-xyz sets up
+    // xy, xz, yz
+    void setup_relax(level3 &F) {
+        const int r = set_.relaxation;
+        if (r == CDR_RELAX_POINT) {
+            k3::setup_recip(F.so, F.ns, F.sor, F.e.ii, F.e.jj, F.e.kk, stream_);
+            check_hip(hipGetLastError(), "setup_recip");
+            return;
+        }
+        cdr_solver_settings s2 = set_.plane;
+        s2.precision = CDR_PREC_F64;
+        s2.graph = 0;
+        if (r == CDR_RELAX_PLANE_XY || r == CDR_RELAX_PLANE_XYZ)
+            F.pl[k3::XY] = std::make_unique<plane_family>(F.so, F.ns, F.e, k3::XY, s2, stream_);
+        if (r == CDR_RELAX_PLANE_XZ || r == CDR_RELAX_PLANE_XYZ)
+            F.pl[k3::XZ] = std::make_unique<plane_family>(F.so, F.ns, F.e, k3::XZ, s2, stream_);
+        if (r == CDR_RELAX_PLANE_YZ || r == CDR_RELAX_PLANE_XYZ)
+            F.pl[k3::YZ] = std::make_unique<plane_family>(F.so, F.ns, F.e, k3::YZ, s2, stream_);
+    }
+
+    // smoother sequencing (include/cedar/multilevel.h:165-222)
+    void smooth(level3 &L, double *x, const double *b, Dir d) {
+        const int nrep = d == Dir::DOWN ? set_.nrelax_pre : set_.nrelax_post;
+        for (int rep = 0; rep < nrep; ++rep) {
+            switch (set_.relaxation) {
+            case CDR_RELAX_POINT:
+                k3::relax_gs<double>(L.so, L.ns, b, x, L.sor, L.e.ii, L.e.jj, L.e.kk, (int)d, stream_);
+                check_hip(hipGetLastError(), "relax_gs");
+                break;
+            case CDR_RELAX_PLANE_XY: L.pl[k3::XY]->run(L.so, x, b, d); break;
+            case CDR_RELAX_PLANE_XZ: L.pl[k3::XZ]->run(L.so, x, b, d); break;
+            case CDR_RELAX_PLANE_YZ: L.pl[k3::YZ]->run(L.so, x, b, d); break;
+            default:
+                if (d == Dir::DOWN) {
+                    L.pl[k3::XY]->run(L.so, x, b, d);
+                    L.pl[k3::YZ]->run(L.so, x, b, d);
+                    L.pl[k3::XZ]->run(L.so, x, b, d);
+                } else {
+                    L.pl[k3::XZ]->run(L.so, x, b, d);
+                    L.pl[k3::YZ]->run(L.so, x, b, d);
+                    L.pl[k3::XY]->run(L.so, x, b, d);
+                }
+            }
+        }
+    }
+
+    void coarse_solve(double *x, const double *b) {
+        const ext3 ec = L_.back().e;
+        k3::solve_cg<double>(x, b, abd_.data(), ldab_, bbd_.data(), ec.ii, ec.jj, ec.kk, stream_);
+        check_hip(hipGetLastError(), "solve_cg");
+    }
+
+    void ncycle(int l, double *x, const double *b) {
+        level3 &F = L_[l];
+        level3 &C = L_[l + 1];
+        smooth(F, x, b, Dir::DOWN);
+        k3::residual<double>(F.so, F.ns, b, x, F.res, F.e.ii, F.e.jj, F.e.kk, stream_);
+        k3::restrict_<double>(C.ci, F.res, C.b, F.e.ii, F.e.jj, F.e.kk, C.e.ii, C.e.jj, C.e.kk, stream_);
+        k2::fill<double>(C.x, 0.0, C.e.points(), stream_);
+        if (l + 1 == nlevels() - 1)
+            coarse_solve(C.x, C.b);
+        else
+            ncycle(l + 1, C.x, C.b);
+        k3::interp_add<double>(x, C.x, F.res, F.so, F.ns, C.ci, C.e.ii, C.e.jj, C.e.kk, F.e.ii, F.e.jj,
+                               F.e.kk, stream_);
+        check_hip(hipGetLastError(), "cycle");
+        smooth(F, x, b, Dir::UP);
+    }
+};
+
+}  // namespace hip
+}  // namespace cedar
+
+// ==========================================================================================
+// C ABI: 3D solver handle
+// ==========================================================================================
+struct cdr_hip_solver3d {
+    std::unique_ptr<cedar::hip::solver3d> impl;
+};
+
+namespace {
+template <typename F>
+int guarded3(F f) {
+    try {
+        return f();
+    } catch (const cedar::hip::error &e) {
+        std::fprintf(stderr, "cedar_hip: %s\n", e.what());
+        return e.status;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "cedar_hip: %s\n", e.what());
+        return CDR_ERR_HIP;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+void cdr_hip_default_settings3(cdr_solver3_settings *s) {
+    if (!s) return;
+    s->relaxation = CDR_RELAX_POINT;
+    s->nrelax_pre = 2;
+    s->nrelax_post = 1;
+    s->num_levels = -1;
+    s->maxiter = 10;
+    s->tol = 1e-8;
+    s->min_coarse = 3;
+    // plane-config defaults (src/kernel_params.cc:83-89): line-xy, max-iter 1
+    cdr_hip_default_settings(&s->plane);
+    s->plane.relaxation = CDR_RELAX_LINE_XY;
+    s->plane.maxiter = 1;
+    s->plane.graph = 0;
+}
+
+int cdr_hip_solver3d_create(const double *so_dev, int nstncl, int nx, int ny, int nz,
+                            const cdr_solver3_settings *s, void *stream, cdr_hip_solver3d **out) {
+    if (!so_dev || !s || !out || nx < 1 || ny < 1 || nz < 1 || (nstncl != 4 && nstncl != 14) ||
+        !(s->relaxation == CDR_RELAX_POINT || cedar::hip::is_plane(s->relaxation)) ||
+        s->nrelax_pre < 0 || s->nrelax_post < 0 || s->maxiter < 0 || s->min_coarse < 2 ||
+        s->plane.relaxation < CDR_RELAX_POINT || s->plane.relaxation > CDR_RELAX_LINE_XY ||
+        s->plane.maxiter < 0 || s->plane.nrelax_pre < 0 || s->plane.nrelax_post < 0 ||
+        s->plane.min_coarse < 2)
+        return CDR_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CDR_ERR_NO_DEVICE;
+    return guarded3([&] {
+        auto h = std::make_unique<cdr_hip_solver3d>();
+        h->impl = std::make_unique<cedar::hip::solver3d>(so_dev, nstncl, nx, ny, nz, *s,
+                                                         reinterpret_cast<hipStream_t>(stream));
+        *out = h.release();
+        return (int)CDR_OK;
+    });
+}
+
+int cdr_hip_solver3d_solve(cdr_hip_solver3d *slv, const double *b_dev, double *x_dev,
+                           double *hist, int maxhist, int *niter) {
+    if (!slv || !b_dev || !x_dev || maxhist < 0) return CDR_ERR_ARG;
+    return guarded3([&] { return slv->impl->solve(b_dev, x_dev, hist, maxhist, niter); });
+}
+
+int cdr_hip_solver3d_vcycle(cdr_hip_solver3d *slv, const double *b_dev, double *x_dev) {
+    if (!slv || !b_dev || !x_dev) return CDR_ERR_ARG;
+    return guarded3([&] {
+        slv->impl->vcycle(b_dev, x_dev);
+        return (int)CDR_OK;
+    });
+}
+
+int cdr_hip_solver3d_relax_planes(cdr_hip_solver3d *slv, int lvl, int family, double *x_dev,
+                                  const double *b_dev, int updown) {
+    if (!slv || !b_dev || !x_dev || (updown != CDR_DOWN && updown != CDR_UP)) return CDR_ERR_ARG;
+    return guarded3([&] {
+        slv->impl->relax_planes(lvl, family, x_dev, b_dev,
+                                updown == CDR_UP ? cedar::hip::Dir::UP : cedar::hip::Dir::DOWN);
+        return (int)CDR_OK;
+    });
+}
+
+int cdr_hip_solver3d_nlevels(const cdr_hip_solver3d *slv) {
+    return slv ? slv->impl->nlevels() : 0;
+}
+
+int cdr_hip_solver3d_level(const cdr_hip_solver3d *slv, int lvl, int *nx, int *ny, int *nz,
+                           const double **so, const double **ci) {
+    if (!slv || lvl < 0 || lvl >= slv->impl->nlevels()) return CDR_ERR_ARG;
+    const auto &L = slv->impl->level(lvl);
+    if (nx) *nx = L.e.ii - 2;
+    if (ny) *ny = L.e.jj - 2;
+    if (nz) *nz = L.e.kk - 2;
+    if (so) *so = L.so;
+    if (ci) *ci = L.ci;
+    return CDR_OK;
+}
+
+int cdr_hip_solver3d_times(const cdr_hip_solver3d *slv, double *setup_s, double *solve_s) {
+    if (!slv) return CDR_ERR_ARG;
+    if (setup_s) *setup_s = slv->impl->setup_seconds();
+    if (solve_s) *solve_s = slv->impl->solve_seconds();
+    return CDR_OK;
+}
+
+void cdr_hip_solver3d_destroy(cdr_hip_solver3d *slv) { delete slv; }
+
+}  // extern "C"

@@ -147,7 +147,11 @@ enum {
     CDR_RELAX_POINT = 0,
     CDR_RELAX_LINE_X = 1,
     CDR_RELAX_LINE_Y = 2,
-    CDR_RELAX_LINE_XY = 3
+    CDR_RELAX_LINE_XY = 3,
+    CDR_RELAX_PLANE_XY = 4,   /* 3D only */
+    CDR_RELAX_PLANE_XZ = 5,
+    CDR_RELAX_PLANE_YZ = 6,
+    CDR_RELAX_PLANE_XYZ = 7
 };
 enum {
     CDR_PREC_F64 = 0,    /* plain FP64 V-cycles (Cedar's solve) */
@@ -202,6 +206,96 @@ int cdr_hip_solver2d_level(const cdr_hip_solver2d *slv, int lvl, int *nx, int *n
 /* Wall time (s, HIP events) of the last create (setup) and solve. */
 int cdr_hip_solver2d_times(const cdr_hip_solver2d *slv, double *setup_s, double *solve_s);
 void cdr_hip_solver2d_destroy(cdr_hip_solver2d *slv);
+
+/* ---------------------------------------------------------------------------------------
+ * 3D kernels (FP64 unless noted).  Arrays are Fortran order with one ghost layer, extents
+ * ii = nx+2, jj, kk; stencils are 4 planes {p, pw, ps, b} (7-point) or 14 planes {p, pw, ps, b,
+ * psw, pnw, bw, bnw, bn, bne, be, bse, bs, bsw} (27-point) (include/cedar/3d/base_types.h:5-20),
+ * the prolongation 26 planes on the coarse grid (src/3d/ftn/BMG_stencils_f90.h:59-65).
+ * ------------------------------------------------------------------------------------- */
+/* BMG3_SymStd_residual (include/cedar/3d/residual.h:9-13) */
+int cdr_hip_residual3d_f64(const double *so, int nstncl, const double *qf, const double *q,
+                           double *res, int ii, int jj, int kk, void *stream);
+int cdr_hip_residual3d_f32(const float *so, int nstncl, const float *qf, const float *q,
+                           float *res, int ii, int jj, int kk, void *stream);
+/* residual plus the deterministic sum of r^2 over the interior into *sumsq_dev (device);
+ * work holds cdr_hip_norm_work3d(ii, jj, kk) doubles (include/cedar/3d/grid_func.h lp_norm) */
+size_t cdr_hip_norm_work3d(int ii, int jj, int kk);
+int cdr_hip_residual_norm3d_f64(const double *so, int nstncl, const double *qf, const double *q,
+                                double *res, int ii, int jj, int kk, double *work,
+                                double *sumsq_dev, void *stream);
+/* BMG3_SymStd_SETUP_recip / BMG3_SymStd_relax_GS (include/cedar/3d/relax.h:9-16); sor has 2
+ * planes, the reciprocal in the second */
+int cdr_hip_setup_recip3d_f64(const double *so, double *sor, int ii, int jj, int kk,
+                              int nstncl, void *stream);
+int cdr_hip_relax_gs3d_f64(const double *so, int nstncl, const double *qf, double *q,
+                           const double *sor, int ii, int jj, int kk, int updown, void *stream);
+int cdr_hip_relax_gs3d_f32(const float *so, int nstncl, const float *qf, float *q,
+                           const float *sor, int ii, int jj, int kk, int updown, void *stream);
+/* BMG3_SymStd_restrict (src/3d/restrict.cc:7-10) */
+int cdr_hip_restrict3d_f64(const double *ci, const double *q, double *qc, int iif, int jjf,
+                           int kkf, int iic, int jjc, int kkc, void *stream);
+/* BMG3_SymStd_interp_add (src/3d/interp.cc:7-11); res is not modified (the Fortran divides it
+ * by SO_p in place) */
+int cdr_hip_interp_add3d_f64(double *q, const double *qc, const double *res, const double *so,
+                             int nstncl, const double *ci, int iic, int jjc, int kkc, int iif,
+                             int jjf, int kkf, void *stream);
+/* BMG3_SymStd_SETUP_interp_OI (include/cedar/3d/interp.h:11-15); no scratch argument */
+int cdr_hip_setup_interp3d_f64(const double *so, int nstncl, double *ci, int iif, int jjf,
+                               int kkf, int iic, int jjc, int kkc, void *stream);
+/* BMG3_SymStd_SETUP_ITLI07_ex / _ITLI27_ex (include/cedar/3d/coarsen.h:12-16): coarse 14-plane
+ * operator */
+int cdr_hip_setup_galerkin3d_f64(const double *so, int nstncl, const double *ci, double *soc,
+                                 int iif, int jjf, int kkf, int iic, int jjc, int kkc,
+                                 void *stream);
+/* BMG3_SymStd_SETUP_cg_LU / BMG3_SymStd_SOLVE_cg (include/cedar/3d/solve_cg.h:10-11,
+ * src/3d/solve_cg.cc:6-9): band ABD(ldab, nx*ny*nz), kd = cdr_hip_cg_bandwidth3d(ii, jj) =
+ * nx*(ny+1)+1, ldab >= kd+1; bbd holds nx*ny*nz doubles (device) */
+int cdr_hip_cg_bandwidth3d(int ii, int jj);
+int cdr_hip_setup_cg3d_f64(const double *so, int nstncl, double *abd, int ldab, int ii, int jj,
+                           int kk, int *info_dev, void *stream);
+int cdr_hip_solve_cg3d_f64(double *q, const double *qf, const double *abd, int ldab, double *bbd,
+                           int ii, int jj, int kk, void *stream);
+
+/* ---------------------------------------------------------------------------------------
+ * 3D solver (cdr3::solver<seven_pt|xxvii_pt>), FP64, point or plane relaxation
+ * ------------------------------------------------------------------------------------- */
+typedef struct {
+    int relaxation;   /* CDR_RELAX_POINT or CDR_RELAX_PLANE_* (default point) */
+    int nrelax_pre;   /* default 2 */
+    int nrelax_post;  /* default 1 */
+    int num_levels;   /* default -1: computed (include/cedar/3d/solver.h:59-77) */
+    int maxiter;      /* default 10 */
+    double tol;       /* default 1e-8 */
+    int min_coarse;   /* default 3 */
+    cdr_solver_settings plane; /* plane-config of the nested 2D plane solves: default line-xy,
+                                  max-iter 1 (src/kernel_params.cc:83-89); its precision and
+                                  graph fields are ignored, and its max-iter cycles always run
+                                  (no per-plane stop test) */
+} cdr_solver3_settings;
+
+typedef struct cdr_hip_solver3d cdr_hip_solver3d;
+
+void cdr_hip_default_settings3(cdr_solver3_settings *s);
+/* Replaces cdr3::solver<fsten>(so, conf) (include/cedar/3d/solver.h:46-57); the fine operator
+ * is held by reference.  Plane relaxation builds, per level and plane family, one batched 2D
+ * hierarchy per colour (the reference's per-plane cdr2 solvers, include/cedar/3d/relax_planes.h
+ * :166-213). */
+int cdr_hip_solver3d_create(const double *so_dev, int nstncl, int nx, int ny, int nz,
+                            const cdr_solver3_settings *s, void *stream, cdr_hip_solver3d **out);
+/* multilevel::solve (include/cedar/multilevel.h:278-298) from the x given */
+int cdr_hip_solver3d_solve(cdr_hip_solver3d *slv, const double *b_dev, double *x_dev,
+                           double *hist, int maxhist, int *niter);
+int cdr_hip_solver3d_vcycle(cdr_hip_solver3d *slv, const double *b_dev, double *x_dev);
+/* One plane-relaxation sweep (both colours) of family 0 = xy, 1 = xz, 2 = yz at level lvl:
+ * planes<rdir>::run (include/cedar/3d/relax_planes.h:217-228) */
+int cdr_hip_solver3d_relax_planes(cdr_hip_solver3d *slv, int lvl, int family, double *x_dev,
+                                  const double *b_dev, int updown);
+int cdr_hip_solver3d_nlevels(const cdr_hip_solver3d *slv);
+int cdr_hip_solver3d_level(const cdr_hip_solver3d *slv, int lvl, int *nx, int *ny, int *nz,
+                           const double **so, const double **ci);
+int cdr_hip_solver3d_times(const cdr_hip_solver3d *slv, double *setup_s, double *solve_s);
+void cdr_hip_solver3d_destroy(cdr_hip_solver3d *slv);
 
 #ifdef __cplusplus
 }

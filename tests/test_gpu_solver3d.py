@@ -1,0 +1,134 @@
+"""Whole 3D solves on the GPU against the FP64 oracle and the reference's own 3D tests.
+
+Bar: residual history within 1e-12 of the oracle's per cycle (relative residuals), the same
+cycle count, the solution within 1e-10 relative l2 (point relaxation uses only kernels that are
+bitwise equal to the oracle, so there the solution agrees to 1e-12)."""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as sla
+
+from cedar_amd import gallery as G
+from oracle import oracle3 as O3
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def C():
+    import cedar_amd
+    return cedar_amd
+
+
+def dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64).cuda()
+
+
+def rel_l2(a, b):
+    s = (slice(1, -1),) * 3
+    return np.linalg.norm((a - b)[s]) / np.linalg.norm(b[s])
+
+
+def compare(C, so, b, **kw):
+    o = O3.Solver3(so, **kw)
+    xo, ho = o.solve(b)
+    g = C.Solver3(dev(so), **kw)
+    assert g.nlevels == o.nlevels
+    xg, hg = g.solve(dev(b))
+    torch.cuda.synchronize()
+    return xo, ho, xg.cpu().numpy(), hg
+
+
+@pytest.mark.parametrize("n,fe", [(31, False), (24, False), (17, True), (20, True)])
+def test_point_solve_matches_oracle(C, n, fe):
+    if fe:
+        so = G.fe3(n, n, n)
+        b = G.rhs_uniform3(n, n, n, 5)
+    else:
+        so, b, _ = G.poisson3_sine(n)
+    xo, ho, xg, hg = compare(C, so, b, relaxation="point", maxiter=20, tol=1e-12)
+    assert len(hg) == len(ho)
+    assert np.abs(hg - ho).max() <= 1e-12
+    assert rel_l2(xg, xo) <= 1e-12
+
+
+@pytest.mark.parametrize("relax,n", [("plane-xy", 31), ("plane-xy", 24), ("plane-xz", 19),
+                                     ("plane-yz", 19), ("plane-xyz", 15)])
+def test_plane_solve_matches_oracle(C, relax, n):
+    so, b, _ = G.heterogeneous_7pt(n, seed=3, block=4)
+    xo, ho, xg, hg = compare(C, so, b, relaxation=relax, nrelax_pre=1, nrelax_post=1, maxiter=12,
+                             tol=1e-10)
+    assert len(hg) == len(ho)
+    assert np.abs(hg - ho).max() <= 1e-12
+    assert rel_l2(xg, xo) <= 1e-10
+    assert hg[-1] < 1e-10
+
+
+def test_poisson3_reference_test(C):
+    """test/3d/test_poisson.cc:73-104: 200^3 Poisson, default solver; ||r|| < 1e-8 and the
+    error against the continuous solution < 1e-4."""
+    n = 200
+    so, b, exact = G.poisson3_sine(n)
+    s = C.Solver3(dev(so))
+    x, hist = s.solve(dev(b))
+    r = torch.zeros_like(x)
+    nrm = C.cdr3.residual_norm(s.so, dev(b), x, r)
+    assert nrm < 1e-8
+    assert np.abs(x.cpu().numpy() - exact).max() < 1e-4
+
+
+@pytest.mark.parametrize("family", ["xy", "xz", "yz"])
+def test_exact_plane_solves(C, family):
+    """test/3d/test_planes.cc restated: with exact plane solves (point, 50 cycles, tol 1e-30)
+    the planes of the second colour solve their plane systems."""
+    from tests.test_oracle3 import stencil_matrix
+    dims = {"xy": (31, 35, 5), "xz": (31, 5, 35), "yz": (5, 31, 35)}[family]
+    nx, ny, nz = dims
+    so = G.poisson3(nx, ny, nz)
+    kk, jj, ii = nz + 2, ny + 2, nx + 2
+    K, J, I = np.meshgrid(np.arange(kk), np.arange(jj), np.arange(ii), indexing="ij")
+    x = np.zeros((kk, jj, ii))
+    x[1:-1, 1:-1, 1:-1] = (K * nx * ny + J * nx + I)[1:-1, 1:-1, 1:-1].astype(float)
+    b = x.copy()
+    plane = C.cdr3.plane_settings(relaxation="point", maxiter=50, tol=1e-30)
+    g = C.Solver3(dev(so), relaxation="plane-" + family, num_levels=2, plane=plane)
+    xg = dev(x)
+    g.relax_planes(0, family, xg, dev(b), 0)
+    xr = xg.cpu().numpy()
+    hx, hy, hz = 1.0 / (nx + 1), 1.0 / (ny + 1), 1.0 / (nz + 1)
+    xh, yh, zh = hy * hz / hx, hx * hz / hy, hx * hy / hz
+    d = 2 * xh + 2 * yh + 2 * zh
+    st = {"xy": [[0, -yh, 0], [-xh, d, -xh], [0, -yh, 0]],
+          "xz": [[0, -zh, 0], [-xh, d, -xh], [0, -zh, 0]],
+          "yz": [[0, -zh, 0], [-yh, d, -yh], [0, -zh, 0]]}[family]
+    _, _, npl = O3.plane_dims(family, ii, jj, kk)
+    for ipl in range(2, npl + 1, 2):
+        b2 = O3.copy_rhs(so, xr, b, family, ipl)
+        n1, n2 = b2.shape[1] - 2, b2.shape[0] - 2
+        sol = sla.spsolve(stencil_matrix(n1, n2, st).tocsc(), b2[1:-1, 1:-1].ravel())
+        got = O3.copy32(xr, family, ipl)[1:-1, 1:-1].ravel()
+        assert np.abs(got - sol).max() < 1e-8, ipl
+
+
+def test_setup_hierarchy_matches_oracle(C):
+    so, b, _ = G.heterogeneous_7pt(23, seed=5, block=4)
+    o = O3.Solver3(so)
+    g = C.Solver3(dev(so))
+    for lvl in range(1, o.nlevels):
+        oso, oci = o.level(lvl)
+        gso, gci = g.level_arrays(lvl)
+        assert np.array_equal(gci, oci), lvl
+        assert np.array_equal(gso, oso), lvl
+
+
+def test_errors_become_status_codes(C):
+    so = G.poisson3(15, 15, 15)
+    with pytest.raises(C.CedarHipError) as e:
+        C.Solver3(dev(so), num_levels=9)
+    assert e.value.status == 6
+    s = C.Solver3(dev(so))
+    b = G.rhs_uniform3(15, 15, 15, 1)
+    b[3, 3, 3] = np.nan
+    with pytest.raises(C.CedarHipError) as e:
+        s.solve(dev(b))
+    assert e.value.status == 4
