@@ -16,6 +16,7 @@ RELAX = {"point": 0, "line-x": 1, "line-y": 2, "line-xy": 3,
          "plane-xy": 4, "plane-xz": 5, "plane-yz": 6, "plane-xyz": 7}
 PLANE_FAMILY = {"xy": 0, "xz": 1, "yz": 2}
 PRECISION = {"f64": 0, "mixed": 1}
+CYCLE = {"v": 0, "f": 1}
 
 _i = ctypes.c_int
 _d = ctypes.c_double
@@ -26,13 +27,15 @@ _sz = ctypes.c_size_t
 class Settings(ctypes.Structure):
     """cdr_solver_settings"""
     _fields_ = [("relaxation", _i), ("nrelax_pre", _i), ("nrelax_post", _i), ("num_levels", _i),
-                ("maxiter", _i), ("tol", _d), ("min_coarse", _i), ("precision", _i), ("graph", _i)]
+                ("maxiter", _i), ("tol", _d), ("min_coarse", _i), ("precision", _i), ("graph", _i),
+                ("cycle", _i)]
 
 
 class Settings3(ctypes.Structure):
     """cdr_solver3_settings"""
     _fields_ = [("relaxation", _i), ("nrelax_pre", _i), ("nrelax_post", _i), ("num_levels", _i),
-                ("maxiter", _i), ("tol", _d), ("min_coarse", _i), ("plane", Settings)]
+                ("maxiter", _i), ("tol", _d), ("min_coarse", _i), ("plane", Settings),
+                ("cycle", _i)]
 
 
 # name -> (restype, argtypes); pointers to device memory are passed as c_void_p

@@ -200,7 +200,7 @@ class Solver:
     relaxation ('point' | 'line-x' | 'line-y' | 'line-xy'), nrelax_pre=2, nrelax_post=1,
     num_levels=-1, maxiter=10, tol=1e-8, min_coarse=3; precision 'f64' (Cedar's algorithm) or
     'mixed' (FP32 V-cycle preconditioning FP64 iterative refinement); graph=True replays each
-    V-cycle from a captured hipGraph.
+    V-cycle from a captured hipGraph; cycle 'v' or 'f' (solver.cycle.type).
 
     A 4-D operator (nb, nstncl, ny+2, nx+2) makes a batched solver: nb independent problems of
     the same extents solved together (every kernel launch covers the batch); solve() then takes
@@ -209,7 +209,7 @@ class Solver:
     """
 
     def __init__(self, so, relaxation="point", nrelax_pre=2, nrelax_post=1, num_levels=-1,
-                 maxiter=10, tol=1e-8, min_coarse=3, precision="f64", graph=True):
+                 maxiter=10, tol=1e-8, min_coarse=3, precision="f64", graph=True, cycle="v"):
         so = _t(so) if not isinstance(so, np.ndarray) else torch.as_tensor(so).cuda()
         if so.dtype != torch.float64:
             so = so.to(torch.float64)
@@ -220,7 +220,8 @@ class Solver:
         self.nb = nb
         ns, jj, ii = self.so.shape[-3:]
         s = _abi.Settings(_abi.RELAX[relaxation], nrelax_pre, nrelax_post, num_levels, maxiter,
-                          tol, min_coarse, _abi.PRECISION[precision], int(bool(graph)))
+                          tol, min_coarse, _abi.PRECISION[precision], int(bool(graph)),
+                          _abi.CYCLE[cycle])
         h = ctypes.c_void_p()
         _abi.call("cdr_hip_solver2d_create_batched", _p(self.so), ns, ii - 2, jj - 2, nb,
                   ctypes.byref(s), _stream(), ctypes.byref(h))

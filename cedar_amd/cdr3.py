@@ -114,7 +114,7 @@ def plane_settings(relaxation="line-xy", nrelax_pre=2, nrelax_post=1, maxiter=1,
                    min_coarse=3, num_levels=-1):
     """plane-config of the nested 2D plane solves (default: src/kernel_params.cc:83-89)."""
     return _abi.Settings(_abi.RELAX[relaxation], nrelax_pre, nrelax_post, num_levels, maxiter, tol,
-                         min_coarse, 0, 0)
+                         min_coarse, 0, 0, 0)
 
 
 class Solver3:
@@ -126,13 +126,13 @@ class Solver3:
     """
 
     def __init__(self, so, relaxation="point", nrelax_pre=2, nrelax_post=1, num_levels=-1,
-                 maxiter=10, tol=1e-8, min_coarse=3, plane=None):
+                 maxiter=10, tol=1e-8, min_coarse=3, plane=None, cycle="v"):
         so = so if isinstance(so, torch.Tensor) else torch.as_tensor(np.ascontiguousarray(so))
         self.so = so.to(device="cuda", dtype=torch.float64).contiguous()  # held by reference
         ns, kk, jj, ii = self.so.shape
         self.maxiter = maxiter
         s = _abi.Settings3(_abi.RELAX[relaxation], nrelax_pre, nrelax_post, num_levels, maxiter, tol,
-                           min_coarse, plane or plane_settings())
+                           min_coarse, plane or plane_settings(), _abi.CYCLE[cycle])
         h = ctypes.c_void_p()
         _abi.call("cdr_hip_solver3d_create", _p(self.so), ns, ii - 2, jj - 2, kk - 2,
                   ctypes.byref(s), _stream(), ctypes.byref(h))

@@ -407,7 +407,29 @@ private:
             km_.run<kernels::solve_cg<T>>(x, b, abd_grid(), bbd_.data());
             return;
         }
-        ncycle<T>(L, 0, x, b);
+        if (set_.cycle == CDR_CYCLE_F)
+            fmg<T>(L, 0, x, b);
+        else
+            ncycle<T>(L, 0, x, b);
+    }
+
+    // fcycle::fmg_cycle (include/cedar/cycle/fcycle.h:49-83): restrict b itself, recurse, then
+    // x = P x_coarse (x and res zeroed first) and one V-cycle from this level.  x is overwritten,
+    // so every F-cycle of a solve starts from zero, as in the reference.
+    template <typename T>
+    void fmg(std::vector<level_data<T>> &L, int lvl, grid_func<T> &x, const grid_func<T> &b) {
+        if (lvl == (int)L.size() - 1) {
+            km_.run<kernels::solve_cg<T>>(x, b, abd_grid(), bbd_.data());
+            return;
+        }
+        auto &F = L[lvl];
+        auto &C = L[lvl + 1];
+        km_.run<kernels::restriction<T>>(C.P, b, C.b);
+        fmg<T>(L, lvl + 1, C.x, C.b);
+        k2::fill<T>(x.p, T(0), F.ext.total(), cur_);
+        k2::fill<T>(F.res.p, T(0), F.ext.total(), cur_);
+        km_.run<kernels::interp_add<T>>(C.P, C.x, F.res, x);
+        ncycle<T>(L, lvl, x, b);
     }
 
     template <typename T>

@@ -142,9 +142,12 @@ public:
         return ng;
     }
 
+    // one cycle of the configured type (V, or F: include/cedar/cycle/fcycle.h:38-83)
     void vcycle(const double *b_dev, double *x_dev) {
         if (L_.size() == 1)
             coarse_solve(x_dev, b_dev);
+        else if (set_.cycle == CDR_CYCLE_F)
+            fmg(0, x_dev, b_dev);
         else
             ncycle(0, x_dev, b_dev);
     }
@@ -334,6 +337,23 @@ private:
         check_hip(hipGetLastError(), "solve_cg");
     }
 
+    void fmg(int l, double *x, const double *b) {
+        if (l == nlevels() - 1) {
+            coarse_solve(x, b);
+            return;
+        }
+        level3 &F = L_[l];
+        level3 &C = L_[l + 1];
+        k3::restrict_<double>(C.ci, b, C.b, F.e.ii, F.e.jj, F.e.kk, C.e.ii, C.e.jj, C.e.kk, stream_);
+        fmg(l + 1, C.x, C.b);
+        k2::fill<double>(x, 0.0, F.e.points(), stream_);
+        k2::fill<double>(F.res, 0.0, F.e.points(), stream_);
+        k3::interp_add<double>(x, C.x, F.res, F.so, F.ns, C.ci, C.e.ii, C.e.jj, C.e.kk, F.e.ii, F.e.jj,
+                               F.e.kk, stream_);
+        check_hip(hipGetLastError(), "fmg");
+        ncycle(l, x, b);
+    }
+
     void ncycle(int l, double *x, const double *b) {
         level3 &F = L_[l];
         level3 &C = L_[l + 1];
@@ -388,6 +408,7 @@ void cdr_hip_default_settings3(cdr_solver3_settings *s) {
     s->maxiter = 10;
     s->tol = 1e-8;
     s->min_coarse = 3;
+    s->cycle = CDR_CYCLE_V;
     // plane-config defaults (src/kernel_params.cc:83-89): line-xy, max-iter 1
     cdr_hip_default_settings(&s->plane);
     s->plane.relaxation = CDR_RELAX_LINE_XY;
@@ -402,7 +423,7 @@ int cdr_hip_solver3d_create(const double *so_dev, int nstncl, int nx, int ny, in
         s->nrelax_pre < 0 || s->nrelax_post < 0 || s->maxiter < 0 || s->min_coarse < 2 ||
         s->plane.relaxation < CDR_RELAX_POINT || s->plane.relaxation > CDR_RELAX_LINE_XY ||
         s->plane.maxiter < 0 || s->plane.nrelax_pre < 0 || s->plane.nrelax_post < 0 ||
-        s->plane.min_coarse < 2)
+        s->plane.min_coarse < 2 || (s->cycle != CDR_CYCLE_V && s->cycle != CDR_CYCLE_F))
         return CDR_ERR_ARG;
     *out = nullptr;
     int ndev = 0;

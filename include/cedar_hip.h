@@ -154,6 +154,10 @@ enum {
     CDR_RELAX_PLANE_XYZ = 7
 };
 enum {
+    CDR_CYCLE_V = 0,     /* solver.cycle.type "v" (include/cedar/cycle/vcycle.h) */
+    CDR_CYCLE_F = 1      /* solver.cycle.type "f" (include/cedar/cycle/fcycle.h) */
+};
+enum {
     CDR_PREC_F64 = 0,    /* plain FP64 V-cycles (Cedar's solve) */
     CDR_PREC_MIXED = 1   /* FP32 V-cycle preconditioning FP64 iterative refinement */
 };
@@ -168,6 +172,9 @@ typedef struct {
     int min_coarse;   /* solver.min_coarse (default 3) */
     int precision;    /* CDR_PREC_* (default CDR_PREC_F64) */
     int graph;        /* 1: each V-cycle is replayed from a captured hipGraph (default), 0: eager */
+    int cycle;        /* solver.cycle.type: CDR_CYCLE_V (default) or CDR_CYCLE_F.  Like the
+                         reference, an F-cycle recomputes x from b alone (include/cedar/cycle/
+                         fcycle.h:75-81), so successive cycles of a solve repeat */
 } cdr_solver_settings;
 
 typedef struct cdr_hip_solver2d cdr_hip_solver2d;
@@ -194,7 +201,8 @@ int cdr_hip_solver2d_create_batched(const double *so_dev, int nstncl, int nx, in
  * the norms.  Replaces multilevel::solve (include/cedar/multilevel.h:278-298). */
 int cdr_hip_solver2d_solve(cdr_hip_solver2d *slv, const double *b_dev, double *x_dev,
                            double *hist, int maxhist, int *niter);
-/* One FP64 V-cycle on (x, b) (include/cedar/cycle/vcycle.h:35-54). */
+/* One FP64 cycle of the configured type on (x, b) (include/cedar/cycle/vcycle.h:35-54,
+ * include/cedar/cycle/fcycle.h:38-47). */
 int cdr_hip_solver2d_vcycle(cdr_hip_solver2d *slv, const double *b_dev, double *x_dev);
 int cdr_hip_solver2d_nlevels(const cdr_hip_solver2d *slv);
 int cdr_hip_solver2d_nbatch(const cdr_hip_solver2d *slv);
@@ -272,6 +280,7 @@ typedef struct {
                                   max-iter 1 (src/kernel_params.cc:83-89); its precision and
                                   graph fields are ignored, and its max-iter cycles always run
                                   (no per-plane stop test) */
+    int cycle;        /* CDR_CYCLE_V (default) or CDR_CYCLE_F */
 } cdr_solver3_settings;
 
 typedef struct cdr_hip_solver3d cdr_hip_solver3d;

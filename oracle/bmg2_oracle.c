@@ -860,6 +860,34 @@ void orc2_solver_vcycle(orc2_solver *slv, double *x, const double *b)
     }
 }
 
+/* fcycle::fmg_cycle (include/cedar/cycle/fcycle.h:49-83): restrict b itself to the next level,
+ * recurse, then x = P x_coarse (x and res zeroed first) and one V-cycle from this level.  The
+ * finest x is overwritten, so every F-cycle of a solve starts from zero. */
+static void orc2_fmg(orc2_solver *slv, int lvl, double *x, const double *b)
+{
+    if (lvl == slv->nlevels - 1) {
+        orc2_level *L = &slv->lv[lvl];
+        orc2_solve_cg(x, b, slv->abd, slv->bbd, L->ii, L->jj);
+        return;
+    }
+    orc2_level *L = &slv->lv[lvl], *K = &slv->lv[lvl + 1];
+    orc2_restrict(K->ci, b, K->b, L->ii, L->jj, K->ii, K->jj);
+    orc2_fmg(slv, lvl + 1, K->x, K->b);
+    const size_t np = (size_t)L->ii * L->jj;
+    memset(x, 0, np * sizeof(double));
+    memset(L->res, 0, np * sizeof(double));
+    orc2_interp_add(x, K->x, L->res, L->so, L->nstncl, K->ci, K->ii, K->jj, L->ii, L->jj);
+    orc2_ncycle(slv, lvl, x, b);
+}
+
+void orc2_solver_cycle(orc2_solver *slv, double *x, const double *b)
+{
+    if (slv->s.cycle == 1 && slv->nlevels > 1)
+        orc2_fmg(slv, 0, x, b);
+    else
+        orc2_solver_vcycle(slv, x, b);
+}
+
 int orc2_solver_solve(orc2_solver *slv, const double *b, double *x, double *hist, int maxhist)
 {
     orc2_level *L = &slv->lv[0];
@@ -867,7 +895,7 @@ int orc2_solver_solve(orc2_solver *slv, const double *b, double *x, double *hist
     double res0 = orc2_norm(L->res, L->ii, L->jj);
     int it = 0;
     for (int i = 0; i < slv->s.maxiter; ++i) {
-        orc2_solver_vcycle(slv, x, b);
+        orc2_solver_cycle(slv, x, b);
         orc2_residual(L->so, L->nstncl, b, x, L->res, L->ii, L->jj);
         double rel = orc2_norm(L->res, L->ii, L->jj) / res0;
         if (i < maxhist) hist[i] = rel;

@@ -66,6 +66,7 @@ typedef struct {
     int maxiter;      /* solver.max-iter */
     double tol;       /* solver.tol */
     int min_coarse;   /* solver.min_coarse, default 3 */
+    int cycle;        /* solver.cycle.type: 0 = V (default), 1 = F */
 } orc2_settings;
 
 typedef struct orc2_solver orc2_solver;
@@ -81,6 +82,8 @@ void orc2_solver_get_level(const orc2_solver *slv, int lvl, double *so, double *
 const double *orc2_solver_abd(const orc2_solver *slv, int *nabd1, int *nabd2);
 /* One V-cycle on (x, b) at the finest level. */
 void orc2_solver_vcycle(orc2_solver *slv, double *x, const double *b);
+/* One cycle of the configured type (V, or F: include/cedar/cycle/fcycle.h:38-83). */
+void orc2_solver_cycle(orc2_solver *slv, double *x, const double *b);
 /* Full solve from the x given (x0 = 0 for Cedar's solve(b)); writes rho_i into hist,
    returns the number of cycles run. */
 int  orc2_solver_solve(orc2_solver *slv, const double *b, double *x, double *hist, int maxhist);

@@ -1225,6 +1225,32 @@ void orc3_solver_vcycle(orc3_solver *slv, double *x, const double *b)
         ncycle(slv, 0, x, b);
 }
 
+/* fcycle::fmg_cycle (include/cedar/cycle/fcycle.h:49-83) */
+static void fmg3(orc3_solver *slv, int l, double *x, const double *b)
+{
+    if (l == slv->nlevels - 1) {
+        coarse_solve(slv, x, b);
+        return;
+    }
+    orc3_level *F = &slv->lv[l], *C = &slv->lv[l + 1];
+    orc3_restrict(C->ci, b, C->b, F->ii, F->jj, F->kk, C->ii, C->jj, C->kk);
+    fmg3(slv, l + 1, C->x, C->b);
+    const size_t np = (size_t)F->ii * F->jj * F->kk;
+    memset(x, 0, np * sizeof(double));
+    memset(F->res, 0, np * sizeof(double));
+    orc3_interp_add(x, C->x, F->res, F->so, F->nstncl, C->ci, C->ii, C->jj, C->kk, F->ii, F->jj,
+                    F->kk);
+    ncycle(slv, l, x, b);
+}
+
+void orc3_solver_cycle(orc3_solver *slv, double *x, const double *b)
+{
+    if (slv->s.cycle == 1 && slv->nlevels > 1)
+        fmg3(slv, 0, x, b);
+    else
+        orc3_solver_vcycle(slv, x, b);
+}
+
 static int plane_relax(int r) { return r >= ORC3_RELAX_PLANE_XY && r <= ORC3_RELAX_PLANE_XYZ; }
 
 orc3_solver *orc3_solver_create(const double *so_fine, int nstncl, int nx, int ny, int nz,
@@ -1310,7 +1336,7 @@ int orc3_solver_solve(orc3_solver *slv, const double *b, double *x, double *hist
     const double res0 = orc3_norm(F->res, F->ii, F->jj, F->kk);
     int it = 0;
     for (int i = 0; i < slv->s.maxiter; ++i) {
-        orc3_solver_vcycle(slv, x, b);
+        orc3_solver_cycle(slv, x, b);
         orc3_residual(F->so, F->nstncl, b, x, F->res, F->ii, F->jj, F->kk);
         const double rel = orc3_norm(F->res, F->ii, F->jj, F->kk) / res0;
         if (i < maxhist) hist[i] = rel;

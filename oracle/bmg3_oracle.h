@@ -75,6 +75,7 @@ typedef struct {
     double tol;
     int min_coarse;   /* default 3 */
     orc2_settings plane; /* nested plane solves: line-xy, max-iter 1 (src/kernel_params.cc:83-89) */
+    int cycle;        /* 0 = V (default), 1 = F */
 } orc3_settings;
 
 typedef struct orc3_solver orc3_solver;
@@ -86,6 +87,7 @@ int  orc3_solver_nlevels(const orc3_solver *slv);
 void orc3_solver_level_dims(const orc3_solver *slv, int lvl, int *nx, int *ny, int *nz);
 void orc3_solver_get_level(const orc3_solver *slv, int lvl, double *so, double *ci);
 void orc3_solver_vcycle(orc3_solver *slv, double *x, const double *b);
+void orc3_solver_cycle(orc3_solver *slv, double *x, const double *b);
 int  orc3_solver_solve(orc3_solver *slv, const double *b, double *x, double *hist, int maxhist);
 /* one plane-relaxation sweep (all planes of direction dir, both colours) at level lvl */
 void orc3_solver_relax_planes(orc3_solver *slv, int lvl, int dir, double *x, const double *b,

@@ -33,7 +33,7 @@ def _ptr(a):
 
 class _Settings(ctypes.Structure):
     _fields_ = [("relax", _i), ("nrelax_pre", _i), ("nrelax_post", _i), ("num_levels", _i),
-                ("maxiter", _i), ("tol", ctypes.c_double), ("min_coarse", _i)]
+                ("maxiter", _i), ("tol", ctypes.c_double), ("min_coarse", _i), ("cycle", _i)]
 
 
 _lib = None
@@ -224,12 +224,12 @@ class Solver:
     """cdr2::solver<fsten>(so, conf): setup in the constructor, solve(b) from x0 = 0."""
 
     def __init__(self, so, relaxation="point", nrelax_pre=2, nrelax_post=1, num_levels=-1,
-                 maxiter=10, tol=1e-8, min_coarse=3):
+                 maxiter=10, tol=1e-8, min_coarse=3, cycle="v"):
         self.so = np.ascontiguousarray(so, dtype=np.float64)  # held for the solver's lifetime
         ns, jj, ii = self.so.shape
         self.ii, self.jj = ii, jj
         s = _Settings(RELAX[relaxation], nrelax_pre, nrelax_post, num_levels, maxiter, tol,
-                      min_coarse)
+                      min_coarse, {"v": 0, "f": 1}[cycle])
         L = lib()
         self._h = ctypes.c_void_p(L.orc2_solver_create(_ptr(self.so), ns, ii - 2, jj - 2,
                                                        ctypes.byref(s)))

@@ -25,7 +25,7 @@ _p = _o._ptr
 class _Settings3(ctypes.Structure):
     _fields_ = [("relax", _i), ("nrelax_pre", _i), ("nrelax_post", _i), ("num_levels", _i),
                 ("maxiter", _i), ("tol", ctypes.c_double), ("min_coarse", _i),
-                ("plane", _o._Settings)]
+                ("plane", _o._Settings), ("cycle", _i)]
 
 
 _init = False
@@ -190,12 +190,12 @@ class Solver3:
     """3D FP64 oracle solver: cdr3::solver<fsten>(so, conf) + solve(b, x)."""
 
     def __init__(self, so, relaxation="point", nrelax_pre=2, nrelax_post=1, num_levels=-1,
-                 maxiter=10, tol=1e-8, min_coarse=3, plane=None):
+                 maxiter=10, tol=1e-8, min_coarse=3, plane=None, cycle="v"):
         self.so = np.ascontiguousarray(so, dtype=np.float64)
         ns, kk, jj, ii = self.so.shape
         self.shape = (kk, jj, ii)
         s = _Settings3(RELAX3[relaxation], nrelax_pre, nrelax_post, num_levels, maxiter, tol,
-                       min_coarse, plane or plane_settings())
+                       min_coarse, plane or plane_settings(), {"v": 0, "f": 1}[cycle])
         self.maxiter = maxiter
         self._h = ctypes.c_void_p(lib().orc3_solver_create(_p(self.so), _i(ns), _i(ii - 2),
                                                            _i(jj - 2), _i(kk - 2),

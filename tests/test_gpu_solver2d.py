@@ -161,3 +161,27 @@ def test_graph_replay_matches_eager(C, precision):
     x_g2, h_g2 = graph.solve(db2, torch.zeros_like(db2))
     assert np.array_equal(h_g2, h_e2)
     assert torch.equal(x_g2, x_e2)
+
+
+@pytest.mark.parametrize("relax,n,nine,precision", [("point", 127, False, "f64"),
+                                                    ("line-xy", 96, True, "f64"),
+                                                    ("line-xy", 127, True, "mixed")])
+def test_fcycle_matches_oracle(C, relax, n, nine, precision):
+    """solver.cycle.type = f (include/cedar/cycle/fcycle.h): each cycle recomputes x from b, so
+    the history repeats; FP64 matches the oracle to the usual bar, mixed reaches the same
+    residual level."""
+    if nine:
+        so, b, _ = G.heterogeneous_9pt(n, seed=6, block=8, sigma=1.0)
+    else:
+        so, b = G.poisson_uniform_rhs(n, seed=6)
+    o = O.Solver(so, relaxation=relax, nrelax_pre=1, nrelax_post=1, maxiter=3, cycle="f")
+    xo, ho = o.solve(b)
+    g = C.Solver(dev(so), relaxation=relax, nrelax_pre=1, nrelax_post=1, maxiter=3, cycle="f",
+                 precision=precision)
+    xg, hg = g.solve(dev(b))
+    assert len(hg) == len(ho) == 3
+    if precision == "f64":
+        assert np.abs(hg - ho).max() <= 1e-12
+        assert rel_l2(xg.cpu().numpy(), xo) <= 1e-10
+    else:
+        assert np.abs(hg - ho).max() <= 1e-5 * ho[0] + 1e-9

@@ -272,3 +272,36 @@ def test_small_solves_converge(relax):
     assert hist[-1] < 1e-10
     A = stencil_matrix(so)
     np.testing.assert_allclose(A @ interior(x), interior(b), rtol=0, atol=1e-8 * np.abs(interior(b)).max())
+
+
+def test_fcycle_oracle_composition():
+    """fcycle::fmg_cycle (include/cedar/cycle/fcycle.h:49-83) on a 2-level hierarchy is: coarse
+    solve of R b, x = P x_c (zero residual term), then one V-cycle.  Rebuilt here from the
+    oracle's separately pinned kernels; every F-cycle restarts from zero, so the residual history
+    repeats."""
+    from cedar_amd import gallery as G
+    so, b = G.poisson_uniform_rhs(31, seed=4)
+    s = O.Solver(so, relaxation="point", num_levels=2, cycle="f", maxiter=3)
+    x_f, h_f = s.solve(b)
+    assert h_f[0] == h_f[1] == h_f[2]
+    v = O.Solver(so, relaxation="point", num_levels=2, maxiter=1)
+    _, ci, _, _ = v.level(1)
+    abd = v.abd()
+    bc = O.restrict(ci, b)
+    xc = O.solve_cg(abd, bc)
+    x0 = np.zeros_like(b)
+    x1, _ = O.interp_add(x0, xc, np.zeros_like(b), so, ci)
+    x_ref = v.vcycle(x1, b)
+    assert np.array_equal(x_f, x_ref)
+    # and the F-cycle beats one V-cycle from zero
+    _, h_v = O.Solver(so, relaxation="point", num_levels=2, maxiter=1).solve(b)
+    assert h_f[0] < h_v[0]
+
+
+def test_fcycle_oracle_3d_repeats():
+    from cedar_amd import gallery as G
+    from oracle import oracle3 as O3
+    so, b, _ = G.poisson3_sine(15)
+    _, h = O3.Solver3(so, cycle="f", maxiter=3).solve(b)
+    _, hv = O3.Solver3(so, maxiter=1).solve(b)
+    assert h[0] == h[1] == h[2] and h[0] < hv[0]
