@@ -10,7 +10,8 @@ resident in HBM.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 4095] [--cpu-cycles 2]
 
 Prints one JSON line (rank 0).  `--workload c3` instead times the 3D plane-relaxation solve
-(255^3 anisotropic 7-point finite-volume problem, plane-xy V(1,1), FP64) and prints its own line.
+(255^3 anisotropic 7-point finite-volume problem, plane-xy V(1,1), FP64 and mixed) and
+`--workload c4` the batched multi-size 2D solves; each prints its own line.
 """
 import argparse
 import ctypes
@@ -102,7 +103,7 @@ def kernel_only(args, so, b):
 
 def bench_c3(args):
     """3D: 7-point FV -div(K grad u) = f, (kx, ky, kz) = (1, 1, 1e-3) k, lognormal k (sigma 1.5)
-    on 8^3 blocks, 27-point Galerkin coarse operators, plane-xy V(1,1) to 1e-10, FP64."""
+    on 8^3 blocks, 27-point Galerkin coarse operators, plane-xy V(1,1) to 1e-10, FP64 and mixed."""
     import torch
 
     import cedar_amd
@@ -114,27 +115,87 @@ def bench_c3(args):
     so = torch.as_tensor(so_np).cuda()
     b = torch.as_tensor(b_np).cuda()
     del so_np
-    s = cedar_amd.Solver3(so, relaxation=args.relax3, nrelax_pre=1, nrelax_post=1, maxiter=50,
-                          tol=1e-10)
-    setup_s = s.times()[0]
-    for _ in range(args.warmup):
-        s.solve(b)
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    for _ in range(args.steps):
-        x, h = s.solve(b)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t) / args.steps
+    res = {}
+    for prec in ("f64", "mixed"):
+        s = cedar_amd.Solver3(so, relaxation=args.relax3, nrelax_pre=1, nrelax_post=1,
+                              maxiter=60, tol=1e-10, precision=prec)
+        setup_s = s.times()[0]
+        for _ in range(args.warmup):
+            s.solve(b)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            x, h = s.solve(b)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / args.steps
+        res[prec] = {"seconds": dt, "iters": int(len(h)), "rho": float(h[-1]),
+                     "setup_s": setup_s, "per_cycle_s": dt / len(h), "x": x}
+        del s
+    x64, x32 = res["f64"].pop("x"), res["mixed"].pop("x")
+    rel = float((torch.linalg.norm(x32 - x64) / torch.linalg.norm(x64)).item())
     print(json.dumps({
-        "metric": "time-to-1e-10 rel. residual (s), 3D plane relaxation",
-        "value": dt, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": dt * 1e3, "higher_is_better": False, "dtype": "f64",
+        "metric": "time-to-1e-10 rel. residual (s), 3D plane relaxation; mixed vs FP64",
+        "value": res["mixed"]["seconds"], "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res["mixed"]["seconds"] * 1e3,
+        "higher_is_better": False, "dtype": "f32 cycle + f64 refinement",
         "data": "synthetic (seeded lognormal block coefficients, uniform RHS)",
         "config": {"workload": f"3D 7-point FV, {n}^3, (kx,ky,kz)=(1,1,1e-3)k, lognormal k "
-                               f"sigma=1.5 on 8^3 blocks, {args.relax3} V(1,1), tol 1e-10",
-                   "levels": s.nlevels},
-        "iters": int(len(h)), "rho": float(h[-1]), "setup_s": setup_s, "gen_s": gen_s,
-        "per_cycle_s": dt / len(h)}))
+                               f"sigma=1.5 on 8^3 blocks, {args.relax3} V(1,1), tol 1e-10"},
+        "fp64": res["f64"], "mixed": dict(res["mixed"], rel_l2_vs_fp64=rel),
+        "speedup_mixed_vs_fp64": res["f64"]["seconds"] / res["mixed"]["seconds"],
+        "gen_s": gen_s}))
+
+
+def bench_c4(args):
+    """Batched solves: 96 independent 2D 9-point problems, sizes drawn uniformly from
+    {257, 513, 1025, 2049}^2 (seed 4), each with its own lognormal block coefficients (sigma 2,
+    16x16 blocks) and 4 uniform right-hand sides; mixed precision to 1e-10 each, line-xy V(1,1).
+    Problems are grouped by size; each group is one batched solver (one launch per kernel per
+    level for the whole group)."""
+    import numpy as np
+    import torch
+
+    from cedar_amd import gallery as G
+    from cedar_amd.batch import BatchSet
+    rng = np.random.default_rng(4)
+    sizes = rng.choice([257, 513, 1025, 2049], size=args.nprob)
+    t0 = time.time()
+    ops, rhs = [], [[] for _ in range(4)]
+    for k, n in enumerate(sizes):
+        so, _, _ = G.heterogeneous_9pt(int(n), seed=4000 + k, block=16, sigma=2.0, contrast=1e4)
+        ops.append(torch.as_tensor(so).cuda())
+        for r in range(4):
+            rhs[r].append(torch.as_tensor(G.rhs_uniform(int(n), int(n), 40000 + 4 * k + r)).cuda())
+    gen_s = time.time() - t0
+    kw = dict(relaxation="line-xy", nrelax_pre=1, nrelax_post=1, maxiter=100, tol=1e-10,
+              precision="mixed")
+    bs = BatchSet(ops, **kw)
+    setup_s = bs.setup_seconds()
+    for _ in range(args.warmup):
+        bs.solve(rhs[0])
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    iters = []
+    worst = 0.0
+    for _ in range(args.steps):
+        for r in range(4):
+            xs, hs = bs.solve(rhs[r])
+            iters += [len(h) for h in hs]
+            worst = max(worst, max(float(h[-1]) for h in hs))
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / args.steps
+    nsolve = 4 * len(ops)
+    print(json.dumps({
+        "metric": "batched 2D solves to 1e-10 each: time for all problems and right-hand sides",
+        "value": dt, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": False,
+        "dtype": "f32 cycle + f64 refinement",
+        "data": "synthetic (seeded lognormal block coefficients, uniform RHS)",
+        "config": {"workload": f"{len(ops)} 2D 9-point problems, sizes uniform in "
+                               "{257,513,1025,2049}^2, 4 RHS each, line-xy V(1,1), tol 1e-10",
+                   "groups": {str(k[1] - 2): len(v) for k, v in bs.groups.items()}},
+        "solves": nsolve, "solves_per_s": nsolve / dt, "setup_s": setup_s,
+        "cycles_min_max": [min(iters), max(iters)], "worst_rho": worst, "gen_s": gen_s}))
 
 
 def main():
@@ -149,7 +210,8 @@ def main():
     ap.add_argument("--kernel-only", action="store_true",
                     help="run only the dominant kernel (fine-level FP32 y-line sweep) and a plain "
                          "256 MiB device copy, for hardware-counter runs")
-    ap.add_argument("--workload", choices=("c2", "c3"), default="c2")
+    ap.add_argument("--workload", choices=("c2", "c3", "c4"), default="c2")
+    ap.add_argument("--nprob", type=int, default=96)
     ap.add_argument("--n3", type=int, default=255)
     ap.add_argument("--relax3", default="plane-xy")
     args = ap.parse_args()
@@ -160,6 +222,9 @@ def main():
 
     if args.workload == "c3":
         bench_c3(args)
+        return
+    if args.workload == "c4":
+        bench_c4(args)
         return
 
     import torch

@@ -35,7 +35,7 @@ class Settings3(ctypes.Structure):
     """cdr_solver3_settings"""
     _fields_ = [("relaxation", _i), ("nrelax_pre", _i), ("nrelax_post", _i), ("num_levels", _i),
                 ("maxiter", _i), ("tol", _d), ("min_coarse", _i), ("plane", Settings),
-                ("cycle", _i)]
+                ("cycle", _i), ("precision", _i)]
 
 
 # name -> (restype, argtypes); pointers to device memory are passed as c_void_p
@@ -57,6 +57,8 @@ PROTOTYPES = {
     "cdr_hip_relax_lines_y2d_f64": (_i, [_P, _i, _P, _P, _P, _P, _i, _i, _i, _vp]),
     "cdr_hip_relax_lines_x2d_f32": (_i, [_P, _i, _P, _P, _P, _P, _i, _i, _i, _vp]),
     "cdr_hip_relax_lines_y2d_f32": (_i, [_P, _i, _P, _P, _P, _P, _i, _i, _i, _vp]),
+    "cdr_hip_relax_lines_xy2d_f64": (_i, [_P, _i, _P, _P, _P, _P, _P, _i, _i, _i, _vp]),
+    "cdr_hip_relax_lines_xy2d_f32": (_i, [_P, _i, _P, _P, _P, _P, _P, _i, _i, _i, _vp]),
     "cdr_hip_restrict2d_f64": (_i, [_P, _P, _P, _i, _i, _i, _i, _vp]),
     "cdr_hip_restrict2d_f32": (_i, [_P, _P, _P, _i, _i, _i, _i, _vp]),
     "cdr_hip_interp_add2d_f64": (_i, [_P, _P, _P, _P, _i, _P, _i, _i, _i, _i, _vp]),

@@ -144,6 +144,17 @@ def relax_lines(so, qf, q, sor, updown, direction):
     return q
 
 
+def relax_lines_xy(so, qf, q, sorx, sory, updown, work=None):
+    """One line-xy smoothing step (x then y going down, y then x going up), both colours of each
+    sweep per launch; `work` is a scratch grid like q (allocated when not given)."""
+    so, qf, q, sorx, sory = map(_t, (so, qf, q, sorx, sory))
+    if work is None:
+        work = torch.empty_like(q)
+    _abi.call(f"cdr_hip_relax_lines_xy2d_{_sfx(q)}", _p(so), so.shape[-3], _p(qf), _p(q), _p(sorx),
+              _p(sory), _p(work), q.shape[-1], q.shape[-2], int(updown), _stream())
+    return q
+
+
 def restrict(ci, q, qc):
     ci, q, qc = map(_t, (ci, q, qc))
     _abi.call(f"cdr_hip_restrict2d_{_sfx(q)}", _p(ci), _p(q), _p(qc), q.shape[1], q.shape[0],

@@ -118,7 +118,8 @@ def plane_settings(relaxation="line-xy", nrelax_pre=2, nrelax_post=1, maxiter=1,
 
 
 class Solver3:
-    """cdr3::solver<fsten>(so, conf) on the GPU, FP64.
+    """cdr3::solver<fsten>(so, conf) on the GPU; precision 'f64' (Cedar's algorithm) or 'mixed'
+    (FP32 3D cycle with FP32 plane solves inside FP64 iterative refinement).
 
     Settings use Cedar's keys and defaults: relaxation ('point' | 'plane-xy' | 'plane-xz' |
     'plane-yz' | 'plane-xyz'), nrelax_pre=2, nrelax_post=1, num_levels=-1, maxiter=10, tol=1e-8,
@@ -126,13 +127,14 @@ class Solver3:
     """
 
     def __init__(self, so, relaxation="point", nrelax_pre=2, nrelax_post=1, num_levels=-1,
-                 maxiter=10, tol=1e-8, min_coarse=3, plane=None, cycle="v"):
+                 maxiter=10, tol=1e-8, min_coarse=3, plane=None, cycle="v", precision="f64"):
         so = so if isinstance(so, torch.Tensor) else torch.as_tensor(np.ascontiguousarray(so))
         self.so = so.to(device="cuda", dtype=torch.float64).contiguous()  # held by reference
         ns, kk, jj, ii = self.so.shape
         self.maxiter = maxiter
         s = _abi.Settings3(_abi.RELAX[relaxation], nrelax_pre, nrelax_post, num_levels, maxiter, tol,
-                           min_coarse, plane or plane_settings(), _abi.CYCLE[cycle])
+                           min_coarse, plane or plane_settings(), _abi.CYCLE[cycle],
+                           _abi.PRECISION[precision])
         h = ctypes.c_void_p()
         _abi.call("cdr_hip_solver3d_create", _p(self.so), ns, ii - 2, jj - 2, kk - 2,
                   ctypes.byref(s), _stream(), ctypes.byref(h))

@@ -528,26 +528,15 @@ __device__ inline void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// x-lines: one wavefront per line, four lines per workgroup.
+// x-lines: one wavefront relaxes row j, reading the neighbouring rows from qin and writing the
+// row to qout (qin == qout for an in-place sweep).  No workgroup barrier inside.
 template <typename T, bool NINE, int NCH>
-__global__ __launch_bounds__(256) void relax_xlines_reg_k(const T *__restrict__ so,
-                                                          const T *__restrict__ qf,
-                                                          T *__restrict__ q,
-                                                          const T *__restrict__ sor, int ii,
-                                                          int jj, int first, int nlines,
-                                                          int ntiles) {
-    constexpr int WPB = 4, LA = line_lookahead<T>();
-    const int tile = xcd_tile(ntiles);
-    if (tile >= ntiles) return;
-    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
-    const int line = tile * WPB + wave;
-    if (line >= nlines) return;  // whole wavefront; no workgroup barrier in this kernel
-    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.y;
-    so += bz * (NINE ? 5 : 3) * P;
-    qf += bz * P;
-    q += bz * P;
-    sor += bz * 2 * P;
-    const int j = first + 2 * line;
+__device__ inline void xline_solve(const T *__restrict__ so, const T *__restrict__ qf,
+                                   const T *qin, T *qout, const T *__restrict__ sor, int ii,
+                                   int jj, int j) {
+    constexpr int LA = line_lookahead<T>();
+    const int lane = threadIdx.x % WAVE;
+    const std::size_t P = (std::size_t)ii * jj;
     const int n = ii - 2;
     const int nact = (n + WAVE - 1) / WAVE;
     const std::size_t row = (std::size_t)j * ii;
@@ -558,7 +547,7 @@ __global__ __launch_bounds__(256) void relax_xlines_reg_k(const T *__restrict__ 
     // exec-masked region splits the prefetch from its use.
     auto fetch = [&](int c, line_raw<T> &r) {
         const int k = min(c * WAVE + lane, n - 1);
-        line_load<T, NINE, 0>(r, so, qf, q, P, ii, row + k + 1);
+        line_load<T, NINE, 0>(r, so, qf, qin, P, ii, row + k + 1);
         r.e = E[max(k - 1, 0)];
         r.d = D[k];
     };
@@ -623,41 +612,87 @@ __global__ __launch_bounds__(256) void relax_xlines_reg_k(const T *__restrict__ 
                 const T x = a[u] * carry + b[u];
                 carry = last_lane(x);
                 const int k = c * WAVE + rl;
-                if (k < n) q[row + k + 1] = x;
+                if (k < n) qout[row + k + 1] = x;
             }
         }
     }
 }
 
-// y-lines: a workgroup of NW wavefronts relaxes NW neighbouring columns of one colour.  The
-// right-hand sides of each 64-row chunk are formed cooperatively with row-contiguous (coalesced)
-// loads and handed to the column owners through LDS; each wavefront then scans its column
-// (lanes along j; the transposed factors are contiguous in j).  Results go back through LDS so
-// that the stores are row-contiguous too.
-template <typename T, bool NINE, int NCH, int NW>
-__global__ __launch_bounds__(NW * 64) void relax_ylines_reg_k(const T *__restrict__ so,
-                                                              const T *__restrict__ qf,
-                                                              T *__restrict__ q,
-                                                              const T *__restrict__ sor, int ii,
-                                                              int jj, int first, int ncols,
-                                                              int ntiles) {
-    constexpr int LA = line_lookahead<T>();
-    constexpr int G = NCH < line_group<T>() ? NCH : line_group<T>();
-    __shared__ T tile_s[G][NW][WAVE + 1];
+
+// x-lines, one colour per launch, in place: one wavefront per line, four lines per workgroup.
+template <typename T, bool NINE, int NCH>
+__global__ __launch_bounds__(256) void relax_xlines_reg_k(const T *__restrict__ so,
+                                                          const T *__restrict__ qf,
+                                                          T *__restrict__ q,
+                                                          const T *__restrict__ sor, int ii,
+                                                          int jj, int first, int nlines,
+                                                          int ntiles) {
+    constexpr int WPB = 4;
     const int tile = xcd_tile(ntiles);
-    if (tile >= ntiles) return;  // uniform over the workgroup
-    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const int col0 = tile * NW;
-    const bool colok = col0 + wave < ncols;
-    const int ime = first + 2 * (col0 + wave);
-    const int tc = threadIdx.x % NW, tr = threadIdx.x / NW;
-    const bool tok = col0 + tc < ncols;
-    const int ti = first + 2 * (col0 + tc);
+    if (tile >= ntiles) return;
+    const int wave = threadIdx.x / WAVE;
+    const int line = tile * WPB + wave;
+    if (line >= nlines) return;  // whole wavefront; no workgroup barrier in this kernel
+    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.y;
+    xline_solve<T, NINE, NCH>(so + bz * (NINE ? 5 : 3) * P, qf + bz * P, q + bz * P, q + bz * P,
+                              sor + bz * 2 * P, ii, jj, first + 2 * line);
+}
+
+// Both colours of an x-line sweep in one launch, out of place (qin -> qout).  Rows are dealt in
+// bands of 2B: band t covers rows R0 .. R0+2B-1, R0 = 1 + 2Bt.  Phase A: wavefront w relaxes the
+// first-colour row R0 - 1 + aoff + 2w (w = 0..B) from the old rows in qin -- B + 1 rows, the
+// outermost one shared with the neighbouring band and computed by both (identical inputs, so
+// identical values); phase B: wavefront w < B relaxes the second-colour row R0 + aoff + 2w from
+// the new first-colour rows.  aoff = 0 when the even rows come first (DOWN), 1 otherwise.
+// Each coupling plane row is read from HBM once per sweep instead of once per colour.  qout's
+// ghost rows and columns must equal qin's.
+template <typename T, bool NINE, int NCH, int B>
+__global__ __launch_bounds__((B + 1) * 64) void relax_xlines_pair_k(const T *__restrict__ so,
+                                                                   const T *__restrict__ qf,
+                                                                   const T *qin, T *qout,
+                                                                   const T *__restrict__ sor,
+                                                                   int ii, int jj, int aoff,
+                                                                   int nbands) {
+    const int band = xcd_tile(nbands);
+    if (band >= nbands) return;  // uniform over the workgroup
+    const int wave = threadIdx.x / WAVE;
     const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.y;
     so += bz * (NINE ? 5 : 3) * P;
     qf += bz * P;
-    q += bz * P;
+    qin += bz * P;
+    qout += bz * P;
     sor += bz * 2 * P;
+    const int ny = jj - 2, r0 = 1 + 2 * B * band;
+    const int ra = r0 - 1 + aoff + 2 * wave;
+    if (ra >= 1 && ra <= ny) xline_solve<T, NINE, NCH>(so, qf, qin, qout, sor, ii, jj, ra);
+    __syncthreads();  // the new first-colour rows are visible to the whole workgroup
+    const int rb = r0 + aoff + 2 * wave;
+    if (wave < B && rb <= ny) xline_solve<T, NINE, NCH>(so, qf, qout, qout, sor, ii, jj, rb);
+}
+
+// y-lines: the NW wavefronts of a workgroup relax the columns cb, cb+2, .., cb+2(nc-1)
+// (nc <= NW, uniform over the workgroup), reading neighbouring columns from qin and writing to
+// qout.  The right-hand sides of each 64-row chunk are formed cooperatively with row-contiguous
+// (coalesced) loads and handed to the column owners through LDS; each wavefront then scans its
+// column (lanes along j; the transposed factors are contiguous in j).  Results go back through
+// LDS so that the stores are row-contiguous too.
+template <typename T, int NCH>
+constexpr int yline_group() { return NCH < line_group<T>() ? NCH : line_group<T>(); }
+
+template <typename T, bool NINE, int NCH, int NW>
+__device__ inline void ylines_solve(const T *__restrict__ so, const T *__restrict__ qf,
+                                    const T *qin, T *qout, const T *__restrict__ sor, int ii,
+                                    int jj, int cb, int nc,
+                                    T (&tile_s)[yline_group<T, NCH>()][NW][WAVE + 1]) {
+    constexpr int G = yline_group<T, NCH>();
+    constexpr int LA = line_lookahead<T>();
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const bool colok = wave < nc;
+    const int ime = cb + 2 * wave;
+    const int tc = threadIdx.x % NW, tr = threadIdx.x / NW;
+    const bool tok = tc < nc;
+    const int ti = cb + 2 * tc;
+    const std::size_t P = (std::size_t)ii * jj;
     const int n = jj - 2;
     const int nact = (n + WAVE - 1) / WAVE;
     const T *D = sor + (std::size_t)ime * jj + 1;
@@ -666,12 +701,12 @@ __global__ __launch_bounds__(NW * 64) void relax_ylines_reg_k(const T *__restric
     // factor entries (column ime, row c*64 + lane)
     // branch-free loads: row and column indices are clamped onto valid points, unused lanes are
     // neutralised where the values are consumed (see the x-line kernel)
-    const int tic = first + 2 * min(col0 + tc, ncols - 1);
-    const T *Dc = colok ? D : sor + (std::size_t)(first + 2 * (ncols - 1)) * jj + 1;
-    const T *Ec = colok ? E : sor + P + (std::size_t)(first + 2 * (ncols - 1)) * jj + 2;
+    const int tic = cb + 2 * min(tc, nc - 1);
+    const T *Dc = colok ? D : sor + (std::size_t)(cb + 2 * (nc - 1)) * jj + 1;
+    const T *Ec = colok ? E : sor + P + (std::size_t)(cb + 2 * (nc - 1)) * jj + 2;
     auto fetch = [&](int c, line_raw<T> &r) {
         const int k = min(c * WAVE + tr, n - 1);
-        line_load<T, NINE, 1>(r, so, qf, q, P, ii, pidx(tic, k + 1, ii));
+        line_load<T, NINE, 1>(r, so, qf, qin, P, ii, pidx(tic, k + 1, ii));
         const int kw = min(c * WAVE + lane, n - 1);
         r.e = Ec[max(kw - 1, 0)];
         r.d = Dc[kw];
@@ -744,11 +779,66 @@ __global__ __launch_bounds__(NW * 64) void relax_ylines_reg_k(const T *__restric
 #pragma unroll
             for (int u = 0; u < G; ++u) {
                 const int kk = (g + u) * WAVE + tr;
-                if (tok && kk < n) q[pidx(ti, kk + 1, ii)] = tile_s[u][tc][tr];
+                if (tok && kk < n) qout[pidx(ti, kk + 1, ii)] = tile_s[u][tc][tr];
             }
             lds_barrier();
         }
     }
+}
+
+
+// y-lines, one colour per launch, in place: NW columns of the colour per workgroup.
+template <typename T, bool NINE, int NCH, int NW>
+__global__ __launch_bounds__(NW * 64) void relax_ylines_reg_k(const T *__restrict__ so,
+                                                              const T *__restrict__ qf,
+                                                              T *__restrict__ q,
+                                                              const T *__restrict__ sor, int ii,
+                                                              int jj, int first, int ncols,
+                                                              int ntiles) {
+    __shared__ T tile_s[yline_group<T, NCH>()][NW][WAVE + 1];
+    const int tile = xcd_tile(ntiles);
+    if (tile >= ntiles) return;  // uniform over the workgroup
+    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.y;
+    const int col0 = tile * NW;
+    ylines_solve<T, NINE, NCH, NW>(so + bz * (NINE ? 5 : 3) * P, qf + bz * P, q + bz * P, q + bz * P,
+                                   sor + bz * 2 * P, ii, jj, first + 2 * col0,
+                                   min(NW, ncols - col0), tile_s);
+}
+
+// Both colours of a y-line sweep in one launch, out of place (qin -> qout): the column analogue
+// of relax_xlines_pair_k.  Band t covers columns C0 .. C0+2B-1, C0 = 1 + 2Bt; phase A relaxes
+// the first-colour columns C0 - 1 + aoff + 2w (w = 0..B, inside 1..nx) from qin, phase B the
+// second-colour columns C0 + aoff + 2w (w < B) from the new values in qout.
+template <typename T, bool NINE, int NCH, int B>
+__global__ __launch_bounds__((B + 1) * 64) void relax_ylines_pair_k(const T *__restrict__ so,
+                                                                   const T *__restrict__ qf,
+                                                                   const T *qin, T *qout,
+                                                                   const T *__restrict__ sor,
+                                                                   int ii, int jj, int aoff,
+                                                                   int nbands) {
+    constexpr int NW = B + 1;
+    __shared__ T tile_s[yline_group<T, NCH>()][NW][WAVE + 1];
+    const int band = xcd_tile(nbands);
+    if (band >= nbands) return;  // uniform over the workgroup
+    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.y;
+    so += bz * (NINE ? 5 : 3) * P;
+    qf += bz * P;
+    qin += bz * P;
+    qout += bz * P;
+    sor += bz * 2 * P;
+    const int nx = ii - 2, c0 = 1 + 2 * B * band;
+    int ca = c0 - 1 + aoff;      // first-colour columns ca, ca+2, .. (B + 1 of them)
+    int na = B + 1;
+    if (ca < 1) {                // the ghost column 0 is not relaxed
+        ca += 2;
+        --na;
+    }
+    if (ca + 2 * (na - 1) > nx) na = ca > nx ? 0 : (nx - ca) / 2 + 1;
+    if (na > 0) ylines_solve<T, NINE, NCH, NW>(so, qf, qin, qout, sor, ii, jj, ca, na, tile_s);
+    __syncthreads();  // new first-colour columns visible to the whole workgroup
+    const int cbb = c0 + aoff;
+    const int nbc = cbb > nx ? 0 : min(B, (nx - cbb) / 2 + 1);
+    if (nbc > 0) ylines_solve<T, NINE, NCH, NW>(so, qf, qout, qout, sor, ii, jj, cbb, nbc, tile_s);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1114,6 +1204,31 @@ __global__ __launch_bounds__(256) void keep_inactive_k(double *__restrict__ x,
         if (active[k / per] == 0) x[k] = xprev[k];
 }
 
+// dst = src on the ghost ring of every problem of a batch
+template <typename T>
+__global__ __launch_bounds__(256) void copy_ghosts_k(const T *__restrict__ src, T *__restrict__ dst,
+                                                     int ii, int jj) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int per = 2 * ii + 2 * (jj - 2);
+    if (t >= per) return;
+    int i, j;
+    if (t < ii) {
+        i = t;
+        j = 0;
+    } else if (t < 2 * ii) {
+        i = t - ii;
+        j = jj - 1;
+    } else if (t < 2 * ii + jj - 2) {
+        i = 0;
+        j = t - 2 * ii + 1;
+    } else {
+        i = ii - 1;
+        j = t - 2 * ii - (jj - 2) + 1;
+    }
+    const std::size_t p = (std::size_t)blockIdx.y * ii * jj + pidx(i, j, ii);
+    dst[p] = src[p];
+}
+
 inline unsigned grid1(std::size_t n) {
     std::size_t b = (n + 255) / 256;
     if (b > 65536) b = 65536;
@@ -1288,6 +1403,75 @@ void relax_lines_y(const T *so, int nstncl, const T *qf, T *q, const T *sor, int
     relax_lines<T, 1>(so, nstncl, qf, q, sor, ii, jj, updown, s, nb);
 }
 
+// both colours of one sweep in one launch, qin -> qout (see relax_xlines_pair_k)
+template <typename T, bool NINE, int DIR, int NCH>
+static void launch_pair_reg(const T *so, const T *qf, const T *qin, T *qout, const T *sor, int ii,
+                            int jj, int aoff, hipStream_t s, int nb) {
+    constexpr int B = (DIR == 0) ? 4 : 7;  // second-colour lines per band
+    const int nl = (DIR == 0) ? jj - 2 : ii - 2;
+    const int nbands = (nl + 2 * B - 1) / (2 * B);
+    const dim3 blocks((unsigned)(8 * ((nbands + 7) / 8)), (unsigned)nb);
+    if (DIR == 0)
+        relax_xlines_pair_k<T, NINE, NCH, B><<<blocks, (B + 1) * 64, 0, s>>>(so, qf, qin, qout, sor,
+                                                                            ii, jj, aoff, nbands);
+    else
+        relax_ylines_pair_k<T, NINE, NCH, B><<<blocks, (B + 1) * 64, 0, s>>>(so, qf, qin, qout, sor,
+                                                                            ii, jj, aoff, nbands);
+}
+
+template <typename T, bool NINE, int DIR>
+static void launch_pair(const T *so, const T *qf, const T *qin, T *qout, const T *sor, int ii,
+                        int jj, int aoff, hipStream_t s, int nb) {
+    const int n = (DIR == 0) ? ii - 2 : jj - 2;
+    const int nch = (n + WAVE - 1) / WAVE;
+#define CDR_P(NC) launch_pair_reg<T, NINE, DIR, NC>(so, qf, qin, qout, sor, ii, jj, aoff, s, nb)
+    if (nch <= 1) CDR_P(1);
+    else if (nch <= 2) CDR_P(2);
+    else if (nch <= 4) CDR_P(4);
+    else if (nch <= 8) CDR_P(8);
+    else if (nch <= 16) CDR_P(16);
+    else if (nch <= 32) CDR_P(32);
+    else CDR_P(64);
+#undef CDR_P
+}
+
+template <typename T>
+void relax_lines_xy(const T *so, int nstncl, const T *qf, T *q, const T *sorx, const T *sory,
+                    T *work, int ii, int jj, int updown, hipStream_t s, int nb) {
+    if (ii - 2 > 64 * WAVE || jj - 2 > 64 * WAVE) {  // lines beyond the register-resident kernels
+        if (updown == 0) {
+            relax_lines<T, 0>(so, nstncl, qf, q, sorx, ii, jj, updown, s, nb);
+            relax_lines<T, 1>(so, nstncl, qf, q, sory, ii, jj, updown, s, nb);
+        } else {
+            relax_lines<T, 1>(so, nstncl, qf, q, sory, ii, jj, updown, s, nb);
+            relax_lines<T, 0>(so, nstncl, qf, q, sorx, ii, jj, updown, s, nb);
+        }
+        return;
+    }
+    const int per = 2 * ii + 2 * (jj - 2);
+    copy_ghosts_k<T><<<dim3((unsigned)((per + 255) / 256), (unsigned)nb), 256, 0, s>>>(q, work, ii, jj);
+    // DOWN relaxes the even lines first (aoff 0), UP the odd ones (aoff 1)
+    const int aoff = updown == 0 ? 0 : 1;
+    const bool nine = nstncl == 5;
+    if (updown == 0) {  // x then y
+        if (nine) {
+            launch_pair<T, true, 0>(so, qf, q, work, sorx, ii, jj, aoff, s, nb);
+            launch_pair<T, true, 1>(so, qf, work, q, sory, ii, jj, aoff, s, nb);
+        } else {
+            launch_pair<T, false, 0>(so, qf, q, work, sorx, ii, jj, aoff, s, nb);
+            launch_pair<T, false, 1>(so, qf, work, q, sory, ii, jj, aoff, s, nb);
+        }
+    } else {  // y then x
+        if (nine) {
+            launch_pair<T, true, 1>(so, qf, q, work, sory, ii, jj, aoff, s, nb);
+            launch_pair<T, true, 0>(so, qf, work, q, sorx, ii, jj, aoff, s, nb);
+        } else {
+            launch_pair<T, false, 1>(so, qf, q, work, sory, ii, jj, aoff, s, nb);
+            launch_pair<T, false, 0>(so, qf, work, q, sorx, ii, jj, aoff, s, nb);
+        }
+    }
+}
+
 template <typename T>
 void restrict_(const T *ci, const T *q, T *qc, int iif, int jjf, int iic, int jjc, hipStream_t s,
                int nb) {
@@ -1374,6 +1558,8 @@ void add_correction(double *x, const float *e, int ii, int jj, hipStream_t s, in
                                    hipStream_t, int);                                             \
     template void relax_lines_y<T>(const T *, int, const T *, T *, const T *, int, int, int,      \
                                    hipStream_t, int);                                             \
+    template void relax_lines_xy<T>(const T *, int, const T *, T *, const T *, const T *, T *, int,\
+                                    int, int, hipStream_t, int);                                  \
     template void restrict_<T>(const T *, const T *, T *, int, int, int, int, hipStream_t, int);  \
     template void interp_add<T>(T *, const T *, const T *, const T *, int, const T *, int, int,   \
                                 int, int, hipStream_t, int);                                      \

@@ -73,6 +73,14 @@ template <typename T>
 void relax_lines_y(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
                    int updown, hipStream_t s, int nb = 1);
 
+// One line-xy smoothing step (include/cedar/multilevel.h:175-178, 204-207): x-lines then y-lines
+// on DOWN, y then x on UP, each sweep relaxing both colours in one launch out of place through
+// `work` (one grid per problem; its interior is scratch, its ghost ring is set from q).  Equal
+// bit for bit to relax_lines_x followed by relax_lines_y (or y then x).
+template <typename T>
+void relax_lines_xy(const T *so, int nstncl, const T *qf, T *q, const T *sorx, const T *sory,
+                    T *work, int ii, int jj, int updown, hipStream_t s, int nb = 1);
+
 template <typename T>
 void restrict_(const T *ci, const T *q, T *qc, int iif, int jjf, int iic, int jjc, hipStream_t s, int nb = 1);
 template <typename T>

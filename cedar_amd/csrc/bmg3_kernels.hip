@@ -131,6 +131,42 @@ __global__ __launch_bounds__(256) void residual3_k(const T *__restrict__ so, con
         block_sum_store(sq, &partials[((std::size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x]);
 }
 
+// Mixed-precision outer step: xnew = x + (double) e (e may be null: xnew = x), r = b - A xnew in
+// FP64 (the residual's term order), r32 = (float) r, per-block sums of r^2.
+#define CDR3_ADDX(si, sj, sk, d, oi, oj, ok)                                                      \
+    v = v + S3(i + (si), j + (sj), k + (sk), d) * X(ix3(i + (oi), j + (oj), k + (ok), ii, jj));
+
+template <bool NINE, bool HAS_E>
+__global__ __launch_bounds__(256) void residual_ir3_k(const double *__restrict__ so,
+                                                      const double *__restrict__ qf,
+                                                      const double *__restrict__ x,
+                                                      const float *__restrict__ e,
+                                                      double *__restrict__ xnew,
+                                                      float *__restrict__ r32, int ii, int jj,
+                                                      int kk, double *__restrict__ partials) {
+    const int i = blockIdx.x * 64 + (threadIdx.x % 64) + 2;
+    const int j = blockIdx.y * 4 + (threadIdx.x / 64) + 2;
+    const int k = blockIdx.z + 2;
+    const std::size_t P = (std::size_t)ii * jj * kk;
+    double sq = 0.0;
+    if (i <= ii - 1 && j <= jj - 1) {
+        auto X = [&](std::size_t p) -> double { return HAS_E ? x[p] + (double)e[p] : x[p]; };
+        double v = Q3(qf, i, j, k);
+        if (NINE) {
+            NB27_LIST(CDR3_ADDX)
+        } else {
+            NB7_LIST(CDR3_ADDX)
+        }
+        const std::size_t p = ix3(i, j, k, ii, jj);
+        const double xc = X(p);
+        const double r = v - S3(i, j, k, kp) * xc;
+        xnew[p] = xc;
+        r32[p] = (float)r;
+        sq = r * r;
+    }
+    block_sum_store(sq, &partials[((std::size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x]);
+}
+
 // SOR(:,:,:,2) = 1 / SO_p over the interior (src/3d/ftn/BMG3_SymStd_SETUP_recip.f90:63-69)
 __global__ __launch_bounds__(256) void recip3_k(const double *__restrict__ so,
                                                 double *__restrict__ sor, int ii, int jj, int kk) {
@@ -1040,6 +1076,24 @@ void residual_norm(const double *so, int nstncl, const double *qf, const double 
         residual3_k<double, true, true><<<g, 256, 0, s>>>(so, qf, q, res, ii, jj, kk, partials);
     else
         residual3_k<double, false, true><<<g, 256, 0, s>>>(so, qf, q, res, ii, jj, kk, partials);
+    k2::finalize_sum(partials, (std::size_t)g.x * g.y * g.z, sumsq, 1, s);
+}
+
+void residual_ir(const double *so, int nstncl, const double *qf, const double *x, const float *e,
+                 double *xnew, float *r32, int ii, int jj, int kk, double *partials,
+                 double *sumsq, hipStream_t s) {
+    const dim3 g = grid3(ii - 2, jj - 2, kk - 2);
+    if (nstncl == 14) {
+        if (e)
+            residual_ir3_k<true, true><<<g, 256, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, kk, partials);
+        else
+            residual_ir3_k<true, false><<<g, 256, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, kk, partials);
+    } else {
+        if (e)
+            residual_ir3_k<false, true><<<g, 256, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, kk, partials);
+        else
+            residual_ir3_k<false, false><<<g, 256, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, kk, partials);
+    }
     k2::finalize_sum(partials, (std::size_t)g.x * g.y * g.z, sumsq, 1, s);
 }
 

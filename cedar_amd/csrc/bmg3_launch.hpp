@@ -26,6 +26,12 @@ void residual(const T *so, int nstncl, const T *qf, const T *q, T *res, int ii, 
 void residual_norm(const double *so, int nstncl, const double *qf, const double *q, double *res,
                    int ii, int jj, int kk, double *partials, double *sumsq, hipStream_t s);
 
+// mixed-precision outer step: xnew = x + (double) e (e may be null), r = b - A xnew in FP64,
+// r32 = (float) r, sum of r^2 into *sumsq (device)
+void residual_ir(const double *so, int nstncl, const double *qf, const double *x, const float *e,
+                 double *xnew, float *r32, int ii, int jj, int kk, double *partials,
+                 double *sumsq, hipStream_t s);
+
 void setup_recip(const double *so, int nstncl, double *sor, int ii, int jj, int kk,
                  hipStream_t s);
 template <typename T>

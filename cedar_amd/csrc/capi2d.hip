@@ -110,6 +110,16 @@ int relax_lines(int dir, const T *so, int ns, const T *qf, T *q, const T *sor, i
 }
 
 template <typename T>
+int relax_lines_xy(const T *so, int ns, const T *qf, T *q, const T *sorx, const T *sory, T *work,
+                   int ii, int jj, int ud, hipStream_t s, int nb) {
+    if (!so || !qf || !q || !sorx || !sory || !work || work == q || !ext_ok(ii, jj) ||
+        !sten_ok(ns) || !dir_ok(ud) || !nb_ok(nb))
+        return CDR_ERR_ARG;
+    k2::relax_lines_xy<T>(so, ns, qf, q, sorx, sory, work, ii, jj, ud, s, nb);
+    return launched();
+}
+
+template <typename T>
 int restrict_(const T *ci, const T *q, T *qc, int iif, int jjf, int iic, int jjc, hipStream_t s,
               int nb) {
     if (!ci || !q || !qc || !coarse_ok(iif, jjf, iic, jjc) || !nb_ok(nb)) return CDR_ERR_ARG;
@@ -164,6 +174,8 @@ int solve_cg(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj, 
                              int);                                                                 \
     template int relax_lines<T>(int, const T *, int, const T *, T *, const T *, int, int, int,      \
                                 hipStream_t, int);                                                 \
+    template int relax_lines_xy<T>(const T *, int, const T *, T *, const T *, const T *, T *, int,  \
+                                   int, int, hipStream_t, int);                                    \
     template int restrict_<T>(const T *, const T *, T *, int, int, int, int, hipStream_t, int);    \
     template int interp_add<T>(T *, const T *, const T *, const T *, int, const T *, int, int, int, \
                                int, hipStream_t, int);                                             \
@@ -260,6 +272,17 @@ int cdr_hip_relax_lines_y2d_f32(const float *so, int ns, const float *qf, float 
                                 const float *sor, float *b, int ii, int jj, int ud, void *st) {
     (void)b;
     return api2::relax_lines<float>(1, so, ns, qf, q, sor, ii, jj, ud, S(st));
+}
+
+int cdr_hip_relax_lines_xy2d_f64(const double *so, int ns, const double *qf, double *q,
+                                 const double *sorx, const double *sory, double *work, int ii,
+                                 int jj, int ud, void *st) {
+    return api2::relax_lines_xy<double>(so, ns, qf, q, sorx, sory, work, ii, jj, ud, S(st));
+}
+int cdr_hip_relax_lines_xy2d_f32(const float *so, int ns, const float *qf, float *q,
+                                 const float *sorx, const float *sory, float *work, int ii, int jj,
+                                 int ud, void *st) {
+    return api2::relax_lines_xy<float>(so, ns, qf, q, sorx, sory, work, ii, jj, ud, S(st));
 }
 
 int cdr_hip_restrict2d_f64(const double *ci, const double *q, double *qc, int iif, int jjf,

@@ -28,6 +28,10 @@ int setup_lines(int dir, const double *so, double *sor, int ii, int jj, int ns, 
 template <typename T>
 int relax_lines(int dir, const T *so, int ns, const T *qf, T *q, const T *sor, int ii, int jj,
                 int ud, hipStream_t s, int nb = 1);
+// one line-xy smoothing step through the work grid (bmg2_launch.hpp relax_lines_xy)
+template <typename T>
+int relax_lines_xy(const T *so, int ns, const T *qf, T *q, const T *sorx, const T *sory, T *work,
+                   int ii, int jj, int ud, hipStream_t s, int nb = 1);
 template <typename T>
 int restrict_(const T *ci, const T *q, T *qc, int iif, int jjf, int iic, int jjc, hipStream_t s,
               int nb = 1);

@@ -31,10 +31,11 @@ namespace api2 = cdr::api2;
 template <typename T>
 struct level_data {
     extents ext;
-    dbuf<T> so_buf, ci_buf, x_buf, b_buf, res_buf, sor_buf[2];
+    dbuf<T> so_buf, ci_buf, x_buf, b_buf, res_buf, sor_buf[2], lw_buf;
     stencil_op<T> A;
     prolong_op<T> P;  // stored on this (coarse) level; unused on level 0
     grid_func<T> x, b, res;
+    grid_func<T> lw;  // line-xy smoothing: the grid each sweep writes before the next reads it
     relax_stencil<T> SOR[2];
 };
 
@@ -268,7 +269,7 @@ private:
 
     template <typename T>
     static void alloc_level(level_data<T> &L, extents e, int nstncl, bool own_op, bool need_ci,
-                            bool need_xb, hipStream_t s) {
+                            bool need_xb, bool need_lw, hipStream_t s) {
         L.ext = e;
         const std::size_t np = e.total();
         if (own_op) {
@@ -287,6 +288,10 @@ private:
         }
         L.res_buf = dbuf<T>(np, s);
         L.res = grid_func<T>{L.res_buf.data(), e};
+        if (need_lw) {
+            L.lw_buf = dbuf<T>(np, s);
+            L.lw = grid_func<T>{L.lw_buf.data(), e};
+        }
         for (int k = 0; k < 2; ++k) {
             L.sor_buf[k] = dbuf<T>(np * 2, s);
             L.SOR[k] = relax_stencil<T>{L.sor_buf[k].data(), e};
@@ -307,10 +312,10 @@ private:
         for (int l = 0; l < nl; ++l) {
             extents e{cx + 2, cy + 2, nb};
             if (l == 0) {
-                alloc_level(L64_[0], e, nstncl, false, false, false, stream_);
+                alloc_level(L64_[0], e, nstncl, false, false, false, line_xy(), stream_);
                 L64_[0].A = stencil_op<double>{const_cast<double *>(so_dev), e, nstncl};
             } else {
-                alloc_level(L64_[l], e, 5, true, true, true, stream_);
+                alloc_level(L64_[l], e, 5, true, true, true, line_xy(), stream_);
             }
             cx = (int)((cx - 1) / 2.0 + 1);
             cy = (int)((cy - 1) / 2.0 + 1);
@@ -349,6 +354,8 @@ private:
         return abd_g_;
     }
 
+    bool line_xy() const { return set_.relaxation == CDR_RELAX_LINE_XY; }
+
     void setup_relax(level_data<double> &F) {
         switch (set_.relaxation) {
         case CDR_RELAX_POINT: km_.setup<kernels::point_relax<double>>(F.A, F.SOR[0]); break;
@@ -371,7 +378,7 @@ private:
         for (int l = 0; l < nl; ++l) {
             auto &D = L64_[l];
             auto &F = L32_[l];
-            alloc_level(F, D.ext, D.A.nstncl, true, l > 0, true, stream_);
+            alloc_level(F, D.ext, D.A.nstncl, true, l > 0, true, line_xy(), stream_);
             const std::size_t np = D.ext.total();
             k2::convert<double, float>(D.A.p, F.A.p, np * D.A.nstncl, stream_);
             if (l > 0) k2::convert<double, float>(D.P.p, F.P.p, np * 8, stream_);
@@ -390,13 +397,11 @@ private:
             case CDR_RELAX_LINE_X: km_.run<kernels::line_relax<T, relax_dir::x>>(L.A, x, b, L.SOR[0], L.res, d); break;
             case CDR_RELAX_LINE_Y: km_.run<kernels::line_relax<T, relax_dir::y>>(L.A, x, b, L.SOR[0], L.res, d); break;
             default:
-                if (d == Dir::DOWN) {
-                    km_.run<kernels::line_relax<T, relax_dir::x>>(L.A, x, b, L.SOR[0], L.res, d);
-                    km_.run<kernels::line_relax<T, relax_dir::y>>(L.A, x, b, L.SOR[1], L.res, d);
-                } else {
-                    km_.run<kernels::line_relax<T, relax_dir::y>>(L.A, x, b, L.SOR[1], L.res, d);
-                    km_.run<kernels::line_relax<T, relax_dir::x>>(L.A, x, b, L.SOR[0], L.res, d);
-                }
+                // x then y going down, y then x going up (include/cedar/multilevel.h:175-178,
+                // 204-207), both sweeps in the fused two-colour form
+                check(api2::relax_lines_xy<T>(L.A.p, L.A.nstncl, b.p, x.p, L.SOR[0].p, L.SOR[1].p,
+                                              L.lw.p, L.ext.ii, L.ext.jj, (int)d, cur_, L.ext.nb),
+                      "relax_lines_xy");
             }
         }
     }

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "bmg2_launch.hpp"
@@ -32,7 +33,8 @@ struct ext3 {
     std::size_t points() const { return (std::size_t)ii * jj * kk; }
 };
 
-// The planes of one family (xy, xz or yz) of one level: a batched 2D solver per colour.
+// The planes of one family (xy, xz or yz) of one level: a batched 2D solver per colour.  In
+// mixed precision the plane solvers also hold FP32 hierarchies and the sweep runs in FP32.
 class plane_family {
 public:
     plane_family(const double *so3, int ns3, ext3 e, int dir, const cdr_solver_settings &s2,
@@ -52,6 +54,10 @@ public:
             col_[c].so2 = dbuf<double>(np2 * ns2 * npl, st);
             col_[c].x2 = dbuf<double>(np2 * npl, st);
             col_[c].b2 = dbuf<double>(np2 * npl, st);
+            if (s2.precision == CDR_PREC_MIXED) {
+                col_[c].x2f = dbuf<float>(np2 * npl, st);
+                col_[c].b2f = dbuf<float>(np2 * npl, st);
+            }
             k3::copy_coeff(dir, so3, ns3, col_[c].so2.data(), first, npl, e.ii, e.jj, e.kk, st);
             check_hip(hipGetLastError(), "copy_coeff");
             col_[c].slv = std::make_unique<solver2d>(col_[c].so2.data(), ns2, n1_ - 2, n2_ - 2, npl,
@@ -60,32 +66,45 @@ public:
     }
 
     // relax_planes: DOWN relaxes the odd planes (1, 3, ...) first, UP the even ones first
-    void run(const double *so3, double *x, const double *b, Dir d) {
+    template <typename T>
+    void run(const T *so3, T *x, const T *b, Dir d) {
         for (int t = 0; t < 2; ++t) {
             const int c = d == Dir::DOWN ? t : 1 - t;
             auto &C = col_[c];
             if (C.npl == 0) continue;
             const int first = 1 + c;
-            k3::copy32<double>(dir_, x, C.x2.data(), first, C.npl, e_.ii, e_.jj, e_.kk, stream_);
-            k3::copy_rhs<double>(dir_, so3, ns3_, x, b, C.b2.data(), first, C.npl, e_.ii, e_.jj,
-                                 e_.kk, stream_);
+            T *x2 = C.template xbuf<T>(), *b2 = C.template bbuf<T>();
+            k3::copy32<T>(dir_, x, x2, first, C.npl, e_.ii, e_.jj, e_.kk, stream_);
+            k3::copy_rhs<T>(dir_, so3, ns3_, x, b, b2, first, C.npl, e_.ii, e_.jj, e_.kk, stream_);
             check_hip(hipGetLastError(), "plane gather");
             // the nested solve: its residual norms only feed its log and stop test, which with
             // max-iter cycles and no early stop leave x unchanged
-            for (int it = 0; it < maxiter_; ++it) C.slv->vcycle<double>(C.b2.data(), C.x2.data());
-            k3::copy23<double>(dir_, C.x2.data(), x, first, C.npl, e_.ii, e_.jj, e_.kk, stream_);
+            for (int it = 0; it < maxiter_; ++it) C.slv->template vcycle<T>(b2, x2);
+            k3::copy23<T>(dir_, x2, x, first, C.npl, e_.ii, e_.jj, e_.kk, stream_);
             check_hip(hipGetLastError(), "plane scatter");
         }
     }
-
-    int nplanes(int colour) const { return col_[colour].npl; }
-    solver2d *solver(int colour) { return col_[colour].slv.get(); }
 
 private:
     struct colour {
         int npl = 0;
         dbuf<double> so2, x2, b2;
+        dbuf<float> x2f, b2f;
         std::unique_ptr<solver2d> slv;
+        template <typename T>
+        T *xbuf() {
+            if constexpr (std::is_same<T, double>::value)
+                return x2.data();
+            else
+                return x2f.data();
+        }
+        template <typename T>
+        T *bbuf() {
+            if constexpr (std::is_same<T, double>::value)
+                return b2.data();
+            else
+                return b2f.data();
+        }
     };
     int dir_, ns3_;
     ext3 e_;
@@ -94,13 +113,28 @@ private:
     colour col_[2];
 };
 
+// one precision's arrays of a level
+template <typename T>
+struct lvl3v {
+    const T *so = nullptr;
+    T *ci = nullptr, *x = nullptr, *b = nullptr, *res = nullptr, *sor = nullptr;
+};
+
 struct level3 {
     ext3 e;
     int ns = 14;
     dbuf<double> so_buf, ci_buf, x_buf, b_buf, res_buf, sor_buf;
-    const double *so = nullptr;
-    double *ci = nullptr, *x = nullptr, *b = nullptr, *res = nullptr, *sor = nullptr;
+    dbuf<float> so_f, ci_f, x_f, b_f, res_f, sor_f;
+    lvl3v<double> d;
+    lvl3v<float> f;
     std::unique_ptr<plane_family> pl[3];
+    template <typename T>
+    lvl3v<T> &v() {
+        if constexpr (std::is_same<T, double>::value)
+            return d;
+        else
+            return f;
+    }
 };
 
 static bool is_plane(int r) { return r >= CDR_RELAX_PLANE_XY && r <= CDR_RELAX_PLANE_XYZ; }
@@ -125,6 +159,7 @@ public:
     const level3 &level(int l) const { return L_[l]; }
     double setup_seconds() const { return setup_s_; }
     double solve_seconds() const { return solve_s_; }
+    bool mixed() const { return set_.precision == CDR_PREC_MIXED; }
 
     // include/cedar/3d/solver.h:59-77
     static int compute_num_levels(int nx, int ny, int nz, int min_coarse) {
@@ -142,36 +177,47 @@ public:
         return ng;
     }
 
-    // one cycle of the configured type (V, or F: include/cedar/cycle/fcycle.h:38-83)
-    void vcycle(const double *b_dev, double *x_dev) {
-        if (L_.size() == 1)
-            coarse_solve(x_dev, b_dev);
-        else if (set_.cycle == CDR_CYCLE_F)
-            fmg(0, x_dev, b_dev);
-        else
-            ncycle(0, x_dev, b_dev);
-    }
+    // one FP64 cycle of the configured type (V, or F: include/cedar/cycle/fcycle.h:38-83)
+    void vcycle(const double *b_dev, double *x_dev) { cycle<double>(x_dev, b_dev); }
 
     void relax_planes(int lvl, int dir, double *x, const double *b, Dir d) {
         if (lvl < 0 || lvl >= nlevels() - 1 || dir < 0 || dir > 2 || !L_[lvl].pl[dir])
             throw error(CDR_ERR_ARG, "no planes of this family on this level");
-        L_[lvl].pl[dir]->run(L_[lvl].so, x, b, d);
+        L_[lvl].pl[dir]->run<double>(L_[lvl].d.so, x, b, d);
     }
 
+    // multilevel::solve (include/cedar/multilevel.h:278-298); mixed precision: FP64 iterative
+    // refinement around the FP32 cycle, x <- x + e and the next FP64 residual in one kernel
     int solve(const double *b_dev, double *x_dev, double *hist, int maxhist, int *niter) {
-        const level3 &F = L_[0];
+        level3 &F = L_[0];
         const ext3 e = F.e;
         check_hip(hipEventRecord(ev0_, stream_), "hipEventRecord");
-        k3::residual_norm(F.so, F.ns, b_dev, x_dev, F.res, e.ii, e.jj, e.kk, work_.data(),
-                          sumsq_.data(), stream_);
-        check_hip(hipGetLastError(), "residual_norm");
+        double *xa = nullptr, *xb = nullptr;
+        if (mixed()) {
+            xa = xa_.data();
+            xb = xb_.data();
+            k3::residual_ir(F.d.so, F.ns, b_dev, x_dev, nullptr, xa, F.f.b, e.ii, e.jj, e.kk,
+                            work_.data(), sumsq_.data(), stream_);
+        } else {
+            k3::residual_norm(F.d.so, F.ns, b_dev, x_dev, F.d.res, e.ii, e.jj, e.kk, work_.data(),
+                              sumsq_.data(), stream_);
+        }
+        check_hip(hipGetLastError(), "residual");
         const double res0 = std::sqrt(read_norm2());
         int it = 0, status = CDR_OK;
         for (int i = 0; i < set_.maxiter; ++i) {
-            vcycle(b_dev, x_dev);
-            k3::residual_norm(F.so, F.ns, b_dev, x_dev, F.res, e.ii, e.jj, e.kk, work_.data(),
-                              sumsq_.data(), stream_);
-            check_hip(hipGetLastError(), "residual_norm");
+            if (mixed()) {
+                k2::fill<float>(F.f.x, 0.0f, e.points(), stream_);
+                cycle<float>(F.f.x, F.f.b);
+                k3::residual_ir(F.d.so, F.ns, b_dev, xa, F.f.x, xb, F.f.b, e.ii, e.jj, e.kk,
+                                work_.data(), sumsq_.data(), stream_);
+                std::swap(xa, xb);
+            } else {
+                cycle<double>(x_dev, b_dev);
+                k3::residual_norm(F.d.so, F.ns, b_dev, x_dev, F.d.res, e.ii, e.jj, e.kk,
+                                  work_.data(), sumsq_.data(), stream_);
+            }
+            check_hip(hipGetLastError(), "residual");
             const double rel = std::sqrt(read_norm2()) / res0;
             if (hist && i < maxhist) hist[i] = rel;
             it = i + 1;
@@ -181,6 +227,9 @@ public:
             }
             if (rel < set_.tol) break;
         }
+        if (mixed())
+            check_hip(hipMemcpyAsync(x_dev, xa, e.points() * sizeof(double),
+                                     hipMemcpyDeviceToDevice, stream_), "copy x");
         check_hip(hipEventRecord(ev1_, stream_), "hipEventRecord");
         check_hip(hipEventSynchronize(ev1_), "hipEventSynchronize");
         float ms = 0.f;
@@ -194,7 +243,7 @@ private:
     cdr_solver3_settings set_;
     hipStream_t stream_;
     std::vector<level3> L_;
-    dbuf<double> abd_, bbd_, work_, sumsq_;
+    dbuf<double> abd_, bbd_, work_, sumsq_, xa_, xb_;
     dbuf<int> info_;
     int ldab_ = 0;
     double *host_norm_ = nullptr;
@@ -225,23 +274,23 @@ private:
             const std::size_t np = L.e.points();
             if (l == 0) {
                 L.ns = ns;
-                L.so = so_dev;  // held by reference
+                L.d.so = so_dev;  // held by reference
             } else {
                 L.ns = 14;
                 L.so_buf = dbuf<double>(np * 14, stream_);
                 L.ci_buf = dbuf<double>(np * 26, stream_);
                 L.x_buf = dbuf<double>(np, stream_);
                 L.b_buf = dbuf<double>(np, stream_);
-                L.so = L.so_buf.data();
-                L.ci = L.ci_buf.data();
-                L.x = L.x_buf.data();
-                L.b = L.b_buf.data();
+                L.d.so = L.so_buf.data();
+                L.d.ci = L.ci_buf.data();
+                L.d.x = L.x_buf.data();
+                L.d.b = L.b_buf.data();
             }
             L.res_buf = dbuf<double>(np, stream_);
-            L.res = L.res_buf.data();
+            L.d.res = L.res_buf.data();
             if (set_.relaxation == CDR_RELAX_POINT) {
                 L.sor_buf = dbuf<double>(np * 2, stream_);
-                L.sor = L.sor_buf.data();
+                L.d.sor = L.sor_buf.data();
             }
             cx = (int)((cx - 1) / 2.0 + 1);
             cy = (int)((cy - 1) / 2.0 + 1);
@@ -261,8 +310,9 @@ private:
         for (int l = 0; l < nl - 1; ++l) {
             level3 &F = L_[l];
             level3 &C = L_[l + 1];
-            k3::setup_interp(F.so, F.ns, C.ci, F.e.ii, F.e.jj, F.e.kk, C.e.ii, C.e.jj, C.e.kk, stream_);
-            k3::setup_galerkin(F.so, F.ns, C.ci, C.so_buf.data(), F.e.ii, F.e.jj, F.e.kk, C.e.ii,
+            k3::setup_interp(F.d.so, F.ns, C.d.ci, F.e.ii, F.e.jj, F.e.kk, C.e.ii, C.e.jj, C.e.kk,
+                             stream_);
+            k3::setup_galerkin(F.d.so, F.ns, C.d.ci, C.so_buf.data(), F.e.ii, F.e.jj, F.e.kk, C.e.ii,
                                C.e.jj, C.e.kk, stream_);
             check_hip(hipGetLastError(), "setup interp/galerkin");
             setup_relax(F);
@@ -270,7 +320,7 @@ private:
         {
             const level3 &C = L_[nl - 1];
             k2::fill<int>(info_.data(), 0, 1, stream_);
-            k3::setup_cg(C.so, C.ns, abd_.data(), ldab_, ec.ii, ec.jj, ec.kk, info_.data(), stream_);
+            k3::setup_cg(C.d.so, C.ns, abd_.data(), ldab_, ec.ii, ec.jj, ec.kk, info_.data(), stream_);
             check_hip(hipGetLastError(), "setup_cg");
             int info = 0;
             check_hip(hipMemcpyAsync(&info, info_.data(), sizeof(int), hipMemcpyDeviceToHost, stream_),
@@ -278,6 +328,7 @@ private:
             check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
             if (info != 0) throw error(CDR_ERR_NOT_SPD, "coarse-grid Cholesky failed");
         }
+        if (mixed()) build_f32();
         check_hip(hipEventRecord(ev1_, stream_), "hipEventRecord");
         check_hip(hipEventSynchronize(ev1_), "hipEventSynchronize");
         float ms = 0.f;
@@ -285,90 +336,142 @@ private:
         setup_s_ = ms * 1e-3;
     }
 
+    // FP32 copy of the hierarchy for the mixed-precision cycle (operators, CI, 1/SO_p); the
+    // coarsest band stays FP64
+    void build_f32() {
+        const std::size_t npf = L_[0].e.points();
+        xa_ = dbuf<double>(npf, stream_);
+        xb_ = dbuf<double>(npf, stream_);
+        for (int l = 0; l < nlevels(); ++l) {
+            level3 &L = L_[l];
+            const std::size_t np = L.e.points();
+            L.so_f = dbuf<float>(np * L.ns, stream_);
+            k2::convert<double, float>(L.d.so, L.so_f.data(), np * L.ns, stream_);
+            L.f.so = L.so_f.data();
+            if (l > 0) {
+                L.ci_f = dbuf<float>(np * 26, stream_);
+                k2::convert<double, float>(L.d.ci, L.ci_f.data(), np * 26, stream_);
+                L.f.ci = L.ci_f.data();
+            }
+            L.x_f = dbuf<float>(np, stream_);
+            L.b_f = dbuf<float>(np, stream_);
+            L.res_f = dbuf<float>(np, stream_);
+            L.f.x = L.x_f.data();
+            L.f.b = L.b_f.data();
+            L.f.res = L.res_f.data();
+            if (L.d.sor) {
+                L.sor_f = dbuf<float>(np * 2, stream_);
+                k2::convert<double, float>(L.d.sor, L.sor_f.data(), np * 2, stream_);
+                L.f.sor = L.sor_f.data();
+            }
+        }
+        check_hip(hipGetLastError(), "convert");
+    }
+
     // multilevel::setup_relax_helper (include/cedar/multilevel.h:135-157): plane-xyz sets up
     // xy, xz, yz
     void setup_relax(level3 &F) {
         const int r = set_.relaxation;
         if (r == CDR_RELAX_POINT) {
-            k3::setup_recip(F.so, F.ns, F.sor, F.e.ii, F.e.jj, F.e.kk, stream_);
+            k3::setup_recip(F.d.so, F.ns, F.d.sor, F.e.ii, F.e.jj, F.e.kk, stream_);
             check_hip(hipGetLastError(), "setup_recip");
             return;
         }
         cdr_solver_settings s2 = set_.plane;
-        s2.precision = CDR_PREC_F64;
+        s2.precision = set_.precision;
         s2.graph = 0;
         if (r == CDR_RELAX_PLANE_XY || r == CDR_RELAX_PLANE_XYZ)
-            F.pl[k3::XY] = std::make_unique<plane_family>(F.so, F.ns, F.e, k3::XY, s2, stream_);
+            F.pl[k3::XY] = std::make_unique<plane_family>(F.d.so, F.ns, F.e, k3::XY, s2, stream_);
         if (r == CDR_RELAX_PLANE_XZ || r == CDR_RELAX_PLANE_XYZ)
-            F.pl[k3::XZ] = std::make_unique<plane_family>(F.so, F.ns, F.e, k3::XZ, s2, stream_);
+            F.pl[k3::XZ] = std::make_unique<plane_family>(F.d.so, F.ns, F.e, k3::XZ, s2, stream_);
         if (r == CDR_RELAX_PLANE_YZ || r == CDR_RELAX_PLANE_XYZ)
-            F.pl[k3::YZ] = std::make_unique<plane_family>(F.so, F.ns, F.e, k3::YZ, s2, stream_);
+            F.pl[k3::YZ] = std::make_unique<plane_family>(F.d.so, F.ns, F.e, k3::YZ, s2, stream_);
     }
 
     // smoother sequencing (include/cedar/multilevel.h:165-222)
-    void smooth(level3 &L, double *x, const double *b, Dir d) {
+    template <typename T>
+    void smooth(level3 &L, T *x, const T *b, Dir d) {
         const int nrep = d == Dir::DOWN ? set_.nrelax_pre : set_.nrelax_post;
+        const lvl3v<T> &V = L.v<T>();
         for (int rep = 0; rep < nrep; ++rep) {
             switch (set_.relaxation) {
             case CDR_RELAX_POINT:
-                k3::relax_gs<double>(L.so, L.ns, b, x, L.sor, L.e.ii, L.e.jj, L.e.kk, (int)d, stream_);
+                k3::relax_gs<T>(V.so, L.ns, b, x, V.sor, L.e.ii, L.e.jj, L.e.kk, (int)d, stream_);
                 check_hip(hipGetLastError(), "relax_gs");
                 break;
-            case CDR_RELAX_PLANE_XY: L.pl[k3::XY]->run(L.so, x, b, d); break;
-            case CDR_RELAX_PLANE_XZ: L.pl[k3::XZ]->run(L.so, x, b, d); break;
-            case CDR_RELAX_PLANE_YZ: L.pl[k3::YZ]->run(L.so, x, b, d); break;
+            case CDR_RELAX_PLANE_XY: L.pl[k3::XY]->run<T>(V.so, x, b, d); break;
+            case CDR_RELAX_PLANE_XZ: L.pl[k3::XZ]->run<T>(V.so, x, b, d); break;
+            case CDR_RELAX_PLANE_YZ: L.pl[k3::YZ]->run<T>(V.so, x, b, d); break;
             default:
                 if (d == Dir::DOWN) {
-                    L.pl[k3::XY]->run(L.so, x, b, d);
-                    L.pl[k3::YZ]->run(L.so, x, b, d);
-                    L.pl[k3::XZ]->run(L.so, x, b, d);
+                    L.pl[k3::XY]->run<T>(V.so, x, b, d);
+                    L.pl[k3::YZ]->run<T>(V.so, x, b, d);
+                    L.pl[k3::XZ]->run<T>(V.so, x, b, d);
                 } else {
-                    L.pl[k3::XZ]->run(L.so, x, b, d);
-                    L.pl[k3::YZ]->run(L.so, x, b, d);
-                    L.pl[k3::XY]->run(L.so, x, b, d);
+                    L.pl[k3::XZ]->run<T>(V.so, x, b, d);
+                    L.pl[k3::YZ]->run<T>(V.so, x, b, d);
+                    L.pl[k3::XY]->run<T>(V.so, x, b, d);
                 }
             }
         }
     }
 
-    void coarse_solve(double *x, const double *b) {
+    template <typename T>
+    void coarse_solve(T *x, const T *b) {
         const ext3 ec = L_.back().e;
-        k3::solve_cg<double>(x, b, abd_.data(), ldab_, bbd_.data(), ec.ii, ec.jj, ec.kk, stream_);
+        k3::solve_cg<T>(x, b, abd_.data(), ldab_, bbd_.data(), ec.ii, ec.jj, ec.kk, stream_);
         check_hip(hipGetLastError(), "solve_cg");
     }
 
-    void fmg(int l, double *x, const double *b) {
+    template <typename T>
+    void cycle(T *x, const T *b) {
+        if (L_.size() == 1)
+            coarse_solve<T>(x, b);
+        else if (set_.cycle == CDR_CYCLE_F)
+            fmg<T>(0, x, b);
+        else
+            ncycle<T>(0, x, b);
+    }
+
+    // fcycle::fmg_cycle (include/cedar/cycle/fcycle.h:49-83)
+    template <typename T>
+    void fmg(int l, T *x, const T *b) {
         if (l == nlevels() - 1) {
-            coarse_solve(x, b);
+            coarse_solve<T>(x, b);
             return;
         }
         level3 &F = L_[l];
         level3 &C = L_[l + 1];
-        k3::restrict_<double>(C.ci, b, C.b, F.e.ii, F.e.jj, F.e.kk, C.e.ii, C.e.jj, C.e.kk, stream_);
-        fmg(l + 1, C.x, C.b);
-        k2::fill<double>(x, 0.0, F.e.points(), stream_);
-        k2::fill<double>(F.res, 0.0, F.e.points(), stream_);
-        k3::interp_add<double>(x, C.x, F.res, F.so, F.ns, C.ci, C.e.ii, C.e.jj, C.e.kk, F.e.ii, F.e.jj,
-                               F.e.kk, stream_);
+        lvl3v<T> &fv = F.v<T>();
+        lvl3v<T> &cv = C.v<T>();
+        k3::restrict_<T>(cv.ci, b, cv.b, F.e.ii, F.e.jj, F.e.kk, C.e.ii, C.e.jj, C.e.kk, stream_);
+        fmg<T>(l + 1, cv.x, cv.b);
+        k2::fill<T>(x, T(0), F.e.points(), stream_);
+        k2::fill<T>(fv.res, T(0), F.e.points(), stream_);
+        k3::interp_add<T>(x, cv.x, fv.res, fv.so, F.ns, cv.ci, C.e.ii, C.e.jj, C.e.kk, F.e.ii, F.e.jj,
+                          F.e.kk, stream_);
         check_hip(hipGetLastError(), "fmg");
-        ncycle(l, x, b);
+        ncycle<T>(l, x, b);
     }
 
-    void ncycle(int l, double *x, const double *b) {
+    template <typename T>
+    void ncycle(int l, T *x, const T *b) {
         level3 &F = L_[l];
         level3 &C = L_[l + 1];
-        smooth(F, x, b, Dir::DOWN);
-        k3::residual<double>(F.so, F.ns, b, x, F.res, F.e.ii, F.e.jj, F.e.kk, stream_);
-        k3::restrict_<double>(C.ci, F.res, C.b, F.e.ii, F.e.jj, F.e.kk, C.e.ii, C.e.jj, C.e.kk, stream_);
-        k2::fill<double>(C.x, 0.0, C.e.points(), stream_);
+        lvl3v<T> &fv = F.v<T>();
+        lvl3v<T> &cv = C.v<T>();
+        smooth<T>(F, x, b, Dir::DOWN);
+        k3::residual<T>(fv.so, F.ns, b, x, fv.res, F.e.ii, F.e.jj, F.e.kk, stream_);
+        k3::restrict_<T>(cv.ci, fv.res, cv.b, F.e.ii, F.e.jj, F.e.kk, C.e.ii, C.e.jj, C.e.kk, stream_);
+        k2::fill<T>(cv.x, T(0), C.e.points(), stream_);
         if (l + 1 == nlevels() - 1)
-            coarse_solve(C.x, C.b);
+            coarse_solve<T>(cv.x, cv.b);
         else
-            ncycle(l + 1, C.x, C.b);
-        k3::interp_add<double>(x, C.x, F.res, F.so, F.ns, C.ci, C.e.ii, C.e.jj, C.e.kk, F.e.ii, F.e.jj,
-                               F.e.kk, stream_);
+            ncycle<T>(l + 1, cv.x, cv.b);
+        k3::interp_add<T>(x, cv.x, fv.res, fv.so, F.ns, cv.ci, C.e.ii, C.e.jj, C.e.kk, F.e.ii, F.e.jj,
+                          F.e.kk, stream_);
         check_hip(hipGetLastError(), "cycle");
-        smooth(F, x, b, Dir::UP);
+        smooth<T>(F, x, b, Dir::UP);
     }
 };
 
@@ -409,6 +512,7 @@ void cdr_hip_default_settings3(cdr_solver3_settings *s) {
     s->tol = 1e-8;
     s->min_coarse = 3;
     s->cycle = CDR_CYCLE_V;
+    s->precision = CDR_PREC_F64;
     // plane-config defaults (src/kernel_params.cc:83-89): line-xy, max-iter 1
     cdr_hip_default_settings(&s->plane);
     s->plane.relaxation = CDR_RELAX_LINE_XY;
@@ -423,7 +527,8 @@ int cdr_hip_solver3d_create(const double *so_dev, int nstncl, int nx, int ny, in
         s->nrelax_pre < 0 || s->nrelax_post < 0 || s->maxiter < 0 || s->min_coarse < 2 ||
         s->plane.relaxation < CDR_RELAX_POINT || s->plane.relaxation > CDR_RELAX_LINE_XY ||
         s->plane.maxiter < 0 || s->plane.nrelax_pre < 0 || s->plane.nrelax_post < 0 ||
-        s->plane.min_coarse < 2 || (s->cycle != CDR_CYCLE_V && s->cycle != CDR_CYCLE_F))
+        s->plane.min_coarse < 2 || (s->cycle != CDR_CYCLE_V && s->cycle != CDR_CYCLE_F) ||
+        (s->precision != CDR_PREC_F64 && s->precision != CDR_PREC_MIXED))
         return CDR_ERR_ARG;
     *out = nullptr;
     int ndev = 0;
@@ -472,8 +577,8 @@ int cdr_hip_solver3d_level(const cdr_hip_solver3d *slv, int lvl, int *nx, int *n
     if (nx) *nx = L.e.ii - 2;
     if (ny) *ny = L.e.jj - 2;
     if (nz) *nz = L.e.kk - 2;
-    if (so) *so = L.so;
-    if (ci) *ci = L.ci;
+    if (so) *so = L.d.so;
+    if (ci) *ci = L.d.ci;
     return CDR_OK;
 }
 

@@ -102,6 +102,18 @@ int cdr_hip_relax_lines_y2d_f32(const float *so, int nstncl, const float *qf, fl
                                 const float *sor, float *b, int ii, int jj, int updown,
                                 void *stream);
 
+/* One line-xy smoothing step: x-lines then y-lines (DOWN) or y then x (UP), the composition
+ * Cedar's smoother makes of the two line kernels (include/cedar/multilevel.h:175-178, 204-207).
+ * Each sweep relaxes both colours in one launch, out of place through `work` (one grid of
+ * ii*jj, device, distinct from q; its contents on return are unspecified), so that every
+ * coupling plane is read once per sweep.  Results are bit for bit those of
+ * cdr_hip_relax_lines_x2d followed by cdr_hip_relax_lines_y2d (or y then x). */
+int cdr_hip_relax_lines_xy2d_f64(const double *so, int nstncl, const double *qf, double *q,
+                                 const double *sorx, const double *sory, double *work, int ii,
+                                 int jj, int updown, void *stream);
+int cdr_hip_relax_lines_xy2d_f32(const float *so, int nstncl, const float *qf, float *q,
+                                 const float *sorx, const float *sory, float *work, int ii, int jj,
+                                 int updown, void *stream);
 /* qc = P^T q.  Replaces BMG2_SymStd_restrict (src/2d/restrict.cc:6-7). */
 int cdr_hip_restrict2d_f64(const double *ci, const double *q, double *qc, int iif, int jjf,
                            int iic, int jjc, void *stream);
@@ -281,6 +293,8 @@ typedef struct {
                                   graph fields are ignored, and its max-iter cycles always run
                                   (no per-plane stop test) */
     int cycle;        /* CDR_CYCLE_V (default) or CDR_CYCLE_F */
+    int precision;    /* CDR_PREC_F64 (default) or CDR_PREC_MIXED: FP32 3D cycle and FP32 plane
+                         solves inside FP64 iterative refinement */
 } cdr_solver3_settings;
 
 typedef struct cdr_hip_solver3d cdr_hip_solver3d;

@@ -76,3 +76,23 @@ def test_batched_not_spd_reported(C):
     with pytest.raises(C.CedarHipError) as e:
         C.Solver(dev(so), relaxation="line-y")
     assert e.value.status == 3
+
+
+def test_batchset_mixed_sizes_equal_individual(C):
+    """Mixed sizes grouped into batched solvers: each problem gets its individual result."""
+    from cedar_amd.batch import BatchSet
+    sizes = [33, 65, 33, 47, 65, 33]
+    ops, rhs = [], []
+    for k, n in enumerate(sizes):
+        so, b, _ = G.heterogeneous_9pt(n, seed=70 + k, block=8, sigma=1.0 + 0.2 * k)
+        ops.append(dev(so))
+        rhs.append(dev(b))
+    kw = dict(relaxation="line-xy", nrelax_pre=1, nrelax_post=1, maxiter=40, tol=1e-10,
+              precision="mixed")
+    bs = BatchSet(ops, **kw)
+    assert len(bs.groups) == 3
+    xs, hs = bs.solve(rhs)
+    for k in range(len(sizes)):
+        x1, h1 = C.Solver(ops[k], **kw).solve(rhs[k])
+        assert np.array_equal(hs[k], h1), k
+        assert torch.equal(xs[k], x1), k

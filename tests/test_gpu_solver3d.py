@@ -143,3 +143,21 @@ def test_fcycle_matches_oracle(C, relax):
     assert len(hg) == len(ho) == 3
     assert np.abs(hg - ho).max() <= 1e-12
     assert rel_l2(xg, xo) <= 1e-10
+
+
+@pytest.mark.parametrize("relax,n", [("point", 31), ("plane-xy", 31), ("plane-xyz", 15)])
+def test_mixed_precision_3d(C, relax, n):
+    """FP32 3D cycle (FP32 plane solves) inside FP64 refinement: reaches the FP64 tolerance in
+    about the FP64 cycle count, solution within 1e-9 of the FP64 one."""
+    if relax == "point":
+        so, b, _ = G.poisson3_sine(n)
+    else:
+        so, b, _ = G.heterogeneous_7pt(n, seed=3, block=4)
+    kw = dict(relaxation=relax, nrelax_pre=1, nrelax_post=1, maxiter=40, tol=1e-10)
+    g64 = C.Solver3(dev(so), **kw)
+    x64, h64 = g64.solve(dev(b))
+    g32 = C.Solver3(dev(so), precision="mixed", **kw)
+    x32, h32 = g32.solve(dev(b))
+    assert h32[-1] < 1e-10
+    assert abs(len(h32) - len(h64)) <= 1
+    assert rel_l2(x32.cpu().numpy(), x64.cpu().numpy()) < 1e-9
