@@ -296,3 +296,20 @@ def poisson3_sine(n):
     exact = np.zeros_like(b)
     exact[1:-1, 1:-1, 1:-1] = s[1:-1, 1:-1, 1:-1]
     return so, b, exact
+
+
+def box_filter(a, w=5):
+    """w x w moving average with edge replication (same shape as a)."""
+    r = w // 2
+    p = np.pad(a, r, mode="edge")
+    c = np.cumsum(np.cumsum(p, axis=0), axis=1)
+    c = np.pad(c, ((1, 0), (1, 0)))
+    n0, n1 = a.shape
+    s = c[w:w + n0, w:w + n1] - c[:n0, w:w + n1] - c[w:w + n0, :n1] + c[:n0, :n1]
+    return s / (w * w)
+
+
+def smooth_theta(n, seed):
+    """Per-cell log-coefficients on (n+1) x (n+1) cells: theta ~ N(0, 1) smoothed by a 5-cell
+    box filter."""
+    return box_filter(normal(seed, (n + 1) * (n + 1)).reshape(n + 1, n + 1), 5)

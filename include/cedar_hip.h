@@ -320,6 +320,20 @@ int cdr_hip_solver3d_level(const cdr_hip_solver3d *slv, int lvl, int *nx, int *n
 int cdr_hip_solver3d_times(const cdr_hip_solver3d *slv, double *setup_s, double *solve_s);
 void cdr_hip_solver3d_destroy(cdr_hip_solver3d *slv);
 
+/* ---------------------------------------------------------------------------------------
+ * Differentiated 2D diffusion solves (no counterpart in the reference).  kcell holds nb blocks
+ * of (ny+1) x (nx+1) per-cell coefficients (cell (cj, ci) at cj*(nx+1) + ci, its corners the
+ * nodes (ci, cj), (ci+1, cj), (ci+1, cj+1), (ci, cj+1) of the ghost-inclusive node grid).
+ * ------------------------------------------------------------------------------------- */
+/* 9-point bilinear finite-element operator of -div(k grad u), Dirichlet couplings dropped,
+ * written as nb stencils {c, w, s, sw, nw} of (nx+2) x (ny+2) */
+int cdr_hip_q1_assemble2d_f64(const double *kcell, double *so, int nx, int ny, int nb,
+                              void *stream);
+/* dL/dk per cell from the state u and the adjoint lambda = A^{-1} dL/du:
+ * grad_k[e] = -lambda_e^T K_e u_e / k_e over the cell's corners (boundary corners count 0) */
+int cdr_hip_q1_gradient2d_f64(const double *u, const double *lam, double *grad_k, int nx, int ny,
+                              int nb, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
