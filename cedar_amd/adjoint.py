@@ -5,7 +5,7 @@ Forward: the 9-point bilinear finite-element operator is assembled on the GPU fr
 (cdr_hip_q1_assemble2d_f64) and solved with the batched multigrid solver (mixed precision: an
 FP32 V-cycle inside FP64 iterative refinement, or plain FP64).  Backward: A is symmetric, so the
 adjoint lambda = A^{-1} dL/du reuses the same hierarchy; the per-cell gradient
-dL/dk_e = -lambda_e^T K_e u_e / k_e comes from a hand-written kernel (cdr_hip_q1_gradient2d_f64),
+dL/dk_e = -lambda_e^T K u_e (element matrix k_e K) comes from a hand-written kernel (cdr_hip_q1_gradient2d_f64),
 and dL/dtheta = k dL/dk, dL/df = lambda.
 """
 import torch
