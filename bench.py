@@ -336,6 +336,8 @@ def main():
     f64, mix = results["f64"], results["mixed"]
     km = kern["mixed"]
     dom = max(("x", km["x_sweep_s"]), ("y", km["y_sweep_s"]), key=lambda t: t[1])
+    shapes = {(r["dir"], r["fp"]): r["kernel"] for r in cedar_amd.cdr2.autotune_report()
+              if r["ii"] == ii and r["jj"] == jj and r["nb"] == 1 and r["stencil"] == 9}
     achieved = km["bytes_sweep"] / 2 / (dom[1] / 2) / 1e9  # per launch: one colour = half
     line = {
         "metric": "time-to-1e-10 rel. residual (s) + HBM GB/s %peak; mixed vs FP64 on same GPU",
@@ -363,7 +365,8 @@ def main():
         "speedup_mixed_vs_fp64": f64["seconds"] / mix["seconds"],
         "roofline": {
             "bound": "hbm",
-            "kernel": f"relax_lines_k<float,9pt,{dom[0]}> fine level (one colour per launch)",
+            "kernel": f"{shapes.get((dom[0], 32), dom[0] + '-lines')} <float, 9-point>, fine "
+                      "level, one colour per launch (shape picked by the autotuner)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -373,6 +376,7 @@ def main():
             "avg_launch_s": dom[1] / 2,
         },
         "kernels": kern,
+        "line_shapes": {f"{d}_f{fp}": k for (d, fp), k in shapes.items()},
         "cpu_baseline": cpu,
         "gen_s": gen_s,
     }

@@ -43,6 +43,10 @@ _P = _vp
 PROTOTYPES = {
     "cdr_hip_status_string": (ctypes.c_char_p, [_i]),
     "cdr_hip_version": (ctypes.c_char_p, []),
+    "cdr_hip_autotune": (_i, [_i]),
+    "cdr_hip_autotune_report": (_i, [ctypes.c_char_p, _i]),
+    "cdr_hip_line_shapes2d": (_i, [_i, _i, _i, _i, _i]),
+    "cdr_hip_set_line_shape2d": (_i, [_i, _i, _i, _i, _i, _i, _i]),
     "cdr_hip_norm_work2d": (_sz, [_i, _i]),
     "cdr_hip_residual2d_f64": (_i, [_P, _i, _P, _P, _P, _i, _i, _vp]),
     "cdr_hip_residual2d_f32": (_i, [_P, _i, _P, _P, _P, _i, _i, _vp]),

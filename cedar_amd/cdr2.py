@@ -144,15 +144,26 @@ def relax_lines(so, qf, q, sor, updown, direction):
     return q
 
 
-def relax_lines_xy(so, qf, q, sorx, sory, updown, work=None):
-    """One line-xy smoothing step (x then y going down, y then x going up), both colours of each
-    sweep per launch; `work` is a scratch grid like q (allocated when not given)."""
+def relax_lines_xy(so, qf, q, sorx, sory, updown):
+    """One line-xy smoothing step: x then y going down, y then x going up."""
     so, qf, q, sorx, sory = map(_t, (so, qf, q, sorx, sory))
-    if work is None:
-        work = torch.empty_like(q)
     _abi.call(f"cdr_hip_relax_lines_xy2d_{_sfx(q)}", _p(so), so.shape[-3], _p(qf), _p(q), _p(sorx),
-              _p(sory), _p(work), q.shape[-1], q.shape[-2], int(updown), _stream())
+              _p(sory), None, q.shape[-1], q.shape[-2], int(updown), _stream())
     return q
+
+
+def autotune(on):
+    """Turns line-kernel shape tuning on or off (process-wide); returns the previous setting."""
+    return bool(_abi.load().cdr_hip_autotune(1 if on else 0))
+
+
+def autotune_report():
+    """The recorded line-kernel shape choices (list of dicts)."""
+    import json
+    n = _abi.load().cdr_hip_autotune_report(None, 0)
+    buf = ctypes.create_string_buffer(n + 1)
+    _abi.load().cdr_hip_autotune_report(buf, n + 1)
+    return json.loads(buf.value.decode())
 
 
 def restrict(ci, q, qc):

@@ -18,7 +18,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
+#include <atomic>
 #include <cstdint>
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <string>
 
 namespace cdr {
 namespace k2 {
@@ -453,21 +459,10 @@ __device__ inline int xcd_tile(int ntiles) {
     return (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
 }
 
-// Forward-substitution results of a line stay in registers between the two passes (NCH chunks
-// of 64, loops fully unrolled so the array is register-allocated; the library is built with a
-// raised unroll threshold for this).  Chunks are handled in groups of G: the G scans of a group
-// are independent (only the final carry application is sequential), which gives the scheduler
-// instruction-level parallelism and, for y-lines, one LDS exchange per group.  The raw inputs of
-// the group LA chunks ahead are loaded while the current group is processed.
-template <typename T>
-constexpr int line_group() { return sizeof(T) == 8 ? 2 : 4; }
-template <typename T>
-constexpr int line_lookahead() { return sizeof(T) == 8 ? 2 : 4; }
-
-// raw operands of one point of a line: right-hand side terms, factor entries
+// raw operands of the right-hand side of one point of a line
 template <typename T>
 struct line_raw {
-    T qf, s0, q0, s1, q1, sw0, q0m, nw0, q0p, nw1, q1m, sw1, q1p, e, d;
+    T qf, s0, q0, s1, q1, sw0, q0m, nw0, q0p, nw1, q1m, sw1, q1p;
 };
 
 // Right-hand side of one point in the reference's order.  X-lines: s0/q0 = coupling to the row
@@ -514,13 +509,6 @@ __device__ inline void line_load(line_raw<T> &r, const T *__restrict__ so, const
     }
 }
 
-template <typename T>
-__device__ inline void line_zero(line_raw<T> &r) {
-    r.qf = r.s0 = r.q0 = r.s1 = r.q1 = r.sw0 = r.q0m = r.nw0 = r.q0p = r.nw1 = r.q1m = r.sw1 =
-        r.q1p = r.e = T(0);
-    r.d = T(1);
-}
-
 // barrier that orders LDS only: prefetched global loads stay in flight across it
 __device__ inline void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
@@ -528,317 +516,215 @@ __device__ inline void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// x-lines: one wavefront relaxes row j, reading the neighbouring rows from qin and writing the
-// row to qout (qin == qout for an in-place sweep).  No workgroup barrier inside.
-template <typename T, bool NINE, int NCH>
-__device__ inline void xline_solve(const T *__restrict__ so, const T *__restrict__ qf,
-                                   const T *qin, T *qout, const T *__restrict__ sor, int ii,
-                                   int jj, int j) {
-    constexpr int LA = line_lookahead<T>();
-    const int lane = threadIdx.x % WAVE;
-    const std::size_t P = (std::size_t)ii * jj;
+// ------------------------------------------------------------------------------------------
+// Segmented line solves.  W wavefronts share a line: wavefront w owns the CPW chunks
+// [w*CPW, (w+1)*CPW) of 64 elements.  Every operand of a wavefront's chunks is loaded before the
+// first scan, so no load latency sits inside the carry chain, and at most CPW chunks live in
+// registers; W wavefronts per line put W times as many wavefronts in flight as there are lines.
+//
+// The carries between chunks are evaluated in exactly the single-wavefront order: chunk m's
+// scanned map (a_m, b_m) at its last element goes to LDS, and the carry into chunk m is
+// c_m = a_{m-1} c_{m-1} + b_{m-1} from c_0 = 0, the same operations the one-wavefront loop
+// performs.  Results are therefore bitwise identical for every (CPW, W) and workgroup shape, so
+// the shape can be picked by timing alone.
+// ------------------------------------------------------------------------------------------
+template <typename T>
+struct seg_map {
+    T a, b;  // carry_out = a * carry_in + b
+};
+
+// Forward recurrence y_k = a_k y_{k-1} + b_k over the wavefront's chunks; on exit
+// b[u] = y_k / D(u) with D(u) the pivots of chunk u (a functor, loaded after the scans).
+// xch: this line's LDS slots, one per chunk of the line.
+template <typename T, int CPW, int W, typename DF>
+__device__ inline void seg_forward(T (&a)[CPW], T (&b)[CPW], DF dval, seg_map<T> *xch, int w) {
+#pragma unroll
+    for (int u = 0; u < CPW; ++u) scan_affine_dpp(a[u], b[u]);
+    T carry = T(0);
+    if (W > 1) {
+        if (w < W - 1) {  // chunk-end maps of all but the last segment
+#pragma unroll
+            for (int u = 0; u < CPW; ++u) {
+                const seg_map<T> m{last_lane(a[u]), last_lane(b[u])};
+                if (threadIdx.x % WAVE == 0) xch[w * CPW + u] = m;
+            }
+        }
+        lds_barrier();
+        for (int m = 0; m < w * CPW; ++m) carry = xch[m].a * carry + xch[m].b;
+    }
+#pragma unroll
+    for (int u = 0; u < CPW; ++u) {
+        const T y = a[u] * carry + b[u];
+        carry = last_lane(y);
+        b[u] = y / dval(u);
+    }
+}
+
+// Backward recurrence x_k = a_k x_{k+1} + b_k, lanes in reversed element order (lane l of chunk
+// c holds element 64c + 63 - l); the carry runs from the line's last chunk to its first.  On
+// exit b[u] = x.
+template <typename T, int CPW, int W>
+__device__ inline void seg_backward(T (&a)[CPW], T (&b)[CPW], seg_map<T> *xch, int w) {
+#pragma unroll
+    for (int u = 0; u < CPW; ++u) scan_affine_dpp(a[u], b[u]);
+    T carry = T(0);
+    if (W > 1) {
+        if (w > 0) {
+#pragma unroll
+            for (int u = 0; u < CPW; ++u) {
+                const seg_map<T> m{last_lane(a[u]), last_lane(b[u])};
+                if (threadIdx.x % WAVE == 0) xch[w * CPW + u] = m;
+            }
+        }
+        lds_barrier();
+        for (int m = W * CPW - 1; m >= (w + 1) * CPW; --m) carry = xch[m].a * carry + xch[m].b;
+    }
+#pragma unroll
+    for (int u = CPW - 1; u >= 0; --u) {
+        const T x = a[u] * carry + b[u];
+        carry = last_lane(x);
+        b[u] = x;
+    }
+}
+
+// x-lines of one colour, in place: L lines per workgroup, W wavefronts per line.
+template <typename T, bool NINE, int CPW, int W, int L>
+__global__ __launch_bounds__(L *W * 64) void relax_xlines_seg_k(const T *__restrict__ so,
+                                                               const T *__restrict__ qf,
+                                                               T *__restrict__ q,
+                                                               const T *__restrict__ sor, int ii,
+                                                               int jj, int first, int nlines,
+                                                               int ntiles) {
+    __shared__ seg_map<T> xch[2][L][W > 1 ? W * CPW : 1];
+    const int tile = xcd_tile(ntiles);
+    if (tile >= ntiles) return;  // uniform over the workgroup
+    const int wl = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int slot = wl / W, w = wl % W;
+    const int line = tile * L + slot;
+    const bool valid = line < nlines;  // invalid wavefronts solve a copy of the last line
+    const int j = first + 2 * min(line, nlines - 1);
+    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.y;
+    so += bz * (NINE ? 5 : 3) * P;
+    qf += bz * P;
+    q += bz * P;
+    sor += bz * 2 * P;
     const int n = ii - 2;
-    const int nact = (n + WAVE - 1) / WAVE;
     const std::size_t row = (std::size_t)j * ii;
     const T *D = sor + row + 1;      // D[k]
     const T *E = sor + P + row + 2;  // E[k], k < n-1
-    // Loads are branch-free: indices past the end of the line are clamped onto its last point
-    // (valid memory) and those lanes are neutralised where their values are used, so no
-    // exec-masked region splits the prefetch from its use.
-    auto fetch = [&](int c, line_raw<T> &r) {
-        const int k = min(c * WAVE + lane, n - 1);
-        line_load<T, NINE, 0>(r, so, qf, qin, P, ii, row + k + 1);
-        r.e = E[max(k - 1, 0)];
-        r.d = D[k];
-    };
-    constexpr int G = NCH < line_group<T>() ? NCH : line_group<T>();
-    line_raw<T> pf[LA];
+    T a[CPW], b[CPW];
 #pragma unroll
-    for (int c = 0; c < LA; ++c)
-        if (c < NCH) fetch(c, pf[c]);
-    T cst[NCH];
-    T carry = T(0);
-#pragma unroll
-    for (int g = 0; g < NCH; g += G) {
-        if (g < nact) {
-            T a[G], b[G], d[G];
-#pragma unroll
-            for (int u = 0; u < G; ++u) {
-                const int c = g + u;
-                b[u] = line_rhs<T, NINE>(pf[c % LA]);
-                a[u] = (c * WAVE + lane > 0) ? -pf[c % LA].e : T(0);
-                d[u] = pf[c % LA].d;
-                if (c + LA < NCH) fetch(c + LA, pf[c % LA]);
-            }
-#pragma unroll
-            for (int u = 0; u < G; ++u) scan_affine_dpp(a[u], b[u]);
-#pragma unroll
-            for (int u = 0; u < G; ++u) {
-                const T y = a[u] * carry + b[u];
-                carry = last_lane(y);
-                cst[g + u] = y / d[u];
-            }
-        }
+    for (int u = 0; u < CPW; ++u) {
+        const int k = (w * CPW + u) * WAVE + lane;
+        const int kc = min(k, n - 1);
+        line_raw<T> r;
+        line_load<T, NINE, 0>(r, so, qf, q, P, ii, row + kc + 1);
+        b[u] = line_rhs<T, NINE>(r);
+        a[u] = (k > 0) ? -E[max(kc - 1, 0)] : T(0);
     }
-    // backward pass in reversed lane order (lane l holds element c*64 + 63 - l), so that the
-    // suffix recurrence x_k = c_k - E_k x_{k+1} is again a prefix scan over lanes
+    seg_forward<T, CPW, W>(
+        a, b, [&](int u) { return D[min((w * CPW + u) * WAVE + lane, n - 1)]; }, xch[0][slot], w);
     const int rl = WAVE - 1 - lane;
-    auto fetch_e = [&](int c) { return E[min(c * WAVE + rl, max(n - 2, 0))]; };
-    // chunks at or above nactg belong to groups that are skipped, so a chunk whose prefetch
-    // would have come from such a group loads its entry directly
-    const int nactg = min(NCH, ((nact + G - 1) / G) * G);
-    T pe[LA];
-    carry = T(0);
 #pragma unroll
-    for (int g = NCH - G; g >= 0; g -= G) {
-        if (g < nact) {
-            T a[G], b[G];
+    for (int u = 0; u < CPW; ++u) {
+        const int k = (w * CPW + u) * WAVE + rl;
+        const T c = __shfl(b[u], rl, WAVE);
+        const T e = E[min(k, max(n - 2, 0))];
+        a[u] = (k < n - 1) ? -e : T(0);
+        a[u] = (k < n) ? a[u] : T(1);
+        b[u] = (k < n) ? c : T(0);
+    }
+    seg_backward<T, CPW, W>(a, b, xch[1][slot], w);
+    if (valid) {
 #pragma unroll
-            for (int u = G - 1; u >= 0; --u) {
-                const int c = g + u;
-                const T e = (c + LA >= nactg) ? fetch_e(c) : pe[c % LA];
-                if (c - LA >= 0) pe[c % LA] = fetch_e(c - LA);
-                const int k = c * WAVE + rl;
-                b[u] = __shfl(cst[c], rl, WAVE);
-                a[u] = (k < n - 1) ? -e : T(0);
-                a[u] = (k < n) ? a[u] : T(1);
-                b[u] = (k < n) ? b[u] : T(0);
-            }
-#pragma unroll
-            for (int u = 0; u < G; ++u) scan_affine_dpp(a[u], b[u]);
-#pragma unroll
-            for (int u = G - 1; u >= 0; --u) {
-                const int c = g + u;
-                const T x = a[u] * carry + b[u];
-                carry = last_lane(x);
-                const int k = c * WAVE + rl;
-                if (k < n) qout[row + k + 1] = x;
-            }
+        for (int u = 0; u < CPW; ++u) {
+            const int k = (w * CPW + u) * WAVE + rl;
+            if (k < n) q[row + k + 1] = b[u];
         }
     }
 }
 
-
-// x-lines, one colour per launch, in place: one wavefront per line, four lines per workgroup.
-template <typename T, bool NINE, int NCH>
-__global__ __launch_bounds__(256) void relax_xlines_reg_k(const T *__restrict__ so,
-                                                          const T *__restrict__ qf,
-                                                          T *__restrict__ q,
-                                                          const T *__restrict__ sor, int ii,
-                                                          int jj, int first, int nlines,
-                                                          int ntiles) {
-    constexpr int WPB = 4;
-    const int tile = xcd_tile(ntiles);
-    if (tile >= ntiles) return;
-    const int wave = threadIdx.x / WAVE;
-    const int line = tile * WPB + wave;
-    if (line >= nlines) return;  // whole wavefront; no workgroup barrier in this kernel
-    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.y;
-    xline_solve<T, NINE, NCH>(so + bz * (NINE ? 5 : 3) * P, qf + bz * P, q + bz * P, q + bz * P,
-                              sor + bz * 2 * P, ii, jj, first + 2 * line);
-}
-
-// Both colours of an x-line sweep in one launch, out of place (qin -> qout).  Rows are dealt in
-// bands of 2B: band t covers rows R0 .. R0+2B-1, R0 = 1 + 2Bt.  Phase A: wavefront w relaxes the
-// first-colour row R0 - 1 + aoff + 2w (w = 0..B) from the old rows in qin -- B + 1 rows, the
-// outermost one shared with the neighbouring band and computed by both (identical inputs, so
-// identical values); phase B: wavefront w < B relaxes the second-colour row R0 + aoff + 2w from
-// the new first-colour rows.  aoff = 0 when the even rows come first (DOWN), 1 otherwise.
-// Each coupling plane row is read from HBM once per sweep instead of once per colour.  qout's
-// ghost rows and columns must equal qin's.
-template <typename T, bool NINE, int NCH, int B>
-__global__ __launch_bounds__((B + 1) * 64) void relax_xlines_pair_k(const T *__restrict__ so,
-                                                                   const T *__restrict__ qf,
-                                                                   const T *qin, T *qout,
-                                                                   const T *__restrict__ sor,
-                                                                   int ii, int jj, int aoff,
-                                                                   int nbands) {
-    const int band = xcd_tile(nbands);
-    if (band >= nbands) return;  // uniform over the workgroup
-    const int wave = threadIdx.x / WAVE;
-    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.y;
-    so += bz * (NINE ? 5 : 3) * P;
-    qf += bz * P;
-    qin += bz * P;
-    qout += bz * P;
-    sor += bz * 2 * P;
-    const int ny = jj - 2, r0 = 1 + 2 * B * band;
-    const int ra = r0 - 1 + aoff + 2 * wave;
-    if (ra >= 1 && ra <= ny) xline_solve<T, NINE, NCH>(so, qf, qin, qout, sor, ii, jj, ra);
-    __syncthreads();  // the new first-colour rows are visible to the whole workgroup
-    const int rb = r0 + aoff + 2 * wave;
-    if (wave < B && rb <= ny) xline_solve<T, NINE, NCH>(so, qf, qout, qout, sor, ii, jj, rb);
-}
-
-// y-lines: the NW wavefronts of a workgroup relax the columns cb, cb+2, .., cb+2(nc-1)
-// (nc <= NW, uniform over the workgroup), reading neighbouring columns from qin and writing to
-// qout.  The right-hand sides of each 64-row chunk are formed cooperatively with row-contiguous
-// (coalesced) loads and handed to the column owners through LDS; each wavefront then scans its
-// column (lanes along j; the transposed factors are contiguous in j).  Results go back through
-// LDS so that the stores are row-contiguous too.
-template <typename T, int NCH>
-constexpr int yline_group() { return NCH < line_group<T>() ? NCH : line_group<T>(); }
-
-template <typename T, bool NINE, int NCH, int NW>
-__device__ inline void ylines_solve(const T *__restrict__ so, const T *__restrict__ qf,
-                                    const T *qin, T *qout, const T *__restrict__ sor, int ii,
-                                    int jj, int cb, int nc,
-                                    T (&tile_s)[yline_group<T, NCH>()][NW][WAVE + 1]) {
-    constexpr int G = yline_group<T, NCH>();
-    constexpr int LA = line_lookahead<T>();
-    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const bool colok = wave < nc;
-    const int ime = cb + 2 * wave;
-    const int tc = threadIdx.x % NW, tr = threadIdx.x / NW;
-    const bool tok = tc < nc;
-    const int ti = cb + 2 * tc;
-    const std::size_t P = (std::size_t)ii * jj;
-    const int n = jj - 2;
-    const int nact = (n + WAVE - 1) / WAVE;
-    const T *D = sor + (std::size_t)ime * jj + 1;
-    const T *E = sor + P + (std::size_t)ime * jj + 2;
-    // this thread's tile entry of chunk c (column ti, row c*64 + tr) plus its wavefront's own
-    // factor entries (column ime, row c*64 + lane)
-    // branch-free loads: row and column indices are clamped onto valid points, unused lanes are
-    // neutralised where the values are consumed (see the x-line kernel)
-    const int tic = cb + 2 * min(tc, nc - 1);
-    const T *Dc = colok ? D : sor + (std::size_t)(cb + 2 * (nc - 1)) * jj + 1;
-    const T *Ec = colok ? E : sor + P + (std::size_t)(cb + 2 * (nc - 1)) * jj + 2;
-    auto fetch = [&](int c, line_raw<T> &r) {
-        const int k = min(c * WAVE + tr, n - 1);
-        line_load<T, NINE, 1>(r, so, qf, qin, P, ii, pidx(tic, k + 1, ii));
-        const int kw = min(c * WAVE + lane, n - 1);
-        r.e = Ec[max(kw - 1, 0)];
-        r.d = Dc[kw];
-    };
-    line_raw<T> pf[LA];
-#pragma unroll
-    for (int c = 0; c < LA; ++c)
-        if (c < NCH) fetch(c, pf[c]);
-    T cst[NCH];
-    T carry = T(0);
-#pragma unroll
-    for (int g = 0; g < NCH; g += G) {
-        if (g < nact) {
-            T a[G], b[G], d[G];
-#pragma unroll
-            for (int u = 0; u < G; ++u) {
-                const int c = g + u;
-                tile_s[u][tc][tr] = line_rhs<T, NINE>(pf[c % LA]);
-                a[u] = (c * WAVE + lane > 0) ? -pf[c % LA].e : T(0);
-                d[u] = pf[c % LA].d;
-                if (c + LA < NCH) fetch(c + LA, pf[c % LA]);
-            }
-            lds_barrier();
-#pragma unroll
-            for (int u = 0; u < G; ++u) b[u] = tile_s[u][wave][lane];
-            lds_barrier();
-#pragma unroll
-            for (int u = 0; u < G; ++u) scan_affine_dpp(a[u], b[u]);
-#pragma unroll
-            for (int u = 0; u < G; ++u) {
-                const T y = a[u] * carry + b[u];
-                carry = last_lane(y);
-                cst[g + u] = y / d[u];
-            }
-        }
-    }
-    // backward pass in reversed lane order (see the x-line kernel)
-    const int rl = WAVE - 1 - lane;
-    auto fetch_e = [&](int c) { return Ec[min(c * WAVE + rl, max(n - 2, 0))]; };
-    // chunks at or above nactg belong to groups that are skipped, so a chunk whose prefetch
-    // would have come from such a group loads its entry directly
-    const int nactg = min(NCH, ((nact + G - 1) / G) * G);
-    T pe[LA];
-    carry = T(0);
-#pragma unroll
-    for (int g = NCH - G; g >= 0; g -= G) {
-        if (g < nact) {
-            T a[G], b[G];
-#pragma unroll
-            for (int u = G - 1; u >= 0; --u) {
-                const int c = g + u;
-                const T e = (c + LA >= nactg) ? fetch_e(c) : pe[c % LA];
-                if (c - LA >= 0) pe[c % LA] = fetch_e(c - LA);
-                const int k = c * WAVE + rl;
-                const bool act = colok && k < n;
-                b[u] = __shfl(cst[c], rl, WAVE);
-                a[u] = (k < n - 1) ? -e : T(0);
-                a[u] = act ? a[u] : T(1);
-                b[u] = act ? b[u] : T(0);
-            }
-#pragma unroll
-            for (int u = 0; u < G; ++u) scan_affine_dpp(a[u], b[u]);
-#pragma unroll
-            for (int u = G - 1; u >= 0; --u) {
-                const T x = a[u] * carry + b[u];
-                carry = last_lane(x);
-                tile_s[u][wave][rl] = x;
-            }
-            lds_barrier();
-#pragma unroll
-            for (int u = 0; u < G; ++u) {
-                const int kk = (g + u) * WAVE + tr;
-                if (tok && kk < n) qout[pidx(ti, kk + 1, ii)] = tile_s[u][tc][tr];
-            }
-            lds_barrier();
-        }
-    }
-}
-
-
-// y-lines, one colour per launch, in place: NW columns of the colour per workgroup.
-template <typename T, bool NINE, int NCH, int NW>
-__global__ __launch_bounds__(NW * 64) void relax_ylines_reg_k(const T *__restrict__ so,
-                                                              const T *__restrict__ qf,
-                                                              T *__restrict__ q,
-                                                              const T *__restrict__ sor, int ii,
-                                                              int jj, int first, int ncols,
-                                                              int ntiles) {
-    __shared__ T tile_s[yline_group<T, NCH>()][NW][WAVE + 1];
+// y-lines of one colour, in place: NC columns per workgroup, W wavefronts per column.  The
+// right-hand sides of G chunk rows at a time are formed by the whole workgroup with
+// row-contiguous loads (thread t: column t % NC, row t / NC of the W*64 rows) and handed to the
+// column owners through LDS; results return through LDS for row-contiguous stores.
+template <typename T, bool NINE, int CPW, int W, int NC, int G>
+__global__ __launch_bounds__(NC *W * 64) void relax_ylines_seg_k(const T *__restrict__ so,
+                                                                const T *__restrict__ qf,
+                                                                T *__restrict__ q,
+                                                                const T *__restrict__ sor,
+                                                                int ii, int jj, int first,
+                                                                int ncols, int ntiles) {
+    static_assert(CPW % G == 0, "G must divide CPW");
+    __shared__ T tile_s[G][W][NC][WAVE + 1];
+    __shared__ seg_map<T> xch[2][NC][W > 1 ? W * CPW : 1];
     const int tile = xcd_tile(ntiles);
     if (tile >= ntiles) return;  // uniform over the workgroup
-    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.y;
-    const int col0 = tile * NW;
-    ylines_solve<T, NINE, NCH, NW>(so + bz * (NINE ? 5 : 3) * P, qf + bz * P, q + bz * P, q + bz * P,
-                                   sor + bz * 2 * P, ii, jj, first + 2 * col0,
-                                   min(NW, ncols - col0), tile_s);
-}
-
-// Both colours of a y-line sweep in one launch, out of place (qin -> qout): the column analogue
-// of relax_xlines_pair_k.  Band t covers columns C0 .. C0+2B-1, C0 = 1 + 2Bt; phase A relaxes
-// the first-colour columns C0 - 1 + aoff + 2w (w = 0..B, inside 1..nx) from qin, phase B the
-// second-colour columns C0 + aoff + 2w (w < B) from the new values in qout.
-template <typename T, bool NINE, int NCH, int B>
-__global__ __launch_bounds__((B + 1) * 64) void relax_ylines_pair_k(const T *__restrict__ so,
-                                                                   const T *__restrict__ qf,
-                                                                   const T *qin, T *qout,
-                                                                   const T *__restrict__ sor,
-                                                                   int ii, int jj, int aoff,
-                                                                   int nbands) {
-    constexpr int NW = B + 1;
-    __shared__ T tile_s[yline_group<T, NCH>()][NW][WAVE + 1];
-    const int band = xcd_tile(nbands);
-    if (band >= nbands) return;  // uniform over the workgroup
+    const int wl = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int cs = wl / W, w = wl % W;
     const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.y;
     so += bz * (NINE ? 5 : 3) * P;
     qf += bz * P;
-    qin += bz * P;
-    qout += bz * P;
+    q += bz * P;
     sor += bz * 2 * P;
-    const int nx = ii - 2, c0 = 1 + 2 * B * band;
-    int ca = c0 - 1 + aoff;      // first-colour columns ca, ca+2, .. (B + 1 of them)
-    int na = B + 1;
-    if (ca < 1) {                // the ghost column 0 is not relaxed
-        ca += 2;
-        --na;
+    const int n = jj - 2;
+    const int col0 = tile * NC;
+    const int nc = min(NC, ncols - col0);
+    const int cb = first + 2 * col0;
+    // the wavefront's own column (factors contiguous in j)
+    const int ime = cb + 2 * min(cs, nc - 1);
+    const T *D = sor + (std::size_t)ime * jj + 1;
+    const T *E = sor + P + (std::size_t)ime * jj + 2;
+    // this thread's tile position for the right-hand sides and the stores
+    const int tc = threadIdx.x % NC, tr = threadIdx.x / NC;
+    const int tseg = tr / WAVE, trow = tr % WAVE;
+    const bool tok = tc < nc;
+    const int ti = cb + 2 * min(tc, nc - 1);
+    T a[CPW], b[CPW];
+#pragma unroll
+    for (int u = 0; u < CPW; ++u) {
+        const int k = (w * CPW + u) * WAVE + lane;
+        a[u] = (k > 0) ? -E[max(min(k, n - 1) - 1, 0)] : T(0);
     }
-    if (ca + 2 * (na - 1) > nx) na = ca > nx ? 0 : (nx - ca) / 2 + 1;
-    if (na > 0) ylines_solve<T, NINE, NCH, NW>(so, qf, qin, qout, sor, ii, jj, ca, na, tile_s);
-    __syncthreads();  // new first-colour columns visible to the whole workgroup
-    const int cbb = c0 + aoff;
-    const int nbc = cbb > nx ? 0 : min(B, (nx - cbb) / 2 + 1);
-    if (nbc > 0) ylines_solve<T, NINE, NCH, NW>(so, qf, qout, qout, sor, ii, jj, cbb, nbc, tile_s);
+#pragma unroll
+    for (int g = 0; g < CPW; g += G) {
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            const int k = min((tseg * CPW + g + u) * WAVE + trow, n - 1);
+            line_raw<T> r;
+            line_load<T, NINE, 1>(r, so, qf, q, P, ii, pidx(ti, k + 1, ii));
+            tile_s[u][tseg][tc][trow] = line_rhs<T, NINE>(r);
+        }
+        lds_barrier();
+#pragma unroll
+        for (int u = 0; u < G; ++u) b[g + u] = tile_s[u][w][cs][lane];
+        lds_barrier();
+    }
+    seg_forward<T, CPW, W>(
+        a, b, [&](int u) { return D[min((w * CPW + u) * WAVE + lane, n - 1)]; }, xch[0][cs], w);
+    const int rl = WAVE - 1 - lane;
+#pragma unroll
+    for (int u = 0; u < CPW; ++u) {
+        const int k = (w * CPW + u) * WAVE + rl;
+        const T c = __shfl(b[u], rl, WAVE);
+        const T e = E[min(k, max(n - 2, 0))];
+        a[u] = (k < n - 1) ? -e : T(0);
+        a[u] = (k < n) ? a[u] : T(1);
+        b[u] = (k < n) ? c : T(0);
+    }
+    seg_backward<T, CPW, W>(a, b, xch[1][cs], w);
+#pragma unroll
+    for (int g = 0; g < CPW; g += G) {
+#pragma unroll
+        for (int u = 0; u < G; ++u) tile_s[u][w][cs][rl] = b[g + u];
+        lds_barrier();
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            const int k = (tseg * CPW + g + u) * WAVE + trow;
+            if (tok && k < n) q[pidx(ti, k + 1, ii)] = tile_s[u][tseg][tc][trow];
+        }
+        lds_barrier();
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1204,31 +1090,6 @@ __global__ __launch_bounds__(256) void keep_inactive_k(double *__restrict__ x,
         if (active[k / per] == 0) x[k] = xprev[k];
 }
 
-// dst = src on the ghost ring of every problem of a batch
-template <typename T>
-__global__ __launch_bounds__(256) void copy_ghosts_k(const T *__restrict__ src, T *__restrict__ dst,
-                                                     int ii, int jj) {
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    const int per = 2 * ii + 2 * (jj - 2);
-    if (t >= per) return;
-    int i, j;
-    if (t < ii) {
-        i = t;
-        j = 0;
-    } else if (t < 2 * ii) {
-        i = t - ii;
-        j = jj - 1;
-    } else if (t < 2 * ii + jj - 2) {
-        i = 0;
-        j = t - 2 * ii + 1;
-    } else {
-        i = ii - 1;
-        j = t - 2 * ii - (jj - 2) + 1;
-    }
-    const std::size_t p = (std::size_t)blockIdx.y * ii * jj + pidx(i, j, ii);
-    dst[p] = src[p];
-}
-
 inline unsigned grid1(std::size_t n) {
     std::size_t b = (n + 255) / 256;
     if (b > 65536) b = 65536;
@@ -1340,43 +1201,190 @@ void setup_lines_y(const double *so, int nstncl, double *sor, int ii, int jj, in
     setup_lines_y_k<<<dim3((n + 63) / 64, nb), 64, 0, s>>>(so, sor, ii, jj, nstncl, info);
 }
 
-template <typename T, bool NINE, int DIR, int NCH>
-static void launch_lines_reg(const T *so, const T *qf, T *q, const T *sor, int ii, int jj,
-                             int first, int nlines, hipStream_t s, int nb) {
-    constexpr int NW = 8;  // y-lines: columns per workgroup
-    const int per_tile = (DIR == 0) ? 4 : NW;
-    const int ntiles = (nlines + per_tile - 1) / per_tile;
+// ---- line kernel shapes and their autotuner ---------------------------------------------------
+// A shape is (CPW, W, L) for x-lines (chunks per wavefront, wavefronts per line, lines per
+// workgroup) and (CPW, W, NC, G) for y-lines (..., columns per workgroup, chunk rows per LDS
+// round).  All shapes of a chunk class give bitwise identical results (see the segmented
+// kernels), so the shape of each (precision, stencil, direction, extents, batch) is picked by
+// timing: tune_lines() times every candidate once on the caller's data and records the fastest in
+// a process-wide table that the launchers consult; without an entry, candidate 0 runs.
+
+template <typename T, bool NINE, int CPW, int W, int L>
+static void launch_x(const T *so, const T *qf, T *q, const T *sor, int ii, int jj, int first,
+                     int nlines, hipStream_t s, int nb) {
+    const int ntiles = (nlines + L - 1) / L;
     const dim3 blocks((unsigned)(8 * ((ntiles + 7) / 8)), (unsigned)nb);
-    if (DIR == 0)
-        relax_xlines_reg_k<T, NINE, NCH><<<blocks, 256, 0, s>>>(so, qf, q, sor, ii, jj, first,
-                                                               nlines, ntiles);
+    relax_xlines_seg_k<T, NINE, CPW, W, L><<<blocks, L * W * 64, 0, s>>>(so, qf, q, sor, ii, jj,
+                                                                        first, nlines, ntiles);
+}
+
+template <typename T, bool NINE, int CPW, int W, int NC, int G>
+static void launch_y(const T *so, const T *qf, T *q, const T *sor, int ii, int jj, int first,
+                     int nlines, hipStream_t s, int nb) {
+    const int ntiles = (nlines + NC - 1) / NC;
+    const dim3 blocks((unsigned)(8 * ((ntiles + 7) / 8)), (unsigned)nb);
+    relax_ylines_seg_k<T, NINE, CPW, W, NC, G><<<blocks, NC * W * 64, 0, s>>>(
+        so, qf, q, sor, ii, jj, first, nlines, ntiles);
+}
+
+// Launches candidate c of chunk class NCH (lines of at most 64*NCH points); returns the number
+// of candidates of the class (c outside [0, count) launches nothing).
+static std::string shape_text(const char *k, int cpw, int w, const char *tn, int t, int g) {
+    char buf[96];
+    if (g > 0)
+        std::snprintf(buf, sizeof(buf), "%s<CPW=%d,W=%d,%s=%d,G=%d>", k, cpw, w, tn, t, g);
     else
-        relax_ylines_reg_k<T, NINE, NCH, NW><<<blocks, NW * 64, 0, s>>>(so, qf, q, sor, ii, jj,
-                                                                       first, nlines, ntiles);
+        std::snprintf(buf, sizeof(buf), "%s<CPW=%d,W=%d,%s=%d>", k, cpw, w, tn, t);
+    return buf;
+}
+
+// With `name` non-null nothing is launched and *name receives candidate c's description.
+template <typename T, bool NINE, int DIR, int NCH>
+static int launch_shape(int c, const T *so, const T *qf, T *q, const T *sor, int ii, int jj,
+                        int first, int nlines, hipStream_t s, int nb, std::string *name = nullptr) {
+    constexpr bool F64 = sizeof(T) == 8;
+#define CDR_X(k, C, W, L)                                                                       \
+    if (c == k) {                                                                               \
+        if (name)                                                                               \
+            *name = shape_text("relax_xlines_seg_k", C, W, "L", L, 0);                          \
+        else                                                                                    \
+            launch_x<T, NINE, C, W, L>(so, qf, q, sor, ii, jj, first, nlines, s, nb);           \
+    }
+#define CDR_Y(k, C, W, NC, G)                                                                   \
+    if (c == k) {                                                                               \
+        if (name)                                                                               \
+            *name = shape_text("relax_ylines_seg_k", C, W, "NC", NC, G);                        \
+        else                                                                                    \
+            launch_y<T, NINE, C, W, NC, G>(so, qf, q, sor, ii, jj, first, nlines, s, nb);       \
+    }
+    if constexpr (DIR == 0) {
+        if constexpr (NCH <= 4) {
+            CDR_X(0, NCH, 1, 4);
+            return 1;
+        } else if constexpr (NCH == 8) {
+            CDR_X(0, 8, 1, 4);
+            CDR_X(1, 4, 2, 2);
+            return 2;
+        } else if constexpr (!F64) {
+            CDR_X(0, 16, NCH / 16, NCH >= 64 ? 1 : 64 / NCH);
+            CDR_X(1, 8, NCH / 8, NCH >= 32 ? 1 : 2);
+            return 2;
+        } else {
+            CDR_X(0, 8, NCH / 8, NCH >= 32 ? 1 : 2);
+            CDR_X(1, 4, NCH / 4, 1);
+            return 2;
+        }
+    } else {
+        constexpr int N1 = F64 ? 4 : 8;  // columns per workgroup for 64-byte rows
+        if constexpr (NCH <= 4) {
+            CDR_Y(0, NCH, 1, N1, NCH);
+            CDR_Y(1, NCH, 1, 2 * N1, NCH < 2 ? NCH : 2);
+            return 2;
+        } else if constexpr (NCH == 8) {
+            CDR_Y(0, 8, 1, N1, 4);
+            CDR_Y(1, 8, 1, 2 * N1, 2);
+            CDR_Y(2, 4, 2, N1, 4);
+            return 3;
+        } else if constexpr (NCH == 16) {
+            CDR_Y(0, 16, 1, N1, 4);
+            CDR_Y(1, 8, 2, N1, 4);
+            CDR_Y(2, 16, 1, 2 * N1, 2);
+            return 3;
+        } else if constexpr (NCH == 32) {
+            CDR_Y(0, 16, 2, N1, 2);
+            CDR_Y(1, 32, 1, N1, 4);
+            CDR_Y(2, 8, 4, N1 / 2, 2);
+            return 3;
+        } else {  // 16-wavefront workgroups get 128 VGPRs: narrower tiles for long segments
+            CDR_Y(0, 16, 4, N1 / 2, 2);
+            CDR_Y(1, 64 / (F64 ? 8 : 1), F64 ? 8 : 1, N1 / (F64 ? 2 : 1), 2);
+            CDR_Y(2, 32, 2, N1 / 2, 2);
+            return 3;
+        }
+    }
+#undef CDR_X
+#undef CDR_Y
+}
+
+// chunk class of a line of n points: 1, 2, 4, ..., 64, or 0 beyond 4096 points
+static int chunk_class(int n) {
+    const int nch = (n + WAVE - 1) / WAVE;
+    int c = 1;
+    while (c < nch && c < 64) c *= 2;
+    return nch > 64 ? 0 : c;
 }
 
 template <typename T, bool NINE, int DIR>
-static void launch_lines(const T *so, const T *qf, T *q, const T *sor, int ii, int jj, int first,
-                         int nlines, hipStream_t s, int nb) {
+static int launch_class(int c, const T *so, const T *qf, T *q, const T *sor, int ii, int jj,
+                        int first, int nlines, hipStream_t s, int nb, std::string *name = nullptr) {
     const int n = (DIR == 0) ? ii - 2 : jj - 2;
-    const int nch = (n + WAVE - 1) / WAVE;
-#define CDR_L(NC) launch_lines_reg<T, NINE, DIR, NC>(so, qf, q, sor, ii, jj, first, nlines, s, nb)
-    if (nch <= 1) CDR_L(1);
-    else if (nch <= 2) CDR_L(2);
-    else if (nch <= 4) CDR_L(4);
-    else if (nch <= 8) CDR_L(8);
-    else if (nch <= 16) CDR_L(16);
-    else if (nch <= 32) CDR_L(32);
-    else if (nch <= 64) CDR_L(64);
-    else  // lines longer than 4096: stage the forward results in q itself
-        relax_lines_k<T, NINE, DIR><<<dim3((unsigned)((nlines + 3) / 4), (unsigned)nb), 256, 0,
-                                      s>>>(so, qf, q, sor, ii, jj, first, nlines);
-#undef CDR_L
+#define CDR_C(K)                                                                                  \
+    case K:                                                                                       \
+        return launch_shape<T, NINE, DIR, K>(c, so, qf, q, sor, ii, jj, first, nlines, s, nb, name)
+    switch (chunk_class(n)) {
+        CDR_C(1);
+        CDR_C(2);
+        CDR_C(4);
+        CDR_C(8);
+        CDR_C(16);
+        CDR_C(32);
+        CDR_C(64);
+    default:  // lines longer than 4096: stage the forward results in q itself
+        if (c == 0 && name) *name = "relax_lines_k";
+        if (c == 0 && !name)
+            relax_lines_k<T, NINE, DIR><<<dim3((unsigned)((nlines + 3) / 4), (unsigned)nb), 256, 0,
+                                          s>>>(so, qf, q, sor, ii, jj, first, nlines);
+        return 1;
+    }
+#undef CDR_C
+}
+
+namespace {
+using tune_key = std::array<int, 6>;  // direction, word size, 9-point, ii, jj, batch
+std::mutex g_tune_mu;
+std::map<tune_key, int> g_tune;
+std::atomic<int> g_autotune{1};
+}  // namespace
+
+template <typename T>
+static tune_key key_of(int dir, int nstncl, int ii, int jj, int nb) {
+    return tune_key{dir, (int)sizeof(T), nstncl == 5 ? 1 : 0, ii, jj, nb};
+}
+
+static int tuned_shape(const tune_key &k) {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    const auto it = g_tune.find(k);
+    return it == g_tune.end() ? 0 : it->second;
+}
+
+template <typename T, int DIR>
+static int launch_colour(int c, const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii,
+                         int jj, int first, int nlines, hipStream_t s, int nb,
+                         std::string *name = nullptr) {
+    if (nstncl == 5)
+        return launch_class<T, true, DIR>(c, so, qf, q, sor, ii, jj, first, nlines, s, nb, name);
+    return launch_class<T, false, DIR>(c, so, qf, q, sor, ii, jj, first, nlines, s, nb, name);
+}
+
+static std::string shape_name(const std::array<int, 6> &k, int c) {
+    std::string name;
+    if (k[1] == 8)
+        (void)(k[0] == 0 ? launch_colour<double, 0>(c, nullptr, k[2] ? 5 : 3, nullptr, nullptr,
+                                                    nullptr, k[3], k[4], 1, 1, nullptr, 1, &name)
+                         : launch_colour<double, 1>(c, nullptr, k[2] ? 5 : 3, nullptr, nullptr,
+                                                    nullptr, k[3], k[4], 1, 1, nullptr, 1, &name));
+    else
+        (void)(k[0] == 0 ? launch_colour<float, 0>(c, nullptr, k[2] ? 5 : 3, nullptr, nullptr,
+                                                   nullptr, k[3], k[4], 1, 1, nullptr, 1, &name)
+                         : launch_colour<float, 1>(c, nullptr, k[2] ? 5 : 3, nullptr, nullptr,
+                                                   nullptr, k[3], k[4], 1, 1, nullptr, 1, &name));
+    return name;
 }
 
 template <typename T, int DIR>
 static void relax_lines(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
                         int updown, hipStream_t s, int nb) {
+    const int c = tuned_shape(key_of<T>(DIR, nstncl, ii, jj, nb));
     // lines of one colour: DOWN relaxes the lines at even array index (odd Cedar index) first,
     // UP the odd ones first (src/2d/ftn/BMG2_SymStd_relax_lines_x.f90:82-97)
     const int nfix = (DIR == 0) ? jj - 2 : ii - 2;
@@ -1384,10 +1392,7 @@ static void relax_lines(const T *so, int nstncl, const T *qf, T *q, const T *sor
         const int first = (updown == 0) ? (t == 0 ? 2 : 1) : (t == 0 ? 1 : 2);
         if (first > nfix) continue;
         const int nlines = (nfix - first) / 2 + 1;
-        if (nstncl == 5)
-            launch_lines<T, true, DIR>(so, qf, q, sor, ii, jj, first, nlines, s, nb);
-        else
-            launch_lines<T, false, DIR>(so, qf, q, sor, ii, jj, first, nlines, s, nb);
+        launch_colour<T, DIR>(c, so, nstncl, qf, q, sor, ii, jj, first, nlines, s, nb);
     }
 }
 
@@ -1403,72 +1408,104 @@ void relax_lines_y(const T *so, int nstncl, const T *qf, T *q, const T *sor, int
     relax_lines<T, 1>(so, nstncl, qf, q, sor, ii, jj, updown, s, nb);
 }
 
-// both colours of one sweep in one launch, qin -> qout (see relax_xlines_pair_k)
-template <typename T, bool NINE, int DIR, int NCH>
-static void launch_pair_reg(const T *so, const T *qf, const T *qin, T *qout, const T *sor, int ii,
-                            int jj, int aoff, hipStream_t s, int nb) {
-    constexpr int B = (DIR == 0) ? 4 : 7;  // second-colour lines per band
-    const int nl = (DIR == 0) ? jj - 2 : ii - 2;
-    const int nbands = (nl + 2 * B - 1) / (2 * B);
-    const dim3 blocks((unsigned)(8 * ((nbands + 7) / 8)), (unsigned)nb);
-    if (DIR == 0)
-        relax_xlines_pair_k<T, NINE, NCH, B><<<blocks, (B + 1) * 64, 0, s>>>(so, qf, qin, qout, sor,
-                                                                            ii, jj, aoff, nbands);
-    else
-        relax_ylines_pair_k<T, NINE, NCH, B><<<blocks, (B + 1) * 64, 0, s>>>(so, qf, qin, qout, sor,
-                                                                            ii, jj, aoff, nbands);
+int set_autotune(int on) { return g_autotune.exchange(on ? 1 : 0); }
+
+int line_shape_count(int dir, int wbytes, int nstncl, int ii, int jj) {
+    const hipStream_t s = nullptr;  // counting launches nothing
+    if (wbytes == 8)
+        return dir == 0 ? launch_colour<double, 0>(-1, nullptr, nstncl, nullptr, nullptr, nullptr,
+                                                   ii, jj, 1, 1, s, 1)
+                        : launch_colour<double, 1>(-1, nullptr, nstncl, nullptr, nullptr, nullptr,
+                                                   ii, jj, 1, 1, s, 1);
+    return dir == 0 ? launch_colour<float, 0>(-1, nullptr, nstncl, nullptr, nullptr, nullptr, ii,
+                                              jj, 1, 1, s, 1)
+                    : launch_colour<float, 1>(-1, nullptr, nstncl, nullptr, nullptr, nullptr, ii,
+                                              jj, 1, 1, s, 1);
 }
 
-template <typename T, bool NINE, int DIR>
-static void launch_pair(const T *so, const T *qf, const T *qin, T *qout, const T *sor, int ii,
-                        int jj, int aoff, hipStream_t s, int nb) {
-    const int n = (DIR == 0) ? ii - 2 : jj - 2;
-    const int nch = (n + WAVE - 1) / WAVE;
-#define CDR_P(NC) launch_pair_reg<T, NINE, DIR, NC>(so, qf, qin, qout, sor, ii, jj, aoff, s, nb)
-    if (nch <= 1) CDR_P(1);
-    else if (nch <= 2) CDR_P(2);
-    else if (nch <= 4) CDR_P(4);
-    else if (nch <= 8) CDR_P(8);
-    else if (nch <= 16) CDR_P(16);
-    else if (nch <= 32) CDR_P(32);
-    else CDR_P(64);
-#undef CDR_P
+void set_line_shape(int dir, int wbytes, int nstncl, int ii, int jj, int nb, int shape) {
+    const tune_key k{dir, wbytes, nstncl == 5 ? 1 : 0, ii, jj, nb};
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    if (shape < 0)
+        g_tune.erase(k);
+    else
+        g_tune[k] = shape;
+}
+
+template <typename T>
+int tune_lines(int dir, const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
+               hipStream_t s, int nb) {
+    const tune_key k = key_of<T>(dir, nstncl, ii, jj, nb);
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        if (g_tune.count(k)) return g_tune[k];
+    }
+    auto run = [&](int c) {
+        return dir == 0 ? launch_colour<T, 0>(c, so, nstncl, qf, q, sor, ii, jj, 1,
+                                              (jj - 2 + 1) / 2, s, nb)
+                        : launch_colour<T, 1>(c, so, nstncl, qf, q, sor, ii, jj, 1,
+                                              (ii - 2 + 1) / 2, s, nb);
+    };
+    const int ncand = run(-1);  // count only
+    if (!g_autotune.load() || ncand < 2) return 0;
+    int best = 0;
+    {
+        hipEvent_t e0, e1;
+        if (hipEventCreate(&e0) != hipSuccess) return -1;
+        if (hipEventCreate(&e1) != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            return -1;
+        }
+        float tbest = 0.f;
+        for (int c = 0; c < ncand; ++c) {
+            run(c);  // warm-up
+            (void)hipEventRecord(e0, s);
+            for (int r = 0; r < 3; ++r) run(c);
+            (void)hipEventRecord(e1, s);
+            if (hipEventSynchronize(e1) != hipSuccess) break;
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (c == 0 || ms < tbest) {
+                tbest = ms;
+                best = c;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (hipGetLastError() != hipSuccess) return -1;
+    }
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune[k] = best;
+    return best;
+}
+
+std::string tune_report() {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    std::string out = "[";
+    for (const auto &kv : g_tune) {
+        const tune_key &k = kv.first;
+        char buf[160];
+        std::snprintf(buf, sizeof(buf),
+                      "%s{\"dir\":\"%c\",\"fp\":%d,\"stencil\":%d,\"ii\":%d,\"jj\":%d,\"nb\":%d,"
+                      "\"shape\":%d,\"kernel\":\"",
+                      out.size() > 1 ? "," : "", k[0] == 0 ? 'x' : 'y', 8 * k[1], k[2] ? 9 : 5,
+                      k[3], k[4], k[5], kv.second);
+        out += buf;
+        out += shape_name(k, kv.second) + "\"}";
+    }
+    return out + "]";
 }
 
 template <typename T>
 void relax_lines_xy(const T *so, int nstncl, const T *qf, T *q, const T *sorx, const T *sory,
                     T *work, int ii, int jj, int updown, hipStream_t s, int nb) {
-    if (ii - 2 > 64 * WAVE || jj - 2 > 64 * WAVE) {  // lines beyond the register-resident kernels
-        if (updown == 0) {
-            relax_lines<T, 0>(so, nstncl, qf, q, sorx, ii, jj, updown, s, nb);
-            relax_lines<T, 1>(so, nstncl, qf, q, sory, ii, jj, updown, s, nb);
-        } else {
-            relax_lines<T, 1>(so, nstncl, qf, q, sory, ii, jj, updown, s, nb);
-            relax_lines<T, 0>(so, nstncl, qf, q, sorx, ii, jj, updown, s, nb);
-        }
-        return;
-    }
-    const int per = 2 * ii + 2 * (jj - 2);
-    copy_ghosts_k<T><<<dim3((unsigned)((per + 255) / 256), (unsigned)nb), 256, 0, s>>>(q, work, ii, jj);
-    // DOWN relaxes the even lines first (aoff 0), UP the odd ones (aoff 1)
-    const int aoff = updown == 0 ? 0 : 1;
-    const bool nine = nstncl == 5;
-    if (updown == 0) {  // x then y
-        if (nine) {
-            launch_pair<T, true, 0>(so, qf, q, work, sorx, ii, jj, aoff, s, nb);
-            launch_pair<T, true, 1>(so, qf, work, q, sory, ii, jj, aoff, s, nb);
-        } else {
-            launch_pair<T, false, 0>(so, qf, q, work, sorx, ii, jj, aoff, s, nb);
-            launch_pair<T, false, 1>(so, qf, work, q, sory, ii, jj, aoff, s, nb);
-        }
-    } else {  // y then x
-        if (nine) {
-            launch_pair<T, true, 1>(so, qf, q, work, sory, ii, jj, aoff, s, nb);
-            launch_pair<T, true, 0>(so, qf, work, q, sorx, ii, jj, aoff, s, nb);
-        } else {
-            launch_pair<T, false, 1>(so, qf, q, work, sory, ii, jj, aoff, s, nb);
-            launch_pair<T, false, 0>(so, qf, work, q, sorx, ii, jj, aoff, s, nb);
-        }
+    (void)work;
+    if (updown == 0) {
+        relax_lines<T, 0>(so, nstncl, qf, q, sorx, ii, jj, updown, s, nb);
+        relax_lines<T, 1>(so, nstncl, qf, q, sory, ii, jj, updown, s, nb);
+    } else {
+        relax_lines<T, 1>(so, nstncl, qf, q, sory, ii, jj, updown, s, nb);
+        relax_lines<T, 0>(so, nstncl, qf, q, sorx, ii, jj, updown, s, nb);
     }
 }
 
@@ -1560,6 +1597,8 @@ void add_correction(double *x, const float *e, int ii, int jj, hipStream_t s, in
                                    hipStream_t, int);                                             \
     template void relax_lines_xy<T>(const T *, int, const T *, T *, const T *, const T *, T *, int,\
                                     int, int, hipStream_t, int);                                  \
+    template int tune_lines<T>(int, const T *, int, const T *, T *, const T *, int, int,          \
+                               hipStream_t, int);                                                 \
     template void restrict_<T>(const T *, const T *, T *, int, int, int, int, hipStream_t, int);  \
     template void interp_add<T>(T *, const T *, const T *, const T *, int, const T *, int, int,   \
                                 int, int, hipStream_t, int);                                      \

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <string>
 
 namespace cdr {
 namespace k2 {
@@ -74,12 +75,27 @@ void relax_lines_y(const T *so, int nstncl, const T *qf, T *q, const T *sor, int
                    int updown, hipStream_t s, int nb = 1);
 
 // One line-xy smoothing step (include/cedar/multilevel.h:175-178, 204-207): x-lines then y-lines
-// on DOWN, y then x on UP, each sweep relaxing both colours in one launch out of place through
-// `work` (one grid per problem; its interior is scratch, its ghost ring is set from q).  Equal
-// bit for bit to relax_lines_x followed by relax_lines_y (or y then x).
+// on DOWN, y then x on UP.  `work` is unused (kept for callers that size it).
 template <typename T>
 void relax_lines_xy(const T *so, int nstncl, const T *qf, T *q, const T *sorx, const T *sory,
                     T *work, int ii, int jj, int updown, hipStream_t s, int nb = 1);
+
+// Line-kernel shape autotuner.  Every shape of the line kernels gives bitwise identical results;
+// tune_lines times each candidate shape for one direction (0: x, 1: y) once on the given data
+// (q is overwritten) and records the fastest for (precision, stencil, direction, ii, jj, nb) in a
+// process-wide table that relax_lines_* consult.  It synchronises `s`; it returns the chosen
+// shape index, or -1 on a HIP error.  With autotuning off (set_autotune(0), which returns the
+// previous setting) nothing is timed or recorded and the default shape 0 runs.
+template <typename T>
+int tune_lines(int dir, const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
+               hipStream_t s, int nb = 1);
+int set_autotune(int on);
+// number of candidate shapes for lines of the given direction, word size (4 or 8) and extents
+int line_shape_count(int dir, int wbytes, int nstncl, int ii, int jj);
+// records `shape` for (direction, word size, stencil, ii, jj, nb) (shape < 0: erases the entry)
+void set_line_shape(int dir, int wbytes, int nstncl, int ii, int jj, int nb, int shape);
+// JSON list of the recorded choices
+std::string tune_report();
 
 template <typename T>
 void restrict_(const T *ci, const T *q, T *qc, int iif, int jjf, int iic, int jjc, hipStream_t s, int nb = 1);

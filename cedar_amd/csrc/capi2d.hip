@@ -3,6 +3,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstring>
+#include <string>
+
 #include "bmg2_launch.hpp"
 #include "capi2d_impl.hpp"
 
@@ -112,8 +116,8 @@ int relax_lines(int dir, const T *so, int ns, const T *qf, T *q, const T *sor, i
 template <typename T>
 int relax_lines_xy(const T *so, int ns, const T *qf, T *q, const T *sorx, const T *sory, T *work,
                    int ii, int jj, int ud, hipStream_t s, int nb) {
-    if (!so || !qf || !q || !sorx || !sory || !work || work == q || !ext_ok(ii, jj) ||
-        !sten_ok(ns) || !dir_ok(ud) || !nb_ok(nb))
+    if (!so || !qf || !q || !sorx || !sory || !ext_ok(ii, jj) || !sten_ok(ns) || !dir_ok(ud) ||
+        !nb_ok(nb))
         return CDR_ERR_ARG;
     k2::relax_lines_xy<T>(so, ns, qf, q, sorx, sory, work, ii, jj, ud, s, nb);
     return launched();
@@ -283,6 +287,35 @@ int cdr_hip_relax_lines_xy2d_f32(const float *so, int ns, const float *qf, float
                                  const float *sorx, const float *sory, float *work, int ii, int jj,
                                  int ud, void *st) {
     return api2::relax_lines_xy<float>(so, ns, qf, q, sorx, sory, work, ii, jj, ud, S(st));
+}
+
+int cdr_hip_autotune(int on) { return cdr::k2::set_autotune(on); }
+
+int cdr_hip_line_shapes2d(int dir, int fp_bits, int nstncl, int ii, int jj) {
+    if ((dir != 0 && dir != 1) || (fp_bits != 32 && fp_bits != 64) || !sten_ok(nstncl) ||
+        !ext_ok(ii, jj))
+        return -CDR_ERR_ARG;
+    return cdr::k2::line_shape_count(dir, fp_bits / 8, nstncl, ii, jj);
+}
+
+int cdr_hip_set_line_shape2d(int dir, int fp_bits, int nstncl, int ii, int jj, int nb,
+                             int shape) {
+    if ((dir != 0 && dir != 1) || (fp_bits != 32 && fp_bits != 64) || !sten_ok(nstncl) ||
+        !ext_ok(ii, jj) || !nb_ok(nb))
+        return CDR_ERR_ARG;
+    if (shape >= cdr::k2::line_shape_count(dir, fp_bits / 8, nstncl, ii, jj)) return CDR_ERR_ARG;
+    cdr::k2::set_line_shape(dir, fp_bits / 8, nstncl, ii, jj, nb, shape);
+    return CDR_OK;
+}
+
+int cdr_hip_autotune_report(char *buf, int len) {
+    const std::string r = cdr::k2::tune_report();
+    if (buf && len > 0) {
+        const std::size_t n = std::min(r.size(), (std::size_t)len - 1);
+        std::memcpy(buf, r.data(), n);
+        buf[n] = '\0';
+    }
+    return (int)r.size();
 }
 
 int cdr_hip_restrict2d_f64(const double *ci, const double *q, double *qc, int iif, int jjf,

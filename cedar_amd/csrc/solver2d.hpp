@@ -31,11 +31,10 @@ namespace api2 = cdr::api2;
 template <typename T>
 struct level_data {
     extents ext;
-    dbuf<T> so_buf, ci_buf, x_buf, b_buf, res_buf, sor_buf[2], lw_buf;
+    dbuf<T> so_buf, ci_buf, x_buf, b_buf, res_buf, sor_buf[2];
     stencil_op<T> A;
     prolong_op<T> P;  // stored on this (coarse) level; unused on level 0
     grid_func<T> x, b, res;
-    grid_func<T> lw;  // line-xy smoothing: the grid each sweep writes before the next reads it
     relax_stencil<T> SOR[2];
 };
 
@@ -269,7 +268,7 @@ private:
 
     template <typename T>
     static void alloc_level(level_data<T> &L, extents e, int nstncl, bool own_op, bool need_ci,
-                            bool need_xb, bool need_lw, hipStream_t s) {
+                            bool need_xb, hipStream_t s) {
         L.ext = e;
         const std::size_t np = e.total();
         if (own_op) {
@@ -288,10 +287,6 @@ private:
         }
         L.res_buf = dbuf<T>(np, s);
         L.res = grid_func<T>{L.res_buf.data(), e};
-        if (need_lw) {
-            L.lw_buf = dbuf<T>(np, s);
-            L.lw = grid_func<T>{L.lw_buf.data(), e};
-        }
         for (int k = 0; k < 2; ++k) {
             L.sor_buf[k] = dbuf<T>(np * 2, s);
             L.SOR[k] = relax_stencil<T>{L.sor_buf[k].data(), e};
@@ -312,10 +307,10 @@ private:
         for (int l = 0; l < nl; ++l) {
             extents e{cx + 2, cy + 2, nb};
             if (l == 0) {
-                alloc_level(L64_[0], e, nstncl, false, false, false, line_xy(), stream_);
+                alloc_level(L64_[0], e, nstncl, false, false, false, stream_);
                 L64_[0].A = stencil_op<double>{const_cast<double *>(so_dev), e, nstncl};
             } else {
-                alloc_level(L64_[l], e, 5, true, true, true, line_xy(), stream_);
+                alloc_level(L64_[l], e, 5, true, true, true, stream_);
             }
             cx = (int)((cx - 1) / 2.0 + 1);
             cy = (int)((cy - 1) / 2.0 + 1);
@@ -340,7 +335,12 @@ private:
         }
         km_.setup<kernels::solve_cg<double>>(L64_[nl - 1].A, abd_grid());
 
-        if (set_.precision == CDR_PREC_MIXED) build_f32();
+        if (set_.precision == CDR_PREC_MIXED) {
+            build_f32();
+            tune_lines(L32_);
+        } else {
+            tune_lines(L64_);
+        }
         check_hip(hipEventRecord(ev1_, stream_), "hipEventRecord");
         check_hip(hipEventSynchronize(ev1_), "hipEventSynchronize");
         float ms = 0.f;
@@ -354,7 +354,27 @@ private:
         return abd_g_;
     }
 
-    bool line_xy() const { return set_.relaxation == CDR_RELAX_LINE_XY; }
+    // Picks the line-kernel launch shapes of every smoothing level by timing (the shapes give
+    // identical results); a no-op for point relaxation and for shapes already recorded.
+    template <typename T>
+    void tune_lines(std::vector<level_data<T>> &L) {
+        const int r = set_.relaxation;
+        if (r != CDR_RELAX_LINE_X && r != CDR_RELAX_LINE_Y && r != CDR_RELAX_LINE_XY) return;
+        const std::size_t np = L[0].ext.total();
+        dbuf<T> q(np, stream_), f(np, stream_);
+        k2::fill<T>(q.data(), T(0), np, stream_);
+        k2::fill<T>(f.data(), T(0), np, stream_);
+        for (std::size_t l = 0; l + 1 < L.size(); ++l) {
+            const level_data<T> &D = L[l];
+            for (int dir = 0; dir < 2; ++dir) {
+                if ((dir == 0 && r == CDR_RELAX_LINE_Y) || (dir == 1 && r == CDR_RELAX_LINE_X)) continue;
+                const relax_stencil<T> &S = D.SOR[r == CDR_RELAX_LINE_XY ? dir : 0];
+                if (k2::tune_lines<T>(dir, D.A.p, D.A.nstncl, f.data(), q.data(), S.p, D.ext.ii,
+                                      D.ext.jj, stream_, D.ext.nb) < 0)
+                    throw error(CDR_ERR_HIP, "line-kernel tuning failed");
+            }
+        }
+    }
 
     void setup_relax(level_data<double> &F) {
         switch (set_.relaxation) {
@@ -378,7 +398,7 @@ private:
         for (int l = 0; l < nl; ++l) {
             auto &D = L64_[l];
             auto &F = L32_[l];
-            alloc_level(F, D.ext, D.A.nstncl, true, l > 0, true, line_xy(), stream_);
+            alloc_level(F, D.ext, D.A.nstncl, true, l > 0, true, stream_);
             const std::size_t np = D.ext.total();
             k2::convert<double, float>(D.A.p, F.A.p, np * D.A.nstncl, stream_);
             if (l > 0) k2::convert<double, float>(D.P.p, F.P.p, np * 8, stream_);
@@ -398,9 +418,9 @@ private:
             case CDR_RELAX_LINE_Y: km_.run<kernels::line_relax<T, relax_dir::y>>(L.A, x, b, L.SOR[0], L.res, d); break;
             default:
                 // x then y going down, y then x going up (include/cedar/multilevel.h:175-178,
-                // 204-207), both sweeps in the fused two-colour form
+                // 204-207)
                 check(api2::relax_lines_xy<T>(L.A.p, L.A.nstncl, b.p, x.p, L.SOR[0].p, L.SOR[1].p,
-                                              L.lw.p, L.ext.ii, L.ext.jj, (int)d, cur_, L.ext.nb),
+                                              nullptr, L.ext.ii, L.ext.jj, (int)d, cur_, L.ext.nb),
                       "relax_lines_xy");
             }
         }

@@ -44,6 +44,24 @@ const char *cdr_hip_status_string(int status);
 /* Library version string ("cedar_amd <version> gfx950"). */
 const char *cdr_hip_version(void);
 
+/* Line-kernel shape autotuning (no counterpart in the reference, whose line solves run on one
+ * CPU core).  The line relaxation kernels have several launch shapes (chunks per wavefront,
+ * wavefronts per line, lines or columns per workgroup) that all give bitwise identical results.
+ * With autotuning on (the default), solver setup times each candidate shape once per level and
+ * direction and records the fastest in a process-wide table used by every later launch of the
+ * same precision, stencil, direction, extents and batch size.  cdr_hip_autotune(0) turns it off
+ * (shape 0 runs where nothing is recorded); returns the previous setting.
+ * cdr_hip_autotune_report writes the recorded choices as a JSON list into buf (len bytes,
+ * NUL-terminated, truncated if short) and returns the full length. */
+int cdr_hip_autotune(int on);
+int cdr_hip_autotune_report(char *buf, int len);
+/* Number of candidate shapes for lines of direction dir (0: x, 1: y) at fp_bits (32 or 64)
+ * precision, stencil nstncl and extents ii x jj; cdr_hip_set_line_shape2d pins one of them for
+ * those extents and batch size nb (shape -1 removes the entry; returns a cdr_status).
+ * cdr_hip_line_shapes2d returns -CDR_ERR_ARG for invalid arguments. */
+int cdr_hip_line_shapes2d(int dir, int fp_bits, int nstncl, int ii, int jj);
+int cdr_hip_set_line_shape2d(int dir, int fp_bits, int nstncl, int ii, int jj, int nb, int shape);
+
 /* ---------------------------------------------------------------------------------------
  * 2D kernels (FP64 and FP32 variants where the V-cycle needs them)
  * ------------------------------------------------------------------------------------- */
@@ -103,11 +121,9 @@ int cdr_hip_relax_lines_y2d_f32(const float *so, int nstncl, const float *qf, fl
                                 void *stream);
 
 /* One line-xy smoothing step: x-lines then y-lines (DOWN) or y then x (UP), the composition
- * Cedar's smoother makes of the two line kernels (include/cedar/multilevel.h:175-178, 204-207).
- * Each sweep relaxes both colours in one launch, out of place through `work` (one grid of
- * ii*jj, device, distinct from q; its contents on return are unspecified), so that every
- * coupling plane is read once per sweep.  Results are bit for bit those of
- * cdr_hip_relax_lines_x2d followed by cdr_hip_relax_lines_y2d (or y then x). */
+ * Cedar's smoother makes of the two line kernels (include/cedar/multilevel.h:175-178, 204-207),
+ * bit for bit cdr_hip_relax_lines_x2d followed by cdr_hip_relax_lines_y2d (or y then x).
+ * `work` is not used and may be NULL. */
 int cdr_hip_relax_lines_xy2d_f64(const double *so, int nstncl, const double *qf, double *q,
                                  const double *sorx, const double *sory, double *work, int ii,
                                  int jj, int updown, void *stream);

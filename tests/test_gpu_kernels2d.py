@@ -3,12 +3,14 @@
 Bitwise: residual, 1/c, Gauss-Seidel, restriction, interpolation, operator-induced
 interpolation, line factorisation, coarse Cholesky (same operation order, no FMA contraction).
 Within 1e-13 (relative to the largest entry): Galerkin coarse operator (formed as P^T A P) and
-line relaxation (wavefront affine scans instead of the sequential LDL^T substitution).
+line relaxation (wavefront affine scans instead of the sequential LDL^T substitution; every
+launch shape of the line kernels gives bitwise identical results).
 FP32 kernels: within 1e-5 relative of the FP64 oracle.
 """
 import numpy as np
 import pytest
 
+from cedar_amd import _abi
 from oracle import oracle as O
 from tests.golden.make_golden import rand_so
 
@@ -119,7 +121,8 @@ def test_pointwise_kernels_bitwise(C, nx, ny, ns):
         assert np.array_equal(host(qq), O.interp_add(q, qc, res, so, ci)[0])
 
 
-LINE_SIZES = SIZES + [(1000, 7), (7, 1000), (700, 33), (33, 700), (300, 5000), (4095, 9)]
+LINE_SIZES = SIZES + [(1000, 7), (7, 1000), (700, 33), (33, 700), (300, 5000), (4095, 9),
+              (1500, 9), (9, 1500), (9, 4095)]
 
 
 @pytest.mark.parametrize("nx,ny", LINE_SIZES)
@@ -139,7 +142,8 @@ def test_line_relaxation(C, nx, ny, ns, direction):
         assert_close(host(qq), O.relax_lines(so, qf, q, sor, ud, direction), 1e-13)
 
 
-@pytest.mark.parametrize("nx,ny", [(700, 33), (33, 700), (4095, 5), (5, 4095), (130, 129)])
+@pytest.mark.parametrize("nx,ny", [(700, 33), (33, 700), (4095, 5), (5, 4095), (130, 129),
+                                   (2047, 7), (7, 2047), (1500, 9), (9, 1500)])
 @pytest.mark.parametrize("direction", ["x", "y"])
 def test_line_relaxation_fp32(C, nx, ny, direction):
     so, q, qf, _ = random_case(nx, ny, 5, 3 * nx + ny + (direction == "y"))
@@ -232,9 +236,9 @@ XY_SIZES = [(3, 3), (4, 4), (5, 8), (9, 7), (33, 17), (64, 65), (130, 129), (257
 @pytest.mark.parametrize("nx,ny", XY_SIZES)
 @pytest.mark.parametrize("ns", [3, 5])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
-def test_fused_line_xy_equals_sequential_sweeps(C, nx, ny, ns, dtype):
-    """The fused two-colour sweeps (x then y down, y then x up) are bit for bit the sequential
-    per-colour kernels, ghost ring untouched."""
+def test_line_xy_equals_sequential_sweeps(C, nx, ny, ns, dtype):
+    """One line-xy step (x then y down, y then x up) is bit for bit the two line kernels, ghost
+    ring untouched."""
     so, q, qf, rng = random_case(nx, ny, ns, 11 * nx + ny + ns)
     q[0, :] = rng.standard_normal(nx + 2)  # nonzero ghosts must pass through unchanged
     q[:, -1] = rng.standard_normal(ny + 2)
@@ -249,26 +253,61 @@ def test_fused_line_xy_equals_sequential_sweeps(C, nx, ny, ns, dtype):
         for d in order:
             C.relax_lines(f(so), f(qf), q2, f(sx if d == "x" else sy), ud, d)
         assert torch.equal(q1, q2), ud
+        assert torch.equal(q1[0], q2[0]) and torch.equal(q1[:, -1], f(q)[:, -1])
 
 
-def test_fused_line_xy_batched(C):
-    nb, nx, ny = 3, 77, 65
-    sos, qs, qfs = [], [], []
-    for k in range(nb):
-        so, q, qf, _ = random_case(nx, ny, 5, 500 + k)
-        sos.append(so)
-        qs.append(q)
-        qfs.append(qf)
-    sx = np.stack([O.setup_lines(so, "x")[0] for so in sos])
-    sy = np.stack([O.setup_lines(so, "y")[0] for so in sos])
-    so, q, qf = np.stack(sos), np.stack(qs), np.stack(qfs)
-    work = torch.empty_like(dev(q))
-    # a shared work buffer sliced per problem (the batched launch itself is exercised by the
-    # batched solver tests); each must equal its sequential sweeps
-    for k in range(nb):
-        a = dev(q[k])
-        C.relax_lines_xy(dev(so[k]), dev(qf[k]), a, dev(sx[k]), dev(sy[k]), 0, work=work[k])
-        b = dev(q[k])
-        C.relax_lines(dev(so[k]), dev(qf[k]), b, dev(sx[k]), 0, "x")
-        C.relax_lines(dev(so[k]), dev(qf[k]), b, dev(sy[k]), 0, "y")
-        assert torch.equal(a, b), k
+# line lengths covering every chunk class (1..64 chunks of 64) and partial last segments
+SHAPE_SIZES = [(5, 9), (100, 9), (200, 9), (500, 9), (1000, 9), (1500, 9), (2047, 9), (3000, 9),
+               (4095, 9)]
+
+
+@pytest.mark.parametrize("n,m", SHAPE_SIZES)
+@pytest.mark.parametrize("direction", ["x", "y"])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_every_line_shape_gives_identical_results(C, n, m, direction, dtype):
+    """All launch shapes of the line kernels (the autotuner's candidates) give bitwise identical
+    results, each within the oracle tolerance."""
+    nx, ny = (n, m) if direction == "x" else (m, n)
+    ii, jj = nx + 2, ny + 2
+    bits = 64 if dtype == torch.float64 else 32
+    lib = _abi.load()
+    for ns in (3, 5):
+        so, q, qf, _ = random_case(nx, ny, ns, 5 * n + m + ns + (direction == "y"))
+        sor, _ = O.setup_lines(so, direction)
+        ref = O.relax_lines(so, qf, q, sor, 1, direction)
+        d = 0 if direction == "x" else 1
+        nshape = lib.cdr_hip_line_shapes2d(d, bits, ns, ii, jj)
+        assert nshape >= 1
+        f = lambda a: dev(a, dtype)
+        outs = []
+        try:
+            for shape in range(nshape):
+                assert lib.cdr_hip_set_line_shape2d(d, bits, ns, ii, jj, 1, shape) == 0
+                qq = f(q)
+                C.relax_lines(f(so), f(qf), qq, f(sor), 1, direction)
+                outs.append(qq)
+        finally:
+            lib.cdr_hip_set_line_shape2d(d, bits, ns, ii, jj, 1, -1)
+        for shape in range(1, nshape):
+            assert torch.equal(outs[0], outs[shape]), (ns, shape)
+        assert_close(host(outs[0]), ref, 1e-13 if bits == 64 else 2e-5)
+        assert lib.cdr_hip_set_line_shape2d(d, bits, ns, ii, jj, 1, nshape) == 1  # out of range
+
+
+def test_autotune_records_shapes(C):
+    """Solver setup with line relaxation records a shape for every smoothing level and direction
+    that has more than one candidate (lines of 1023 points: 2 x-shapes, 3 y-shapes)."""
+    import cedar_amd
+    from cedar_amd import gallery as G
+    so, b, _ = G.heterogeneous_9pt(1023, seed=4, block=16, sigma=1.0, contrast=1e2)
+    prev = cedar_amd.cdr2.autotune(True)
+    try:
+        s = cedar_amd.Solver(so, relaxation="line-xy", precision="mixed", nrelax_pre=1,
+                             nrelax_post=1, maxiter=30, tol=1e-10)
+        rep = cedar_amd.cdr2.autotune_report()
+        fine = [r for r in rep if r["ii"] == 1025 and r["jj"] == 1025 and r["fp"] == 32]
+        assert {r["dir"] for r in fine} == {"x", "y"}
+        x, h = s.solve(torch.as_tensor(b).cuda())
+        assert h[-1] < 1e-10
+    finally:
+        cedar_amd.cdr2.autotune(prev)
