@@ -612,29 +612,60 @@ __global__ __launch_bounds__(L *W * 64) void relax_xlines_seg_k(const T *__restr
     const std::size_t row = (std::size_t)j * ii;
     const T *D = sor + row + 1;      // D[k]
     const T *E = sor + P + row + 2;  // E[k], k < n-1
-    T a[CPW], b[CPW];
-#pragma unroll
-    for (int u = 0; u < CPW; ++u) {
-        const int k = (w * CPW + u) * WAVE + lane;
-        const int kc = min(k, n - 1);
-        line_raw<T> r;
-        line_load<T, NINE, 0>(r, so, qf, q, P, ii, row + kc + 1);
-        b[u] = line_rhs<T, NINE>(r);
-        a[u] = (k > 0) ? -E[max(kc - 1, 0)] : T(0);
-    }
-    seg_forward<T, CPW, W>(
-        a, b, [&](int u) { return D[min((w * CPW + u) * WAVE + lane, n - 1)]; }, xch[0][slot], w);
     const int rl = WAVE - 1 - lane;
-#pragma unroll
-    for (int u = 0; u < CPW; ++u) {
+    auto bwd_coef = [&](int u, T &au, T &bu) {  // chunk u of the backward pass, reversed lanes
         const int k = (w * CPW + u) * WAVE + rl;
-        const T c = __shfl(b[u], rl, WAVE);
+        const T c = __shfl(bu, rl, WAVE);
         const T e = E[min(k, max(n - 2, 0))];
-        a[u] = (k < n - 1) ? -e : T(0);
-        a[u] = (k < n) ? a[u] : T(1);
-        b[u] = (k < n) ? c : T(0);
+        au = (k < n - 1) ? -e : T(0);
+        au = (k < n) ? au : T(1);
+        bu = (k < n) ? c : T(0);
+    };
+    T b[CPW];
+    if constexpr (W == 1) {
+        // one wavefront per line: scan and carry chunk by chunk, only the results kept
+        T carry = T(0);
+#pragma unroll
+        for (int u = 0; u < CPW; ++u) {
+            const int k = u * WAVE + lane;
+            const int kc = min(k, n - 1);
+            line_raw<T> r;
+            line_load<T, NINE, 0>(r, so, qf, q, P, ii, row + kc + 1);
+            T bu = line_rhs<T, NINE>(r);
+            T au = (k > 0) ? -E[max(kc - 1, 0)] : T(0);
+            scan_affine_dpp(au, bu);
+            const T y = au * carry + bu;
+            carry = last_lane(y);
+            b[u] = y / D[kc];
+        }
+        carry = T(0);
+#pragma unroll
+        for (int u = CPW - 1; u >= 0; --u) {
+            T au;
+            bwd_coef(u, au, b[u]);
+            scan_affine_dpp(au, b[u]);
+            const T x = au * carry + b[u];
+            carry = last_lane(x);
+            b[u] = x;
+        }
+    } else {
+        T a[CPW];
+#pragma unroll
+        for (int u = 0; u < CPW; ++u) {
+            const int k = (w * CPW + u) * WAVE + lane;
+            const int kc = min(k, n - 1);
+            line_raw<T> r;
+            line_load<T, NINE, 0>(r, so, qf, q, P, ii, row + kc + 1);
+            b[u] = line_rhs<T, NINE>(r);
+            a[u] = (k > 0) ? -E[max(kc - 1, 0)] : T(0);
+        }
+        seg_forward<T, CPW, W>(
+            a, b, [&](int u) { return D[min((w * CPW + u) * WAVE + lane, n - 1)]; },
+            xch[0][slot], w);
+#pragma unroll
+        for (int u = 0; u < CPW; ++u) bwd_coef(u, a[u], b[u]);
+        seg_backward<T, CPW, W>(a, b, xch[1][slot], w);
     }
-    seg_backward<T, CPW, W>(a, b, xch[1][slot], w);
     if (valid) {
 #pragma unroll
         for (int u = 0; u < CPW; ++u) {
@@ -680,14 +711,12 @@ __global__ __launch_bounds__(NC *W * 64) void relax_ylines_seg_k(const T *__rest
     const int tseg = tr / WAVE, trow = tr % WAVE;
     const bool tok = tc < nc;
     const int ti = cb + 2 * min(tc, nc - 1);
-    T a[CPW], b[CPW];
-#pragma unroll
-    for (int u = 0; u < CPW; ++u) {
+    const int rl = WAVE - 1 - lane;
+    auto fwd_a = [&](int u) {  // forward coefficient of chunk u
         const int k = (w * CPW + u) * WAVE + lane;
-        a[u] = (k > 0) ? -E[max(min(k, n - 1) - 1, 0)] : T(0);
-    }
-#pragma unroll
-    for (int g = 0; g < CPW; g += G) {
+        return (k > 0) ? -E[max(min(k, n - 1) - 1, 0)] : T(0);
+    };
+    auto form_tile = [&](int g) {  // right-hand sides of chunk rows g .. g+G-1 into the tile
 #pragma unroll
         for (int u = 0; u < G; ++u) {
             const int k = min((tseg * CPW + g + u) * WAVE + trow, n - 1);
@@ -695,24 +724,65 @@ __global__ __launch_bounds__(NC *W * 64) void relax_ylines_seg_k(const T *__rest
             line_load<T, NINE, 1>(r, so, qf, q, P, ii, pidx(ti, k + 1, ii));
             tile_s[u][tseg][tc][trow] = line_rhs<T, NINE>(r);
         }
-        lds_barrier();
-#pragma unroll
-        for (int u = 0; u < G; ++u) b[g + u] = tile_s[u][w][cs][lane];
-        lds_barrier();
-    }
-    seg_forward<T, CPW, W>(
-        a, b, [&](int u) { return D[min((w * CPW + u) * WAVE + lane, n - 1)]; }, xch[0][cs], w);
-    const int rl = WAVE - 1 - lane;
-#pragma unroll
-    for (int u = 0; u < CPW; ++u) {
+    };
+    auto bwd_coef = [&](int u, T &au, T &bu) {  // chunk u of the backward pass, reversed lanes
         const int k = (w * CPW + u) * WAVE + rl;
-        const T c = __shfl(b[u], rl, WAVE);
+        const T c = __shfl(bu, rl, WAVE);
         const T e = E[min(k, max(n - 2, 0))];
-        a[u] = (k < n - 1) ? -e : T(0);
-        a[u] = (k < n) ? a[u] : T(1);
-        b[u] = (k < n) ? c : T(0);
+        au = (k < n - 1) ? -e : T(0);
+        au = (k < n) ? au : T(1);
+        bu = (k < n) ? c : T(0);
+    };
+    T b[CPW];
+    if constexpr (W == 1) {
+        // one wavefront per column: all right-hand sides first (memory streaming only), then
+        // scan and carry chunk by chunk, so that only the results stay in registers
+#pragma unroll
+        for (int g = 0; g < CPW; g += G) {
+            form_tile(g);
+            lds_barrier();
+#pragma unroll
+            for (int u = 0; u < G; ++u) b[g + u] = tile_s[u][0][cs][lane];
+            lds_barrier();
+        }
+        T carry = T(0);
+#pragma unroll
+        for (int u = 0; u < CPW; ++u) {
+            T au = fwd_a(u);
+            scan_affine_dpp(au, b[u]);
+            const T y = au * carry + b[u];
+            carry = last_lane(y);
+            b[u] = y / D[min(u * WAVE + lane, n - 1)];
+        }
+        carry = T(0);
+#pragma unroll
+        for (int u = CPW - 1; u >= 0; --u) {
+            T au;
+            bwd_coef(u, au, b[u]);
+            scan_affine_dpp(au, b[u]);
+            const T x = au * carry + b[u];
+            carry = last_lane(x);
+            b[u] = x;
+        }
+    } else {
+        T a[CPW];
+#pragma unroll
+        for (int u = 0; u < CPW; ++u) a[u] = fwd_a(u);
+#pragma unroll
+        for (int g = 0; g < CPW; g += G) {
+            form_tile(g);
+            lds_barrier();
+#pragma unroll
+            for (int u = 0; u < G; ++u) b[g + u] = tile_s[u][w][cs][lane];
+            lds_barrier();
+        }
+        seg_forward<T, CPW, W>(
+            a, b, [&](int u) { return D[min((w * CPW + u) * WAVE + lane, n - 1)]; }, xch[0][cs],
+            w);
+#pragma unroll
+        for (int u = 0; u < CPW; ++u) bwd_coef(u, a[u], b[u]);
+        seg_backward<T, CPW, W>(a, b, xch[1][cs], w);
     }
-    seg_backward<T, CPW, W>(a, b, xch[1][cs], w);
 #pragma unroll
     for (int g = 0; g < CPW; g += G) {
 #pragma unroll
@@ -1268,14 +1338,17 @@ static int launch_shape(int c, const T *so, const T *qf, T *q, const T *sor, int
         } else if constexpr (!F64) {
             CDR_X(0, 16, NCH / 16, NCH >= 64 ? 1 : 64 / NCH);
             CDR_X(1, 8, NCH / 8, NCH >= 32 ? 1 : 2);
-            return 2;
+            CDR_X(2, NCH, 1, 4);
+            return 3;
         } else {
             CDR_X(0, 8, NCH / 8, NCH >= 32 ? 1 : 2);
             CDR_X(1, 4, NCH / 4, 1);
-            return 2;
+            CDR_X(2, NCH, 1, 4);
+            return 3;
         }
     } else {
-        constexpr int N1 = F64 ? 4 : 8;  // columns per workgroup for 64-byte rows
+        // columns per workgroup for 64-byte (N1) and 128-byte (2 N1) row segments
+        constexpr int N1 = F64 ? 4 : 8;
         if constexpr (NCH <= 4) {
             CDR_Y(0, NCH, 1, N1, NCH);
             CDR_Y(1, NCH, 1, 2 * N1, NCH < 2 ? NCH : 2);
@@ -1294,12 +1367,14 @@ static int launch_shape(int c, const T *so, const T *qf, T *q, const T *sor, int
             CDR_Y(0, 16, 2, N1, 2);
             CDR_Y(1, 32, 1, N1, 4);
             CDR_Y(2, 8, 4, N1 / 2, 2);
-            return 3;
+            CDR_Y(3, 32, 1, 2 * N1, 2);
+            return 4;
         } else {  // 16-wavefront workgroups get 128 VGPRs: narrower tiles for long segments
-            CDR_Y(0, 16, 4, N1 / 2, 2);
-            CDR_Y(1, 64 / (F64 ? 8 : 1), F64 ? 8 : 1, N1 / (F64 ? 2 : 1), 2);
+            CDR_Y(0, 64, 1, N1, 2);
+            CDR_Y(1, 16, 4, N1 / 2, 2);
             CDR_Y(2, 32, 2, N1 / 2, 2);
-            return 3;
+            CDR_Y(3, 64, 1, 2 * N1, 2);
+            return 4;
         }
     }
 #undef CDR_X
