@@ -157,8 +157,11 @@ __global__ __launch_bounds__(256) void residual_ir_k(const double *__restrict__ 
                                                      float *__restrict__ r32, int ii, int jj,
                                                      double *__restrict__ partials,
                                                      const int *__restrict__ active) {
-    const int i = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
-    const int j = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
+    // the updated iterate of the 64 x 4 block and its one-point halo, formed once per point
+    __shared__ double xs[6][66];
+    const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+    const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 4;  // halo corner of the block
+    const int i = i0 + tx + 1, j = j0 + ty + 1;
     const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.z;
     so += bz * (NINE ? 5 : 3) * P;
     qf += bz * P;
@@ -169,24 +172,32 @@ __global__ __launch_bounds__(256) void residual_ir_k(const double *__restrict__ 
     partials += bz * gridDim.x * gridDim.y;
     // a problem of a batch that has already converged keeps its iterate (active[b] == 0)
     const bool upd = HAS_E && (active == nullptr || active[bz] != 0);
+    for (int t = threadIdx.x; t < 6 * 66; t += 256) {
+        const int ti = i0 + t % 66, tj = j0 + t / 66;
+        double v = 0.0;
+        if (ti < ii && tj < jj) {
+            const std::size_t k = pidx(ti, tj, ii);
+            v = upd ? x[k] + (double)e[k] : x[k];
+        }
+        xs[t / 66][t % 66] = v;
+    }
+    __syncthreads();
+    auto X = [&](int di, int dj) { return xs[ty + 1 + dj][tx + 1 + di]; };
     double sq = 0.0;
     if (i <= ii - 2 && j <= jj - 2) {
         const std::size_t p = pidx(i, j, ii);
-        auto X = [&](std::size_t k) -> double {
-            return upd ? x[k] + (double)e[k] : x[k];
-        };
         double r = qf[p];
-        r = r + so[P * KW + p] * X(p - 1);
-        r = r + so[P * KW + p + 1] * X(p + 1);
-        r = r + so[P * KS + p] * X(p - ii);
-        r = r + so[P * KS + p + ii] * X(p + ii);
+        r = r + so[P * KW + p] * X(-1, 0);
+        r = r + so[P * KW + p + 1] * X(1, 0);
+        r = r + so[P * KS + p] * X(0, -1);
+        r = r + so[P * KS + p + ii] * X(0, 1);
         if (NINE) {
-            r = r + so[P * KSW + p] * X(p - ii - 1);
-            r = r + so[P * KNW + p + 1] * X(p - ii + 1);
-            r = r + so[P * KNW + p + ii] * X(p + ii - 1);
-            r = r + so[P * KSW + p + ii + 1] * X(p + ii + 1);
+            r = r + so[P * KSW + p] * X(-1, -1);
+            r = r + so[P * KNW + p + 1] * X(1, -1);
+            r = r + so[P * KNW + p + ii] * X(-1, 1);
+            r = r + so[P * KSW + p + ii + 1] * X(1, 1);
         }
-        const double xc = X(p);
+        const double xc = X(0, 0);
         r = r - so[p] * xc;
         xnew[p] = xc;
         r32[p] = (float)r;
@@ -716,15 +727,23 @@ __global__ __launch_bounds__(NC *W * 64) void relax_ylines_seg_k(const T *__rest
         const int k = (w * CPW + u) * WAVE + lane;
         return (k > 0) ? -E[max(min(k, n - 1) - 1, 0)] : T(0);
     };
-    auto form_tile = [&](int g) {  // right-hand sides of chunk rows g .. g+G-1 into the tile
+    // Right-hand sides of chunk rows g .. g+G-1 go through the tile.  The raw operands of the
+    // next group are loaded before the current group's barrier (software pipelining), so one
+    // group of loads is always in flight across the LDS exchange.
+    line_raw<T> raw[G];
+    auto load_group = [&](int g) {
 #pragma unroll
         for (int u = 0; u < G; ++u) {
             const int k = min((tseg * CPW + g + u) * WAVE + trow, n - 1);
-            line_raw<T> r;
-            line_load<T, NINE, 1>(r, so, qf, q, P, ii, pidx(ti, k + 1, ii));
-            tile_s[u][tseg][tc][trow] = line_rhs<T, NINE>(r);
+            line_load<T, NINE, 1>(raw[u], so, qf, q, P, ii, pidx(ti, k + 1, ii));
         }
     };
+    auto form_tile = [&](int g) {
+#pragma unroll
+        for (int u = 0; u < G; ++u) tile_s[u][tseg][tc][trow] = line_rhs<T, NINE>(raw[u]);
+        if (g + G < CPW) load_group(g + G);
+    };
+    load_group(0);
     auto bwd_coef = [&](int u, T &au, T &bu) {  // chunk u of the backward pass, reversed lanes
         const int k = (w * CPW + u) * WAVE + rl;
         const T c = __shfl(bu, rl, WAVE);
