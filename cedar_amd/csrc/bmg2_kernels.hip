@@ -379,6 +379,22 @@ __device__ inline void scan_affine_dpp(T &a, T &b) {
     dpp_scan_step<0x143, 0xC>(a, b);  // row_bcast:31 -> rows 2 and 3
 }
 
+// the multiplier part of scan_affine_dpp alone (the same products in the same order)
+template <int CTRL, int RMASK, typename T>
+__device__ inline void dpp_prod_step(T &a) {
+    a = a * dpp_move<CTRL, RMASK>(a, T(1));
+}
+
+template <typename T>
+__device__ inline void scan_prod_dpp(T &a) {
+    dpp_prod_step<0x111, 0xF>(a);
+    dpp_prod_step<0x112, 0xF>(a);
+    dpp_prod_step<0x114, 0xF>(a);
+    dpp_prod_step<0x118, 0xF>(a);
+    dpp_prod_step<0x142, 0xA>(a);
+    dpp_prod_step<0x143, 0xC>(a);
+}
+
 // value of lane 63, broadcast (v_readlane: no LDS round trip)
 __device__ inline float last_lane(float v) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), WAVE - 1));
@@ -544,11 +560,82 @@ struct seg_map {
     T a, b;  // carry_out = a * carry_in + b
 };
 
-// Forward recurrence y_k = a_k y_{k-1} + b_k over the wavefront's chunks; on exit
+// Forward recurrence y_k = a_k y_{k-1} + b_k over the wavefront's chunks.  On entry b[u] holds
+// chunk u's right-hand sides; afun(u) returns its multipliers a_k (reloaded when needed, so that
+// only b stays in registers) and dval(u) its pivots.  On exit b[u] = y_k / d_k.  xch: this
+// line's LDS slots, one per chunk of the line.
+template <typename T, int CPW, int W, typename AF, typename DF>
+__device__ inline void seg_forward(T (&b)[CPW], AF afun, DF dval, seg_map<T> *xch, int w) {
+    if (W > 1) {
+#pragma unroll
+        for (int u = 0; u < CPW; ++u) {
+            T a = afun(u);
+            scan_affine_dpp(a, b[u]);
+            const seg_map<T> m{last_lane(a), last_lane(b[u])};
+            if (w < W - 1 && threadIdx.x % WAVE == 0) xch[w * CPW + u] = m;
+        }
+        lds_barrier();
+        // the multipliers are reloaded below rather than kept live across the exchange
+        asm volatile("" ::: "memory");
+    }
+    T carry = T(0);
+    for (int m = 0; m < w * CPW; ++m) carry = xch[m].a * carry + xch[m].b;
+#pragma unroll
+    for (int u = 0; u < CPW; ++u) {
+        T a = afun(u);
+        if (W > 1)
+            scan_prod_dpp(a);
+        else
+            scan_affine_dpp(a, b[u]);
+        const T y = a * carry + b[u];
+        carry = last_lane(y);
+        b[u] = y / dval(u);
+    }
+}
+
+// Backward recurrence x_k = a_k x_{k+1} + b_k, lanes in reversed element order (lane l of chunk
+// c holds element 64c + 63 - l); the carry runs from the line's last chunk to its first.
+// coef(u, a, b[u]) turns b[u] (the forward results) into the chunk's coefficients and returns
+// the multipliers in a; afun(u) returns the multipliers alone.  On exit b[u] = x.
+template <typename T, int CPW, int W, typename CF, typename AF>
+__device__ inline void seg_backward(T (&b)[CPW], CF coef, AF afun, seg_map<T> *xch, int w) {
+    if (W > 1) {
+#pragma unroll
+        for (int u = 0; u < CPW; ++u) {
+            T a;
+            coef(u, a, b[u]);
+            scan_affine_dpp(a, b[u]);
+            const seg_map<T> m{last_lane(a), last_lane(b[u])};
+            if (w > 0 && threadIdx.x % WAVE == 0) xch[w * CPW + u] = m;
+        }
+        lds_barrier();
+        // the multipliers are reloaded below rather than kept live across the exchange
+        asm volatile("" ::: "memory");
+    }
+    T carry = T(0);
+    for (int m = W * CPW - 1; m >= (w + 1) * CPW; --m) carry = xch[m].a * carry + xch[m].b;
+#pragma unroll
+    for (int u = CPW - 1; u >= 0; --u) {
+        T a;
+        if (W > 1) {
+            a = afun(u);
+            scan_prod_dpp(a);
+        } else {
+            coef(u, a, b[u]);
+            scan_affine_dpp(a, b[u]);
+        }
+        const T x = a * carry + b[u];
+        carry = last_lane(x);
+        b[u] = x;
+    }
+}
+
+// Register-rich forms of the two recurrences (scanned multipliers kept in a[]), used by the
+// x-line kernel: forward recurrence y_k = a_k y_{k-1} + b_k over the wavefront's chunks; on exit
 // b[u] = y_k / D(u) with D(u) the pivots of chunk u (a functor, loaded after the scans).
 // xch: this line's LDS slots, one per chunk of the line.
 template <typename T, int CPW, int W, typename DF>
-__device__ inline void seg_forward(T (&a)[CPW], T (&b)[CPW], DF dval, seg_map<T> *xch, int w) {
+__device__ inline void seg_forward_ab(T (&a)[CPW], T (&b)[CPW], DF dval, seg_map<T> *xch, int w) {
 #pragma unroll
     for (int u = 0; u < CPW; ++u) scan_affine_dpp(a[u], b[u]);
     T carry = T(0);
@@ -575,7 +662,7 @@ __device__ inline void seg_forward(T (&a)[CPW], T (&b)[CPW], DF dval, seg_map<T>
 // c holds element 64c + 63 - l); the carry runs from the line's last chunk to its first.  On
 // exit b[u] = x.
 template <typename T, int CPW, int W>
-__device__ inline void seg_backward(T (&a)[CPW], T (&b)[CPW], seg_map<T> *xch, int w) {
+__device__ inline void seg_backward_ab(T (&a)[CPW], T (&b)[CPW], seg_map<T> *xch, int w) {
 #pragma unroll
     for (int u = 0; u < CPW; ++u) scan_affine_dpp(a[u], b[u]);
     T carry = T(0);
@@ -670,12 +757,12 @@ __global__ __launch_bounds__(L *W * 64) void relax_xlines_seg_k(const T *__restr
             b[u] = line_rhs<T, NINE>(r);
             a[u] = (k > 0) ? -E[max(kc - 1, 0)] : T(0);
         }
-        seg_forward<T, CPW, W>(
+        seg_forward_ab<T, CPW, W>(
             a, b, [&](int u) { return D[min((w * CPW + u) * WAVE + lane, n - 1)]; },
             xch[0][slot], w);
 #pragma unroll
         for (int u = 0; u < CPW; ++u) bwd_coef(u, a[u], b[u]);
-        seg_backward<T, CPW, W>(a, b, xch[1][slot], w);
+        seg_backward_ab<T, CPW, W>(a, b, xch[1][slot], w);
     }
     if (valid) {
 #pragma unroll
@@ -744,64 +831,32 @@ __global__ __launch_bounds__(NC *W * 64) void relax_ylines_seg_k(const T *__rest
         if (g + G < CPW) load_group(g + G);
     };
     load_group(0);
-    auto bwd_coef = [&](int u, T &au, T &bu) {  // chunk u of the backward pass, reversed lanes
+    auto bwd_a = [&](int u) {  // backward multipliers of chunk u, reversed lanes
+        const int k = (w * CPW + u) * WAVE + rl;
+        const T e = E[min(k, max(n - 2, 0))];
+        const T a = (k < n - 1) ? -e : T(0);
+        return (k < n) ? a : T(1);
+    };
+    auto bwd_coef = [&](int u, T &au, T &bu) {
         const int k = (w * CPW + u) * WAVE + rl;
         const T c = __shfl(bu, rl, WAVE);
-        const T e = E[min(k, max(n - 2, 0))];
-        au = (k < n - 1) ? -e : T(0);
-        au = (k < n) ? au : T(1);
+        au = bwd_a(u);
         bu = (k < n) ? c : T(0);
     };
+    // all right-hand sides first (memory streaming only), then the scans
     T b[CPW];
-    if constexpr (W == 1) {
-        // one wavefront per column: all right-hand sides first (memory streaming only), then
-        // scan and carry chunk by chunk, so that only the results stay in registers
 #pragma unroll
-        for (int g = 0; g < CPW; g += G) {
-            form_tile(g);
-            lds_barrier();
+    for (int g = 0; g < CPW; g += G) {
+        form_tile(g);
+        lds_barrier();
 #pragma unroll
-            for (int u = 0; u < G; ++u) b[g + u] = tile_s[u][0][cs][lane];
-            lds_barrier();
-        }
-        T carry = T(0);
-#pragma unroll
-        for (int u = 0; u < CPW; ++u) {
-            T au = fwd_a(u);
-            scan_affine_dpp(au, b[u]);
-            const T y = au * carry + b[u];
-            carry = last_lane(y);
-            b[u] = y / D[min(u * WAVE + lane, n - 1)];
-        }
-        carry = T(0);
-#pragma unroll
-        for (int u = CPW - 1; u >= 0; --u) {
-            T au;
-            bwd_coef(u, au, b[u]);
-            scan_affine_dpp(au, b[u]);
-            const T x = au * carry + b[u];
-            carry = last_lane(x);
-            b[u] = x;
-        }
-    } else {
-        T a[CPW];
-#pragma unroll
-        for (int u = 0; u < CPW; ++u) a[u] = fwd_a(u);
-#pragma unroll
-        for (int g = 0; g < CPW; g += G) {
-            form_tile(g);
-            lds_barrier();
-#pragma unroll
-            for (int u = 0; u < G; ++u) b[g + u] = tile_s[u][w][cs][lane];
-            lds_barrier();
-        }
-        seg_forward<T, CPW, W>(
-            a, b, [&](int u) { return D[min((w * CPW + u) * WAVE + lane, n - 1)]; }, xch[0][cs],
-            w);
-#pragma unroll
-        for (int u = 0; u < CPW; ++u) bwd_coef(u, a[u], b[u]);
-        seg_backward<T, CPW, W>(a, b, xch[1][cs], w);
+        for (int u = 0; u < G; ++u) b[g + u] = tile_s[u][w][cs][lane];
+        lds_barrier();
     }
+    seg_forward<T, CPW, W>(
+        b, fwd_a, [&](int u) { return D[min((w * CPW + u) * WAVE + lane, n - 1)]; }, xch[0][cs],
+        w);
+    seg_backward<T, CPW, W>(b, bwd_coef, bwd_a, xch[1][cs], w);
 #pragma unroll
     for (int g = 0; g < CPW; g += G) {
 #pragma unroll
