@@ -11,7 +11,8 @@ resident in HBM.
 
 Prints one JSON line (rank 0).  `--workload c3` instead times the 3D plane-relaxation solve
 (255^3 anisotropic 7-point finite-volume problem, plane-xy V(1,1), FP64 and mixed) and
-`--workload c4` the batched multi-size 2D solves; each prints its own line.
+`--workload c4` the batched multi-size 2D solves and `--workload c5` the differentiated solves
+(forward + adjoint gradient, 16 x 1025^2); each prints its own line.
 """
 import argparse
 import ctypes
@@ -198,6 +199,62 @@ def bench_c4(args):
         "cycles_min_max": [min(iters), max(iters)], "worst_rho": worst, "gen_s": gen_s}))
 
 
+def bench_c5(args):
+    """Differentiated solves: 16 problems of 1025^2 nodes, k = exp(theta) per cell with theta a
+    5-cell box-filtered N(0,1) field (seeded), f ~ U[-1,1]; L = 0.5 ||u - u_obs||^2 with u_obs
+    from a second seeded theta plus 1e-3 Gaussian noise.  One step = forward (assembly, setup,
+    mixed solve to 1e-10) + backward (adjoint solve on the same hierarchy + gradient kernel)
+    for the whole batch.  The gradient is compared with an FP64 solve pair (tol 1e-12)."""
+    import torch
+
+    from cedar_amd import adjoint as A
+    from cedar_amd import gallery as G
+    nb, n = 16, args.n5
+    t0 = time.time()
+    theta = np.stack([G.smooth_theta(n, 700 + t) for t in range(nb)])
+    theta_obs = np.stack([G.smooth_theta(n, 900 + t) for t in range(nb)])
+    f = np.stack([G.rhs_uniform(n, n, 800 + t) for t in range(nb)])
+    noise = np.stack([G.normal(1000 + t, (n + 2) ** 2).reshape(n + 2, n + 2) for t in range(nb)])
+    gen_s = time.time() - t0
+    dev = lambda a: torch.as_tensor(a, dtype=torch.float64).cuda()
+    f_d = dev(f)
+    with torch.no_grad():
+        uobs = A.solve(dev(theta_obs), f_d, precision="f64", tol=1e-12)
+    uobs = uobs + 1e-3 * dev(noise)
+    uobs[..., 0, :] = 0.0
+    uobs[..., -1, :] = 0.0
+    uobs[..., :, 0] = 0.0
+    uobs[..., :, -1] = 0.0
+
+    def step(**kw):
+        th = dev(theta).requires_grad_(True)
+        u = A.solve(th, f_d, **kw)
+        loss = 0.5 * ((u - uobs) ** 2)[..., 1:-1, 1:-1].sum()
+        loss.backward()
+        return float(loss.item()), th.grad
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        loss, g = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / args.steps
+    loss64, g64 = step(precision="f64", tol=1e-12)
+    rel = float((g - g64).abs().max() / g64.abs().max())
+    print(json.dumps({
+        "metric": "differentiated 2D solves: forward + adjoint gradient for the batch",
+        "value": dt, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": False,
+        "dtype": "f32 cycle + f64 refinement (forward and adjoint), f64 gradient kernel",
+        "data": "synthetic (seeded smoothed Gaussian log-coefficients, uniform RHS)",
+        "config": {"workload": f"{nb} x {n}^2 9-point Q1 diffusion, k = exp(theta), "
+                               "L = 0.5||u - u_obs||^2, line-xy V(1,1), tol 1e-10"},
+        "gradients_per_s": nb / dt, "loss": loss, "loss_fp64": loss64,
+        "grad_rel_err_vs_fp64": rel, "gen_s": gen_s}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,7 +267,8 @@ def main():
     ap.add_argument("--kernel-only", action="store_true",
                     help="run only the dominant kernel (fine-level FP32 y-line sweep) and a plain "
                          "256 MiB device copy, for hardware-counter runs")
-    ap.add_argument("--workload", choices=("c2", "c3", "c4"), default="c2")
+    ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5"), default="c2")
+    ap.add_argument("--n5", type=int, default=1025)
     ap.add_argument("--nprob", type=int, default=96)
     ap.add_argument("--n3", type=int, default=255)
     ap.add_argument("--relax3", default="plane-xy")
@@ -225,6 +283,9 @@ def main():
         return
     if args.workload == "c4":
         bench_c4(args)
+        return
+    if args.workload == "c5":
+        bench_c5(args)
         return
 
     import torch
