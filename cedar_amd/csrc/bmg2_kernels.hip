@@ -25,6 +25,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <utility>
 
 namespace cdr {
 namespace k2 {
@@ -494,15 +495,22 @@ struct line_raw {
 
 // Right-hand side of one point in the reference's order.  X-lines: s0/q0 = coupling to the row
 // below, s1/q1 = to the row above; y-lines: the column to the left / right.
-template <typename T, bool NINE>
+// SWAP adds the two NW terms in the opposite order: an x-line of the transposed grid then sums
+// its terms in exactly the order of the corresponding y-line of the original grid.
+template <typename T, bool NINE, bool SWAP = false>
 __device__ inline T line_rhs(const line_raw<T> &r) {
     T rhs = r.qf;
     rhs = rhs + r.s0 * r.q0;
     rhs = rhs + r.s1 * r.q1;
     if (NINE) {
         rhs = rhs + r.sw0 * r.q0m;
-        rhs = rhs + r.nw0 * r.q0p;
-        rhs = rhs + r.nw1 * r.q1m;
+        if (SWAP) {
+            rhs = rhs + r.nw1 * r.q1m;
+            rhs = rhs + r.nw0 * r.q0p;
+        } else {
+            rhs = rhs + r.nw0 * r.q0p;
+            rhs = rhs + r.nw1 * r.q1m;
+        }
         rhs = rhs + r.sw1 * r.q1p;
     }
     return rhs;
@@ -685,8 +693,9 @@ __device__ inline void seg_backward_ab(T (&a)[CPW], T (&b)[CPW], seg_map<T> *xch
     }
 }
 
-// x-lines of one colour, in place: L lines per workgroup, W wavefronts per line.
-template <typename T, bool NINE, int CPW, int W, int L>
+// x-lines of one colour, in place: L lines per workgroup, W wavefronts per line.  TR: the grid
+// is the transpose of a y-line problem (right-hand side terms in the y-line order).
+template <typename T, bool NINE, int CPW, int W, int L, bool TR = false>
 __global__ __launch_bounds__(L *W * 64) void relax_xlines_seg_k(const T *__restrict__ so,
                                                                const T *__restrict__ qf,
                                                                T *__restrict__ q,
@@ -729,7 +738,7 @@ __global__ __launch_bounds__(L *W * 64) void relax_xlines_seg_k(const T *__restr
             const int kc = min(k, n - 1);
             line_raw<T> r;
             line_load<T, NINE, 0>(r, so, qf, q, P, ii, row + kc + 1);
-            T bu = line_rhs<T, NINE>(r);
+            T bu = line_rhs<T, NINE, TR>(r);
             T au = (k > 0) ? -E[max(kc - 1, 0)] : T(0);
             scan_affine_dpp(au, bu);
             const T y = au * carry + bu;
@@ -754,7 +763,7 @@ __global__ __launch_bounds__(L *W * 64) void relax_xlines_seg_k(const T *__restr
             const int kc = min(k, n - 1);
             line_raw<T> r;
             line_load<T, NINE, 0>(r, so, qf, q, P, ii, row + kc + 1);
-            b[u] = line_rhs<T, NINE>(r);
+            b[u] = line_rhs<T, NINE, TR>(r);
             a[u] = (k > 0) ? -E[max(kc - 1, 0)] : T(0);
         }
         seg_forward_ab<T, CPW, W>(
@@ -1345,6 +1354,34 @@ void setup_lines_y(const double *so, int nstncl, double *sor, int ii, int jj, in
     setup_lines_y_k<<<dim3((n + 63) / 64, nb), 64, 0, s>>>(so, sor, ii, jj, nstncl, info);
 }
 
+// dst (cols x rows) = transpose of src (rows x cols), row-major, for each problem of a batch;
+// 64 x 64 tiles through LDS
+template <typename T>
+__global__ __launch_bounds__(256) void transpose_k(const T *__restrict__ src, T *__restrict__ dst,
+                                                  int rows, int cols) {
+    __shared__ T tile[64][65];
+    const std::size_t off = (std::size_t)blockIdx.z * rows * cols;
+    src += off;
+    dst += off;
+    const int c0 = blockIdx.x * 64, r0 = blockIdx.y * 64;
+    const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+    for (int k = ty; k < 64; k += 4) {
+        const int r = r0 + k, c = c0 + tx;
+        if (r < rows && c < cols) tile[k][tx] = src[(std::size_t)r * cols + c];
+    }
+    __syncthreads();
+    for (int k = ty; k < 64; k += 4) {
+        const int c = c0 + k, r = r0 + tx;
+        if (c < cols && r < rows) dst[(std::size_t)c * rows + r] = tile[tx][k];
+    }
+}
+
+template <typename T>
+void transpose(const T *src, T *dst, int rows, int cols, hipStream_t s, int nb) {
+    const dim3 g((unsigned)((cols + 63) / 64), (unsigned)((rows + 63) / 64), (unsigned)nb);
+    transpose_k<T><<<g, 256, 0, s>>>(src, dst, rows, cols);
+}
+
 // ---- line kernel shapes and their autotuner ---------------------------------------------------
 // A shape is (CPW, W, L) for x-lines (chunks per wavefront, wavefronts per line, lines per
 // workgroup) and (CPW, W, NC, G) for y-lines (..., columns per workgroup, chunk rows per LDS
@@ -1353,13 +1390,13 @@ void setup_lines_y(const double *so, int nstncl, double *sor, int ii, int jj, in
 // timing: tune_lines() times every candidate once on the caller's data and records the fastest in
 // a process-wide table that the launchers consult; without an entry, candidate 0 runs.
 
-template <typename T, bool NINE, int CPW, int W, int L>
+template <typename T, bool NINE, int CPW, int W, int L, bool TR>
 static void launch_x(const T *so, const T *qf, T *q, const T *sor, int ii, int jj, int first,
                      int nlines, hipStream_t s, int nb) {
     const int ntiles = (nlines + L - 1) / L;
     const dim3 blocks((unsigned)(8 * ((ntiles + 7) / 8)), (unsigned)nb);
-    relax_xlines_seg_k<T, NINE, CPW, W, L><<<blocks, L * W * 64, 0, s>>>(so, qf, q, sor, ii, jj,
-                                                                        first, nlines, ntiles);
+    relax_xlines_seg_k<T, NINE, CPW, W, L, TR><<<blocks, L * W * 64, 0, s>>>(
+        so, qf, q, sor, ii, jj, first, nlines, ntiles);
 }
 
 template <typename T, bool NINE, int CPW, int W, int NC, int G>
@@ -1383,16 +1420,18 @@ static std::string shape_text(const char *k, int cpw, int w, const char *tn, int
 }
 
 // With `name` non-null nothing is launched and *name receives candidate c's description.
+// DIR 0: x-lines, 1: y-lines, 2: x-lines of a transposed y-line problem.
 template <typename T, bool NINE, int DIR, int NCH>
 static int launch_shape(int c, const T *so, const T *qf, T *q, const T *sor, int ii, int jj,
                         int first, int nlines, hipStream_t s, int nb, std::string *name = nullptr) {
+    constexpr bool TR = DIR == 2;
     constexpr bool F64 = sizeof(T) == 8;
 #define CDR_X(k, C, W, L)                                                                       \
     if (c == k) {                                                                               \
         if (name)                                                                               \
             *name = shape_text("relax_xlines_seg_k", C, W, "L", L, 0);                          \
         else                                                                                    \
-            launch_x<T, NINE, C, W, L>(so, qf, q, sor, ii, jj, first, nlines, s, nb);           \
+            launch_x<T, NINE, C, W, L, TR>(so, qf, q, sor, ii, jj, first, nlines, s, nb);       \
     }
 #define CDR_Y(k, C, W, NC, G)                                                                   \
     if (c == k) {                                                                               \
@@ -1401,47 +1440,68 @@ static int launch_shape(int c, const T *so, const T *qf, T *q, const T *sor, int
         else                                                                                    \
             launch_y<T, NINE, C, W, NC, G>(so, qf, q, sor, ii, jj, first, nlines, s, nb);       \
     }
-    if constexpr (DIR == 0) {
-        if constexpr (NCH <= 4) {
+    // x: (CPW, W, L); y: (CPW, W, NC, G).  At most 16 wavefronts per workgroup.
+    if constexpr (DIR != 1) {
+        if constexpr (NCH <= 2) {
             CDR_X(0, NCH, 1, 4);
             return 1;
+        } else if constexpr (NCH == 4) {
+            CDR_X(0, 4, 1, 4);
+            CDR_X(1, 1, 4, 1);
+            return 2;
         } else if constexpr (NCH == 8) {
             CDR_X(0, 8, 1, 4);
             CDR_X(1, 4, 2, 2);
-            return 2;
-        } else if constexpr (!F64) {
-            CDR_X(0, 16, NCH / 16, NCH >= 64 ? 1 : 64 / NCH);
-            CDR_X(1, 8, NCH / 8, NCH >= 32 ? 1 : 2);
-            CDR_X(2, NCH, 1, 4);
+            CDR_X(2, 2, 4, 1);
             return 3;
+        } else if constexpr (NCH == 16) {
+            CDR_X(0, 16, 1, 4);
+            CDR_X(1, 8, 2, 2);
+            CDR_X(2, 4, 4, 1);
+            CDR_X(3, 2, 8, 1);
+            return 4;
+        } else if constexpr (NCH == 32) {
+            CDR_X(0, 16, 2, 2);
+            CDR_X(1, 8, 4, 1);
+            CDR_X(2, 4, 8, 1);
+            CDR_X(3, F64 ? 2 : 32, F64 ? 16 : 1, F64 ? 1 : 4);
+            return 4;
         } else {
-            CDR_X(0, 8, NCH / 8, NCH >= 32 ? 1 : 2);
-            CDR_X(1, 4, NCH / 4, 1);
-            CDR_X(2, NCH, 1, 4);
-            return 3;
+            CDR_X(0, 16, 4, 1);
+            CDR_X(1, 8, 8, 1);
+            CDR_X(2, 4, 16, 1);
+            CDR_X(3, 64, 1, 4);
+            return 4;
         }
     } else {
         // columns per workgroup for 64-byte (N1) and 128-byte (2 N1) row segments
         constexpr int N1 = F64 ? 4 : 8;
-        if constexpr (NCH <= 4) {
+        if constexpr (NCH <= 2) {
             CDR_Y(0, NCH, 1, N1, NCH);
-            CDR_Y(1, NCH, 1, 2 * N1, NCH < 2 ? NCH : 2);
+            CDR_Y(1, NCH, 1, 2 * N1, NCH);
             return 2;
+        } else if constexpr (NCH == 4) {
+            CDR_Y(0, 4, 1, N1, 4);
+            CDR_Y(1, 4, 1, 2 * N1, 2);
+            CDR_Y(2, 1, 4, N1 / 2, 1);
+            return 3;
         } else if constexpr (NCH == 8) {
             CDR_Y(0, 8, 1, N1, 4);
             CDR_Y(1, 8, 1, 2 * N1, 2);
             CDR_Y(2, 4, 2, N1, 4);
-            return 3;
+            CDR_Y(3, 2, 4, N1 / 2, 2);
+            return 4;
         } else if constexpr (NCH == 16) {
             CDR_Y(0, 16, 1, N1, 4);
             CDR_Y(1, 8, 2, N1, 4);
             CDR_Y(2, 16, 1, 2 * N1, 2);
-            return 3;
+            CDR_Y(3, 4, 4, N1 / 2, 4);
+            return 4;
         } else if constexpr (NCH == 32) {
             CDR_Y(0, 16, 2, N1, 2);
             CDR_Y(1, 32, 1, N1, 4);
             CDR_Y(2, 8, 4, N1 / 2, 2);
-            CDR_Y(3, 32, 1, 2 * N1, 2);
+            CDR_Y(3, 4, 8, N1 / 4, 4);
             return 4;
         } else {  // 16-wavefront workgroups get 128 VGPRs: narrower tiles for long segments
             CDR_Y(0, 64, 1, N1, 2);
@@ -1466,7 +1526,7 @@ static int chunk_class(int n) {
 template <typename T, bool NINE, int DIR>
 static int launch_class(int c, const T *so, const T *qf, T *q, const T *sor, int ii, int jj,
                         int first, int nlines, hipStream_t s, int nb, std::string *name = nullptr) {
-    const int n = (DIR == 0) ? ii - 2 : jj - 2;
+    const int n = (DIR != 1) ? ii - 2 : jj - 2;
 #define CDR_C(K)                                                                                  \
     case K:                                                                                       \
         return launch_shape<T, NINE, DIR, K>(c, so, qf, q, sor, ii, jj, first, nlines, s, nb, name)
@@ -1479,11 +1539,15 @@ static int launch_class(int c, const T *so, const T *qf, T *q, const T *sor, int
         CDR_C(32);
         CDR_C(64);
     default:  // lines longer than 4096: stage the forward results in q itself
-        if (c == 0 && name) *name = "relax_lines_k";
-        if (c == 0 && !name)
-            relax_lines_k<T, NINE, DIR><<<dim3((unsigned)((nlines + 3) / 4), (unsigned)nb), 256, 0,
-                                          s>>>(so, qf, q, sor, ii, jj, first, nlines);
-        return 1;
+        if constexpr (DIR == 2) {
+            return 0;  // no transposed path for such lines
+        } else {
+            if (c == 0 && name) *name = "relax_lines_k";
+            if (c == 0 && !name)
+                relax_lines_k<T, NINE, DIR><<<dim3((unsigned)((nlines + 3) / 4), (unsigned)nb), 256,
+                                              0, s>>>(so, qf, q, sor, ii, jj, first, nlines);
+            return 1;
+        }
     }
 #undef CDR_C
 }
@@ -1517,17 +1581,22 @@ static int launch_colour(int c, const T *so, int nstncl, const T *qf, T *q, cons
 
 static std::string shape_name(const std::array<int, 6> &k, int c) {
     std::string name;
+    const int ns = k[2] ? 5 : 3;
+    // direction 2 (y-lines through the transposed grid): an x-line shape on the swapped extents
+    if (k[0] == 2 && c < 0) return "direct y-lines";
+    const int dir = k[0] == 1 ? 1 : 0;
+    const int ii = k[0] == 2 ? k[4] : k[3], jj = k[0] == 2 ? k[3] : k[4];
     if (k[1] == 8)
-        (void)(k[0] == 0 ? launch_colour<double, 0>(c, nullptr, k[2] ? 5 : 3, nullptr, nullptr,
-                                                    nullptr, k[3], k[4], 1, 1, nullptr, 1, &name)
-                         : launch_colour<double, 1>(c, nullptr, k[2] ? 5 : 3, nullptr, nullptr,
-                                                    nullptr, k[3], k[4], 1, 1, nullptr, 1, &name));
+        (void)(dir == 0 ? launch_colour<double, 0>(c, nullptr, ns, nullptr, nullptr, nullptr, ii,
+                                                   jj, 1, 1, nullptr, 1, &name)
+                        : launch_colour<double, 1>(c, nullptr, ns, nullptr, nullptr, nullptr, ii,
+                                                   jj, 1, 1, nullptr, 1, &name));
     else
-        (void)(k[0] == 0 ? launch_colour<float, 0>(c, nullptr, k[2] ? 5 : 3, nullptr, nullptr,
-                                                   nullptr, k[3], k[4], 1, 1, nullptr, 1, &name)
-                         : launch_colour<float, 1>(c, nullptr, k[2] ? 5 : 3, nullptr, nullptr,
-                                                   nullptr, k[3], k[4], 1, 1, nullptr, 1, &name));
-    return name;
+        (void)(dir == 0 ? launch_colour<float, 0>(c, nullptr, ns, nullptr, nullptr, nullptr, ii,
+                                                  jj, 1, 1, nullptr, 1, &name)
+                        : launch_colour<float, 1>(c, nullptr, ns, nullptr, nullptr, nullptr, ii,
+                                                  jj, 1, 1, nullptr, 1, &name));
+    return k[0] == 2 ? "transposed, " + name : name;
 }
 
 template <typename T, int DIR>
@@ -1561,6 +1630,11 @@ int set_autotune(int on) { return g_autotune.exchange(on ? 1 : 0); }
 
 int line_shape_count(int dir, int wbytes, int nstncl, int ii, int jj) {
     const hipStream_t s = nullptr;  // counting launches nothing
+    if (dir == 2) {  // x-line shapes of the transposed grid; none for lines over 4096 points
+        if (jj - 2 > 64 * WAVE) return 0;
+        dir = 0;
+        std::swap(ii, jj);
+    }
     if (wbytes == 8)
         return dir == 0 ? launch_colour<double, 0>(-1, nullptr, nstncl, nullptr, nullptr, nullptr,
                                                    ii, jj, 1, 1, s, 1)
@@ -1628,6 +1702,114 @@ int tune_lines(int dir, const T *so, int nstncl, const T *qf, T *q, const T *sor
     return best;
 }
 
+// ---- y-lines through the transposed grid ------------------------------------------------------
+// With the operator transposed once at setup (soT: c, w, s, sw, nw of the transposed grid, i.e.
+// the transposes of c, s, w, sw, nw), a y-line sweep is an x-line sweep of the transposed grid:
+// q and f are transposed, both colours are relaxed with the x-line kernel (right-hand side terms in
+// the y-line order, the y-line factors are already stored column-contiguous), and q is transposed
+// back.  The result is bitwise that of relax_lines_y.
+
+template <typename T>
+void transpose_op(const T *so, T *soT, int nstncl, int ii, int jj, hipStream_t s, int nb) {
+    const std::size_t P = (std::size_t)ii * jj;
+    // plane of soT <- plane of so: c <- c, w <- s, s <- w, sw <- sw, nw <- nw
+    const int from[5] = {KO, KS, KW, KSW, KNW};
+    for (int k = 0; k < nstncl; ++k)
+        for (int b = 0; b < nb; ++b)
+            transpose<T>(so + (std::size_t)b * nstncl * P + P * from[k],
+                         soT + (std::size_t)b * nstncl * P + P * k, jj, ii, s, 1);
+}
+
+template <typename T>
+static void relax_lines_ytr(const T *soT, int nstncl, const T *qf, T *q, const T *sory, T *qT,
+                            T *qfT, int ii, int jj, int updown, int shape, hipStream_t s, int nb,
+                            bool qfT_ready = false) {
+    transpose<T>(q, qT, jj, ii, s, nb);
+    if (!qfT_ready) transpose<T>(qf, qfT, jj, ii, s, nb);
+    const int nfix = ii - 2;  // columns of the original grid = rows of the transposed one
+    for (int t = 0; t < 2; ++t) {
+        const int first = (updown == 0) ? (t == 0 ? 2 : 1) : (t == 0 ? 1 : 2);
+        if (first > nfix) continue;
+        const int nlines = (nfix - first) / 2 + 1;
+        if (nstncl == 5)
+            launch_class<T, true, 2>(shape, soT, qfT, qT, sory, jj, ii, first, nlines, s, nb);
+        else
+            launch_class<T, false, 2>(shape, soT, qfT, qT, sory, jj, ii, first, nlines, s, nb);
+    }
+    transpose<T>(qT, q, ii, jj, s, nb);
+}
+
+static int tuned_transposed(const tune_key &k) {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    const auto it = g_tune.find(k);
+    return it == g_tune.end() ? -1 : it->second;
+}
+
+template <typename T>
+bool relax_lines_y_auto(const T *so, const T *soT, int nstncl, const T *qf, T *q, const T *sory,
+                        T *qT, T *qfT, int ii, int jj, int updown, hipStream_t s, int nb,
+                        bool qfT_ready) {
+    const int c = soT ? tuned_transposed(key_of<T>(2, nstncl, ii, jj, nb)) : -1;
+    if (c < 0) {
+        relax_lines<T, 1>(so, nstncl, qf, q, sory, ii, jj, updown, s, nb);
+        return false;
+    }
+    relax_lines_ytr<T>(soT, nstncl, qf, q, sory, qT, qfT, ii, jj, updown, c, s, nb, qfT_ready);
+    return true;
+}
+
+template <typename T>
+int tune_lines_ytr(const T *so, const T *soT, int nstncl, const T *qf, T *q, const T *sory,
+                   T *qT, T *qfT, int ii, int jj, hipStream_t s, int nb, int sweeps) {
+    sweeps = sweeps < 1 ? 1 : sweeps;
+    const tune_key k = key_of<T>(2, nstncl, ii, jj, nb);
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        if (g_tune.count(k)) return g_tune[k];
+    }
+    if (!g_autotune.load()) return -1;
+    if (tune_lines<T>(1, so, nstncl, qf, q, sory, ii, jj, s, nb) < 0) return -2;
+    const int ncand = nstncl == 5 ? launch_class<T, true, 2>(-1, soT, qf, q, sory, jj, ii, 1, 1, s, nb)
+                                  : launch_class<T, false, 2>(-1, soT, qf, q, sory, jj, ii, 1, 1, s, nb);
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return -2;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return -2;
+    }
+    auto timed = [&](auto &&run) {
+        run();  // warm-up
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < 3; ++r) run();
+        (void)hipEventRecord(e1, s);
+        float ms = -1.f;
+        if (hipEventSynchronize(e1) == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+        return ms;
+    };
+    // `sweeps` y sweeps share one right-hand side (one transpose of qf)
+    const float tdirect = timed([&] {
+        for (int k = 0; k < sweeps; ++k) relax_lines<T, 1>(so, nstncl, qf, q, sory, ii, jj, 0, s, nb);
+    });
+    int best = -1;
+    float tbest = tdirect;
+    for (int c = 0; c < ncand; ++c) {
+        const float t = timed([&] {
+            for (int k = 0; k < sweeps; ++k)
+                relax_lines_ytr<T>(soT, nstncl, qf, q, sory, qT, qfT, ii, jj, 0, c, s, nb, k > 0);
+        });
+        if (t >= 0.f && t < tbest) {
+            tbest = t;
+            best = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (hipGetLastError() != hipSuccess) return -2;
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune[k] = best;
+    return best;
+}
+
 std::string tune_report() {
     std::lock_guard<std::mutex> g(g_tune_mu);
     std::string out = "[";
@@ -1637,7 +1819,8 @@ std::string tune_report() {
         std::snprintf(buf, sizeof(buf),
                       "%s{\"dir\":\"%c\",\"fp\":%d,\"stencil\":%d,\"ii\":%d,\"jj\":%d,\"nb\":%d,"
                       "\"shape\":%d,\"kernel\":\"",
-                      out.size() > 1 ? "," : "", k[0] == 0 ? 'x' : 'y', 8 * k[1], k[2] ? 9 : 5,
+                      out.size() > 1 ? "," : "", k[0] == 0 ? 'x' : (k[0] == 1 ? 'y' : 't'),
+                      8 * k[1], k[2] ? 9 : 5,
                       k[3], k[4], k[5], kv.second);
         out += buf;
         out += shape_name(k, kv.second) + "\"}";
@@ -1748,6 +1931,12 @@ void add_correction(double *x, const float *e, int ii, int jj, hipStream_t s, in
                                     int, int, hipStream_t, int);                                  \
     template int tune_lines<T>(int, const T *, int, const T *, T *, const T *, int, int,          \
                                hipStream_t, int);                                                 \
+    template void transpose<T>(const T *, T *, int, int, hipStream_t, int);                       \
+    template void transpose_op<T>(const T *, T *, int, int, int, hipStream_t, int);               \
+    template bool relax_lines_y_auto<T>(const T *, const T *, int, const T *, T *, const T *, T *, \
+                                        T *, int, int, int, hipStream_t, int, bool);              \
+    template int tune_lines_ytr<T>(const T *, const T *, int, const T *, T *, const T *, T *, T *, \
+                                   int, int, hipStream_t, int, int);                              \
     template void restrict_<T>(const T *, const T *, T *, int, int, int, int, hipStream_t, int);  \
     template void interp_add<T>(T *, const T *, const T *, const T *, int, const T *, int, int,   \
                                 int, int, hipStream_t, int);                                      \

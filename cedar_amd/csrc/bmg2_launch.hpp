@@ -90,6 +90,31 @@ template <typename T>
 int tune_lines(int dir, const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
                hipStream_t s, int nb = 1);
 int set_autotune(int on);
+
+// dst (cols x rows) = transpose of src (rows x cols), row-major, per problem of a batch
+template <typename T>
+void transpose(const T *src, T *dst, int rows, int cols, hipStream_t s, int nb = 1);
+// soT = the operator of the transposed grid: planes c, w, s, sw, nw = transposes of c, s, w, sw,
+// nw of so (ii x jj per plane of so, jj x ii per plane of soT)
+template <typename T>
+void transpose_op(const T *so, T *soT, int nstncl, int ii, int jj, hipStream_t s, int nb = 1);
+// One y-line sweep (relax_lines_y), run as an x-line sweep of the transposed grid (q and qf
+// transposed into qT, qfT and q transposed back) when tune_lines_ytr found that faster for these
+// extents; bitwise the same result either way.  soT may be null (direct y-lines).  With
+// qfT_ready the caller guarantees that qfT already holds the transpose of qf.  Returns whether
+// the transposed path ran (qfT then holds the transpose of qf).
+template <typename T>
+bool relax_lines_y_auto(const T *so, const T *soT, int nstncl, const T *qf, T *q, const T *sory,
+                        T *qT, T *qfT, int ii, int jj, int updown, hipStream_t s, int nb = 1,
+                        bool qfT_ready = false);
+// Times the direct y-lines against the transposed path (each x-line shape) once and records the
+// choice (-1: direct); `sweeps` y sweeps are timed per right-hand side (one transpose of qf), as
+// a level visit of a V(nu1, nu2) cycle makes nu1 + nu2 of them.  Overwrites q, qT, qfT;
+// synchronises s; returns the choice or -2 on a HIP error.  A no-op returning -1 with
+// autotuning off.
+template <typename T>
+int tune_lines_ytr(const T *so, const T *soT, int nstncl, const T *qf, T *q, const T *sory, T *qT,
+                   T *qfT, int ii, int jj, hipStream_t s, int nb = 1, int sweeps = 2);
 // number of candidate shapes for lines of the given direction, word size (4 or 8) and extents
 int line_shape_count(int dir, int wbytes, int nstncl, int ii, int jj);
 // records `shape` for (direction, word size, stencil, ii, jj, nb) (shape < 0: erases the entry)

@@ -292,7 +292,7 @@ int cdr_hip_relax_lines_xy2d_f32(const float *so, int ns, const float *qf, float
 int cdr_hip_autotune(int on) { return cdr::k2::set_autotune(on); }
 
 int cdr_hip_line_shapes2d(int dir, int fp_bits, int nstncl, int ii, int jj) {
-    if ((dir != 0 && dir != 1) || (fp_bits != 32 && fp_bits != 64) || !sten_ok(nstncl) ||
+    if (dir < 0 || dir > 2 || (fp_bits != 32 && fp_bits != 64) || !sten_ok(nstncl) ||
         !ext_ok(ii, jj))
         return -CDR_ERR_ARG;
     return cdr::k2::line_shape_count(dir, fp_bits / 8, nstncl, ii, jj);
@@ -300,7 +300,7 @@ int cdr_hip_line_shapes2d(int dir, int fp_bits, int nstncl, int ii, int jj) {
 
 int cdr_hip_set_line_shape2d(int dir, int fp_bits, int nstncl, int ii, int jj, int nb,
                              int shape) {
-    if ((dir != 0 && dir != 1) || (fp_bits != 32 && fp_bits != 64) || !sten_ok(nstncl) ||
+    if (dir < 0 || dir > 2 || (fp_bits != 32 && fp_bits != 64) || !sten_ok(nstncl) ||
         !ext_ok(ii, jj) || !nb_ok(nb))
         return CDR_ERR_ARG;
     if (shape >= cdr::k2::line_shape_count(dir, fp_bits / 8, nstncl, ii, jj)) return CDR_ERR_ARG;

@@ -32,6 +32,9 @@ template <typename T>
 struct level_data {
     extents ext;
     dbuf<T> so_buf, ci_buf, x_buf, b_buf, res_buf, sor_buf[2];
+    // line-xy smoothing: the operator of the transposed grid and transposed copies of x and b,
+    // for y-line sweeps run as x-line sweeps of the transposed grid (empty unless line-xy)
+    dbuf<T> soT_buf, xT_buf, bT_buf;
     stencil_op<T> A;
     prolong_op<T> P;  // stored on this (coarse) level; unused on level 0
     grid_func<T> x, b, res;
@@ -365,13 +368,31 @@ private:
         k2::fill<T>(q.data(), T(0), np, stream_);
         k2::fill<T>(f.data(), T(0), np, stream_);
         for (std::size_t l = 0; l + 1 < L.size(); ++l) {
-            const level_data<T> &D = L[l];
+            level_data<T> &D = L[l];
             for (int dir = 0; dir < 2; ++dir) {
                 if ((dir == 0 && r == CDR_RELAX_LINE_Y) || (dir == 1 && r == CDR_RELAX_LINE_X)) continue;
                 const relax_stencil<T> &S = D.SOR[r == CDR_RELAX_LINE_XY ? dir : 0];
                 if (k2::tune_lines<T>(dir, D.A.p, D.A.nstncl, f.data(), q.data(), S.p, D.ext.ii,
                                       D.ext.jj, stream_, D.ext.nb) < 0)
                     throw error(CDR_ERR_HIP, "line-kernel tuning failed");
+            }
+            if (r != CDR_RELAX_LINE_XY) continue;
+            // y-lines through the transposed grid: kept only where it is faster
+            const std::size_t nl = D.ext.total();
+            D.soT_buf = dbuf<T>(nl * (std::size_t)D.A.nstncl, stream_);
+            D.xT_buf = dbuf<T>(nl, stream_);
+            D.bT_buf = dbuf<T>(nl, stream_);
+            k2::transpose_op<T>(D.A.p, D.soT_buf.data(), D.A.nstncl, D.ext.ii, D.ext.jj, stream_,
+                                D.ext.nb);
+            const int c = k2::tune_lines_ytr<T>(D.A.p, D.soT_buf.data(), D.A.nstncl, f.data(),
+                                                q.data(), D.SOR[1].p, D.xT_buf.data(),
+                                                D.bT_buf.data(), D.ext.ii, D.ext.jj, stream_,
+                                                D.ext.nb, set_.nrelax_pre + set_.nrelax_post);
+            if (c == -2) throw error(CDR_ERR_HIP, "line-kernel tuning failed");
+            if (c < 0) {  // direct y-lines: the transposed workspace is not needed
+                D.soT_buf = dbuf<T>();
+                D.xT_buf = dbuf<T>();
+                D.bT_buf = dbuf<T>();
             }
         }
     }
@@ -408,8 +429,11 @@ private:
         check_hip(hipGetLastError(), "convert");
     }
 
+    // bT_ok: the level's transposed right-hand side (L.bT_buf) already holds b; set when a y sweep
+    // through the transposed grid has transposed it (b does not change within one visit of a
+    // level, so the DOWN and UP smoothing of a visit share it)
     template <typename T>
-    void smooth(level_data<T> &L, grid_func<T> &x, const grid_func<T> &b, Dir d) {
+    void smooth(level_data<T> &L, grid_func<T> &x, const grid_func<T> &b, Dir d, bool &bT_ok) {
         const int nrep = d == Dir::DOWN ? set_.nrelax_pre : set_.nrelax_post;
         for (int r = 0; r < nrep; ++r) {
             switch (set_.relaxation) {
@@ -418,10 +442,21 @@ private:
             case CDR_RELAX_LINE_Y: km_.run<kernels::line_relax<T, relax_dir::y>>(L.A, x, b, L.SOR[0], L.res, d); break;
             default:
                 // x then y going down, y then x going up (include/cedar/multilevel.h:175-178,
-                // 204-207)
-                check(api2::relax_lines_xy<T>(L.A.p, L.A.nstncl, b.p, x.p, L.SOR[0].p, L.SOR[1].p,
-                                              nullptr, L.ext.ii, L.ext.jj, (int)d, cur_, L.ext.nb),
-                      "relax_lines_xy");
+                // 204-207); the y sweep runs through the transposed grid where that was faster
+                for (int half = 0; half < 2; ++half) {
+                    const bool xs = (half == 0) == (d == Dir::DOWN);
+                    if (xs) {
+                        check(api2::relax_lines<T>(0, L.A.p, L.A.nstncl, b.p, x.p, L.SOR[0].p,
+                                                   L.ext.ii, L.ext.jj, (int)d, cur_, L.ext.nb),
+                              "relax_lines_x");
+                    } else {
+                        bT_ok = k2::relax_lines_y_auto<T>(
+                            L.A.p, L.soT_buf.data(), L.A.nstncl, b.p, x.p, L.SOR[1].p,
+                            L.xT_buf.data(), L.bT_buf.data(), L.ext.ii, L.ext.jj, (int)d, cur_,
+                            L.ext.nb, bT_ok);
+                        check_hip(hipGetLastError(), "relax_lines_y");
+                    }
+                }
             }
         }
     }
@@ -461,7 +496,8 @@ private:
     void ncycle(std::vector<level_data<T>> &L, int lvl, grid_func<T> &x, const grid_func<T> &b) {
         auto &F = L[lvl];
         auto &C = L[lvl + 1];
-        smooth<T>(F, x, b, Dir::DOWN);
+        bool bT_ok = false;  // b is fixed for this visit of the level
+        smooth<T>(F, x, b, Dir::DOWN, bT_ok);
         km_.run<kernels::residual<T>>(F.A, x, b, F.res);
         km_.run<kernels::restriction<T>>(C.P, F.res, C.b);
         k2::fill<T>(C.x.p, T(0), C.ext.total(), cur_);
@@ -470,7 +506,7 @@ private:
         else
             ncycle<T>(L, lvl + 1, C.x, C.b);
         km_.run<kernels::interp_add<T>>(C.P, C.x, F.res, x);
-        smooth<T>(F, x, b, Dir::UP);
+        smooth<T>(F, x, b, Dir::UP, bT_ok);
     }
 };
 

@@ -58,6 +58,9 @@ int cdr_hip_autotune_report(char *buf, int len);
 /* Number of candidate shapes for lines of direction dir (0: x, 1: y) at fp_bits (32 or 64)
  * precision, stencil nstncl and extents ii x jj; cdr_hip_set_line_shape2d pins one of them for
  * those extents and batch size nb (shape -1 removes the entry; returns a cdr_status).
+ * dir 2 is the line-xy smoother's y sweep run as an x-line sweep of the transposed grid (used by
+ * the solvers where tuning found it faster; the same result bit for bit): its shapes are x-line
+ * shapes for the transposed extents, and a pinned shape makes the solvers take that path.
  * cdr_hip_line_shapes2d returns -CDR_ERR_ARG for invalid arguments. */
 int cdr_hip_line_shapes2d(int dir, int fp_bits, int nstncl, int ii, int jj);
 int cdr_hip_set_line_shape2d(int dir, int fp_bits, int nstncl, int ii, int jj, int nb, int shape);

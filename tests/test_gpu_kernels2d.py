@@ -296,7 +296,7 @@ def test_every_line_shape_gives_identical_results(C, n, m, direction, dtype):
 
 def test_autotune_records_shapes(C):
     """Solver setup with line relaxation records a shape for every smoothing level and direction
-    that has more than one candidate (lines of 1023 points: 2 x-shapes, 3 y-shapes)."""
+    that has more than one candidate (lines of 1023 points have several of each)."""
     import cedar_amd
     from cedar_amd import gallery as G
     so, b, _ = G.heterogeneous_9pt(1023, seed=4, block=16, sigma=1.0, contrast=1e2)
@@ -306,8 +306,45 @@ def test_autotune_records_shapes(C):
                              nrelax_post=1, maxiter=30, tol=1e-10)
         rep = cedar_amd.cdr2.autotune_report()
         fine = [r for r in rep if r["ii"] == 1025 and r["jj"] == 1025 and r["fp"] == 32]
-        assert {r["dir"] for r in fine} == {"x", "y"}
+        # x, y, and the choice between direct and transposed y sweeps ("t")
+        assert {r["dir"] for r in fine} == {"x", "y", "t"}
         x, h = s.solve(torch.as_tensor(b).cuda())
         assert h[-1] < 1e-10
     finally:
         cedar_amd.cdr2.autotune(prev)
+
+
+@pytest.mark.parametrize("precision", ["f64", "mixed"])
+@pytest.mark.parametrize("ns", [3, 5])
+def test_transposed_y_sweeps_are_bitwise_direct(C, precision, ns):
+    """Line-xy solves with the y sweeps run as x sweeps of the transposed grid (pinned at every
+    level) give bitwise the iterates and histories of the direct y-line kernels."""
+    from cedar_amd import gallery as G
+    n = 257
+    if ns == 5:
+        so, b, _ = G.heterogeneous_9pt(n, seed=8, block=16, sigma=1.5, contrast=1e3)
+    else:
+        so, b = G.poisson_uniform_rhs(n, seed=8)
+    lib = _abi.load()
+    bits = 64 if precision == "f64" else 32
+    sizes, m = [], n
+    while m >= 3:
+        sizes.append(m)
+        m = int((m - 1) / 2.0 + 1)
+    prev = lib.cdr_hip_autotune(0)
+    kw = dict(relaxation="line-xy", nrelax_pre=1, nrelax_post=1, maxiter=40, tol=1e-10,
+              precision=precision)
+    try:
+        x0, h0 = C.Solver(so, **kw).solve(dev(b))
+        for lvl, m in enumerate(sizes):
+            stencil = ns if lvl == 0 else 5
+            if lib.cdr_hip_line_shapes2d(2, bits, stencil, m + 2, m + 2) > 0:
+                assert lib.cdr_hip_set_line_shape2d(2, bits, stencil, m + 2, m + 2, 1, 0) == 0
+        x1, h1 = C.Solver(so, **kw).solve(dev(b))
+    finally:
+        for lvl, m in enumerate(sizes):
+            lib.cdr_hip_set_line_shape2d(2, bits, ns if lvl == 0 else 5, m + 2, m + 2, 1, -1)
+        lib.cdr_hip_autotune(prev)
+    assert np.array_equal(h0, h1)
+    assert torch.equal(x0, x1)
+    assert h0[-1] < 1e-10
