@@ -923,11 +923,16 @@ __global__ __launch_bounds__(256) void interp_add_k(T *__restrict__ q, const T *
                                                     const T *__restrict__ so,
                                                     const T *__restrict__ ci, int iic, int jjc,
                                                     int iif, int jjf, int ns) {
-    const int i = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
-    const int j = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
+    // one thread per 2 x 2 block of fine points (2m-1 .. 2m, 2n-1 .. 2n): the coarse point
+    // (m, n), the x-edge point (2m, 2n-1), the y-edge point (2m-1, 2n) and the cell point
+    // (2m, 2n), sharing the four coarse values around the cell
+    const int m = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
+    const int n = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
     const int imax = iif - 2 + ((iif - 2) % 2 == 0 ? 1 : 0);
     const int jmax = jjf - 2 + ((jjf - 2) % 2 == 0 ? 1 : 0);
-    if (i > imax || j > jmax) return;
+    const int i0 = 2 * m - 1, j0 = 2 * n - 1;
+    if (i0 > imax || j0 > jmax) return;
+    const bool xe = i0 + 1 <= imax, ye = j0 + 1 <= jmax;  // the even fine points exist
     const std::size_t Pc = (std::size_t)iic * jjc, Pf = (std::size_t)iif * jjf, bz = blockIdx.z;
     q += bz * Pf;
     qc += bz * Pc;
@@ -936,29 +941,33 @@ __global__ __launch_bounds__(256) void interp_add_k(T *__restrict__ q, const T *
     ci += bz * 8 * Pc;
     auto CI = [&](int a, int b, int k) { return ci[Pc * k + pidx(a, b, iic)]; };
     auto QC = [&](int a, int b) { return qc[pidx(a, b, iic)]; };
-    const std::size_t p = pidx(i, j, iif);
-    const bool ci_col = (i & 1), ci_row = (j & 1);
-    if (ci_col && ci_row) {
-        q[p] = q[p] + QC((i + 1) / 2, (j + 1) / 2);
-        return;
+    auto R = [&](int i, int j, std::size_t p) {
+        return (i > iif - 2 || j > jjf - 2) ? res[p] : res[p] / so[p];
+    };
+    const T q00 = QC(m, n);
+    // coarse point
+    {
+        const std::size_t p = pidx(i0, j0, iif);
+        q[p] = q[p] + q00;
     }
-    const bool ghost = i > iif - 2 || j > jjf - 2;
-    const T r = ghost ? res[p] : res[p] / so[p];
-    T a;
-    if (!ci_col && ci_row) {
-        const int icr = i / 2 + 1, jc = (j + 1) / 2;
-        a = CI(icr, jc, LR) * QC(icr, jc) + CI(icr, jc, LL) * QC(icr - 1, jc);
-    } else if (ci_col && !ci_row) {
-        const int ic = (i + 1) / 2, jca = j / 2 + 1;
-        a = CI(ic, jca, LA) * QC(ic, jca) + CI(ic, jca, LB) * QC(ic, jca - 1);
-    } else {
-        const int icr = i / 2 + 1, jca = j / 2 + 1;
-        a = CI(icr, jca, LSW) * QC(icr - 1, jca - 1);
-        a = a + CI(icr, jca, LNW) * QC(icr - 1, jca);
-        a = a + CI(icr, jca, LNE) * QC(icr, jca);
-        a = a + CI(icr, jca, LSE) * QC(icr, jca - 1);
+    if (xe) {  // (2m, 2n-1): between coarse (m, n) and (m+1, n)
+        const std::size_t p = pidx(i0 + 1, j0, iif);
+        const T a = CI(m + 1, n, LR) * QC(m + 1, n) + CI(m + 1, n, LL) * q00;
+        q[p] = q[p] + a + R(i0 + 1, j0, p);
     }
-    q[p] = q[p] + a + r;
+    if (ye) {  // (2m-1, 2n): between coarse (m, n) and (m, n+1)
+        const std::size_t p = pidx(i0, j0 + 1, iif);
+        const T a = CI(m, n + 1, LA) * QC(m, n + 1) + CI(m, n + 1, LB) * q00;
+        q[p] = q[p] + a + R(i0, j0 + 1, p);
+    }
+    if (xe && ye) {  // (2m, 2n): the cell of coarse (m..m+1, n..n+1)
+        const std::size_t p = pidx(i0 + 1, j0 + 1, iif);
+        T a = CI(m + 1, n + 1, LSW) * q00;
+        a = a + CI(m + 1, n + 1, LNW) * QC(m, n + 1);
+        a = a + CI(m + 1, n + 1, LNE) * QC(m + 1, n + 1);
+        a = a + CI(m + 1, n + 1, LSE) * QC(m + 1, n);
+        q[p] = q[p] + a + R(i0 + 1, j0 + 1, p);
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1850,7 +1859,8 @@ void restrict_(const T *ci, const T *q, T *qc, int iif, int jjf, int iic, int jj
 template <typename T>
 void interp_add(T *q, const T *qc, const T *res, const T *so, int nstncl, const T *ci, int iic,
                 int jjc, int iif, int jjf, hipStream_t s, int nb) {
-    interp_add_k<T><<<gridb(iif - 1, jjf - 1, nb), 256, 0, s>>>(q, qc, res, so, ci, iic, jjc, iif,
+    // one thread per 2 x 2 fine block (the range includes the ghost point of even sizes)
+    interp_add_k<T><<<gridb(iif / 2, jjf / 2, nb), 256, 0, s>>>(q, qc, res, so, ci, iic, jjc, iif,
                                                                 jjf, nstncl);
 }
 
