@@ -28,7 +28,7 @@ class Settings(ctypes.Structure):
     """cdr_solver_settings"""
     _fields_ = [("relaxation", _i), ("nrelax_pre", _i), ("nrelax_post", _i), ("num_levels", _i),
                 ("maxiter", _i), ("tol", _d), ("min_coarse", _i), ("precision", _i), ("graph", _i),
-                ("cycle", _i)]
+                ("cycle", _i), ("fused_coarse", _i)]
 
 
 class Settings3(ctypes.Structure):

@@ -231,7 +231,8 @@ class Solver:
     """
 
     def __init__(self, so, relaxation="point", nrelax_pre=2, nrelax_post=1, num_levels=-1,
-                 maxiter=10, tol=1e-8, min_coarse=3, precision="f64", graph=True, cycle="v"):
+                 maxiter=10, tol=1e-8, min_coarse=3, precision="f64", graph=True, cycle="v",
+                 fused_coarse=True):
         so = _t(so) if not isinstance(so, np.ndarray) else torch.as_tensor(so).cuda()
         if so.dtype != torch.float64:
             so = so.to(torch.float64)
@@ -243,7 +244,7 @@ class Solver:
         ns, jj, ii = self.so.shape[-3:]
         s = _abi.Settings(_abi.RELAX[relaxation], nrelax_pre, nrelax_post, num_levels, maxiter,
                           tol, min_coarse, _abi.PRECISION[precision], int(bool(graph)),
-                          _abi.CYCLE[cycle])
+                          _abi.CYCLE[cycle], int(bool(fused_coarse)))
         h = ctypes.c_void_p()
         _abi.call("cdr_hip_solver2d_create_batched", _p(self.so), ns, ii - 2, jj - 2, nb,
                   ctypes.byref(s), _stream(), ctypes.byref(h))

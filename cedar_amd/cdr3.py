@@ -111,10 +111,10 @@ def solve_cg(q, qf, abd):
 
 
 def plane_settings(relaxation="line-xy", nrelax_pre=2, nrelax_post=1, maxiter=1, tol=1e-8,
-                   min_coarse=3, num_levels=-1):
+                   min_coarse=3, num_levels=-1, fused_coarse=True):
     """plane-config of the nested 2D plane solves (default: src/kernel_params.cc:83-89)."""
     return _abi.Settings(_abi.RELAX[relaxation], nrelax_pre, nrelax_post, num_levels, maxiter, tol,
-                         min_coarse, 0, 0, 0)
+                         min_coarse, 0, 0, 0, int(bool(fused_coarse)))
 
 
 class Solver3:

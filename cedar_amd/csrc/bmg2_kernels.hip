@@ -13,6 +13,7 @@
 //
 // All kernels are stencils on HBM-resident arrays (~0.2-0.3 flop/byte): no MFMA, coalesced
 // accesses along i, 64-wide wavefronts, deterministic fixed-order reductions.
+#include "../../include/cedar_hip.h"
 #include "band_chol.hpp"
 #include "bmg2_launch.hpp"
 
@@ -884,17 +885,10 @@ __global__ __launch_bounds__(NC *W * 64) void relax_ylines_seg_k(const T *__rest
 // restriction qc = P^T q (src/2d/ftn/BMG2_SymStd_restrict.f90:76-92); coarse (ic, jc) sits on
 // fine (2ic-1, 2jc-1)
 // ------------------------------------------------------------------------------------------
+// coarse point (ic, jc) of qc = P^T q (the reference's operation order); ci, q, qc of one problem
 template <typename T>
-__global__ __launch_bounds__(256) void restrict_k(const T *__restrict__ ci, const T *__restrict__ q,
-                                                  T *__restrict__ qc, int iif, int jjf, int iic,
-                                                  int jjc) {
-    const int ic = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
-    const int jc = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
-    if (ic > iic - 2 || jc > jjc - 2) return;
-    const std::size_t Pc = (std::size_t)iic * jjc, bz = blockIdx.z;
-    ci += bz * 8 * Pc;
-    q += bz * iif * (std::size_t)jjf;
-    qc += bz * Pc;
+__device__ inline T restrict_point(const T *__restrict__ ci, const T *__restrict__ q, int iif,
+                                   int iic, std::size_t Pc, int ic, int jc) {
     const int i = 2 * ic - 1, j = 2 * jc - 1;
     auto CI = [&](int a, int b, int k) { return ci[Pc * k + pidx(a, b, iic)]; };
     auto Q = [&](int a, int b) { return q[pidx(a, b, iif)]; };
@@ -907,46 +901,51 @@ __global__ __launch_bounds__(256) void restrict_k(const T *__restrict__ ci, cons
     v = v + CI(ic, jc + 1, LSE) * Q(i - 1, j + 1);
     v = v + CI(ic, jc + 1, LB) * Q(i, j + 1);
     v = v + CI(ic + 1, jc + 1, LSW) * Q(i + 1, j + 1);
-    qc[pidx(ic, jc, iic)] = v;
+    return v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void restrict_k(const T *__restrict__ ci, const T *__restrict__ q,
+                                                  T *__restrict__ qc, int iif, int jjf, int iic,
+                                                  int jjc) {
+    const int ic = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
+    const int jc = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
+    if (ic > iic - 2 || jc > jjc - 2) return;
+    const std::size_t Pc = (std::size_t)iic * jjc, bz = blockIdx.z;
+    ci += bz * 8 * Pc;
+    q += bz * iif * (std::size_t)jjf;
+    qc += bz * Pc;
+    qc[pidx(ic, jc, iic)] = restrict_point<T>(ci, q, iif, iic, Pc, ic, jc);
 }
 
 // ------------------------------------------------------------------------------------------
 // interp_add q += P qc + res / c at F-points (src/2d/ftn/BMG2_SymStd_interp_add.f90:101-137);
-// one thread per fine point.  res is only read (the reference divides it in place; nothing
+// one thread per 2 x 2 block of fine points.  res is only read (the reference divides it in place; nothing
 // reads it afterwards).  For an even fine extent the reference's loops also reach the last
 // fine ghost column / row (coarse index IICF1 = IIC); those points are updated the same way,
 // with the raw res value there since the reference divides res only in the interior.
 // ------------------------------------------------------------------------------------------
+// One 2 x 2 block of fine points (2m-1 .. 2m, 2n-1 .. 2n) of interp_add: the coarse point
+// (m, n), the x-edge point (2m, 2n-1), the y-edge point (2m-1, 2n) and the cell point (2m, 2n),
+// sharing the four coarse values around the cell.  Arrays of one problem.
 template <typename T>
-__global__ __launch_bounds__(256) void interp_add_k(T *__restrict__ q, const T *__restrict__ qc,
-                                                    const T *__restrict__ res,
-                                                    const T *__restrict__ so,
-                                                    const T *__restrict__ ci, int iic, int jjc,
-                                                    int iif, int jjf, int ns) {
-    // one thread per 2 x 2 block of fine points (2m-1 .. 2m, 2n-1 .. 2n): the coarse point
-    // (m, n), the x-edge point (2m, 2n-1), the y-edge point (2m-1, 2n) and the cell point
-    // (2m, 2n), sharing the four coarse values around the cell
-    const int m = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
-    const int n = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
+__device__ inline void interp_block(T *__restrict__ q, const T *__restrict__ qc,
+                                    const T *__restrict__ res, const T *__restrict__ so,
+                                    const T *__restrict__ ci, int iic, int jjc, int iif, int jjf,
+                                    int m, int n) {
     const int imax = iif - 2 + ((iif - 2) % 2 == 0 ? 1 : 0);
     const int jmax = jjf - 2 + ((jjf - 2) % 2 == 0 ? 1 : 0);
     const int i0 = 2 * m - 1, j0 = 2 * n - 1;
     if (i0 > imax || j0 > jmax) return;
     const bool xe = i0 + 1 <= imax, ye = j0 + 1 <= jmax;  // the even fine points exist
-    const std::size_t Pc = (std::size_t)iic * jjc, Pf = (std::size_t)iif * jjf, bz = blockIdx.z;
-    q += bz * Pf;
-    qc += bz * Pc;
-    res += bz * Pf;
-    so += bz * ns * Pf;
-    ci += bz * 8 * Pc;
+    const std::size_t Pc = (std::size_t)iic * jjc;
     auto CI = [&](int a, int b, int k) { return ci[Pc * k + pidx(a, b, iic)]; };
     auto QC = [&](int a, int b) { return qc[pidx(a, b, iic)]; };
     auto R = [&](int i, int j, std::size_t p) {
         return (i > iif - 2 || j > jjf - 2) ? res[p] : res[p] / so[p];
     };
     const T q00 = QC(m, n);
-    // coarse point
-    {
+    {  // coarse point
         const std::size_t p = pidx(i0, j0, iif);
         q[p] = q[p] + q00;
     }
@@ -968,6 +967,19 @@ __global__ __launch_bounds__(256) void interp_add_k(T *__restrict__ q, const T *
         a = a + CI(m + 1, n + 1, LSE) * QC(m + 1, n);
         q[p] = q[p] + a + R(i0 + 1, j0 + 1, p);
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void interp_add_k(T *__restrict__ q, const T *__restrict__ qc,
+                                                    const T *__restrict__ res,
+                                                    const T *__restrict__ so,
+                                                    const T *__restrict__ ci, int iic, int jjc,
+                                                    int iif, int jjf, int ns) {
+    const int m = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
+    const int n = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
+    const std::size_t Pc = (std::size_t)iic * jjc, Pf = (std::size_t)iif * jjf, bz = blockIdx.z;
+    interp_block<T>(q + bz * Pf, qc + bz * Pc, res + bz * Pf, so + bz * ns * Pf, ci + bz * 8 * Pc,
+                    iic, jjc, iif, jjf, m, n);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1364,9 +1376,9 @@ void setup_lines_y(const double *so, int nstncl, double *sor, int ii, int jj, in
 }
 
 // dst (cols x rows) = transpose of src (rows x cols), row-major, for each problem of a batch;
-// 64 x 64 tiles through LDS
+// 64 x 64 tiles through LDS, 512 threads (8 rows per pass)
 template <typename T>
-__global__ __launch_bounds__(256) void transpose_k(const T *__restrict__ src, T *__restrict__ dst,
+__global__ __launch_bounds__(512) void transpose_k(const T *__restrict__ src, T *__restrict__ dst,
                                                   int rows, int cols) {
     __shared__ T tile[64][65];
     const std::size_t off = (std::size_t)blockIdx.z * rows * cols;
@@ -1374,12 +1386,12 @@ __global__ __launch_bounds__(256) void transpose_k(const T *__restrict__ src, T 
     dst += off;
     const int c0 = blockIdx.x * 64, r0 = blockIdx.y * 64;
     const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
-    for (int k = ty; k < 64; k += 4) {
+    for (int k = ty; k < 64; k += 8) {
         const int r = r0 + k, c = c0 + tx;
         if (r < rows && c < cols) tile[k][tx] = src[(std::size_t)r * cols + c];
     }
     __syncthreads();
-    for (int k = ty; k < 64; k += 4) {
+    for (int k = ty; k < 64; k += 8) {
         const int c = c0 + k, r = r0 + tx;
         if (c < cols && r < rows) dst[(std::size_t)c * rows + r] = tile[tx][k];
     }
@@ -1388,7 +1400,184 @@ __global__ __launch_bounds__(256) void transpose_k(const T *__restrict__ src, T 
 template <typename T>
 void transpose(const T *src, T *dst, int rows, int cols, hipStream_t s, int nb) {
     const dim3 g((unsigned)((cols + 63) / 64), (unsigned)((rows + 63) / 64), (unsigned)nb);
-    transpose_k<T><<<g, 256, 0, s>>>(src, dst, rows, cols);
+    transpose_k<T><<<g, 512, 0, s>>>(src, dst, rows, cols);
+}
+
+// ---- fused coarse-level V-cycle ---------------------------------------------------------------
+// One workgroup per problem runs the whole V-cycle below a level of at most 64*CH points per side
+// (smoothing, residual, restriction, coarsest band solve, interpolation) with workgroup barriers
+// between the steps instead of one launch per step.  Every point is computed by the same device
+// functions and in the same order as in the grid kernels, so the result is bitwise that of the
+// unfused cycle.  Levels of a fused cycle are always 9-point (Galerkin operators).
+
+// one x-line (row j) or y-line (column i) of at most 64*CH points, by one wavefront, in place
+template <typename T, int DIR, int CH>
+__device__ inline void wave_line(const T *__restrict__ so, const T *__restrict__ qf, T *q,
+                                 const T *__restrict__ sor, int ii, int jj, int fixed) {
+    const int lane = threadIdx.x % WAVE, rl = WAVE - 1 - lane;
+    const std::size_t P = (std::size_t)ii * jj;
+    const int n = DIR == 0 ? ii - 2 : jj - 2;
+    const std::size_t f0 = DIR == 0 ? (std::size_t)fixed * ii : (std::size_t)fixed * jj;
+    const T *D = sor + f0 + 1;
+    const T *E = sor + P + f0 + 2;
+    auto at = [&](int k) { return DIR == 0 ? pidx(k + 1, fixed, ii) : pidx(fixed, k + 1, ii); };
+    T a[CH], b[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+        const int k = u * WAVE + lane, kc = min(k, n - 1);
+        line_raw<T> r;
+        line_load<T, true, DIR>(r, so, qf, q, P, ii, at(kc));
+        b[u] = line_rhs<T, true>(r);
+        a[u] = (k > 0) ? -E[max(kc - 1, 0)] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) scan_affine_dpp(a[u], b[u]);
+    T carry = T(0);
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+        const T y = a[u] * carry + b[u];
+        carry = last_lane(y);
+        b[u] = y / D[min(u * WAVE + lane, n - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+        const int k = u * WAVE + rl;
+        const T c = __shfl(b[u], rl, WAVE);
+        const T e = E[min(k, max(n - 2, 0))];
+        a[u] = (k < n - 1) ? -e : T(0);
+        a[u] = (k < n) ? a[u] : T(1);
+        b[u] = (k < n) ? c : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) scan_affine_dpp(a[u], b[u]);
+    carry = T(0);
+#pragma unroll
+    for (int u = CH - 1; u >= 0; --u) {
+        const T x = a[u] * carry + b[u];
+        carry = last_lane(x);
+        b[u] = x;
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+        const int k = u * WAVE + rl;
+        if (k < n) q[at(k)] = b[u];
+    }
+}
+
+template <typename T, int CH>
+__global__ __launch_bounds__(1024) void coarse_cycle_k(const coarse_level<T> *__restrict__ lv,
+                                                       int nlev, int relax, int npre, int npost,
+                                                       const double *__restrict__ abd,
+                                                       double *__restrict__ bbd) {
+    constexpr int NT = 1024, NWAVE = NT / WAVE;
+    const int tid = threadIdx.x, wave = tid / WAVE;
+    const std::size_t bz = blockIdx.x;
+    auto level = [&](int l) {  // the level's arrays of problem bz
+        coarse_level<T> d = lv[l];
+        const std::size_t P = (std::size_t)d.ii * d.jj;
+        d.so += bz * 5 * P;
+        d.x += bz * P;
+        d.b += bz * P;
+        d.res += bz * P;
+        d.sor0 += bz * 2 * P;
+        d.sor1 += bz * 2 * P;
+        if (d.ci) d.ci += bz * 8 * P;
+        return d;
+    };
+    auto lines = [&](const coarse_level<T> &d, const T *sor, int dir, int updown) {
+        const int nfix = dir == 0 ? d.jj - 2 : d.ii - 2;
+        for (int t = 0; t < 2; ++t) {
+            const int first = (updown == 0) ? (t == 0 ? 2 : 1) : (t == 0 ? 1 : 2);
+            if (first <= nfix) {
+                const int nlines = (nfix - first) / 2 + 1;
+                for (int line = wave; line < nlines; line += NWAVE) {
+                    if (dir == 0)
+                        wave_line<T, 0, CH>(d.so, d.b, d.x, sor, d.ii, d.jj, first + 2 * line);
+                    else
+                        wave_line<T, 1, CH>(d.so, d.b, d.x, sor, d.ii, d.jj, first + 2 * line);
+                }
+            }
+            __syncthreads();
+        }
+    };
+    auto gauss_seidel = [&](const coarse_level<T> &d, int updown) {
+        static constexpr int order[4][2] = {{1, 1}, {0, 1}, {1, 0}, {0, 0}};
+        const std::size_t P = (std::size_t)d.ii * d.jj;
+        const int nx = d.ii - 2, npts = nx * (d.jj - 2);
+        for (int t = 0; t < 4; ++t) {
+            const int c = updown == 0 ? t : 3 - t;
+            for (int k = tid; k < npts; k += NT) {
+                const int i = k % nx + 1, j = k / nx + 1;
+                if ((i & 1) != order[c][0] || (j & 1) != order[c][1]) continue;
+                const std::size_t p = pidx(i, j, d.ii);
+                const T v = apply_offdiag<T, true>(d.so, P, d.x, p, d.ii, d.b[p]);
+                d.x[p] = v * d.sor0[P + p];
+            }
+            __syncthreads();
+        }
+    };
+    auto smooth = [&](const coarse_level<T> &d, int updown) {
+        const int nrep = updown == 0 ? npre : npost;
+        for (int r = 0; r < nrep; ++r) {
+            if (relax == CDR_RELAX_POINT) {
+                gauss_seidel(d, updown);
+            } else if (relax == CDR_RELAX_LINE_X) {
+                lines(d, d.sor0, 0, updown);
+            } else if (relax == CDR_RELAX_LINE_Y) {
+                lines(d, d.sor0, 1, updown);
+            } else if (updown == 0) {
+                lines(d, d.sor0, 0, updown);
+                lines(d, d.sor1, 1, updown);
+            } else {
+                lines(d, d.sor1, 1, updown);
+                lines(d, d.sor0, 0, updown);
+            }
+        }
+    };
+    // down: smooth, residual, restrict to the next level's b, zero its x
+    for (int l = 0; l + 1 < nlev; ++l) {
+        const coarse_level<T> F = level(l), C = level(l + 1);
+        smooth(F, 0);
+        const std::size_t P = (std::size_t)F.ii * F.jj;
+        const int nx = F.ii - 2, npts = nx * (F.jj - 2);
+        for (int k = tid; k < npts; k += NT) {
+            const std::size_t p = pidx(k % nx + 1, k / nx + 1, F.ii);
+            T r = apply_offdiag<T, true>(F.so, P, F.x, p, F.ii, F.b[p]);
+            r = r - F.so[p] * F.x[p];
+            F.res[p] = r;
+        }
+        __syncthreads();
+        const std::size_t Pc = (std::size_t)C.ii * C.jj;
+        const int nxc = C.ii - 2, nc = nxc * (C.jj - 2);
+        for (int k = tid; k < nc; k += NT) {
+            const int ic = k % nxc + 1, jc = k / nxc + 1;
+            C.b[pidx(ic, jc, C.ii)] = restrict_point<T>(C.ci, F.res, F.ii, C.ii, Pc, ic, jc);
+        }
+        for (std::size_t k = tid; k < Pc; k += NT) C.x[k] = T(0);
+        __syncthreads();
+    }
+    // coarsest level: band Cholesky solve (solve_cg_k)
+    {
+        const coarse_level<T> C = level(nlev - 1);
+        if (tid == 0) {
+            const int nx = C.ii - 2, n = nx * (C.jj - 2), ld = C.ii;
+            double *xs = bbd + bz * n;
+            for (int kk = 0; kk < n; ++kk) xs[kk] = (double)C.b[pidx(kk % nx + 1, kk / nx + 1, C.ii)];
+            band_solve_upper(abd + bz * (std::size_t)ld * n, n, C.ii - 1, ld, xs);
+            for (int kk = 0; kk < n; ++kk) C.x[pidx(kk % nx + 1, kk / nx + 1, C.ii)] = (T)xs[kk];
+        }
+        __syncthreads();
+    }
+    // up: interpolate and add, smooth
+    for (int l = nlev - 2; l >= 0; --l) {
+        const coarse_level<T> F = level(l), C = level(l + 1);
+        const int mb = F.ii / 2, nblk = mb * (F.jj / 2);
+        for (int k = tid; k < nblk; k += NT)
+            interp_block<T>(F.x, C.x, F.res, F.so, C.ci, C.ii, C.jj, F.ii, F.jj, k % mb + 1,
+                            k / mb + 1);
+        __syncthreads();
+        smooth(F, 1);
+    }
 }
 
 // ---- line kernel shapes and their autotuner ---------------------------------------------------
@@ -1900,6 +2089,15 @@ void solve_cg(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj,
     solve_cg_k<T><<<nb, 64, 0, s>>>(q, qf, abd, bbd, ii, jj);
 }
 
+template <typename T>
+void coarse_cycle(const coarse_level<T> *lv, int nlev, int max_side, int relax, int npre,
+                  int npost, const double *abd, double *bbd, hipStream_t s, int nb) {
+    if (max_side <= 64)
+        coarse_cycle_k<T, 1><<<nb, 1024, 0, s>>>(lv, nlev, relax, npre, npost, abd, bbd);
+    else
+        coarse_cycle_k<T, 2><<<nb, 1024, 0, s>>>(lv, nlev, relax, npre, npost, abd, bbd);
+}
+
 template <typename Tin, typename Tout>
 void convert(const Tin *in, Tout *out, std::size_t n, hipStream_t s) {
     convert_k<Tin, Tout><<<grid1(n), 256, 0, s>>>(in, out, n);
@@ -1942,6 +2140,8 @@ void add_correction(double *x, const float *e, int ii, int jj, hipStream_t s, in
     template int tune_lines<T>(int, const T *, int, const T *, T *, const T *, int, int,          \
                                hipStream_t, int);                                                 \
     template void transpose<T>(const T *, T *, int, int, hipStream_t, int);                       \
+    template void coarse_cycle<T>(const coarse_level<T> *, int, int, int, int, int, const double *, \
+                                  double *, hipStream_t, int);                                    \
     template void transpose_op<T>(const T *, T *, int, int, int, hipStream_t, int);               \
     template bool relax_lines_y_auto<T>(const T *, const T *, int, const T *, T *, const T *, T *, \
                                         T *, int, int, int, hipStream_t, int, bool);              \

@@ -91,6 +91,27 @@ int tune_lines(int dir, const T *so, int nstncl, const T *qf, T *q, const T *sor
                hipStream_t s, int nb = 1);
 int set_autotune(int on);
 
+// One level of a fused coarse-level V-cycle: arrays of problem 0 of a batch (problem b at b times
+// each array's size), 9-point operator, relaxation data as the solver stores it (point: sor0
+// holds 1/c in its plane 1; lines: sor0 = x factors or the only direction's factors, sor1 = y
+// factors of line-xy), ci = interpolation stored on this level (null on the first level).
+template <typename T>
+struct coarse_level {
+    const T *so;
+    T *x, *b, *res;
+    const T *sor0, *sor1;
+    const T *ci;
+    int ii, jj;
+};
+// Runs one V-cycle over nlev levels (lv: device array, finest first; the last level is the
+// coarsest, solved with the band factor abd) in one launch of one workgroup per problem, for x of
+// the first level from its b (x zero on entry).  max_side: the largest nx or ny of the levels
+// (at most 128).  relax: CDR_RELAX_*; npre / npost: smoothing steps.  Bitwise equal to the
+// unfused V-cycle over the same levels.
+template <typename T>
+void coarse_cycle(const coarse_level<T> *lv, int nlev, int max_side, int relax, int npre,
+                  int npost, const double *abd, double *bbd, hipStream_t s, int nb = 1);
+
 // dst (cols x rows) = transpose of src (rows x cols), row-major, per problem of a batch
 template <typename T>
 void transpose(const T *src, T *dst, int rows, int cols, hipStream_t s, int nb = 1);

@@ -169,6 +169,7 @@ void cdr_hip_default_settings(cdr_solver_settings *s) {
     s->precision = CDR_PREC_F64;
     s->graph = 1;
     s->cycle = CDR_CYCLE_V;
+    s->fused_coarse = 1;
 }
 
 int cdr_hip_solver2d_create(const double *so_dev, int nstncl, int nx, int ny,
@@ -184,7 +185,8 @@ int cdr_hip_solver2d_create_batched(const double *so_dev, int nstncl, int nx, in
         s->relaxation < CDR_RELAX_POINT || s->relaxation > CDR_RELAX_LINE_XY ||
         s->nrelax_pre < 0 || s->nrelax_post < 0 || s->maxiter < 0 || s->min_coarse < 2 ||
         (s->precision != CDR_PREC_F64 && s->precision != CDR_PREC_MIXED) ||
-        (s->graph != 0 && s->graph != 1) || (s->cycle != CDR_CYCLE_V && s->cycle != CDR_CYCLE_F))
+        (s->graph != 0 && s->graph != 1) || (s->cycle != CDR_CYCLE_V && s->cycle != CDR_CYCLE_F) ||
+        (s->fused_coarse != 0 && s->fused_coarse != 1))
         return CDR_ERR_ARG;
     *out = nullptr;
     int ndev = 0;

@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <type_traits>
 #include <vector>
@@ -217,6 +218,9 @@ private:
     dbuf<int> info_, active_;
     grid_func<double> abd_g_;
     double *host_norm_ = nullptr;  // nb squared norms (pinned)
+    // fused coarse-level cycle: first level (-1: none), largest side, device level descriptors
+    int fuse_lvl_ = -1, fuse_side_ = 0;
+    dbuf<unsigned char> fuse64_, fuse32_;
     int *host_active_ = nullptr;   // nb flags (pinned)
     hipEvent_t ev0_{}, ev1_{};
     double setup_s_ = 0, solve_s_ = 0;
@@ -341,8 +345,10 @@ private:
         if (set_.precision == CDR_PREC_MIXED) {
             build_f32();
             tune_lines(L32_);
+            setup_fused(L32_, fuse32_);
         } else {
             tune_lines(L64_);
+            setup_fused(L64_, fuse64_);
         }
         check_hip(hipEventRecord(ev1_, stream_), "hipEventRecord");
         check_hip(hipEventSynchronize(ev1_), "hipEventSynchronize");
@@ -355,6 +361,38 @@ private:
         const extents ec = L64_.back().ext;
         abd_g_ = grid_func<double>{abd_.data(), extents{ec.ii, ec.nx() * ec.ny(), ec.nb}};
         return abd_g_;
+    }
+
+    // The fused coarse-level cycle starts at the first level below the finest whose sides are at
+    // most kFuseSide points and covers it and every coarser level (it needs at least two of
+    // them).  One workgroup per problem is latency-bound on larger levels, where one launch per
+    // step over the whole GPU is faster.
+    static constexpr int kFuseSide = 32;
+    template <typename T>
+    void setup_fused(std::vector<level_data<T>> &L, dbuf<unsigned char> &dst) {
+        fuse_lvl_ = -1;
+        if (!set_.fused_coarse) return;
+        const int nl = (int)L.size();
+        int lt = -1;
+        for (int l = 1; l + 1 < nl; ++l)
+            if (std::max(L[l].ext.nx(), L[l].ext.ny()) <= kFuseSide) {
+                lt = l;
+                break;
+            }
+        if (lt < 0) return;
+        std::vector<k2::coarse_level<T>> v;
+        for (int l = lt; l < nl; ++l) {
+            const level_data<T> &D = L[l];
+            v.push_back(k2::coarse_level<T>{D.A.p, D.x.p, D.b.p, D.res.p, D.SOR[0].p, D.SOR[1].p,
+                                            l == lt ? nullptr : D.P.p, D.ext.ii, D.ext.jj});
+        }
+        const std::size_t bytes = v.size() * sizeof(k2::coarse_level<T>);
+        dst = dbuf<unsigned char>(bytes, stream_);
+        check_hip(hipMemcpyAsync(dst.data(), v.data(), bytes, hipMemcpyHostToDevice, stream_),
+                  "copy level descriptors");
+        check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+        fuse_lvl_ = lt;
+        fuse_side_ = std::max(L[lt].ext.nx(), L[lt].ext.ny());
     }
 
     // Picks the line-kernel launch shapes of every smoothing level by timing (the shapes give
@@ -494,6 +532,18 @@ private:
 
     template <typename T>
     void ncycle(std::vector<level_data<T>> &L, int lvl, grid_func<T> &x, const grid_func<T> &b) {
+        if (lvl == fuse_lvl_ && x.p == L[lvl].x.p && b.p == L[lvl].b.p) {
+            // this level and all coarser ones in one launch (bitwise the same V-cycle)
+            const dbuf<unsigned char> &d = std::is_same<T, double>::value ? fuse64_ : fuse32_;
+            if (d.data()) {
+                k2::coarse_cycle<T>(reinterpret_cast<const k2::coarse_level<T> *>(d.data()),
+                                    (int)L.size() - lvl, fuse_side_, set_.relaxation,
+                                    set_.nrelax_pre, set_.nrelax_post, abd_.data(), bbd_.data(),
+                                    cur_, L[lvl].ext.nb);
+                check_hip(hipGetLastError(), "coarse_cycle");
+                return;
+            }
+        }
         auto &F = L[lvl];
         auto &C = L[lvl + 1];
         bool bT_ok = false;  // b is fixed for this visit of the level

@@ -528,6 +528,7 @@ int cdr_hip_solver3d_create(const double *so_dev, int nstncl, int nx, int ny, in
         s->plane.relaxation < CDR_RELAX_POINT || s->plane.relaxation > CDR_RELAX_LINE_XY ||
         s->plane.maxiter < 0 || s->plane.nrelax_pre < 0 || s->plane.nrelax_post < 0 ||
         s->plane.min_coarse < 2 || (s->cycle != CDR_CYCLE_V && s->cycle != CDR_CYCLE_F) ||
+        (s->plane.fused_coarse != 0 && s->plane.fused_coarse != 1) ||
         (s->precision != CDR_PREC_F64 && s->precision != CDR_PREC_MIXED))
         return CDR_ERR_ARG;
     *out = nullptr;

@@ -206,6 +206,9 @@ typedef struct {
     int cycle;        /* solver.cycle.type: CDR_CYCLE_V (default) or CDR_CYCLE_F.  Like the
                          reference, an F-cycle recomputes x from b alone (include/cedar/cycle/
                          fcycle.h:75-81), so successive cycles of a solve repeat */
+    int fused_coarse; /* 1 (default): in V-cycles, the levels of at most 32 points per side (below
+                         the finest) run as one fused launch of one workgroup per problem; the
+                         result is bitwise that of one launch per step.  0: one launch per step */
 } cdr_solver_settings;
 
 typedef struct cdr_hip_solver2d cdr_hip_solver2d;

@@ -187,3 +187,38 @@ def test_fcycle_matches_oracle(C, relax, n, nine, precision):
         # matches the plain F-cycle and the later ones keep reducing the residual
         assert abs(hg[0] - ho[0]) <= 1e-5 * ho[0]
         assert hg[2] < hg[1] < hg[0]
+
+
+@pytest.mark.parametrize("relaxation", ["point", "line-x", "line-y", "line-xy"])
+@pytest.mark.parametrize("precision", ["f64", "mixed"])
+@pytest.mark.parametrize("nx,ny,cycle", [(255, 255, "v"), (200, 131, "v"), (257, 257, "f")])
+def test_fused_coarse_cycle_is_bitwise_unfused(relaxation, precision, nx, ny, cycle):
+    """The fused coarse-level cycle (one launch for all levels of at most 32 points per side)
+    gives bitwise the iterates and residual histories of one launch per step."""
+    import cedar_amd
+    from cedar_amd import gallery as G
+    k = G.block_lognormal(nx + 1, ny + 1, 8, 1.5, 31 + nx, 1e3)
+    so = torch.as_tensor(G.q1_diffusion(k)).cuda()
+    b = torch.as_tensor(G.rhs_uniform(nx, ny, 5 + ny)).cuda()
+    kw = dict(relaxation=relaxation, nrelax_pre=2, nrelax_post=1, maxiter=12, tol=1e-12,
+              precision=precision, cycle=cycle)
+    x0, h0 = cedar_amd.Solver(so, fused_coarse=False, **kw).solve(b)
+    x1, h1 = cedar_amd.Solver(so, fused_coarse=True, **kw).solve(b)
+    assert np.array_equal(h0, h1)
+    assert torch.equal(x0, x1)
+
+
+def test_fused_coarse_cycle_batched():
+    import cedar_amd
+    from cedar_amd import gallery as G
+    n, nb = 63, 3
+    sos = torch.stack([torch.as_tensor(G.q1_diffusion(G.block_lognormal(n + 1, n + 1, 8, 2.0, 60 + t, 1e4)))
+                       for t in range(nb)]).cuda()
+    bs = torch.stack([torch.as_tensor(G.rhs_uniform(n, n, 70 + t)) for t in range(nb)]).cuda()
+    kw = dict(relaxation="line-xy", nrelax_pre=1, nrelax_post=1, maxiter=30, tol=1e-10,
+              precision="mixed")
+    x0, h0 = cedar_amd.Solver(sos, fused_coarse=False, **kw).solve(bs)
+    x1, h1 = cedar_amd.Solver(sos, fused_coarse=True, **kw).solve(bs)
+    for t in range(nb):
+        assert np.array_equal(h0[t], h1[t])
+    assert torch.equal(x0, x1)
