@@ -85,6 +85,9 @@ PROTOTYPES = {
                                     ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                     ctypes.POINTER(_vp)]),
     "cdr_hip_solver2d_times": (_i, [_vp, ctypes.POINTER(_d), ctypes.POINTER(_d)]),
+    "cdr_hip_solver2d_timers": (_i, [_vp, _i]),
+    "cdr_hip_solver2d_save_timers": (_i, [_vp, ctypes.c_char_p]),
+    "cdr_hip_stencil2d_write": (_i, [_P, _i, _i, _i, ctypes.c_char_p]),
     "cdr_hip_solver2d_destroy": (None, [_vp]),
     # 3D kernels
     "cdr_hip_norm_work3d": (_sz, [_i, _i, _i]),

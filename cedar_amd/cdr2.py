@@ -301,6 +301,15 @@ class Solver:
         hs = [np.array(hist[k * m:k * m + nit[k]]) for k in range(self.nb)]
         return x, (hs if self.batched else hs[0])
 
+    def timers(self, on=True):
+        """Per-level device timers of the V-cycle phases (cycles then run without graph replay
+        and fused coarse levels; the results are the same)."""
+        _abi.call("cdr_hip_solver2d_timers", self._h, 1 if on else 0)
+
+    def save_timers(self, path):
+        """Writes the timers as JSON in the reference's time_log layout and resets them."""
+        _abi.call("cdr_hip_solver2d_save_timers", self._h, str(path).encode())
+
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             _abi.load().cdr_hip_solver2d_destroy(self._h)
@@ -332,3 +341,12 @@ def _hip():
         _hiprt.hipMemcpy.restype = ctypes.c_int
         _hiprt.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
     return _hiprt
+
+
+def write_stencil(so, path):
+    """Text dump of a 2D stencil in the reference's layout (src/2d/stencil_op.cc:5-51)."""
+    so = _t(so)
+    so = so if isinstance(so, torch.Tensor) else torch.as_tensor(np.ascontiguousarray(so))
+    so = so.to(device="cuda", dtype=torch.float64).contiguous()
+    ns, jj, ii = so.shape
+    _abi.call("cdr_hip_stencil2d_write", _p(so), ns, ii, jj, str(path).encode())

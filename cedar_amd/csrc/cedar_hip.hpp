@@ -78,6 +78,47 @@ private:
     std::size_t n_ = 0;
 };
 
+// Text dump of a 2D stencil (host copy, ghost-inclusive ii x jj per plane) in the reference's
+// layout (src/2d/stencil_op.cc:5-51): per interior point the running index, i, j (1-based with
+// the ghost layer) and the neighbour coefficients as a 3x3 (9-point) or cross (5-point) row,
+// off-diagonals with their sign restored, 7 significant digits.
+inline std::string stencil_text(const double *so, int nstncl, int ii, int jj) {
+    const std::size_t P = (std::size_t)ii * jj;
+    auto S = [&](int i, int j, int k) { return so[P * k + (std::size_t)j * ii + i]; };
+    std::string out;
+    char buf[64];
+    auto num = [&](double v) {
+        std::snprintf(buf, sizeof(buf), "%.7e", v);
+        out += buf;
+    };
+    for (int j = 1; j <= jj - 2; ++j)
+        for (int i = 1; i <= ii - 2; ++i) {
+            std::snprintf(buf, sizeof(buf), "%4d %4d, %4d, ", (j - 1) * ii + i - 1, i + 1, j + 1);
+            out += buf;
+            if (nstncl == 3) {  // c, w, s
+                num(-S(i, j + 1, 2));
+                num(-S(i, j, 1));
+                out += ' ';
+                num(S(i, j, 0));
+                num(-S(i + 1, j, 1));
+                num(-S(i, j, 2));
+            } else {  // c, w, s, sw, nw
+                num(-S(i, j + 1, 4));
+                num(-S(i, j + 1, 2));
+                num(-S(i + 1, j + 1, 3));
+                num(-S(i, j, 1));
+                out += ' ';
+                num(S(i, j, 0));
+                num(-S(i + 1, j, 1));
+                num(-S(i, j, 3));
+                num(-S(i, j, 2));
+                num(-S(i + 1, j, 4));
+            }
+            out += '\n';
+        }
+    return out;
+}
+
 enum class Dir { DOWN = CDR_DOWN, UP = CDR_UP };  // cycle::Dir
 enum class relax_dir { x, y };
 

@@ -243,6 +243,36 @@ int cdr_hip_solver2d_times(const cdr_hip_solver2d *slv, double *setup_s, double 
     return CDR_OK;
 }
 
+int cdr_hip_solver2d_timers(cdr_hip_solver2d *slv, int on) {
+    if (!slv || (on != 0 && on != 1)) return CDR_ERR_ARG;
+    slv->impl->set_timers(on != 0);
+    return CDR_OK;
+}
+
+int cdr_hip_solver2d_save_timers(cdr_hip_solver2d *slv, const char *path) {
+    if (!slv || !path) return CDR_ERR_ARG;
+    return guarded([&] {
+        slv->impl->save_timers(path);
+        return CDR_OK;
+    });
+}
+
+int cdr_hip_stencil2d_write(const double *so_dev, int nstncl, int ii, int jj, const char *path) {
+    if (!so_dev || !path || (nstncl != 3 && nstncl != 5) || ii < 3 || jj < 3) return CDR_ERR_ARG;
+    return guarded([&] {
+        std::vector<double> so((std::size_t)nstncl * ii * jj);
+        cedar::hip::check_hip(hipMemcpy(so.data(), so_dev, so.size() * sizeof(double),
+                                        hipMemcpyDeviceToHost),
+                              "hipMemcpy");
+        const std::string text = cedar::hip::stencil_text(so.data(), nstncl, ii, jj);
+        std::FILE *f = std::fopen(path, "w");
+        if (!f) throw cedar::hip::error(CDR_ERR_ARG, std::string("cannot open ") + path);
+        std::fwrite(text.data(), 1, text.size(), f);
+        std::fclose(f);
+        return CDR_OK;
+    });
+}
+
 void cdr_hip_solver2d_destroy(cdr_hip_solver2d *slv) { delete slv; }
 
 }  // extern "C"

@@ -13,6 +13,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -197,6 +202,7 @@ public:
         float ms = 0.f;
         check_hip(hipEventElapsedTime(&ms, ev0_, ev1_), "hipEventElapsedTime");
         solve_s_ = ms * 1e-3;
+        log_time(0, "solve", solve_s_);
         if (niter)
             for (int p = 0; p < nb; ++p) niter[p] = it[p];
         return status;
@@ -231,7 +237,7 @@ private:
     // nodes only (fills are kernels, no memset nodes).
     template <typename F>
     void run_maybe_graph(graph_t &g, const void *k1, const void *k2, F body) {
-        if (!set_.graph) {
+        if (!set_.graph || timing_) {
             body();
             return;
         }
@@ -355,6 +361,7 @@ private:
         float ms = 0.f;
         check_hip(hipEventElapsedTime(&ms, ev0_, ev1_), "hipEventElapsedTime");
         setup_s_ = ms * 1e-3;
+        log_time(0, "setup", setup_s_);
     }
 
     grid_func<double> &abd_grid() {
@@ -532,7 +539,7 @@ private:
 
     template <typename T>
     void ncycle(std::vector<level_data<T>> &L, int lvl, grid_func<T> &x, const grid_func<T> &b) {
-        if (lvl == fuse_lvl_ && x.p == L[lvl].x.p && b.p == L[lvl].b.p) {
+        if (lvl == fuse_lvl_ && !timing_ && x.p == L[lvl].x.p && b.p == L[lvl].b.p) {
             // this level and all coarser ones in one launch (bitwise the same V-cycle)
             const dbuf<unsigned char> &d = std::is_same<T, double>::value ? fuse64_ : fuse32_;
             if (d.data()) {
@@ -547,16 +554,100 @@ private:
         auto &F = L[lvl];
         auto &C = L[lvl + 1];
         bool bT_ok = false;  // b is fixed for this visit of the level
-        smooth<T>(F, x, b, Dir::DOWN, bT_ok);
-        km_.run<kernels::residual<T>>(F.A, x, b, F.res);
-        km_.run<kernels::restriction<T>>(C.P, F.res, C.b);
+        // timer labels and levels as in include/cedar/cycle/vcycle.h:61-109
+        timed(lvl, "relaxation", [&] { smooth<T>(F, x, b, Dir::DOWN, bT_ok); });
+        timed(lvl, "residual", [&] { km_.run<kernels::residual<T>>(F.A, x, b, F.res); });
+        timed(lvl, "restrict", [&] { km_.run<kernels::restriction<T>>(C.P, F.res, C.b); });
         k2::fill<T>(C.x.p, T(0), C.ext.total(), cur_);
         if (lvl + 1 == (int)L.size() - 1)
-            km_.run<kernels::solve_cg<T>>(C.x, C.b, abd_grid(), bbd_.data());
+            timed(lvl + 1, "coarse-solve", [&] {
+                km_.run<kernels::solve_cg<T>>(C.x, C.b, abd_grid(), bbd_.data());
+            });
         else
             ncycle<T>(L, lvl + 1, C.x, C.b);
-        km_.run<kernels::interp_add<T>>(C.P, C.x, F.res, x);
-        smooth<T>(F, x, b, Dir::UP, bT_ok);
+        timed(lvl, "interp-add", [&] { km_.run<kernels::interp_add<T>>(C.P, C.x, F.res, x); });
+        timed(lvl, "relaxation", [&] { smooth<T>(F, x, b, Dir::UP, bT_ok); });
+    }
+
+public:
+    // ---- timers (include/cedar/util/time_log.h) -------------------------------------------
+    // With timers on, cycles run eagerly (no graph replay, no fused coarse levels) and every
+    // phase of the V-cycle is bracketed by events on the solver's stream; save_timers writes the
+    // accumulated device times per level and label in the reference's serial JSON layout
+    // (src/util/time_log.cc:103-129) and resets them.
+    void set_timers(bool on) { timing_ = on; }
+
+    void save_timers(const std::string &path) {
+        collect_timers();
+        std::ostringstream os;
+        os << "{\n    \"levels\": [\n";
+        for (int i = (int)tlog_.size() - 1; i >= 0; --i) {
+            os << "        {";
+            bool first = true;
+            for (const auto &kv : tlog_[i]) {
+                os << (first ? "\n" : ",\n") << "            \"" << kv.first << "\": {\n"
+                   << "                \"time\": \"" << fmt(kv.second.first) << "\",\n"
+                   << "                \"count\": \"" << kv.second.second << "\"\n"
+                   << "            }";
+                first = false;
+            }
+            os << (first ? "}" : "\n        }") << (i > 0 ? ",\n" : "\n");
+        }
+        os << "    ]\n}\n";
+        std::ofstream f(path);
+        if (!f) throw error(CDR_ERR_ARG, "cannot open " + path);
+        f << os.str();
+        tlog_.clear();
+    }
+
+private:
+    struct timed_phase {
+        int level;
+        const char *label;
+        hipEvent_t a, b;
+    };
+    bool timing_ = false;
+    std::vector<timed_phase> phases_;
+    std::vector<std::map<std::string, std::pair<double, int>>> tlog_;  // level -> label -> (s, n)
+
+    template <typename F>
+    void timed(int level, const char *label, F &&f) {
+        if (!timing_) {
+            f();
+            return;
+        }
+        timed_phase t{level, label, nullptr, nullptr};
+        check_hip(hipEventCreate(&t.a), "hipEventCreate");
+        check_hip(hipEventCreate(&t.b), "hipEventCreate");
+        check_hip(hipEventRecord(t.a, cur_), "hipEventRecord");
+        f();
+        check_hip(hipEventRecord(t.b, cur_), "hipEventRecord");
+        phases_.push_back(t);
+    }
+
+    void log_time(int level, const char *label, double seconds) {
+        if ((int)tlog_.size() <= level) tlog_.resize(level + 1);
+        auto &e = tlog_[level][label];
+        e.first += seconds;
+        e.second += 1;
+    }
+
+    void collect_timers() {
+        for (auto &t : phases_) {
+            float ms = 0.f;
+            if (hipEventSynchronize(t.b) == hipSuccess &&
+                hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess)
+                log_time(t.level, t.label, ms * 1e-3);
+            (void)hipEventDestroy(t.a);
+            (void)hipEventDestroy(t.b);
+        }
+        phases_.clear();
+    }
+
+    static std::string fmt(double v) {
+        char buf[40];
+        std::snprintf(buf, sizeof(buf), "%.17g", v);
+        return buf;
     }
 };
 

@@ -247,6 +247,18 @@ int cdr_hip_solver2d_level(const cdr_hip_solver2d *slv, int lvl, int *nx, int *n
                            const double **sor1);
 /* Wall time (s, HIP events) of the last create (setup) and solve. */
 int cdr_hip_solver2d_times(const cdr_hip_solver2d *slv, double *setup_s, double *solve_s);
+/* Per-level timers (timer_begin / timer_end / timer_save of include/cedar/util/time_log.h:95-127):
+ * with timers on, cycles run eagerly (no graph replay, no fused coarse levels; the same results)
+ * and each phase of the V-cycle ("relaxation", "residual", "restrict", "coarse-solve",
+ * "interp-add") is timed per level on the device; "setup" and "solve" are logged at level 0.
+ * cdr_hip_solver2d_save_timers writes them to path in the reference's serial JSON layout
+ * (src/util/time_log.cc:103-129: {"levels": [coarsest .. finest, label: {time, count}]}) and
+ * resets them. */
+int cdr_hip_solver2d_timers(cdr_hip_solver2d *slv, int on);
+int cdr_hip_solver2d_save_timers(cdr_hip_solver2d *slv, const char *path);
+/* Writes a device stencil (nstncl planes of ii x jj) as text in the layout of the reference's
+ * operator<< (src/2d/stencil_op.cc:5-51). */
+int cdr_hip_stencil2d_write(const double *so_dev, int nstncl, int ii, int jj, const char *path);
 void cdr_hip_solver2d_destroy(cdr_hip_solver2d *slv);
 
 /* ---------------------------------------------------------------------------------------
