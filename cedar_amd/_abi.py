@@ -116,6 +116,9 @@ PROTOTYPES = {
                                     ctypes.POINTER(_i), ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "cdr_hip_solver3d_times": (_i, [_vp, ctypes.POINTER(_d), ctypes.POINTER(_d)]),
     "cdr_hip_solver3d_destroy": (None, [_vp]),
+    # matvec
+    "cdr_hip_matvec2d_f64": (_i, [_P, _i, _P, _P, _i, _i, _vp]),
+    "cdr_hip_matvec3d_f64": (_i, [_P, _i, _P, _P, _i, _i, _i, _vp]),
     # differentiated 2D solves
     "cdr_hip_q1_assemble2d_f64": (_i, [_P, _P, _i, _i, _i, _vp]),
     "cdr_hip_q1_gradient2d_f64": (_i, [_P, _P, _P, _i, _i, _i, _vp]),

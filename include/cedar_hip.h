@@ -355,6 +355,18 @@ int cdr_hip_solver3d_times(const cdr_hip_solver3d *slv, double *setup_s, double 
 void cdr_hip_solver3d_destroy(cdr_hip_solver3d *slv);
 
 /* ---------------------------------------------------------------------------------------
+ * y = A q over the interior (ghosts of y untouched), terms in the order of
+ * BMG2_SymStd_UTILS_matvec (src/2d/ftn/mpi/BMG2_SymStd_UTILS_matvec.f90: 9-point branch for
+ * nstncl 5, 5-point for 3) and BMG3_SymStd_UTILS_matvec (src/3d/ftn/mpi/
+ * BMG3_SymStd_UTILS_matvec.f90:79-124, nstncl 14 or 4); the kernels behind cedar_matvec
+ * (include/cedar/kernels/matvec.h).  kk - 2 <= 65535.
+ * ------------------------------------------------------------------------------------- */
+int cdr_hip_matvec2d_f64(const double *so, int nstncl, const double *q, double *y, int ii, int jj,
+                         void *stream);
+int cdr_hip_matvec3d_f64(const double *so, int nstncl, const double *q, double *y, int ii, int jj,
+                         int kk, void *stream);
+
+/* ---------------------------------------------------------------------------------------
  * Differentiated 2D diffusion solves (no counterpart in the reference).  kcell holds nb blocks
  * of (ny+1) x (nx+1) per-cell coefficients (cell (cj, ci) at cj*(nx+1) + ci, its corners the
  * nodes (ci, cj), (ci+1, cj), (ci+1, cj+1), (ci, cj+1) of the ghost-inclusive node grid).

@@ -69,6 +69,26 @@ double orc2_norm(const double *x, int ii, int jj)
     return sqrt(s);
 }
 
+/* y = A q: src/2d/ftn/mpi/BMG2_SymStd_UTILS_matvec.f90 (9-point branch for a 5-plane
+ * operator, 5-point branch for a 3-plane one), terms summed left to right as written there */
+void orc2_matvec(const double *so, int nstncl, const double *q, double *y, int ii, int jj)
+{
+#define SO(i, j, k) P2(so, ii, jj, i, j, k)
+#define Q(i, j) V2(q, ii, i, j)
+    for (int j = 2; j <= jj - 1; ++j)
+        for (int i = 2; i <= ii - 1; ++i) {
+            double v = SO(i, j, KO) * Q(i, j) - SO(i, j, KW) * Q(i - 1, j)
+                     - SO(i + 1, j, KW) * Q(i + 1, j) - SO(i, j, KS) * Q(i, j - 1)
+                     - SO(i, j + 1, KS) * Q(i, j + 1);
+            if (nstncl == 5)
+                v = v - SO(i, j, KSW) * Q(i - 1, j - 1) - SO(i + 1, j, KNW) * Q(i + 1, j - 1)
+                      - SO(i, j + 1, KNW) * Q(i - 1, j + 1) - SO(i + 1, j + 1, KSW) * Q(i + 1, j + 1);
+            V2(y, ii, i, j) = v;
+        }
+#undef SO
+#undef Q
+}
+
 /* SOR(:,:,msor=2) = 1/SO_c: src/2d/ftn/BMG2_SymStd_SETUP_recip.f90:63-67 */
 void orc2_setup_recip(const double *so, double *sor, int ii, int jj)
 {

@@ -50,6 +50,8 @@ void orc2_setup_galerkin(const double *so, int nstncl, const double *ci, double 
 int  orc2_setup_cg(const double *so, int nstncl, double *abd, int ii, int jj);
 int  orc2_solve_cg(double *q, const double *qf, const double *abd, double *bbd, int ii, int jj);
 double orc2_norm(const double *x, int ii, int jj);
+/* y = A q over the interior (ghosts of y untouched) */
+void orc2_matvec(const double *so, int nstncl, const double *q, double *y, int ii, int jj);
 
 /* ---- textbook LAPACK restatements (reference LAPACK 3.x algorithms) ---- */
 int  orc_dpttrf(int n, double *d, double *e);

@@ -889,6 +889,54 @@ double orc3_norm(const double *x, int ii, int jj, int kk)
     return pow(s, 0.5);
 }
 
+/* y = A q: src/3d/ftn/mpi/BMG3_SymStd_UTILS_matvec.f90:79-124, terms summed left to right in
+ * the order written there (27-point: the 26 neighbours plane by plane; 7-point: w, n, e, s, b, t) */
+void orc3_matvec(const double *so, int nstncl, const double *q, double *y, int ii, int jj, int kk)
+{
+    const size_t P = (size_t)ii * jj * kk;
+    for (int k = 2; k <= kk - 1; ++k)
+        for (int j = 2; j <= jj - 1; ++j)
+            for (int i = 2; i <= ii - 1; ++i) {
+                double v;
+                if (nstncl == 14) {
+                    v = S3(i, j, k, kp) * Q3(q, i, j, k) - S3(i, j, k, kpw) * Q3(q, i - 1, j, k)
+                      - S3(i, j + 1, k, kpnw) * Q3(q, i - 1, j + 1, k)
+                      - S3(i, j + 1, k, kps) * Q3(q, i, j + 1, k)
+                      - S3(i + 1, j + 1, k, kpsw) * Q3(q, i + 1, j + 1, k)
+                      - S3(i + 1, j, k, kpw) * Q3(q, i + 1, j, k)
+                      - S3(i + 1, j, k, kpnw) * Q3(q, i + 1, j - 1, k)
+                      - S3(i, j, k, kps) * Q3(q, i, j - 1, k)
+                      - S3(i, j, k, kpsw) * Q3(q, i - 1, j - 1, k)
+                      - S3(i, j, k, kb) * Q3(q, i, j, k - 1)
+                      - S3(i, j, k, kbw) * Q3(q, i - 1, j, k - 1)
+                      - S3(i, j + 1, k, kbnw) * Q3(q, i - 1, j + 1, k - 1)
+                      - S3(i, j + 1, k, kbn) * Q3(q, i, j + 1, k - 1)
+                      - S3(i + 1, j + 1, k, kbne) * Q3(q, i + 1, j + 1, k - 1)
+                      - S3(i + 1, j, k, kbe) * Q3(q, i + 1, j, k - 1)
+                      - S3(i + 1, j, k, kbse) * Q3(q, i + 1, j - 1, k - 1)
+                      - S3(i, j, k, kbs) * Q3(q, i, j - 1, k - 1)
+                      - S3(i, j, k, kbsw) * Q3(q, i - 1, j - 1, k - 1)
+                      - S3(i, j, k + 1, kb) * Q3(q, i, j, k + 1)
+                      - S3(i, j, k + 1, kbe) * Q3(q, i - 1, j, k + 1)
+                      - S3(i, j + 1, k + 1, kbse) * Q3(q, i - 1, j + 1, k + 1)
+                      - S3(i, j + 1, k + 1, kbs) * Q3(q, i, j + 1, k + 1)
+                      - S3(i + 1, j + 1, k + 1, kbsw) * Q3(q, i + 1, j + 1, k + 1)
+                      - S3(i + 1, j, k + 1, kbw) * Q3(q, i + 1, j, k + 1)
+                      - S3(i + 1, j, k + 1, kbnw) * Q3(q, i + 1, j - 1, k + 1)
+                      - S3(i, j, k + 1, kbn) * Q3(q, i, j - 1, k + 1)
+                      - S3(i, j, k + 1, kbne) * Q3(q, i - 1, j - 1, k + 1);
+                } else {
+                    v = S3(i, j, k, kp) * Q3(q, i, j, k) - S3(i, j, k, kpw) * Q3(q, i - 1, j, k)
+                      - S3(i, j + 1, k, kps) * Q3(q, i, j + 1, k)
+                      - S3(i + 1, j, k, kpw) * Q3(q, i + 1, j, k)
+                      - S3(i, j, k, kps) * Q3(q, i, j - 1, k)
+                      - S3(i, j, k, kb) * Q3(q, i, j, k - 1)
+                      - S3(i, j, k + 1, kb) * Q3(q, i, j, k + 1);
+                }
+                Q3(y, i, j, k) = v;
+            }
+}
+
 /* ------------------------------------------------------------------------------------------
  * plane relaxation (include/cedar/3d/relax_planes.h:35-163, src/3d/relax_planes.cc:25-238).
  * Here ipl and the 2D/3D indices are the reference's C++ indices (ghost = 0); plane operators

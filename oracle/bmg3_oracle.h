@@ -55,6 +55,9 @@ int  orc3_setup_cg(const double *so, int nstncl, double *abd, int ldab, int ii, 
 void orc3_solve_cg(double *q, const double *qf, const double *abd, int ldab, double *bbd,
                    int ii, int jj, int kk);
 double orc3_norm(const double *x, int ii, int jj, int kk);
+/* y = A q over the interior (ghosts of y untouched) */
+void orc3_matvec(const double *so, int nstncl, const double *q, double *y, int ii, int jj,
+                 int kk);
 
 /* ---- plane relaxation pieces (include/cedar/3d/relax_planes.h, src/3d/relax_planes.cc) ---- */
 /* 2D operator of plane ipl (1..n of the direction normal to the plane) */

@@ -90,6 +90,14 @@ def norm(x):
     return lib().orc2_norm(_ptr(x), _i(ii), _i(jj))
 
 
+def matvec(so, q, y=None):
+    """y = A q over the interior; ghosts of y kept (zero when y is not given)."""
+    jj, ii = q.shape
+    y = np.zeros_like(q) if y is None else y.copy()
+    lib().orc2_matvec(_ptr(so), _i(so.shape[0]), _ptr(q), _ptr(y), _i(ii), _i(jj))
+    return y
+
+
 def setup_recip(so):
     _, jj, ii = so.shape
     sor = np.zeros((2, jj, ii))

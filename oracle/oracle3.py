@@ -63,6 +63,14 @@ def norm(x):
     return lib().orc3_norm(_p(x), _i(ii), _i(jj), _i(kk))
 
 
+def matvec(so, q, y=None):
+    """y = A q over the interior; ghosts of y kept (zero when y is not given)."""
+    kk, jj, ii = q.shape
+    y = np.zeros_like(q) if y is None else y.copy()
+    lib().orc3_matvec(_p(so), _i(so.shape[0]), _p(q), _p(y), _i(ii), _i(jj), _i(kk))
+    return y
+
+
 def setup_recip(so):
     _, kk, jj, ii = so.shape
     sor = np.zeros((2, kk, jj, ii))
