@@ -162,6 +162,42 @@ def readme_problem(n=400):
     return so, b, exact
 
 
+def periodic_problem(nx, ny, periodic=(True, False)):
+    """Cedar's periodic example (examples/basic-2d-ser/periodic.cc:16-155): 5-point Poisson on
+    nx x ny points, periodic in the flagged directions (a periodic direction of n points has
+    period n, h = 1/n; its ghost rows/columns of the operator and of b hold the periodic images),
+    b = 8 pi^2 sin(2 pi x) sin(2 pi y) h_x h_y; returns (so, b, exact), exact = sin(2 pi x)
+    sin(2 pi y)."""
+    px, py = bool(periodic[0]), bool(periodic[1])
+    so = np.zeros((3, ny + 2, nx + 2))
+    mx, my = nx - 1 if px else nx, ny - 1 if py else ny
+    hx, hy = 1.0 / (mx + 1), 1.0 / (my + 1)
+    xh, yh = hy / hx, hx / hy
+    ibeg, jbeg = (1 if px else 2), (1 if py else 2)
+    so[S, jbeg:ny + 1, 1:nx + 1] = 1.0 * yh
+    so[W, 1:ny + 1, ibeg:nx + 1] = 1.0 * xh
+    so[C, 1:ny + 1, 1:nx + 1] = 2 * xh + 2 * yh
+    if px:
+        so[:, 1:ny + 1, ibeg - 1] = so[:, 1:ny + 1, nx]
+        so[:, 1:ny + 1, nx + 1] = so[:, 1:ny + 1, ibeg]
+    if py:
+        so[:, jbeg - 1, 1:nx + 1] = so[:, ny, 1:nx + 1]
+        so[:, ny + 1, 1:nx + 1] = so[:, jbeg, 1:nx + 1]
+    X, Y = np.meshgrid(np.arange(nx + 2) * hx, np.arange(ny + 2) * hy)
+    b = np.zeros((ny + 2, nx + 2))
+    b[1:ny + 1, 1:nx + 1] = (8 * (np.pi * np.pi) * np.sin(2 * np.pi * X) * np.sin(2 * np.pi * Y)
+                             * (hx * hy))[1:ny + 1, 1:nx + 1]
+    if px:
+        b[:, 0] = b[:, nx]
+        b[:, nx + 1] = b[:, 1]
+    if py:
+        b[0, :] = b[ny, :]
+        b[ny + 1, :] = b[1, :]
+    exact = np.zeros((ny + 2, nx + 2))
+    exact[1:ny + 1, 1:nx + 1] = (np.sin(2 * np.pi * X) * np.sin(2 * np.pi * Y))[1:ny + 1, 1:nx + 1]
+    return so, b, exact
+
+
 # ---------------------------------------------------------------------------------------------
 # 3D: a grid function is (nz+2, ny+2, nx+2), a stencil (4, ...) {p, pw, ps, b} (7-point) or
 # (14, ...) {p, pw, ps, b, psw, pnw, bw, bnw, bn, bne, be, bse, bs, bsw} (27-point)

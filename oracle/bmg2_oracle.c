@@ -724,6 +724,7 @@ typedef struct {
 
 struct orc2_solver {
     orc2_settings s;
+    int jpn;          /* BMG_get_bc(per_mask) */
     int nlevels;
     orc2_level *lv;
     double *abd, *bbd;
@@ -752,7 +753,7 @@ static void orc2_smooth(orc2_solver *slv, orc2_level *L, double *x, const double
     for (int r = 0; r < nrep; ++r) {
         switch (slv->s.relax) {
         case ORC_RELAX_POINT:
-            orc2_relax_gs(L->so, L->nstncl, b, x, L->sor[0], L->ii, L->jj, dir);
+            orc2_relax_gs_bc(L->so, L->nstncl, b, x, L->sor[0], L->ii, L->jj, dir, slv->jpn);
             break;
         case ORC_RELAX_LINE_X:
             orc2_relax_lines_x(L->so, L->nstncl, b, x, L->sor[0], L->ii, L->jj, dir);
@@ -775,8 +776,11 @@ static void orc2_smooth(orc2_solver *slv, orc2_level *L, double *x, const double
 orc2_solver *orc2_solver_create(const double *so_fine, int nstncl, int nx, int ny,
                                 const orc2_settings *s)
 {
+    const int jpn = orc2_bc_from_mask(s->periodic);
+    if (jpn != 0 && s->relax != ORC_RELAX_POINT) return NULL; /* periodic lines: not restated */
     orc2_solver *slv = (orc2_solver *)calloc(1, sizeof(orc2_solver));
     slv->s = *s;
+    slv->jpn = jpn;
     int nl = orc2_num_levels(nx, ny, s->min_coarse);
     if (s->num_levels > 0 && s->num_levels <= nl) nl = s->num_levels;
     slv->nlevels = nl;
@@ -806,7 +810,8 @@ orc2_solver *orc2_solver_create(const double *so_fine, int nstncl, int nx, int n
     }
     slv->scratch = orc_zeros((size_t)maxjj);
     orc2_level *C = &slv->lv[nl - 1];
-    slv->nabd1 = C->nx + 2;
+    /* include/cedar/2d/solver.h:116-121: a dense n x n array when periodic */
+    slv->nabd1 = jpn ? C->nx * C->ny : C->nx + 2;
     slv->nabd2 = C->nx * C->ny;
     slv->abd = orc_zeros((size_t)slv->nabd1 * slv->nabd2);
     slv->bbd = orc_zeros((size_t)slv->nabd2);
@@ -814,8 +819,8 @@ orc2_solver *orc2_solver_create(const double *so_fine, int nstncl, int nx, int n
     /* setup: interp -> galerkin -> relax per level (include/cedar/multilevel.h:243-265) */
     for (int l = 0; l < nl - 1; ++l) {
         orc2_level *F = &slv->lv[l], *K = &slv->lv[l + 1];
-        orc2_setup_interp(F->so, F->nstncl, K->ci, F->ii, F->jj, K->ii, K->jj);
-        orc2_setup_galerkin(F->so, F->nstncl, K->ci, K->so, F->ii, F->jj, K->ii, K->jj);
+        orc2_setup_interp_bc(F->so, F->nstncl, K->ci, F->ii, F->jj, K->ii, K->jj, jpn);
+        orc2_setup_galerkin_bc(F->so, F->nstncl, K->ci, K->so, F->ii, F->jj, K->ii, K->jj, jpn);
         switch (s->relax) {
         case ORC_RELAX_POINT: orc2_setup_recip(F->so, F->sor[0], F->ii, F->jj); break;
         case ORC_RELAX_LINE_X: orc2_setup_lines_x(F->so, F->sor[0], F->ii, F->jj); break;
@@ -825,7 +830,7 @@ orc2_solver *orc2_solver_create(const double *so_fine, int nstncl, int nx, int n
             orc2_setup_lines_y(F->so, F->sor[1], F->ii, F->jj);
         }
     }
-    orc2_setup_cg(C->so, C->nstncl, slv->abd, C->ii, C->jj);
+    orc2_setup_cg_bc(C->so, C->nstncl, slv->abd, C->ii, C->jj, jpn);
     return slv;
 }
 
@@ -860,13 +865,14 @@ static void orc2_ncycle(orc2_solver *slv, int lvl, double *x, const double *b)
     orc2_level *L = &slv->lv[lvl], *K = &slv->lv[lvl + 1];
     orc2_smooth(slv, L, x, b, ORC_DOWN);
     orc2_residual(L->so, L->nstncl, b, x, L->res, L->ii, L->jj);
-    orc2_restrict(K->ci, L->res, K->b, L->ii, L->jj, K->ii, K->jj);
+    orc2_restrict_bc(K->ci, L->res, K->b, L->ii, L->jj, K->ii, K->jj, slv->jpn);
     memset(K->x, 0, (size_t)K->ii * K->jj * sizeof(double));
     if (lvl + 1 == slv->nlevels - 1)
-        orc2_solve_cg(K->x, K->b, slv->abd, slv->bbd, K->ii, K->jj);
+        orc2_solve_cg_bc(K->x, K->b, slv->abd, slv->bbd, K->ii, K->jj, slv->jpn);
     else
         orc2_ncycle(slv, lvl + 1, K->x, K->b);
-    orc2_interp_add(x, K->x, L->res, L->so, L->nstncl, K->ci, K->ii, K->jj, L->ii, L->jj);
+    orc2_interp_add_bc(x, K->x, L->res, L->so, L->nstncl, K->ci, K->ii, K->jj, L->ii, L->jj,
+                       slv->jpn);
     orc2_smooth(slv, L, x, b, ORC_UP);
 }
 
@@ -874,7 +880,7 @@ void orc2_solver_vcycle(orc2_solver *slv, double *x, const double *b)
 {
     if (slv->nlevels == 1) {
         orc2_level *L = &slv->lv[0];
-        orc2_solve_cg(x, b, slv->abd, slv->bbd, L->ii, L->jj);
+        orc2_solve_cg_bc(x, b, slv->abd, slv->bbd, L->ii, L->jj, slv->jpn);
     } else {
         orc2_ncycle(slv, 0, x, b);
     }
@@ -887,16 +893,18 @@ static void orc2_fmg(orc2_solver *slv, int lvl, double *x, const double *b)
 {
     if (lvl == slv->nlevels - 1) {
         orc2_level *L = &slv->lv[lvl];
-        orc2_solve_cg(x, b, slv->abd, slv->bbd, L->ii, L->jj);
+        orc2_solve_cg_bc(x, b, slv->abd, slv->bbd, L->ii, L->jj, slv->jpn);
         return;
     }
     orc2_level *L = &slv->lv[lvl], *K = &slv->lv[lvl + 1];
-    orc2_restrict(K->ci, b, K->b, L->ii, L->jj, K->ii, K->jj);
+    /* the reference's restriction refreshes the ghosts of b in place when periodic */
+    orc2_restrict_bc(K->ci, (double *)b, K->b, L->ii, L->jj, K->ii, K->jj, slv->jpn);
     orc2_fmg(slv, lvl + 1, K->x, K->b);
     const size_t np = (size_t)L->ii * L->jj;
     memset(x, 0, np * sizeof(double));
     memset(L->res, 0, np * sizeof(double));
-    orc2_interp_add(x, K->x, L->res, L->so, L->nstncl, K->ci, K->ii, K->jj, L->ii, L->jj);
+    orc2_interp_add_bc(x, K->x, L->res, L->so, L->nstncl, K->ci, K->ii, K->jj, L->ii, L->jj,
+                       slv->jpn);
     orc2_ncycle(slv, lvl, x, b);
 }
 

@@ -53,6 +53,26 @@ double orc2_norm(const double *x, int ii, int jj);
 /* y = A q over the interior (ghosts of y untouched) */
 void orc2_matvec(const double *so, int nstncl, const double *q, double *y, int ii, int jj);
 
+/* ---- periodic branches (bmg2_periodic.c); jpn = BMG_get_bc(per_mask), 0 = Dirichlet ---- */
+int  orc2_bc_from_mask(int mask);
+void orc2_relax_gs_bc(const double *so, int nstncl, const double *qf, double *q,
+                      const double *sor, int ii, int jj, int updown, int jpn);
+void orc2_restrict_bc(const double *ci, double *q, double *qc, int iif, int jjf, int iic, int jjc,
+                      int jpn);
+void orc2_interp_add_bc(double *q, const double *qc, double *res, const double *so, int nstncl,
+                        const double *ci, int iic, int jjc, int iif, int jjf, int jpn);
+void orc2_setup_interp_bc(const double *so, int nstncl, double *ci, int iif, int jjf, int iic,
+                          int jjc, int jpn);
+void orc2_setup_galerkin_bc(const double *so, int nstncl, const double *ci, double *soc, int iif,
+                            int jjf, int iic, int jjc, int jpn);
+/* periodic coarse matrix: dense n x n, n = nx*ny, column-major, upper triangle */
+void orc2_assemble_cg_periodic(const double *so, int nstncl, double *abd, int ii, int jj, int jpn);
+int  orc2_setup_cg_bc(const double *so, int nstncl, double *abd, int ii, int jj, int jpn);
+int  orc2_solve_cg_bc(double *q, const double *qf, const double *abd, double *bbd, int ii, int jj,
+                      int jpn);
+int  orc_dpotrf_u(int n, double *a, int lda);
+void orc_dpotrs_u(int n, const double *a, int lda, double *b);
+
 /* ---- textbook LAPACK restatements (reference LAPACK 3.x algorithms) ---- */
 int  orc_dpttrf(int n, double *d, double *e);
 void orc_dpttrs(int n, const double *d, const double *e, double *b);
@@ -69,6 +89,7 @@ typedef struct {
     double tol;       /* solver.tol */
     int min_coarse;   /* solver.min_coarse, default 3 */
     int cycle;        /* solver.cycle.type: 0 = V (default), 1 = F */
+    int periodic;     /* grid.periodic as Cedar's per_mask: bit 0 x, bit 1 y (point relaxation) */
 } orc2_settings;
 
 typedef struct orc2_solver orc2_solver;

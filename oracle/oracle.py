@@ -33,7 +33,13 @@ def _ptr(a):
 
 class _Settings(ctypes.Structure):
     _fields_ = [("relax", _i), ("nrelax_pre", _i), ("nrelax_post", _i), ("num_levels", _i),
-                ("maxiter", _i), ("tol", ctypes.c_double), ("min_coarse", _i), ("cycle", _i)]
+                ("maxiter", _i), ("tol", ctypes.c_double), ("min_coarse", _i), ("cycle", _i),
+                ("periodic", _i)]
+
+
+def bc_from_mask(mask):
+    """BMG_get_bc (src/2d/ftn/BMG_get_bc.f90): per_mask bit 0 = x, bit 1 = y -> BoxMG BC id."""
+    return (0, 2, 1, 3)[mask]
 
 
 _lib = None
@@ -105,11 +111,11 @@ def setup_recip(so):
     return sor
 
 
-def relax_gs(so, qf, q, sor, updown):
+def relax_gs(so, qf, q, sor, updown, jpn=0):
     jj, ii = q.shape
     q = q.copy()
-    lib().orc2_relax_gs(_ptr(so), _i(so.shape[0]), _ptr(qf), _ptr(q), _ptr(sor), _i(ii), _i(jj),
-                        _i(updown))
+    lib().orc2_relax_gs_bc(_ptr(so), _i(so.shape[0]), _ptr(qf), _ptr(q), _ptr(sor), _i(ii),
+                           _i(jj), _i(updown), _i(jpn))
     return q
 
 
@@ -134,22 +140,25 @@ def relax_lines(so, qf, q, sor, updown, direction):
     return q
 
 
-def restrict(ci, q):
+def restrict(ci, q, jpn=0):
+    """P^T q; with periodic jpn the ghosts of q are refreshed first (on a copy)."""
     jjf, iif = q.shape
     _, jjc, iic = ci.shape
     qc = np.zeros((jjc, iic))
-    lib().orc2_restrict(_ptr(ci), _ptr(q), _ptr(qc), _i(iif), _i(jjf), _i(iic), _i(jjc))
+    q = np.ascontiguousarray(q, dtype=np.float64).copy()
+    lib().orc2_restrict_bc(_ptr(ci), _ptr(q), _ptr(qc), _i(iif), _i(jjf), _i(iic), _i(jjc),
+                           _i(jpn))
     return qc
 
 
-def interp_add(q, qc, res, so, ci):
+def interp_add(q, qc, res, so, ci, jpn=0):
     """Returns (q + P qc + affine term, res divided by SO_c in place as the reference does)."""
     jjf, iif = q.shape
     jjc, iic = qc.shape
     q = q.copy()
     res = res.copy()
-    lib().orc2_interp_add(_ptr(q), _ptr(qc), _ptr(res), _ptr(so), _i(so.shape[0]), _ptr(ci),
-                          _i(iic), _i(jjc), _i(iif), _i(jjf))
+    lib().orc2_interp_add_bc(_ptr(q), _ptr(qc), _ptr(res), _ptr(so), _i(so.shape[0]), _ptr(ci),
+                             _i(iic), _i(jjc), _i(iif), _i(jjf), _i(jpn))
     return q, res
 
 
@@ -158,21 +167,22 @@ def coarse_dims(ii, jj):
     return int((nx - 1) / 2.0 + 1) + 2, int((ny - 1) / 2.0 + 1) + 2
 
 
-def setup_interp(so, iic=None, jjc=None):
+def setup_interp(so, iic=None, jjc=None, jpn=0):
     ns, jjf, iif = so.shape
     if iic is None:
         iic, jjc = coarse_dims(iif, jjf)
     ci = np.zeros((8, jjc, iic))
-    lib().orc2_setup_interp(_ptr(so), _i(ns), _ptr(ci), _i(iif), _i(jjf), _i(iic), _i(jjc))
+    lib().orc2_setup_interp_bc(_ptr(so), _i(ns), _ptr(ci), _i(iif), _i(jjf), _i(iic), _i(jjc),
+                               _i(jpn))
     return ci
 
 
-def setup_galerkin(so, ci):
+def setup_galerkin(so, ci, jpn=0):
     ns, jjf, iif = so.shape
     _, jjc, iic = ci.shape
     soc = np.zeros((5, jjc, iic))
-    lib().orc2_setup_galerkin(_ptr(so), _i(ns), _ptr(ci), _ptr(soc), _i(iif), _i(jjf), _i(iic),
-                              _i(jjc))
+    lib().orc2_setup_galerkin_bc(_ptr(so), _i(ns), _ptr(ci), _ptr(soc), _i(iif), _i(jjf),
+                                 _i(iic), _i(jjc), _i(jpn))
     return soc
 
 
@@ -190,6 +200,40 @@ def solve_cg(abd, qf):
     bbd = np.zeros(abd.shape[0])
     lib().orc2_solve_cg(_ptr(q), _ptr(qf), _ptr(abd), _ptr(bbd), _i(ii), _i(jj))
     return q
+
+
+def assemble_cg_periodic(so, jpn):
+    """The dense upper triangle the periodic coarse setup factors: (n, n) C array == A(n, n)."""
+    ns, jj, ii = so.shape
+    n = (ii - 2) * (jj - 2)
+    abd = np.zeros((n, n))
+    lib().orc2_assemble_cg_periodic(_ptr(so), _i(ns), _ptr(abd), _i(ii), _i(jj), _i(jpn))
+    return abd
+
+
+def setup_cg_periodic(so, jpn):
+    ns, jj, ii = so.shape
+    n = (ii - 2) * (jj - 2)
+    abd = np.zeros((n, n))
+    info = lib().orc2_setup_cg_bc(_ptr(so), _i(ns), _ptr(abd), _i(ii), _i(jj), _i(jpn))
+    return abd, info
+
+
+def solve_cg_periodic(abd, qf, jpn):
+    jj, ii = qf.shape
+    q = np.zeros_like(qf)
+    bbd = np.zeros(abd.shape[0])
+    lib().orc2_solve_cg_bc(_ptr(q), _ptr(np.ascontiguousarray(qf)), _ptr(abd), _ptr(bbd), _i(ii),
+                           _i(jj), _i(jpn))
+    return q
+
+
+def dpotrf_u(a):
+    """Upper Cholesky of a symmetric (n, n) array (upper triangle read, Fortran order = a.T)."""
+    a = np.array(a, dtype=np.float64)
+    n = a.shape[0]
+    info = lib().orc_dpotrf_u(_i(n), _ptr(a), _i(n))
+    return a, info
 
 
 def dpttrf(d, e):
@@ -232,15 +276,18 @@ class Solver:
     """cdr2::solver<fsten>(so, conf): setup in the constructor, solve(b) from x0 = 0."""
 
     def __init__(self, so, relaxation="point", nrelax_pre=2, nrelax_post=1, num_levels=-1,
-                 maxiter=10, tol=1e-8, min_coarse=3, cycle="v"):
+                 maxiter=10, tol=1e-8, min_coarse=3, cycle="v", periodic=(False, False)):
         self.so = np.ascontiguousarray(so, dtype=np.float64)  # held for the solver's lifetime
         ns, jj, ii = self.so.shape
         self.ii, self.jj = ii, jj
+        mask = int(bool(periodic[0])) | (int(bool(periodic[1])) << 1)
         s = _Settings(RELAX[relaxation], nrelax_pre, nrelax_post, num_levels, maxiter, tol,
-                      min_coarse, {"v": 0, "f": 1}[cycle])
+                      min_coarse, {"v": 0, "f": 1}[cycle], mask)
         L = lib()
-        self._h = ctypes.c_void_p(L.orc2_solver_create(_ptr(self.so), ns, ii - 2, jj - 2,
-                                                       ctypes.byref(s)))
+        h = L.orc2_solver_create(_ptr(self.so), ns, ii - 2, jj - 2, ctypes.byref(s))
+        if not h:
+            raise ValueError("periodic boundaries need point relaxation")
+        self._h = ctypes.c_void_p(h)
         self.maxiter = maxiter
 
     @property

@@ -38,37 +38,38 @@ def setup_recip(so):
     return sor
 
 
-def relax_gs(so, qf, q, sor, updown):
+def relax_gs(so, qf, q, sor, updown, jpn=0):
     """src/2d/relax.cc:38-68 (k = kf = 1)."""
     ns, jj, ii = so.shape
     q = q.copy()
     ifd = 1 if ns == 3 else 0
     _o.ref().BMG2_SymStd_relax_GS(_i(1), _p(so), _p(qf), _p(q), _p(sor), _i(ii), _i(jj), _i(1),
-                                  _i(ifd), _i(ns), _i(2), _i(1), _i(updown), _i(0))
+                                  _i(ifd), _i(ns), _i(2), _i(1), _i(updown), _i(jpn))
     return q
 
 
-def restrict(ci, q):
-    """src/2d/restrict.cc:14-29."""
+def restrict(ci, q, jpn=0):
+    """src/2d/restrict.cc:14-29 (q is refreshed in place when periodic: a copy is passed)."""
     jjf, iif = q.shape
+    q = q.copy()
     _, jjc, iic = ci.shape
     qc = np.zeros((jjc, iic))
-    _o.ref().BMG2_SymStd_restrict(_p(q), _p(qc), _p(ci), _i(iif), _i(jjf), _i(iic), _i(jjc), _i(0))
+    _o.ref().BMG2_SymStd_restrict(_p(q), _p(qc), _p(ci), _i(iif), _i(jjf), _i(iic), _i(jjc), _i(jpn))
     return qc
 
 
-def interp_add(q, qc, res, so, ci):
+def interp_add(q, qc, res, so, ci, jpn=0):
     """src/2d/interp.cc:20-46 (res is divided by SO_c in place)."""
     jjf, iif = q.shape
     jjc, iic = qc.shape
     q = q.copy()
     res = res.copy()
     _o.ref().BMG2_SymStd_interp_add(_p(q), _p(qc), _p(res), _p(so), _p(ci), _i(iic), _i(jjc),
-                                    _i(iif), _i(jjf), _i(so.shape[0]), _i(0))
+                                    _i(iif), _i(jjf), _i(so.shape[0]), _i(jpn))
     return q, res
 
 
-def setup_interp(so, iic=None, jjc=None):
+def setup_interp(so, iic=None, jjc=None, jpn=0):
     """src/2d/interp.cc:49-106 (ifd = 1 for a 5-point fine operator)."""
     ns, jjf, iif = so.shape
     if iic is None:
@@ -77,16 +78,16 @@ def setup_interp(so, iic=None, jjc=None):
     soc = np.zeros((5, jjc, iic))
     ifd = 1 if ns == 3 else 0
     _o.ref().BMG2_SymStd_SETUP_interp_OI(_p(so), _p(soc), _p(ci), _i(iif), _i(jjf), _i(iic),
-                                         _i(jjc), _i(ifd), _i(ns), _i(0), _i(0))
+                                         _i(jjc), _i(ifd), _i(ns), _i(jpn), _i(0))
     return ci
 
 
-def setup_galerkin(so, ci):
+def setup_galerkin(so, ci, jpn=0):
     """include/cedar/2d/coarsen.h:34-61."""
     ns, jjf, iif = so.shape
     _, jjc, iic = ci.shape
     soc = np.zeros((5, jjc, iic))
     ifd = 1 if ns == 3 else 0
     _o.ref().BMG2_SymStd_SETUP_ITLI_ex(_p(so), _p(soc), _p(ci), _i(iif), _i(jjf), _i(iic),
-                                       _i(jjc), _i(ifd), _i(ns), _i(0))
+                                       _i(jjc), _i(ifd), _i(ns), _i(jpn))
     return soc
