@@ -28,7 +28,7 @@ class Settings(ctypes.Structure):
     """cdr_solver_settings"""
     _fields_ = [("relaxation", _i), ("nrelax_pre", _i), ("nrelax_post", _i), ("num_levels", _i),
                 ("maxiter", _i), ("tol", _d), ("min_coarse", _i), ("precision", _i), ("graph", _i),
-                ("cycle", _i), ("fused_coarse", _i)]
+                ("cycle", _i), ("fused_coarse", _i), ("periodic", _i)]
 
 
 class Settings3(ctypes.Structure):
@@ -116,6 +116,18 @@ PROTOTYPES = {
                                     ctypes.POINTER(_i), ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "cdr_hip_solver3d_times": (_i, [_vp, ctypes.POINTER(_d), ctypes.POINTER(_d)]),
     "cdr_hip_solver3d_destroy": (None, [_vp]),
+    # periodic boundaries
+    "cdr_hip_relax_gs2d_per_f64": (_i, [_P, _i, _P, _P, _P, _i, _i, _i, _i, _vp]),
+    "cdr_hip_relax_gs2d_per_f32": (_i, [_P, _i, _P, _P, _P, _i, _i, _i, _i, _vp]),
+    "cdr_hip_restrict2d_per_f64": (_i, [_P, _P, _P, _i, _i, _i, _i, _i, _vp]),
+    "cdr_hip_restrict2d_per_f32": (_i, [_P, _P, _P, _i, _i, _i, _i, _i, _vp]),
+    "cdr_hip_interp_add2d_per_f64": (_i, [_P, _P, _P, _P, _i, _P, _i, _i, _i, _i, _i, _vp]),
+    "cdr_hip_interp_add2d_per_f32": (_i, [_P, _P, _P, _P, _i, _P, _i, _i, _i, _i, _i, _vp]),
+    "cdr_hip_setup_interp2d_per_f64": (_i, [_P, _i, _P, _i, _i, _i, _i, _i, _vp]),
+    "cdr_hip_setup_galerkin2d_per_f64": (_i, [_P, _i, _P, _P, _i, _i, _i, _i, _i, _vp]),
+    "cdr_hip_setup_cg2d_per_f64": (_i, [_P, _i, _P, _i, _i, _i, _P, _vp]),
+    "cdr_hip_solve_cg2d_per_f64": (_i, [_P, _P, _P, _P, _i, _i, _i, _vp]),
+    "cdr_hip_solve_cg2d_per_f32": (_i, [_P, _P, _P, _P, _i, _i, _i, _vp]),
     # matvec
     "cdr_hip_matvec2d_f64": (_i, [_P, _i, _P, _P, _i, _i, _vp]),
     "cdr_hip_matvec3d_f64": (_i, [_P, _i, _P, _P, _i, _i, _i, _vp]),

@@ -122,8 +122,17 @@ def setup_recip(so, sor):
     return sor
 
 
-def relax_gs(so, qf, q, sor, updown):
+def bc_from_mask(periodic):
+    """BMG_get_bc (src/2d/ftn/BMG_get_bc.f90) of periodic flags (x, y): 0, 2 (x), 1 (y), 3."""
+    return (0, 2, 1, 3)[int(bool(periodic[0])) | (int(bool(periodic[1])) << 1)]
+
+
+def relax_gs(so, qf, q, sor, updown, jpn=0):
     so, qf, q, sor = map(_t, (so, qf, q, sor))
+    if jpn:
+        _abi.call(f"cdr_hip_relax_gs2d_per_{_sfx(q)}", _p(so), so.shape[0], _p(qf), _p(q),
+                  _p(sor), q.shape[1], q.shape[0], int(updown), int(jpn), _stream())
+        return q
     _abi.call(f"cdr_hip_relax_gs2d_{_sfx(q)}", _p(so), so.shape[0], _p(qf), _p(q), _p(sor),
               q.shape[1], q.shape[0], int(updown), _stream())
     return q
@@ -166,29 +175,44 @@ def autotune_report():
     return json.loads(buf.value.decode())
 
 
-def restrict(ci, q, qc):
+def restrict(ci, q, qc, jpn=0):
+    """qc = P^T q; with a periodic jpn the images in the ghosts of q are refreshed in place."""
     ci, q, qc = map(_t, (ci, q, qc))
+    if jpn:
+        _abi.call(f"cdr_hip_restrict2d_per_{_sfx(q)}", _p(ci), _p(q), _p(qc), q.shape[1],
+                  q.shape[0], qc.shape[1], qc.shape[0], int(jpn), _stream())
+        return qc
     _abi.call(f"cdr_hip_restrict2d_{_sfx(q)}", _p(ci), _p(q), _p(qc), q.shape[1], q.shape[0],
               qc.shape[1], qc.shape[0], _stream())
     return qc
 
 
-def interp_add(q, qc, res, so, ci):
+def interp_add(q, qc, res, so, ci, jpn=0):
     q, qc, res, so, ci = map(_t, (q, qc, res, so, ci))
-    _abi.call(f"cdr_hip_interp_add2d_{_sfx(q)}", _p(q), _p(qc), _p(res), _p(so), so.shape[0],
-              _p(ci), qc.shape[1], qc.shape[0], q.shape[1], q.shape[0], _stream())
+    name = "interp_add2d_per" if jpn else "interp_add2d"
+    extra = (int(jpn),) if jpn else ()
+    _abi.call(f"cdr_hip_{name}_{_sfx(q)}", _p(q), _p(qc), _p(res), _p(so), so.shape[0],
+              _p(ci), qc.shape[1], qc.shape[0], q.shape[1], q.shape[0], *extra, _stream())
     return q
 
 
-def setup_interp(so, ci):
+def setup_interp(so, ci, jpn=0):
     so, ci = _t(so), _t(ci)
+    if jpn:
+        _abi.call("cdr_hip_setup_interp2d_per_f64", _p(so), so.shape[0], _p(ci), so.shape[2],
+                  so.shape[1], ci.shape[2], ci.shape[1], int(jpn), _stream())
+        return ci
     _abi.call("cdr_hip_setup_interp2d_f64", _p(so), so.shape[0], _p(ci), so.shape[2], so.shape[1],
               ci.shape[2], ci.shape[1], _stream())
     return ci
 
 
-def setup_galerkin(so, ci, soc):
+def setup_galerkin(so, ci, soc, jpn=0):
     so, ci, soc = map(_t, (so, ci, soc))
+    if jpn:
+        _abi.call("cdr_hip_setup_galerkin2d_per_f64", _p(so), so.shape[0], _p(ci), _p(soc),
+                  so.shape[2], so.shape[1], ci.shape[2], ci.shape[1], int(jpn), _stream())
+        return soc
     _abi.call("cdr_hip_setup_galerkin2d_f64", _p(so), so.shape[0], _p(ci), _p(soc), so.shape[2],
               so.shape[1], ci.shape[2], ci.shape[1], _stream())
     return soc
@@ -212,6 +236,26 @@ def solve_cg(abd, qf, q):
     return q
 
 
+def setup_cg_periodic(so, jpn):
+    """The factored dense periodic coarse matrix as an (n, n) device tensor (Fortran ABD(n, n))."""
+    so = _t(so)
+    ii, jj = so.shape[2], so.shape[1]
+    n = (ii - 2) * (jj - 2)
+    abd = torch.zeros((n, n), dtype=torch.float64, device=so.device)
+    info = torch.zeros(1, dtype=torch.int32, device=so.device)
+    _abi.call("cdr_hip_setup_cg2d_per_f64", _p(so), so.shape[0], _p(abd), ii, jj, int(jpn),
+              _p(info), _stream())
+    return abd
+
+
+def solve_cg_periodic(abd, qf, q, jpn):
+    qf, q = _t(qf), _t(q)
+    bbd = torch.zeros(abd.shape[0], dtype=torch.float64, device=q.device)
+    _abi.call(f"cdr_hip_solve_cg2d_per_{_sfx(q)}", _p(q), _p(qf), _p(abd), _p(bbd), q.shape[1],
+              q.shape[0], int(jpn), _stream())
+    return q
+
+
 # ---------------------------------------------------------------------------------------------
 # solver
 # ---------------------------------------------------------------------------------------------
@@ -222,7 +266,9 @@ class Solver:
     relaxation ('point' | 'line-x' | 'line-y' | 'line-xy'), nrelax_pre=2, nrelax_post=1,
     num_levels=-1, maxiter=10, tol=1e-8, min_coarse=3; precision 'f64' (Cedar's algorithm) or
     'mixed' (FP32 V-cycle preconditioning FP64 iterative refinement); graph=True replays each
-    V-cycle from a captured hipGraph; cycle 'v' or 'f' (solver.cycle.type).
+    V-cycle from a captured hipGraph; cycle 'v' or 'f' (solver.cycle.type); periodic = (x, y)
+    flags of grid.periodic (point relaxation; the operator, b and x hold the periodic images in
+    their ghosts, as in examples/basic-2d-ser/periodic.cc).
 
     A 4-D operator (nb, nstncl, ny+2, nx+2) makes a batched solver: nb independent problems of
     the same extents solved together (every kernel launch covers the batch); solve() then takes
@@ -232,7 +278,7 @@ class Solver:
 
     def __init__(self, so, relaxation="point", nrelax_pre=2, nrelax_post=1, num_levels=-1,
                  maxiter=10, tol=1e-8, min_coarse=3, precision="f64", graph=True, cycle="v",
-                 fused_coarse=True):
+                 fused_coarse=True, periodic=(False, False)):
         so = _t(so) if not isinstance(so, np.ndarray) else torch.as_tensor(so).cuda()
         if so.dtype != torch.float64:
             so = so.to(torch.float64)
@@ -244,7 +290,8 @@ class Solver:
         ns, jj, ii = self.so.shape[-3:]
         s = _abi.Settings(_abi.RELAX[relaxation], nrelax_pre, nrelax_post, num_levels, maxiter,
                           tol, min_coarse, _abi.PRECISION[precision], int(bool(graph)),
-                          _abi.CYCLE[cycle], int(bool(fused_coarse)))
+                          _abi.CYCLE[cycle], int(bool(fused_coarse)),
+                          int(bool(periodic[0])) | (int(bool(periodic[1])) << 1))
         h = ctypes.c_void_p()
         _abi.call("cdr_hip_solver2d_create_batched", _p(self.so), ns, ii - 2, jj - 2, nb,
                   ctypes.byref(s), _stream(), ctypes.byref(h))

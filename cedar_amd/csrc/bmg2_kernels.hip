@@ -2122,6 +2122,686 @@ void add_correction(double *x, const float *e, int ii, int jj, hipStream_t s, in
     add_correction_k<<<gridb(ii - 2, jj - 2, nb), 256, 0, s>>>(x, e, ii, jj);
 }
 
+// ------------------------------------------------------------------------------------------
+// Periodic boundaries (the JPN != 0 branches of the reference's 2D kernels).  jpn is the BoxMG
+// boundary id of BMG_get_bc (src/2d/ftn/BMG_get_bc.f90): 2 periodic in x, 1 in y, 3 in both.
+// The periodic images live in the ghost rows and columns: Q(1,J) = Q(II-1,J), Q(II,J) = Q(2,J)
+// and Q(I,1) = Q(I,JJ-1), Q(I,JJ) = Q(I,2) in the reference's 1-based indices.
+// ------------------------------------------------------------------------------------------
+namespace {
+
+__host__ __device__ inline bool per_x(int jpn) { return jpn == 2 || jpn == 3 || jpn == -2 || jpn == -3; }
+__host__ __device__ inline bool per_y(int jpn) { return jpn == 1 || jpn == 3 || jpn == -1 || jpn == -3; }
+
+// x images of rows j0, j0 + jstep, ... <= j1 (0-based rows)
+template <typename T>
+__global__ __launch_bounds__(256) void ghost_x_k(T *__restrict__ q, int ii, int jj, int j0, int j1,
+                                                 int jstep) {
+    const int j = j0 + (int)(blockIdx.x * 256 + threadIdx.x) * jstep;
+    if (j > j1) return;
+    q += (std::size_t)blockIdx.z * ii * jj;
+    const std::size_t r = (std::size_t)j * ii;
+    q[r] = q[r + ii - 2];
+    q[r + ii - 1] = q[r + 1];
+}
+
+// y images of columns i0 .. i1
+template <typename T>
+__global__ __launch_bounds__(256) void ghost_y_k(T *__restrict__ q, int ii, int jj, int i0, int i1) {
+    const int i = i0 + (int)(blockIdx.x * 256 + threadIdx.x);
+    if (i > i1) return;
+    q += (std::size_t)blockIdx.z * ii * jj;
+    q[i] = q[(std::size_t)(jj - 2) * ii + i];
+    q[(std::size_t)(jj - 1) * ii + i] = q[(std::size_t)ii + i];
+}
+
+template <typename T>
+void ghost_x(T *q, int ii, int jj, int j0, int j1, int jstep, hipStream_t s, int nb) {
+    if (j1 < j0) return;
+    const int n = (j1 - j0) / jstep + 1;
+    ghost_x_k<T><<<dim3((unsigned)((n + 255) / 256), 1, (unsigned)nb), 256, 0, s>>>(q, ii, jj, j0,
+                                                                                   j1, jstep);
+}
+template <typename T>
+void ghost_y(T *q, int ii, int jj, int i0, int i1, hipStream_t s, int nb) {
+    if (i1 < i0) return;
+    const int n = i1 - i0 + 1;
+    ghost_y_k<T><<<dim3((unsigned)((n + 255) / 256), 1, (unsigned)nb), 256, 0, s>>>(q, ii, jj, i0,
+                                                                                   i1);
+}
+
+// ---- operator-induced interpolation, periodic branch -------------------------------------
+// (src/2d/ftn/BMG2_SymStd_SETUP_interp_OI.f90:259-616).  Loop bounds of the reference; the fine
+// index of coarse index c follows I = MAX(MOD(I+2, N), MIN(I+2, 3)) from I = beg, which is
+// 2(c-1) except at the periodic end, where it wraps to 3.
+struct oi_bounds {
+    int ibeg, ibegc, iendc, ibeg_x, iend_x, iifc;
+    int jbeg, jbegc, jendc, jbeg_y, jend_y, jjfc;
+};
+
+__device__ inline int oi_fine(int c, int cbeg, int beg, int n) {
+    const int prev = c == cbeg ? beg : 2 * (c - 2);
+    const int a = (prev + 2) % n, b = prev + 2 < 3 ? prev + 2 : 3;
+    return a > b ? a : b;
+}
+
+#define SOF(i, j, k) so[(std::size_t)((k) - 1) * P + (std::size_t)((j) - 1) * iif + ((i) - 1)]
+#define CIF(i, j, k) ci[(std::size_t)((k) - 1) * Pc + (std::size_t)((j) - 1) * iic + ((i) - 1)]
+
+__device__ inline double oi_lumped(double c, double sum, double ep) {
+    const double emach = 0x1.0p-53;  // D1MACH(3)
+    return (c - sum) * fmax(c - (1.0 + ep) * sum, 0.0) / (fabs(c - (1.0 + ep) * sum) + emach);
+}
+
+template <bool NINE>
+__device__ void oi_xedge(const double *__restrict__ so, double *__restrict__ ci, std::size_t P,
+                         std::size_t Pc, int iif, int iic, int i, int j, int ic, int jc) {
+    constexpr int KO = 1, KW = 2, KS = 3, KSW = 4, KNW = 5, LL = 1, LR = 2;
+    double a, b;
+    if (NINE) {
+        a = SOF(i, j, KW) + SOF(i, j, KNW) + SOF(i, j + 1, KSW);
+        b = SOF(i - 1, j, KW) + SOF(i - 1, j, KSW) + SOF(i - 1, j + 1, KNW);
+    } else {
+        a = SOF(i, j, KW);
+        b = SOF(i - 1, j, KW);
+    }
+    const double c = SOF(i - 1, j, KO);
+    const double ep = fmin(fabs(a / c), fabs(b / c));
+    double sum = a + b + SOF(i - 1, j, KS) + SOF(i - 1, j + 1, KS);
+    sum = a + b + oi_lumped(c, sum, ep);
+    sum = 1.0 / sum;
+    CIF(ic, jc, LR) = a * sum;
+    CIF(ic, jc, LL) = b * sum;
+}
+
+template <bool NINE>
+__device__ void oi_yedge(const double *__restrict__ so, double *__restrict__ ci, std::size_t P,
+                         std::size_t Pc, int iif, int iic, int i, int j, int ic, int jc) {
+    constexpr int KO = 1, KW = 2, KS = 3, KSW = 4, KNW = 5, LA = 3, LB = 4;
+    double a, b;
+    if (NINE) {
+        a = SOF(i, j, KS) + SOF(i, j, KNW) + SOF(i + 1, j, KSW);
+        b = SOF(i, j - 1, KS) + SOF(i, j - 1, KSW) + SOF(i + 1, j - 1, KNW);
+    } else {
+        a = SOF(i, j, KS);
+        b = SOF(i, j - 1, KS);
+    }
+    const double c = SOF(i, j - 1, KO);
+    double sum = a + b + SOF(i, j - 1, KW) + SOF(i + 1, j - 1, KW);
+    const double ep = fmin(fabs(a / c), fabs(b / c));
+    sum = a + b + oi_lumped(c, sum, ep);
+    sum = 1.0 / sum;
+    CIF(ic, jc, LA) = a * sum;
+    CIF(ic, jc, LB) = b * sum;
+}
+
+template <bool NINE>
+__device__ void oi_corner(const double *__restrict__ so, double *__restrict__ ci, std::size_t P,
+                          std::size_t Pc, int iif, int iic, int i, int j, int ic, int jc) {
+    constexpr int KO = 1, KW = 2, KS = 3, KSW = 4, KNW = 5;
+    constexpr int LL = 1, LR = 2, LA = 3, LB = 4, LSW = 5, LNW = 6, LNE = 7, LSE = 8;
+    const double c = SOF(i - 1, j - 1, KO);
+    double sum, ep;
+    if (NINE) {
+        sum = SOF(i - 1, j - 1, KW) + SOF(i - 1, j, KNW) + SOF(i - 1, j, KS) + SOF(i, j, KSW) +
+              SOF(i, j - 1, KW) + SOF(i, j - 1, KNW) + SOF(i - 1, j - 1, KS) + SOF(i - 1, j - 1, KSW);
+        const double e1 = fabs((SOF(i - 1, j - 1, KSW) + SOF(i - 1, j - 1, KW) + SOF(i - 1, j, KNW)) / c);
+        const double e2 = fabs((SOF(i - 1, j, KNW) + SOF(i - 1, j, KS) + SOF(i, j, KSW)) / c);
+        const double e3 = fabs((SOF(i, j, KSW) + SOF(i, j - 1, KW) + SOF(i, j - 1, KNW)) / c);
+        const double e4 = fabs((SOF(i, j - 1, KNW) + SOF(i - 1, j - 1, KS) + SOF(i - 1, j - 1, KSW)) / c);
+        ep = fmin(fmin(fmin(e1, e2), e3), e4);
+    } else {
+        sum = SOF(i - 1, j - 1, KW) + SOF(i - 1, j, KS) + SOF(i, j - 1, KW) + SOF(i - 1, j - 1, KS);
+        const double e1 = fabs(SOF(i - 1, j - 1, KW) / c);
+        const double e2 = fabs(SOF(i - 1, j, KS) / c);
+        const double e3 = fabs(SOF(i, j - 1, KW) / c);
+        const double e4 = fabs(SOF(i - 1, j - 1, KS) / c);
+        ep = fmin(fmin(fmin(e1, e2), e3), e4);
+    }
+    sum = sum + oi_lumped(c, sum, ep);
+    const double s = 1.0 / sum;
+    if (NINE) {
+        CIF(ic, jc, LSW) = (SOF(i - 1, j - 1, KS) * CIF(ic, jc - 1, LL) +
+                            SOF(i - 1, j - 1, KW) * CIF(ic - 1, jc, LB) + SOF(i - 1, j - 1, KSW)) * s;
+        CIF(ic, jc, LSE) = (SOF(i - 1, j - 1, KS) * CIF(ic, jc - 1, LR) +
+                            SOF(i, j - 1, KW) * CIF(ic, jc, LB) + SOF(i, j - 1, KNW)) * s;
+        CIF(ic, jc, LNW) = (SOF(i - 1, j - 1, KW) * CIF(ic - 1, jc, LA) +
+                            SOF(i - 1, j, KS) * CIF(ic, jc, LL) + SOF(i - 1, j, KNW)) * s;
+        CIF(ic, jc, LNE) = (SOF(i - 1, j, KS) * CIF(ic, jc, LR) +
+                            SOF(i, j - 1, KW) * CIF(ic, jc, LA) + SOF(i, j, KSW)) * s;
+    } else {
+        CIF(ic, jc, LSW) = (SOF(i - 1, j - 1, KS) * CIF(ic, jc - 1, LL) +
+                            SOF(i - 1, j - 1, KW) * CIF(ic - 1, jc, LB)) * s;
+        CIF(ic, jc, LSE) = (SOF(i - 1, j - 1, KS) * CIF(ic, jc - 1, LR) +
+                            SOF(i, j - 1, KW) * CIF(ic, jc, LB)) * s;
+        CIF(ic, jc, LNW) = (SOF(i - 1, j - 1, KW) * CIF(ic - 1, jc, LA) +
+                            SOF(i - 1, j, KS) * CIF(ic, jc, LL)) * s;
+        CIF(ic, jc, LNE) = (SOF(i - 1, j, KS) * CIF(ic, jc, LR) +
+                            SOF(i, j - 1, KW) * CIF(ic, jc, LA)) * s;
+    }
+}
+#undef SOF
+#undef CIF
+
+// x-edge weights: rows jc = 2..jjc-1 (blockIdx.y rows of 4), and when periodic in y the two
+// ghost rows jjc (from fine row jbeg_y) and 1 (from fine row jend_y) as rows jjc-1+1, jjc-1+2
+template <bool NINE>
+__global__ __launch_bounds__(256) void oi_per_xedge_k(const double *__restrict__ so,
+                                                      double *__restrict__ ci, int iif, int jjf,
+                                                      int iic, int jjc, oi_bounds B, int extra) {
+    const int ic = B.ibegc + (int)(blockIdx.x * 64 + (threadIdx.x % 64));
+    const int r = (int)(blockIdx.y * 4 + (threadIdx.x / 64));  // 0-based row item
+    const int nrow = jjc - 2 + extra;
+    if (ic > B.iendc || r >= nrow) return;
+    const std::size_t P = (std::size_t)iif * jjf, Pc = (std::size_t)iic * jjc, bz = blockIdx.z;
+    so += bz * (NINE ? 5 : 3) * P;
+    ci += bz * 8 * Pc;
+    const int i = oi_fine(ic, B.ibegc, B.ibeg, B.iifc);
+    int jc, j;
+    if (r < jjc - 2) {
+        jc = r + 2;
+        j = 2 * (jc - 1);
+    } else if (r == jjc - 2) {
+        jc = jjc;
+        j = B.jbeg_y;
+    } else {
+        jc = 1;
+        j = B.jend_y;
+    }
+    oi_xedge<NINE>(so, ci, P, Pc, iif, iic, i, j, ic, jc);
+}
+
+// y-edge weights: columns ic = 2..iic-1, and when periodic in x the ghost columns iic (from fine
+// column ibeg_x) and 1 (from iend_x); rows jc = jbegc..jendc
+template <bool NINE>
+__global__ __launch_bounds__(256) void oi_per_yedge_k(const double *__restrict__ so,
+                                                      double *__restrict__ ci, int iif, int jjf,
+                                                      int iic, int jjc, oi_bounds B, int extra) {
+    const int c = (int)(blockIdx.x * 64 + (threadIdx.x % 64));  // 0-based column item
+    const int jc = B.jbegc + (int)(blockIdx.y * 4 + (threadIdx.x / 64));
+    const int ncol = iic - 2 + extra;
+    if (c >= ncol || jc > B.jendc) return;
+    const std::size_t P = (std::size_t)iif * jjf, Pc = (std::size_t)iic * jjc, bz = blockIdx.z;
+    so += bz * (NINE ? 5 : 3) * P;
+    ci += bz * 8 * Pc;
+    const int j = oi_fine(jc, B.jbegc, B.jbeg, B.jjfc);
+    int ic, i;
+    if (c < iic - 2) {
+        ic = c + 2;
+        i = 2 * (ic - 1);
+    } else if (c == iic - 2) {
+        ic = iic;
+        i = B.ibeg_x;
+    } else {
+        ic = 1;
+        i = B.iend_x;
+    }
+    oi_yedge<NINE>(so, ci, P, Pc, iif, iic, i, j, ic, jc);
+}
+
+template <bool NINE>
+__global__ __launch_bounds__(256) void oi_per_corner_k(const double *__restrict__ so,
+                                                       double *__restrict__ ci, int iif, int jjf,
+                                                       int iic, int jjc, oi_bounds B) {
+    const int ic = B.ibegc + (int)(blockIdx.x * 64 + (threadIdx.x % 64));
+    const int jc = B.jbegc + (int)(blockIdx.y * 4 + (threadIdx.x / 64));
+    if (ic > B.iendc || jc > B.jendc) return;
+    const std::size_t P = (std::size_t)iif * jjf, Pc = (std::size_t)iic * jjc, bz = blockIdx.z;
+    so += bz * (NINE ? 5 : 3) * P;
+    ci += bz * 8 * Pc;
+    const int i = oi_fine(ic, B.ibegc, B.ibeg, B.iifc);
+    const int j = oi_fine(jc, B.jbegc, B.jbeg, B.jjfc);
+    oi_corner<NINE>(so, ci, P, Pc, iif, iic, i, j, ic, jc);
+}
+
+// Galerkin coarse operator in the reference's hand-expanded form (the ITLI products of
+// src/2d/ftn/BMG2_SymStd_SETUP_ITLI_ex.f90:94-331), one thread per coarse point.  With periodic
+// boundaries these formulas read the periodic images in the ghost rows and columns of SO and CI,
+// which the generic P^T A P of galerkin_k does not visit; Dirichlet problems keep galerkin_k.
+template <bool NINE>
+__global__ __launch_bounds__(256) void galerkin_itli_k(const double *__restrict__ so,
+                                                       const double *__restrict__ ci,
+                                                       double *__restrict__ soc, int iif, int jjf,
+                                                       int iic, int jjc) {
+    const int ic = blockIdx.x * 64 + (threadIdx.x % 64) + 2;  // Fortran indices
+    const int jc = blockIdx.y * 4 + (threadIdx.x / 64) + 2;
+    if (ic > iic - 1 || jc > jjc - 1) return;
+    const std::size_t P = (std::size_t)iif * jjf, Pc = (std::size_t)iic * jjc, bz = blockIdx.z;
+    so += bz * (NINE ? 5 : 3) * P;
+    ci += bz * 8 * Pc;
+    soc += bz * 5 * Pc;
+#define SO(i, j, k) so[(std::size_t)((k) - 1) * P + (std::size_t)((j) - 1) * iif + ((i) - 1)]
+#define CI(i, j, k) ci[(std::size_t)((k) - 1) * Pc + (std::size_t)((j) - 1) * iic + ((i) - 1)]
+#define SOC(i, j, k) soc[(std::size_t)((k) - 1) * Pc + (std::size_t)((j) - 1) * iic + ((i) - 1)]
+    constexpr int KO = 1, KW = 2, KS = 3, KSW = 4, KNW = 5;
+    constexpr int LL = 1, LR = 2, LA = 3, LB = 4, LSW = 5, LNW = 6, LNE = 7, LSE = 8;
+    const int j = 2 * jc - 2, i = 2 * ic - 2;
+    if (NINE) {
+        double CO = SO(i, j + 1, KNW) * CI(ic, jc + 1, LSW) + SO(i, j, KW) * CI(ic, jc, LL)
+                  + SO(i, j, KSW) * CI(ic, jc, LNW);
+        double CS = SO(i, j - 1, KW) * CI(ic, jc, LNW) + SO(i, j, KNW) * CI(ic, jc, LL);
+        double CSW = -SO(i - 1, j - 1, KO) * CI(ic, jc, LNW) + SO(i - 1, j - 1, KW) * CI(ic - 1, jc, LA)
+                   + SO(i - 1, j, KNW) + SO(i - 1, j, KS) * CI(ic, jc, LL);
+        double CW = -SO(i - 1, j, KO) * CI(ic, jc, LL) + SO(i - 1, j, KS) * CI(ic, jc, LNW)
+                  + SO(i - 1, j, KSW) * CI(ic - 1, jc, LA) + SO(i - 1, j, KW)
+                  + SO(i - 1, j + 1, KNW) * CI(ic - 1, jc + 1, LB)
+                  + SO(i - 1, j + 1, KS) * CI(ic, jc + 1, LSW);
+        double CNW = -SO(i - 1, j + 1, KO) * CI(ic, jc + 1, LSW) + SO(i - 1, j + 1, KS) * CI(ic, jc, LL)
+                   + SO(i - 1, j + 1, KSW) + SO(i - 1, j + 1, KW) * CI(ic - 1, jc + 1, LB);
+        double CN = SO(i, j + 1, KSW) * CI(ic, jc, LL) + SO(i, j + 1, KW) * CI(ic, jc + 1, LSW);
+        SOC(ic, jc, KW) = CO + CI(ic, jc, LA) * CS + CI(ic, jc, LNE) * CSW + CI(ic, jc, LR) * CW
+                        + CI(ic, jc + 1, LSE) * CNW + CI(ic, jc + 1, LB) * CN;
+        double COSW = SO(i, j, KSW) * CI(ic, jc, LSW);
+        double CSSW = SO(i, j - 1, KSW) * CI(ic, jc - 1, LL) + SO(i, j - 1, KW) * CI(ic, jc, LSW);
+        double CSWSW = -SO(i - 1, j - 1, KO) * CI(ic, jc, LSW) + SO(i - 1, j - 1, KS) * CI(ic, jc - 1, LL)
+                     + SO(i - 1, j - 1, KSW) + SO(i - 1, j - 1, KW) * CI(ic - 1, jc, LB);
+        double CWSW = SO(i - 1, j, KS) * CI(ic, jc, LSW) + SO(i - 1, j, KSW) * CI(ic - 1, jc, LB);
+        SOC(ic, jc, KSW) = COSW + CI(ic, jc, LA) * CSSW + CI(ic, jc, LNE) * CSWSW
+                         + CI(ic, jc, LR) * CWSW;
+        double COA = SO(i, j, KSW) * CI(ic, jc, LSE) + SO(i, j, KS) * CI(ic, jc, LB)
+                   + SO(i + 1, j, KNW) * CI(ic + 1, jc, LSW);
+        double CEA = SO(i + 1, j, KS) * CI(ic + 1, jc, LSW) + SO(i + 1, j, KSW) * CI(ic, jc, LB);
+        double CSEA = -SO(i + 1, j - 1, KO) * CI(ic + 1, jc, LSW) + SO(i + 1, j - 1, KS) * CI(ic + 1, jc - 1, LL)
+                    + SO(i + 1, j - 1, KSW) + SO(i + 1, j - 1, KW) * CI(ic, jc, LB);
+        double CSA = -SO(i, j - 1, KO) * CI(ic, jc, LB) + SO(i + 1, j - 1, KW) * CI(ic + 1, jc, LSW)
+                   + SO(i + 1, j - 1, KNW) * CI(ic + 1, jc - 1, LL) + SO(i, j - 1, KS)
+                   + SO(i, j - 1, KSW) * CI(ic, jc - 1, LR) + SO(i, j - 1, KW) * CI(ic, jc, LSE);
+        double CSWA = -SO(i - 1, j - 1, KO) * CI(ic, jc, LSE) + SO(i, j - 1, KW) * CI(ic, jc, LB)
+                    + SO(i, j - 1, KNW) + SO(i - 1, j - 1, KS) * CI(ic, jc - 1, LR);
+        double CWA = SO(i, j, KNW) * CI(ic, jc, LB) + SO(i - 1, j, KS) * CI(ic, jc, LSE);
+        SOC(ic, jc, KS) = COA + CI(ic + 1, jc, LL) * CEA + CI(ic + 1, jc, LNW) * CSEA
+                        + CI(ic, jc, LA) * CSA + CI(ic, jc, LNE) * CSWA + CI(ic, jc, LR) * CWA;
+        double CONW = SO(i - 1, j, KNW) * CI(ic, jc, LSE);
+        double CENW = SO(i, j, KNW) * CI(ic, jc, LB) + SO(i - 1, j, KS) * CI(ic, jc, LSE);
+        double CSENW = -SO(i - 1, j - 1, KO) * CI(ic, jc, LSE) + SO(i, j - 1, KW) * CI(ic, jc, LB)
+                     + SO(i, j - 1, KNW) + SO(i - 1, j - 1, KS) * CI(ic, jc - 1, LR);
+        double CFNW = SO(i - 1, j - 1, KW) * CI(ic, jc, LSE) + SO(i - 1, j - 1, KNW) * CI(ic, jc - 1, LR);
+        SOC(ic, jc, KNW) = CONW + CI(ic, jc, LL) * CENW + CI(ic, jc, LNW) * CSENW
+                         + CI(ic - 1, jc, LA) * CFNW;
+        {
+            double CO = SO(i, j, KW) * CI(ic, jc, LR) + SO(i, j + 1, KNW) * CI(ic, jc + 1, LSE)
+                      + SO(i, j + 1, KS) * CI(ic, jc + 1, LB) + SO(i + 1, j + 1, KSW) * CI(ic + 1, jc + 1, LSW)
+                      + SO(i + 1, j, KW) * CI(ic + 1, jc, LL) + SO(i + 1, j, KNW) * CI(ic + 1, jc, LNW)
+                      + SO(i, j, KS) * CI(ic, jc, LA) + SO(i, j, KSW) * CI(ic, jc, LNE) - SO(i, j, KO);
+            double CW = -SO(i - 1, j, KO) * CI(ic, jc, LR) + SO(i - 1, j + 1, KS) * CI(ic, jc + 1, LSE)
+                      + SO(i, j + 1, KSW) * CI(ic, jc + 1, LB) + SO(i, j, KW) + SO(i, j, KNW) * CI(ic, jc, LA)
+                      + SO(i - 1, j, KS) * CI(ic, jc, LNE);
+            double CNW = -SO(i - 1, j + 1, KO) * CI(ic, jc + 1, LSE) + SO(i, j + 1, KW) * CI(ic, jc + 1, LB)
+                       + SO(i, j + 1, KNW) + SO(i - 1, j + 1, KS) * CI(ic, jc, LR);
+            double CN = -SO(i, j + 1, KO) * CI(ic, jc + 1, LB) + SO(i + 1, j + 1, KNW) * CI(ic + 1, jc, LL)
+                      + SO(i, j + 1, KS) + SO(i, j + 1, KSW) * CI(ic, jc, LR)
+                      + SO(i, j + 1, KW) * CI(ic, jc + 1, LSE) + SO(i + 1, j + 1, KW) * CI(ic + 1, jc + 1, LSW);
+            double CNE = -SO(i + 1, j + 1, KO) * CI(ic + 1, jc + 1, LSW) + SO(i + 1, j + 1, KS) * CI(ic + 1, jc, LL)
+                       + SO(i + 1, j + 1, KSW) + SO(i + 1, j + 1, KW) * CI(ic, jc + 1, LB);
+            double CE = -SO(i + 1, j, KO) * CI(ic + 1, jc, LL) + SO(i + 1, j, KS) * CI(ic + 1, jc, LNW)
+                      + SO(i + 1, j, KSW) * CI(ic, jc, LA) + SO(i + 1, j, KW)
+                      + SO(i + 1, j + 1, KNW) * CI(ic, jc + 1, LB) + SO(i + 1, j + 1, KS) * CI(ic + 1, jc + 1, LSW);
+            double CSE = -SO(i + 1, j - 1, KO) * CI(ic + 1, jc, LNW) + SO(i + 1, j - 1, KW) * CI(ic, jc, LA)
+                       + SO(i + 1, j, KNW) + SO(i + 1, j, KS) * CI(ic + 1, jc, LL);
+            double CS = -SO(i, j - 1, KO) * CI(ic, jc, LA) + SO(i, j - 1, KW) * CI(ic, jc, LNE)
+                      + SO(i, j, KNW) * CI(ic, jc, LR) + SO(i, j, KS) + SO(i + 1, j, KSW) * CI(ic + 1, jc, LL)
+                      + SO(i + 1, j - 1, KW) * CI(ic + 1, jc, LNW);
+            double CSW = -SO(i - 1, j - 1, KO) * CI(ic, jc, LNE) + SO(i - 1, j, KS) * CI(ic, jc, LR)
+                       + SO(i, j, KSW) + SO(i, j - 1, KW) * CI(ic, jc, LA);
+            SOC(ic, jc, KO) = -CI(ic, jc + 1, LSE) * CNW - CI(ic, jc + 1, LB) * CN
+                            - CI(ic + 1, jc + 1, LSW) * CNE - CI(ic, jc, LR) * CW - CO
+                            - CI(ic + 1, jc, LL) * CE - CI(ic, jc, LNE) * CSW
+                            - CI(ic, jc, LA) * CS - CI(ic + 1, jc, LNW) * CSE;
+        }
+    } else {
+        double CO = SO(i, j, KW) * CI(ic, jc, LL);
+        double CS = SO(i, j - 1, KW) * CI(ic, jc, LNW);
+        double CSW = -SO(i - 1, j - 1, KO) * CI(ic, jc, LNW) + SO(i - 1, j - 1, KW) * CI(ic - 1, jc, LA)
+                   + SO(i - 1, j, KS) * CI(ic, jc, LL);
+        double CW = -SO(i - 1, j, KO) * CI(ic, jc, LL) + SO(i - 1, j, KS) * CI(ic, jc, LNW)
+                  + SO(i - 1, j, KW) + SO(i - 1, j + 1, KS) * CI(ic, jc + 1, LSW);
+        double CNW = -SO(i - 1, j + 1, KO) * CI(ic, jc + 1, LSW) + SO(i - 1, j + 1, KS) * CI(ic, jc, LL)
+                   + SO(i - 1, j + 1, KW) * CI(ic - 1, jc + 1, LB);
+        double CN = SO(i, j + 1, KW) * CI(ic, jc + 1, LSW);
+        SOC(ic, jc, KW) = CO + CI(ic, jc, LA) * CS + CI(ic, jc, LNE) * CSW + CI(ic, jc, LR) * CW
+                        + CI(ic, jc + 1, LSE) * CNW + CI(ic, jc + 1, LB) * CN;
+        double CSSW = SO(i, j - 1, KW) * CI(ic, jc, LSW);
+        double CSWSW = -SO(i - 1, j - 1, KO) * CI(ic, jc, LSW) + SO(i - 1, j - 1, KS) * CI(ic, jc - 1, LL)
+                     + SO(i - 1, j - 1, KW) * CI(ic - 1, jc, LB);
+        double CWSW = SO(i - 1, j, KS) * CI(ic, jc, LSW);
+        SOC(ic, jc, KSW) = CI(ic, jc, LA) * CSSW + CI(ic, jc, LNE) * CSWSW + CI(ic, jc, LR) * CWSW;
+        double COA = SO(i, j, KS) * CI(ic, jc, LB);
+        double CEA = SO(i + 1, j, KS) * CI(ic + 1, jc, LSW);
+        double CSEA = -SO(i + 1, j - 1, KO) * CI(ic + 1, jc, LSW) + SO(i + 1, j - 1, KS) * CI(ic + 1, jc - 1, LL)
+                    + SO(i + 1, j - 1, KW) * CI(ic, jc, LB);
+        double CSA = -SO(i, j - 1, KO) * CI(ic, jc, LB) + SO(i + 1, j - 1, KW) * CI(ic + 1, jc, LSW)
+                   + SO(i, j - 1, KS) + SO(i, j - 1, KW) * CI(ic, jc, LSE);
+        double CSWA = -SO(i - 1, j - 1, KO) * CI(ic, jc, LSE) + SO(i, j - 1, KW) * CI(ic, jc, LB)
+                    + SO(i - 1, j - 1, KS) * CI(ic, jc - 1, LR);
+        double CWA = SO(i - 1, j, KS) * CI(ic, jc, LSE);
+        SOC(ic, jc, KS) = COA + CI(ic + 1, jc, LL) * CEA + CI(ic + 1, jc, LNW) * CSEA
+                        + CI(ic, jc, LA) * CSA + CI(ic, jc, LNE) * CSWA + CI(ic, jc, LR) * CWA;
+        double CENW = SO(i - 1, j, KS) * CI(ic, jc, LSE);
+        double CSENW = -SO(i - 1, j - 1, KO) * CI(ic, jc, LSE) + SO(i, j - 1, KW) * CI(ic, jc, LB)
+                     + SO(i - 1, j - 1, KS) * CI(ic, jc - 1, LR);
+        double CSNW = SO(i - 1, j - 1, KW) * CI(ic, jc, LSE);
+        SOC(ic, jc, KNW) = CI(ic, jc, LL) * CENW + CI(ic, jc, LNW) * CSENW + CI(ic - 1, jc, LA) * CSNW;
+        CO = SO(i, j, KW) * CI(ic, jc, LR) + SO(i, j + 1, KS) * CI(ic, jc + 1, LB)
+           + SO(i + 1, j, KW) * CI(ic + 1, jc, LL) + SO(i, j, KS) * CI(ic, jc, LA) - SO(i, j, KO);
+        CW = -SO(i - 1, j, KO) * CI(ic, jc, LR) + SO(i - 1, j + 1, KS) * CI(ic, jc + 1, LSE)
+           + SO(i, j, KW) + SO(i - 1, j, KS) * CI(ic, jc, LNE);
+        CNW = -SO(i - 1, j + 1, KO) * CI(ic, jc + 1, LSE) + SO(i, j + 1, KW) * CI(ic, jc + 1, LB)
+            + SO(i - 1, j + 1, KS) * CI(ic, jc, LR);
+        CN = -SO(i, j + 1, KO) * CI(ic, jc + 1, LB) + SO(i, j + 1, KS)
+           + SO(i, j + 1, KW) * CI(ic, jc + 1, LSE) + SO(i + 1, j + 1, KW) * CI(ic + 1, jc + 1, LSW);
+        double CNE = -SO(i + 1, j + 1, KO) * CI(ic + 1, jc + 1, LSW) + SO(i + 1, j + 1, KS) * CI(ic + 1, jc, LL)
+                   + SO(i + 1, j + 1, KW) * CI(ic, jc + 1, LB);
+        double CE = -SO(i + 1, j, KO) * CI(ic + 1, jc, LL) + SO(i + 1, j, KS) * CI(ic + 1, jc, LNW)
+                  + SO(i + 1, j, KW) + SO(i + 1, j + 1, KS) * CI(ic + 1, jc + 1, LSW);
+        double CSE = -SO(i + 1, j - 1, KO) * CI(ic + 1, jc, LNW) + SO(i + 1, j - 1, KW) * CI(ic, jc, LA)
+                   + SO(i + 1, j, KS) * CI(ic + 1, jc, LL);
+        CS = -SO(i, j - 1, KO) * CI(ic, jc, LA) + SO(i, j - 1, KW) * CI(ic, jc, LNE) + SO(i, j, KS)
+           + SO(i + 1, j - 1, KW) * CI(ic + 1, jc, LNW);
+        CSW = -SO(i - 1, j - 1, KO) * CI(ic, jc, LNE) + SO(i - 1, j, KS) * CI(ic, jc, LR)
+            + SO(i, j - 1, KW) * CI(ic, jc, LA);
+        SOC(ic, jc, KO) = -CI(ic, jc + 1, LSE) * CNW - CI(ic, jc + 1, LB) * CN
+                        - CI(ic + 1, jc + 1, LSW) * CNE - CI(ic, jc, LR) * CW - CO
+                        - CI(ic + 1, jc, LL) * CE - CI(ic, jc, LNE) * CSW
+                        - CI(ic, jc, LA) * CS - CI(ic + 1, jc, LNW) * CSE;
+    }
+#undef SO
+#undef CI
+#undef SOC
+}
+
+
+// periodic images of the coarse operator (all 5 planes): rows first, then columns
+// (src/2d/ftn/BMG2_SymStd_SETUP_ITLI_ex.f90:219-246)
+__global__ __launch_bounds__(256) void soc_ghost_y_k(double *__restrict__ soc, int iic, int jjc) {
+    const int i = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (i >= iic) return;
+    const std::size_t Pc = (std::size_t)iic * jjc;
+    soc += (std::size_t)blockIdx.z * 5 * Pc;
+    for (int k = 0; k < 5; ++k) {
+        double *s = soc + k * Pc;
+        s[(std::size_t)(jjc - 1) * iic + i] = s[(std::size_t)iic + i];
+        s[i] = s[(std::size_t)(jjc - 2) * iic + i];
+    }
+}
+__global__ __launch_bounds__(256) void soc_ghost_x_k(double *__restrict__ soc, int iic, int jjc) {
+    const int j = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (j >= jjc) return;
+    const std::size_t Pc = (std::size_t)iic * jjc;
+    soc += (std::size_t)blockIdx.z * 5 * Pc;
+    for (int k = 0; k < 5; ++k) {
+        double *s = soc + k * Pc + (std::size_t)j * iic;
+        s[iic - 1] = s[1];
+        s[0] = s[iic - 2];
+    }
+}
+
+// ---- coarsest grid, periodic: dense matrix, Cholesky, solve with mean removal ----------------
+// Assembly of the upper triangle as src/2d/ftn/BMG2_SymStd_SETUP_cg_LU.f90:119-227 (9-point)
+// and :251-335 (5-point) write it (one thread: the writes overlap by design), then an upper
+// Cholesky in DPOTF2's order: each pivot by one lane, the row of U across the workgroup, every
+// entry's dot product summed in index order.  ABD(n, n) column-major, n = nx*ny, per problem.
+__global__ __launch_bounds__(256) void setup_cg_per_k(const double *__restrict__ so, int nstncl,
+                                                      double *__restrict__ abd, int ii, int jj,
+                                                      int jpn, int *__restrict__ info) {
+    const int i1 = ii - 1, j1 = jj - 1, i2 = i1 - 1, n = i2 * (j1 - 1);
+    const std::size_t P = (std::size_t)ii * jj;
+    so += (std::size_t)blockIdx.x * nstncl * P;
+    abd += (std::size_t)blockIdx.x * n * n;
+    for (std::size_t t = threadIdx.x; t < (std::size_t)n * n; t += blockDim.x) abd[t] = 0.0;
+    __syncthreads();
+    auto SO = [&](int i, int j, int k) { return so[(std::size_t)(k - 1) * P + (std::size_t)(j - 1) * ii + (i - 1)]; };
+    auto A = [&](int r, int c) -> double & { return abd[(std::size_t)(c - 1) * n + (r - 1)]; };
+    constexpr int KO = 1, KW = 2, KS = 3, KSW = 4, KNW = 5;
+    const bool nine = nstncl == 5, px = per_x(jpn), py = per_y(jpn), pxy = jpn == 3 || jpn == -3;
+    if (threadIdx.x == 0) {
+        int kk = 1;
+        A(1, 1) = SO(2, 2, KO);
+        for (int i = 3; i <= i1; ++i) {
+            ++kk;
+            A(kk, kk) = SO(i, 2, KO);
+            A(kk - 1, kk) = -SO(i, 2, KW);
+        }
+        if (px) A(kk - i2 + 1, kk) = -SO(ii, 2, KW);
+        for (int j = 3; j <= j1; ++j) {
+            if (px) A(kk, kk + 1) = nine ? -SO(2, j, KSW) : 0.0;
+            for (int i = 2; i <= i1; ++i) {
+                ++kk;
+                A(kk, kk) = SO(i, j, KO);
+                if (i != 2) {
+                    A(kk - 1, kk) = -SO(i, j, KW);
+                    A(kk - i2 - 1, kk) = nine ? -SO(i, j, KSW) : 0.0;
+                }
+                A(kk - i2 + 1, kk) = nine ? -SO(i + 1, j, KNW) : 0.0;
+                A(kk - i2, kk) = -SO(i, j, KS);
+            }
+            if (px) {
+                A(kk - i2 + 1, kk) = -SO(ii, j, KW);
+                A(kk - 2 * i2 + 1, kk) = nine ? -SO(ii, j, KNW) : 0.0;
+            }
+        }
+        if (py) {
+            kk = kk - i2;
+            const int j2 = (j1 - 2) * i2;
+            ++kk;
+            A(kk - j2, kk) = -SO(2, jj, KS);
+            A(kk - j2 + 1, kk) = nine ? -SO(3, jj, KSW) : 0.0;
+            if (pxy) A(i2, kk) = nine ? -SO(2, jj, KNW) : 0.0;
+            for (int i = 3; i <= i1; ++i) {
+                ++kk;
+                A(kk - j2, kk) = -SO(i, jj, KS);
+                A(kk - j2 - 1, kk) = nine ? -SO(i, jj, KNW) : 0.0;
+                A(kk - j2 + 1, kk) = nine ? -SO(i + 1, jj, KSW) : 0.0;
+            }
+            A(kk - j2 + 1, kk) = 0.0;
+            if (pxy) A(1, kk) = nine ? -SO(ii, jj, KSW) : 0.0;
+            if (px) A(kk - 2 * i2 + 1, kk) = nine ? -SO(ii, j1, KNW) : 0.0;
+        }
+    }
+    __syncthreads();
+    __shared__ double piv;
+    __shared__ int bad;
+    if (threadIdx.x == 0) bad = 0;
+    for (int j = 1; j <= n; ++j) {
+        if (threadIdx.x == 0) {
+            double ajj = A(j, j);
+            for (int k = 1; k < j; ++k) ajj -= A(k, j) * A(k, j);
+            if (!(ajj > 0.0)) {
+                A(j, j) = ajj;
+                bad = j;
+            } else {
+                ajj = sqrt(ajj);
+                A(j, j) = ajj;
+            }
+            piv = ajj;
+        }
+        __syncthreads();
+        if (bad) break;
+        for (int c = j + 1 + (int)threadIdx.x; c <= n; c += (int)blockDim.x) {
+            double s = A(j, c);
+            for (int k = 1; k < j; ++k) s -= A(k, j) * A(k, c);
+            A(j, c) = s / piv;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && bad) atomicMax(info, bad);
+}
+
+// DPOTRS on the factor, then the mean removal and the periodic images
+// (src/2d/ftn/BMG2_SymStd_SOLVE_cg.f90:93-163); one lane per problem
+template <typename T>
+__global__ void solve_cg_per_k(T *__restrict__ q, const T *__restrict__ qf,
+                               const double *__restrict__ abd, double *__restrict__ x, int ii,
+                               int jj, int jpn) {
+    if (threadIdx.x != 0) return;
+    const int i1 = ii - 1, j1 = jj - 1, nx = ii - 2, n = nx * (jj - 2);
+    q += (std::size_t)blockIdx.x * ii * jj;
+    qf += (std::size_t)blockIdx.x * ii * jj;
+    abd += (std::size_t)blockIdx.x * n * n;
+    x += (std::size_t)blockIdx.x * n;
+    auto A = [&](int r, int c) { return abd[(std::size_t)(c - 1) * n + (r - 1)]; };
+    for (int kk = 0; kk < n; ++kk) x[kk] = (double)qf[pidx(kk % nx + 1, kk / nx + 1, ii)];
+    for (int j = 1; j <= n; ++j) {
+        double s = x[j - 1];
+        for (int k = 1; k < j; ++k) s -= A(k, j) * x[k - 1];
+        x[j - 1] = s / A(j, j);
+    }
+    for (int j = n; j >= 1; --j) {
+        double s = x[j - 1];
+        for (int k = j + 1; k <= n; ++k) s -= A(j, k) * x[k - 1];
+        x[j - 1] = s / A(j, j);
+    }
+    // mean removal, sums in FP64 and in Fortran order (kk runs i fastest)
+    double qint = 0.0, cint = 0.0;
+    for (int kk = 0; kk < n; ++kk) {
+        qint = qint + x[kk];
+        cint = cint + 1.0;
+    }
+    const double c = -qint / cint;
+    for (int kk = 0; kk < n; ++kk) q[pidx(kk % nx + 1, kk / nx + 1, ii)] = (T)(x[kk] + c);
+    auto Q = [&](int i, int j) -> T & { return q[pidx(i - 1, j - 1, ii)]; };  // 1-based
+    if (per_y(jpn))
+        for (int i = 2; i <= i1; ++i) {
+            Q(i, jj) = Q(i, 2);
+            Q(i, 1) = Q(i, j1);
+        }
+    if (per_x(jpn))
+        for (int j = 2; j <= j1; ++j) {
+            Q(ii, j) = Q(2, j);
+            Q(1, j) = Q(i1, j);
+        }
+    if (jpn == 3 || jpn == -3) {
+        Q(1, 1) = Q(i1, j1);
+        Q(ii, 1) = Q(2, j1);
+        Q(1, jj) = Q(i1, 2);
+        Q(ii, jj) = Q(2, 2);
+    }
+}
+
+}  // namespace
+
+template <typename T>
+void relax_gs_per(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
+                  int updown, int jpn, hipStream_t s, int nb) {
+    if (nstncl == 5) {
+        // per colour (i parity, j parity) as in relax_gs; after a colour the x images of the rows
+        // it swept (the reference refreshes them row by row, src/2d/ftn/BMG2_SymStd_relax_GS.f90
+        // :176-182)
+        static const int order[4][2] = {{1, 1}, {0, 1}, {1, 0}, {0, 0}};
+        const dim3 g = gridb(ii / 2 + 1, jj / 2 + 1, nb);
+        for (int t = 0; t < 4; ++t) {
+            const int c = updown == 0 ? t : 3 - t;
+            gs_colour_k<T, true><<<g, 256, 0, s>>>(so, qf, q, sor, ii, jj, order[c][0], order[c][1]);
+            if (per_x(jpn)) ghost_x<T>(q, ii, jj, order[c][1] == 1 ? 1 : 2, jj - 2, 2, s, nb);
+        }
+    } else {
+        const dim3 g = gridb(ii / 2 + 1, jj - 2, nb);
+        for (int t = 0; t < 2; ++t) {
+            const int parity = updown == 0 ? t : 1 - t;
+            gs_colour_k<T, false><<<g, 256, 0, s>>>(so, qf, q, sor, ii, jj, parity, 0);
+            if (per_x(jpn)) ghost_x<T>(q, ii, jj, 1, jj - 2, 1, s, nb);
+        }
+    }
+    if (per_y(jpn)) ghost_y<T>(q, ii, jj, 0, ii - 1, s, nb);
+}
+
+template <typename T>
+void restrict_per(const T *ci, T *q, T *qc, int iif, int jjf, int iic, int jjc, int jpn,
+                  hipStream_t s, int nb) {
+    // src/2d/ftn/BMG2_SymStd_restrict.f90:100-111: images of the fine grid refreshed in place
+    // when the fine extent is even
+    if (per_y(jpn) && jjf / 2 + 1 == jjc) ghost_y<T>(q, iif, jjf, 0, iif - 1, s, nb);
+    if (per_x(jpn) && iif / 2 + 1 == iic) ghost_x<T>(q, iif, jjf, 0, jjf - 1, 1, s, nb);
+    restrict_<T>(ci, q, qc, iif, jjf, iic, jjc, s, nb);
+}
+
+template <typename T>
+void interp_add_per(T *q, const T *qc, const T *res, const T *so, int nstncl, const T *ci, int iic,
+                    int jjc, int iif, int jjf, int jpn, hipStream_t s, int nb) {
+    interp_add<T>(q, qc, res, so, nstncl, ci, iic, jjc, iif, jjf, s, nb);
+    // src/2d/ftn/BMG2_SymStd_interp_add.f90:145-156
+    if (per_y(jpn)) ghost_y<T>(q, iif, jjf, 0, iif - 1, s, nb);
+    if (per_x(jpn)) ghost_x<T>(q, iif, jjf, 0, jjf - 1, 1, s, nb);
+}
+
+void setup_interp_per(const double *so, int nstncl, double *ci, int iif, int jjf, int iic, int jjc,
+                      int jpn, hipStream_t s, int nb) {
+    oi_bounds B;
+    const int iicf = (iif - 2) / 2 + 3, jjcf = (jjf - 2) / 2 + 3;
+    B.iifc = 2 * (iicf - 2) + 2;
+    B.jjfc = 2 * (jjcf - 2) + 2;
+    B.ibeg = 2; B.ibegc = 3; B.ibeg_x = 2; B.iend_x = iif - 2; B.iendc = iicf - 1;
+    B.jendc = jjcf - 1;
+    if (per_x(jpn)) {
+        B.ibeg = 0;
+        B.ibegc = 2;
+        if (iic == iicf) {
+            B.iendc = iicf;
+            B.ibeg_x = 3;
+            B.iend_x = iif - 1;
+        }
+    }
+    B.jbeg = 2; B.jbegc = 3; B.jbeg_y = 2; B.jend_y = jjf - 2;
+    if (per_y(jpn)) {
+        B.jbeg = 0;
+        B.jbegc = 2;
+        if (jjc == jjcf) {
+            B.jendc = jjcf;
+            B.jbeg_y = 3;
+            B.jend_y = jjf - 1;
+        }
+    }
+    const int ex = per_y(jpn) ? 2 : 0, ey = per_x(jpn) ? 2 : 0;
+    const int nic = B.iendc - B.ibegc + 1, njc = B.jendc - B.jbegc + 1;
+    const dim3 gx = gridb(nic, jjc - 2 + ex, nb), gy = gridb(iic - 2 + ey, njc, nb);
+    const dim3 gc = gridb(nic, njc, nb);
+    if (nstncl == 5) {
+        if (nic > 0) oi_per_xedge_k<true><<<gx, 256, 0, s>>>(so, ci, iif, jjf, iic, jjc, B, ex);
+        if (njc > 0) oi_per_yedge_k<true><<<gy, 256, 0, s>>>(so, ci, iif, jjf, iic, jjc, B, ey);
+        if (nic > 0 && njc > 0)
+            oi_per_corner_k<true><<<gc, 256, 0, s>>>(so, ci, iif, jjf, iic, jjc, B);
+    } else {
+        if (nic > 0) oi_per_xedge_k<false><<<gx, 256, 0, s>>>(so, ci, iif, jjf, iic, jjc, B, ex);
+        if (njc > 0) oi_per_yedge_k<false><<<gy, 256, 0, s>>>(so, ci, iif, jjf, iic, jjc, B, ey);
+        if (nic > 0 && njc > 0)
+            oi_per_corner_k<false><<<gc, 256, 0, s>>>(so, ci, iif, jjf, iic, jjc, B);
+    }
+}
+
+void setup_galerkin_per(const double *so, int nstncl, const double *ci, double *soc, int iif,
+                        int jjf, int iic, int jjc, int jpn, hipStream_t s, int nb) {
+    const dim3 g = gridb(iic - 2, jjc - 2, nb);
+    if (nstncl == 5)
+        galerkin_itli_k<true><<<g, 256, 0, s>>>(so, ci, soc, iif, jjf, iic, jjc);
+    else
+        galerkin_itli_k<false><<<g, 256, 0, s>>>(so, ci, soc, iif, jjf, iic, jjc);
+    if (per_y(jpn))
+        soc_ghost_y_k<<<dim3((unsigned)((iic + 255) / 256), 1, (unsigned)nb), 256, 0, s>>>(soc, iic, jjc);
+    if (per_x(jpn))
+        soc_ghost_x_k<<<dim3((unsigned)((jjc + 255) / 256), 1, (unsigned)nb), 256, 0, s>>>(soc, iic, jjc);
+}
+
+void setup_cg_per(const double *so, int nstncl, double *abd, int ii, int jj, int jpn, int *info,
+                  hipStream_t s, int nb) {
+    setup_cg_per_k<<<nb, 256, 0, s>>>(so, nstncl, abd, ii, jj, jpn, info);
+}
+
+template <typename T>
+void solve_cg_per(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj, int jpn,
+                  hipStream_t s, int nb) {
+    solve_cg_per_k<T><<<nb, 64, 0, s>>>(q, qf, abd, bbd, ii, jj, jpn);
+}
+
+template <typename T>
+void periodic_images(T *q, int ii, int jj, int jpn, hipStream_t s, int nb) {
+    if (per_y(jpn)) ghost_y<T>(q, ii, jj, 0, ii - 1, s, nb);
+    if (per_x(jpn)) ghost_x<T>(q, ii, jj, 0, jj - 1, 1, s, nb);
+}
+
+template <typename T>
+void copy(const T *src, T *dst, std::size_t n, hipStream_t s) {
+    convert_k<T, T><<<grid1(n), 256, 0, s>>>(src, dst, n);
+}
+
 // explicit instantiations
 #define CDR_INST(T)                                                                               \
     template void residual<T>(const T *, int, const T *, const T *, T *, int, int, hipStream_t,   \
@@ -2152,7 +2832,16 @@ void add_correction(double *x, const float *e, int ii, int jj, hipStream_t s, in
                                 int, int, hipStream_t, int);                                      \
     template void solve_cg<T>(T *, const T *, const double *, double *, int, int, hipStream_t,    \
                               int);                                                               \
-    template void fill<T>(T *, T, std::size_t, hipStream_t);
+    template void fill<T>(T *, T, std::size_t, hipStream_t);                                     \
+    template void relax_gs_per<T>(const T *, int, const T *, T *, const T *, int, int, int, int,  \
+                                  hipStream_t, int);                                              \
+    template void restrict_per<T>(const T *, T *, T *, int, int, int, int, int, hipStream_t, int);\
+    template void interp_add_per<T>(T *, const T *, const T *, const T *, int, const T *, int,    \
+                                    int, int, int, int, hipStream_t, int);                        \
+    template void solve_cg_per<T>(T *, const T *, const double *, double *, int, int, int,        \
+                                  hipStream_t, int);                                              \
+    template void copy<T>(const T *, T *, std::size_t, hipStream_t);                              \
+    template void periodic_images<T>(T *, int, int, int, hipStream_t, int);
 CDR_INST(double)
 CDR_INST(float)
 #undef CDR_INST

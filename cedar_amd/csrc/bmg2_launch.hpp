@@ -160,7 +160,38 @@ void setup_cg(const double *so, int nstncl, double *abd, int ii, int jj, int *in
 template <typename T>
 void solve_cg(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj, hipStream_t s, int nb = 1);
 
+// Periodic boundaries: jpn = BMG_get_bc(per_mask) (2 periodic in x, 1 in y, 3 both; 0 runs the
+// Dirichlet launcher).  Each is the Dirichlet kernel plus the reference's periodic images in the
+// ghost rows/columns; setup_interp_per and setup_galerkin_per are the reference's periodic loop
+// structure and hand-expanded products; the coarse solve is a dense Cholesky of the n x n
+// periodic matrix (ABD(n, n), n = nx*ny) with the mean of the solution removed.
+template <typename T>
+void relax_gs_per(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
+                  int updown, int jpn, hipStream_t s, int nb = 1);
+// refreshes the images of q in place first when the fine extent is even (as the reference does)
+template <typename T>
+void restrict_per(const T *ci, T *q, T *qc, int iif, int jjf, int iic, int jjc, int jpn,
+                  hipStream_t s, int nb = 1);
+template <typename T>
+void interp_add_per(T *q, const T *qc, const T *res, const T *so, int nstncl, const T *ci, int iic,
+                    int jjc, int iif, int jjf, int jpn, hipStream_t s, int nb = 1);
+void setup_interp_per(const double *so, int nstncl, double *ci, int iif, int jjf, int iic, int jjc,
+                      int jpn, hipStream_t s, int nb = 1);
+void setup_galerkin_per(const double *so, int nstncl, const double *ci, double *soc, int iif,
+                        int jjf, int iic, int jjc, int jpn, hipStream_t s, int nb = 1);
+void setup_cg_per(const double *so, int nstncl, double *abd, int ii, int jj, int jpn, int *info,
+                  hipStream_t s, int nb = 1);
+template <typename T>
+void solve_cg_per(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj, int jpn,
+                  hipStream_t s, int nb = 1);
+
+// periodic images of every ghost row (all columns) and then every ghost column (all rows)
+template <typename T>
+void periodic_images(T *q, int ii, int jj, int jpn, hipStream_t s, int nb = 1);
+
 // elementwise helpers
+template <typename T>
+void copy(const T *src, T *dst, std::size_t n, hipStream_t s);
 template <typename Tin, typename Tout>
 void convert(const Tin *in, Tout *out, std::size_t n, hipStream_t s);
 template <typename T>

@@ -171,7 +171,98 @@ int solve_cg(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj, 
     return launched();
 }
 
+// ---- periodic boundaries (jpn = BMG_get_bc(per_mask): 0, 1, 2 or 3) -------------------------
+inline bool jpn_ok(int jpn) { return jpn >= 0 && jpn <= 3; }
+
+template <typename T>
+int relax_gs_per(const T *so, int ns, const T *qf, T *q, const T *sor, int ii, int jj, int ud,
+                 int jpn, hipStream_t s, int nb) {
+    if (!so || !qf || !q || !sor || !ext_ok(ii, jj) || !sten_ok(ns) || !dir_ok(ud) ||
+        !nb_ok(nb) || !jpn_ok(jpn))
+        return CDR_ERR_ARG;
+    if (jpn == 0)
+        k2::relax_gs<T>(so, ns, qf, q, sor, ii, jj, ud, s, nb);
+    else
+        k2::relax_gs_per<T>(so, ns, qf, q, sor, ii, jj, ud, jpn, s, nb);
+    return launched();
+}
+
+template <typename T>
+int restrict_per(const T *ci, T *q, T *qc, int iif, int jjf, int iic, int jjc, int jpn,
+                 hipStream_t s, int nb) {
+    if (!ci || !q || !qc || !coarse_ok(iif, jjf, iic, jjc) || !nb_ok(nb) || !jpn_ok(jpn))
+        return CDR_ERR_ARG;
+    k2::restrict_per<T>(ci, q, qc, iif, jjf, iic, jjc, jpn, s, nb);
+    return launched();
+}
+
+template <typename T>
+int interp_add_per(T *q, const T *qc, const T *res, const T *so, int ns, const T *ci, int iic,
+                   int jjc, int iif, int jjf, int jpn, hipStream_t s, int nb) {
+    if (!q || !qc || !res || !so || !ci || !sten_ok(ns) || !coarse_ok(iif, jjf, iic, jjc) ||
+        !nb_ok(nb) || !jpn_ok(jpn))
+        return CDR_ERR_ARG;
+    k2::interp_add_per<T>(q, qc, res, so, ns, ci, iic, jjc, iif, jjf, jpn, s, nb);
+    return launched();
+}
+
+int setup_interp_per(const double *so, int ns, double *ci, int iif, int jjf, int iic, int jjc,
+                     int jpn, hipStream_t s, int nb) {
+    if (!so || !ci || !sten_ok(ns) || !coarse_ok(iif, jjf, iic, jjc) || !nb_ok(nb) ||
+        !jpn_ok(jpn))
+        return CDR_ERR_ARG;
+    if (jpn == 0)
+        k2::setup_interp(so, ns, ci, iif, jjf, iic, jjc, s, nb);
+    else
+        k2::setup_interp_per(so, ns, ci, iif, jjf, iic, jjc, jpn, s, nb);
+    return launched();
+}
+
+int setup_galerkin_per(const double *so, int ns, const double *ci, double *soc, int iif, int jjf,
+                       int iic, int jjc, int jpn, hipStream_t s, int nb) {
+    if (!so || !ci || !soc || !sten_ok(ns) || !coarse_ok(iif, jjf, iic, jjc) || !nb_ok(nb) ||
+        !jpn_ok(jpn))
+        return CDR_ERR_ARG;
+    if (jpn == 0)
+        k2::setup_galerkin(so, ns, ci, soc, iif, jjf, iic, jjc, s, nb);
+    else
+        k2::setup_galerkin_per(so, ns, ci, soc, iif, jjf, iic, jjc, jpn, s, nb);
+    return launched();
+}
+
+// the dense periodic matrix needs at least 2 x 2 points (all of its writes then stay inside it)
+inline bool per_cg_ok(int ii, int jj) {
+    return ii >= 4 && jj >= 4 && (long long)(ii - 2) * (jj - 2) <= 2048;
+}
+
+int setup_cg_per(const double *so, int ns, double *abd, int ii, int jj, int jpn, int *info_dev,
+                 hipStream_t s, int nb) {
+    if (!so || !abd || !ext_ok(ii, jj) || !sten_ok(ns) || !nb_ok(nb) || !jpn_ok(jpn))
+        return CDR_ERR_ARG;
+    if (jpn == 0) return setup_cg(so, ns, abd, ii, jj, info_dev, s, nb);
+    if (!per_cg_ok(ii, jj)) return CDR_ERR_ARG;
+    return with_info(info_dev, s, [&] { k2::setup_cg_per(so, ns, abd, ii, jj, jpn, info_dev, s, nb); });
+}
+
+template <typename T>
+int solve_cg_per(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj, int jpn,
+                 hipStream_t s, int nb) {
+    if (!q || !qf || !abd || !bbd || !ext_ok(ii, jj) || !nb_ok(nb) || !jpn_ok(jpn))
+        return CDR_ERR_ARG;
+    if (jpn == 0) return solve_cg<T>(q, qf, abd, bbd, ii, jj, s, nb);
+    if (!per_cg_ok(ii, jj)) return CDR_ERR_ARG;
+    k2::solve_cg_per<T>(q, qf, abd, bbd, ii, jj, jpn, s, nb);
+    return launched();
+}
+
 #define CDR_API_INST(T)                                                                            \
+    template int relax_gs_per<T>(const T *, int, const T *, T *, const T *, int, int, int, int,    \
+                                 hipStream_t, int);                                                \
+    template int restrict_per<T>(const T *, T *, T *, int, int, int, int, int, hipStream_t, int);  \
+    template int interp_add_per<T>(T *, const T *, const T *, const T *, int, const T *, int, int,  \
+                                   int, int, int, hipStream_t, int);                               \
+    template int solve_cg_per<T>(T *, const T *, const double *, double *, int, int, int,          \
+                                 hipStream_t, int);                                                \
     template int residual<T>(const T *, int, const T *, const T *, T *, int, int, hipStream_t,     \
                              int);                                                                 \
     template int relax_gs<T>(const T *, int, const T *, T *, const T *, int, int, int, hipStream_t, \
@@ -359,6 +450,54 @@ int cdr_hip_solve_cg2d_f64(double *q, const double *qf, const double *abd, doubl
 int cdr_hip_solve_cg2d_f32(float *q, const float *qf, const double *abd, double *bbd, int ii,
                            int jj, void *st) {
     return api2::solve_cg<float>(q, qf, abd, bbd, ii, jj, S(st));
+}
+
+// ---- periodic boundaries ------------------------------------------------------------------
+int cdr_hip_relax_gs2d_per_f64(const double *so, int ns, const double *qf, double *q,
+                               const double *sor, int ii, int jj, int ud, int jpn, void *st) {
+    return api2::relax_gs_per<double>(so, ns, qf, q, sor, ii, jj, ud, jpn, S(st));
+}
+int cdr_hip_relax_gs2d_per_f32(const float *so, int ns, const float *qf, float *q,
+                               const float *sor, int ii, int jj, int ud, int jpn, void *st) {
+    return api2::relax_gs_per<float>(so, ns, qf, q, sor, ii, jj, ud, jpn, S(st));
+}
+int cdr_hip_restrict2d_per_f64(const double *ci, double *q, double *qc, int iif, int jjf, int iic,
+                               int jjc, int jpn, void *st) {
+    return api2::restrict_per<double>(ci, q, qc, iif, jjf, iic, jjc, jpn, S(st));
+}
+int cdr_hip_restrict2d_per_f32(const float *ci, float *q, float *qc, int iif, int jjf, int iic,
+                               int jjc, int jpn, void *st) {
+    return api2::restrict_per<float>(ci, q, qc, iif, jjf, iic, jjc, jpn, S(st));
+}
+int cdr_hip_interp_add2d_per_f64(double *q, const double *qc, const double *res, const double *so,
+                                 int ns, const double *ci, int iic, int jjc, int iif, int jjf,
+                                 int jpn, void *st) {
+    return api2::interp_add_per<double>(q, qc, res, so, ns, ci, iic, jjc, iif, jjf, jpn, S(st));
+}
+int cdr_hip_interp_add2d_per_f32(float *q, const float *qc, const float *res, const float *so,
+                                 int ns, const float *ci, int iic, int jjc, int iif, int jjf,
+                                 int jpn, void *st) {
+    return api2::interp_add_per<float>(q, qc, res, so, ns, ci, iic, jjc, iif, jjf, jpn, S(st));
+}
+int cdr_hip_setup_interp2d_per_f64(const double *so, int ns, double *ci, int iif, int jjf, int iic,
+                                   int jjc, int jpn, void *st) {
+    return api2::setup_interp_per(so, ns, ci, iif, jjf, iic, jjc, jpn, S(st));
+}
+int cdr_hip_setup_galerkin2d_per_f64(const double *so, int ns, const double *ci, double *soc,
+                                     int iif, int jjf, int iic, int jjc, int jpn, void *st) {
+    return api2::setup_galerkin_per(so, ns, ci, soc, iif, jjf, iic, jjc, jpn, S(st));
+}
+int cdr_hip_setup_cg2d_per_f64(const double *so, int ns, double *abd, int ii, int jj, int jpn,
+                               int *info_dev, void *st) {
+    return api2::setup_cg_per(so, ns, abd, ii, jj, jpn, info_dev, S(st));
+}
+int cdr_hip_solve_cg2d_per_f64(double *q, const double *qf, const double *abd, double *bbd, int ii,
+                               int jj, int jpn, void *st) {
+    return api2::solve_cg_per<double>(q, qf, abd, bbd, ii, jj, jpn, S(st));
+}
+int cdr_hip_solve_cg2d_per_f32(float *q, const float *qf, const double *abd, double *bbd, int ii,
+                               int jj, int jpn, void *st) {
+    return api2::solve_cg_per<float>(q, qf, abd, bbd, ii, jj, jpn, S(st));
 }
 
 }  // extern "C"

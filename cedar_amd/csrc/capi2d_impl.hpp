@@ -48,5 +48,26 @@ template <typename T>
 int solve_cg(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj, hipStream_t s,
              int nb = 1);
 
+// periodic forms: jpn = BMG_get_bc(per_mask) in 0..3 (0 runs the Dirichlet kernel); the coarse
+// matrix is dense, ABD(n, n) with n = nx*ny
+template <typename T>
+int relax_gs_per(const T *so, int ns, const T *qf, T *q, const T *sor, int ii, int jj, int ud,
+                 int jpn, hipStream_t s, int nb = 1);
+template <typename T>
+int restrict_per(const T *ci, T *q, T *qc, int iif, int jjf, int iic, int jjc, int jpn,
+                 hipStream_t s, int nb = 1);
+template <typename T>
+int interp_add_per(T *q, const T *qc, const T *res, const T *so, int ns, const T *ci, int iic,
+                   int jjc, int iif, int jjf, int jpn, hipStream_t s, int nb = 1);
+int setup_interp_per(const double *so, int ns, double *ci, int iif, int jjf, int iic, int jjc,
+                     int jpn, hipStream_t s, int nb = 1);
+int setup_galerkin_per(const double *so, int ns, const double *ci, double *soc, int iif, int jjf,
+                       int iic, int jjc, int jpn, hipStream_t s, int nb = 1);
+int setup_cg_per(const double *so, int ns, double *abd, int ii, int jj, int jpn, int *info_dev,
+                 hipStream_t s, int nb = 1);
+template <typename T>
+int solve_cg_per(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj, int jpn,
+                 hipStream_t s, int nb = 1);
+
 }  // namespace api2
 }  // namespace cdr

@@ -388,6 +388,7 @@ struct mat_obj {
 struct solver_obj {
     int nd;
     int maxiter;
+    int pmask = 0;   // periodic directions (2D)
     cdr_hip_solver2d *s2 = nullptr;
     cdr_hip_solver3d *s3 = nullptr;
     std::size_t npts;
@@ -527,12 +528,15 @@ bool read_precision(const jvalue &root, int &prec, std::string &err) {
     return true;
 }
 
-bool periodic_requested(const jvalue &root) {
+// grid.periodic as Cedar's per_mask (src/kernel_params.cc:74-76, include/cedar/kernel_params.h
+// :35-42): bit d set when entry d is true
+int periodic_mask(const jvalue &root) {
     const jvalue *p = root.at("grid.periodic");
-    if (!p || p->kind != jvalue::ARR) return false;
-    for (auto &e : p->arr)
-        if (e.kind == jvalue::BOOL && e.b) return true;
-    return false;
+    if (!p || p->kind != jvalue::ARR) return 0;
+    int mask = 0;
+    for (std::size_t d = 0; d < p->arr.size() && d < 3; ++d)
+        if (p->arr[d].kind == jvalue::BOOL && p->arr[d].b) mask |= 1 << d;
+    return mask;
 }
 
 // ml_settings operator<< (src/multilevel_settings.cc:63-95)
@@ -550,6 +554,40 @@ std::string settings_text(const ml_keys &k) {
     os << "maxiter:         " << k.maxiter << '\n';
     os << "tol:             " << k.tol;
     return os.str();
+}
+
+// Periodic images in the ghost layer, which Cedar's distributed objects get from their halo
+// exchange: an operator's images as examples/basic-2d-ser/periodic.cc:56-78 writes them
+// (interior rows for x, interior columns for y, every plane), a grid function's for every ghost
+// row and then every ghost column.
+void operator_images(double *so, int planes, std::size_t ii, std::size_t jj, int mask) {
+    const std::size_t P = ii * jj, nx = ii - 2, ny = jj - 2;
+    for (int k = 0; k < planes; ++k) {
+        double *a = so + (std::size_t)k * P;
+        if (mask & 1)
+            for (std::size_t j = 1; j <= ny; ++j) {
+                a[j * ii] = a[j * ii + nx];
+                a[j * ii + nx + 1] = a[j * ii + 1];
+            }
+        if (mask & 2)
+            for (std::size_t i = 1; i <= nx; ++i) {
+                a[i] = a[ny * ii + i];
+                a[(ny + 1) * ii + i] = a[ii + i];
+            }
+    }
+}
+void vector_images(double *q, std::size_t ii, std::size_t jj, int mask) {
+    const std::size_t nx = ii - 2, ny = jj - 2;
+    if (mask & 2)
+        for (std::size_t i = 0; i < ii; ++i) {
+            q[i] = q[ny * ii + i];
+            q[(ny + 1) * ii + i] = q[ii + i];
+        }
+    if (mask & 1)
+        for (std::size_t j = 0; j < jj; ++j) {
+            q[j * ii] = q[j * ii + nx];
+            q[j * ii + nx + 1] = q[j * ii + 1];
+        }
 }
 
 bool upload(double *dst, const double *src, std::size_t n) {
@@ -848,11 +886,28 @@ int cedar_matvec(cedar_mat mat, cedar_vec x, cedar_vec y) {
     const std::size_t n = m->npts;
     if (!m->dso.reserve(m->so.size()) || !m->dx.reserve(n) || !m->dy.reserve(n))
         return CEDAR_ERR_OTHER;
+    // the reference exchanges the halo of x first (src/2d/interface/c/mat.cc:172-183); on one
+    // grid that fills the periodic images
+    const int pmask = m->nd == 2 ? (periodic_mask(m->cfg->root) & 3) : 0;
     if (m->dirty) {
-        if (!upload(m->dso.p, m->so.data(), m->so.size())) return CEDAR_ERR_OTHER;
+        const double *src = m->so.data();
+        std::vector<double> tmp;
+        if (pmask) {
+            tmp = m->so;
+            operator_images(tmp.data(), m->nstncl, m->len[0], m->len[1], pmask);
+            src = tmp.data();
+        }
+        if (!upload(m->dso.p, src, m->so.size())) return CEDAR_ERR_OTHER;
         m->dirty = false;
     }
-    if (!upload(m->dx.p, xv->data.data(), n) || !upload(m->dy.p, yv->data.data(), n))
+    const double *xs = xv->data.data();
+    std::vector<double> xtmp;
+    if (pmask) {
+        xtmp = xv->data;
+        vector_images(xtmp.data(), m->len[0], m->len[1], pmask);
+        xs = xtmp.data();
+    }
+    if (!upload(m->dx.p, xs, n) || !upload(m->dy.p, yv->data.data(), n))
         return CEDAR_ERR_OTHER;
     int st = m->nd == 2
                  ? cdr_hip_matvec2d_f64(m->dso.p, m->nstncl, m->dx.p, m->dy.p, (int)m->len[0],
@@ -919,8 +974,9 @@ int cedar_solver_create(cedar_mat mat, cedar_solver *solver) {
         log_line(LOG_ERROR, err);
         return CEDAR_ERR_CONFIG;
     }
-    if (periodic_requested(root)) {
-        log_line(LOG_ERROR, "grid.periodic: periodic boundaries are not supported");
+    const int pmask = periodic_mask(root);
+    if (pmask != 0 && (m->nd != 2 || (pmask & 4) || k.relaxation != CDR_RELAX_POINT)) {
+        log_line(LOG_ERROR, "grid.periodic: periodic boundaries are built for 2D point relaxation");
         return CEDAR_ERR_CONFIG;
     }
     if (get_or(root, "gpu.autotune", false, as_bool)) cdr_hip_autotune(1);
@@ -933,7 +989,12 @@ int cedar_solver_create(cedar_mat mat, cedar_solver *solver) {
     for (int d = 0; d < 3; ++d) s->len[d] = m->len[d];
     if (!s->dso.reserve(m->so.size()) || !s->dx.reserve(s->npts) || !s->db.reserve(s->npts))
         return CEDAR_ERR_OTHER;
-    if (!upload(s->dso.p, m->so.data(), m->so.size())) return CEDAR_ERR_OTHER;
+    s->pmask = pmask;
+    {
+        std::vector<double> op = m->so;
+        if (pmask) operator_images(op.data(), m->nstncl, m->len[0], m->len[1], pmask);
+        if (!upload(s->dso.p, op.data(), op.size())) return CEDAR_ERR_OTHER;
+    }
     int st;
     if (m->nd == 2) {
         if (k.relaxation >= CDR_RELAX_PLANE_XY) {
@@ -943,6 +1004,7 @@ int cedar_solver_create(cedar_mat mat, cedar_solver *solver) {
         cdr_solver_settings cs;
         to_settings(k, root, cs);
         cs.precision = prec;
+        cs.periodic = pmask;
         st = cdr_hip_solver2d_create(s->dso.p, m->nstncl, (int)m->grid->n[0],
                                      (int)m->grid->n[1], &cs, nullptr, &s->s2);
     } else {
@@ -998,8 +1060,17 @@ int cedar_solver_run(cedar_solver solver, cedar_vec x, cedar_vec b) {
     if (!xv || !bv) return CEDAR_ERR_VEC;
     if (s->nd != xv->nd || s->nd != bv->nd) return CEDAR_ERR_DIM;
     if (xv->data.size() != s->npts || bv->data.size() != s->npts) return CEDAR_ERR_DIM;
-    if (!upload(s->dx.p, xv->data.data(), s->npts) || !upload(s->db.p, bv->data.data(), s->npts))
-        return CEDAR_ERR_OTHER;
+    const double *xs = xv->data.data(), *bs = bv->data.data();
+    std::vector<double> xtmp, btmp;
+    if (s->pmask) {  // images of x and b, as the reference's halo exchange provides them
+        xtmp = xv->data;
+        btmp = bv->data;
+        vector_images(xtmp.data(), s->len[0], s->len[1], s->pmask);
+        vector_images(btmp.data(), s->len[0], s->len[1], s->pmask);
+        xs = xtmp.data();
+        bs = btmp.data();
+    }
+    if (!upload(s->dx.p, xs, s->npts) || !upload(s->db.p, bs, s->npts)) return CEDAR_ERR_OTHER;
     std::vector<double> hist((std::size_t)(s->maxiter > 0 ? s->maxiter : 1));
     int nit = 0;
     int st = s->nd == 2 ? cdr_hip_solver2d_solve(s->s2, s->db.p, s->dx.p, hist.data(),

@@ -175,6 +175,9 @@ struct relax_stencil {   // 2 planes
 struct kernel_base {
     virtual ~kernel_base() = default;
     hipStream_t stream = nullptr;
+    // boundary id BMG_get_bc(per_mask) of the kernel parameters (include/cedar/kernel_params.h
+    // :35-42): 0 Dirichlet, 2 periodic in x, 1 in y, 3 in both
+    int jpn = 0;
 };
 
 namespace kernels {
@@ -278,6 +281,12 @@ public:
     void run(Args &&...args) { get<I>().run(std::forward<Args>(args)...); }
 
     hipStream_t stream() const { return stream_; }
+
+    // sets the boundary id of every registered implementation
+    void set_bc(int jpn) {
+        for (auto &t : table_)
+            for (auto &kv : t.second.impls) kv.second->jpn = jpn;
+    }
 
     // points every registered implementation at stream s (used while capturing a graph)
     void set_stream(hipStream_t s) {

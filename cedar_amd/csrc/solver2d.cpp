@@ -41,7 +41,10 @@ struct hip_rbgs : kernels::point_relax<T> {
     }
     void run(const stencil_op<T> &so, grid_func<T> &x, const grid_func<T> &b,
              const relax_stencil<T> &sor, Dir d) override {
-        check(api2::relax_gs<T>(so.p, so.nstncl, b.p, x.p, sor.p, so.ext.ii, so.ext.jj, (int)d, this->stream, so.ext.nb), "relax_gs");
+        if (this->jpn)
+            check(api2::relax_gs_per<T>(so.p, so.nstncl, b.p, x.p, sor.p, so.ext.ii, so.ext.jj, (int)d, this->jpn, this->stream, so.ext.nb), "relax_gs");
+        else
+            check(api2::relax_gs<T>(so.p, so.nstncl, b.p, x.p, sor.p, so.ext.ii, so.ext.jj, (int)d, this->stream, so.ext.nb), "relax_gs");
     }
 };
 
@@ -66,21 +69,32 @@ struct hip_setup_interp : kernels::setup_interp {
     void run(const stencil_op<double> &fop, const stencil_op<double> &cop,
              prolong_op<double> &P) override {
         P.fine_op = &fop;
-        check(api2::setup_interp(fop.p, fop.nstncl, P.p, fop.ext.ii, fop.ext.jj, cop.ext.ii, cop.ext.jj, this->stream, fop.ext.nb), "setup_interp");
+        if (this->jpn)
+            check(api2::setup_interp_per(fop.p, fop.nstncl, P.p, fop.ext.ii, fop.ext.jj, cop.ext.ii, cop.ext.jj, this->jpn, this->stream, fop.ext.nb), "setup_interp");
+        else
+            check(api2::setup_interp(fop.p, fop.nstncl, P.p, fop.ext.ii, fop.ext.jj, cop.ext.ii, cop.ext.jj, this->stream, fop.ext.nb), "setup_interp");
     }
 };
 
 struct hip_galerkin : kernels::coarsen_op {
     void run(const prolong_op<double> &P, const stencil_op<double> &fop,
              stencil_op<double> &cop) override {
-        check(api2::setup_galerkin(fop.p, fop.nstncl, P.p, cop.p, fop.ext.ii, fop.ext.jj, cop.ext.ii, cop.ext.jj, this->stream, fop.ext.nb), "galerkin");
+        if (this->jpn)
+            check(api2::setup_galerkin_per(fop.p, fop.nstncl, P.p, cop.p, fop.ext.ii, fop.ext.jj, cop.ext.ii, cop.ext.jj, this->jpn, this->stream, fop.ext.nb), "galerkin");
+        else
+            check(api2::setup_galerkin(fop.p, fop.nstncl, P.p, cop.p, fop.ext.ii, fop.ext.jj, cop.ext.ii, cop.ext.jj, this->stream, fop.ext.nb), "galerkin");
     }
 };
 
 template <typename T>
 struct hip_restrict : kernels::restriction<T> {
     void run(const prolong_op<T> &R, const grid_func<T> &fine, grid_func<T> &coarse) override {
-        check(api2::restrict_<T>(R.p, fine.p, coarse.p, fine.ext.ii, fine.ext.jj, coarse.ext.ii, coarse.ext.jj, this->stream, fine.ext.nb), "restrict");
+        // periodic: the images of the fine grid are refreshed in place, as the reference does
+        // (src/2d/restrict.cc:23-27 passes the fine grid function as mutable)
+        if (this->jpn)
+            check(api2::restrict_per<T>(R.p, fine.p, coarse.p, fine.ext.ii, fine.ext.jj, coarse.ext.ii, coarse.ext.jj, this->jpn, this->stream, fine.ext.nb), "restrict");
+        else
+            check(api2::restrict_<T>(R.p, fine.p, coarse.p, fine.ext.ii, fine.ext.jj, coarse.ext.ii, coarse.ext.jj, this->stream, fine.ext.nb), "restrict");
     }
 };
 
@@ -89,7 +103,10 @@ struct hip_interp : kernels::interp_add<T> {
     void run(const prolong_op<T> &P, const grid_func<T> &coarse, const grid_func<T> &residual,
              grid_func<T> &fine) override {
         const stencil_op<T> &fop = *P.fine_op;
-        check(api2::interp_add<T>(fine.p, coarse.p, residual.p, fop.p, fop.nstncl, P.p, coarse.ext.ii, coarse.ext.jj, fine.ext.ii, fine.ext.jj, this->stream, fine.ext.nb), "interp_add");
+        if (this->jpn)
+            check(api2::interp_add_per<T>(fine.p, coarse.p, residual.p, fop.p, fop.nstncl, P.p, coarse.ext.ii, coarse.ext.jj, fine.ext.ii, fine.ext.jj, this->jpn, this->stream, fine.ext.nb), "interp_add");
+        else
+            check(api2::interp_add<T>(fine.p, coarse.p, residual.p, fop.p, fop.nstncl, P.p, coarse.ext.ii, coarse.ext.jj, fine.ext.ii, fine.ext.jj, this->stream, fine.ext.nb), "interp_add");
     }
 };
 
@@ -98,11 +115,17 @@ struct hip_solve_cg : kernels::solve_cg<T> {
     explicit hip_solve_cg(int *info) : info_(info) {}
     void setup(const stencil_op<double> &so, grid_func<double> &abd) override {
         (void)abd.ext;
-        check(api2::setup_cg(so.p, so.nstncl, abd.p, so.ext.ii, so.ext.jj, info_, this->stream, so.ext.nb), "setup_cg");
+        if (this->jpn)
+            check(api2::setup_cg_per(so.p, so.nstncl, abd.p, so.ext.ii, so.ext.jj, this->jpn, info_, this->stream, so.ext.nb), "setup_cg");
+        else
+            check(api2::setup_cg(so.p, so.nstncl, abd.p, so.ext.ii, so.ext.jj, info_, this->stream, so.ext.nb), "setup_cg");
     }
     void run(grid_func<T> &x, const grid_func<T> &b, const grid_func<double> &abd,
              double *bbd) override {
-        check(api2::solve_cg<T>(x.p, b.p, abd.p, bbd, x.ext.ii, x.ext.jj, this->stream, x.ext.nb), "solve_cg");
+        if (this->jpn)
+            check(api2::solve_cg_per<T>(x.p, b.p, abd.p, bbd, x.ext.ii, x.ext.jj, this->jpn, this->stream, x.ext.nb), "solve_cg");
+        else
+            check(api2::solve_cg<T>(x.p, b.p, abd.p, bbd, x.ext.ii, x.ext.jj, this->stream, x.ext.nb), "solve_cg");
     }
 
 private:
@@ -170,6 +193,7 @@ void cdr_hip_default_settings(cdr_solver_settings *s) {
     s->graph = 1;
     s->cycle = CDR_CYCLE_V;
     s->fused_coarse = 1;
+    s->periodic = 0;
 }
 
 int cdr_hip_solver2d_create(const double *so_dev, int nstncl, int nx, int ny,
@@ -186,7 +210,8 @@ int cdr_hip_solver2d_create_batched(const double *so_dev, int nstncl, int nx, in
         s->nrelax_pre < 0 || s->nrelax_post < 0 || s->maxiter < 0 || s->min_coarse < 2 ||
         (s->precision != CDR_PREC_F64 && s->precision != CDR_PREC_MIXED) ||
         (s->graph != 0 && s->graph != 1) || (s->cycle != CDR_CYCLE_V && s->cycle != CDR_CYCLE_F) ||
-        (s->fused_coarse != 0 && s->fused_coarse != 1))
+        (s->fused_coarse != 0 && s->fused_coarse != 1) || s->periodic < 0 || s->periodic > 3 ||
+        (s->periodic != 0 && s->relaxation != CDR_RELAX_POINT))
         return CDR_ERR_ARG;
     *out = nullptr;
     int ndev = 0;

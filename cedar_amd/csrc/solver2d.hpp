@@ -57,6 +57,13 @@ public:
         check_hip(hipEventCreate(&ev1_), "hipEventCreate");
         info_ = dbuf<int>(1, stream_);
         register_hip_kernels(km_, info_.data());
+        // BMG_get_bc (src/2d/ftn/BMG_get_bc.f90): per_mask 1 (x) -> 2, 2 (y) -> 1, 3 -> 3
+        static const int bc_of_mask[4] = {0, 2, 1, 3};
+        if (s.periodic < 0 || s.periodic > 3) throw error(CDR_ERR_ARG, "invalid periodic mask");
+        if (s.periodic != 0 && s.relaxation != CDR_RELAX_POINT)
+            throw error(CDR_ERR_ARG, "periodic boundaries need point relaxation");
+        jpn_ = bc_of_mask[s.periodic];
+        km_.set_bc(jpn_);
         build(so_dev, nstncl, nx, ny, nb);
     }
 
@@ -178,6 +185,7 @@ public:
             k2::residual_ir(A.p, A.nstncl, b_dev, x_dev, nullptr, xa, F.b.p, e.ii, e.jj,
                             work_.data(), sumsq_.data(), stream_, nb);
             check_hip(hipGetLastError(), "residual_ir");
+            periodic_images(xa, e, nb);
             read_norms();
             for (int p = 0; p < nb; ++p) res0[p] = std::sqrt(host_norm_[p]);
             for (int i = 0; i < set_.maxiter; ++i) {
@@ -191,6 +199,7 @@ public:
                                 nactive < nb ? active_.data() : nullptr);
                 check_hip(hipGetLastError(), "residual_ir");
                 std::swap(xa, xb);
+                periodic_images(xa, e, nb);
                 read_norms();
                 if (!record(i)) break;
             }
@@ -209,6 +218,17 @@ public:
     }
 
 private:
+    // Periodic boundaries: the FP64 iterate of the mixed-precision loop is updated in the interior
+    // (x += e), so its periodic images are refreshed after each update from the interior values;
+    // the residual of the next step then reads x + e at the images, the image of the updated
+    // iterate.  (The FP64 cycle keeps its images through the periodic kernels themselves.)
+    void periodic_images(double *x, const extents &e, int nb) {
+        if (jpn_ == 0) return;
+        k2::periodic_images<double>(x, e.ii, e.jj, jpn_, stream_, nb);
+        check_hip(hipGetLastError(), "periodic_images");
+    }
+
+    int jpn_ = 0;
     cdr_solver_settings set_;
     hipStream_t stream_;
     hipStream_t cur_;                  // stream the cycle kernels go to (capture stream while capturing)
@@ -329,7 +349,10 @@ private:
             cy = (int)((cy - 1) / 2.0 + 1);
         }
         const extents ec = L64_[nl - 1].ext;
-        abd_ = dbuf<double>((std::size_t)ec.ii * ec.nx() * ec.ny() * nb, stream_);
+        // the coarse matrix: band ABD(ii, n), or dense ABD(n, n) when periodic
+        // (include/cedar/2d/solver.h:116-121)
+        const std::size_t ncg = (std::size_t)ec.nx() * ec.ny();
+        abd_ = dbuf<double>((jpn_ ? ncg : (std::size_t)ec.ii) * ncg * nb, stream_);
         bbd_ = dbuf<double>((std::size_t)ec.nx() * ec.ny() * nb, stream_);
         work_ = dbuf<double>(cdr_hip_norm_work2d(L64_[0].ext.ii, L64_[0].ext.jj) * nb, stream_);
         sumsq_ = dbuf<double>(nb, stream_);
@@ -366,7 +389,8 @@ private:
 
     grid_func<double> &abd_grid() {
         const extents ec = L64_.back().ext;
-        abd_g_ = grid_func<double>{abd_.data(), extents{ec.ii, ec.nx() * ec.ny(), ec.nb}};
+        const int ncg = ec.nx() * ec.ny();
+        abd_g_ = grid_func<double>{abd_.data(), extents{jpn_ ? ncg : ec.ii, ncg, ec.nb}};
         return abd_g_;
     }
 
@@ -378,7 +402,7 @@ private:
     template <typename T>
     void setup_fused(std::vector<level_data<T>> &L, dbuf<unsigned char> &dst) {
         fuse_lvl_ = -1;
-        if (!set_.fused_coarse) return;
+        if (!set_.fused_coarse || jpn_ != 0) return;  // the fused cycle has no periodic images
         const int nl = (int)L.size();
         int lt = -1;
         for (int l = 1; l + 1 < nl; ++l)
@@ -529,7 +553,14 @@ private:
         }
         auto &F = L[lvl];
         auto &C = L[lvl + 1];
-        km_.run<kernels::restriction<T>>(C.P, b, C.b);
+        if (jpn_ != 0 && lvl == 0) {
+            // the periodic restriction refreshes the images of its input in place; the caller's
+            // b is restricted through a copy (same values, b left untouched)
+            k2::copy<T>(b.p, F.res.p, F.ext.total(), cur_);
+            km_.run<kernels::restriction<T>>(C.P, F.res, C.b);
+        } else {
+            km_.run<kernels::restriction<T>>(C.P, b, C.b);
+        }
         fmg<T>(L, lvl + 1, C.x, C.b);
         k2::fill<T>(x.p, T(0), F.ext.total(), cur_);
         k2::fill<T>(F.res.p, T(0), F.ext.total(), cur_);

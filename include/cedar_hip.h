@@ -172,6 +172,54 @@ int cdr_hip_solve_cg2d_f32(float *q, const float *qf, const double *abd, double 
                            int jj, void *stream);
 
 /* ---------------------------------------------------------------------------------------
+ * Periodic boundaries (grid.periodic, include/cedar/kernel_params.h:35-42).  jpn is the BoxMG
+ * boundary id BMG_get_bc returns for the periodic mask (src/2d/ftn/BMG_get_bc.f90:1-24): 0
+ * Dirichlet (the kernels above), 2 periodic in x, 1 in y, 3 in both.  The periodic images live
+ * in the ghost layer: a periodic direction of n points has period n, and the ghosts of the
+ * operator, of q and of the right-hand side hold the values of the opposite interior points.
+ * Each entry point is the kernel above with the reference's JPN != 0 branch:
+ * ------------------------------------------------------------------------------------- */
+/* src/2d/ftn/BMG2_SymStd_relax_GS.f90:139-226: images refreshed after each colour (x) and after
+ * the sweep (y) */
+int cdr_hip_relax_gs2d_per_f64(const double *so, int nstncl, const double *qf, double *q,
+                               const double *sor, int ii, int jj, int updown, int jpn,
+                               void *stream);
+int cdr_hip_relax_gs2d_per_f32(const float *so, int nstncl, const float *qf, float *q,
+                               const float *sor, int ii, int jj, int updown, int jpn,
+                               void *stream);
+/* src/2d/ftn/BMG2_SymStd_restrict.f90:94-126: when the fine extent is even, the images of q are
+ * refreshed in place before restricting (q is written) */
+int cdr_hip_restrict2d_per_f64(const double *ci, double *q, double *qc, int iif, int jjf, int iic,
+                               int jjc, int jpn, void *stream);
+int cdr_hip_restrict2d_per_f32(const float *ci, float *q, float *qc, int iif, int jjf, int iic,
+                               int jjc, int jpn, void *stream);
+/* src/2d/ftn/BMG2_SymStd_interp_add.f90:139-157: images of q refreshed afterwards */
+int cdr_hip_interp_add2d_per_f64(double *q, const double *qc, const double *res, const double *so,
+                                 int nstncl, const double *ci, int iic, int jjc, int iif, int jjf,
+                                 int jpn, void *stream);
+int cdr_hip_interp_add2d_per_f32(float *q, const float *qc, const float *res, const float *so,
+                                 int nstncl, const float *ci, int iic, int jjc, int iif, int jjf,
+                                 int jpn, void *stream);
+/* src/2d/ftn/BMG2_SymStd_SETUP_interp_OI.f90:259-616 (weights also in the ghost rows/columns of
+ * CI that the periodic interpolation reads; ci must be zero on entry) */
+int cdr_hip_setup_interp2d_per_f64(const double *so, int nstncl, double *ci, int iif, int jjf,
+                                   int iic, int jjc, int jpn, void *stream);
+/* src/2d/ftn/BMG2_SymStd_SETUP_ITLI_ex.f90:94-363: the hand-expanded products, then the images
+ * of the coarse operator */
+int cdr_hip_setup_galerkin2d_per_f64(const double *so, int nstncl, const double *ci, double *soc,
+                                     int iif, int jjf, int iic, int jjc, int jpn, void *stream);
+/* src/2d/ftn/BMG2_SymStd_SETUP_cg_LU.f90:119-227,251-335 and BMG2_SymStd_SOLVE_cg.f90:93-163:
+ * the dense upper triangle abd(n, n), n = nx*ny (2 <= nx, ny; n <= 2048), factored by an upper
+ * Cholesky (DPOTF2 order; CDR_ERR_NOT_SPD for a non-positive pivot); the solve removes the mean
+ * of the coarse solution and refreshes its images.  bbd holds n doubles.  Synchronises (setup). */
+int cdr_hip_setup_cg2d_per_f64(const double *so, int nstncl, double *abd, int ii, int jj, int jpn,
+                               int *info_dev, void *stream);
+int cdr_hip_solve_cg2d_per_f64(double *q, const double *qf, const double *abd, double *bbd, int ii,
+                               int jj, int jpn, void *stream);
+int cdr_hip_solve_cg2d_per_f32(float *q, const float *qf, const double *abd, double *bbd, int ii,
+                               int jj, int jpn, void *stream);
+
+/* ---------------------------------------------------------------------------------------
  * 2D solver (cdr2::solver<five_pt|nine_pt>: setup in create, solve(b, x))
  * ------------------------------------------------------------------------------------- */
 enum {
@@ -209,6 +257,10 @@ typedef struct {
     int fused_coarse; /* 1 (default): in V-cycles, the levels of at most 32 points per side (below
                          the finest) run as one fused launch of one workgroup per problem; the
                          result is bitwise that of one launch per step.  0: one launch per step */
+    int periodic;     /* grid.periodic as Cedar's per_mask (include/cedar/kernel_params.h:35-42):
+                         bit 0 periodic in x, bit 1 in y; default 0 (Dirichlet).  Needs point
+                         relaxation; the operator, b and x carry the periodic images in their
+                         ghosts (the periodic kernels above); fused_coarse does not apply */
 } cdr_solver_settings;
 
 typedef struct cdr_hip_solver2d cdr_hip_solver2d;
