@@ -15,6 +15,7 @@ from .cdr2 import (  # noqa: E402,F401
     GridFunc, StencilOp, Solver, residual, residual_norm, setup_recip, relax_gs, setup_lines,
     relax_lines, relax_lines_xy, restrict, interp_add, setup_interp, setup_galerkin, setup_cg, solve_cg,
     write_stencil, autotune, autotune_report, bc_from_mask, setup_cg_periodic, solve_cg_periodic,
+    setup_lines_periodic, relax_lines_periodic,
 )
 from . import cdr3  # noqa: E402,F401
 from .cdr3 import Solver3  # noqa: E402,F401

@@ -119,6 +119,12 @@ PROTOTYPES = {
     # periodic boundaries
     "cdr_hip_relax_gs2d_per_f64": (_i, [_P, _i, _P, _P, _P, _i, _i, _i, _i, _vp]),
     "cdr_hip_relax_gs2d_per_f32": (_i, [_P, _i, _P, _P, _P, _i, _i, _i, _i, _vp]),
+    "cdr_hip_setup_lines_x2d_per_f64": (_i, [_P, _P, _i, _i, _i, _i, _P, _vp]),
+    "cdr_hip_setup_lines_y2d_per_f64": (_i, [_P, _P, _i, _i, _i, _i, _P, _vp]),
+    "cdr_hip_relax_lines_x2d_per_f64": (_i, [_P, _i, _P, _P, _P, _P, _i, _i, _i, _i, _vp]),
+    "cdr_hip_relax_lines_y2d_per_f64": (_i, [_P, _i, _P, _P, _P, _P, _i, _i, _i, _i, _vp]),
+    "cdr_hip_relax_lines_x2d_per_f32": (_i, [_P, _i, _P, _P, _P, _P, _i, _i, _i, _i, _vp]),
+    "cdr_hip_relax_lines_y2d_per_f32": (_i, [_P, _i, _P, _P, _P, _P, _i, _i, _i, _i, _vp]),
     "cdr_hip_restrict2d_per_f64": (_i, [_P, _P, _P, _i, _i, _i, _i, _i, _vp]),
     "cdr_hip_restrict2d_per_f32": (_i, [_P, _P, _P, _i, _i, _i, _i, _i, _vp]),
     "cdr_hip_interp_add2d_per_f64": (_i, [_P, _P, _P, _P, _i, _P, _i, _i, _i, _i, _i, _vp]),

@@ -146,6 +146,23 @@ def setup_lines(so, sor, direction):
     return sor
 
 
+def setup_lines_periodic(so, sor, direction, jpn):
+    so, sor = _t(so), _t(sor)
+    info = torch.zeros(1, dtype=torch.int32, device=so.device)
+    _abi.call(f"cdr_hip_setup_lines_{direction}2d_per_f64", _p(so), _p(sor), so.shape[2],
+              so.shape[1], so.shape[0], int(jpn), _p(info), _stream())
+    return sor
+
+
+def relax_lines_periodic(so, qf, q, sor, updown, direction, jpn):
+    """One periodic zebra line sweep (work: an internal scratch grid)."""
+    so, qf, q, sor = map(_t, (so, qf, q, sor))
+    work = torch.zeros_like(q)
+    _abi.call(f"cdr_hip_relax_lines_{direction}2d_per_{_sfx(q)}", _p(so), so.shape[0], _p(qf),
+              _p(q), _p(sor), _p(work), q.shape[1], q.shape[0], int(updown), int(jpn), _stream())
+    return q
+
+
 def relax_lines(so, qf, q, sor, updown, direction):
     so, qf, q, sor = map(_t, (so, qf, q, sor))
     _abi.call(f"cdr_hip_relax_lines_{direction}2d_{_sfx(q)}", _p(so), so.shape[0], _p(qf), _p(q),
@@ -267,8 +284,8 @@ class Solver:
     num_levels=-1, maxiter=10, tol=1e-8, min_coarse=3; precision 'f64' (Cedar's algorithm) or
     'mixed' (FP32 V-cycle preconditioning FP64 iterative refinement); graph=True replays each
     V-cycle from a captured hipGraph; cycle 'v' or 'f' (solver.cycle.type); periodic = (x, y)
-    flags of grid.periodic (point relaxation; the operator, b and x hold the periodic images in
-    their ghosts, as in examples/basic-2d-ser/periodic.cc).
+    flags of grid.periodic (the operator, b and x hold the periodic images in their ghosts, as in
+    examples/basic-2d-ser/periodic.cc).
 
     A 4-D operator (nb, nstncl, ny+2, nx+2) makes a batched solver: nb independent problems of
     the same extents solved together (every kernel launch covers the batch); solve() then takes

@@ -258,8 +258,10 @@ __global__ __launch_bounds__(256) void gs_colour_k(const T *__restrict__ so, con
 // x factors: sor[0][p] = D, sor[1][p] = E at (i, j) as in the reference; y factors stored
 // transposed: plane k element (j, i) at k*ii*jj + i*jj + j.
 // ------------------------------------------------------------------------------------------
+// per: the line direction is periodic, so its wrap couplings are added to the end diagonals
+// (src/2d/ftn/BMG2_SymStd_SETUP_lines_x.f90:75-82, _y.f90:77-84) before the factorisation
 __global__ void setup_lines_x_k(const double *__restrict__ so, double *__restrict__ sor, int ii,
-                                int jj, int ns, int *__restrict__ info) {
+                                int jj, int ns, int *__restrict__ info, int per = 0) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
     if (j > jj - 2) return;
     const std::size_t P = (std::size_t)ii * jj;
@@ -272,6 +274,10 @@ __global__ void setup_lines_x_k(const double *__restrict__ so, double *__restric
         const std::size_t p = pidx(i, j, ii);
         sor[P + p] = -so[P * KW + p];
         sor[p] = so[p];
+    }
+    if (per) {
+        D[0] = D[0] + so[P * KW + pidx(1, j, ii)];
+        D[n - 1] = D[n - 1] + so[P * KW + pidx(ii - 1, j, ii)];
     }
     for (int k = 0; k < n - 1; ++k) {
         const double d = D[k];
@@ -287,7 +293,7 @@ __global__ void setup_lines_x_k(const double *__restrict__ so, double *__restric
 }
 
 __global__ void setup_lines_y_k(const double *__restrict__ so, double *__restrict__ sor, int ii,
-                                int jj, int ns, int *__restrict__ info) {
+                                int jj, int ns, int *__restrict__ info, int per = 0) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     if (i > ii - 2) return;
     const std::size_t P = (std::size_t)ii * jj;
@@ -300,6 +306,10 @@ __global__ void setup_lines_y_k(const double *__restrict__ so, double *__restric
         const std::size_t p = pidx(i, j, ii);
         sor[P + (std::size_t)i * jj + j] = -so[P * KS + p];
         sor[(std::size_t)i * jj + j] = so[p];
+    }
+    if (per) {
+        D[0] = D[0] + so[P * KS + pidx(i, 1, ii)];
+        D[n - 1] = D[n - 1] + so[P * KS + pidx(i, jj - 1, ii)];
     }
     for (int k = 0; k < n - 1; ++k) {
         const double d = D[k];
@@ -2677,6 +2687,70 @@ __global__ void solve_cg_per_k(T *__restrict__ q, const T *__restrict__ qf,
     }
 }
 
+// ---- periodic line relaxation: one lane per line, the reference's sequential DPTTRS order ----
+// (src/2d/ftn/BMG2_SymStd_relax_lines_x.f90:170-300, _y.f90:176-310).  The right-hand side is
+// formed in place (it reads only the neighbouring lines), solved with the line's LDL^T factors,
+// and, when the line direction is periodic (sm), corrected by Sherman-Morrison with the solve
+// z = T^-1 B of the wrap couplings, kept in the scratch z at the line's positions.
+template <typename T, bool NINE, int DIR>
+__global__ __launch_bounds__(64) void relax_lines_per_k(const T *__restrict__ so,
+                                                        const T *__restrict__ qf, T *__restrict__ q,
+                                                        const T *__restrict__ sor, T *__restrict__ z,
+                                                        int ii, int jj, int first, int sm) {
+    const int L = first + 2 * (int)(blockIdx.x * 64 + threadIdx.x);
+    if (L > (DIR == 0 ? jj - 2 : ii - 2)) return;
+    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.y;
+    so += bz * (NINE ? 5 : 3) * P;
+    qf += bz * P;
+    q += bz * P;
+    z += bz * P;
+    sor += bz * 2 * P;
+    const int n = DIR == 0 ? ii - 2 : jj - 2;
+    auto pos = [&](int k) { return DIR == 0 ? pidx(k + 1, L, ii) : pidx(L, k + 1, ii); };
+    auto Dk = [&](int k) { return DIR == 0 ? sor[pidx(k + 1, L, ii)] : sor[(std::size_t)L * jj + k + 1]; };
+    auto Ek = [&](int k) {
+        return DIR == 0 ? sor[P + pidx(k + 2, L, ii)] : sor[P + (std::size_t)L * jj + k + 2];
+    };
+    for (int k = 0; k < n; ++k) {
+        const int i = DIR == 0 ? k + 1 : L, j = DIR == 0 ? L : k + 1;
+        const std::size_t p = pidx(i, j, ii);
+        T v;
+        if (DIR == 0) {
+            v = qf[p] + so[P * KS + p] * q[p - ii] + so[P * KS + p + ii] * q[p + ii];
+        } else {
+            v = qf[p] + so[P * KW + p] * q[p - 1] + so[P * KW + p + 1] * q[p + 1];
+        }
+        if (NINE)
+            v = v + so[P * KSW + p] * q[p - ii - 1] + so[P * KNW + p + 1] * q[p - ii + 1] +
+                so[P * KNW + p + ii] * q[p + ii - 1] + so[P * KSW + p + ii + 1] * q[p + ii + 1];
+        q[p] = v;
+    }
+    auto dpttrs = [&](T *b) {
+        if (n <= 1) {
+            b[pos(0)] = b[pos(0)] * (T(1) / Dk(0));
+            return;
+        }
+        for (int k = 1; k < n; ++k) b[pos(k)] = b[pos(k)] - b[pos(k - 1)] * Ek(k - 1);
+        b[pos(n - 1)] = b[pos(n - 1)] / Dk(n - 1);
+        for (int k = n - 2; k >= 0; --k) b[pos(k)] = b[pos(k)] / Dk(k) - b[pos(k + 1)] * Ek(k);
+    };
+    dpttrs(q);
+    if (!sm) return;
+    for (int k = 0; k < n; ++k) z[pos(k)] = T(0);
+    if (DIR == 0) {
+        z[pos(0)] = -so[P * KW + pidx(1, L, ii)];
+        z[pos(n - 1)] = -so[P * KW + pidx(ii - 1, L, ii)];
+    } else {
+        z[pos(0)] = -so[P * KS + pidx(L, 1, ii)];
+        z[pos(n - 1)] = -so[P * KS + pidx(L, jj - 1, ii)];
+    }
+    dpttrs(z);
+    const T alpha = z[pos(0)] + z[pos(n - 1)];
+    T beta = q[pos(0)] + q[pos(n - 1)];
+    beta = beta / (T(1) + alpha);
+    for (int k = 0; k < n; ++k) q[pos(k)] = q[pos(k)] - beta * z[pos(k)];
+}
+
 }  // namespace
 
 template <typename T>
@@ -2702,6 +2776,53 @@ void relax_gs_per(const T *so, int nstncl, const T *qf, T *q, const T *sor, int 
         }
     }
     if (per_y(jpn)) ghost_y<T>(q, ii, jj, 0, ii - 1, s, nb);
+}
+
+template <typename T>
+void relax_lines_per(int dir, const T *so, int nstncl, const T *qf, T *q, const T *sor, T *work,
+                     int ii, int jj, int updown, int jpn, hipStream_t s, int nb) {
+    // DOWN relaxes the lines of odd Fortran index 3, 5, ... (C index 2, 4, ...) first
+    const bool sm = dir == 0 ? per_x(jpn) : per_y(jpn);
+    const int nl = dir == 0 ? jj - 2 : ii - 2;
+    for (int pass = 0; pass < 2; ++pass) {
+        const int first = (pass == 0) == (updown == 0) ? 2 : 1;
+        const int cnt = first > nl ? 0 : (nl - first) / 2 + 1;
+        if (cnt > 0) {
+            const dim3 g((unsigned)((cnt + 63) / 64), (unsigned)nb);
+            if (dir == 0) {
+                if (nstncl == 5)
+                    relax_lines_per_k<T, true, 0><<<g, 64, 0, s>>>(so, qf, q, sor, work, ii, jj, first, sm);
+                else
+                    relax_lines_per_k<T, false, 0><<<g, 64, 0, s>>>(so, qf, q, sor, work, ii, jj, first, sm);
+            } else {
+                if (nstncl == 5)
+                    relax_lines_per_k<T, true, 1><<<g, 64, 0, s>>>(so, qf, q, sor, work, ii, jj, first, sm);
+                else
+                    relax_lines_per_k<T, false, 1><<<g, 64, 0, s>>>(so, qf, q, sor, work, ii, jj, first, sm);
+            }
+        }
+        if (sm) {  // images refreshed after each colour, y first
+            if (per_y(jpn)) ghost_y<T>(q, ii, jj, 0, ii - 1, s, nb);
+            if (per_x(jpn)) ghost_x<T>(q, ii, jj, 0, jj - 1, 1, s, nb);
+        }
+    }
+    if (!sm) {  // lines across the periodic direction: its images after the sweep
+        if (per_y(jpn)) ghost_y<T>(q, ii, jj, 0, ii - 1, s, nb);
+        if (per_x(jpn)) ghost_x<T>(q, ii, jj, 0, jj - 1, 1, s, nb);
+    }
+}
+
+void setup_lines_per(int dir, const double *so, int nstncl, double *sor, int ii, int jj, int jpn,
+                     int *info, hipStream_t s, int nb) {
+    if (dir == 0) {
+        const int n = jj - 2;
+        setup_lines_x_k<<<dim3((n + 63) / 64, nb), 64, 0, s>>>(so, sor, ii, jj, nstncl, info,
+                                                               per_x(jpn) ? 1 : 0);
+    } else {
+        const int n = ii - 2;
+        setup_lines_y_k<<<dim3((n + 63) / 64, nb), 64, 0, s>>>(so, sor, ii, jj, nstncl, info,
+                                                               per_y(jpn) ? 1 : 0);
+    }
 }
 
 template <typename T>
@@ -2841,6 +2962,8 @@ void copy(const T *src, T *dst, std::size_t n, hipStream_t s) {
     template void solve_cg_per<T>(T *, const T *, const double *, double *, int, int, int,        \
                                   hipStream_t, int);                                              \
     template void copy<T>(const T *, T *, std::size_t, hipStream_t);                              \
+    template void relax_lines_per<T>(int, const T *, int, const T *, T *, const T *, T *, int, int, \
+                                     int, int, hipStream_t, int);                                 \
     template void periodic_images<T>(T *, int, int, int, hipStream_t, int);
 CDR_INST(double)
 CDR_INST(float)

@@ -168,6 +168,14 @@ void solve_cg(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj,
 template <typename T>
 void relax_gs_per(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
                   int updown, int jpn, hipStream_t s, int nb = 1);
+// lines (dir 0 x, 1 y): a periodic line direction gets the wrap couplings on its end diagonals
+// and a Sherman-Morrison line solve (work: scratch grid of ii*jj per problem); lines across a
+// periodic direction refresh its images after the sweep
+void setup_lines_per(int dir, const double *so, int nstncl, double *sor, int ii, int jj, int jpn,
+                     int *info, hipStream_t s, int nb = 1);
+template <typename T>
+void relax_lines_per(int dir, const T *so, int nstncl, const T *qf, T *q, const T *sor, T *work,
+                     int ii, int jj, int updown, int jpn, hipStream_t s, int nb = 1);
 // refreshes the images of q in place first when the fine extent is even (as the reference does)
 template <typename T>
 void restrict_per(const T *ci, T *q, T *qc, int iif, int jjf, int iic, int jjc, int jpn,

@@ -230,6 +230,26 @@ int setup_galerkin_per(const double *so, int ns, const double *ci, double *soc, 
     return launched();
 }
 
+int setup_lines_per(int dir, const double *so, double *sor, int ii, int jj, int ns, int jpn,
+                    int *info_dev, hipStream_t s, int nb) {
+    if (!so || !sor || !ext_ok(ii, jj) || !sten_ok(ns) || !nb_ok(nb) || (dir != 0 && dir != 1) ||
+        !jpn_ok(jpn))
+        return CDR_ERR_ARG;
+    return with_info(info_dev, s, [&] {
+        k2::setup_lines_per(dir, so, ns, sor, ii, jj, jpn, info_dev, s, nb);
+    });
+}
+
+template <typename T>
+int relax_lines_per(int dir, const T *so, int ns, const T *qf, T *q, const T *sor, T *work, int ii,
+                    int jj, int ud, int jpn, hipStream_t s, int nb) {
+    if (!so || !qf || !q || !sor || !work || !ext_ok(ii, jj) || !sten_ok(ns) || !dir_ok(ud) ||
+        !nb_ok(nb) || (dir != 0 && dir != 1) || !jpn_ok(jpn))
+        return CDR_ERR_ARG;
+    k2::relax_lines_per<T>(dir, so, ns, qf, q, sor, work, ii, jj, ud, jpn, s, nb);
+    return launched();
+}
+
 // the dense periodic matrix needs at least 2 x 2 points (all of its writes then stay inside it)
 inline bool per_cg_ok(int ii, int jj) {
     return ii >= 4 && jj >= 4 && (long long)(ii - 2) * (jj - 2) <= 2048;
@@ -263,6 +283,8 @@ int solve_cg_per(T *q, const T *qf, const double *abd, double *bbd, int ii, int 
                                    int, int, int, hipStream_t, int);                               \
     template int solve_cg_per<T>(T *, const T *, const double *, double *, int, int, int,          \
                                  hipStream_t, int);                                                \
+    template int relax_lines_per<T>(int, const T *, int, const T *, T *, const T *, T *, int, int, \
+                                    int, int, hipStream_t, int);                                   \
     template int residual<T>(const T *, int, const T *, const T *, T *, int, int, hipStream_t,     \
                              int);                                                                 \
     template int relax_gs<T>(const T *, int, const T *, T *, const T *, int, int, int, hipStream_t, \
@@ -460,6 +482,34 @@ int cdr_hip_relax_gs2d_per_f64(const double *so, int ns, const double *qf, doubl
 int cdr_hip_relax_gs2d_per_f32(const float *so, int ns, const float *qf, float *q,
                                const float *sor, int ii, int jj, int ud, int jpn, void *st) {
     return api2::relax_gs_per<float>(so, ns, qf, q, sor, ii, jj, ud, jpn, S(st));
+}
+int cdr_hip_setup_lines_x2d_per_f64(const double *so, double *sor, int ii, int jj, int ns, int jpn,
+                                    int *info_dev, void *st) {
+    return api2::setup_lines_per(0, so, sor, ii, jj, ns, jpn, info_dev, S(st));
+}
+int cdr_hip_setup_lines_y2d_per_f64(const double *so, double *sor, int ii, int jj, int ns, int jpn,
+                                    int *info_dev, void *st) {
+    return api2::setup_lines_per(1, so, sor, ii, jj, ns, jpn, info_dev, S(st));
+}
+int cdr_hip_relax_lines_x2d_per_f64(const double *so, int ns, const double *qf, double *q,
+                                    const double *sor, double *work, int ii, int jj, int ud,
+                                    int jpn, void *st) {
+    return api2::relax_lines_per<double>(0, so, ns, qf, q, sor, work, ii, jj, ud, jpn, S(st));
+}
+int cdr_hip_relax_lines_y2d_per_f64(const double *so, int ns, const double *qf, double *q,
+                                    const double *sor, double *work, int ii, int jj, int ud,
+                                    int jpn, void *st) {
+    return api2::relax_lines_per<double>(1, so, ns, qf, q, sor, work, ii, jj, ud, jpn, S(st));
+}
+int cdr_hip_relax_lines_x2d_per_f32(const float *so, int ns, const float *qf, float *q,
+                                    const float *sor, float *work, int ii, int jj, int ud, int jpn,
+                                    void *st) {
+    return api2::relax_lines_per<float>(0, so, ns, qf, q, sor, work, ii, jj, ud, jpn, S(st));
+}
+int cdr_hip_relax_lines_y2d_per_f32(const float *so, int ns, const float *qf, float *q,
+                                    const float *sor, float *work, int ii, int jj, int ud, int jpn,
+                                    void *st) {
+    return api2::relax_lines_per<float>(1, so, ns, qf, q, sor, work, ii, jj, ud, jpn, S(st));
 }
 int cdr_hip_restrict2d_per_f64(const double *ci, double *q, double *qc, int iif, int jjf, int iic,
                                int jjc, int jpn, void *st) {

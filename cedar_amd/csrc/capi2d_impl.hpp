@@ -65,6 +65,12 @@ int setup_galerkin_per(const double *so, int ns, const double *ci, double *soc, 
                        int iic, int jjc, int jpn, hipStream_t s, int nb = 1);
 int setup_cg_per(const double *so, int ns, double *abd, int ii, int jj, int jpn, int *info_dev,
                  hipStream_t s, int nb = 1);
+// lines (dir 0 x, 1 y); work: scratch grid, ii*jj values per problem
+int setup_lines_per(int dir, const double *so, double *sor, int ii, int jj, int ns, int jpn,
+                    int *info_dev, hipStream_t s, int nb = 1);
+template <typename T>
+int relax_lines_per(int dir, const T *so, int ns, const T *qf, T *q, const T *sor, T *work, int ii,
+                    int jj, int ud, int jpn, hipStream_t s, int nb = 1);
 template <typename T>
 int solve_cg_per(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj, int jpn,
                  hipStream_t s, int nb = 1);

@@ -975,8 +975,8 @@ int cedar_solver_create(cedar_mat mat, cedar_solver *solver) {
         return CEDAR_ERR_CONFIG;
     }
     const int pmask = periodic_mask(root);
-    if (pmask != 0 && (m->nd != 2 || (pmask & 4) || k.relaxation != CDR_RELAX_POINT)) {
-        log_line(LOG_ERROR, "grid.periodic: periodic boundaries are built for 2D point relaxation");
+    if (pmask != 0 && (m->nd != 2 || (pmask & 4))) {
+        log_line(LOG_ERROR, "grid.periodic: periodic boundaries are built for 2D problems");
         return CEDAR_ERR_CONFIG;
     }
     if (get_or(root, "gpu.autotune", false, as_bool)) cdr_hip_autotune(1);

@@ -53,12 +53,19 @@ struct hip_lines : kernels::line_relax<T, RD> {
     static constexpr int dir = RD == relax_dir::x ? 0 : 1;
     explicit hip_lines(int *info) : info_(info) {}
     void setup(const stencil_op<double> &so, relax_stencil<double> &sor) override {
-        check(api2::setup_lines(dir, so.p, sor.p, so.ext.ii, so.ext.jj, so.nstncl, info_, this->stream, so.ext.nb), "setup_lines");
+        if (this->jpn)
+            check(api2::setup_lines_per(dir, so.p, sor.p, so.ext.ii, so.ext.jj, so.nstncl, this->jpn, info_, this->stream, so.ext.nb), "setup_lines");
+        else
+            check(api2::setup_lines(dir, so.p, sor.p, so.ext.ii, so.ext.jj, so.nstncl, info_, this->stream, so.ext.nb), "setup_lines");
     }
     void run(const stencil_op<T> &so, grid_func<T> &x, const grid_func<T> &b,
              const relax_stencil<T> &sor, grid_func<T> &res, Dir d) override {
-        (void)res;  // the reference's scratch; the GPU sweep keeps its intermediates on chip
-        check(api2::relax_lines<T>(dir, so.p, so.nstncl, b.p, x.p, sor.p, so.ext.ii, so.ext.jj, (int)d, this->stream, so.ext.nb), "relax_lines");
+        // res is the reference's line scratch; the scan kernels keep their intermediates on chip,
+        // the periodic kernel keeps the Sherman-Morrison vector there
+        if (this->jpn)
+            check(api2::relax_lines_per<T>(dir, so.p, so.nstncl, b.p, x.p, sor.p, res.p, so.ext.ii, so.ext.jj, (int)d, this->jpn, this->stream, so.ext.nb), "relax_lines");
+        else
+            check(api2::relax_lines<T>(dir, so.p, so.nstncl, b.p, x.p, sor.p, so.ext.ii, so.ext.jj, (int)d, this->stream, so.ext.nb), "relax_lines");
     }
 
 private:
@@ -210,8 +217,7 @@ int cdr_hip_solver2d_create_batched(const double *so_dev, int nstncl, int nx, in
         s->nrelax_pre < 0 || s->nrelax_post < 0 || s->maxiter < 0 || s->min_coarse < 2 ||
         (s->precision != CDR_PREC_F64 && s->precision != CDR_PREC_MIXED) ||
         (s->graph != 0 && s->graph != 1) || (s->cycle != CDR_CYCLE_V && s->cycle != CDR_CYCLE_F) ||
-        (s->fused_coarse != 0 && s->fused_coarse != 1) || s->periodic < 0 || s->periodic > 3 ||
-        (s->periodic != 0 && s->relaxation != CDR_RELAX_POINT))
+        (s->fused_coarse != 0 && s->fused_coarse != 1) || s->periodic < 0 || s->periodic > 3)
         return CDR_ERR_ARG;
     *out = nullptr;
     int ndev = 0;

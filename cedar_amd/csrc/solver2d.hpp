@@ -60,8 +60,6 @@ public:
         // BMG_get_bc (src/2d/ftn/BMG_get_bc.f90): per_mask 1 (x) -> 2, 2 (y) -> 1, 3 -> 3
         static const int bc_of_mask[4] = {0, 2, 1, 3};
         if (s.periodic < 0 || s.periodic > 3) throw error(CDR_ERR_ARG, "invalid periodic mask");
-        if (s.periodic != 0 && s.relaxation != CDR_RELAX_POINT)
-            throw error(CDR_ERR_ARG, "periodic boundaries need point relaxation");
         jpn_ = bc_of_mask[s.periodic];
         km_.set_bc(jpn_);
         build(so_dev, nstncl, nx, ny, nb);
@@ -432,6 +430,7 @@ private:
     void tune_lines(std::vector<level_data<T>> &L) {
         const int r = set_.relaxation;
         if (r != CDR_RELAX_LINE_X && r != CDR_RELAX_LINE_Y && r != CDR_RELAX_LINE_XY) return;
+        if (jpn_ != 0) return;  // periodic lines run one lane per line, nothing to tune
         const std::size_t np = L[0].ext.total();
         dbuf<T> q(np, stream_), f(np, stream_);
         k2::fill<T>(q.data(), T(0), np, stream_);
@@ -514,7 +513,12 @@ private:
                 // 204-207); the y sweep runs through the transposed grid where that was faster
                 for (int half = 0; half < 2; ++half) {
                     const bool xs = (half == 0) == (d == Dir::DOWN);
-                    if (xs) {
+                    if (jpn_ != 0) {  // periodic: one lane per line, L.res as line scratch
+                        check(api2::relax_lines_per<T>(xs ? 0 : 1, L.A.p, L.A.nstncl, b.p, x.p,
+                                                       L.SOR[xs ? 0 : 1].p, L.res.p, L.ext.ii,
+                                                       L.ext.jj, (int)d, jpn_, cur_, L.ext.nb),
+                              "relax_lines_per");
+                    } else if (xs) {
                         check(api2::relax_lines<T>(0, L.A.p, L.A.nstncl, b.p, x.p, L.SOR[0].p,
                                                    L.ext.ii, L.ext.jj, (int)d, cur_, L.ext.nb),
                               "relax_lines_x");

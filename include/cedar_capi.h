@@ -147,9 +147,8 @@ int cedar_mat_free(cedar_mat *mat);
  * "cedar" and "redist" name MPI coarse solvers and run the same band Cholesky on one GPU), and for
  * 3D plane relaxation the "plane-config" subtree (default line-xy, max-iter 1).  GPU keys, not in
  * the reference: gpu.precision ("f64" default, or "mixed"), gpu.graph (true), gpu.fused-coarse
- * (true), gpu.autotune (false).  grid.periodic (Cedar's per_mask) is honoured for 2D point
- * relaxation; elsewhere a true entry gives CEDAR_ERR_CONFIG, as do unknown relaxation or cycle
- * names.
+ * (true), gpu.autotune (false).  grid.periodic (Cedar's per_mask) is honoured for 2D problems;
+ * for 3D a true entry gives CEDAR_ERR_CONFIG, as do unknown relaxation or cycle names.
  * cedar_solver_run solves from the x given (multilevel::solve(b, x),
  * include/cedar/multilevel.h:278-298) and logs "Iteration i relative l2 norm: ..." at the status
  * level; CEDAR_ERR_OTHER when the GPU reports an error. */

@@ -187,6 +187,27 @@ int cdr_hip_relax_gs2d_per_f64(const double *so, int nstncl, const double *qf, d
 int cdr_hip_relax_gs2d_per_f32(const float *so, int nstncl, const float *qf, float *q,
                                const float *sor, int ii, int jj, int updown, int jpn,
                                void *stream);
+/* Lines (src/2d/ftn/BMG2_SymStd_SETUP_lines_{x,y}.f90 and BMG2_SymStd_relax_lines_{x,y}.f90,
+ * JPN != 0): a periodic line direction gets its wrap couplings on the end diagonals of the
+ * factors and a Sherman-Morrison line solve (BETA/ALPHA correction); lines across a periodic
+ * direction run the Dirichlet solve and refresh that direction's images after the sweep.  One
+ * lane per line in the reference's sequential DPTTRS order.  work: device scratch of ii*jj. */
+int cdr_hip_setup_lines_x2d_per_f64(const double *so, double *sor, int ii, int jj, int nstncl,
+                                    int jpn, int *info_dev, void *stream);
+int cdr_hip_setup_lines_y2d_per_f64(const double *so, double *sor, int ii, int jj, int nstncl,
+                                    int jpn, int *info_dev, void *stream);
+int cdr_hip_relax_lines_x2d_per_f64(const double *so, int nstncl, const double *qf, double *q,
+                                    const double *sor, double *work, int ii, int jj, int updown,
+                                    int jpn, void *stream);
+int cdr_hip_relax_lines_y2d_per_f64(const double *so, int nstncl, const double *qf, double *q,
+                                    const double *sor, double *work, int ii, int jj, int updown,
+                                    int jpn, void *stream);
+int cdr_hip_relax_lines_x2d_per_f32(const float *so, int nstncl, const float *qf, float *q,
+                                    const float *sor, float *work, int ii, int jj, int updown,
+                                    int jpn, void *stream);
+int cdr_hip_relax_lines_y2d_per_f32(const float *so, int nstncl, const float *qf, float *q,
+                                    const float *sor, float *work, int ii, int jj, int updown,
+                                    int jpn, void *stream);
 /* src/2d/ftn/BMG2_SymStd_restrict.f90:94-126: when the fine extent is even, the images of q are
  * refreshed in place before restricting (q is written) */
 int cdr_hip_restrict2d_per_f64(const double *ci, double *q, double *qc, int iif, int jjf, int iic,
@@ -258,9 +279,10 @@ typedef struct {
                          the finest) run as one fused launch of one workgroup per problem; the
                          result is bitwise that of one launch per step.  0: one launch per step */
     int periodic;     /* grid.periodic as Cedar's per_mask (include/cedar/kernel_params.h:35-42):
-                         bit 0 periodic in x, bit 1 in y; default 0 (Dirichlet).  Needs point
-                         relaxation; the operator, b and x carry the periodic images in their
-                         ghosts (the periodic kernels above); fused_coarse does not apply */
+                         bit 0 periodic in x, bit 1 in y; default 0 (Dirichlet).  The operator, b
+                         and x carry the periodic images in their ghosts (the periodic kernels
+                         above); line relaxation then runs one lane per line, and fused_coarse
+                         and line-shape tuning do not apply */
 } cdr_solver_settings;
 
 typedef struct cdr_hip_solver2d cdr_hip_solver2d;

@@ -756,18 +756,24 @@ static void orc2_smooth(orc2_solver *slv, orc2_level *L, double *x, const double
             orc2_relax_gs_bc(L->so, L->nstncl, b, x, L->sor[0], L->ii, L->jj, dir, slv->jpn);
             break;
         case ORC_RELAX_LINE_X:
-            orc2_relax_lines_x(L->so, L->nstncl, b, x, L->sor[0], L->ii, L->jj, dir);
+            orc2_relax_lines_x_bc(L->so, L->nstncl, b, x, L->sor[0], slv->scratch, L->ii, L->jj,
+                                  dir, slv->jpn);
             break;
         case ORC_RELAX_LINE_Y:
-            orc2_relax_lines_y(L->so, L->nstncl, b, x, L->sor[0], slv->scratch, L->ii, L->jj, dir);
+            orc2_relax_lines_y_bc(L->so, L->nstncl, b, x, L->sor[0], slv->scratch, L->ii, L->jj,
+                                  dir, slv->jpn);
             break;
         default: /* line-xy: x then y on DOWN, y then x on UP (include/cedar/multilevel.h:175-207) */
             if (dir == ORC_DOWN) {
-                orc2_relax_lines_x(L->so, L->nstncl, b, x, L->sor[0], L->ii, L->jj, dir);
-                orc2_relax_lines_y(L->so, L->nstncl, b, x, L->sor[1], slv->scratch, L->ii, L->jj, dir);
+                orc2_relax_lines_x_bc(L->so, L->nstncl, b, x, L->sor[0], slv->scratch, L->ii,
+                                      L->jj, dir, slv->jpn);
+                orc2_relax_lines_y_bc(L->so, L->nstncl, b, x, L->sor[1], slv->scratch, L->ii,
+                                      L->jj, dir, slv->jpn);
             } else {
-                orc2_relax_lines_y(L->so, L->nstncl, b, x, L->sor[1], slv->scratch, L->ii, L->jj, dir);
-                orc2_relax_lines_x(L->so, L->nstncl, b, x, L->sor[0], L->ii, L->jj, dir);
+                orc2_relax_lines_y_bc(L->so, L->nstncl, b, x, L->sor[1], slv->scratch, L->ii,
+                                      L->jj, dir, slv->jpn);
+                orc2_relax_lines_x_bc(L->so, L->nstncl, b, x, L->sor[0], slv->scratch, L->ii,
+                                      L->jj, dir, slv->jpn);
             }
         }
     }
@@ -777,7 +783,6 @@ orc2_solver *orc2_solver_create(const double *so_fine, int nstncl, int nx, int n
                                 const orc2_settings *s)
 {
     const int jpn = orc2_bc_from_mask(s->periodic);
-    if (jpn != 0 && s->relax != ORC_RELAX_POINT) return NULL; /* periodic lines: not restated */
     orc2_solver *slv = (orc2_solver *)calloc(1, sizeof(orc2_solver));
     slv->s = *s;
     slv->jpn = jpn;
@@ -808,7 +813,7 @@ orc2_solver *orc2_solver_create(const double *so_fine, int nstncl, int nx, int n
         cx = (int)((cx - 1) / 2.0 + 1);
         cy = (int)((cy - 1) / 2.0 + 1);
     }
-    slv->scratch = orc_zeros((size_t)maxjj);
+    slv->scratch = orc_zeros((size_t)(2 * (maxjj > nx + 2 ? maxjj : nx + 2)));
     orc2_level *C = &slv->lv[nl - 1];
     /* include/cedar/2d/solver.h:116-121: a dense n x n array when periodic */
     slv->nabd1 = jpn ? C->nx * C->ny : C->nx + 2;
@@ -823,11 +828,11 @@ orc2_solver *orc2_solver_create(const double *so_fine, int nstncl, int nx, int n
         orc2_setup_galerkin_bc(F->so, F->nstncl, K->ci, K->so, F->ii, F->jj, K->ii, K->jj, jpn);
         switch (s->relax) {
         case ORC_RELAX_POINT: orc2_setup_recip(F->so, F->sor[0], F->ii, F->jj); break;
-        case ORC_RELAX_LINE_X: orc2_setup_lines_x(F->so, F->sor[0], F->ii, F->jj); break;
-        case ORC_RELAX_LINE_Y: orc2_setup_lines_y(F->so, F->sor[0], F->ii, F->jj); break;
+        case ORC_RELAX_LINE_X: orc2_setup_lines_x_bc(F->so, F->sor[0], F->ii, F->jj, jpn); break;
+        case ORC_RELAX_LINE_Y: orc2_setup_lines_y_bc(F->so, F->sor[0], F->ii, F->jj, jpn); break;
         default:
-            orc2_setup_lines_x(F->so, F->sor[0], F->ii, F->jj);
-            orc2_setup_lines_y(F->so, F->sor[1], F->ii, F->jj);
+            orc2_setup_lines_x_bc(F->so, F->sor[0], F->ii, F->jj, jpn);
+            orc2_setup_lines_y_bc(F->so, F->sor[1], F->ii, F->jj, jpn);
         }
     }
     orc2_setup_cg_bc(C->so, C->nstncl, slv->abd, C->ii, C->jj, jpn);

@@ -70,6 +70,13 @@ void orc2_assemble_cg_periodic(const double *so, int nstncl, double *abd, int ii
 int  orc2_setup_cg_bc(const double *so, int nstncl, double *abd, int ii, int jj, int jpn);
 int  orc2_solve_cg_bc(double *q, const double *qf, const double *abd, double *bbd, int ii, int jj,
                       int jpn);
+/* periodic line relaxation; b: scratch of >= 2*max(ii, jj) doubles */
+int  orc2_setup_lines_x_bc(const double *so, double *sor, int ii, int jj, int jpn);
+int  orc2_setup_lines_y_bc(const double *so, double *sor, int ii, int jj, int jpn);
+void orc2_relax_lines_x_bc(const double *so, int nstncl, const double *qf, double *q,
+                           const double *sor, double *b, int ii, int jj, int updown, int jpn);
+void orc2_relax_lines_y_bc(const double *so, int nstncl, const double *qf, double *q,
+                           const double *sor, double *b, int ii, int jj, int updown, int jpn);
 int  orc_dpotrf_u(int n, double *a, int lda);
 void orc_dpotrs_u(int n, const double *a, int lda, double *b);
 

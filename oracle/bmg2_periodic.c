@@ -504,3 +504,143 @@ int orc2_solve_cg_bc(double *q, const double *qf, const double *abd, double *bbd
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* line relaxation, periodic: the factors of a periodic line get the wrap couplings added to   */
+/* their end diagonals (src/2d/ftn/BMG2_SymStd_SETUP_lines_x.f90:75-82, _y.f90:77-84), and a    */
+/* line solve becomes a Sherman-Morrison correction of the tridiagonal one                      */
+/* (src/2d/ftn/BMG2_SymStd_relax_lines_x.f90:170-300, _y.f90:176-310).  Lines across the other  */
+/* direction's periodicity run the Dirichlet branch and refresh that direction's images after   */
+/* the sweep.                                                                                    */
+/* ------------------------------------------------------------------------------------------ */
+int orc2_setup_lines_x_bc(const double *so, double *sor, int ii, int jj, int jpn)
+{
+    if (!per_x(jpn)) return orc2_setup_lines_x(so, sor, ii, jj);
+    int info = 0;
+    for (int j = 2; j <= jj - 1; ++j)
+        for (int i = 2; i <= ii - 1; ++i) {
+            P2(sor, ii, jj, i, j, 2) = -P2(so, ii, jj, i, j, KW);
+            P2(sor, ii, jj, i, j, 1) = P2(so, ii, jj, i, j, KO);
+        }
+    for (int j = 2; j <= jj - 1; ++j) {
+        P2(sor, ii, jj, 2, j, 1) = P2(sor, ii, jj, 2, j, 1) + P2(so, ii, jj, 2, j, KW);
+        P2(sor, ii, jj, ii - 1, j, 1) = P2(sor, ii, jj, ii - 1, j, 1) + P2(so, ii, jj, ii, j, KW);
+    }
+    for (int j = 2; j <= jj - 1; ++j) {
+        int r = orc_dpttrf(ii - 2, &P2(sor, ii, jj, 2, j, 1), &P2(sor, ii, jj, 3, j, 2));
+        if (r && !info) info = r;
+    }
+    return info;
+}
+
+int orc2_setup_lines_y_bc(const double *so, double *sor, int ii, int jj, int jpn)
+{
+    if (!per_y(jpn)) return orc2_setup_lines_y(so, sor, ii, jj);
+    int info = 0;
+    for (int i = 2; i <= ii - 1; ++i)
+        for (int j = 2; j <= jj - 1; ++j) {
+            P2(sor, jj, ii, j, i, 2) = -P2(so, ii, jj, i, j, KS);
+            P2(sor, jj, ii, j, i, 1) = P2(so, ii, jj, i, j, KO);
+        }
+    for (int i = 2; i <= ii - 1; ++i) {
+        P2(sor, jj, ii, 2, i, 1) = P2(sor, jj, ii, 2, i, 1) + P2(so, ii, jj, i, 2, KS);
+        P2(sor, jj, ii, jj - 1, i, 1) = P2(sor, jj, ii, jj - 1, i, 1) + P2(so, ii, jj, i, jj, KS);
+    }
+    for (int i = 2; i <= ii - 1; ++i) {
+        int r = orc_dpttrf(jj - 2, &P2(sor, jj, ii, 2, i, 1), &P2(sor, jj, ii, 3, i, 2));
+        if (r && !info) info = r;
+    }
+    return info;
+}
+
+/* b: scratch of >= 2*max(ii, jj) doubles */
+void orc2_relax_lines_x_bc(const double *so, int nstncl, const double *qf, double *q,
+                           const double *sor, double *b, int ii, int jj, int updown, int jpn)
+{
+    if (!per_x(jpn)) {
+        orc2_relax_lines_x(so, nstncl, qf, q, sor, ii, jj, updown);
+        if (per_y(jpn)) ghost_y(q, ii, jj);
+        return;
+    }
+#define SO(i, j, k) P2(so, ii, jj, i, j, k)
+#define Q(i, j) V2(q, ii, i, j)
+#define QF(i, j) V2(qf, ii, i, j)
+    const int i1 = ii - 1, j1 = jj - 1;
+    const int first = (updown == ORC_DOWN) ? 3 : 2;
+    for (int pass = 0; pass < 2; ++pass) {
+        const int jbeg = (pass == 0) ? first : 5 - first;
+        const int jend = 2 * ((j1 - jbeg) / 2) + jbeg;
+        for (int j = jbeg; j <= jend; j += 2) {
+            for (int i = 2; i <= i1; ++i) {
+                if (nstncl == 5)
+                    Q(i, j) = QF(i, j) + SO(i, j, KS) * Q(i, j - 1) + SO(i, j + 1, KS) * Q(i, j + 1)
+                            + SO(i, j, KSW) * Q(i - 1, j - 1) + SO(i + 1, j, KNW) * Q(i + 1, j - 1)
+                            + SO(i, j + 1, KNW) * Q(i - 1, j + 1)
+                            + SO(i + 1, j + 1, KSW) * Q(i + 1, j + 1);
+                else
+                    Q(i, j) = QF(i, j) + SO(i, j, KS) * Q(i, j - 1) + SO(i, j + 1, KS) * Q(i, j + 1);
+            }
+            const double *d = &P2(sor, ii, jj, 2, j, 1), *e = &P2(sor, ii, jj, 3, j, 2);
+            orc_dpttrs(i1 - 1, d, e, &Q(2, j));
+            for (int i = 2; i <= i1; ++i) b[i - 1] = 0.0;
+            b[1] = -SO(2, j, KW);
+            b[i1 - 1] = -SO(ii, j, KW);
+            orc_dpttrs(i1 - 1, d, e, &b[1]);
+            const double alpha = b[1] + b[i1 - 1];
+            double beta = Q(2, j) + Q(i1, j);
+            beta = beta / (1.0 + alpha);
+            for (int i = 2; i <= i1; ++i) Q(i, j) = Q(i, j) - beta * b[i - 1];
+        }
+        if (per_y(jpn)) ghost_y(q, ii, jj);
+        ghost_x(q, ii, 1, jj);
+    }
+#undef SO
+#undef Q
+#undef QF
+}
+
+void orc2_relax_lines_y_bc(const double *so, int nstncl, const double *qf, double *q,
+                           const double *sor, double *b, int ii, int jj, int updown, int jpn)
+{
+    if (!per_y(jpn)) {
+        orc2_relax_lines_y(so, nstncl, qf, q, sor, b, ii, jj, updown);
+        if (per_x(jpn)) ghost_x(q, ii, 1, jj);
+        return;
+    }
+#define SO(i, j, k) P2(so, ii, jj, i, j, k)
+#define Q(i, j) V2(q, ii, i, j)
+#define QF(i, j) V2(qf, ii, i, j)
+    const int i1 = ii - 1, j1 = jj - 1;
+    double *z = b + jj; /* B(JJ+J) */
+    const int first = (updown == ORC_DOWN) ? 3 : 2;
+    for (int pass = 0; pass < 2; ++pass) {
+        const int ibeg = (pass == 0) ? first : 5 - first;
+        const int iend = 2 * ((i1 - ibeg) / 2) + ibeg;
+        for (int i = ibeg; i <= iend; i += 2) {
+            for (int j = 2; j <= j1; ++j) {
+                if (nstncl == 5)
+                    b[j - 1] = QF(i, j) + SO(i, j, KW) * Q(i - 1, j) + SO(i + 1, j, KW) * Q(i + 1, j)
+                             + SO(i, j, KSW) * Q(i - 1, j - 1) + SO(i + 1, j, KNW) * Q(i + 1, j - 1)
+                             + SO(i, j + 1, KNW) * Q(i - 1, j + 1)
+                             + SO(i + 1, j + 1, KSW) * Q(i + 1, j + 1);
+                else
+                    b[j - 1] = QF(i, j) + SO(i, j, KW) * Q(i - 1, j) + SO(i + 1, j, KW) * Q(i + 1, j);
+            }
+            const double *d = &P2(sor, jj, ii, 2, i, 1), *e = &P2(sor, jj, ii, 3, i, 2);
+            orc_dpttrs(j1 - 1, d, e, &b[1]);
+            for (int j = 2; j <= j1; ++j) z[j - 1] = 0.0;
+            z[1] = -SO(i, 2, KS);
+            z[j1 - 1] = -SO(i, jj, KS);
+            orc_dpttrs(j1 - 1, d, e, &z[1]);
+            const double alpha = z[1] + z[j1 - 1];
+            double beta = b[1] + b[j1 - 1];
+            beta = beta / (1.0 + alpha);
+            for (int j = 2; j <= j1; ++j) Q(i, j) = b[j - 1] - beta * z[j - 1];
+        }
+        ghost_y(q, ii, jj);
+        if (per_x(jpn)) ghost_x(q, ii, 1, jj);
+    }
+#undef SO
+#undef Q
+#undef QF
+}

@@ -127,6 +127,24 @@ def setup_lines(so, direction):
     return sor, info
 
 
+def setup_lines_periodic(so, direction, jpn):
+    _, jj, ii = so.shape
+    sor = np.zeros((2, jj, ii))
+    fn = lib().orc2_setup_lines_x_bc if direction == "x" else lib().orc2_setup_lines_y_bc
+    info = fn(_ptr(so), _ptr(sor), _i(ii), _i(jj), _i(jpn))
+    return sor, info
+
+
+def relax_lines_periodic(so, qf, q, sor, updown, direction, jpn):
+    jj, ii = q.shape
+    q = q.copy()
+    b = np.zeros(2 * max(ii, jj))
+    fn = lib().orc2_relax_lines_x_bc if direction == "x" else lib().orc2_relax_lines_y_bc
+    fn(_ptr(so), _i(so.shape[0]), _ptr(qf), _ptr(q), _ptr(sor), _ptr(b), _i(ii), _i(jj),
+       _i(updown), _i(jpn))
+    return q
+
+
 def relax_lines(so, qf, q, sor, updown, direction):
     jj, ii = q.shape
     q = q.copy()
@@ -284,10 +302,8 @@ class Solver:
         s = _Settings(RELAX[relaxation], nrelax_pre, nrelax_post, num_levels, maxiter, tol,
                       min_coarse, {"v": 0, "f": 1}[cycle], mask)
         L = lib()
-        h = L.orc2_solver_create(_ptr(self.so), ns, ii - 2, jj - 2, ctypes.byref(s))
-        if not h:
-            raise ValueError("periodic boundaries need point relaxation")
-        self._h = ctypes.c_void_p(h)
+        self._h = ctypes.c_void_p(L.orc2_solver_create(_ptr(self.so), ns, ii - 2, jj - 2,
+                                                       ctypes.byref(s)))
         self.maxiter = maxiter
 
     @property

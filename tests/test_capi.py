@@ -197,7 +197,6 @@ def test_solver_create_rejects_bad_settings_before_any_gpu_work(tmp_path):
     topo = C.topo2d(7, 7)
     for conf in ({"solver": {"relaxation": "zebra"}}, {"solver": {"cycle": {"type": "w"}}},
                  {"solver": {"cg-solver": "qr"}},
-                 {"solver": {"relaxation": "line-xy"}, "grid": {"periodic": [True, False, False]}},
                  {"grid": {"periodic": [False, False, True]}},
                  {"gpu": {"precision": "fp16"}}):
         c = C.config(_write(tmp_path, conf))

@@ -175,3 +175,54 @@ def test_capi_periodic_solve(tmp_path):
     xi = xa.copy()
     xi[:, 0], xi[:, -1] = xi[:, -2], xi[:, 1]
     assert np.array_equal(A.vec_array(yv), O.matvec(so, xi))
+
+
+@pytest.mark.parametrize("nx,ny", [(12, 10), (33, 20), (64, 31)])
+@pytest.mark.parametrize("mask", [1, 2, 3])
+@pytest.mark.parametrize("ns", [3, 5])
+def test_periodic_line_kernels_bitwise(nx, ny, mask, ns):
+    import cedar_amd as C
+    print(f"\nperiodic lines {nx}x{ny} ns={ns} mask={mask}", flush=True)
+    rng = np.random.default_rng(nx + ny * 7 + mask * 3 + ns)
+    jpn = O.bc_from_mask(mask)
+    so = rand_periodic_so(rng, nx, ny, ns, mask)
+    q = rng.standard_normal((ny + 2, nx + 2))
+    qf = rng.standard_normal(q.shape)
+    for d in ("x", "y"):
+        sor, info = O.setup_lines_periodic(so, d, jpn)
+        assert info == 0
+        gsor = torch.zeros(sor.shape, dtype=torch.float64, device="cuda")
+        C.setup_lines_periodic(dev(so), gsor, d, jpn)
+        assert np.array_equal(host(gsor), sor), d
+        for ud in (O.DOWN, O.UP):
+            g = C.relax_lines_periodic(dev(so), dev(qf), dev(q), dev(sor), ud, d, jpn)
+            assert np.array_equal(host(g), O.relax_lines_periodic(so, qf, q, sor, ud, d, jpn)), (d, ud)
+
+
+@pytest.mark.parametrize("relax", ["line-x", "line-y", "line-xy"])
+@pytest.mark.parametrize("mask", [1, 2, 3])
+def test_periodic_line_solve_matches_oracle(relax, mask):
+    import cedar_amd as C
+    print(f"\nperiodic line solve {relax} mask={mask}", flush=True)
+    so = rand_periodic_so(np.random.default_rng(mask + 11), 95, 70, 5, mask)
+    so[0] += 0.2
+    nx, ny = 95, 70
+    if mask & 1:
+        so[:, :, 0], so[:, :, nx + 1] = so[:, :, nx], so[:, :, 1]
+    if mask & 2:
+        so[:, 0, :], so[:, ny + 1, :] = so[:, ny, :], so[:, 1, :]
+    b = np.random.default_rng(4).standard_normal((ny + 2, nx + 2))
+    if mask & 1:
+        b[:, 0], b[:, -1] = b[:, -2], b[:, 1]
+    if mask & 2:
+        b[0, :], b[-1, :] = b[-2, :], b[1, :]
+    per = (bool(mask & 1), bool(mask & 2))
+    kw = dict(relaxation=relax, nrelax_pre=1, nrelax_post=1, maxiter=30, tol=1e-10)
+    xo, ho = O.Solver(so, periodic=per, **kw).solve(b)
+    x, h = C.Solver(dev(so), periodic=per, **kw).solve(dev(b))
+    assert len(h) == len(ho)
+    np.testing.assert_allclose(h, ho, rtol=1e-12, atol=0)
+    assert np.linalg.norm(host(x) - xo) <= 1e-10 * np.linalg.norm(xo)
+    xm, hm = C.Solver(dev(so), periodic=per, precision="mixed", **kw).solve(dev(b))
+    assert hm[-1] < 1e-10
+    assert torch.linalg.norm(xm - x) <= 1e-8 * torch.linalg.norm(x)

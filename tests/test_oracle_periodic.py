@@ -161,3 +161,71 @@ def test_periodic_xy_definite_solve():
     assert hist[-1] < 1e-11
     ref = np.linalg.solve(dense_periodic(so, 3), b[1:-1, 1:-1].ravel())
     np.testing.assert_allclose(x[1:-1, 1:-1].ravel(), ref, rtol=0, atol=1e-9 * np.abs(ref).max())
+
+
+def _full_residual(so, b, q, mask):
+    """b - A q with the periodic images of q (the oracle's residual, pinned to the reference)."""
+    return O.residual(so, b, periodic_fill(q.copy(), mask))
+
+
+@pytest.mark.parametrize("mask", [1, 2, 3])
+@pytest.mark.parametrize("ns", [3, 5])
+@pytest.mark.parametrize("direction", ["x", "y"])
+def test_periodic_line_solves_are_exact(mask, ns, direction):
+    """After a periodic zebra sweep, every line of the colour relaxed last solves its (cyclic,
+    when its direction is periodic) line system: the residual vanishes on it.  Lines next to a
+    periodic boundary across the lines read images refreshed only after the sweep and are left
+    out."""
+    from tests.golden.make_golden_periodic import rand_periodic_so
+    nx, ny = 12, 10
+    rng = np.random.default_rng(mask * 10 + ns)
+    so = rand_periodic_so(rng, nx, ny, ns, mask)
+    jpn = O.bc_from_mask(mask)
+    b = periodic_fill(rng.standard_normal((ny + 2, nx + 2)), mask)
+    q = periodic_fill(rng.standard_normal((ny + 2, nx + 2)), mask)
+    sor, info = O.setup_lines_periodic(so, direction, jpn)
+    assert info == 0
+    for ud in (O.DOWN, O.UP):
+        out = O.relax_lines_periodic(so, b, q, sor, ud, direction, jpn)
+        r = _full_residual(so, b, out, mask)
+        last = 2 if ud == O.UP else 1          # C index parity of the colour relaxed last
+        lines = range(last, (ny if direction == "x" else nx) + 1, 2)
+        across = (mask & 2) if direction == "x" else (mask & 1)
+        n = ny if direction == "x" else nx
+        for L in lines:
+            if across and L in (1, n):
+                continue
+            row = r[L, 1:-1] if direction == "x" else r[1:-1, L]
+            assert np.abs(row).max() < 1e-12, (ud, L)
+
+
+@pytest.mark.parametrize("relax", ["line-x", "line-y", "line-xy"])
+@pytest.mark.parametrize("mask", [1, 2, 3])
+def test_periodic_line_relaxation_solves(relax, mask):
+    """Periodic line-relaxation cycles converge to the dense solution of a definite problem."""
+    from tests.golden.make_golden_periodic import rand_periodic_so
+    nx, ny = 31, 24
+    so = rand_periodic_so(np.random.default_rng(mask + 7), nx, ny, 5, mask)
+    so[0] += 0.2
+    so = periodic_fill_op(so, mask)
+    b = periodic_fill(np.random.default_rng(3).standard_normal((ny + 2, nx + 2)), mask)
+    per = (bool(mask & 1), bool(mask & 2))
+    slv = O.Solver(so, relaxation=relax, nrelax_pre=1, nrelax_post=1, maxiter=40, tol=1e-11,
+                   periodic=per)
+    x, hist = slv.solve(b)
+    assert hist[-1] < 1e-11
+    ref = np.linalg.solve(dense_periodic(so, mask), b[1:-1, 1:-1].ravel())
+    np.testing.assert_allclose(x[1:-1, 1:-1].ravel(), ref, rtol=0, atol=1e-9 * np.abs(ref).max())
+
+
+def periodic_fill_op(so, mask):
+    """Images of every plane of an operator (after changing its interior)."""
+    so = so.copy()
+    nx, ny = so.shape[2] - 2, so.shape[1] - 2
+    if mask & 1:
+        so[:, :, 0] = so[:, :, nx]
+        so[:, :, nx + 1] = so[:, :, 1]
+    if mask & 2:
+        so[:, 0, :] = so[:, ny, :]
+        so[:, ny + 1, :] = so[:, 1, :]
+    return so
