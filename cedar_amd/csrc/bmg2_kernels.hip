@@ -493,9 +493,12 @@ __global__ __launch_bounds__(256) void relax_lines_k(const T *__restrict__ so, c
 // Workgroup index remapped so that consecutive logical tiles run on the same XCD (workgroups
 // are dealt round-robin over the 8 XCDs): neighbouring lines share rows of q and of the
 // operator, which then stay in one XCD's L2.  Launch 8 * ceil(ntiles / 8) workgroups.
-__device__ inline int xcd_tile(int ntiles) {
+// rev: each XCD walks its range of tiles from the end, so that a launch starts on the lines the
+// previous launch touched last (still in the memory-side cache); the work is unchanged.
+__device__ inline int xcd_tile(int ntiles, int rev = 0) {
     const int per = (ntiles + 7) / 8;
-    return (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    const int k = (int)(blockIdx.x / 8);
+    return (int)(blockIdx.x % 8) * per + (rev ? per - 1 - k : k);
 }
 
 // raw operands of the right-hand side of one point of a line
@@ -712,9 +715,9 @@ __global__ __launch_bounds__(L *W * 64) void relax_xlines_seg_k(const T *__restr
                                                                T *__restrict__ q,
                                                                const T *__restrict__ sor, int ii,
                                                                int jj, int first, int nlines,
-                                                               int ntiles) {
+                                                               int ntiles, int rev) {
     __shared__ seg_map<T> xch[2][L][W > 1 ? W * CPW : 1];
-    const int tile = xcd_tile(ntiles);
+    const int tile = xcd_tile(ntiles, rev);
     if (tile >= ntiles) return;  // uniform over the workgroup
     const int wl = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int slot = wl / W, w = wl % W;
@@ -803,11 +806,11 @@ __global__ __launch_bounds__(NC *W * 64) void relax_ylines_seg_k(const T *__rest
                                                                 T *__restrict__ q,
                                                                 const T *__restrict__ sor,
                                                                 int ii, int jj, int first,
-                                                                int ncols, int ntiles) {
+                                                                int ncols, int ntiles, int rev) {
     static_assert(CPW % G == 0, "G must divide CPW");
     __shared__ T tile_s[G][W][NC][WAVE + 1];
     __shared__ seg_map<T> xch[2][NC][W > 1 ? W * CPW : 1];
-    const int tile = xcd_tile(ntiles);
+    const int tile = xcd_tile(ntiles, rev);
     if (tile >= ntiles) return;  // uniform over the workgroup
     const int wl = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int cs = wl / W, w = wl % W;
@@ -1600,20 +1603,20 @@ __global__ __launch_bounds__(1024) void coarse_cycle_k(const coarse_level<T> *__
 
 template <typename T, bool NINE, int CPW, int W, int L, bool TR>
 static void launch_x(const T *so, const T *qf, T *q, const T *sor, int ii, int jj, int first,
-                     int nlines, hipStream_t s, int nb) {
+                     int nlines, hipStream_t s, int nb, int rev) {
     const int ntiles = (nlines + L - 1) / L;
     const dim3 blocks((unsigned)(8 * ((ntiles + 7) / 8)), (unsigned)nb);
     relax_xlines_seg_k<T, NINE, CPW, W, L, TR><<<blocks, L * W * 64, 0, s>>>(
-        so, qf, q, sor, ii, jj, first, nlines, ntiles);
+        so, qf, q, sor, ii, jj, first, nlines, ntiles, rev);
 }
 
 template <typename T, bool NINE, int CPW, int W, int NC, int G>
 static void launch_y(const T *so, const T *qf, T *q, const T *sor, int ii, int jj, int first,
-                     int nlines, hipStream_t s, int nb) {
+                     int nlines, hipStream_t s, int nb, int rev) {
     const int ntiles = (nlines + NC - 1) / NC;
     const dim3 blocks((unsigned)(8 * ((ntiles + 7) / 8)), (unsigned)nb);
     relax_ylines_seg_k<T, NINE, CPW, W, NC, G><<<blocks, NC * W * 64, 0, s>>>(
-        so, qf, q, sor, ii, jj, first, nlines, ntiles);
+        so, qf, q, sor, ii, jj, first, nlines, ntiles, rev);
 }
 
 // Launches candidate c of chunk class NCH (lines of at most 64*NCH points); returns the number
@@ -1631,7 +1634,8 @@ static std::string shape_text(const char *k, int cpw, int w, const char *tn, int
 // DIR 0: x-lines, 1: y-lines, 2: x-lines of a transposed y-line problem.
 template <typename T, bool NINE, int DIR, int NCH>
 static int launch_shape(int c, const T *so, const T *qf, T *q, const T *sor, int ii, int jj,
-                        int first, int nlines, hipStream_t s, int nb, std::string *name = nullptr) {
+                        int first, int nlines, hipStream_t s, int nb, std::string *name = nullptr,
+                        int rev = 0) {
     constexpr bool TR = DIR == 2;
     constexpr bool F64 = sizeof(T) == 8;
 #define CDR_X(k, C, W, L)                                                                       \
@@ -1639,14 +1643,14 @@ static int launch_shape(int c, const T *so, const T *qf, T *q, const T *sor, int
         if (name)                                                                               \
             *name = shape_text("relax_xlines_seg_k", C, W, "L", L, 0);                          \
         else                                                                                    \
-            launch_x<T, NINE, C, W, L, TR>(so, qf, q, sor, ii, jj, first, nlines, s, nb);       \
+            launch_x<T, NINE, C, W, L, TR>(so, qf, q, sor, ii, jj, first, nlines, s, nb, rev);  \
     }
 #define CDR_Y(k, C, W, NC, G)                                                                   \
     if (c == k) {                                                                               \
         if (name)                                                                               \
             *name = shape_text("relax_ylines_seg_k", C, W, "NC", NC, G);                        \
         else                                                                                    \
-            launch_y<T, NINE, C, W, NC, G>(so, qf, q, sor, ii, jj, first, nlines, s, nb);       \
+            launch_y<T, NINE, C, W, NC, G>(so, qf, q, sor, ii, jj, first, nlines, s, nb, rev);  \
     }
     // x: (CPW, W, L); y: (CPW, W, NC, G).  At most 16 wavefronts per workgroup.
     if constexpr (DIR != 1) {
@@ -1733,11 +1737,13 @@ static int chunk_class(int n) {
 
 template <typename T, bool NINE, int DIR>
 static int launch_class(int c, const T *so, const T *qf, T *q, const T *sor, int ii, int jj,
-                        int first, int nlines, hipStream_t s, int nb, std::string *name = nullptr) {
+                        int first, int nlines, hipStream_t s, int nb, std::string *name = nullptr,
+                        int rev = 0) {
     const int n = (DIR != 1) ? ii - 2 : jj - 2;
 #define CDR_C(K)                                                                                  \
     case K:                                                                                       \
-        return launch_shape<T, NINE, DIR, K>(c, so, qf, q, sor, ii, jj, first, nlines, s, nb, name)
+        return launch_shape<T, NINE, DIR, K>(c, so, qf, q, sor, ii, jj, first, nlines, s, nb, name, \
+                                             rev)
     switch (chunk_class(n)) {
         CDR_C(1);
         CDR_C(2);
@@ -1781,10 +1787,10 @@ static int tuned_shape(const tune_key &k) {
 template <typename T, int DIR>
 static int launch_colour(int c, const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii,
                          int jj, int first, int nlines, hipStream_t s, int nb,
-                         std::string *name = nullptr) {
+                         std::string *name = nullptr, int rev = 0) {
     if (nstncl == 5)
-        return launch_class<T, true, DIR>(c, so, qf, q, sor, ii, jj, first, nlines, s, nb, name);
-    return launch_class<T, false, DIR>(c, so, qf, q, sor, ii, jj, first, nlines, s, nb, name);
+        return launch_class<T, true, DIR>(c, so, qf, q, sor, ii, jj, first, nlines, s, nb, name, rev);
+    return launch_class<T, false, DIR>(c, so, qf, q, sor, ii, jj, first, nlines, s, nb, name, rev);
 }
 
 static std::string shape_name(const std::array<int, 6> &k, int c) {
@@ -1818,7 +1824,8 @@ static void relax_lines(const T *so, int nstncl, const T *qf, T *q, const T *sor
         const int first = (updown == 0) ? (t == 0 ? 2 : 1) : (t == 0 ? 1 : 2);
         if (first > nfix) continue;
         const int nlines = (nfix - first) / 2 + 1;
-        launch_colour<T, DIR>(c, so, nstncl, qf, q, sor, ii, jj, first, nlines, s, nb);
+        // the second colour walks the lines in the opposite order (see xcd_tile)
+        launch_colour<T, DIR>(c, so, nstncl, qf, q, sor, ii, jj, first, nlines, s, nb, nullptr, t);
     }
 }
 
@@ -1940,9 +1947,11 @@ static void relax_lines_ytr(const T *soT, int nstncl, const T *qf, T *q, const T
         if (first > nfix) continue;
         const int nlines = (nfix - first) / 2 + 1;
         if (nstncl == 5)
-            launch_class<T, true, 2>(shape, soT, qfT, qT, sory, jj, ii, first, nlines, s, nb);
+            launch_class<T, true, 2>(shape, soT, qfT, qT, sory, jj, ii, first, nlines, s, nb,
+                                     nullptr, t);
         else
-            launch_class<T, false, 2>(shape, soT, qfT, qT, sory, jj, ii, first, nlines, s, nb);
+            launch_class<T, false, 2>(shape, soT, qfT, qT, sory, jj, ii, first, nlines, s, nb,
+                                      nullptr, t);
     }
     transpose<T>(qT, q, ii, jj, s, nb);
 }

@@ -406,7 +406,9 @@ struct slot {
     bool released = false;   // its handle was freed while still referenced
     std::shared_ptr<void> obj;
 };
-std::vector<slot> g_slots;
+// The table is never destroyed: objects the caller did not free would otherwise release device
+// memory during static destruction, after the HIP runtime may already be gone.
+std::vector<slot> &g_slots = *new std::vector<slot>();
 
 template <typename T>
 int create(int kind, std::shared_ptr<T> obj) {
