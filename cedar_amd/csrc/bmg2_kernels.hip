@@ -226,11 +226,15 @@ __global__ __launch_bounds__(256) void recip_k(const double *__restrict__ so,
 
 // 5-point: colour = parity of (i + j); 9-point: colour = (i parity pi, j parity pj)
 template <typename T, bool NINE>
+// rev: the tiles are visited in reverse (a colour then starts where the previous one ended, in
+// the memory-side cache); points of one colour are independent, so the result is unchanged
 __global__ __launch_bounds__(256) void gs_colour_k(const T *__restrict__ so, const T *__restrict__ qf,
                                                    T *__restrict__ q, const T *__restrict__ sor,
-                                                   int ii, int jj, int pi, int pj) {
-    const int k = blockIdx.x * 64 + (threadIdx.x % 64);
-    const int m = blockIdx.y * 4 + (threadIdx.x / 64);
+                                                   int ii, int jj, int pi, int pj, int rev = 0) {
+    const int bx = rev ? (int)(gridDim.x - 1 - blockIdx.x) : (int)blockIdx.x;
+    const int by = rev ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.y;
+    const int k = bx * 64 + (threadIdx.x % 64);
+    const int m = by * 4 + (threadIdx.x / 64);
     int i, j;
     if (NINE) {
         i = 2 * k + pi;
@@ -1364,14 +1368,15 @@ void relax_gs(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, 
         const dim3 g = gridb(ii / 2 + 1, jj / 2 + 1, nb);
         for (int t = 0; t < 4; ++t) {
             const int c = updown == 0 ? t : 3 - t;
-            gs_colour_k<T, true><<<g, 256, 0, s>>>(so, qf, q, sor, ii, jj, order[c][0], order[c][1]);
+            gs_colour_k<T, true><<<g, 256, 0, s>>>(so, qf, q, sor, ii, jj, order[c][0], order[c][1],
+                                                   t & 1);
         }
     } else {
         // DOWN: (i + j) even first; UP: odd first
         const dim3 g = gridb(ii / 2 + 1, jj - 2, nb);
         for (int t = 0; t < 2; ++t) {
             const int parity = updown == 0 ? t : 1 - t;
-            gs_colour_k<T, false><<<g, 256, 0, s>>>(so, qf, q, sor, ii, jj, parity, 0);
+            gs_colour_k<T, false><<<g, 256, 0, s>>>(so, qf, q, sor, ii, jj, parity, 0, t & 1);
         }
     }
 }

@@ -186,10 +186,14 @@ __global__ __launch_bounds__(256) void recip3_k(const double *__restrict__ so,
 template <typename T>
 __global__ __launch_bounds__(256) void gs27_colour_k(const T *__restrict__ so, const T *__restrict__ qf,
                                                      T *__restrict__ q, const T *__restrict__ sor,
-                                                     int ii, int jj, int kk, int pi, int pj, int pk) {
+                                                     int ii, int jj, int kk, int pi, int pj, int pk,
+                                                     int rev) {
+    // rev: planes visited in reverse, so that a colour starts where the previous one ended (in
+    // the memory-side cache); the points of a colour are independent
+    const int bz = rev ? (int)(gridDim.z - 1 - blockIdx.z) : (int)blockIdx.z;
     const int i = pi + 2 * (int)(blockIdx.x * 64 + (threadIdx.x % 64));
     const int j = pj + 2 * (int)(blockIdx.y * 4 + (threadIdx.x / 64));
-    const int k = pk + 2 * (int)blockIdx.z;
+    const int k = pk + 2 * bz;
     if (i > ii - 1 || j > jj - 1 || k > kk - 1) return;
     const std::size_t P = (std::size_t)ii * jj * kk;
     Q3(q, i, j, k) = offdiag3<T, true>(so, qf, q, i, j, k, ii, jj, P) * sor[P + ix3(i, j, k, ii, jj)];
@@ -198,9 +202,9 @@ __global__ __launch_bounds__(256) void gs27_colour_k(const T *__restrict__ so, c
 template <typename T>
 __global__ __launch_bounds__(256) void gs7_colour_k(const T *__restrict__ so, const T *__restrict__ qf,
                                                     T *__restrict__ q, const T *__restrict__ sor,
-                                                    int ii, int jj, int kk, int pts) {
+                                                    int ii, int jj, int kk, int pts, int rev) {
     const int j = blockIdx.y * 4 + (threadIdx.x / 64) + 2;
-    const int k = blockIdx.z + 2;
+    const int k = (rev ? (int)(gridDim.z - 1 - blockIdx.z) : (int)blockIdx.z) + 2;
     if (j > jj - 1) return;
     const int i = (j + k + pts) % 2 + 2 + 2 * (int)(blockIdx.x * 64 + (threadIdx.x % 64));
     if (i > ii - 1) return;
@@ -1114,13 +1118,15 @@ void relax_gs(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, 
             const int pi = 2 + (pts - 1) % 2, pj = 2 + ((pts - 1) / 2) % 2, pk = 2 + ((pts - 1) / 4) % 2;
             const int ni = (ii - 1 - pi) / 2 + 1, nj = (jj - 1 - pj) / 2 + 1, nk = (kk - 1 - pk) / 2 + 1;
             if (ni <= 0 || nj <= 0 || nk <= 0) continue;
-            gs27_colour_k<T><<<grid3(ni, nj, nk), 256, 0, s>>>(so, qf, q, sor, ii, jj, kk, pi, pj, pk);
+            gs27_colour_k<T><<<grid3(ni, nj, nk), 256, 0, s>>>(so, qf, q, sor, ii, jj, kk, pi, pj, pk,
+                                                               t & 1);
         }
     } else {
         // DOWN: pts = 1 then 0; UP: 0 then 1
         for (int t = 0; t < 2; ++t) {
             const int pts = updown == 0 ? 1 - t : t;
-            gs7_colour_k<T><<<grid3((ii - 1) / 2, jj - 2, kk - 2), 256, 0, s>>>(so, qf, q, sor, ii, jj, kk, pts);
+            gs7_colour_k<T><<<grid3((ii - 1) / 2, jj - 2, kk - 2), 256, 0, s>>>(so, qf, q, sor, ii, jj, kk,
+                                                                                pts, t & 1);
         }
     }
 }
