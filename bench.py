@@ -69,6 +69,40 @@ def measured_traffic():
     return None
 
 
+def level_rates(solver, b, word, nstncl, torch):
+    """Achieved bandwidth of every V-cycle phase on every level of one solve, from the solver's
+    per-level device timers (cycles run eagerly while timed).  Algorithmic bytes per call, with
+    N the level's points and w the word size: relaxation (one line-xy step: an x and a y sweep)
+    2 (S+3) w N, residual (S+3) w N, restriction 3.25 w N_fine, interp_add 6.25 w N_fine."""
+    import json as _json
+    import tempfile
+    solver.timers(True)
+    solver.solve(b)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "t.json")
+        solver.save_timers(path)
+        levels = _json.load(open(path))["levels"]
+    solver.timers(False)
+    nl = solver.nlevels
+    per = {"relaxation": 2 * (nstncl + 3), "residual": nstncl + 3, "restrict": 3.25,
+           "interp-add": 6.25}
+    out = []
+    for k, lv in enumerate(levels):
+        lvl = nl - 1 - k
+        nx, ny = solver.level(lvl)[:2]
+        npts = nx * ny
+        row = {"level": lvl, "n": [nx, ny]}
+        for label, words in per.items():
+            if label not in lv:
+                continue
+            t, cnt = float(lv[label]["time"]), int(lv[label]["count"])
+            byt = words * word * npts * cnt
+            row[label] = {"s": t, "calls": cnt, "GBps": byt / t / 1e9,
+                          "frac": byt / t / 1e9 / HBM_PEAK_GBS}
+        out.append(row)
+    return sorted(out, key=lambda r: r["level"])
+
+
 def kernel_only(args, so, b):
     """Dominant kernel alone (the fine-level FP32 x-line kernel, 5 sweeps = 10 launches) plus a
     copy of known size whose counters calibrate FETCH_SIZE / WRITE_SIZE."""
@@ -329,6 +363,10 @@ def main():
         dt = (time.perf_counter() - t) / args.steps
         results[prec] = dict(seconds=dt, iters=int(len(h)), rho=float(h[-1]), x=x, hist=h)
 
+    # ---- achieved bandwidth per level and phase (the solver's device timers) ---------------
+    per_level = {prec: level_rates(solvers[prec][0], b, 8 if prec == "f64" else 4, so.shape[0],
+                                   torch) for prec in ("f64", "mixed")}
+
     x64 = results["f64"]["x"].cpu().numpy()
     x32 = results["mixed"]["x"].cpu().numpy()
     mixed_vs_f64 = float(np.linalg.norm((x32 - x64)[1:-1, 1:-1]) / np.linalg.norm(x64[1:-1, 1:-1]))
@@ -441,6 +479,7 @@ def main():
             "avg_launch_s": dom[1] / 2,
         },
         "kernels": kern,
+        "levels": per_level,
         "line_shapes": {f"{d}_f{fp}": k for (d, fp), k in shapes.items()},
         "cpu_baseline": cpu,
         "gen_s": gen_s,
