@@ -1,0 +1,178 @@
+"""Parity at the benchmark sizes themselves: the C2 problem (4095^2 9-point Q1 diffusion, lognormal
+block coefficients) and the C3 problem (255^3 7-point finite volumes, anisotropic lognormal
+coefficients), against the C oracle.
+
+Every level kernel of the finest level is checked on the real operator: residual, operator-induced
+interpolation, restriction, interpolation and 3D Gauss-Seidel bit for bit, the 2D Galerkin
+product to 1e-13 of its largest entry, 2D line factors bit for bit and line sweeps to 1e-13 (FP64,
+scan against the sequential substitution) and 2e-5 (FP32).  Whole solves: the FP64 C2 solve
+against the oracle's (residual history within 1e-12 per iteration, solution within 1e-10); the
+mixed solves and the C3 solves through properties that do not depend on the size: the same
+iteration count as FP64, the solution within 1e-9 of the FP64 one, and the oracle's own residual
+of the returned solution below the tolerance."""
+import numpy as np
+import pytest
+
+from cedar_amd import gallery as G
+from oracle import oracle as O
+from oracle import oracle3 as O3
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def C():
+    import cedar_amd
+    return cedar_amd
+
+
+def dev(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.double().cpu().numpy()
+
+
+def rel_l2(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def c2():
+    so, b, _ = G.heterogeneous_9pt(4095, seed=2, block=16, sigma=2.0, contrast=1e4)
+    return so, b
+
+
+@pytest.fixture(scope="module")
+def c3():
+    so, b, _ = G.heterogeneous_7pt(255, seed=3, block=8, sigma=1.5, aniso=(1.0, 1.0, 1e-3))
+    return so, b
+
+
+def _interior_normal(shape, seed):
+    q = np.zeros(shape)
+    inner = tuple(slice(1, -1) for _ in shape)
+    q[inner] = np.random.default_rng(seed).standard_normal(tuple(s - 2 for s in shape))
+    return q
+
+
+def test_c2_level_kernels_full_size(C, c2):
+    so, b = c2
+    print("\nC2 4095^2 level kernels", flush=True)
+    q = _interior_normal(b.shape, 21)
+    dso = dev(so)
+    r = torch.zeros_like(dev(b))
+    C.cdr2.residual(dso, dev(b), dev(q), r)
+    ores = O.residual(so, b, q)
+    assert np.array_equal(host(r), ores)
+
+    oci = O.setup_interp(so)
+    ci = torch.zeros(oci.shape, dtype=torch.float64, device="cuda")
+    C.cdr2.setup_interp(dso, ci)
+    assert np.array_equal(host(ci), oci)
+    osoc = O.setup_galerkin(so, oci)
+    soc = torch.zeros(osoc.shape, dtype=torch.float64, device="cuda")
+    C.cdr2.setup_galerkin(dso, ci, soc)
+    assert np.abs(host(soc) - osoc).max() <= 1e-13 * np.abs(osoc).max()
+
+    qc = torch.zeros(oci.shape[1:], dtype=torch.float64, device="cuda")
+    C.cdr2.restrict(ci, r, qc)
+    assert np.array_equal(host(qc), O.restrict(oci, ores))
+    qcv = _interior_normal(oci.shape[1:], 22)
+    x = dev(q)
+    C.cdr2.interp_add(x, dev(qcv), dev(ores), dso, ci)
+    oq, _ = O.interp_add(q, qcv, ores, so, oci)
+    assert np.array_equal(host(x), oq)
+
+
+@pytest.mark.parametrize("direction", ["x", "y"])
+def test_c2_line_sweeps_full_size(C, c2, direction):
+    so, b = c2
+    print(f"\nC2 4095^2 {direction}-line sweeps", flush=True)
+    q = _interior_normal(b.shape, 23)
+    dso = dev(so)
+    osor, info = O.setup_lines(so, direction)
+    assert info == 0
+    sor = torch.zeros(osor.shape, dtype=torch.float64, device="cuda")
+    C.setup_lines(dso, sor, direction)
+    assert np.array_equal(host(sor), osor)
+    for ud in (0, 1):
+        want = O.relax_lines(so, b, q, osor, ud, direction)
+        scale = np.abs(want).max()
+        for dtype, tol in ((torch.float64, 1e-13), (torch.float32, 2e-5)):
+            x = dev(q, dtype)
+            C.relax_lines(dev(so, dtype), dev(b, dtype), x, sor.to(dtype), ud, direction)
+            assert np.abs(host(x) - want).max() <= tol * scale, (ud, dtype)
+
+
+def test_c2_solve_full_size(C, c2):
+    so, b = c2
+    kw = dict(relaxation="line-xy", nrelax_pre=1, nrelax_post=1, maxiter=40, tol=1e-10)
+    print("\nC2 4095^2 oracle solve", flush=True)
+    x_cpu, h_cpu = O.Solver(so, **kw).solve(b)
+    print("C2 4095^2 GPU solves", flush=True)
+    dso, db = dev(so), dev(b)
+    x64, h64 = C.Solver(dso, **kw).solve(db)
+    x32, h32 = C.Solver(dso, precision="mixed", **kw).solve(db)
+    x64, x32 = x64.cpu().numpy(), x32.cpu().numpy()
+    assert len(h64) == len(h_cpu)
+    assert np.abs(h64 - h_cpu).max() <= 1e-12
+    assert rel_l2(x64, x_cpu) <= 1e-10
+    assert h64[-1] < 1e-10
+    assert len(h32) == len(h64) and h32[-1] < 1e-10
+    assert rel_l2(x32, x64) < 1e-9
+    assert O.norm(O.residual(so, b, x32)) / O.norm(b) < 1e-10
+
+
+def test_c3_level_kernels_full_size(C, c3):
+    so, b = c3
+    print("\nC3 255^3 level kernels", flush=True)
+    K = C.cdr3
+    q = _interior_normal(b.shape, 31)
+    dso = dev(so)
+    r = torch.zeros_like(dev(b))
+    K.residual(dso, dev(b), dev(q), r)
+    ores = O3.residual(so, b, q)
+    assert np.array_equal(host(r), ores)
+    osor = O3.setup_recip(so)
+    sor = torch.zeros(osor.shape, dtype=torch.float64, device="cuda")
+    K.setup_recip(dso, sor)
+    assert np.array_equal(host(sor), osor)
+    x = dev(q)
+    K.relax_gs(dso, dev(b), x, sor, K.DOWN)
+    assert np.array_equal(host(x), O3.relax_gs(so, b, q, osor, K.DOWN))
+
+    oci = O3.setup_interp(so)
+    ci = torch.zeros(oci.shape, dtype=torch.float64, device="cuda")
+    K.setup_interp(dso, ci)
+    assert np.array_equal(host(ci), oci)
+    soc = torch.zeros((14,) + oci.shape[1:], dtype=torch.float64, device="cuda")
+    K.setup_galerkin(dso, ci, soc)
+    assert np.array_equal(host(soc), O3.setup_galerkin(so, oci))
+    qc = torch.zeros(oci.shape[1:], dtype=torch.float64, device="cuda")
+    K.restrict(ci, r, qc)
+    assert np.array_equal(host(qc), O3.restrict(oci, ores))
+    qcv = _interior_normal(oci.shape[1:], 32)
+    x = dev(q)
+    K.interp_add(x, dev(qcv), dev(ores), dso, ci)
+    oq, _ = O3.interp_add(q, qcv, ores, so, oci)
+    assert np.array_equal(host(x), oq)
+
+
+def test_c3_solves_full_size(C, c3):
+    so, b = c3
+    print("\nC3 255^3 GPU solves", flush=True)
+    dso, db = dev(so), dev(b)
+    kw = dict(relaxation="plane-xy", nrelax_pre=1, nrelax_post=1, maxiter=60, tol=1e-10)
+    x64, h64 = C.Solver3(dso, **kw).solve(db)
+    x32, h32 = C.Solver3(dso, precision="mixed", **kw).solve(db)
+    x64, x32 = x64.cpu().numpy(), x32.cpu().numpy()
+    assert h64[-1] < 1e-10 and h32[-1] < 1e-10
+    assert len(h32) == len(h64)
+    assert rel_l2(x32, x64) < 1e-9
+    nb = O3.norm(b)
+    assert O3.norm(O3.residual(so, b, x64)) / nb < 1e-10
+    assert O3.norm(O3.residual(so, b, x32)) / nb < 1e-10
