@@ -1,6 +1,6 @@
 """Parity at the benchmark sizes themselves: the C2 problem (4095^2 9-point Q1 diffusion, lognormal
-block coefficients) and the C3 problem (255^3 7-point finite volumes, anisotropic lognormal
-coefficients), against the C oracle.
+block coefficients), the C3 problem (255^3 7-point finite volumes, anisotropic lognormal
+coefficients) and the C1 problem (511^2 5-point Poisson, point V(2,1)), against the C oracle.
 
 Every level kernel of the finest level is checked on the real operator: residual, operator-induced
 interpolation, restriction, interpolation and 3D Gauss-Seidel bit for bit, the 2D Galerkin
@@ -125,6 +125,23 @@ def test_c2_solve_full_size(C, c2):
     assert len(h32) == len(h64) and h32[-1] < 1e-10
     assert rel_l2(x32, x64) < 1e-9
     assert O.norm(O.residual(so, b, x32)) / O.norm(b) < 1e-10
+
+
+def test_c1_solve(C):
+    """The reference's CPU-runnable case: 5-point Poisson on 511^2, f ~ U[-1, 1], V(2,1) with
+    red-black point Gauss-Seidel to 1e-10, FP64 against the oracle; mixed: same cycle count."""
+    so, b = G.poisson_uniform_rhs(511, seed=1)
+    kw = dict(relaxation="point", nrelax_pre=2, nrelax_post=1, maxiter=50, tol=1e-10)
+    x_cpu, h_cpu = O.Solver(so, **kw).solve(b)
+    dso, db = dev(so), dev(b)
+    x64, h64 = C.Solver(dso, **kw).solve(db)
+    x32, h32 = C.Solver(dso, precision="mixed", **kw).solve(db)
+    x64, x32 = x64.cpu().numpy(), x32.cpu().numpy()
+    assert len(h64) == len(h_cpu) and 8 <= len(h64) <= 12
+    assert np.abs(h64 - h_cpu).max() <= 1e-12
+    assert rel_l2(x64, x_cpu) <= 1e-10
+    assert len(h32) == len(h64) and h32[-1] < 1e-10
+    assert rel_l2(x32, x64) < 1e-9
 
 
 def test_c3_level_kernels_full_size(C, c3):
