@@ -162,6 +162,19 @@ def readme_problem(n=400):
     return so, b, exact
 
 
+def sine_problem(nx, ny):
+    """poisson(nx, ny) with b = 8 pi^2 sin(2 pi x) sin(2 pi y) hx hy, hx = 1/(nx+1), hy = 1/(ny+1),
+    and the continuous solution sin(2 pi x) sin(2 pi y) (test/2d/test_poisson.cc:13-61); returns
+    (so, b, exact)."""
+    so = poisson(nx, ny)
+    hx, hy = 1.0 / (nx + 1), 1.0 / (ny + 1)
+    X, Y = np.meshgrid(np.arange(nx + 2) * hx, np.arange(ny + 2) * hy)  # X[j, i] = i hx
+    s = np.sin(2 * np.pi * X) * np.sin(2 * np.pi * Y)
+    b = np.zeros((ny + 2, nx + 2))
+    b[1:ny + 1, 1:nx + 1] = (8 * (np.pi * np.pi) * s * (hx * hy))[1:ny + 1, 1:nx + 1]
+    return so, b, s
+
+
 def periodic_problem(nx, ny, periodic=(True, False)):
     """Cedar's periodic example (examples/basic-2d-ser/periodic.cc:16-155): 5-point Poisson on
     nx x ny points, periodic in the flagged directions (a periodic direction of n points has

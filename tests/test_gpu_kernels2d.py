@@ -155,6 +155,26 @@ def test_line_relaxation_fp32(C, nx, ny, direction):
         assert_close(host(qq), O.relax_lines(so, qf, q, sor, ud, direction), 2e-5)
 
 
+@pytest.mark.parametrize("direction", ["x", "y"])
+def test_reference_line_smoothing_tests(C, direction):
+    """test/2d/test_relax.cc:96-165 (LineX5, LineY5) on the GPU kernels: 15 DOWN line sweeps on
+    the 132^2 operator strong along the lines, b = 0, random x, leave high frequencies below 1e-10
+    (measured as test/2d/pyrelax.pyx get_high_freq does)."""
+    from tests.test_oracle_math import high_freq
+    from cedar_amd import gallery as G
+    dx, dy = (1.0, 1e-4) if direction == "x" else (1e-4, 1.0)
+    so = G.diag_diffusion(132, 132, dx, dy)
+    x0 = np.zeros((134, 134))
+    x0[1:-1, 1:-1] = np.random.default_rng(5).uniform(size=(132, 132))
+    dso, b = dev(so), torch.zeros((134, 134), dtype=torch.float64, device="cuda")
+    sor = torch.zeros((2, 134, 134), dtype=torch.float64, device="cuda")
+    C.setup_lines(dso, sor, direction)
+    x = dev(x0)
+    for _ in range(15):
+        C.relax_lines(dso, b, x, sor, 0, direction)
+    assert high_freq(host(x)) < 1e-10
+
+
 def test_line_factorisation_reports_non_spd(C):
     so = np.zeros((3, 7, 7))
     so[0, 1:-1, 1:-1] = 1.0

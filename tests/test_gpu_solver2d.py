@@ -55,6 +55,20 @@ def test_readme_known_answer(C):
     assert float("%.6g" % abs(d[np.argmax(np.abs(d))])) == g["solution_norm"]
 
 
+@pytest.mark.parametrize("nx,ny,relax", [(200, 200, "point"), (800, 200, "line-x"),
+                                          (200, 800, "line-y")])
+@pytest.mark.parametrize("precision", ["f64", "mixed"])
+def test_reference_poisson_tests(C, nx, ny, relax, precision):
+    """test/2d/test_poisson.cc:64-159 (Isotropic, StretchX, StretchY) with the reference's default
+    settings: ||r||_2 < 1e-8 after the solve and the error against the continuous solution
+    sin(2 pi x) sin(2 pi y) below 1e-4."""
+    so, b, exact = G.sine_problem(nx, ny)
+    x, _ = C.Solver(dev(so), relaxation=relax, precision=precision).solve(dev(b))
+    x = x.cpu().numpy()
+    assert O.norm(O.residual(so, b, x)) < 1e-8
+    assert np.abs(x - exact)[1:-1, 1:-1].max() < 1e-4
+
+
 @pytest.mark.parametrize("relax,n,nine", [("point", 511, False), ("point", 200, True),
                                           ("line-xy", 255, True), ("line-x", 130, True),
                                           ("line-y", 97, False), ("line-xy", 256, False)])

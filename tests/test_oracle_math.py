@@ -264,6 +264,45 @@ def test_readme_known_answer():
     assert float("%.6g" % abs(d[np.argmax(np.abs(d))])) == g["solution_norm"]
 
 
+def high_freq(x):
+    """Largest |FFT| coefficient in the upper half of the frequencies of interior row ny//4 and
+    interior column nx//4 (test/2d/pyrelax.pyx get_high_freq, switch = 2)."""
+    xi = x[1:-1, 1:-1]
+    ny, nx = xi.shape
+    f0 = np.fft.fft(xi[ny // 4, :])
+    f1 = np.fft.fft(xi[:, nx // 4])
+    return max(np.abs(f0[nx // 4:nx // 2]).max(), np.abs(f1[ny // 4:ny // 2]).max())
+
+
+@pytest.mark.parametrize("direction", ["x", "y"])
+def test_reference_line_smoothing_tests(direction):
+    """test/2d/test_relax.cc:96-165 (LineX5, LineY5): 15 DOWN line sweeps on the 132^2 operator
+    that is strong along the lines (diag_diffusion with 1 and 1e-4), b = 0, random x, remove the
+    high frequencies (below 1e-10)."""
+    dx, dy = (1.0, 1e-4) if direction == "x" else (1e-4, 1.0)
+    so = G.diag_diffusion(132, 132, dx, dy)
+    x = np.zeros((134, 134))
+    x[1:-1, 1:-1] = np.random.default_rng(5).uniform(size=(132, 132))
+    b = np.zeros_like(x)
+    sor, info = O.setup_lines(so, direction)
+    assert info == 0
+    assert high_freq(x) > 1.0
+    for _ in range(15):
+        x = O.relax_lines(so, b, x, sor, 0, direction)
+    assert high_freq(x) < 1e-10
+
+
+@pytest.mark.parametrize("nx,ny,relax", [(200, 200, "point"), (800, 200, "line-x"),
+                                          (200, 800, "line-y")])
+def test_reference_poisson_tests(nx, ny, relax):
+    """test/2d/test_poisson.cc:64-159 (Isotropic, StretchX, StretchY): the default solver with the
+    given relaxation leaves ||r||_2 < 1e-8 and is within 1e-4 of sin(2 pi x) sin(2 pi y)."""
+    so, b, exact = G.sine_problem(nx, ny)
+    x, _ = O.Solver(so, relaxation=relax).solve(b)
+    assert O.norm(O.residual(so, b, x)) < 1e-8
+    assert np.abs(x - exact)[1:-1, 1:-1].max() < 1e-4
+
+
 @pytest.mark.parametrize("relax", ["point", "line-x", "line-y", "line-xy"])
 def test_small_solves_converge(relax):
     so, b, _ = G.heterogeneous_9pt(63, seed=11, block=4)
