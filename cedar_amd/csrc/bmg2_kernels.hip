@@ -1980,6 +1980,33 @@ bool relax_lines_y_auto(const T *so, const T *soT, int nstncl, const T *qf, T *q
     return true;
 }
 
+// Milliseconds for 3 replays of what run(cs) launches, captured as a graph on the capture stream
+// cs and replayed on s, the way the solvers execute their cycles (eager back-to-back launches
+// would leave gaps that favour the variant with fewer launches); -1 if capture fails.
+template <typename F>
+static float time_captured(hipStream_t cs, hipStream_t s, hipEvent_t e0, hipEvent_t e1, F &&run) {
+    if (!cs || hipStreamSynchronize(s) != hipSuccess) return -1.f;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) return -1.f;
+    run(cs);
+    hipGraph_t g = nullptr;
+    if (hipStreamEndCapture(cs, &g) != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        return -1.f;
+    }
+    hipGraphExec_t ge = nullptr;
+    float ms = -1.f;
+    if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) == hipSuccess) {
+        (void)hipGraphLaunch(ge, s);  // warm-up
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < 3; ++r) (void)hipGraphLaunch(ge, s);
+        (void)hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+        (void)hipGraphExecDestroy(ge);
+    }
+    (void)hipGraphDestroy(g);
+    return ms;
+}
+
 template <typename T>
 int tune_lines_ytr(const T *so, const T *soT, int nstncl, const T *qf, T *q, const T *sory,
                    T *qT, T *qfT, int ii, int jj, hipStream_t s, int nb, int sweeps) {
@@ -1999,31 +2026,37 @@ int tune_lines_ytr(const T *so, const T *soT, int nstncl, const T *qf, T *q, con
         (void)hipEventDestroy(e0);
         return -2;
     }
-    auto timed = [&](auto &&run) {
-        run();  // warm-up
+    hipStream_t cs = nullptr;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) cs = nullptr;
+    auto timed = [&](auto &&run) {  // run(stream) launches one timed unit on that stream
+        const float tc = time_captured(cs, s, e0, e1, run);
+        if (tc >= 0.f) return tc;
+        (void)hipGetLastError();  // eager timing when capture is not available
+        run(s);  // warm-up
         (void)hipEventRecord(e0, s);
-        for (int r = 0; r < 3; ++r) run();
+        for (int r = 0; r < 3; ++r) run(s);
         (void)hipEventRecord(e1, s);
         float ms = -1.f;
         if (hipEventSynchronize(e1) == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
         return ms;
     };
     // `sweeps` y sweeps share one right-hand side (one transpose of qf)
-    const float tdirect = timed([&] {
-        for (int k = 0; k < sweeps; ++k) relax_lines<T, 1>(so, nstncl, qf, q, sory, ii, jj, 0, s, nb);
+    const float tdirect = timed([&](hipStream_t st) {
+        for (int k = 0; k < sweeps; ++k) relax_lines<T, 1>(so, nstncl, qf, q, sory, ii, jj, 0, st, nb);
     });
     int best = -1;
     float tbest = tdirect;
     for (int c = 0; c < ncand; ++c) {
-        const float t = timed([&] {
+        const float t = timed([&](hipStream_t st) {
             for (int k = 0; k < sweeps; ++k)
-                relax_lines_ytr<T>(soT, nstncl, qf, q, sory, qT, qfT, ii, jj, 0, c, s, nb, k > 0);
+                relax_lines_ytr<T>(soT, nstncl, qf, q, sory, qT, qfT, ii, jj, 0, c, st, nb, k > 0);
         });
         if (t >= 0.f && t < tbest) {
             tbest = t;
             best = c;
         }
     }
+    if (cs) (void)hipStreamDestroy(cs);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (hipGetLastError() != hipSuccess) return -2;
