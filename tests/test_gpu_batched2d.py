@@ -78,8 +78,11 @@ def test_batched_not_spd_reported(C):
     assert e.value.status == 3
 
 
-def test_batchset_mixed_sizes_equal_individual(C):
-    """Mixed sizes grouped into batched solvers: each problem gets its individual result."""
+@pytest.mark.parametrize("concurrent", [True, False])
+def test_batchset_mixed_sizes_equal_individual(C, concurrent):
+    """Mixed sizes grouped into batched solvers (the groups on streams of their own, driven from
+    one host thread each, or one after another): each problem gets its individual result, on the
+    first solve and again on the second (graph replay)."""
     from cedar_amd.batch import BatchSet
     sizes = [33, 65, 33, 47, 65, 33]
     ops, rhs = [], []
@@ -89,10 +92,11 @@ def test_batchset_mixed_sizes_equal_individual(C):
         rhs.append(dev(b))
     kw = dict(relaxation="line-xy", nrelax_pre=1, nrelax_post=1, maxiter=40, tol=1e-10,
               precision="mixed")
-    bs = BatchSet(ops, **kw)
+    bs = BatchSet(ops, concurrent=concurrent, **kw)
     assert len(bs.groups) == 3
-    xs, hs = bs.solve(rhs)
+    runs = [bs.solve(rhs), bs.solve(rhs)]
     for k in range(len(sizes)):
         x1, h1 = C.Solver(ops[k], **kw).solve(rhs[k])
-        assert np.array_equal(hs[k], h1), k
-        assert torch.equal(xs[k], x1), k
+        for xs, hs in runs:
+            assert np.array_equal(hs[k], h1), k
+            assert torch.equal(xs[k], x1), k
