@@ -945,6 +945,22 @@ __global__ __launch_bounds__(256) void restrict_k(const T *__restrict__ ci, cons
 // One 2 x 2 block of fine points (2m-1 .. 2m, 2n-1 .. 2n) of interp_add: the coarse point
 // (m, n), the x-edge point (2m, 2n-1), the y-edge point (2m-1, 2n) and the cell point (2m, 2n),
 // sharing the four coarse values around the cell.  Arrays of one problem.
+// Two adjacent floats moved by one 8-byte access.  On this GPU an 8-byte global access needs only
+// 4-byte alignment, so a pair may start at any float (rows of odd length included).
+struct __attribute__((aligned(4))) float_pair {
+    float a, b;
+};
+__device__ inline void load_pair(const float *p, float &a, float &b) {
+    float_pair v;
+    __builtin_memcpy(&v, p, sizeof(v));
+    a = v.a;
+    b = v.b;
+}
+__device__ inline void store_pair(float *p, float a, float b) {
+    const float_pair v{a, b};
+    __builtin_memcpy(p, &v, sizeof(v));
+}
+
 template <typename T>
 __device__ inline void interp_block(T *__restrict__ q, const T *__restrict__ qc,
                                     const T *__restrict__ res, const T *__restrict__ so,
@@ -961,6 +977,50 @@ __device__ inline void interp_block(T *__restrict__ q, const T *__restrict__ qc,
     auto R = [&](int i, int j, std::size_t p) {
         return (i > iif - 2 || j > jjf - 2) ? res[p] : res[p] / so[p];
     };
+    if constexpr (sizeof(T) == 4) {
+        // FP32: the same operations, with horizontally adjacent values moved as pairs where the
+        // point (2m, .) exists (xe): then i0 + 1 <= iif - 1 and m + 1 <= iic - 1, so every pair
+        // lies inside its row (for even sizes i0 can be the ghost column, where xe is false)
+        const std::size_t p = pidx(i0, j0, iif);
+        T c00, c10 = T(0), c01 = T(0), c11 = T(0);
+        if (xe) {
+            load_pair(qc + pidx(m, n, iic), c00, c10);
+            if (ye) load_pair(qc + pidx(m, n + 1, iic), c01, c11);
+        } else {
+            c00 = QC(m, n);
+            if (ye) c01 = QC(m, n + 1);
+        }
+        if (xe) {  // coarse point and (2m, 2n-1)
+            T q0, q1;
+            load_pair(q + p, q0, q1);
+            q0 = q0 + c00;
+            const T a = CI(m + 1, n, LR) * c10 + CI(m + 1, n, LL) * c00;
+            q1 = q1 + a + R(i0 + 1, j0, p + 1);
+            store_pair(q + p, q0, q1);
+        } else {
+            q[p] = q[p] + c00;
+        }
+        if (ye) {  // (2m-1, 2n) and (2m, 2n)
+            const std::size_t p2 = p + iif;
+            const T a = CI(m, n + 1, LA) * c01 + CI(m, n + 1, LB) * c00;
+            if (xe) {
+                T q2, q3, r2, r3, s2, s3;
+                load_pair(q + p2, q2, q3);
+                load_pair(res + p2, r2, r3);
+                load_pair(so + p2, s2, s3);
+                q2 = q2 + a + ((i0 > iif - 2 || j0 + 1 > jjf - 2) ? r2 : r2 / s2);
+                T b = CI(m + 1, n + 1, LSW) * c00;
+                b = b + CI(m + 1, n + 1, LNW) * c01;
+                b = b + CI(m + 1, n + 1, LNE) * c11;
+                b = b + CI(m + 1, n + 1, LSE) * c10;
+                q3 = q3 + b + ((i0 + 1 > iif - 2 || j0 + 1 > jjf - 2) ? r3 : r3 / s3);
+                store_pair(q + p2, q2, q3);
+            } else {
+                q[p2] = q[p2] + a + R(i0, j0 + 1, p2);
+            }
+        }
+        return;
+    }
     const T q00 = QC(m, n);
     {  // coarse point
         const std::size_t p = pidx(i0, j0, iif);
@@ -1415,12 +1475,8 @@ __global__ __launch_bounds__(512) void transpose_k(const T *__restrict__ src, T 
     }
 }
 
-// FP32: 64 x 128 tiles moved as pairs of floats.  An 8-byte access needs only 4-byte alignment
-// on this GPU (rows of odd length start at any float), and a pair per lane moves twice the bytes
-// per instruction of the one-float form (5.1 against 4.6 TB/s on a 4097^2 grid).
-struct __attribute__((aligned(4))) float_pair {
-    float a, b;
-};
+// FP32: 64 x 128 tiles moved as pairs of floats (see float_pair), twice the bytes per instruction
+// of the one-float form (5.1 against 4.6 TB/s on a 4097^2 grid).
 __global__ __launch_bounds__(512) void transpose_pairs_k(const float *__restrict__ src,
                                                          float *__restrict__ dst, int rows,
                                                          int cols) {
