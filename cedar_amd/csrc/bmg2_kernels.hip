@@ -149,9 +149,11 @@ __global__ __launch_bounds__(256) void sumsq_k(const T *__restrict__ x, int ii, 
     block_sum_store(sq, &partials[blockIdx.y * gridDim.x + blockIdx.x]);
 }
 
-// mixed precision outer step: xnew = x + e, r = f - A xnew (FP64), r32 = (float) r
+// mixed precision outer step: xnew = x + e, r = f - A xnew (FP64), r32 = (float) r; blocks of
+// 64 (i) x IR_ROWS (j) points, 64 * IR_ROWS threads
+constexpr int IR_ROWS = 8;
 template <bool NINE, bool HAS_E>
-__global__ __launch_bounds__(256) void residual_ir_k(const double *__restrict__ so,
+__global__ __launch_bounds__(64 * IR_ROWS) void residual_ir_k(const double *__restrict__ so,
                                                      const double *__restrict__ qf,
                                                      const double *__restrict__ x,
                                                      const float *__restrict__ e,
@@ -159,10 +161,10 @@ __global__ __launch_bounds__(256) void residual_ir_k(const double *__restrict__ 
                                                      float *__restrict__ r32, int ii, int jj,
                                                      double *__restrict__ partials,
                                                      const int *__restrict__ active) {
-    // the updated iterate of the 64 x 4 block and its one-point halo, formed once per point
-    __shared__ double xs[6][66];
+    // the updated iterate of the block and its one-point halo, formed once per point
+    __shared__ double xs[IR_ROWS + 2][66];
     const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
-    const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 4;  // halo corner of the block
+    const int i0 = blockIdx.x * 64, j0 = blockIdx.y * IR_ROWS;  // halo corner of the block
     const int i = i0 + tx + 1, j = j0 + ty + 1;
     const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.z;
     so += bz * (NINE ? 5 : 3) * P;
@@ -174,7 +176,7 @@ __global__ __launch_bounds__(256) void residual_ir_k(const double *__restrict__ 
     partials += bz * gridDim.x * gridDim.y;
     // a problem of a batch that has already converged keeps its iterate (active[b] == 0)
     const bool upd = HAS_E && (active == nullptr || active[bz] != 0);
-    for (int t = threadIdx.x; t < 6 * 66; t += 256) {
+    for (int t = threadIdx.x; t < (IR_ROWS + 2) * 66; t += 64 * IR_ROWS) {
         const int ti = i0 + t % 66, tj = j0 + t / 66;
         double v = 0.0;
         if (ti < ii && tj < jj) {
@@ -205,7 +207,7 @@ __global__ __launch_bounds__(256) void residual_ir_k(const double *__restrict__ 
         r32[p] = (float)r;
         sq = r * r;
     }
-    block_sum_store(sq, &partials[blockIdx.y * gridDim.x + blockIdx.x]);
+    block_sum_store<IR_ROWS>(sq, &partials[blockIdx.y * gridDim.x + blockIdx.x]);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1395,21 +1397,23 @@ void sumsq(const T *x, int ii, int jj, double *partials, double *out, hipStream_
 void residual_ir(const double *so, int nstncl, const double *qf, const double *x, const float *e,
                  double *xnew, float *r32, int ii, int jj, double *partials, double *sumsq_out,
                  hipStream_t s, int nb, const int *active) {
-    const dim3 g = gridb(ii - 2, jj - 2, nb);
+    const dim3 g((unsigned)((ii - 2 + 63) / 64), (unsigned)((jj - 2 + IR_ROWS - 1) / IR_ROWS),
+                 (unsigned)nb);
+    const unsigned nt = 64 * IR_ROWS;
     if (nstncl == 5) {
         if (e)
-            residual_ir_k<true, true><<<g, 256, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, partials,
-                                                               active);
+            residual_ir_k<true, true><<<g, nt, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, partials,
+                                                       active);
         else
-            residual_ir_k<true, false><<<g, 256, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, partials,
-                                                               active);
+            residual_ir_k<true, false><<<g, nt, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, partials,
+                                                        active);
     } else {
         if (e)
-            residual_ir_k<false, true><<<g, 256, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, partials,
-                                                               active);
+            residual_ir_k<false, true><<<g, nt, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, partials,
+                                                        active);
         else
-            residual_ir_k<false, false><<<g, 256, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, partials,
-                                                               active);
+            residual_ir_k<false, false><<<g, nt, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, partials,
+                                                         active);
     }
     finalize_sum_k<<<nb, 1024, 0, s>>>(partials, (std::size_t)g.x * g.y, sumsq_out);
 }
