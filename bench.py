@@ -7,12 +7,16 @@ V(1,1) cycles with alternating x/y line relaxation, 11 levels.  One "step" is on
 solve from x0 = 0 to ||r|| / ||r0|| < 1e-10 with the hierarchy already built and all inputs
 resident in HBM.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 4095] [--cpu-cycles 2]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 4095] [--cpu-runs 3]
+                    [--levels-json PATH]
 
-Prints one JSON line (rank 0).  `--workload c3` instead times the 3D plane-relaxation solve
-(255^3 anisotropic 7-point finite-volume problem, plane-xy V(1,1), FP64 and mixed) and
-`--workload c4` the batched multi-size 2D solves and `--workload c5` the differentiated solves
-(forward + adjoint gradient, 16 x 1025^2); each prints its own line.
+Prints one JSON line (rank 0), headline fields first.  `--workload c1` times the reference's
+CPU-runnable case (511^2 5-point Poisson, point V(2,1)), `--workload c3` the 3D
+plane-relaxation solve (255^3 anisotropic 7-point finite-volume problem, plane-xy V(1,1), FP64
+and mixed), `--workload c4` the batched multi-size 2D solves and `--workload c5` the
+differentiated solves (forward + adjoint gradient, 16 x 1025^2); each prints its own line.  A
+record whose timed solves did not reach the tolerance is marked "valid": false and the run exits
+with status 1.
 """
 import argparse
 import ctypes
@@ -187,20 +191,15 @@ def bench_c4(args):
     16x16 blocks) and 4 uniform right-hand sides; mixed precision to 1e-10 each, line-xy V(1,1).
     Problems are grouped by size; each group is one batched solver (one launch per kernel per
     level for the whole group)."""
-    import numpy as np
     import torch
 
     from cedar_amd import gallery as G
     from cedar_amd.batch import BatchSet
-    rng = np.random.default_rng(4)
-    sizes = rng.choice([257, 513, 1025, 2049], size=args.nprob)
     t0 = time.time()
-    ops, rhs = [], [[] for _ in range(4)]
-    for k, n in enumerate(sizes):
-        so, _, _ = G.heterogeneous_9pt(int(n), seed=4000 + k, block=16, sigma=2.0, contrast=1e4)
-        ops.append(torch.as_tensor(so).cuda())
-        for r in range(4):
-            rhs[r].append(torch.as_tensor(G.rhs_uniform(int(n), int(n), 40000 + 4 * k + r)).cuda())
+    sizes, ops_np, rhs_np = G.batched_set(nprob=args.nprob)
+    ops = [torch.as_tensor(so).cuda() for so in ops_np]
+    rhs = [[torch.as_tensor(b).cuda() for b in rr] for rr in rhs_np]
+    del ops_np, rhs_np
     gen_s = time.time() - t0
     kw = dict(relaxation="line-xy", nrelax_pre=1, nrelax_post=1, maxiter=100, tol=1e-10,
               precision="mixed")
@@ -245,10 +244,7 @@ def bench_c5(args):
     from cedar_amd import gallery as G
     nb, n = 16, args.n5
     t0 = time.time()
-    theta = np.stack([G.smooth_theta(n, 700 + t) for t in range(nb)])
-    theta_obs = np.stack([G.smooth_theta(n, 900 + t) for t in range(nb)])
-    f = np.stack([G.rhs_uniform(n, n, 800 + t) for t in range(nb)])
-    noise = np.stack([G.normal(1000 + t, (n + 2) ** 2).reshape(n + 2, n + 2) for t in range(nb)])
+    theta, theta_obs, f, noise = G.differentiated_set(nb, n)
     gen_s = time.time() - t0
     dev = lambda a: torch.as_tensor(a, dtype=torch.float64).cuda()
     f_d = dev(f)
@@ -289,19 +285,137 @@ def bench_c5(args):
         "grad_rel_err_vs_fp64": rel, "gen_s": gen_s}))
 
 
+def cpu_solve_baseline(make_solver, b_np, runs, what):
+    """The FP64 oracle (plain C, one thread) on the same problem, run to convergence: one
+    warm-up (setup and one cycle), then `runs` runs of setup followed by a solve from x0 = 0 to
+    the tolerance; median setup and median solve times.  `value` is the solve alone, like the
+    GPU's `value` (hierarchy prebuilt); the setup is reported beside it."""
+    import statistics
+    o = make_solver()
+    o.vcycle(np.zeros_like(b_np), b_np)
+    setups, solves = [], []
+    hist = None
+    for _ in range(runs):
+        t = time.perf_counter()
+        o = make_solver()
+        setups.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        _, hist = o.solve(b_np)
+        solves.append(time.perf_counter() - t)
+    return {
+        "value": statistics.median(solves),
+        "unit": "s (solve from x0=0 to 1e-10; setup excluded, as in value)",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"FP64 C oracle, {what}: warm-up (setup + 1 cycle), then median of {runs} "
+                   f"runs of setup + full solve ({len(hist)} cycles to rho={hist[-1]:.2e}); host "
+                   f"'{cpu_model()}', nproc {os.cpu_count()}, 1 thread"),
+        "setup_s": statistics.median(setups),
+        "setup_plus_solve_s": statistics.median(setups) + statistics.median(solves),
+        "iters": int(len(hist)),
+        "runs": runs,
+    }
+
+
+def converged(h, tol, maxiter):
+    return bool(len(h) > 0 and h[-1] < tol and len(h) <= maxiter)
+
+
+def emit(line, ok):
+    """Prints the JSON line; a record whose solves did not reach the tolerance is marked
+    invalid and the run exits with status 1."""
+    if not ok:
+        line["valid"] = False
+        line["invalid_reason"] = "a timed solve did not reach the tolerance"
+    print(json.dumps(line), flush=True)
+    if not ok:
+        sys.exit(1)
+
+
+def bench_c1(args):
+    """The reference's CPU-runnable case: 2D 5-point Poisson on 511 x 511 interior nodes
+    (src/2d/gallery.cc:7-39), f ~ U[-1, 1] (seed 1), point red-black Gauss-Seidel V(2,1) cycles
+    to 1e-10, 8 levels; FP64 and mixed."""
+    import torch
+
+    import cedar_amd
+    from cedar_amd import _abi
+    from cedar_amd import gallery as G
+    n = 511
+    so_np, b_np = G.poisson_uniform_rhs(n, seed=1)
+    so = torch.as_tensor(so_np).cuda()
+    b = torch.as_tensor(b_np).cuda()
+    kw = dict(relaxation="point", nrelax_pre=2, nrelax_post=1, maxiter=50, tol=1e-10)
+    res = {}
+    for prec in ("f64", "mixed"):
+        s = cedar_amd.Solver(so, precision=prec, **kw)
+        setup_s = s.times()[0]
+        for _ in range(args.warmup):
+            s.solve(b)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            x, h = s.solve(b)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / args.steps
+        res[prec] = {"seconds": dt, "iters": int(len(h)), "rho": float(h[-1]),
+                     "converged": converged(h, kw["tol"], kw["maxiter"]), "setup_s": setup_s,
+                     "setup_plus_solve_s": setup_s + dt}
+    # dominant kernel: one colour of the fine-level point Gauss-Seidel (FP64), live timing
+    lib = _abi.load()
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    sor = torch.zeros((2,) + tuple(b.shape), dtype=torch.float64, device="cuda")
+    _abi.call("cdr_hip_setup_recip2d_f64", ctypes.c_void_p(so.data_ptr()),
+              ctypes.c_void_p(sor.data_ptr()), n + 2, n + 2, 3, stream)
+    q = torch.zeros_like(b)
+    tg = time_kernel(lambda: lib.cdr_hip_relax_gs2d_f64(so.data_ptr(), 3, b.data_ptr(),
+                                                        q.data_ptr(), sor.data_ptr(), n + 2,
+                                                        n + 2, 0, stream), args.kernel_reps,
+                     torch)
+    npts = n * n
+    bytes_sweep = 6 * 8 * npts  # w, s, 1/c, f, q read, q write: (S+3) words per point
+    achieved = bytes_sweep / tg / 1e9
+    cpu = None
+    if args.cpu_runs > 0:
+        from oracle import oracle as O
+        cpu = cpu_solve_baseline(lambda: O.Solver(so_np, **kw), b_np, args.cpu_runs,
+                                 f"{n}x{n} Poisson, point V(2,1)")
+    f64, mix = res["f64"], res["mixed"]
+    emit({
+        "metric": "time-to-1e-10 rel. residual (s), 2D 5-point Poisson; mixed vs FP64",
+        "value": f64["seconds"], "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": f64["seconds"] * 1e3, "higher_is_better": False,
+        "scaling": "strong", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (Poisson operator, seeded uniform RHS)",
+        "config": {"workload": f"2D 5-point Poisson {n}x{n}, f~U[-1,1] seed 1, point GS "
+                               "V(2,1), tol 1e-10", "levels": cedar_amd.Solver(so, **kw).nlevels,
+                   "step": "one FP64 solve from x0=0 to rho<1e-10, hierarchy prebuilt"},
+        "fp64": f64, "mixed": mix, "speedup_mixed_vs_fp64": f64["seconds"] / mix["seconds"],
+        "roofline": {"bound": "hbm", "kernel": "gs_colour_k<double, 5-point>, fine level, one "
+                     "sweep = 2 colour launches", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": bytes_sweep / 2, "avg_launch_s": tg / 2,
+                     "note": "the 511^2 level (2 MB per array) stays in the caches; each colour "
+                             "launch is bound by its launch latency, not by HBM"},
+        "cpu_baseline": cpu,
+    }, f64["converged"] and mix["converged"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=4095)
-    ap.add_argument("--cpu-cycles", type=int, default=2,
-                    help="V-cycles of the CPU oracle to time (0 disables the CPU baseline)")
+    ap.add_argument("--cpu-runs", type=int, default=3,
+                    help="full CPU oracle solves (median) for the CPU baseline; 0 disables it")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--kernel-only", action="store_true",
-                    help="run only the dominant kernel (fine-level FP32 y-line sweep) and a plain "
+                    help="run only the dominant kernel (fine-level FP32 x-line sweep) and a plain "
                          "256 MiB device copy, for hardware-counter runs")
-    ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5"), default="c2")
+    ap.add_argument("--levels-json", default="",
+                    help="write the per-level, per-phase bandwidth table (device timers) here")
+    ap.add_argument("--workload", choices=("c1", "c2", "c3", "c4", "c5"), default="c2")
     ap.add_argument("--n5", type=int, default=1025)
     ap.add_argument("--nprob", type=int, default=96)
     ap.add_argument("--n3", type=int, default=255)
@@ -312,14 +426,8 @@ def main():
               "(--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
 
-    if args.workload == "c3":
-        bench_c3(args)
-        return
-    if args.workload == "c4":
-        bench_c4(args)
-        return
-    if args.workload == "c5":
-        bench_c5(args)
+    if args.workload in ("c1", "c3", "c4", "c5"):
+        {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload](args)
         return
 
     import torch
@@ -361,11 +469,27 @@ def main():
             x, h = solve(prec)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t) / args.steps
-        results[prec] = dict(seconds=dt, iters=int(len(h)), rho=float(h[-1]), x=x, hist=h)
+        results[prec] = dict(seconds=dt, iters=int(len(h)), rho=float(h[-1]), x=x, hist=h,
+                             converged=converged(h, kw["tol"], kw["maxiter"]))
 
     # ---- achieved bandwidth per level and phase (the solver's device timers) ---------------
     per_level = {prec: level_rates(solvers[prec][0], b, 8 if prec == "f64" else 4, so.shape[0],
                                    torch) for prec in ("f64", "mixed")}
+    if args.levels_json:
+        with open(args.levels_json, "w") as f:
+            json.dump(per_level, f, indent=1)
+
+    def level_summary(rows):
+        """per level: all phases' algorithmic bytes / all phases' time, as a fraction of peak"""
+        out = []
+        for r in rows:
+            ph = [v for k, v in r.items() if isinstance(v, dict)]
+            if not ph:
+                continue
+            byt = sum(v["GBps"] * v["s"] for v in ph)
+            t = sum(v["s"] for v in ph)
+            out.append([r["level"], r["n"][0], round(byt / t / HBM_PEAK_GBS, 3)])
+        return out
 
     x64 = results["f64"]["x"].cpu().numpy()
     x32 = results["mixed"]["x"].cpu().numpy()
@@ -383,7 +507,6 @@ def main():
         _, _, _, _, sor0, sor1 = s.level(0)
         sor_x = torch.empty((2, jj, ii), dtype=torch.float64, device="cuda")
         sor_y = torch.empty((2, jj, ii), dtype=torch.float64, device="cuda")
-        cedar_amd.cdr2._hip()  # load runtime for copies
         nbytes = 2 * ii * jj * 8
         hip = cedar_amd.cdr2._hip()
         hip.hipMemcpy(ctypes.c_void_p(sor_x.data_ptr()), ctypes.c_void_p(sor0), nbytes, 3)
@@ -400,37 +523,17 @@ def main():
                                     None, ii, jj, 0, stream), args.kernel_reps, torch)
         ty = time_kernel(lambda: fy(A.data_ptr(), ns, f.data_ptr(), q.data_ptr(), sy.data_ptr(),
                                     None, ii, jj, 0, stream), args.kernel_reps, torch)
-        kern[prec] = dict(x_sweep_s=tx, y_sweep_s=ty, bytes_sweep=bytes_sweep,
-                          x_gbs=bytes_sweep / tx / 1e9, y_gbs=bytes_sweep / ty / 1e9)
+        kern[prec] = dict(x_sweep_s=tx, y_sweep_direct_s=ty, bytes_sweep=bytes_sweep,
+                          x_gbs=round(bytes_sweep / tx / 1e9, 1),
+                          y_direct_gbs=round(bytes_sweep / ty / 1e9, 1))
         del A, sx, sy, q, f, sor_x, sor_y
 
     # ---- CPU baseline: the FP64 oracle (single thread) on the same problem -----------------
     cpu = None
-    if args.cpu_cycles > 0:
+    if args.cpu_runs > 0:
         from oracle import oracle as O
-        t = time.perf_counter()
-        o = O.Solver(so_np, **kw)
-        setup_cpu = time.perf_counter() - t
-        xo = np.zeros_like(b_np)
-        t = time.perf_counter()
-        for _ in range(args.cpu_cycles):
-            xo = o.vcycle(xo, b_np)
-            r = O.residual(so_np, b_np, xo)
-            O.norm(r)
-        per_cycle = (time.perf_counter() - t) / args.cpu_cycles
-        it = results["f64"]["iters"]
-        cpu = {
-            "value": setup_cpu + per_cycle * it,
-            "unit": "s (setup + cycles to 1e-10, extrapolated)",
-            "cores": 1,
-            "kind": "port",
-            "sample": (f"FP64 C oracle on this {n}x{n} problem: setup ({setup_cpu:.2f} s) + "
-                       f"{args.cpu_cycles} timed V(1,1) line-xy cycles with residual+norm "
-                       f"({per_cycle:.2f} s/cycle), scaled to the {it} cycles the FP64 GPU solve "
-                       f"takes; host '{cpu_model()}', nproc {os.cpu_count()}, 1 thread"),
-            "solve_only_s": per_cycle * it,
-            "setup_s": setup_cpu,
-        }
+        cpu = cpu_solve_baseline(lambda: O.Solver(so_np, **kw), b_np, args.cpu_runs,
+                                 f"this {n}x{n} problem, line-xy V(1,1)")
 
     f64, mix = results["f64"], results["mixed"]
     km = kern["mixed"]
@@ -439,8 +542,9 @@ def main():
     # The x-line kernel is the dominant kernel of the mixed solve: it runs the x sweeps and, where
     # the autotuner chose the transposed grid ("t" entry), the y sweeps as well.
     y_transposed = shapes.get(("t", 32), "").startswith("transposed")
-    dom = ("x", km["x_sweep_s"])
-    achieved = km["bytes_sweep"] / 2 / (dom[1] / 2) / 1e9  # per launch: one colour = half
+    achieved = km["bytes_sweep"] / 2 / (km["x_sweep_s"] / 2) / 1e9  # per launch: one colour
+    traffic = measured_traffic() if n == 4095 else None
+    ok = f64["converged"] and mix["converged"]
     line = {
         "metric": "time-to-1e-10 rel. residual (s) + HBM GB/s %peak; mixed vs FP64 on same GPU",
         "value": mix["seconds"],
@@ -461,9 +565,12 @@ def main():
             "step": "one solve from x0=0 to rho<1e-10, hierarchy prebuilt",
         },
         "fp64": {"seconds": f64["seconds"], "iters": f64["iters"], "rho": f64["rho"],
-                 "setup_s": solvers["f64"][1]},
+                 "converged": f64["converged"], "setup_s": solvers["f64"][1],
+                 "setup_plus_solve_s": solvers["f64"][1] + f64["seconds"]},
         "mixed": {"seconds": mix["seconds"], "iters": mix["iters"], "rho": mix["rho"],
-                  "setup_s": solvers["mixed"][1], "rel_l2_vs_fp64": mixed_vs_f64},
+                  "converged": mix["converged"], "setup_s": solvers["mixed"][1],
+                  "setup_plus_solve_s": solvers["mixed"][1] + mix["seconds"],
+                  "rel_l2_vs_fp64": mixed_vs_f64},
         "speedup_mixed_vs_fp64": f64["seconds"] / mix["seconds"],
         "roofline": {
             "bound": "hbm",
@@ -474,17 +581,20 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic() if n == 4095 else None,
+            "traffic": traffic,
+            "traffic_source": ("profiles/relax_xlines_f32_4095_counters.md: rocprofv3 --pmc "
+                               "FETCH_SIZE and WRITE_SIZE runs of bench.py --kernel-only, "
+                               "calibrated on a 256 MiB copy" if traffic else None),
             "algorithmic_bytes_per_launch": km["bytes_sweep"] / 2,
-            "avg_launch_s": dom[1] / 2,
+            "avg_launch_s": km["x_sweep_s"] / 2,
         },
-        "kernels": kern,
-        "levels": per_level,
-        "line_shapes": {f"{d}_f{fp}": k for (d, fp), k in shapes.items()},
         "cpu_baseline": cpu,
-        "gen_s": gen_s,
+        "levels_frac": {p: level_summary(per_level[p]) for p in ("f64", "mixed")},
+        "kernels": kern,
+        "line_shapes": {f"{d}_f{fp}": k for (d, fp), k in shapes.items()},
+        "gen_s": round(gen_s, 2),
     }
-    print(json.dumps(line))
+    emit(line, ok)
 
 
 if __name__ == "__main__":

@@ -28,14 +28,14 @@ class Settings(ctypes.Structure):
     """cdr_solver_settings"""
     _fields_ = [("relaxation", _i), ("nrelax_pre", _i), ("nrelax_post", _i), ("num_levels", _i),
                 ("maxiter", _i), ("tol", _d), ("min_coarse", _i), ("precision", _i), ("graph", _i),
-                ("cycle", _i), ("fused_coarse", _i), ("periodic", _i)]
+                ("cycle", _i), ("fused_coarse", _i), ("periodic", _i), ("stall_cycles", _i)]
 
 
 class Settings3(ctypes.Structure):
     """cdr_solver3_settings"""
     _fields_ = [("relaxation", _i), ("nrelax_pre", _i), ("nrelax_post", _i), ("num_levels", _i),
                 ("maxiter", _i), ("tol", _d), ("min_coarse", _i), ("plane", Settings),
-                ("cycle", _i), ("precision", _i)]
+                ("cycle", _i), ("precision", _i), ("stall_cycles", _i)]
 
 
 # name -> (restype, argtypes); pointers to device memory are passed as c_void_p
@@ -73,6 +73,7 @@ PROTOTYPES = {
     "cdr_hip_solve_cg2d_f64": (_i, [_P, _P, _P, _P, _i, _i, _vp]),
     "cdr_hip_solve_cg2d_f32": (_i, [_P, _P, _P, _P, _i, _i, _vp]),
     "cdr_hip_default_settings": (None, [ctypes.POINTER(Settings)]),
+    "cdr_hip_stop_rule": (_i, [ctypes.POINTER(_d), _i, _d, _i, ctypes.POINTER(_i)]),
     "cdr_hip_solver2d_create": (_i, [_P, _i, _i, _i, ctypes.POINTER(Settings), _vp,
                                      ctypes.POINTER(_vp)]),
     "cdr_hip_solver2d_create_batched": (_i, [_P, _i, _i, _i, _i, ctypes.POINTER(Settings), _vp,

@@ -1,6 +1,6 @@
 """Builds the in-tree HIP library cedar_amd/lib/libcedar_hip.so for gfx950 with hipcc.
 
-    python -m cedar_amd.build        (or __graft_entry__.build())
+    python -m cedar_amd.build [--incremental]       (or __graft_entry__.build())
 
 Flags: -O3, -ffp-contract=off (no fused multiply-add: the FP64 kernels keep the reference's
 rounding order), and the packed-FP32 instructions disabled for the device code.
@@ -27,8 +27,19 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=
          "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument"]
 
 
-def _compile(src):
+def _headers_mtime():
+    hdrs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h"))]
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    hdrs += [os.path.join(inc, f) for f in os.listdir(inc) if f.endswith(".h")]
+    return max(os.path.getmtime(h) for h in hdrs)
+
+
+def _compile(src, incremental=False):
     obj = os.path.join(OBJDIR, os.path.basename(src) + ".o")
+    if incremental and os.path.exists(obj):
+        newest = max(os.path.getmtime(os.path.join(CSRC, src)), _headers_mtime())
+        if os.path.getmtime(obj) > newest:
+            return obj
     lang = ["-x", "hip"] if src.endswith(".hip") else ["-x", "hip"]
     cmd = [HIPCC] + FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -41,11 +52,13 @@ def _compile(src):
     return obj
 
 
-def build(verbose=True):
+def build(verbose=True, incremental=False):
+    """Compiles every source (incremental=True skips objects newer than their source and every
+    header) and links the library."""
     os.makedirs(OBJDIR, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     with ThreadPoolExecutor(max_workers=min(len(SOURCES), 4)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(lambda f: _compile(f, incremental), SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -58,4 +71,4 @@ def build(verbose=True):
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv)

@@ -285,7 +285,9 @@ class Solver:
     'mixed' (FP32 V-cycle preconditioning FP64 iterative refinement); graph=True replays each
     V-cycle from a captured hipGraph; cycle 'v' or 'f' (solver.cycle.type); periodic = (x, y)
     flags of grid.periodic (the operator, b and x hold the periodic images in their ghosts, as in
-    examples/basic-2d-ser/periodic.cc).
+    examples/basic-2d-ser/periodic.cc); stall_cycles=k > 0 stops a problem whose residual has
+    not reached a new minimum for k cycles (solve() then raises CedarHipError with status
+    CDR_ERR_STALLED, carrying .x and .hist), 0 (default) keeps the reference's loop.
 
     A 4-D operator (nb, nstncl, ny+2, nx+2) makes a batched solver: nb independent problems of
     the same extents solved together (every kernel launch covers the batch); solve() then takes
@@ -295,7 +297,7 @@ class Solver:
 
     def __init__(self, so, relaxation="point", nrelax_pre=2, nrelax_post=1, num_levels=-1,
                  maxiter=10, tol=1e-8, min_coarse=3, precision="f64", graph=True, cycle="v",
-                 fused_coarse=True, periodic=(False, False)):
+                 fused_coarse=True, periodic=(False, False), stall_cycles=0):
         so = _t(so) if not isinstance(so, np.ndarray) else torch.as_tensor(so).cuda()
         if so.dtype != torch.float64:
             so = so.to(torch.float64)
@@ -308,7 +310,8 @@ class Solver:
         s = _abi.Settings(_abi.RELAX[relaxation], nrelax_pre, nrelax_post, num_levels, maxiter,
                           tol, min_coarse, _abi.PRECISION[precision], int(bool(graph)),
                           _abi.CYCLE[cycle], int(bool(fused_coarse)),
-                          int(bool(periodic[0])) | (int(bool(periodic[1])) << 1))
+                          int(bool(periodic[0])) | (int(bool(periodic[1])) << 1),
+                          int(stall_cycles))
         h = ctypes.c_void_p()
         _abi.call("cdr_hip_solver2d_create_batched", _p(self.so), ns, ii - 2, jj - 2, nb,
                   ctypes.byref(s), _stream(), ctypes.byref(h))
@@ -361,9 +364,14 @@ class Solver:
         m = max(self.maxiter, 1)
         hist = (ctypes.c_double * (m * self.nb))()
         nit = (ctypes.c_int * self.nb)()
-        _abi.call("cdr_hip_solver2d_solve", self._h, _p(b), _p(x), hist, m, nit)
+        st = _abi.load().cdr_hip_solver2d_solve(self._h, _p(b), _p(x), hist, m, nit)
         hs = [np.array(hist[k * m:k * m + nit[k]]) for k in range(self.nb)]
-        return x, (hs if self.batched else hs[0])
+        out = hs if self.batched else hs[0]
+        if st != _abi.CDR_OK:
+            err = _abi.CedarHipError(st, "cdr_hip_solver2d_solve")
+            err.x, err.hist = x, out
+            raise err
+        return x, out
 
     def timers(self, on=True):
         """Per-level device timers of the V-cycle phases (cycles then run without graph replay

@@ -123,18 +123,20 @@ class Solver3:
 
     Settings use Cedar's keys and defaults: relaxation ('point' | 'plane-xy' | 'plane-xz' |
     'plane-yz' | 'plane-xyz'), nrelax_pre=2, nrelax_post=1, num_levels=-1, maxiter=10, tol=1e-8,
-    min_coarse=3, plane = plane_settings() (line-xy, one V(2,1) cycle per plane solve).
+    min_coarse=3, plane = plane_settings() (line-xy, one V(2,1) cycle per plane solve);
+    stall_cycles as for the 2D Solver.
     """
 
     def __init__(self, so, relaxation="point", nrelax_pre=2, nrelax_post=1, num_levels=-1,
-                 maxiter=10, tol=1e-8, min_coarse=3, plane=None, cycle="v", precision="f64"):
+                 maxiter=10, tol=1e-8, min_coarse=3, plane=None, cycle="v", precision="f64",
+                 stall_cycles=0):
         so = so if isinstance(so, torch.Tensor) else torch.as_tensor(np.ascontiguousarray(so))
         self.so = so.to(device="cuda", dtype=torch.float64).contiguous()  # held by reference
         ns, kk, jj, ii = self.so.shape
         self.maxiter = maxiter
         s = _abi.Settings3(_abi.RELAX[relaxation], nrelax_pre, nrelax_post, num_levels, maxiter, tol,
                            min_coarse, plane or plane_settings(), _abi.CYCLE[cycle],
-                           _abi.PRECISION[precision])
+                           _abi.PRECISION[precision], int(stall_cycles))
         h = ctypes.c_void_p()
         _abi.call("cdr_hip_solver3d_create", _p(self.so), ns, ii - 2, jj - 2, kk - 2,
                   ctypes.byref(s), _stream(), ctypes.byref(h))
@@ -186,9 +188,14 @@ class Solver3:
             x = torch.zeros_like(b)
         hist = (ctypes.c_double * max(self.maxiter, 1))()
         nit = ctypes.c_int(0)
-        _abi.call("cdr_hip_solver3d_solve", self._h, _p(b), _p(x), hist, len(hist),
-                  ctypes.byref(nit))
-        return x, np.array(hist[:nit.value])
+        st = _abi.load().cdr_hip_solver3d_solve(self._h, _p(b), _p(x), hist, len(hist),
+                                                ctypes.byref(nit))
+        h = np.array(hist[:nit.value])
+        if st != _abi.CDR_OK:
+            err = _abi.CedarHipError(st, "cdr_hip_solver3d_solve")
+            err.x, err.hist = x, h
+            raise err
+        return x, h
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:

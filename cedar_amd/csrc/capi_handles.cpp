@@ -517,6 +517,7 @@ void to_settings(const ml_keys &k, const jvalue &root, cdr_solver_settings &s) {
     s.cycle = k.cycle;
     s.graph = get_or(root, "gpu.graph", true, as_bool) ? 1 : 0;
     s.fused_coarse = get_or(root, "gpu.fused-coarse", true, as_bool) ? 1 : 0;
+    s.stall_cycles = std::max(0, get_or(root, "gpu.stall-cycles", 0, as_int));
 }
 
 bool read_precision(const jvalue &root, int &prec, std::string &err) {
@@ -1025,6 +1026,7 @@ int cedar_solver_create(cedar_mat mat, cedar_solver *solver) {
         cs.min_coarse = k.min_coarse;
         cs.cycle = k.cycle;
         cs.precision = prec;
+        cs.stall_cycles = std::max(0, get_or(root, "gpu.stall-cycles", 0, as_int));
         // plane-config: its own ml settings when present, else line-xy with one cycle
         // (src/kernel_params.cc:79-85)
         const jvalue *pc = root.member("plane-config");
@@ -1079,8 +1081,9 @@ int cedar_solver_run(cedar_solver solver, cedar_vec x, cedar_vec b) {
                                                  (int)hist.size(), &nit)
                         : cdr_hip_solver3d_solve(s->s3, s->db.p, s->dx.p, hist.data(),
                                                  (int)hist.size(), &nit);
-    // like the reference, a solve whose residual turns non-finite still returns success
-    if (st != CDR_OK && st != CDR_ERR_NONFINITE) return CEDAR_ERR_OTHER;
+    // like the reference, a solve whose residual turns non-finite still returns success; so
+    // does one stopped by the stall guard (gpu.stall-cycles), which the reference lacks
+    if (st != CDR_OK && st != CDR_ERR_NONFINITE && st != CDR_ERR_STALLED) return CEDAR_ERR_OTHER;
     if (!download(xv->data.data(), s->dx.p, s->npts)) return CEDAR_ERR_OTHER;
     for (int i = 0; i < nit; ++i) {
         std::ostringstream os;
@@ -1088,6 +1091,7 @@ int cedar_solver_run(cedar_solver solver, cedar_vec x, cedar_vec b) {
         log_line(LOG_STATUS, os.str());
     }
     if (st == CDR_ERR_NONFINITE) log_line(LOG_ERROR, "non-finite residual norm");
+    if (st == CDR_ERR_STALLED) log_line(LOG_ERROR, "residual stopped decreasing");
     return CEDAR_SUCCESS;
 }
 

@@ -201,6 +201,22 @@ void cdr_hip_default_settings(cdr_solver_settings *s) {
     s->cycle = CDR_CYCLE_V;
     s->fused_coarse = 1;
     s->periodic = 0;
+    s->stall_cycles = 0;
+}
+
+int cdr_hip_stop_rule(const double *rho, int n, double tol, int stall_cycles, int *ncycles) {
+    if ((!rho && n > 0) || n < 0 || !ncycles || !(tol >= 0.0) || stall_cycles < 0)
+        return CDR_ERR_ARG;
+    cedar::hip::stop_rule rule(tol, stall_cycles);
+    for (int i = 0; i < n; ++i) {
+        const cedar::hip::stop_rule::verdict v = rule.step(rho[i]);
+        if (v != cedar::hip::stop_rule::GO) {
+            *ncycles = i + 1;
+            return cedar::hip::stop_rule::status(v);
+        }
+    }
+    *ncycles = n;
+    return CDR_OK;
 }
 
 int cdr_hip_solver2d_create(const double *so_dev, int nstncl, int nx, int ny,
@@ -217,7 +233,8 @@ int cdr_hip_solver2d_create_batched(const double *so_dev, int nstncl, int nx, in
         s->nrelax_pre < 0 || s->nrelax_post < 0 || s->maxiter < 0 || s->min_coarse < 2 ||
         (s->precision != CDR_PREC_F64 && s->precision != CDR_PREC_MIXED) ||
         (s->graph != 0 && s->graph != 1) || (s->cycle != CDR_CYCLE_V && s->cycle != CDR_CYCLE_F) ||
-        (s->fused_coarse != 0 && s->fused_coarse != 1) || s->periodic < 0 || s->periodic > 3)
+        (s->fused_coarse != 0 && s->fused_coarse != 1) || s->periodic < 0 || s->periodic > 3 ||
+        s->stall_cycles < 0)
         return CDR_ERR_ARG;
     *out = nullptr;
     int ndev = 0;

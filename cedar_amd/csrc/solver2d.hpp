@@ -24,6 +24,7 @@
 #include "bmg2_launch.hpp"
 #include "capi2d_impl.hpp"
 #include "cedar_hip.hpp"
+#include "stop_rule.hpp"
 
 namespace cedar {
 namespace hip {
@@ -132,18 +133,21 @@ public:
         for (int p = 0; p < nb; ++p) host_active_[p] = 1;
         bool mask_dirty = false;
         // per-problem bookkeeping after a norm; returns false when every problem has stopped
+        std::vector<stop_rule> rule(nb, stop_rule(set_.tol, set_.stall_cycles));
         auto record = [&](int i) {
             for (int p = 0; p < nb; ++p) {
                 if (!host_active_[p]) continue;
                 const double rel = std::sqrt(host_norm_[p]) / res0[p];
                 if (i < maxhist && hist) hist[(std::size_t)p * maxhist + i] = rel;
                 it[p] = i + 1;
-                if (!std::isfinite(rel)) status = CDR_ERR_NONFINITE;
-                if (!std::isfinite(rel) || rel < set_.tol) {
-                    host_active_[p] = 0;
-                    --nactive;
-                    mask_dirty = true;
-                }
+                const stop_rule::verdict v = rule[p].step(rel);
+                if (v == stop_rule::GO) continue;
+                // a non-finite norm outranks a stall in the returned status
+                const int st = stop_rule::status(v);
+                if (st == CDR_ERR_NONFINITE || (st != CDR_OK && status == CDR_OK)) status = st;
+                host_active_[p] = 0;
+                --nactive;
+                mask_dirty = true;
             }
             return nactive > 0;
         };
@@ -162,8 +166,8 @@ public:
             for (int i = 0; i < set_.maxiter; ++i) {
                 const bool masked = nactive < nb;
                 if (masked) {
-                    // problems that have stopped keep their iterate through this cycle
-                    if (xprev_.size() < e.total()) xprev_ = dbuf<double>(e.total(), stream_);
+                    // problems that have stopped keep their iterate through this cycle (xprev_
+                    // is allocated with the hierarchy whenever nb > 1)
                     check_hip(hipMemcpyAsync(xprev_.data(), x.p, e.total() * sizeof(double),
                                              hipMemcpyDeviceToDevice, stream_), "save x");
                     push_mask();
@@ -355,6 +359,10 @@ private:
         work_ = dbuf<double>(cdr_hip_norm_work2d(L64_[0].ext.ii, L64_[0].ext.jj) * nb, stream_);
         sumsq_ = dbuf<double>(nb, stream_);
         active_ = dbuf<int>(nb, stream_);
+        // the FP64 loop of a batch saves the iterate of a cycle in which some problems have
+        // stopped, so that they keep theirs; allocated here, solve() allocates nothing
+        if (nb > 1 && set_.precision == CDR_PREC_F64)
+            xprev_ = dbuf<double>(L64_[0].ext.total(), stream_);
         check_hip(hipHostMalloc(&host_norm_, nb * sizeof(double)), "hipHostMalloc");
         check_hip(hipHostMalloc(&host_active_, nb * sizeof(int)), "hipHostMalloc");
 

@@ -22,6 +22,7 @@
 #include "bmg3_launch.hpp"
 #include "cedar_hip.hpp"
 #include "solver2d.hpp"
+#include "stop_rule.hpp"
 
 namespace cedar {
 namespace hip {
@@ -205,6 +206,7 @@ public:
         check_hip(hipGetLastError(), "residual");
         const double res0 = std::sqrt(read_norm2());
         int it = 0, status = CDR_OK;
+        stop_rule rule(set_.tol, set_.stall_cycles);
         for (int i = 0; i < set_.maxiter; ++i) {
             if (mixed()) {
                 k2::fill<float>(F.f.x, 0.0f, e.points(), stream_);
@@ -221,11 +223,11 @@ public:
             const double rel = std::sqrt(read_norm2()) / res0;
             if (hist && i < maxhist) hist[i] = rel;
             it = i + 1;
-            if (!std::isfinite(rel)) {
-                status = CDR_ERR_NONFINITE;
+            const stop_rule::verdict v = rule.step(rel);
+            if (v != stop_rule::GO) {
+                status = stop_rule::status(v);
                 break;
             }
-            if (rel < set_.tol) break;
         }
         if (mixed())
             check_hip(hipMemcpyAsync(x_dev, xa, e.points() * sizeof(double),
@@ -513,6 +515,7 @@ void cdr_hip_default_settings3(cdr_solver3_settings *s) {
     s->min_coarse = 3;
     s->cycle = CDR_CYCLE_V;
     s->precision = CDR_PREC_F64;
+    s->stall_cycles = 0;
     // plane-config defaults (src/kernel_params.cc:83-89): line-xy, max-iter 1
     cdr_hip_default_settings(&s->plane);
     s->plane.relaxation = CDR_RELAX_LINE_XY;
@@ -529,7 +532,7 @@ int cdr_hip_solver3d_create(const double *so_dev, int nstncl, int nx, int ny, in
         s->plane.maxiter < 0 || s->plane.nrelax_pre < 0 || s->plane.nrelax_post < 0 ||
         s->plane.min_coarse < 2 || (s->cycle != CDR_CYCLE_V && s->cycle != CDR_CYCLE_F) ||
         (s->plane.fused_coarse != 0 && s->plane.fused_coarse != 1) ||
-        (s->precision != CDR_PREC_F64 && s->precision != CDR_PREC_MIXED))
+        (s->precision != CDR_PREC_F64 && s->precision != CDR_PREC_MIXED) || s->stall_cycles < 0)
         return CDR_ERR_ARG;
     *out = nullptr;
     int ndev = 0;

@@ -362,3 +362,37 @@ def smooth_theta(n, seed):
     """Per-cell log-coefficients on (n+1) x (n+1) cells: theta ~ N(0, 1) smoothed by a 5-cell
     box filter."""
     return box_filter(normal(seed, (n + 1) * (n + 1)).reshape(n + 1, n + 1), 5)
+
+
+# ---------------------------------------------------------------------------------------------
+# workloads of many problems (batched and differentiated solves)
+# ---------------------------------------------------------------------------------------------
+def batched_set(nprob=96, sizes=(257, 513, 1025, 2049), seed=4, nrhs=4):
+    """Many independent 9-point diffusion problems of mixed sizes.
+
+    Side lengths are drawn uniformly from `sizes` (numpy Generator(seed)); problem k has its own
+    lognormal block coefficients (sigma 2, 16x16-cell blocks, contrast <= 1e4, seed
+    1000 seed + k) and `nrhs` right-hand sides f ~ U[-1, 1] (seed 10000 seed + nrhs k + r).
+    Returns (side lengths, operators, rhs) with rhs[r][k] the r-th right-hand side of problem k.
+    """
+    rng = np.random.default_rng(seed)
+    ns = [int(n) for n in rng.choice(list(sizes), size=nprob)]
+    ops, rhs = [], [[] for _ in range(nrhs)]
+    for k, n in enumerate(ns):
+        so, _, _ = heterogeneous_9pt(n, seed=1000 * seed + k, block=16, sigma=2.0, contrast=1e4)
+        ops.append(so)
+        for r in range(nrhs):
+            rhs[r].append(rhs_uniform(n, n, 10000 * seed + nrhs * k + r))
+    return ns, ops, rhs
+
+
+def differentiated_set(nb=16, n=1025):
+    """Inputs of the differentiated solves: per problem t, theta_t = smooth_theta(n, 700 + t)
+    (k = exp(theta) per cell), a second field smooth_theta(n, 900 + t) whose solution gives the
+    observations, f ~ U[-1, 1] (seed 800 + t) and N(0, 1) observation noise (seed 1000 + t).
+    Returns (theta, theta_obs, f, noise) stacked over the nb problems."""
+    theta = np.stack([smooth_theta(n, 700 + t) for t in range(nb)])
+    theta_obs = np.stack([smooth_theta(n, 900 + t) for t in range(nb)])
+    f = np.stack([rhs_uniform(n, n, 800 + t) for t in range(nb)])
+    noise = np.stack([normal(1000 + t, (n + 2) ** 2).reshape(n + 2, n + 2) for t in range(nb)])
+    return theta, theta_obs, f, noise

@@ -283,12 +283,26 @@ typedef struct {
                          and x carry the periodic images in their ghosts (the periodic kernels
                          above); line relaxation then runs one lane per line, and fused_coarse
                          and line-shape tuning do not apply */
+    int stall_cycles; /* 0 (default, the reference's loop, include/cedar/multilevel.h:283-297):
+                         cycle until rho < tol or max-iter.  k > 0: a problem whose rho has not
+                         gone below its smallest earlier value (rho_0 = 1) for k consecutive
+                         cycles stops there, and the solve returns CDR_ERR_STALLED (see
+                         cdr_hip_stop_rule) */
 } cdr_solver_settings;
 
 typedef struct cdr_hip_solver2d cdr_hip_solver2d;
 
 /* Cedar's defaults (src/multilevel_settings.cc:15-61). */
 void cdr_hip_default_settings(cdr_solver_settings *s);
+
+/* The stopping rule of every solve loop (2D, 3D, batched), on host data: feeds rho[0..n-1]
+ * (the relative residuals after cycles 1..n) through the rule in order and stores in *ncycles
+ * the number of cycles the loop runs (n when none stops it, as when max-iter ends the loop).
+ * Returns CDR_OK (converged, or no stop within n), CDR_ERR_NONFINITE (a NaN/Inf rho) or
+ * CDR_ERR_STALLED (stall_cycles > 0 cycles without a new smallest rho).  tol < 0 or
+ * stall_cycles < 0 is CDR_ERR_ARG.  No GPU needed.  Host-side counterpart of the loop test in
+ * include/cedar/multilevel.h:283-297 plus the two guards. */
+int cdr_hip_stop_rule(const double *rho, int n, double tol, int stall_cycles, int *ncycles);
 
 /* Builds the hierarchy on the GPU from the fine operator so_dev (device, held by reference as
  * Cedar holds the fine operator, include/cedar/level.h:25,30) of nx x ny interior points.
@@ -403,6 +417,7 @@ typedef struct {
     int cycle;        /* CDR_CYCLE_V (default) or CDR_CYCLE_F */
     int precision;    /* CDR_PREC_F64 (default) or CDR_PREC_MIXED: FP32 3D cycle and FP32 plane
                          solves inside FP64 iterative refinement */
+    int stall_cycles; /* as in cdr_solver_settings (default 0: no stall test) */
 } cdr_solver3_settings;
 
 typedef struct cdr_hip_solver3d cdr_hip_solver3d;

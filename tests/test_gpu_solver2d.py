@@ -201,6 +201,38 @@ def test_fcycle_matches_oracle(C, relax, n, nine, precision):
         # matches the plain F-cycle and the later ones keep reducing the residual
         assert abs(hg[0] - ho[0]) <= 1e-5 * ho[0]
         assert hg[2] < hg[1] < hg[0]
+        # run to the tolerance: the FP64 oracle residual of the returned x meets it
+        g = C.Solver(dev(so), relaxation=relax, nrelax_pre=1, nrelax_post=1, maxiter=60,
+                     tol=1e-10, cycle="f", precision="mixed")
+        xg, hg = g.solve(dev(b))
+        assert hg[-1] < 1e-10 and len(hg) < 60
+        rr = O.norm(O.residual(so, b, xg.cpu().numpy())) / O.norm(b)
+        assert rr < 1e-10, rr
+
+
+@pytest.mark.parametrize("precision", ["f64", "mixed"])
+def test_stall_guard_stops_a_solve_that_cannot_reach_tol(C, precision):
+    """stall_cycles = 3 with tol = 0: once rounding floors the residual, three cycles without a
+    new minimum stop the solve with CDR_ERR_STALLED; the cycles before the stop are those of the
+    unguarded solve, and the stopping rule applied to the history gives the same stop."""
+    import ctypes
+    so, b = G.poisson_uniform_rhs(63, seed=3)
+    kw = dict(relaxation="point", nrelax_pre=2, nrelax_post=1, tol=0.0, precision=precision)
+    s = C.Solver(dev(so), maxiter=200, stall_cycles=3, **kw)
+    with pytest.raises(C.CedarHipError) as ei:
+        s.solve(dev(b))
+    assert ei.value.status == 7
+    h = ei.value.hist
+    assert 3 < len(h) < 200
+    assert min(h[-3:]) >= min(h[:-3])
+    assert h[-4] < 1e-12  # it stalled at the rounding floor, not early
+    arr = (ctypes.c_double * len(h))(*h)
+    n = ctypes.c_int()
+    assert C.load().cdr_hip_stop_rule(arr, len(h), 0.0, 3, ctypes.byref(n)) == 7
+    assert n.value == len(h)
+    x2, h2 = C.Solver(dev(so), maxiter=len(h) + 4, **kw).solve(dev(b))
+    assert len(h2) == len(h) + 4
+    assert np.array_equal(h2[:len(h)], h)
 
 
 @pytest.mark.parametrize("relaxation", ["point", "line-x", "line-y", "line-xy"])

@@ -1,0 +1,144 @@
+"""The batched and differentiated workloads at their full sizes, checked against the FP64 oracle.
+
+Batched (96 9-point problems, sides drawn from {257, 513, 1025, 2049}, 4 right-hand sides each,
+mixed precision, line-xy V(1,1), tol 1e-10; `gallery.batched_set`): every returned solution has
+an oracle FP64 relative residual below 1e-10, and for the first problem of each size the solution
+is within 1e-9 (relative l2) of the oracle's own FP64 solve with the same cycle count.
+
+Differentiated (16 problems of 1025^2, k = exp(theta); `gallery.differentiated_set`): dL/dtheta
+from the adjoint solve matches the FP64 CPU gradient (oracle forward and adjoint solves,
+tests/adjoint_ref.py) for two of the problems, and central finite differences of the GPU loss on
+32 random cells spread over the 16 problems.
+
+Finite-difference tolerance.  The loss is about 1e5 (u reaches 1e2), and an FP64 multigrid solve
+evaluates it reproducibly only to about 1e-12 relative: its rounding-floor error is smooth, so
+it adds up coherently over the 10^6 nodes.  At h = 1e-6, the step the workload prescribes, that
+is a noise of about 1e-12 * 1e5 / 1e-6 = 0.1 in each difference quotient, against a largest
+gradient entry near 190.  The CPU oracle shows it: the difference to the adjoint gradient scales
+like 1/h (h = 1e-5: 4e-3, 1e-4: 1e-3, 1e-3: 7e-5 at one cell).  The h = 1e-6 check therefore
+allows 2e-3 of the largest entry of the problem's gradient.  A second check at h = 1e-3, where
+that noise is 1e3 times smaller and the O(h^2) truncation stays near 1e-7, allows 5e-6.
+Both use losses from exactly 16 cycles of every solve (tol 0), so the + and - solves run the
+same iteration.
+"""
+import numpy as np
+import pytest
+
+from cedar_amd import gallery as G
+from oracle import oracle as O
+from tests import adjoint_ref as R
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64).cuda()
+
+
+def rel_l2(a, b):
+    return np.linalg.norm((a - b)[1:-1, 1:-1]) / np.linalg.norm(b[1:-1, 1:-1])
+
+
+KW = dict(relaxation="line-xy", nrelax_pre=1, nrelax_post=1, maxiter=100, tol=1e-10)
+
+
+def test_batched_workload_against_oracle():
+    from cedar_amd.batch import BatchSet
+    ns, ops, rhs = G.batched_set()
+    print(f"batched workload: {len(ns)} problems, sizes {sorted(set(ns))}, "
+          f"counts {[ns.count(n) for n in sorted(set(ns))]}", flush=True)
+    bs = BatchSet([dev(o) for o in ops], precision="mixed", **KW)
+    first = {n: ns.index(n) for n in sorted(set(ns))}
+    worst = 0.0
+    hist0 = None
+    for r in range(len(rhs)):
+        xs, hs = bs.solve([dev(b) for b in rhs[r]])
+        torch.cuda.synchronize()
+        if r == 0:
+            hist0 = hs
+            x0 = {n: xs[k].cpu().numpy() for n, k in first.items()}
+        for k in range(len(ns)):
+            assert hs[k][-1] < 1e-10 and len(hs[k]) < KW["maxiter"], (r, k, hs[k][-1])
+            x = xs[k].cpu().numpy()
+            rr = O.norm(O.residual(ops[k], rhs[r][k], x)) / O.norm(rhs[r][k])
+            worst = max(worst, rr)
+            assert rr < 1e-10, (r, k, ns[k], rr)
+        print(f"rhs {r}: worst oracle residual so far {worst:.3e}", flush=True)
+    for n, k in first.items():
+        xo, ho = O.Solver(ops[k], **KW).solve(rhs[0][k])
+        err = rel_l2(x0[n], xo)
+        print(f"size {n} (problem {k}): {len(hist0[k])} cycles (oracle {len(ho)}), "
+              f"rel l2 vs oracle {err:.3e}", flush=True)
+        assert len(hist0[k]) == len(ho), (n, len(hist0[k]), len(ho))
+        assert err <= 1e-9, (n, err)
+
+
+@pytest.fixture(scope="module")
+def diff_set():
+    import cedar_amd.adjoint as A
+    n, nb = 1025, 16
+    theta, theta_obs, f, noise = G.differentiated_set(nb, n)
+    with torch.no_grad():
+        uobs = A.solve(dev(theta_obs), dev(f), precision="f64", tol=1e-12)
+    uobs = (uobs + 1e-3 * dev(noise)).cpu().numpy()
+    uobs[:, 0, :] = uobs[:, -1, :] = 0.0
+    uobs[:, :, 0] = uobs[:, :, -1] = 0.0
+    return A, theta, f, uobs
+
+
+def loss_grad(A, theta, f, uobs, **kw):
+    th = dev(theta).requires_grad_(True)
+    u = A.solve(th, dev(f), **kw)
+    per = 0.5 * ((u - dev(uobs)) ** 2)[..., 1:-1, 1:-1].sum(dim=(-1, -2))
+    per.sum().backward()
+    return per.detach().cpu().numpy(), th.grad.cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def cpu_grads(diff_set):
+    """FP64 CPU gradients (oracle forward and adjoint solves to 1e-13) of problems 0 and 15."""
+    _, theta, f, uobs = diff_set
+    return {t: R.gradient(theta[t], f[t], uobs[t])[0] for t in (0, 15)}
+
+
+@pytest.mark.parametrize("precision,tol,bound", [("mixed", 1e-10, 1e-6), ("f64", 1e-13, 1e-9)])
+def test_differentiated_workload_gradient_vs_cpu_fp64(diff_set, cpu_grads, precision, tol, bound):
+    A, theta, f, uobs = diff_set
+    _, g = loss_grad(A, theta, f, uobs, precision=precision, tol=tol, maxiter=100)
+    for t, gc in cpu_grads.items():
+        err = np.abs(g[t] - gc).max() / np.abs(gc).max()
+        print(f"problem {t} {precision}: max |g - g_cpu| / max |g_cpu| = {err:.3e}", flush=True)
+        assert err <= bound, (t, err)
+
+
+def test_differentiated_workload_gradient_vs_finite_differences(diff_set):
+    """Central differences of the FP64 GPU loss at 32 cells, two per problem, for h = 1e-6 and
+    h = 1e-3; all 128 perturbed problems are solved as one batch."""
+    A, theta, f, uobs = diff_set
+    nb, n = theta.shape[0], theta.shape[1] - 1
+    _, g = loss_grad(A, theta, f, uobs, precision="f64", tol=1e-13, maxiter=100)
+    rng = np.random.default_rng(5)
+    cells = [(t, int(cj), int(ci)) for t in range(nb)
+             for cj, ci in rng.integers(0, n + 1, size=(2, 2))]
+    steps = [(1e-6, 2e-3), (1e-3, 5e-6)]
+    runs = [(t, cj, ci, h, sgn) for h, _ in steps for t, cj, ci in cells for sgn in (1, -1)]
+    tp = np.stack([theta[t] for t, _, _, _, _ in runs])
+    for m, (t, cj, ci, h, sgn) in enumerate(runs):
+        tp[m, cj, ci] += sgn * h
+    fp = np.stack([f[t] for t, _, _, _, _ in runs])
+    up = np.stack([uobs[t] for t, _, _, _, _ in runs])
+    with torch.no_grad():
+        u = A.solve(dev(tp), dev(fp), precision="f64", tol=0.0, maxiter=16)
+        per = (0.5 * ((u - dev(up)) ** 2)[..., 1:-1, 1:-1].sum(dim=(-1, -2))).cpu().numpy()
+    del u
+    m = 0
+    for h, bound in steps:
+        worst = 0.0
+        for t, cj, ci in cells:
+            fd = (per[m] - per[m + 1]) / (2 * h)
+            m += 2
+            dev_rel = abs(fd - g[t, cj, ci]) / np.abs(g[t]).max()
+            worst = max(worst, dev_rel)
+            assert dev_rel <= bound, (h, t, cj, ci, fd, g[t, cj, ci])
+        print(f"finite differences h={h:g}: max |fd - g| / max |g| = {worst:.3e}", flush=True)
