@@ -89,6 +89,8 @@ PROTOTYPES = {
     "cdr_hip_solver2d_timers": (_i, [_vp, _i]),
     "cdr_hip_solver2d_save_timers": (_i, [_vp, ctypes.c_char_p]),
     "cdr_hip_stencil2d_write": (_i, [_P, _i, _i, _i, ctypes.c_char_p]),
+    "cdr_hip_solver2d_kernels": (_i, [_vp, ctypes.c_char_p, _i]),
+    "cdr_hip_solver2d_set_kernel": (_i, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
     "cdr_hip_solver2d_destroy": (None, [_vp]),
     # 3D kernels
     "cdr_hip_norm_work3d": (_sz, [_i, _i, _i]),
@@ -99,12 +101,15 @@ PROTOTYPES = {
     "cdr_hip_relax_gs3d_f64": (_i, [_P, _i, _P, _P, _P, _i, _i, _i, _i, _vp]),
     "cdr_hip_relax_gs3d_f32": (_i, [_P, _i, _P, _P, _P, _i, _i, _i, _i, _vp]),
     "cdr_hip_restrict3d_f64": (_i, [_P, _P, _P, _i, _i, _i, _i, _i, _i, _vp]),
+    "cdr_hip_restrict3d_f32": (_i, [_P, _P, _P, _i, _i, _i, _i, _i, _i, _vp]),
     "cdr_hip_interp_add3d_f64": (_i, [_P, _P, _P, _P, _i, _P, _i, _i, _i, _i, _i, _i, _vp]),
+    "cdr_hip_interp_add3d_f32": (_i, [_P, _P, _P, _P, _i, _P, _i, _i, _i, _i, _i, _i, _vp]),
     "cdr_hip_setup_interp3d_f64": (_i, [_P, _i, _P, _i, _i, _i, _i, _i, _i, _vp]),
     "cdr_hip_setup_galerkin3d_f64": (_i, [_P, _i, _P, _P, _i, _i, _i, _i, _i, _i, _vp]),
     "cdr_hip_cg_bandwidth3d": (_i, [_i, _i]),
     "cdr_hip_setup_cg3d_f64": (_i, [_P, _i, _P, _i, _i, _i, _i, _P, _vp]),
     "cdr_hip_solve_cg3d_f64": (_i, [_P, _P, _P, _i, _P, _i, _i, _i, _vp]),
+    "cdr_hip_solve_cg3d_f32": (_i, [_P, _P, _P, _i, _P, _i, _i, _i, _vp]),
     # 3D solver
     "cdr_hip_default_settings3": (None, [ctypes.POINTER(Settings3)]),
     "cdr_hip_solver3d_create": (_i, [_P, _i, _i, _i, _i, ctypes.POINTER(Settings3), _vp,
@@ -116,6 +121,8 @@ PROTOTYPES = {
     "cdr_hip_solver3d_level": (_i, [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i),
                                     ctypes.POINTER(_i), ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "cdr_hip_solver3d_times": (_i, [_vp, ctypes.POINTER(_d), ctypes.POINTER(_d)]),
+    "cdr_hip_solver3d_kernels": (_i, [_vp, ctypes.c_char_p, _i]),
+    "cdr_hip_solver3d_set_kernel": (_i, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
     "cdr_hip_solver3d_destroy": (None, [_vp]),
     # periodic boundaries
     "cdr_hip_relax_gs2d_per_f64": (_i, [_P, _i, _P, _P, _P, _i, _i, _i, _i, _vp]),

@@ -373,6 +373,15 @@ class Solver:
             raise err
         return x, out
 
+    def kernels(self):
+        """The kernel registry: {interface: {"impls": [...], "default": name}}."""
+        from .cdr3 import _registry
+        return _registry(_abi.load().cdr_hip_solver2d_kernels, self._h)
+
+    def set_kernel(self, interface, impl):
+        """kman->set<I>(impl) for the interface named `interface` (both precisions)."""
+        _abi.call("cdr_hip_solver2d_set_kernel", self._h, interface.encode(), impl.encode())
+
     def timers(self, on=True):
         """Per-level device timers of the V-cycle phases (cycles then run without graph replay
         and fused coarse levels; the results are the same)."""

@@ -61,13 +61,14 @@ def relax_gs(so, qf, q, sor, updown):
 
 
 def restrict(ci, q, qc):
-    _abi.call("cdr_hip_restrict3d_f64", _p(ci), _p(q), _p(qc), *_dims(q), *_dims(qc), _stream())
+    _abi.call(f"cdr_hip_restrict3d_{_sfx(q)}", _p(ci), _p(q), _p(qc), *_dims(q), *_dims(qc),
+              _stream())
     return qc
 
 
 def interp_add(q, qc, res, so, ci):
-    _abi.call("cdr_hip_interp_add3d_f64", _p(q), _p(qc), _p(res), _p(so), so.shape[0], _p(ci),
-              *_dims(qc), *_dims(q), _stream())
+    _abi.call(f"cdr_hip_interp_add3d_{_sfx(q)}", _p(q), _p(qc), _p(res), _p(so), so.shape[0],
+              _p(ci), *_dims(qc), *_dims(q), _stream())
     return q
 
 
@@ -104,10 +105,21 @@ def setup_cg(so):
 
 
 def solve_cg(q, qf, abd):
+    """Coarsest solve with the FP64 band abd; q and qf FP64 or FP32 (then solved in FP64)."""
     bbd = torch.zeros(abd.shape[0], dtype=torch.float64, device=q.device)
-    _abi.call("cdr_hip_solve_cg3d_f64", _p(q), _p(qf), _p(abd), abd.shape[1], _p(bbd), *_dims(q),
-              _stream())
+    _abi.call(f"cdr_hip_solve_cg3d_{_sfx(q)}", _p(q), _p(qf), _p(abd), abd.shape[1], _p(bbd),
+              *_dims(q), _stream())
     return q
+
+
+def _registry(fn, h):
+    import json
+    n = fn(h, None, 0)
+    if n < 0:
+        raise _abi.CedarHipError(-n, "kernel registry")
+    buf = ctypes.create_string_buffer(n + 1)
+    fn(h, buf, n + 1)
+    return json.loads(buf.value.decode())
 
 
 def plane_settings(relaxation="line-xy", nrelax_pre=2, nrelax_post=1, maxiter=1, tol=1e-8,
@@ -176,6 +188,14 @@ class Solver3:
     def vcycle(self, x, b):
         _abi.call("cdr_hip_solver3d_vcycle", self._h, _p(b), _p(x))
         return x
+
+    def kernels(self):
+        """The kernel registry: {interface: {"impls": [...], "default": name}}."""
+        return _registry(_abi.load().cdr_hip_solver3d_kernels, self._h)
+
+    def set_kernel(self, interface, impl):
+        """kman->set<I>(impl) for the interface named `interface`."""
+        _abi.call("cdr_hip_solver3d_set_kernel", self._h, interface.encode(), impl.encode())
 
     def relax_planes(self, lvl, family, x, b, updown):
         _abi.call("cdr_hip_solver3d_relax_planes", self._h, lvl, _abi.PLANE_FAMILY[family], _p(x),

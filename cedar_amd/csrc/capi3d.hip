@@ -40,6 +40,33 @@ int gs3_t(const T *so, int ns, const T *qf, T *q, const T *sor, int ii, int jj, 
     return launched();
 }
 
+template <typename T>
+int restrict3_t(const T *ci, const T *q, T *qc, int iif, int jjf, int kkf, int iic, int jjc,
+                int kkc, void *st) {
+    if (!ci || !q || !qc || !coarse3_ok(iif, jjf, kkf, iic, jjc, kkc)) return CDR_ERR_ARG;
+    k3::restrict_<T>(ci, q, qc, iif, jjf, kkf, iic, jjc, kkc, S(st));
+    return launched();
+}
+
+template <typename T>
+int interp3_t(T *q, const T *qc, const T *res, const T *so, int ns, const T *ci, int iic, int jjc,
+              int kkc, int iif, int jjf, int kkf, void *st) {
+    if (!q || !qc || !res || !so || !ci || !sten3_ok(ns) ||
+        !coarse3_ok(iif, jjf, kkf, iic, jjc, kkc))
+        return CDR_ERR_ARG;
+    k3::interp_add<T>(q, qc, res, so, ns, ci, iic, jjc, kkc, iif, jjf, kkf, S(st));
+    return launched();
+}
+
+template <typename T>
+int solve_cg3_t(T *q, const T *qf, const double *abd, int ldab, double *bbd, int ii, int jj,
+                int kk, void *st) {
+    if (!q || !qf || !abd || !bbd || !ext3_ok(ii, jj, kk) || ldab < k3::cg_bandwidth(ii, jj) + 1)
+        return CDR_ERR_ARG;
+    k3::solve_cg<T>(q, qf, abd, ldab, bbd, ii, jj, kk, S(st));
+    return launched();
+}
+
 }  // namespace
 
 extern "C" {
@@ -84,19 +111,22 @@ int cdr_hip_relax_gs3d_f32(const float *so, int ns, const float *qf, float *q, c
 
 int cdr_hip_restrict3d_f64(const double *ci, const double *q, double *qc, int iif, int jjf,
                            int kkf, int iic, int jjc, int kkc, void *st) {
-    if (!ci || !q || !qc || !coarse3_ok(iif, jjf, kkf, iic, jjc, kkc)) return CDR_ERR_ARG;
-    k3::restrict_<double>(ci, q, qc, iif, jjf, kkf, iic, jjc, kkc, S(st));
-    return launched();
+    return restrict3_t<double>(ci, q, qc, iif, jjf, kkf, iic, jjc, kkc, st);
+}
+int cdr_hip_restrict3d_f32(const float *ci, const float *q, float *qc, int iif, int jjf, int kkf,
+                           int iic, int jjc, int kkc, void *st) {
+    return restrict3_t<float>(ci, q, qc, iif, jjf, kkf, iic, jjc, kkc, st);
 }
 
 int cdr_hip_interp_add3d_f64(double *q, const double *qc, const double *res, const double *so,
                              int ns, const double *ci, int iic, int jjc, int kkc, int iif,
                              int jjf, int kkf, void *st) {
-    if (!q || !qc || !res || !so || !ci || !sten3_ok(ns) ||
-        !coarse3_ok(iif, jjf, kkf, iic, jjc, kkc))
-        return CDR_ERR_ARG;
-    k3::interp_add<double>(q, qc, res, so, ns, ci, iic, jjc, kkc, iif, jjf, kkf, S(st));
-    return launched();
+    return interp3_t<double>(q, qc, res, so, ns, ci, iic, jjc, kkc, iif, jjf, kkf, st);
+}
+int cdr_hip_interp_add3d_f32(float *q, const float *qc, const float *res, const float *so, int ns,
+                             const float *ci, int iic, int jjc, int kkc, int iif, int jjf, int kkf,
+                             void *st) {
+    return interp3_t<float>(q, qc, res, so, ns, ci, iic, jjc, kkc, iif, jjf, kkf, st);
 }
 
 int cdr_hip_setup_interp3d_f64(const double *so, int ns, double *ci, int iif, int jjf, int kkf,
@@ -134,10 +164,11 @@ int cdr_hip_setup_cg3d_f64(const double *so, int ns, double *abd, int ldab, int 
 
 int cdr_hip_solve_cg3d_f64(double *q, const double *qf, const double *abd, int ldab, double *bbd,
                            int ii, int jj, int kk, void *st) {
-    if (!q || !qf || !abd || !bbd || !ext3_ok(ii, jj, kk) || ldab < k3::cg_bandwidth(ii, jj) + 1)
-        return CDR_ERR_ARG;
-    k3::solve_cg<double>(q, qf, abd, ldab, bbd, ii, jj, kk, S(st));
-    return launched();
+    return solve_cg3_t<double>(q, qf, abd, ldab, bbd, ii, jj, kk, st);
+}
+int cdr_hip_solve_cg3d_f32(float *q, const float *qf, const double *abd, int ldab, double *bbd,
+                           int ii, int jj, int kk, void *st) {
+    return solve_cg3_t<float>(q, qf, abd, ldab, bbd, ii, jj, kk, st);
 }
 
 }  // extern "C"

@@ -12,6 +12,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
 #include <map>
 #include <memory>
@@ -254,8 +255,45 @@ public:
         auto k = std::make_shared<Impl>(std::forward<Args>(args)...);
         k->stream = stream_;
         auto &slot = table_[std::type_index(typeid(I))];
+        slot.iface = I::name();
         slot.impls[name] = k;
         if (slot.def.empty()) slot.def = name;
+    }
+
+    // set<I>(name) by the interface's name (I::name()); applies to every registered interface
+    // of that name (the FP64 and FP32 forms share it).  Unknown interface or implementation:
+    // CDR_ERR_ARG, nothing changed.
+    void set_by_name(const std::string &iface, const std::string &name) {
+        bool found = false;
+        for (auto &t : table_)
+            if (t.second.iface == iface) {
+                if (!t.second.impls.count(name))
+                    throw error(CDR_ERR_ARG, "kernel not found: " + iface + "/" + name);
+                found = true;
+            }
+        if (!found) throw error(CDR_ERR_ARG, "no such kernel interface: " + iface);
+        for (auto &t : table_)
+            if (t.second.iface == iface) t.second.def = name;
+    }
+
+    // {"interface": {"impls": [...], "default": "..."}, ...} over the registered interfaces
+    std::string describe() const {
+        std::map<std::string, std::pair<std::vector<std::string>, std::string>> by;
+        for (const auto &t : table_) {
+            auto &e = by[t.second.iface];
+            for (const auto &kv : t.second.impls)
+                if (std::find(e.first.begin(), e.first.end(), kv.first) == e.first.end())
+                    e.first.push_back(kv.first);
+            e.second = t.second.def;
+        }
+        std::string out = "{";
+        for (const auto &kv : by) {
+            out += (out.size() > 1 ? "," : "") + std::string("\"") + kv.first + "\": {\"impls\": [";
+            for (std::size_t i = 0; i < kv.second.first.size(); ++i)
+                out += (i ? ",\"" : "\"") + kv.second.first[i] + "\"";
+            out += "], \"default\": \"" + kv.second.second + "\"}";
+        }
+        return out + "}";
     }
 
     template <class I>
@@ -297,6 +335,7 @@ public:
 
 private:
     struct slot_t {
+        std::string iface;  // I::name()
         std::map<std::string, std::shared_ptr<kernel_base>> impls;
         std::string def;
     };

@@ -2,10 +2,12 @@
 // solver half of the 2D C ABI; the solver itself is in solver2d.hpp.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -318,6 +320,25 @@ int cdr_hip_stencil2d_write(const double *so_dev, int nstncl, int ii, int jj, co
         std::fwrite(text.data(), 1, text.size(), f);
         std::fclose(f);
         return CDR_OK;
+    });
+}
+
+int cdr_hip_solver2d_kernels(cdr_hip_solver2d *slv, char *buf, int len) {
+    if (!slv || len < 0 || (len > 0 && !buf)) return -CDR_ERR_ARG;
+    const std::string d = slv->impl->kernels().describe();
+    if (len > 0) {
+        const std::size_t n = std::min<std::size_t>(d.size(), (std::size_t)len - 1);
+        std::memcpy(buf, d.data(), n);
+        buf[n] = '\0';
+    }
+    return (int)d.size();
+}
+
+int cdr_hip_solver2d_set_kernel(cdr_hip_solver2d *slv, const char *iface, const char *impl) {
+    if (!slv || !iface || !impl) return CDR_ERR_ARG;
+    return guarded([&] {
+        slv->impl->kernels().set_by_name(iface, impl);
+        return (int)CDR_OK;
     });
 }
 

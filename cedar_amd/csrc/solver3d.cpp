@@ -9,18 +9,26 @@
 // the reference's per-plane cdr2 solvers do.
 //
 // Control flow: include/cedar/3d/solver.h:17-128 (levels), include/cedar/multilevel.h:95-298
-// (setup order, smoother sequencing, solve loop), include/cedar/cycle/vcycle.h:57-115.
+// (setup order, smoother sequencing, solve loop), include/cedar/cycle/vcycle.h:57-115.  The
+// solver calls every kernel by interface through its kernel_manager (the "hip" implementations
+// below, registered by register_hip_kernels3), as multilevel.h does through kman.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
+#include <map>
 #include <memory>
+#include <string>
+#include <tuple>
 #include <type_traits>
 #include <vector>
 
 #include "bmg2_launch.hpp"
 #include "bmg3_launch.hpp"
 #include "cedar_hip.hpp"
+#include "cedar_hip3.hpp"
 #include "solver2d.hpp"
 #include "stop_rule.hpp"
 
@@ -114,11 +122,171 @@ private:
     colour col_[2];
 };
 
-// one precision's arrays of a level
+// ------------------------------------------------------------------------------------------
+// "hip" implementations of the 3D interfaces
+// ------------------------------------------------------------------------------------------
+namespace {
+
+inline void launched3(const char *what) { check_hip(hipGetLastError(), what); }
+
+template <typename T>
+struct hip_residual3 : kernels3::residual<T> {
+    void run(const stencil_op3<T> &so, const grid_func3<T> &x, const grid_func3<T> &b,
+             grid_func3<T> &r) override {
+        const extents3 e = so.ext;
+        k3::residual<T>(so.p, so.nstncl, b.p, x.p, r.p, e.ii, e.jj, e.kk, this->stream);
+        launched3("residual3");
+    }
+};
+
+template <typename T>
+struct hip_rbgs3 : kernels3::point_relax<T> {
+    void setup(const stencil_op3<double> &so, relax_stencil3<double> &sor) override {
+        const extents3 e = so.ext;
+        k3::setup_recip(so.p, so.nstncl, sor.p, e.ii, e.jj, e.kk, this->stream);
+        launched3("setup_recip3");
+    }
+    void run(const stencil_op3<T> &so, grid_func3<T> &x, const grid_func3<T> &b,
+             const relax_stencil3<T> &sor, Dir d) override {
+        const extents3 e = so.ext;
+        k3::relax_gs<T>(so.p, so.nstncl, b.p, x.p, sor.p, e.ii, e.jj, e.kk, (int)d, this->stream);
+        launched3("relax_gs3");
+    }
+};
+
+// The plane solvers of one family, per level (keyed by the level's extents, which differ from
+// level to level; the reference keys them by the first extent, include/cedar/3d/relax_planes.h
+// :157-161).  Shared by the FP64 and FP32 registrations of the family: a mixed-precision
+// family holds both hierarchies.
+struct plane_store {
+    cdr_solver_settings s2;
+    std::map<std::tuple<int, int, int>, std::unique_ptr<plane_family>> fam;
+    plane_family *find(const extents3 &e) {
+        const auto it = fam.find(std::make_tuple(e.ii, e.jj, e.kk));
+        if (it == fam.end()) throw error(CDR_ERR_ARG, "plane relaxation: level not set up");
+        return it->second.get();
+    }
+};
+
+template <typename T, plane_dir PD>
+struct hip_planes : kernels3::plane_relax<T, PD> {
+    explicit hip_planes(std::shared_ptr<plane_store> st) : store(std::move(st)) {}
+    std::shared_ptr<plane_store> store;
+    void setup(const stencil_op3<double> &so) override {
+        const extents3 e = so.ext;
+        store->fam[std::make_tuple(e.ii, e.jj, e.kk)] = std::make_unique<plane_family>(
+            so.p, so.nstncl, ext3{e.ii, e.jj, e.kk}, (int)PD, store->s2, this->stream);
+    }
+    void run(const stencil_op3<T> &so, grid_func3<T> &x, const grid_func3<T> &b,
+             Dir d) override {
+        store->find(so.ext)->template run<T>(so.p, x.p, b.p, d);
+    }
+};
+
+struct hip_setup_interp3 : kernels3::setup_interp {
+    void run(const stencil_op3<double> &fop, const stencil_op3<double> &cop,
+             prolong_op3<double> &P) override {
+        const extents3 f = fop.ext, c = cop.ext;
+        k3::setup_interp(fop.p, fop.nstncl, P.p, f.ii, f.jj, f.kk, c.ii, c.jj, c.kk,
+                         this->stream);
+        P.fine_op = &fop;
+        launched3("setup_interp3");
+    }
+};
+
+struct hip_galerkin3 : kernels3::coarsen_op {
+    void run(const prolong_op3<double> &P, const stencil_op3<double> &fop,
+             stencil_op3<double> &cop) override {
+        const extents3 f = fop.ext, c = cop.ext;
+        k3::setup_galerkin(fop.p, fop.nstncl, P.p, cop.p, f.ii, f.jj, f.kk, c.ii, c.jj, c.kk,
+                           this->stream);
+        launched3("setup_galerkin3");
+    }
+};
+
+template <typename T>
+struct hip_restrict3 : kernels3::restriction<T> {
+    void run(const prolong_op3<T> &R, const grid_func3<T> &fine, grid_func3<T> &coarse) override {
+        const extents3 f = fine.ext, c = coarse.ext;
+        k3::restrict_<T>(R.p, fine.p, coarse.p, f.ii, f.jj, f.kk, c.ii, c.jj, c.kk, this->stream);
+        launched3("restrict3");
+    }
+};
+
+template <typename T>
+struct hip_interp3 : kernels3::interp_add<T> {
+    void run(const prolong_op3<T> &P, const grid_func3<T> &coarse, const grid_func3<T> &residual,
+             grid_func3<T> &fine) override {
+        if (!P.fine_op) throw error(CDR_ERR_ARG, "interp_add: prolongation without fine operator");
+        const extents3 f = fine.ext, c = coarse.ext;
+        k3::interp_add<T>(fine.p, coarse.p, residual.p, P.fine_op->p, P.fine_op->nstncl, P.p, c.ii,
+                          c.jj, c.kk, f.ii, f.jj, f.kk, this->stream);
+        launched3("interp_add3");
+    }
+};
+
+template <typename T>
+struct hip_solve_cg3 : kernels3::solve_cg<T> {
+    explicit hip_solve_cg3(int *info) : info_dev(info) {}
+    int *info_dev;
+    void setup(const stencil_op3<double> &so, double *abd, int ldab) override {
+        const extents3 e = so.ext;
+        k2::fill<int>(info_dev, 0, 1, this->stream);
+        k3::setup_cg(so.p, so.nstncl, abd, ldab, e.ii, e.jj, e.kk, info_dev, this->stream);
+        launched3("setup_cg3");
+        int info = 0;
+        check_hip(hipMemcpyAsync(&info, info_dev, sizeof(int), hipMemcpyDeviceToHost,
+                                 this->stream), "read info");
+        check_hip(hipStreamSynchronize(this->stream), "hipStreamSynchronize");
+        if (info != 0) throw error(CDR_ERR_NOT_SPD, "coarse-grid Cholesky failed");
+    }
+    void run(grid_func3<T> &x, const grid_func3<T> &b, const double *abd, int ldab,
+             double *bbd) override {
+        const extents3 e = x.ext;
+        k3::solve_cg<T>(x.p, b.p, abd, ldab, bbd, e.ii, e.jj, e.kk, this->stream);
+        launched3("solve_cg3");
+    }
+};
+
+template <plane_dir PD>
+void add_planes(kernel_manager &km, const cdr_solver_settings &s2) {
+    auto st = std::make_shared<plane_store>();
+    st->s2 = s2;
+    km.add<kernels3::plane_relax<double, PD>, hip_planes<double, PD>>("hip", st);
+    km.add<kernels3::plane_relax<float, PD>, hip_planes<float, PD>>("hip", st);
+}
+
+}  // namespace
+
+void register_hip_kernels3(kernel_manager &km, int *info_dev, const cdr_solver_settings &plane,
+                           int precision) {
+    km.add<kernels3::residual<double>, hip_residual3<double>>("hip");
+    km.add<kernels3::residual<float>, hip_residual3<float>>("hip");
+    km.add<kernels3::point_relax<double>, hip_rbgs3<double>>("hip");
+    km.add<kernels3::point_relax<float>, hip_rbgs3<float>>("hip");
+    cdr_solver_settings s2 = plane;
+    s2.precision = precision;
+    s2.graph = 0;
+    add_planes<plane_dir::xy>(km, s2);
+    add_planes<plane_dir::xz>(km, s2);
+    add_planes<plane_dir::yz>(km, s2);
+    km.add<kernels3::setup_interp, hip_setup_interp3>("hip");
+    km.add<kernels3::coarsen_op, hip_galerkin3>("hip");
+    km.add<kernels3::restriction<double>, hip_restrict3<double>>("hip");
+    km.add<kernels3::restriction<float>, hip_restrict3<float>>("hip");
+    km.add<kernels3::interp_add<double>, hip_interp3<double>>("hip");
+    km.add<kernels3::interp_add<float>, hip_interp3<float>>("hip");
+    km.add<kernels3::solve_cg<double>, hip_solve_cg3<double>>("hip", info_dev);
+    km.add<kernels3::solve_cg<float>, hip_solve_cg3<float>>("hip", info_dev);
+}
+
+// one precision's views of a level (device arrays owned by level3)
 template <typename T>
 struct lvl3v {
-    const T *so = nullptr;
-    T *ci = nullptr, *x = nullptr, *b = nullptr, *res = nullptr, *sor = nullptr;
+    stencil_op3<T> A;
+    prolong_op3<T> P;  // on this (coarse) level; fine_op points at the finer level's A
+    grid_func3<T> x, b, res;
+    relax_stencil3<T> sor;
 };
 
 struct level3 {
@@ -128,7 +296,6 @@ struct level3 {
     dbuf<float> so_f, ci_f, x_f, b_f, res_f, sor_f;
     lvl3v<double> d;
     lvl3v<float> f;
-    std::unique_ptr<plane_family> pl[3];
     template <typename T>
     lvl3v<T> &v() {
         if constexpr (std::is_same<T, double>::value)
@@ -144,10 +311,11 @@ class solver3d {
 public:
     solver3d(const double *so_dev, int ns, int nx, int ny, int nz, const cdr_solver3_settings &s,
              hipStream_t stream)
-        : set_(s), stream_(stream) {
+        : set_(s), stream_(stream), km_(stream) {
         check_hip(hipEventCreate(&ev0_), "hipEventCreate");
         check_hip(hipEventCreate(&ev1_), "hipEventCreate");
         info_ = dbuf<int>(1, stream_);
+        register_hip_kernels3(km_, info_.data(), set_.plane, set_.precision);
         build(so_dev, ns, nx, ny, nz);
     }
     ~solver3d() {
@@ -161,6 +329,7 @@ public:
     double setup_seconds() const { return setup_s_; }
     double solve_seconds() const { return solve_s_; }
     bool mixed() const { return set_.precision == CDR_PREC_MIXED; }
+    kernel_manager &kernels() { return km_; }
 
     // include/cedar/3d/solver.h:59-77
     static int compute_num_levels(int nx, int ny, int nz, int min_coarse) {
@@ -181,10 +350,21 @@ public:
     // one FP64 cycle of the configured type (V, or F: include/cedar/cycle/fcycle.h:38-83)
     void vcycle(const double *b_dev, double *x_dev) { cycle<double>(x_dev, b_dev); }
 
+    // one FP64 sweep of the planes of family dir on level lvl, through the registry
     void relax_planes(int lvl, int dir, double *x, const double *b, Dir d) {
-        if (lvl < 0 || lvl >= nlevels() - 1 || dir < 0 || dir > 2 || !L_[lvl].pl[dir])
+        const int r = set_.relaxation;
+        const bool has = r == CDR_RELAX_PLANE_XYZ || (dir == 0 && r == CDR_RELAX_PLANE_XY) ||
+                         (dir == 1 && r == CDR_RELAX_PLANE_XZ) ||
+                         (dir == 2 && r == CDR_RELAX_PLANE_YZ);
+        if (lvl < 0 || lvl >= nlevels() - 1 || dir < 0 || dir > 2 || !has)
             throw error(CDR_ERR_ARG, "no planes of this family on this level");
-        L_[lvl].pl[dir]->run<double>(L_[lvl].d.so, x, b, d);
+        level3 &L = L_[lvl];
+        grid_func3<double> xg{x, L.d.A.ext}, bg{const_cast<double *>(b), L.d.A.ext};
+        switch (dir) {
+        case 0: km_.run<kernels3::plane_relax<double, plane_dir::xy>>(L.d.A, xg, bg, d); break;
+        case 1: km_.run<kernels3::plane_relax<double, plane_dir::xz>>(L.d.A, xg, bg, d); break;
+        default: km_.run<kernels3::plane_relax<double, plane_dir::yz>>(L.d.A, xg, bg, d);
+        }
     }
 
     // multilevel::solve (include/cedar/multilevel.h:278-298); mixed precision: FP64 iterative
@@ -197,11 +377,11 @@ public:
         if (mixed()) {
             xa = xa_.data();
             xb = xb_.data();
-            k3::residual_ir(F.d.so, F.ns, b_dev, x_dev, nullptr, xa, F.f.b, e.ii, e.jj, e.kk,
+            k3::residual_ir(F.d.A.p, F.ns, b_dev, x_dev, nullptr, xa, F.f.b.p, e.ii, e.jj, e.kk,
                             work_.data(), sumsq_.data(), stream_);
         } else {
-            k3::residual_norm(F.d.so, F.ns, b_dev, x_dev, F.d.res, e.ii, e.jj, e.kk, work_.data(),
-                              sumsq_.data(), stream_);
+            k3::residual_norm(F.d.A.p, F.ns, b_dev, x_dev, F.d.res.p, e.ii, e.jj, e.kk,
+                              work_.data(), sumsq_.data(), stream_);
         }
         check_hip(hipGetLastError(), "residual");
         const double res0 = std::sqrt(read_norm2());
@@ -209,14 +389,14 @@ public:
         stop_rule rule(set_.tol, set_.stall_cycles);
         for (int i = 0; i < set_.maxiter; ++i) {
             if (mixed()) {
-                k2::fill<float>(F.f.x, 0.0f, e.points(), stream_);
-                cycle<float>(F.f.x, F.f.b);
-                k3::residual_ir(F.d.so, F.ns, b_dev, xa, F.f.x, xb, F.f.b, e.ii, e.jj, e.kk,
+                k2::fill<float>(F.f.x.p, 0.0f, e.points(), stream_);
+                cycle<float>(F.f.x.p, F.f.b.p);
+                k3::residual_ir(F.d.A.p, F.ns, b_dev, xa, F.f.x.p, xb, F.f.b.p, e.ii, e.jj, e.kk,
                                 work_.data(), sumsq_.data(), stream_);
                 std::swap(xa, xb);
             } else {
                 cycle<double>(x_dev, b_dev);
-                k3::residual_norm(F.d.so, F.ns, b_dev, x_dev, F.d.res, e.ii, e.jj, e.kk,
+                k3::residual_norm(F.d.A.p, F.ns, b_dev, x_dev, F.d.res.p, e.ii, e.jj, e.kk,
                                   work_.data(), sumsq_.data(), stream_);
             }
             check_hip(hipGetLastError(), "residual");
@@ -244,6 +424,7 @@ public:
 private:
     cdr_solver3_settings set_;
     hipStream_t stream_;
+    kernel_manager km_;
     std::vector<level3> L_;
     dbuf<double> abd_, bbd_, work_, sumsq_, xa_, xb_;
     dbuf<int> info_;
@@ -257,6 +438,18 @@ private:
                                  stream_), "read norm");
         check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
         return *host_norm_;
+    }
+
+    template <typename T>
+    static void set_views(lvl3v<T> &v, const ext3 &e, int ns, T *so, T *ci, T *x, T *b, T *res,
+                          T *sor) {
+        const extents3 x3{e.ii, e.jj, e.kk};
+        v.A = stencil_op3<T>{so, x3, ns};
+        v.P = prolong_op3<T>{ci, x3, nullptr};
+        v.x = grid_func3<T>{x, x3};
+        v.b = grid_func3<T>{b, x3};
+        v.res = grid_func3<T>{res, x3};
+        v.sor = relax_stencil3<T>{sor, x3};
     }
 
     void build(const double *so_dev, int ns, int nx, int ny, int nz) {
@@ -274,30 +467,27 @@ private:
             level3 &L = L_[l];
             L.e = ext3{cx + 2, cy + 2, cz + 2};
             const std::size_t np = L.e.points();
+            double *so = nullptr;
             if (l == 0) {
                 L.ns = ns;
-                L.d.so = so_dev;  // held by reference
+                so = const_cast<double *>(so_dev);  // held by reference
             } else {
                 L.ns = 14;
                 L.so_buf = dbuf<double>(np * 14, stream_);
                 L.ci_buf = dbuf<double>(np * 26, stream_);
                 L.x_buf = dbuf<double>(np, stream_);
                 L.b_buf = dbuf<double>(np, stream_);
-                L.d.so = L.so_buf.data();
-                L.d.ci = L.ci_buf.data();
-                L.d.x = L.x_buf.data();
-                L.d.b = L.b_buf.data();
+                so = L.so_buf.data();
             }
             L.res_buf = dbuf<double>(np, stream_);
-            L.d.res = L.res_buf.data();
-            if (set_.relaxation == CDR_RELAX_POINT) {
-                L.sor_buf = dbuf<double>(np * 2, stream_);
-                L.d.sor = L.sor_buf.data();
-            }
+            if (set_.relaxation == CDR_RELAX_POINT) L.sor_buf = dbuf<double>(np * 2, stream_);
+            set_views<double>(L.d, L.e, L.ns, so, L.ci_buf.data(), L.x_buf.data(), L.b_buf.data(),
+                              L.res_buf.data(), L.sor_buf.data());
             cx = (int)((cx - 1) / 2.0 + 1);
             cy = (int)((cy - 1) / 2.0 + 1);
             cz = (int)((cz - 1) / 2.0 + 1);
         }
+        for (int l = 1; l < nl; ++l) L_[l].d.P.fine_op = &L_[l - 1].d.A;
         const ext3 ec = L_[nl - 1].e;
         const int kd = k3::cg_bandwidth(ec.ii, ec.jj);
         ldab_ = kd + 1;
@@ -309,27 +499,16 @@ private:
         check_hip(hipHostMalloc(&host_norm_, sizeof(double)), "hipHostMalloc");
 
         check_hip(hipEventRecord(ev0_, stream_), "hipEventRecord");
+        // multilevel::setup (include/cedar/multilevel.h:243-265): interp, RAP, relaxation per
+        // level, then the coarse factorisation
         for (int l = 0; l < nl - 1; ++l) {
             level3 &F = L_[l];
             level3 &C = L_[l + 1];
-            k3::setup_interp(F.d.so, F.ns, C.d.ci, F.e.ii, F.e.jj, F.e.kk, C.e.ii, C.e.jj, C.e.kk,
-                             stream_);
-            k3::setup_galerkin(F.d.so, F.ns, C.d.ci, C.so_buf.data(), F.e.ii, F.e.jj, F.e.kk, C.e.ii,
-                               C.e.jj, C.e.kk, stream_);
-            check_hip(hipGetLastError(), "setup interp/galerkin");
+            km_.run<kernels3::setup_interp>(F.d.A, C.d.A, C.d.P);
+            km_.run<kernels3::coarsen_op>(C.d.P, F.d.A, C.d.A);
             setup_relax(F);
         }
-        {
-            const level3 &C = L_[nl - 1];
-            k2::fill<int>(info_.data(), 0, 1, stream_);
-            k3::setup_cg(C.d.so, C.ns, abd_.data(), ldab_, ec.ii, ec.jj, ec.kk, info_.data(), stream_);
-            check_hip(hipGetLastError(), "setup_cg");
-            int info = 0;
-            check_hip(hipMemcpyAsync(&info, info_.data(), sizeof(int), hipMemcpyDeviceToHost, stream_),
-                      "read info");
-            check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-            if (info != 0) throw error(CDR_ERR_NOT_SPD, "coarse-grid Cholesky failed");
-        }
+        km_.setup<kernels3::solve_cg<double>>(L_[nl - 1].d.A, abd_.data(), ldab_);
         if (mixed()) build_f32();
         check_hip(hipEventRecord(ev1_, stream_), "hipEventRecord");
         check_hip(hipEventSynchronize(ev1_), "hipEventSynchronize");
@@ -348,25 +527,22 @@ private:
             level3 &L = L_[l];
             const std::size_t np = L.e.points();
             L.so_f = dbuf<float>(np * L.ns, stream_);
-            k2::convert<double, float>(L.d.so, L.so_f.data(), np * L.ns, stream_);
-            L.f.so = L.so_f.data();
+            k2::convert<double, float>(L.d.A.p, L.so_f.data(), np * L.ns, stream_);
             if (l > 0) {
                 L.ci_f = dbuf<float>(np * 26, stream_);
-                k2::convert<double, float>(L.d.ci, L.ci_f.data(), np * 26, stream_);
-                L.f.ci = L.ci_f.data();
+                k2::convert<double, float>(L.d.P.p, L.ci_f.data(), np * 26, stream_);
             }
             L.x_f = dbuf<float>(np, stream_);
             L.b_f = dbuf<float>(np, stream_);
             L.res_f = dbuf<float>(np, stream_);
-            L.f.x = L.x_f.data();
-            L.f.b = L.b_f.data();
-            L.f.res = L.res_f.data();
-            if (L.d.sor) {
+            if (L.d.sor.p) {
                 L.sor_f = dbuf<float>(np * 2, stream_);
-                k2::convert<double, float>(L.d.sor, L.sor_f.data(), np * 2, stream_);
-                L.f.sor = L.sor_f.data();
+                k2::convert<double, float>(L.d.sor.p, L.sor_f.data(), np * 2, stream_);
             }
+            set_views<float>(L.f, L.e, L.ns, L.so_f.data(), L.ci_f.data(), L.x_f.data(),
+                             L.b_f.data(), L.res_f.data(), L.sor_f.data());
         }
+        for (int l = 1; l < nlevels(); ++l) L_[l].f.P.fine_op = &L_[l - 1].f.A;
         check_hip(hipGetLastError(), "convert");
     }
 
@@ -375,58 +551,56 @@ private:
     void setup_relax(level3 &F) {
         const int r = set_.relaxation;
         if (r == CDR_RELAX_POINT) {
-            k3::setup_recip(F.d.so, F.ns, F.d.sor, F.e.ii, F.e.jj, F.e.kk, stream_);
-            check_hip(hipGetLastError(), "setup_recip");
+            km_.setup<kernels3::point_relax<double>>(F.d.A, F.d.sor);
             return;
         }
-        cdr_solver_settings s2 = set_.plane;
-        s2.precision = set_.precision;
-        s2.graph = 0;
         if (r == CDR_RELAX_PLANE_XY || r == CDR_RELAX_PLANE_XYZ)
-            F.pl[k3::XY] = std::make_unique<plane_family>(F.d.so, F.ns, F.e, k3::XY, s2, stream_);
+            km_.setup<kernels3::plane_relax<double, plane_dir::xy>>(F.d.A);
         if (r == CDR_RELAX_PLANE_XZ || r == CDR_RELAX_PLANE_XYZ)
-            F.pl[k3::XZ] = std::make_unique<plane_family>(F.d.so, F.ns, F.e, k3::XZ, s2, stream_);
+            km_.setup<kernels3::plane_relax<double, plane_dir::xz>>(F.d.A);
         if (r == CDR_RELAX_PLANE_YZ || r == CDR_RELAX_PLANE_XYZ)
-            F.pl[k3::YZ] = std::make_unique<plane_family>(F.d.so, F.ns, F.e, k3::YZ, s2, stream_);
+            km_.setup<kernels3::plane_relax<double, plane_dir::yz>>(F.d.A);
+    }
+
+    template <typename T, plane_dir PD>
+    void planes(level3 &L, grid_func3<T> &x, const grid_func3<T> &b, Dir d) {
+        km_.run<kernels3::plane_relax<T, PD>>(L.v<T>().A, x, b, d);
     }
 
     // smoother sequencing (include/cedar/multilevel.h:165-222)
     template <typename T>
-    void smooth(level3 &L, T *x, const T *b, Dir d) {
+    void smooth(level3 &L, grid_func3<T> &x, const grid_func3<T> &b, Dir d) {
         const int nrep = d == Dir::DOWN ? set_.nrelax_pre : set_.nrelax_post;
-        const lvl3v<T> &V = L.v<T>();
+        lvl3v<T> &V = L.v<T>();
         for (int rep = 0; rep < nrep; ++rep) {
             switch (set_.relaxation) {
-            case CDR_RELAX_POINT:
-                k3::relax_gs<T>(V.so, L.ns, b, x, V.sor, L.e.ii, L.e.jj, L.e.kk, (int)d, stream_);
-                check_hip(hipGetLastError(), "relax_gs");
-                break;
-            case CDR_RELAX_PLANE_XY: L.pl[k3::XY]->run<T>(V.so, x, b, d); break;
-            case CDR_RELAX_PLANE_XZ: L.pl[k3::XZ]->run<T>(V.so, x, b, d); break;
-            case CDR_RELAX_PLANE_YZ: L.pl[k3::YZ]->run<T>(V.so, x, b, d); break;
+            case CDR_RELAX_POINT: km_.run<kernels3::point_relax<T>>(V.A, x, b, V.sor, d); break;
+            case CDR_RELAX_PLANE_XY: planes<T, plane_dir::xy>(L, x, b, d); break;
+            case CDR_RELAX_PLANE_XZ: planes<T, plane_dir::xz>(L, x, b, d); break;
+            case CDR_RELAX_PLANE_YZ: planes<T, plane_dir::yz>(L, x, b, d); break;
             default:
                 if (d == Dir::DOWN) {
-                    L.pl[k3::XY]->run<T>(V.so, x, b, d);
-                    L.pl[k3::YZ]->run<T>(V.so, x, b, d);
-                    L.pl[k3::XZ]->run<T>(V.so, x, b, d);
+                    planes<T, plane_dir::xy>(L, x, b, d);
+                    planes<T, plane_dir::yz>(L, x, b, d);
+                    planes<T, plane_dir::xz>(L, x, b, d);
                 } else {
-                    L.pl[k3::XZ]->run<T>(V.so, x, b, d);
-                    L.pl[k3::YZ]->run<T>(V.so, x, b, d);
-                    L.pl[k3::XY]->run<T>(V.so, x, b, d);
+                    planes<T, plane_dir::xz>(L, x, b, d);
+                    planes<T, plane_dir::yz>(L, x, b, d);
+                    planes<T, plane_dir::xy>(L, x, b, d);
                 }
             }
         }
     }
 
     template <typename T>
-    void coarse_solve(T *x, const T *b) {
-        const ext3 ec = L_.back().e;
-        k3::solve_cg<T>(x, b, abd_.data(), ldab_, bbd_.data(), ec.ii, ec.jj, ec.kk, stream_);
-        check_hip(hipGetLastError(), "solve_cg");
+    void coarse_solve(grid_func3<T> &x, const grid_func3<T> &b) {
+        km_.run<kernels3::solve_cg<T>>(x, b, abd_.data(), ldab_, bbd_.data());
     }
 
     template <typename T>
-    void cycle(T *x, const T *b) {
+    void cycle(T *x_dev, const T *b_dev) {
+        const extents3 e = L_[0].v<T>().A.ext;
+        grid_func3<T> x{x_dev, e}, b{const_cast<T *>(b_dev), e};
         if (L_.size() == 1)
             coarse_solve<T>(x, b);
         else if (set_.cycle == CDR_CYCLE_F)
@@ -437,7 +611,7 @@ private:
 
     // fcycle::fmg_cycle (include/cedar/cycle/fcycle.h:49-83)
     template <typename T>
-    void fmg(int l, T *x, const T *b) {
+    void fmg(int l, grid_func3<T> &x, const grid_func3<T> &b) {
         if (l == nlevels() - 1) {
             coarse_solve<T>(x, b);
             return;
@@ -446,33 +620,30 @@ private:
         level3 &C = L_[l + 1];
         lvl3v<T> &fv = F.v<T>();
         lvl3v<T> &cv = C.v<T>();
-        k3::restrict_<T>(cv.ci, b, cv.b, F.e.ii, F.e.jj, F.e.kk, C.e.ii, C.e.jj, C.e.kk, stream_);
+        km_.run<kernels3::restriction<T>>(cv.P, b, cv.b);
         fmg<T>(l + 1, cv.x, cv.b);
-        k2::fill<T>(x, T(0), F.e.points(), stream_);
-        k2::fill<T>(fv.res, T(0), F.e.points(), stream_);
-        k3::interp_add<T>(x, cv.x, fv.res, fv.so, F.ns, cv.ci, C.e.ii, C.e.jj, C.e.kk, F.e.ii, F.e.jj,
-                          F.e.kk, stream_);
-        check_hip(hipGetLastError(), "fmg");
+        k2::fill<T>(x.p, T(0), F.e.points(), stream_);
+        k2::fill<T>(fv.res.p, T(0), F.e.points(), stream_);
+        km_.run<kernels3::interp_add<T>>(cv.P, cv.x, fv.res, x);
         ncycle<T>(l, x, b);
     }
 
+    // vcycle::ncycle_helper (include/cedar/cycle/vcycle.h:57-115)
     template <typename T>
-    void ncycle(int l, T *x, const T *b) {
+    void ncycle(int l, grid_func3<T> &x, const grid_func3<T> &b) {
         level3 &F = L_[l];
         level3 &C = L_[l + 1];
         lvl3v<T> &fv = F.v<T>();
         lvl3v<T> &cv = C.v<T>();
         smooth<T>(F, x, b, Dir::DOWN);
-        k3::residual<T>(fv.so, F.ns, b, x, fv.res, F.e.ii, F.e.jj, F.e.kk, stream_);
-        k3::restrict_<T>(cv.ci, fv.res, cv.b, F.e.ii, F.e.jj, F.e.kk, C.e.ii, C.e.jj, C.e.kk, stream_);
-        k2::fill<T>(cv.x, T(0), C.e.points(), stream_);
+        km_.run<kernels3::residual<T>>(fv.A, x, b, fv.res);
+        km_.run<kernels3::restriction<T>>(cv.P, fv.res, cv.b);
+        k2::fill<T>(cv.x.p, T(0), C.e.points(), stream_);
         if (l + 1 == nlevels() - 1)
             coarse_solve<T>(cv.x, cv.b);
         else
             ncycle<T>(l + 1, cv.x, cv.b);
-        k3::interp_add<T>(x, cv.x, fv.res, fv.so, F.ns, cv.ci, C.e.ii, C.e.jj, C.e.kk, F.e.ii, F.e.jj,
-                          F.e.kk, stream_);
-        check_hip(hipGetLastError(), "cycle");
+        km_.run<kernels3::interp_add<T>>(cv.P, cv.x, fv.res, x);
         smooth<T>(F, x, b, Dir::UP);
     }
 };
@@ -581,8 +752,8 @@ int cdr_hip_solver3d_level(const cdr_hip_solver3d *slv, int lvl, int *nx, int *n
     if (nx) *nx = L.e.ii - 2;
     if (ny) *ny = L.e.jj - 2;
     if (nz) *nz = L.e.kk - 2;
-    if (so) *so = L.d.so;
-    if (ci) *ci = L.d.ci;
+    if (so) *so = L.d.A.p;
+    if (ci) *ci = L.d.P.p;
     return CDR_OK;
 }
 
@@ -591,6 +762,25 @@ int cdr_hip_solver3d_times(const cdr_hip_solver3d *slv, double *setup_s, double 
     if (setup_s) *setup_s = slv->impl->setup_seconds();
     if (solve_s) *solve_s = slv->impl->solve_seconds();
     return CDR_OK;
+}
+
+int cdr_hip_solver3d_kernels(cdr_hip_solver3d *slv, char *buf, int len) {
+    if (!slv || len < 0 || (len > 0 && !buf)) return -CDR_ERR_ARG;
+    const std::string d = slv->impl->kernels().describe();
+    if (len > 0) {
+        const std::size_t n = std::min<std::size_t>(d.size(), (std::size_t)len - 1);
+        std::memcpy(buf, d.data(), n);
+        buf[n] = '\0';
+    }
+    return (int)d.size();
+}
+
+int cdr_hip_solver3d_set_kernel(cdr_hip_solver3d *slv, const char *iface, const char *impl) {
+    if (!slv || !iface || !impl) return CDR_ERR_ARG;
+    return guarded3([&] {
+        slv->impl->kernels().set_by_name(iface, impl);
+        return (int)CDR_OK;
+    });
 }
 
 void cdr_hip_solver3d_destroy(cdr_hip_solver3d *slv) { delete slv; }

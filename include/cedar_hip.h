@@ -347,6 +347,16 @@ int cdr_hip_solver2d_save_timers(cdr_hip_solver2d *slv, const char *path);
 /* Writes a device stencil (nstncl planes of ii x jj) as text in the layout of the reference's
  * operator<< (src/2d/stencil_op.cc:5-51). */
 int cdr_hip_stencil2d_write(const double *so_dev, int nstncl, int ii, int jj, const char *path);
+/* The solver's kernel registry (kernel_manager, include/cedar/kernel_manager.h:38-72):
+ * cdr_hip_solver2d_kernels writes {"interface": {"impls": [...], "default": "..."}} as JSON into
+ * buf (len bytes, NUL-terminated, truncated if short) and returns the full length;
+ * cdr_hip_solver2d_set_kernel is kman->set<I>(impl) for the interface named iface (both
+ * precisions), CDR_ERR_ARG for an unknown interface or implementation.  The interfaces are those
+ * of ser_kernels (include/cedar/kernel_registry.h:23-35) that 2D solves use: "residual",
+ * "point-relax", "line-relax-x", "line-relax-y", "setup-interp", "coarsen-op", "restriction",
+ * "interp-add", "solve-cg"; each has the "hip" implementation. */
+int cdr_hip_solver2d_kernels(cdr_hip_solver2d *slv, char *buf, int len);
+int cdr_hip_solver2d_set_kernel(cdr_hip_solver2d *slv, const char *iface, const char *impl);
 void cdr_hip_solver2d_destroy(cdr_hip_solver2d *slv);
 
 /* ---------------------------------------------------------------------------------------
@@ -377,10 +387,17 @@ int cdr_hip_relax_gs3d_f32(const float *so, int nstncl, const float *qf, float *
 /* BMG3_SymStd_restrict (src/3d/restrict.cc:7-10) */
 int cdr_hip_restrict3d_f64(const double *ci, const double *q, double *qc, int iif, int jjf,
                            int kkf, int iic, int jjc, int kkc, void *stream);
+/* FP32 form (the mixed-precision 3D cycle's restriction; same operation order in float) */
+int cdr_hip_restrict3d_f32(const float *ci, const float *q, float *qc, int iif, int jjf, int kkf,
+                           int iic, int jjc, int kkc, void *stream);
 /* BMG3_SymStd_interp_add (src/3d/interp.cc:7-11); res is not modified (the Fortran divides it
  * by SO_p in place) */
 int cdr_hip_interp_add3d_f64(double *q, const double *qc, const double *res, const double *so,
                              int nstncl, const double *ci, int iic, int jjc, int kkc, int iif,
+                             int jjf, int kkf, void *stream);
+/* FP32 form (the mixed-precision 3D cycle's interpolation) */
+int cdr_hip_interp_add3d_f32(float *q, const float *qc, const float *res, const float *so,
+                             int nstncl, const float *ci, int iic, int jjc, int kkc, int iif,
                              int jjf, int kkf, void *stream);
 /* BMG3_SymStd_SETUP_interp_OI (include/cedar/3d/interp.h:11-15); no scratch argument */
 int cdr_hip_setup_interp3d_f64(const double *so, int nstncl, double *ci, int iif, int jjf,
@@ -397,6 +414,10 @@ int cdr_hip_cg_bandwidth3d(int ii, int jj);
 int cdr_hip_setup_cg3d_f64(const double *so, int nstncl, double *abd, int ldab, int ii, int jj,
                            int kk, int *info_dev, void *stream);
 int cdr_hip_solve_cg3d_f64(double *q, const double *qf, const double *abd, int ldab, double *bbd,
+                           int ii, int jj, int kk, void *stream);
+/* FP32 right-hand side and solution with the FP64 band (the coarsest solve stays FP64: qf is
+ * widened, the solve runs in FP64, q is rounded) */
+int cdr_hip_solve_cg3d_f32(float *q, const float *qf, const double *abd, int ldab, double *bbd,
                            int ii, int jj, int kk, void *stream);
 
 /* ---------------------------------------------------------------------------------------
@@ -441,6 +462,12 @@ int cdr_hip_solver3d_nlevels(const cdr_hip_solver3d *slv);
 int cdr_hip_solver3d_level(const cdr_hip_solver3d *slv, int lvl, int *nx, int *ny, int *nz,
                            const double **so, const double **ci);
 int cdr_hip_solver3d_times(const cdr_hip_solver3d *slv, double *setup_s, double *solve_s);
+/* The 3D solver's kernel registry, as for 2D: interfaces "residual (3D)", "point-relax (3D)",
+ * "plane-relax-xy", "plane-relax-xz", "plane-relax-yz" (include/cedar/kernels/plane_relax.h
+ * :11-31), "setup-interp (3D)", "coarsen-op (3D)", "restriction (3D)", "interp-add (3D)",
+ * "solve-cg (3D)"; the solver runs every kernel of its setup and cycle through it. */
+int cdr_hip_solver3d_kernels(cdr_hip_solver3d *slv, char *buf, int len);
+int cdr_hip_solver3d_set_kernel(cdr_hip_solver3d *slv, const char *iface, const char *impl);
 void cdr_hip_solver3d_destroy(cdr_hip_solver3d *slv);
 
 /* ---------------------------------------------------------------------------------------
