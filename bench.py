@@ -168,11 +168,17 @@ def bench_c3(args):
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t) / args.steps
         res[prec] = {"seconds": dt, "iters": int(len(h)), "rho": float(h[-1]),
-                     "setup_s": setup_s, "per_cycle_s": dt / len(h), "x": x}
+                     "converged": converged(h, 1e-10, 60), "setup_s": setup_s,
+                     "setup_plus_solve_s": setup_s + dt, "per_cycle_s": dt / len(h), "x": x}
         del s
     x64, x32 = res["f64"].pop("x"), res["mixed"].pop("x")
     rel = float((torch.linalg.norm(x32 - x64) / torch.linalg.norm(x64)).item())
-    print(json.dumps({
+    roof = plane_sweep_roofline(args, so, torch)
+    cpu = None
+    if args.cpu_runs > 0:
+        cpu = cpu_cycles_baseline3(so.cpu().numpy(), b.cpu().numpy(), args.relax3,
+                                   res["f64"]["iters"], args.cpu_cycles3)
+    emit({
         "metric": "time-to-1e-10 rel. residual (s), 3D plane relaxation; mixed vs FP64",
         "value": res["mixed"]["seconds"], "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": res["mixed"]["seconds"] * 1e3,
@@ -182,7 +188,71 @@ def bench_c3(args):
                                f"sigma=1.5 on 8^3 blocks, {args.relax3} V(1,1), tol 1e-10"},
         "fp64": res["f64"], "mixed": dict(res["mixed"], rel_l2_vs_fp64=rel),
         "speedup_mixed_vs_fp64": res["f64"]["seconds"] / res["mixed"]["seconds"],
-        "gen_s": gen_s}))
+        "roofline": roof, "cpu_baseline": cpu,
+        "gen_s": gen_s}, res["f64"]["converged"] and res["mixed"]["converged"])
+
+
+def plane_sweep_roofline(args, so, torch):
+    """Fine-level xy-plane line sweep of the mixed 3D cycle: the batched FP32 x-line kernel over
+    the 128 odd xy-planes of the 255^3 grid (5-point plane operators {p, pw, ps} of the 7-point
+    stencil), one colour per launch, live HIP-event timing on the launching stream.  Algorithmic
+    bytes per launch: (S+3) w / 2 per point with S = 3 planes, w = 4: c/w/s read once, f read,
+    q read and written, over all planes of the batch."""
+    import cedar_amd
+    from cedar_amd import _abi
+    ns3, kk, jj, ii = so.shape
+    planes = list(range(1, kk - 1, 2))
+    so2 = so[0:3, planes].permute(1, 0, 2, 3).contiguous()           # (nb, 3, jj, ii) FP64
+    nb = so2.shape[0]
+    sor = torch.zeros((nb, 2, jj, ii), dtype=torch.float64, device="cuda")
+    for t in range(nb):
+        cedar_amd.cdr2.setup_lines(so2[t], sor[t], "x")
+    A, sx = so2.float().contiguous(), sor.float().contiguous()
+    g = torch.Generator(device="cpu").manual_seed(7)
+    f = (torch.rand((nb, jj, ii), generator=g, dtype=torch.float32) * 2 - 1).cuda()
+    q = torch.zeros_like(f)
+    lib = _abi.load()
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    fx = lib.cdr_hip_relax_lines2d_batched_f32
+    t = time_kernel(lambda: fx(0, A.data_ptr(), 3, f.data_ptr(), q.data_ptr(), sx.data_ptr(), ii,
+                               jj, nb, 0, stream), args.kernel_reps, torch)
+    npts = (ii - 2) * (jj - 2) * nb
+    byt = (3 + 3) * 4 * npts / 2
+    return {"bound": "hbm", "kernel": f"relax_xlines_seg_k<float, 5-point> batched over {nb} "
+            f"xy-planes of {ii - 2}^2 (the fine-level plane sweep of the mixed cycle), one colour "
+            "per launch, shape picked by the autotuner", "achieved": byt / (t / 2) / 1e9,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": byt / (t / 2) / 1e9 / HBM_PEAK_GBS,
+            "traffic": None, "algorithmic_bytes_per_launch": byt, "avg_launch_s": t / 2}
+
+
+def cpu_cycles_baseline3(so_np, b_np, relax, gpu_iters, ncyc):
+    """The 3D FP64 oracle (plain C, one thread) on the same problem: setup, then `ncyc` V-cycles
+    from x0 = 0 timed one by one (a full solve takes minutes on one core); the value is the
+    median cycle time times the GPU's cycle count."""
+    import statistics
+
+    from oracle import oracle3 as O3
+    t = time.perf_counter()
+    o = O3.Solver3(so_np, relaxation=relax, nrelax_pre=1, nrelax_post=1, maxiter=60, tol=1e-10)
+    setup = time.perf_counter() - t
+    x = np.zeros_like(b_np)
+    cyc = []
+    for k in range(ncyc):
+        t = time.perf_counter()
+        x = o.vcycle(x, b_np)
+        cyc.append(time.perf_counter() - t)
+        print(f"cpu baseline 3D cycle {k + 1}/{ncyc}: {cyc[-1]:.2f} s", file=sys.stderr,
+              flush=True)
+    med = statistics.median(cyc)
+    return {"value": med * gpu_iters,
+            "unit": "s (solve from x0=0 to 1e-10 = median cycle time x the cycle count; setup "
+                    "excluded, as in value)",
+            "cores": 1, "kind": "port",
+            "sample": f"FP64 C oracle, this {so_np.shape[1] - 2}^3 problem, {relax} V(1,1): setup "
+                      f"then {ncyc} cycles timed one by one (median {med:.2f} s per cycle) x "
+                      f"{gpu_iters} cycles; host '{cpu_model()}', nproc {os.cpu_count()}, 1 "
+                      "thread",
+            "setup_s": setup, "per_cycle_s": med, "cycles_timed": ncyc}
 
 
 def bench_c4(args):
@@ -302,6 +372,8 @@ def cpu_solve_baseline(make_solver, b_np, runs, what):
         t = time.perf_counter()
         _, hist = o.solve(b_np)
         solves.append(time.perf_counter() - t)
+        print(f"cpu baseline run {len(solves)}/{runs}: setup {setups[-1]:.2f} s, solve "
+              f"{solves[-1]:.2f} s ({len(hist)} cycles)", file=sys.stderr, flush=True)
     return {
         "value": statistics.median(solves),
         "unit": "s (solve from x0=0 to 1e-10; setup excluded, as in value)",
@@ -420,6 +492,8 @@ def main():
     ap.add_argument("--nprob", type=int, default=96)
     ap.add_argument("--n3", type=int, default=255)
     ap.add_argument("--relax3", default="plane-xy")
+    ap.add_argument("--cpu-cycles3", type=int, default=3,
+                    help="3D oracle cycles timed for the C3 CPU baseline")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "

@@ -57,6 +57,8 @@ PROTOTYPES = {
     "cdr_hip_relax_gs2d_f32": (_i, [_P, _i, _P, _P, _P, _i, _i, _i, _vp]),
     "cdr_hip_setup_lines_x2d_f64": (_i, [_P, _P, _i, _i, _i, _P, _vp]),
     "cdr_hip_setup_lines_y2d_f64": (_i, [_P, _P, _i, _i, _i, _P, _vp]),
+    "cdr_hip_relax_lines2d_batched_f64": (_i, [_i, _P, _i, _P, _P, _P, _i, _i, _i, _i, _vp]),
+    "cdr_hip_relax_lines2d_batched_f32": (_i, [_i, _P, _i, _P, _P, _P, _i, _i, _i, _i, _vp]),
     "cdr_hip_relax_lines_x2d_f64": (_i, [_P, _i, _P, _P, _P, _P, _i, _i, _i, _vp]),
     "cdr_hip_relax_lines_y2d_f64": (_i, [_P, _i, _P, _P, _P, _P, _i, _i, _i, _vp]),
     "cdr_hip_relax_lines_x2d_f32": (_i, [_P, _i, _P, _P, _P, _P, _i, _i, _i, _vp]),

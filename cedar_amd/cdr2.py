@@ -170,6 +170,16 @@ def relax_lines(so, qf, q, sor, updown, direction):
     return q
 
 
+def relax_lines_batched(so, qf, q, sor, updown, direction):
+    """One zebra line sweep over nb problems of the same extents, stacked along a leading batch
+    dimension: so (nb, nstncl, jj, ii), qf and q (nb, jj, ii), sor (nb, 2, ...)."""
+    so, qf, q, sor = map(_t, (so, qf, q, sor))
+    _abi.call(f"cdr_hip_relax_lines2d_batched_{_sfx(q)}", 0 if direction == "x" else 1, _p(so),
+              so.shape[1], _p(qf), _p(q), _p(sor), q.shape[-1], q.shape[-2], q.shape[0],
+              int(updown), _stream())
+    return q
+
+
 def relax_lines_xy(so, qf, q, sorx, sory, updown):
     """One line-xy smoothing step: x then y going down, y then x going up."""
     so, qf, q, sorx, sory = map(_t, (so, qf, q, sorx, sory))

@@ -1520,6 +1520,123 @@ __global__ __launch_bounds__(512) void transpose_pairs_k(const float *__restrict
     }
 }
 
+// ------------------------------------------------------------------------------------------
+// Kernels that take the iterate in the transposed layout (qT: ii rows of jj values,
+// qT[i * jj + j] = q(i, j)), for levels whose y sweeps run as x sweeps of the transposed grid:
+// after the last y sweep of the pre-smoothing the iterate stays transposed through the residual
+// and interp_add, and the first y sweep of the post-smoothing starts from it, so the two grid
+// transposes between them are not needed.  Every point is evaluated with the same operations in
+// the same order as residual_k / interp_add_k, so results are bitwise identical.
+// ------------------------------------------------------------------------------------------
+constexpr int kTI = 64, kTJ = 32;  // residual_qT_k tile: 64 (i) x 32 (j) points, 256 threads
+
+template <typename T, bool NINE>
+__global__ __launch_bounds__(256) void residual_qT_k(const T *__restrict__ so,
+                                                     const T *__restrict__ qf,
+                                                     const T *__restrict__ qT,
+                                                     T *__restrict__ res, int ii, int jj) {
+    __shared__ T sq[kTJ + 2][kTI + 3];  // q(i0-1 .. i0+64, j0-1 .. j0+32) as [j][i]
+    const int i0 = blockIdx.x * kTI + 1, j0 = blockIdx.y * kTJ + 1;
+    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.z;
+    so += bz * (NINE ? 5 : 3) * P;
+    qf += bz * P;
+    qT += bz * P;
+    res += bz * P;
+    // stage: consecutive threads read consecutive j of one row of qT (a column of q)
+    for (int e = threadIdx.x; e < (kTI + 2) * (kTJ + 2); e += 256) {
+        const int a = e / (kTJ + 2), b = e % (kTJ + 2);
+        const int i = i0 - 1 + a, j = j0 - 1 + b;
+        sq[b][a] = (i < ii && j < jj) ? qT[(std::size_t)i * jj + j] : T(0);
+    }
+    __syncthreads();
+    const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+    const int i = i0 + tx;
+    if (i > ii - 2) return;
+    for (int k = ty; k < kTJ; k += 4) {
+        const int j = j0 + k;
+        if (j > jj - 2) break;
+        const std::size_t p = pidx(i, j, ii);
+        auto Q = [&](int di, int dj) { return sq[k + 1 + dj][tx + 1 + di]; };
+        T acc = qf[p];
+        acc = acc + so[P * KW + p] * Q(-1, 0);
+        acc = acc + so[P * KW + p + 1] * Q(1, 0);
+        acc = acc + so[P * KS + p] * Q(0, -1);
+        acc = acc + so[P * KS + p + ii] * Q(0, 1);
+        if (NINE) {
+            acc = acc + so[P * KSW + p] * Q(-1, -1);
+            acc = acc + so[P * KNW + p + 1] * Q(1, -1);
+            acc = acc + so[P * KNW + p + ii] * Q(-1, 1);
+            acc = acc + so[P * KSW + p + ii + 1] * Q(1, 1);
+        }
+        res[p] = acc - so[p] * Q(0, 0);
+    }
+}
+
+// interp_add with the fine iterate transposed: phase 1 forms, per fine point of a 64 x 32 tile
+// (32 x 16 blocks of 2 x 2 points, one thread each), the one (coarse point) or two (others) terms
+// that interp_add_k adds, in its order; phase 2 adds them to qT with j-contiguous accesses.
+template <typename T>
+__global__ __launch_bounds__(512) void interp_add_qT_k(T *__restrict__ qT, const T *__restrict__ qc,
+                                                       const T *__restrict__ res,
+                                                       const T *__restrict__ so,
+                                                       const T *__restrict__ ci, int iic, int jjc,
+                                                       int iif, int jjf, int ns) {
+    __shared__ T t1[32][65], t2[32][65];  // [j - jt][i - it]
+    const int m0 = blockIdx.x * 32 + 1, n0 = blockIdx.y * 16 + 1;
+    const int it = 2 * m0 - 1, jt = 2 * n0 - 1;  // first fine point of the tile
+    const std::size_t Pc = (std::size_t)iic * jjc, Pf = (std::size_t)iif * jjf, bz = blockIdx.z;
+    qT += bz * Pf;
+    qc += bz * Pc;
+    res += bz * Pf;
+    so += bz * ns * Pf;
+    ci += bz * 8 * Pc;
+    const int imax = iif - 2 + ((iif - 2) % 2 == 0 ? 1 : 0);
+    const int jmax = jjf - 2 + ((jjf - 2) % 2 == 0 ? 1 : 0);
+    {
+        const int m = m0 + threadIdx.x % 32, n = n0 + threadIdx.x / 32;
+        const int i0 = 2 * m - 1, j0 = 2 * n - 1;
+        if (i0 <= imax && j0 <= jmax) {
+            const bool xe = i0 + 1 <= imax, ye = j0 + 1 <= jmax;
+            auto CI = [&](int a, int b, int k) { return ci[Pc * k + pidx(a, b, iic)]; };
+            auto QC = [&](int a, int b) { return qc[pidx(a, b, iic)]; };
+            auto R = [&](int i, int j) {
+                const std::size_t p = pidx(i, j, iif);
+                return (i > iif - 2 || j > jjf - 2) ? res[p] : res[p] / so[p];
+            };
+            const int li = i0 - it, lj = j0 - jt;
+            const T q00 = QC(m, n);
+            t1[lj][li] = q00;
+            if (xe) {
+                t1[lj][li + 1] = CI(m + 1, n, LR) * QC(m + 1, n) + CI(m + 1, n, LL) * q00;
+                t2[lj][li + 1] = R(i0 + 1, j0);
+            }
+            if (ye) {
+                t1[lj + 1][li] = CI(m, n + 1, LA) * QC(m, n + 1) + CI(m, n + 1, LB) * q00;
+                t2[lj + 1][li] = R(i0, j0 + 1);
+            }
+            if (xe && ye) {
+                T a = CI(m + 1, n + 1, LSW) * q00;
+                a = a + CI(m + 1, n + 1, LNW) * QC(m, n + 1);
+                a = a + CI(m + 1, n + 1, LNE) * QC(m + 1, n + 1);
+                a = a + CI(m + 1, n + 1, LSE) * QC(m + 1, n);
+                t1[lj + 1][li + 1] = a;
+                t2[lj + 1][li + 1] = R(i0 + 1, j0 + 1);
+            }
+        }
+    }
+    __syncthreads();
+    // phase 2: thread e -> (i offset e / 32, j offset e % 32): j-contiguous runs of 32 in qT
+    for (int e = threadIdx.x; e < 64 * 32; e += 512) {
+        const int li = e / 32, lj = e % 32;
+        const int i = it + li, j = jt + lj;
+        if (i > imax || j > jmax) continue;
+        T *p = qT + (std::size_t)i * jjf + j;
+        T v = *p + t1[lj][li];
+        if (!((li % 2 == 0) && (lj % 2 == 0))) v = v + t2[lj][li];
+        *p = v;
+    }
+}
+
 template <typename T>
 void transpose(const T *src, T *dst, int rows, int cols, hipStream_t s, int nb) {
     if constexpr (sizeof(T) == 4) {
@@ -2053,8 +2170,8 @@ void transpose_op(const T *so, T *soT, int nstncl, int ii, int jj, hipStream_t s
 template <typename T>
 static void relax_lines_ytr(const T *soT, int nstncl, const T *qf, T *q, const T *sory, T *qT,
                             T *qfT, int ii, int jj, int updown, int shape, hipStream_t s, int nb,
-                            bool qfT_ready = false) {
-    transpose<T>(q, qT, jj, ii, s, nb);
+                            bool qfT_ready = false, bool q_in_T = false, bool keep_T = false) {
+    if (!q_in_T) transpose<T>(q, qT, jj, ii, s, nb);
     if (!qfT_ready) transpose<T>(qf, qfT, jj, ii, s, nb);
     const int nfix = ii - 2;  // columns of the original grid = rows of the transposed one
     for (int t = 0; t < 2; ++t) {
@@ -2068,7 +2185,7 @@ static void relax_lines_ytr(const T *soT, int nstncl, const T *qf, T *q, const T
             launch_class<T, false, 2>(shape, soT, qfT, qT, sory, jj, ii, first, nlines, s, nb,
                                       nullptr, t);
     }
-    transpose<T>(qT, q, ii, jj, s, nb);
+    if (!keep_T) transpose<T>(qT, q, ii, jj, s, nb);
 }
 
 static int tuned_transposed(const tune_key &k) {
@@ -2080,14 +2197,21 @@ static int tuned_transposed(const tune_key &k) {
 template <typename T>
 bool relax_lines_y_auto(const T *so, const T *soT, int nstncl, const T *qf, T *q, const T *sory,
                         T *qT, T *qfT, int ii, int jj, int updown, hipStream_t s, int nb,
-                        bool qfT_ready) {
-    const int c = soT ? tuned_transposed(key_of<T>(2, nstncl, ii, jj, nb)) : -1;
+                        bool qfT_ready, bool q_in_T, bool keep_T) {
+    const int c = uses_transposed_y<T>(soT, nstncl, ii, jj, nb);
     if (c < 0) {
+        if (q_in_T) transpose<T>(qT, q, ii, jj, s, nb);
         relax_lines<T, 1>(so, nstncl, qf, q, sory, ii, jj, updown, s, nb);
         return false;
     }
-    relax_lines_ytr<T>(soT, nstncl, qf, q, sory, qT, qfT, ii, jj, updown, c, s, nb, qfT_ready);
+    relax_lines_ytr<T>(soT, nstncl, qf, q, sory, qT, qfT, ii, jj, updown, c, s, nb, qfT_ready,
+                       q_in_T, keep_T);
     return true;
+}
+
+template <typename T>
+int uses_transposed_y(const T *soT, int nstncl, int ii, int jj, int nb) {
+    return soT ? tuned_transposed(key_of<T>(2, nstncl, ii, jj, nb)) : -1;
 }
 
 // Milliseconds for 3 replays of what run(cs) launches, captured as a graph on the capture stream
@@ -2159,7 +2283,8 @@ int tune_lines_ytr(const T *so, const T *soT, int nstncl, const T *qf, T *q, con
     for (int c = 0; c < ncand; ++c) {
         const float t = timed([&](hipStream_t st) {
             for (int k = 0; k < sweeps; ++k)
-                relax_lines_ytr<T>(soT, nstncl, qf, q, sory, qT, qfT, ii, jj, 0, c, st, nb, k > 0);
+                relax_lines_ytr<T>(soT, nstncl, qf, q, sory, qT, qfT, ii, jj, 0, c, st, nb, k > 0,
+                                   k > 0, k < sweeps - 1);
         });
         if (t >= 0.f && t < tbest) {
             tbest = t;
@@ -2218,6 +2343,25 @@ void interp_add(T *q, const T *qc, const T *res, const T *so, int nstncl, const 
     // one thread per 2 x 2 fine block (the range includes the ghost point of even sizes)
     interp_add_k<T><<<gridb(iif / 2, jjf / 2, nb), 256, 0, s>>>(q, qc, res, so, ci, iic, jjc, iif,
                                                                 jjf, nstncl);
+}
+
+template <typename T>
+void residual_qT(const T *so, int nstncl, const T *qf, const T *qT, T *res, int ii, int jj,
+                 hipStream_t s, int nb) {
+    const dim3 g((unsigned)((ii - 2 + kTI - 1) / kTI), (unsigned)((jj - 2 + kTJ - 1) / kTJ),
+                 (unsigned)nb);
+    if (nstncl == 5)
+        residual_qT_k<T, true><<<g, 256, 0, s>>>(so, qf, qT, res, ii, jj);
+    else
+        residual_qT_k<T, false><<<g, 256, 0, s>>>(so, qf, qT, res, ii, jj);
+}
+
+template <typename T>
+void interp_add_qT(T *qT, const T *qc, const T *res, const T *so, int nstncl, const T *ci,
+                   int iic, int jjc, int iif, int jjf, hipStream_t s, int nb) {
+    // tiles of 32 x 16 blocks of 2 x 2 fine points over the blocks m <= iif/2, n <= jjf/2
+    const dim3 g((unsigned)((iif / 2 + 31) / 32), (unsigned)((jjf / 2 + 15) / 16), (unsigned)nb);
+    interp_add_qT_k<T><<<g, 512, 0, s>>>(qT, qc, res, so, ci, iic, jjc, iif, jjf, nstncl);
 }
 
 void setup_interp(const double *so, int nstncl, double *ci, int iif, int jjf, int iic, int jjc,
@@ -3102,7 +3246,12 @@ void copy(const T *src, T *dst, std::size_t n, hipStream_t s) {
                                   double *, hipStream_t, int);                                    \
     template void transpose_op<T>(const T *, T *, int, int, int, hipStream_t, int);               \
     template bool relax_lines_y_auto<T>(const T *, const T *, int, const T *, T *, const T *, T *, \
-                                        T *, int, int, int, hipStream_t, int, bool);              \
+                                        T *, int, int, int, hipStream_t, int, bool, bool, bool);  \
+    template int uses_transposed_y<T>(const T *, int, int, int, int);                           \
+    template void residual_qT<T>(const T *, int, const T *, const T *, T *, int, int, hipStream_t, \
+                                 int);                                                            \
+    template void interp_add_qT<T>(T *, const T *, const T *, const T *, int, const T *, int, int, \
+                                   int, int, hipStream_t, int);                                   \
     template int tune_lines_ytr<T>(const T *, const T *, int, const T *, T *, const T *, T *, T *, \
                                    int, int, hipStream_t, int, int);                              \
     template void restrict_<T>(const T *, const T *, T *, int, int, int, int, hipStream_t, int);  \

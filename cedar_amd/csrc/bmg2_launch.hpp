@@ -124,10 +124,25 @@ void transpose_op(const T *so, T *soT, int nstncl, int ii, int jj, hipStream_t s
 // extents; bitwise the same result either way.  soT may be null (direct y-lines).  With
 // qfT_ready the caller guarantees that qfT already holds the transpose of qf.  Returns whether
 // the transposed path ran (qfT then holds the transpose of qf).
+// q_in_T: qT already holds the iterate (q is stale; no transpose in); keep_T: leave the result in
+// qT only (no transpose back; q stays stale).  Both apply only where the transposed path runs.
 template <typename T>
 bool relax_lines_y_auto(const T *so, const T *soT, int nstncl, const T *qf, T *q, const T *sory,
                         T *qT, T *qfT, int ii, int jj, int updown, hipStream_t s, int nb = 1,
-                        bool qfT_ready = false);
+                        bool qfT_ready = false, bool q_in_T = false, bool keep_T = false);
+// The shape the transposed y path runs with at these extents, or -1 where the direct y-line
+// kernel runs (no transposed operator, or tuning chose it).
+template <typename T>
+int uses_transposed_y(const T *soT, int nstncl, int ii, int jj, int nb);
+// residual and interp_add with the fine iterate in the transposed layout qT (ii rows of jj
+// values); the other arrays in the normal layout.  Bitwise equal to residual / interp_add on the
+// untransposed iterate.  interp_add_qT updates qT in place.
+template <typename T>
+void residual_qT(const T *so, int nstncl, const T *qf, const T *qT, T *res, int ii, int jj,
+                 hipStream_t s, int nb = 1);
+template <typename T>
+void interp_add_qT(T *qT, const T *qc, const T *res, const T *so, int nstncl, const T *ci,
+                   int iic, int jjc, int iif, int jjf, hipStream_t s, int nb = 1);
 // Times the direct y-lines against the transposed path (each x-line shape) once and records the
 // choice (-1: direct); `sweeps` y sweeps are timed per right-hand side (one transpose of qf), as
 // a level visit of a V(nu1, nu2) cycle makes nu1 + nu2 of them.  Overwrites q, qT, qfT;

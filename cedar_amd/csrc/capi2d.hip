@@ -391,6 +391,17 @@ int cdr_hip_relax_lines_y2d_f32(const float *so, int ns, const float *qf, float 
     return api2::relax_lines<float>(1, so, ns, qf, q, sor, ii, jj, ud, S(st));
 }
 
+int cdr_hip_relax_lines2d_batched_f64(int dir, const double *so, int ns, const double *qf,
+                                      double *q, const double *sor, int ii, int jj, int nb,
+                                      int ud, void *st) {
+    return api2::relax_lines<double>(dir, so, ns, qf, q, sor, ii, jj, ud, S(st), nb);
+}
+int cdr_hip_relax_lines2d_batched_f32(int dir, const float *so, int ns, const float *qf,
+                                      float *q, const float *sor, int ii, int jj, int nb, int ud,
+                                      void *st) {
+    return api2::relax_lines<float>(dir, so, ns, qf, q, sor, ii, jj, ud, S(st), nb);
+}
+
 int cdr_hip_relax_lines_xy2d_f64(const double *so, int ns, const double *qf, double *q,
                                  const double *sorx, const double *sory, double *work, int ii,
                                  int jj, int ud, void *st) {

@@ -507,9 +507,15 @@ private:
 
     // bT_ok: the level's transposed right-hand side (L.bT_buf) already holds b; set when a y sweep
     // through the transposed grid has transposed it (b does not change within one visit of a
-    // level, so the DOWN and UP smoothing of a visit share it)
+    // level, so the DOWN and UP smoothing of a visit share it).
+    // xT (optional): the iterate lives in L.xT_buf, transposed, and x itself is stale.  Going in,
+    // a y sweep through the transposed grid then starts from L.xT_buf; going out, a DOWN
+    // smoothing whose last sweep is such a y sweep leaves the iterate there (the residual and
+    // interp_add of the visit read it in that layout) and an UP smoothing always ends with x
+    // current.  Without xT every sweep leaves x current.
     template <typename T>
-    void smooth(level_data<T> &L, grid_func<T> &x, const grid_func<T> &b, Dir d, bool &bT_ok) {
+    void smooth(level_data<T> &L, grid_func<T> &x, const grid_func<T> &b, Dir d, bool &bT_ok,
+                bool *xT = nullptr) {
         const int nrep = d == Dir::DOWN ? set_.nrelax_pre : set_.nrelax_post;
         for (int r = 0; r < nrep; ++r) {
             switch (set_.relaxation) {
@@ -527,14 +533,24 @@ private:
                                                        L.ext.jj, (int)d, jpn_, cur_, L.ext.nb),
                               "relax_lines_per");
                     } else if (xs) {
+                        if (xT && *xT) {  // the iterate back from the transposed layout
+                            k2::transpose<T>(L.xT_buf.data(), x.p, L.ext.ii, L.ext.jj, cur_,
+                                             L.ext.nb);
+                            *xT = false;
+                        }
                         check(api2::relax_lines<T>(0, L.A.p, L.A.nstncl, b.p, x.p, L.SOR[0].p,
                                                    L.ext.ii, L.ext.jj, (int)d, cur_, L.ext.nb),
                               "relax_lines_x");
                     } else {
+                        // the last sweep of a DOWN smoothing may leave the iterate transposed
+                        const bool last = r == nrep - 1 && half == 1;
+                        const bool keep = xT && d == Dir::DOWN && last;
+                        const bool in_T = xT && *xT;
                         bT_ok = k2::relax_lines_y_auto<T>(
                             L.A.p, L.soT_buf.data(), L.A.nstncl, b.p, x.p, L.SOR[1].p,
                             L.xT_buf.data(), L.bT_buf.data(), L.ext.ii, L.ext.jj, (int)d, cur_,
-                            L.ext.nb, bT_ok);
+                            L.ext.nb, bT_ok, in_T, keep);
+                        if (xT) *xT = bT_ok && keep;
                         check_hip(hipGetLastError(), "relax_lines_y");
                     }
                 }
@@ -597,9 +613,18 @@ private:
         auto &F = L[lvl];
         auto &C = L[lvl + 1];
         bool bT_ok = false;  // b is fixed for this visit of the level
+        // xT: between the smoothings the iterate may stay in the transposed layout of the y
+        // sweeps (F.xT_buf), read there by the residual and updated there by interp_add
+        bool xT = false;
         // timer labels and levels as in include/cedar/cycle/vcycle.h:61-109
-        timed(lvl, "relaxation", [&] { smooth<T>(F, x, b, Dir::DOWN, bT_ok); });
-        timed(lvl, "residual", [&] { km_.run<kernels::residual<T>>(F.A, x, b, F.res); });
+        timed(lvl, "relaxation", [&] { smooth<T>(F, x, b, Dir::DOWN, bT_ok, &xT); });
+        timed(lvl, "residual", [&] {
+            if (xT)
+                k2::residual_qT<T>(F.A.p, F.A.nstncl, b.p, F.xT_buf.data(), F.res.p, F.ext.ii,
+                                   F.ext.jj, cur_, F.ext.nb);
+            else
+                km_.run<kernels::residual<T>>(F.A, x, b, F.res);
+        });
         timed(lvl, "restrict", [&] { km_.run<kernels::restriction<T>>(C.P, F.res, C.b); });
         k2::fill<T>(C.x.p, T(0), C.ext.total(), cur_);
         if (lvl + 1 == (int)L.size() - 1)
@@ -608,8 +633,18 @@ private:
             });
         else
             ncycle<T>(L, lvl + 1, C.x, C.b);
-        timed(lvl, "interp-add", [&] { km_.run<kernels::interp_add<T>>(C.P, C.x, F.res, x); });
-        timed(lvl, "relaxation", [&] { smooth<T>(F, x, b, Dir::UP, bT_ok); });
+        timed(lvl, "interp-add", [&] {
+            if (xT)
+                k2::interp_add_qT<T>(F.xT_buf.data(), C.x.p, F.res.p, F.A.p, F.A.nstncl, C.P.p,
+                                     C.ext.ii, C.ext.jj, F.ext.ii, F.ext.jj, cur_, F.ext.nb);
+            else
+                km_.run<kernels::interp_add<T>>(C.P, C.x, F.res, x);
+        });
+        timed(lvl, "relaxation", [&] { smooth<T>(F, x, b, Dir::UP, bT_ok, &xT); });
+        if (xT) {  // an UP smoothing without x sweeps: the iterate back to its own layout
+            k2::transpose<T>(F.xT_buf.data(), x.p, F.ext.ii, F.ext.jj, cur_, F.ext.nb);
+            xT = false;
+        }
     }
 
 public:

@@ -122,6 +122,18 @@ int cdr_hip_relax_lines_x2d_f32(const float *so, int nstncl, const float *qf, fl
 int cdr_hip_relax_lines_y2d_f32(const float *so, int nstncl, const float *qf, float *q,
                                 const float *sor, float *b, int ii, int jj, int updown,
                                 void *stream);
+/* Batched forms of the two line sweeps above (no counterpart in the reference, which relaxes one
+ * grid per call): nb problems of the same extents stored back to back -- so as nb blocks of
+ * nstncl planes, qf and q as nb grids, sor as nb blocks of 2 planes (each problem's own line
+ * factors) -- relaxed by one launch per colour.  dir 0: x-lines, 1: y-lines.  The plane
+ * relaxation of the 3D solver and the batched 2D solver run this form; each problem gets
+ * bitwise the result of its own single-problem call. */
+int cdr_hip_relax_lines2d_batched_f64(int dir, const double *so, int nstncl, const double *qf,
+                                      double *q, const double *sor, int ii, int jj, int nb,
+                                      int updown, void *stream);
+int cdr_hip_relax_lines2d_batched_f32(int dir, const float *so, int nstncl, const float *qf,
+                                      float *q, const float *sor, int ii, int jj, int nb,
+                                      int updown, void *stream);
 
 /* One line-xy smoothing step: x-lines then y-lines (DOWN) or y then x (UP), the composition
  * Cedar's smoother makes of the two line kernels (include/cedar/multilevel.h:175-178, 204-207),

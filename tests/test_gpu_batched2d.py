@@ -100,3 +100,26 @@ def test_batchset_mixed_sizes_equal_individual(C, concurrent):
         for xs, hs in runs:
             assert np.array_equal(hs[k], h1), k
             assert torch.equal(xs[k], x1), k
+
+
+@pytest.mark.parametrize("direction", ["x", "y"])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("nine", [True, False])
+def test_batched_line_sweep_entry_points(C, direction, dtype, nine):
+    """cdr_hip_relax_lines2d_batched_*: one call over a stack of problems gives every problem
+    bitwise the result of its own single-problem sweep, both colour orders."""
+    K = C.cdr2
+    so, b = problems(77, 3, nine)
+    rng = np.random.default_rng(3)
+    q = np.zeros_like(b)
+    q[:, 1:-1, 1:-1] = rng.standard_normal((3, 77, 77))
+    sor = torch.stack([K.setup_lines(dev(so[k]), torch.zeros((2,) + b.shape[1:],
+                                                           dtype=torch.float64, device="cuda"),
+                                     direction) for k in range(3)])
+    for ud in (0, 1):
+        want = [K.relax_lines(dev(so[k]).to(dtype), dev(b[k]).to(dtype), dev(q[k]).to(dtype),
+                              sor[k].to(dtype), ud, direction) for k in range(3)]
+        got = K.relax_lines_batched(dev(so).to(dtype), dev(b).to(dtype), dev(q).to(dtype),
+                                    sor.to(dtype).contiguous(), ud, direction)
+        for k in range(3):
+            assert torch.equal(got[k], want[k]), (k, ud)

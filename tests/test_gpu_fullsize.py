@@ -118,6 +118,10 @@ def test_c2_solve_full_size(C, c2):
     x64, h64 = C.Solver(dso, **kw).solve(db)
     x32, h32 = C.Solver(dso, precision="mixed", **kw).solve(db)
     x64, x32 = x64.cpu().numpy(), x32.cpu().numpy()
+    rel = np.abs(h64 - h_cpu) / h_cpu
+    print(f"C2 FP64: {len(h64)} cycles (oracle {len(h_cpu)}), max |drho| "
+          f"{np.abs(h64 - h_cpu).max():.2e}, max |drho_i|/rho_i {rel.max():.2e}, rel l2 "
+          f"{rel_l2(x64, x_cpu):.2e}", flush=True)
     assert len(h64) == len(h_cpu)
     assert np.abs(h64 - h_cpu).max() <= 1e-12
     assert rel_l2(x64, x_cpu) <= 1e-10
@@ -137,8 +141,14 @@ def test_c1_solve(C):
     x64, h64 = C.Solver(dso, **kw).solve(db)
     x32, h32 = C.Solver(dso, precision="mixed", **kw).solve(db)
     x64, x32 = x64.cpu().numpy(), x32.cpu().numpy()
+    rel = np.abs(h64 - h_cpu) / h_cpu
+    print(f"C1 FP64: {len(h64)} cycles, max |drho_i|/rho_i {rel.max():.2e}", flush=True)
     assert len(h64) == len(h_cpu) and 8 <= len(h64) <= 12
     assert np.abs(h64 - h_cpu).max() <= 1e-12
+    # the smoothing and transfer kernels are bitwise the oracle's, but the coarse operators (a
+    # generic P^T A P) differ from the oracle's at the last bit, so the deviation relative to
+    # rho_i grows as rho_i falls: about 1e-14 in the first cycles, 4e-6 at rho = 1e-10
+    assert rel[:2].max() <= 1e-12
     assert rel_l2(x64, x_cpu) <= 1e-10
     assert len(h32) == len(h64) and h32[-1] < 1e-10
     assert rel_l2(x32, x64) < 1e-9

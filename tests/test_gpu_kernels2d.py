@@ -336,15 +336,23 @@ def test_autotune_records_shapes(C):
 
 @pytest.mark.parametrize("precision", ["f64", "mixed"])
 @pytest.mark.parametrize("ns", [3, 5])
-def test_transposed_y_sweeps_are_bitwise_direct(C, precision, ns):
+@pytest.mark.parametrize("pre,post,nb", [(1, 1, 1), (2, 1, 1), (1, 2, 1), (1, 1, 2)])
+def test_transposed_y_sweeps_are_bitwise_direct(C, precision, ns, pre, post, nb):
     """Line-xy solves with the y sweeps run as x sweeps of the transposed grid (pinned at every
-    level) give bitwise the iterates and histories of the direct y-line kernels."""
+    level) give bitwise the iterates and histories of the direct y-line kernels.  On that path
+    the iterate stays transposed from the last pre-smoothing y sweep through the residual and
+    interp_add to the first post-smoothing y sweep, so this also pins the residual and
+    interp_add kernels that read and update it in that layout."""
     from cedar_amd import gallery as G
     n = 257
     if ns == 5:
         so, b, _ = G.heterogeneous_9pt(n, seed=8, block=16, sigma=1.5, contrast=1e3)
     else:
         so, b = G.poisson_uniform_rhs(n, seed=8)
+    if nb > 1:
+        so2, b2, _ = G.heterogeneous_9pt(n, seed=9, block=16, sigma=0.5, contrast=1e2)
+        so = np.stack([so, so2 if ns == 5 else so2[:3]])
+        b = np.stack([b, b2])
     lib = _abi.load()
     bits = 64 if precision == "f64" else 32
     sizes, m = [], n
@@ -352,19 +360,22 @@ def test_transposed_y_sweeps_are_bitwise_direct(C, precision, ns):
         sizes.append(m)
         m = int((m - 1) / 2.0 + 1)
     prev = lib.cdr_hip_autotune(0)
-    kw = dict(relaxation="line-xy", nrelax_pre=1, nrelax_post=1, maxiter=40, tol=1e-10,
+    kw = dict(relaxation="line-xy", nrelax_pre=pre, nrelax_post=post, maxiter=40, tol=1e-10,
               precision=precision)
     try:
         x0, h0 = C.Solver(so, **kw).solve(dev(b))
         for lvl, m in enumerate(sizes):
             stencil = ns if lvl == 0 else 5
             if lib.cdr_hip_line_shapes2d(2, bits, stencil, m + 2, m + 2) > 0:
-                assert lib.cdr_hip_set_line_shape2d(2, bits, stencil, m + 2, m + 2, 1, 0) == 0
+                assert lib.cdr_hip_set_line_shape2d(2, bits, stencil, m + 2, m + 2, nb, 0) == 0
         x1, h1 = C.Solver(so, **kw).solve(dev(b))
     finally:
         for lvl, m in enumerate(sizes):
-            lib.cdr_hip_set_line_shape2d(2, bits, ns if lvl == 0 else 5, m + 2, m + 2, 1, -1)
+            lib.cdr_hip_set_line_shape2d(2, bits, ns if lvl == 0 else 5, m + 2, m + 2, nb, -1)
         lib.cdr_hip_autotune(prev)
-    assert np.array_equal(h0, h1)
+    h0, h1 = [(h if nb > 1 else [h]) for h in (h0, h1)]
+    assert len(h0) == len(h1) == nb
+    for a, c in zip(h0, h1):
+        assert np.array_equal(a, c)
+        assert a[-1] < 1e-10
     assert torch.equal(x0, x1)
-    assert h0[-1] < 1e-10

@@ -78,10 +78,12 @@ def test_poisson3_reference_test(C):
     assert np.abs(x.cpu().numpy() - exact).max() < 1e-4
 
 
+@pytest.mark.parametrize("updown", [0, 1])
 @pytest.mark.parametrize("family", ["xy", "xz", "yz"])
-def test_exact_plane_solves(C, family):
+def test_exact_plane_solves(C, family, updown):
     """test/3d/test_planes.cc restated: with exact plane solves (point, 50 cycles, tol 1e-30)
-    the planes of the second colour solve their plane systems."""
+    the planes of the colour solved last solve their plane systems: the even planes going DOWN,
+    the odd planes going UP (include/cedar/3d/relax_planes.h:44-56)."""
     from tests.test_oracle3 import stencil_matrix
     dims = {"xy": (31, 35, 5), "xz": (31, 5, 35), "yz": (5, 31, 35)}[family]
     nx, ny, nz = dims
@@ -94,7 +96,7 @@ def test_exact_plane_solves(C, family):
     plane = C.cdr3.plane_settings(relaxation="point", maxiter=50, tol=1e-30)
     g = C.Solver3(dev(so), relaxation="plane-" + family, num_levels=2, plane=plane)
     xg = dev(x)
-    g.relax_planes(0, family, xg, dev(b), 0)
+    g.relax_planes(0, family, xg, dev(b), updown)
     xr = xg.cpu().numpy()
     hx, hy, hz = 1.0 / (nx + 1), 1.0 / (ny + 1), 1.0 / (nz + 1)
     xh, yh, zh = hy * hz / hx, hx * hz / hy, hx * hy / hz
@@ -103,7 +105,7 @@ def test_exact_plane_solves(C, family):
           "xz": [[0, -zh, 0], [-xh, d, -xh], [0, -zh, 0]],
           "yz": [[0, -zh, 0], [-yh, d, -yh], [0, -zh, 0]]}[family]
     _, _, npl = O3.plane_dims(family, ii, jj, kk)
-    for ipl in range(2, npl + 1, 2):
+    for ipl in range(2 if updown == 0 else 1, npl + 1, 2):
         b2 = O3.copy_rhs(so, xr, b, family, ipl)
         n1, n2 = b2.shape[1] - 2, b2.shape[0] - 2
         sol = sla.spsolve(stencil_matrix(n1, n2, st).tocsc(), b2[1:-1, 1:-1].ravel())
@@ -161,3 +163,55 @@ def test_mixed_precision_3d(C, relax, n):
     assert h32[-1] < 1e-10
     assert abs(len(h32) - len(h64)) <= 1
     assert rel_l2(x32.cpu().numpy(), x64.cpu().numpy()) < 1e-9
+
+
+@pytest.mark.parametrize("relax,down,up", [
+    # include/cedar/multilevel.h:179-183 (pre) and :208-212 (post)
+    ("plane-xyz", ("xy", "yz", "xz"), ("xz", "yz", "xy")), ("plane-xy", ("xy",), ("xy",))])
+def test_plane_cycle_is_the_family_sweeps_in_reference_order(C, relax, down, up):
+    """One two-level V(1,1) plane cycle of the GPU solver equals, bit for bit, the GPU family
+    sweeps in the reference's order around the GPU residual, restriction, coarse band solve and
+    interp_add (each of which is pinned to the oracle on its own)."""
+    K = C.cdr3
+    so, b, _ = G.heterogeneous_7pt(13, seed=8, block=4, sigma=1.0, aniso=(1.0, 0.3, 0.1))
+    g = C.Solver3(dev(so), relaxation=relax, nrelax_pre=1, nrelax_post=1, num_levels=2)
+    x0 = np.zeros_like(b)
+    x0[1:-1, 1:-1, 1:-1] = np.random.default_rng(9).standard_normal(tuple(n - 2 for n in b.shape))
+    want = g.vcycle(dev(x0), dev(b))
+    db, dso = dev(b), dev(so)
+    x = dev(x0)
+    for d in down:
+        g.relax_planes(0, d, x, db, 0)
+    r = torch.zeros_like(x)
+    K.residual(dso, db, x, r)
+    so1, ci1 = g.level_arrays(1)
+    so1, ci1 = dev(so1), dev(ci1)
+    bc = torch.zeros(so1.shape[1:], dtype=torch.float64, device="cuda")
+    K.restrict(ci1, r, bc)
+    xc = K.solve_cg(torch.zeros_like(bc), bc, K.setup_cg(so1))
+    K.interp_add(x, xc, r, dso, ci1)
+    for d in up:
+        g.relax_planes(0, d, x, db, 1)
+    torch.cuda.synchronize()
+    assert torch.equal(x, want)
+    # and the oracle's cycle agrees to rounding (its plane solves use the sequential line order)
+    o = O3.Solver3(so, relaxation=relax, nrelax_pre=1, nrelax_post=1, num_levels=2)
+    xo = o.vcycle(x0, b)
+    assert rel_l2(want.cpu().numpy(), xo) <= 1e-12
+
+
+def test_c3_shaped_plane_solve_history(C):
+    """The C3 problem family at 63^3 (7-point FV, (kx, ky, kz) = (1, 1, 1e-3) k, lognormal k with
+    sigma 1.5 on 8^3 blocks, plane-xy V(1,1) to 1e-10), FP64 GPU against the oracle: the same
+    cycle count, every cycle's relative residual within 1e-12 (absolute, of rho = |r_i| / |r_0|),
+    the solution within 1e-10; the largest per-cycle relative deviation |drho_i| / rho_i is
+    printed (the plane solves' line scans round differently from the sequential substitution)."""
+    so, b, _ = G.heterogeneous_7pt(63, seed=3, block=8, sigma=1.5, aniso=(1.0, 1.0, 1e-3))
+    xo, ho, xg, hg = compare(C, so, b, relaxation="plane-xy", nrelax_pre=1, nrelax_post=1,
+                             maxiter=60, tol=1e-10)
+    rel = np.abs(hg - ho) / ho
+    print(f"C3-shaped 63^3: {len(hg)} cycles (oracle {len(ho)}), max |drho| {np.abs(hg - ho).max():.2e}, "
+          f"max |drho_i|/rho_i {rel.max():.2e}, rel l2 {rel_l2(xg, xo):.2e}", flush=True)
+    assert len(hg) == len(ho) and hg[-1] < 1e-10
+    assert np.abs(hg - ho).max() <= 1e-12
+    assert rel_l2(xg, xo) <= 1e-10

@@ -140,8 +140,12 @@ PLANE_CASES = [("xy", (31, 35, 5), False), ("xy", (31, 35, 5), True),
                ("yz", (5, 31, 35), False), ("yz", (5, 31, 35), True)]
 
 
+@pytest.mark.parametrize("updown", [O3.DOWN, O3.UP])
 @pytest.mark.parametrize("direction,dims,fe", PLANE_CASES)
-def test_plane_relaxation_solves_planes(direction, dims, fe):
+def test_plane_relaxation_solves_planes(direction, dims, fe, updown):
+    """The colour solved last satisfies its plane systems exactly: the even planes going DOWN
+    (colours 1 then 2) and the odd planes going UP (colours 2 then 1), as in
+    include/cedar/3d/relax_planes.h:44-56."""
     nx, ny, nz = dims
     so = G.fe3(nx, ny, nz) if fe else G.poisson3(nx, ny, nz)
     kk, jj, ii = nz + 2, ny + 2, nx + 2
@@ -151,7 +155,7 @@ def test_plane_relaxation_solves_planes(direction, dims, fe):
     b = x.copy()
     exact_planes = O3.plane_settings(relaxation="point", maxiter=50, tol=1e-30)
     s = O3.Solver3(so, relaxation="plane-" + direction, num_levels=2, plane=exact_planes)
-    xr = s.relax_planes(0, direction, x, b, O3.DOWN)
+    xr = s.relax_planes(0, direction, x, b, updown)
     hx, hy, hz = 1.0 / (nx + 1), 1.0 / (ny + 1), 1.0 / (nz + 1)
     xh, yh, zh = hy * hz / hx, hx * hz / hy, hx * hy / hz
     d = 2 * xh + 2 * yh + 2 * zh
@@ -164,12 +168,50 @@ def test_plane_relaxation_solves_planes(direction, dims, fe):
     else:
         st = [[0, -zh, 0], [-yh, d, -yh], [0, -zh, 0]]
     _, _, npl = O3.plane_dims(direction, ii, jj, kk)
-    for ipl in range(2, npl + 1, 2):
+    for ipl in range(2 if updown == O3.DOWN else 1, npl + 1, 2):
         b2 = O3.copy_rhs(so, xr, b, direction, ipl)
         n1, n2 = b2.shape[1] - 2, b2.shape[0] - 2
         sol = sla.spsolve(stencil_matrix(n1, n2, st).tocsc(), b2[1:-1, 1:-1].ravel())
         got = O3.copy32(xr, direction, ipl)[1:-1, 1:-1].ravel()
         assert np.abs(got - sol).max() < 1e-8, ipl
+
+
+def compose_vcycle2(s, so, b, x, families_down, families_up):
+    """One two-level V(1,1) cycle assembled from separately pinned pieces: plane sweeps of the
+    given families, residual, restriction, band Cholesky coarse solve, interp_add."""
+    for d in families_down:
+        x = s.relax_planes(0, d, x, b, O3.DOWN)
+    r = O3.residual(so, b, x)
+    so1, ci1 = s.level(1)
+    abd, info = O3.setup_cg(so1)
+    assert info == 0
+    xc = O3.solve_cg(abd, O3.restrict(ci1, r))
+    x, _ = O3.interp_add(x, xc, r, so, ci1)
+    for d in families_up:
+        x = s.relax_planes(0, d, x, b, O3.UP)
+    return x
+
+
+@pytest.mark.parametrize("relax,down,up", [
+    # include/cedar/multilevel.h:179-183 (pre) and :208-212 (post)
+    ("plane-xyz", ("xy", "yz", "xz"), ("xz", "yz", "xy")),
+    ("plane-xy", ("xy",), ("xy",)), ("plane-xz", ("xz",), ("xz",)),
+    ("plane-yz", ("yz",), ("yz",))])
+def test_plane_cycle_is_the_family_sweeps_in_reference_order(relax, down, up):
+    """The oracle's plane V-cycle equals, bit for bit, the family sweeps in the order
+    include/cedar/multilevel.h gives them (pre: xy, yz, xz going DOWN; post: xz, yz, xy going UP)
+    around the pinned residual / restriction / coarse solve / interpolation; any other order of
+    the three families gives a different result."""
+    so, b, _ = G.heterogeneous_7pt(13, seed=8, block=4, sigma=1.0, aniso=(1.0, 0.3, 0.1))
+    s = O3.Solver3(so, relaxation=relax, nrelax_pre=1, nrelax_post=1, num_levels=2)
+    x0 = np.zeros_like(b)
+    x0[1:-1, 1:-1, 1:-1] = np.random.default_rng(9).standard_normal(tuple(n - 2 for n in b.shape))
+    want = s.vcycle(x0, b)
+    assert np.array_equal(compose_vcycle2(s, so, b, x0, down, up), want)
+    if relax == "plane-xyz":
+        for d2, u2 in ((("xy", "xz", "yz"), up), (down, ("xy", "yz", "xz")),
+                       (("yz", "xy", "xz"), up)):
+            assert not np.array_equal(compose_vcycle2(s, so, b, x0, d2, u2), want)
 
 
 def test_poisson3_point_solve():
