@@ -51,6 +51,7 @@ PROTOTYPES = {
     "cdr_hip_residual2d_f64": (_i, [_P, _i, _P, _P, _P, _i, _i, _vp]),
     "cdr_hip_residual2d_f32": (_i, [_P, _i, _P, _P, _P, _i, _i, _vp]),
     "cdr_hip_residual_norm2d_f64": (_i, [_P, _i, _P, _P, _P, _i, _i, _P, _P, _vp]),
+    "cdr_hip_residual_norm2d_f64_noalloc": (_i, [_P, _i, _P, _P, _P, _i, _i, _P, _vp]),
     "cdr_hip_sumsq2d_f64": (_i, [_P, _i, _i, _P, _P, _vp]),
     "cdr_hip_setup_recip2d_f64": (_i, [_P, _P, _i, _i, _i, _vp]),
     "cdr_hip_relax_gs2d_f64": (_i, [_P, _i, _P, _P, _P, _i, _i, _i, _vp]),

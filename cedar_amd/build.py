@@ -17,7 +17,7 @@ LIB = os.path.join(LIBDIR, "libcedar_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
 
 SOURCES = ["bmg2_kernels.hip", "capi2d.hip", "solver2d.cpp", "bmg3_kernels.hip", "capi3d.hip",
-           "solver3d.cpp", "adjoint2d.hip", "matvec.hip", "capi_handles.cpp"]
+           "solver3d.cpp", "adjoint2d.hip", "matvec.hip", "capi_handles.cpp", "noalloc2d.hip"]
 ARCH = os.environ.get("CEDAR_AMD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -318,6 +318,7 @@ const char *cdr_hip_status_string(int s) {
     case CDR_ERR_NO_DEVICE: return "no GPU available";
     case CDR_ERR_LEVELS: return "too many levels specified";
     case CDR_ERR_STALLED: return "residual stopped decreasing";
+    case CDR_ERR_TOO_LARGE: return "input larger than the library's fixed-size device buffer";
     default: return "unknown status";
     }
 }

@@ -35,7 +35,8 @@ typedef enum {
     CDR_ERR_NONFINITE = 4,   /* NaN or Inf in a residual norm */
     CDR_ERR_NO_DEVICE = 5,   /* no usable GPU */
     CDR_ERR_LEVELS = 6,      /* num_levels larger than the grid allows */
-    CDR_ERR_STALLED = 7      /* the solve stopped reducing the residual */
+    CDR_ERR_STALLED = 7,     /* the solve stopped reducing the residual */
+    CDR_ERR_TOO_LARGE = 8    /* the input needs more than a fixed-size library buffer holds */
 } cdr_status;
 
 enum { CDR_DOWN = 0, CDR_UP = 1 };  /* cycle::Dir, BMG_DOWN / BMG_UP */
@@ -84,6 +85,19 @@ size_t cdr_hip_norm_work2d(int ii, int jj);
 int cdr_hip_residual_norm2d_f64(const double *so, int nstncl, const double *qf, const double *q,
                                 double *res, int ii, int jj, double *work, double *sumsq_dev,
                                 void *stream);
+/* cdr_hip_residual_norm2d_f64 without the work argument, allocating nothing.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The per-block partial sums
+ * live in a library-owned device array of CDR_NOALLOC_NORM_PARTIALS doubles (4 MiB), one per GPU,
+ * held until the process ends; every partial a call reads was written by that call, so the
+ * array needs no clearing.  Extents with cdr_hip_norm_work2d(ii, jj) > CDR_NOALLOC_NORM_PARTIALS
+ * (above about 8190 x 8190 interior points) return CDR_ERR_TOO_LARGE.  Calls that overlap on one
+ * GPU (other streams or host threads) share the array and race: serialise them.  Results are
+ * bitwise those of cdr_hip_residual_norm2d_f64. */
+#define CDR_NOALLOC_NORM_PARTIALS 524288
+int cdr_hip_residual_norm2d_f64_noalloc(const double *so, int nstncl, const double *qf,
+                                        const double *q, double *res, int ii, int jj,
+                                        double *sumsq_dev, void *stream);
 /* sum of squares of the interior of x into *sumsq_dev (device). */
 int cdr_hip_sumsq2d_f64(const double *x, int ii, int jj, double *work, double *sumsq_dev,
                         void *stream);
