@@ -379,38 +379,83 @@ __device__ inline double dpp_move(double v, double ident) {
     return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
-template <int CTRL, int RMASK, typename T>
-__device__ inline void dpp_scan_step(T &a, T &b) {
-    const T ap = dpp_move<CTRL, RMASK>(a, T(1));
-    const T bp = dpp_move<CTRL, RMASK>(b, T(0));
-    b = a * bp + b;
-    a = a * ap;
+// The affine step a * c + b of the line recurrences: a fused multiply-add in FP32 (the
+// preconditioning V-cycle of the mixed solver), separate rounding in FP64.  Every line kernel
+// evaluates its recurrences through this one function, so all of them (every launch shape, the
+// direct and the transposed y sweeps, the fused coarse cycle) still round identically.
+__device__ inline float affine(float a, float c, float b) { return __builtin_fmaf(a, c, b); }
+__device__ inline double affine(double a, double c, double b) { return a * c + b; }
+// y / d for a line pivot d: FP32 multiplies by the hardware reciprocal (about 1 ulp), FP64
+// divides (correctly rounded, as the sequential substitution does).
+__device__ inline float div_pivot(float y, float d) { return y * __builtin_amdgcn_rcpf(d); }
+__device__ inline double div_pivot(double y, double d) { return y / d; }
+
+// the source lane's value, 0 where the source lane is out of range or the row is masked off
+template <int CTRL, int RMASK>
+__device__ inline float dpp_zero(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                 CTRL, RMASK, 0xF, true));
+}
+template <int CTRL, int RMASK>
+__device__ inline double dpp_zero(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, RMASK, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, RMASK, 0xF, true);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// lanes that receive a value in each step of the scan (row_shr:1/2/4/8 within 16-lane rows,
+// row_bcast:15 into rows 1 and 3, row_bcast:31 into rows 2 and 3); the others keep (a, b)
+__device__ inline int lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+template <int STEP>
+__device__ inline bool dpp_valid(int lane) {
+    return STEP == 0 ? (lane & 15) >= 1 : STEP == 1 ? (lane & 15) >= 2 : STEP == 2 ? (lane & 15) >= 4
+         : STEP == 3 ? (lane & 15) >= 8 : STEP == 4 ? ((lane >> 4) & 1) != 0 : lane >= 32;
+}
+
+// One step: (a, b) <- (a * a', a * b' + b) with (a', b') from the source lane.  Where no value
+// arrives, b' reads 0 (a * 0 + b = b exactly) and a is kept: the identity map.  The product
+// a * b' is rounded on its own in both precisions (a multiply that takes its lane move for free
+// is cheaper here than a fused multiply-add, which cannot).
+// The broadcast steps move data into every row (row 0 reads 0); the lanes outside the rows the
+// step feeds are then restored by the select, as the row_shr steps restore a.
+template <int CTRL, int STEP, typename T>
+__device__ inline void dpp_scan_step(T &a, T &b, int lane) {
+    const T ap = dpp_zero<CTRL, 0xF>(a);
+    const T bp = dpp_zero<CTRL, 0xF>(b);
+    const bool v = dpp_valid<STEP>(lane);
+    const T bn = a * bp + b;
+    b = (STEP < 4 || v) ? bn : b;
+    a = v ? a * ap : a;
 }
 
 template <typename T>
 __device__ inline void scan_affine_dpp(T &a, T &b) {
-    dpp_scan_step<0x111, 0xF>(a, b);  // row_shr:1
-    dpp_scan_step<0x112, 0xF>(a, b);  // row_shr:2
-    dpp_scan_step<0x114, 0xF>(a, b);  // row_shr:4
-    dpp_scan_step<0x118, 0xF>(a, b);  // row_shr:8
-    dpp_scan_step<0x142, 0xA>(a, b);  // row_bcast:15 -> rows 1 and 3
-    dpp_scan_step<0x143, 0xC>(a, b);  // row_bcast:31 -> rows 2 and 3
+    const int lane = lane_id();
+    dpp_scan_step<0x111, 0>(a, b, lane);  // row_shr:1
+    dpp_scan_step<0x112, 1>(a, b, lane);  // row_shr:2
+    dpp_scan_step<0x114, 2>(a, b, lane);  // row_shr:4
+    dpp_scan_step<0x118, 3>(a, b, lane);  // row_shr:8
+    dpp_scan_step<0x142, 4>(a, b, lane);  // row_bcast:15, kept in rows 1 and 3
+    dpp_scan_step<0x143, 5>(a, b, lane);  // row_bcast:31, kept in rows 2 and 3
 }
 
 // the multiplier part of scan_affine_dpp alone (the same products in the same order)
-template <int CTRL, int RMASK, typename T>
-__device__ inline void dpp_prod_step(T &a) {
-    a = a * dpp_move<CTRL, RMASK>(a, T(1));
+template <int CTRL, int STEP, typename T>
+__device__ inline void dpp_prod_step(T &a, int lane) {
+    const T ap = dpp_zero<CTRL, 0xF>(a);
+    a = dpp_valid<STEP>(lane) ? a * ap : a;
 }
 
 template <typename T>
 __device__ inline void scan_prod_dpp(T &a) {
-    dpp_prod_step<0x111, 0xF>(a);
-    dpp_prod_step<0x112, 0xF>(a);
-    dpp_prod_step<0x114, 0xF>(a);
-    dpp_prod_step<0x118, 0xF>(a);
-    dpp_prod_step<0x142, 0xA>(a);
-    dpp_prod_step<0x143, 0xC>(a);
+    const int lane = lane_id();
+    dpp_prod_step<0x111, 0>(a, lane);
+    dpp_prod_step<0x112, 1>(a, lane);
+    dpp_prod_step<0x114, 2>(a, lane);
+    dpp_prod_step<0x118, 3>(a, lane);
+    dpp_prod_step<0x142, 4>(a, lane);
+    dpp_prod_step<0x143, 5>(a, lane);
 }
 
 // value of lane 63, broadcast (v_readlane: no LDS round trip)
@@ -476,9 +521,9 @@ __global__ __launch_bounds__(256) void relax_lines_k(const T *__restrict__ so, c
             a = (k > 0) ? -E[k - 1] : T(0);
         }
         scan_affine_up(a, b, lane);
-        const T y = a * carry + b;
+        const T y = affine(a, carry, b);
         carry = __shfl(y, WAVE - 1, WAVE);
-        if (act) q[base + (std::size_t)k * stride] = y / D[k];
+        if (act) q[base + (std::size_t)k * stride] = div_pivot(y, D[k]);
     }
     carry = T(0);
     for (int c = nch - 1; c >= 0; --c) {
@@ -490,7 +535,7 @@ __global__ __launch_bounds__(256) void relax_lines_k(const T *__restrict__ so, c
             a = (k < n - 1) ? -E[k] : T(0);
         }
         scan_affine_down(a, b, lane);
-        const T x = a * carry + b;
+        const T x = affine(a, carry, b);
         carry = __shfl(x, 0, WAVE);
         if (act) q[base + (std::size_t)k * stride] = x;
     }
@@ -520,18 +565,18 @@ struct line_raw {
 template <typename T, bool NINE, bool SWAP = false>
 __device__ inline T line_rhs(const line_raw<T> &r) {
     T rhs = r.qf;
-    rhs = rhs + r.s0 * r.q0;
-    rhs = rhs + r.s1 * r.q1;
+    rhs = affine(r.s0, r.q0, rhs);
+    rhs = affine(r.s1, r.q1, rhs);
     if (NINE) {
-        rhs = rhs + r.sw0 * r.q0m;
+        rhs = affine(r.sw0, r.q0m, rhs);
         if (SWAP) {
-            rhs = rhs + r.nw1 * r.q1m;
-            rhs = rhs + r.nw0 * r.q0p;
+            rhs = affine(r.nw1, r.q1m, rhs);
+            rhs = affine(r.nw0, r.q0p, rhs);
         } else {
-            rhs = rhs + r.nw0 * r.q0p;
-            rhs = rhs + r.nw1 * r.q1m;
+            rhs = affine(r.nw0, r.q0p, rhs);
+            rhs = affine(r.nw1, r.q1m, rhs);
         }
-        rhs = rhs + r.sw1 * r.q1p;
+        rhs = affine(r.sw1, r.q1p, rhs);
     }
     return rhs;
 }
@@ -607,7 +652,7 @@ __device__ inline void seg_forward(T (&b)[CPW], AF afun, DF dval, seg_map<T> *xc
         asm volatile("" ::: "memory");
     }
     T carry = T(0);
-    for (int m = 0; m < w * CPW; ++m) carry = xch[m].a * carry + xch[m].b;
+    for (int m = 0; m < w * CPW; ++m) carry = affine(xch[m].a, carry, xch[m].b);
 #pragma unroll
     for (int u = 0; u < CPW; ++u) {
         T a = afun(u);
@@ -615,9 +660,9 @@ __device__ inline void seg_forward(T (&b)[CPW], AF afun, DF dval, seg_map<T> *xc
             scan_prod_dpp(a);
         else
             scan_affine_dpp(a, b[u]);
-        const T y = a * carry + b[u];
+        const T y = affine(a, carry, b[u]);
         carry = last_lane(y);
-        b[u] = y / dval(u);
+        b[u] = div_pivot(y, dval(u));
     }
 }
 
@@ -641,7 +686,7 @@ __device__ inline void seg_backward(T (&b)[CPW], CF coef, AF afun, seg_map<T> *x
         asm volatile("" ::: "memory");
     }
     T carry = T(0);
-    for (int m = W * CPW - 1; m >= (w + 1) * CPW; --m) carry = xch[m].a * carry + xch[m].b;
+    for (int m = W * CPW - 1; m >= (w + 1) * CPW; --m) carry = affine(xch[m].a, carry, xch[m].b);
 #pragma unroll
     for (int u = CPW - 1; u >= 0; --u) {
         T a;
@@ -652,7 +697,7 @@ __device__ inline void seg_backward(T (&b)[CPW], CF coef, AF afun, seg_map<T> *x
             coef(u, a, b[u]);
             scan_affine_dpp(a, b[u]);
         }
-        const T x = a * carry + b[u];
+        const T x = affine(a, carry, b[u]);
         carry = last_lane(x);
         b[u] = x;
     }
@@ -676,13 +721,13 @@ __device__ inline void seg_forward_ab(T (&a)[CPW], T (&b)[CPW], DF dval, seg_map
             }
         }
         lds_barrier();
-        for (int m = 0; m < w * CPW; ++m) carry = xch[m].a * carry + xch[m].b;
+        for (int m = 0; m < w * CPW; ++m) carry = affine(xch[m].a, carry, xch[m].b);
     }
 #pragma unroll
     for (int u = 0; u < CPW; ++u) {
-        const T y = a[u] * carry + b[u];
+        const T y = affine(a[u], carry, b[u]);
         carry = last_lane(y);
-        b[u] = y / dval(u);
+        b[u] = div_pivot(y, dval(u));
     }
 }
 
@@ -703,11 +748,11 @@ __device__ inline void seg_backward_ab(T (&a)[CPW], T (&b)[CPW], seg_map<T> *xch
             }
         }
         lds_barrier();
-        for (int m = W * CPW - 1; m >= (w + 1) * CPW; --m) carry = xch[m].a * carry + xch[m].b;
+        for (int m = W * CPW - 1; m >= (w + 1) * CPW; --m) carry = affine(xch[m].a, carry, xch[m].b);
     }
 #pragma unroll
     for (int u = CPW - 1; u >= 0; --u) {
-        const T x = a[u] * carry + b[u];
+        const T x = affine(a[u], carry, b[u]);
         carry = last_lane(x);
         b[u] = x;
     }
@@ -761,9 +806,9 @@ __global__ __launch_bounds__(L *W * 64) void relax_xlines_seg_k(const T *__restr
             T bu = line_rhs<T, NINE, TR>(r);
             T au = (k > 0) ? -E[max(kc - 1, 0)] : T(0);
             scan_affine_dpp(au, bu);
-            const T y = au * carry + bu;
+            const T y = affine(au, carry, bu);
             carry = last_lane(y);
-            b[u] = y / D[kc];
+            b[u] = div_pivot(y, D[kc]);
         }
         carry = T(0);
 #pragma unroll
@@ -771,7 +816,7 @@ __global__ __launch_bounds__(L *W * 64) void relax_xlines_seg_k(const T *__restr
             T au;
             bwd_coef(u, au, b[u]);
             scan_affine_dpp(au, b[u]);
-            const T x = au * carry + b[u];
+            const T x = affine(au, carry, b[u]);
             carry = last_lane(x);
             b[u] = x;
         }
@@ -1479,6 +1524,30 @@ __global__ __launch_bounds__(512) void transpose_k(const T *__restrict__ src, T 
     }
 }
 
+// The operator of the transposed grid: plane k of problem b of soT is the transpose of plane
+// from[k] of problem b of so (c <- c, w <- s, s <- w, sw <- sw, nw <- nw); z = b * nstncl + k.
+template <typename T>
+__global__ __launch_bounds__(512) void transpose_op_k(const T *__restrict__ so, T *__restrict__ soT,
+                                                     int rows, int cols, int nstncl) {
+    __shared__ T tile[64][65];
+    const int k = (int)blockIdx.z % nstncl, b = (int)blockIdx.z / nstncl;
+    const int from = k == 0 ? KO : k == 1 ? KS : k == 2 ? KW : k;
+    const std::size_t P = (std::size_t)rows * cols;
+    const T *src = so + ((std::size_t)b * nstncl + from) * P;
+    T *dst = soT + ((std::size_t)b * nstncl + k) * P;
+    const int c0 = blockIdx.x * 64, r0 = blockIdx.y * 64;
+    const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+    for (int m = ty; m < 64; m += 8) {
+        const int r = r0 + m, c = c0 + tx;
+        if (r < rows && c < cols) tile[m][tx] = src[(std::size_t)r * cols + c];
+    }
+    __syncthreads();
+    for (int m = ty; m < 64; m += 8) {
+        const int c = c0 + m, r = r0 + tx;
+        if (c < cols && r < rows) dst[(std::size_t)c * rows + r] = tile[tx][m];
+    }
+}
+
 // FP32: 64 x 128 tiles moved as pairs of floats (see float_pair), twice the bytes per instruction
 // of the one-float form (5.1 against 4.6 TB/s on a 4097^2 grid).
 __global__ __launch_bounds__(512) void transpose_pairs_k(const float *__restrict__ src,
@@ -1680,9 +1749,9 @@ __device__ inline void wave_line(const T *__restrict__ so, const T *__restrict__
     T carry = T(0);
 #pragma unroll
     for (int u = 0; u < CH; ++u) {
-        const T y = a[u] * carry + b[u];
+        const T y = affine(a[u], carry, b[u]);
         carry = last_lane(y);
-        b[u] = y / D[min(u * WAVE + lane, n - 1)];
+        b[u] = div_pivot(y, D[min(u * WAVE + lane, n - 1)]);
     }
 #pragma unroll
     for (int u = 0; u < CH; ++u) {
@@ -1698,7 +1767,7 @@ __device__ inline void wave_line(const T *__restrict__ so, const T *__restrict__
     carry = T(0);
 #pragma unroll
     for (int u = CH - 1; u >= 0; --u) {
-        const T x = a[u] * carry + b[u];
+        const T x = affine(a[u], carry, b[u]);
         carry = last_lane(x);
         b[u] = x;
     }
@@ -1909,13 +1978,18 @@ static int launch_shape(int c, const T *so, const T *qf, T *q, const T *sor, int
             CDR_X(1, 8, 4, 1);
             CDR_X(2, 4, 8, 1);
             CDR_X(3, F64 ? 2 : 32, F64 ? 16 : 1, F64 ? 1 : 4);
-            return 4;
+            // two neighbouring lines of a colour per workgroup share the row of q between them
+            CDR_X(4, 8, 4, 2);
+            CDR_X(5, 4, 8, 2);
+            return 6;
         } else {
             CDR_X(0, 16, 4, 1);
             CDR_X(1, 8, 8, 1);
             CDR_X(2, 4, 16, 1);
             CDR_X(3, 64, 1, 4);
-            return 4;
+            CDR_X(4, 8, 8, 2);
+            CDR_X(5, 16, 4, 2);
+            return 6;
         }
     } else {
         // columns per workgroup for 64-byte (N1) and 128-byte (2 N1) row segments
@@ -2074,6 +2148,7 @@ void relax_lines_y(const T *so, int nstncl, const T *qf, T *q, const T *sor, int
 }
 
 int set_autotune(int on) { return g_autotune.exchange(on ? 1 : 0); }
+bool autotune_enabled() { return g_autotune.load() != 0; }
 
 int line_shape_count(int dir, int wbytes, int nstncl, int ii, int jj) {
     const hipStream_t s = nullptr;  // counting launches nothing
@@ -2158,13 +2233,9 @@ int tune_lines(int dir, const T *so, int nstncl, const T *qf, T *q, const T *sor
 
 template <typename T>
 void transpose_op(const T *so, T *soT, int nstncl, int ii, int jj, hipStream_t s, int nb) {
-    const std::size_t P = (std::size_t)ii * jj;
-    // plane of soT <- plane of so: c <- c, w <- s, s <- w, sw <- sw, nw <- nw
-    const int from[5] = {KO, KS, KW, KSW, KNW};
-    for (int k = 0; k < nstncl; ++k)
-        for (int b = 0; b < nb; ++b)
-            transpose<T>(so + (std::size_t)b * nstncl * P + P * from[k],
-                         soT + (std::size_t)b * nstncl * P + P * k, jj, ii, s, 1);
+    // every stencil plane of every problem in one launch (z = problem * nstncl + plane)
+    const dim3 g((unsigned)((ii + 63) / 64), (unsigned)((jj + 63) / 64), (unsigned)(nb * nstncl));
+    transpose_op_k<T><<<g, 512, 0, s>>>(so, soT, jj, ii, nstncl);
 }
 
 template <typename T>

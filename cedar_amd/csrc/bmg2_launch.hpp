@@ -90,6 +90,7 @@ template <typename T>
 int tune_lines(int dir, const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
                hipStream_t s, int nb = 1);
 int set_autotune(int on);
+bool autotune_enabled();
 
 // One level of a fused coarse-level V-cycle: arrays of problem 0 of a batch (problem b at b times
 // each array's size), 9-point operator, relaxation data as the solver stores it (point: sor0

@@ -401,22 +401,46 @@ private:
     }
 
     // The fused coarse-level cycle starts at the first level below the finest whose sides are at
-    // most kFuseSide points and covers it and every coarser level (it needs at least two of
+    // most 32, 64 or 128 points and covers it and every coarser level (it needs at least two of
     // them).  One workgroup per problem is latency-bound on larger levels, where one launch per
-    // step over the whole GPU is faster.
-    static constexpr int kFuseSide = 32;
+    // step over the whole GPU is faster; where the three starting levels differ, the cycle from
+    // the highest of them down is timed with each (captured graphs, replayed) and the fastest is
+    // kept.  The fused cycle is bitwise the unfused one, so the choice never changes a result.
     template <typename T>
     void setup_fused(std::vector<level_data<T>> &L, dbuf<unsigned char> &dst) {
         fuse_lvl_ = -1;
         if (!set_.fused_coarse || jpn_ != 0) return;  // the fused cycle has no periodic images
         const int nl = (int)L.size();
-        int lt = -1;
-        for (int l = 1; l + 1 < nl; ++l)
-            if (std::max(L[l].ext.nx(), L[l].ext.ny()) <= kFuseSide) {
-                lt = l;
-                break;
+        std::vector<int> cand;
+        for (int side : {32, 64, 128}) {
+            for (int l = 1; l + 1 < nl; ++l)
+                if (std::max(L[l].ext.nx(), L[l].ext.ny()) <= side) {
+                    if (std::find(cand.begin(), cand.end(), l) == cand.end()) cand.push_back(l);
+                    break;
+                }
+        }
+        if (cand.empty()) return;
+        int lt = cand[0];
+        if (cand.size() > 1 && k2::autotune_enabled()) {
+            const int top = *std::min_element(cand.begin(), cand.end());
+            float tbest = -1.f;
+            for (int c : cand) {
+                build_fused<T>(L, c, dst);
+                k2::fill<T>(L[top].b.p, T(0), L[top].ext.total(), stream_);
+                const float t = time_graph([&] { ncycle<T>(L, top, L[top].x, L[top].b); });
+                if (t >= 0.f && (tbest < 0.f || t < tbest)) {
+                    tbest = t;
+                    lt = c;
+                }
             }
-        if (lt < 0) return;
+        }
+        build_fused<T>(L, lt, dst);
+    }
+
+    // level descriptors of the fused cycle from level lt down (fuse_lvl_ = lt)
+    template <typename T>
+    void build_fused(std::vector<level_data<T>> &L, int lt, dbuf<unsigned char> &dst) {
+        const int nl = (int)L.size();
         std::vector<k2::coarse_level<T>> v;
         for (int l = lt; l < nl; ++l) {
             const level_data<T> &D = L[l];
@@ -430,6 +454,58 @@ private:
         check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
         fuse_lvl_ = lt;
         fuse_side_ = std::max(L[lt].ext.nx(), L[lt].ext.ny());
+    }
+
+    // Milliseconds per replay of what body() launches, captured as a graph on the capture stream
+    // and replayed 3 times on the solver's stream after a warm-up; -1 if capture fails.
+    template <typename F>
+    float time_graph(F body) {
+        if (!cap_stream_)
+            check_hip(hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking),
+                      "hipStreamCreateWithFlags");
+        check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+        set_stream(cap_stream_);
+        if (hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            set_stream(stream_);
+            (void)hipGetLastError();
+            return -1.f;
+        }
+        hipGraph_t graph = nullptr;
+        try {
+            body();
+        } catch (...) {
+            (void)hipStreamEndCapture(cap_stream_, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            set_stream(stream_);
+            throw;
+        }
+        const hipError_t ec = hipStreamEndCapture(cap_stream_, &graph);
+        set_stream(stream_);
+        if (ec != hipSuccess || !graph) {
+            if (graph) (void)hipGraphDestroy(graph);
+            (void)hipGetLastError();
+            return -1.f;
+        }
+        hipGraphExec_t exec = nullptr;
+        float ms = -1.f;
+        if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+            hipEvent_t a, b;
+            check_hip(hipEventCreate(&a), "hipEventCreate");
+            check_hip(hipEventCreate(&b), "hipEventCreate");
+            check_hip(hipGraphLaunch(exec, stream_), "hipGraphLaunch");
+            check_hip(hipEventRecord(a, stream_), "hipEventRecord");
+            for (int r = 0; r < 3; ++r) check_hip(hipGraphLaunch(exec, stream_), "hipGraphLaunch");
+            check_hip(hipEventRecord(b, stream_), "hipEventRecord");
+            check_hip(hipEventSynchronize(b), "hipEventSynchronize");
+            check_hip(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+            ms /= 3.f;
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            (void)hipGraphExecDestroy(exec);
+        }
+        (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        return ms;
     }
 
     // Picks the line-kernel launch shapes of every smoothing level by timing (the shapes give
