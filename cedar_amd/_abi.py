@@ -81,6 +81,8 @@ PROTOTYPES = {
                                      ctypes.POINTER(_vp)]),
     "cdr_hip_solver2d_create_batched": (_i, [_P, _i, _i, _i, _i, ctypes.POINTER(Settings), _vp,
                                              ctypes.POINTER(_vp)]),
+    "cdr_hip_solver2d_create_shared": (_i, [_P, _i, _i, _i, _i, _i, ctypes.POINTER(Settings),
+                                            _vp, ctypes.POINTER(_vp)]),
     "cdr_hip_solver2d_nbatch": (_i, [_vp]),
     "cdr_hip_solver2d_solve": (_i, [_vp, _P, _P, ctypes.POINTER(_d), _i, ctypes.POINTER(_i)]),
     "cdr_hip_solver2d_vcycle": (_i, [_vp, _P, _P]),

@@ -2,8 +2,10 @@
 
 Every kernel launch of a group's setup and cycle covers all problems of the group (the problem
 index is the last grid dimension), so a set of problems costs one launch per kernel per level per
-size group instead of one per problem.  Each problem stops at its own tolerance and gets exactly
-the result of its individual solve.
+size group instead of one per problem.  With nrhs > 1 every operator is solved for nrhs
+right-hand sides at once: the group's solver builds one hierarchy per operator and its cycle
+kernels read each operator once for its nrhs problems.  Each problem stops at its own tolerance
+and gets exactly the result of its individual solve.
 
 The groups are independent: each solver runs on a stream of its own, and `solve` drives the
 groups from one host thread each (the C calls release the GIL), so the latency-bound coarse levels
@@ -20,16 +22,19 @@ from .cdr2 import Solver
 class BatchSet:
     """Solvers for a list of 2D operators (device tensors (nstncl, ny+2, nx+2), any mix of sizes).
 
-    `solve(rhs)` takes one right-hand side per problem (same order) and returns the solutions and
-    residual histories in that order.  `concurrent=False` solves the groups one after another on
-    the current stream.  Other keyword arguments are those of `Solver`.
+    `solve(rhs)` takes, per operator (same order), one right-hand side (nrhs = 1) or a stack of
+    nrhs of them, and returns the solutions (shaped like the right-hand sides) and the residual
+    histories (one array, or a list of nrhs arrays, per operator).  `concurrent=False` solves the
+    groups one after another on the current stream.  Other keyword arguments are those of
+    `Solver`.
     """
 
-    def __init__(self, operators, concurrent=True, **kw):
+    def __init__(self, operators, concurrent=True, nrhs=1, **kw):
         self.groups = OrderedDict()   # shape -> list of problem indices
         for k, so in enumerate(operators):
             self.groups.setdefault(tuple(so.shape), []).append(k)
         self.n = len(operators)
+        self.nrhs = int(nrhs)
         self.concurrent = concurrent and len(self.groups) > 1
         self.solvers, self.streams = {}, {}
         cur = torch.cuda.current_stream()
@@ -38,7 +43,7 @@ class BatchSet:
             st = torch.cuda.Stream() if self.concurrent else cur  # default priority
             st.wait_stream(cur)
             with torch.cuda.stream(st):
-                self.solvers[shape] = Solver(stacked, **kw)
+                self.solvers[shape] = Solver(stacked, nrhs=self.nrhs, **kw)
             self.streams[shape] = st
         for st in self.streams.values():
             cur.wait_stream(st)
@@ -50,8 +55,10 @@ class BatchSet:
         xs = [None] * self.n
         hs = [None] * self.n
         cur = torch.cuda.current_stream()
-        bs = {shape: torch.stack([rhs[k] for k in idx]).contiguous()
-              for shape, idx in self.groups.items()}
+        m = self.nrhs
+        # problem-major stacks: right-hand side r of operator k at k * nrhs + r
+        bs = {shape: torch.stack([rhs[k] for k in idx]).reshape((len(idx) * m,) + shape[-2:])
+              .contiguous() for shape, idx in self.groups.items()}
         out, errors = {}, []
 
         def run(shape):
@@ -79,6 +86,10 @@ class BatchSet:
         for shape, idx in self.groups.items():
             x, h = out[shape]
             for t, k in enumerate(idx):
-                xs[k] = x[t]
-                hs[k] = h[t]
+                if m == 1:
+                    xs[k] = x[t]
+                    hs[k] = h[t]
+                else:
+                    xs[k] = x[t * m:(t + 1) * m]
+                    hs[k] = h[t * m:(t + 1) * m]
         return xs, hs

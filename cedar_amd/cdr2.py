@@ -302,19 +302,23 @@ class Solver:
     A 4-D operator (nb, nstncl, ny+2, nx+2) makes a batched solver: nb independent problems of
     the same extents solved together (every kernel launch covers the batch); solve() then takes
     and returns (nb, ny+2, nx+2) grids and a list of nb residual histories, each the history of
-    solving that problem alone.
+    solving that problem alone.  nrhs > 1 solves every operator for nrhs right-hand sides at once
+    (cdr_hip_solver2d_create_shared): the grids then hold nops * nrhs problems, right-hand side r
+    of operator k at index k * nrhs + r, and the operators are read once per group.
     """
 
     def __init__(self, so, relaxation="point", nrelax_pre=2, nrelax_post=1, num_levels=-1,
                  maxiter=10, tol=1e-8, min_coarse=3, precision="f64", graph=True, cycle="v",
-                 fused_coarse=True, periodic=(False, False), stall_cycles=0):
+                 fused_coarse=True, periodic=(False, False), stall_cycles=0, nrhs=1):
         so = _t(so) if not isinstance(so, np.ndarray) else torch.as_tensor(so).cuda()
         if so.dtype != torch.float64:
             so = so.to(torch.float64)
         self.so = so.contiguous()  # held by reference for the solver's lifetime
         self.maxiter = maxiter
-        self.batched = self.so.dim() == 4
-        nb = self.so.shape[0] if self.batched else 1
+        self.batched = self.so.dim() == 4 or nrhs > 1
+        nops = self.so.shape[0] if self.so.dim() == 4 else 1
+        self.nops, self.nrhs = nops, int(nrhs)
+        nb = nops * self.nrhs
         self.nb = nb
         ns, jj, ii = self.so.shape[-3:]
         s = _abi.Settings(_abi.RELAX[relaxation], nrelax_pre, nrelax_post, num_levels, maxiter,
@@ -323,8 +327,8 @@ class Solver:
                           int(bool(periodic[0])) | (int(bool(periodic[1])) << 1),
                           int(stall_cycles))
         h = ctypes.c_void_p()
-        _abi.call("cdr_hip_solver2d_create_batched", _p(self.so), ns, ii - 2, jj - 2, nb,
-                  ctypes.byref(s), _stream(), ctypes.byref(h))
+        _abi.call("cdr_hip_solver2d_create_shared", _p(self.so), ns, ii - 2, jj - 2, nops,
+                  self.nrhs, ctypes.byref(s), _stream(), ctypes.byref(h))
         self._h = h
 
     @property
@@ -344,7 +348,7 @@ class Solver:
         nx, ny, so, ci, s0, s1 = self.level(lvl)
         ii, jj = nx + 2, ny + 2
         ns = self.so.shape[-3] if lvl == 0 else 5
-        lead = (self.nb,) if self.batched else ()
+        lead = (self.nops,) if self.batched else ()
 
         def grab(ptr, planes):
             if not ptr:
@@ -360,8 +364,16 @@ class Solver:
         _abi.call("cdr_hip_solver2d_times", self._h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
 
+    def _check_grids(self, *gs):
+        jj, ii = self.so.shape[-2:]
+        for g in gs:
+            if g.numel() != self.nb * jj * ii or g.dtype != torch.float64:
+                raise ValueError(f"expected {self.nb} float64 grids of {jj} x {ii}, got "
+                                 f"{tuple(g.shape)} {g.dtype}")
+
     def vcycle(self, x, b):
         x, b = _t(x), _t(b)
+        self._check_grids(x, b)
         _abi.call("cdr_hip_solver2d_vcycle", self._h, _p(b), _p(x))
         return x
 
@@ -371,6 +383,7 @@ class Solver:
         if x is None:
             x = torch.zeros_like(b)
         x = _t(x)
+        self._check_grids(x, b)
         m = max(self.maxiter, 1)
         hist = (ctypes.c_double * (m * self.nb))()
         nit = (ctypes.c_int * self.nb)()

@@ -113,11 +113,12 @@ __device__ inline T apply_offdiag(const T *__restrict__ so, std::size_t P, const
 template <typename T, bool NINE, bool NORM>
 __global__ __launch_bounds__(256) void residual_k(const T *__restrict__ so, const T *__restrict__ qf,
                                                   const T *__restrict__ q, T *__restrict__ res,
-                                                  int ii, int jj, double *__restrict__ partials) {
+                                                  int ii, int jj, double *__restrict__ partials,
+                                                  int rep) {
     const int i = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
     const int j = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
     const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.z;
-    so += bz * (NINE ? 5 : 3) * P;
+    so += (bz / rep) * (NINE ? 5 : 3) * P;
     qf += bz * P;
     q += bz * P;
     res += bz * P;
@@ -160,14 +161,16 @@ __global__ __launch_bounds__(64 * IR_ROWS) void residual_ir_k(const double *__re
                                                      double *__restrict__ xnew,
                                                      float *__restrict__ r32, int ii, int jj,
                                                      double *__restrict__ partials,
-                                                     const int *__restrict__ active) {
+                                                     const int *__restrict__ active, int rep,
+                                                     float *__restrict__ r32T) {
     // the updated iterate of the block and its one-point halo, formed once per point
     __shared__ double xs[IR_ROWS + 2][66];
+    __shared__ float rt[IR_ROWS][65];  // r32 of the block, for the transposed copy
     const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
     const int i0 = blockIdx.x * 64, j0 = blockIdx.y * IR_ROWS;  // halo corner of the block
     const int i = i0 + tx + 1, j = j0 + ty + 1;
     const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.z;
-    so += bz * (NINE ? 5 : 3) * P;
+    so += (bz / rep) * (NINE ? 5 : 3) * P;
     qf += bz * P;
     x += bz * P;
     if (HAS_E) e += bz * P;
@@ -205,9 +208,18 @@ __global__ __launch_bounds__(64 * IR_ROWS) void residual_ir_k(const double *__re
         r = r - so[p] * xc;
         xnew[p] = xc;
         r32[p] = (float)r;
+        rt[ty][tx] = (float)r;
         sq = r * r;
     }
     block_sum_store<IR_ROWS>(sq, &partials[blockIdx.y * gridDim.x + blockIdx.x]);
+    if (r32T) {  // uniform over the launch; block_sum_store ends with a workgroup barrier
+        r32T += bz * P;
+        __syncthreads();
+        // consecutive threads write consecutive j of one row of the transposed grid
+        const int a = threadIdx.x / IR_ROWS, c = threadIdx.x % IR_ROWS;
+        const int i2 = i0 + a + 1, j2 = j0 + c + 1;
+        if (i2 <= ii - 2 && j2 <= jj - 2) r32T[(std::size_t)i2 * jj + j2] = rt[c][a];
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -232,7 +244,8 @@ template <typename T, bool NINE>
 // the memory-side cache); points of one colour are independent, so the result is unchanged
 __global__ __launch_bounds__(256) void gs_colour_k(const T *__restrict__ so, const T *__restrict__ qf,
                                                    T *__restrict__ q, const T *__restrict__ sor,
-                                                   int ii, int jj, int pi, int pj, int rev = 0) {
+                                                   int ii, int jj, int pi, int pj, int rep,
+                                                   int rev = 0) {
     const int bx = rev ? (int)(gridDim.x - 1 - blockIdx.x) : (int)blockIdx.x;
     const int by = rev ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.y;
     const int k = bx * 64 + (threadIdx.x % 64);
@@ -246,11 +259,11 @@ __global__ __launch_bounds__(256) void gs_colour_k(const T *__restrict__ so, con
         i = 2 * k + ((pi + j) & 1);  // (i + j) % 2 == pi
     }
     if (i < 1 || i > ii - 2 || j < 1 || j > jj - 2) return;
-    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.z;
-    so += bz * (NINE ? 5 : 3) * P;
+    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.z, bo = bz / rep;
+    so += bo * (NINE ? 5 : 3) * P;
     qf += bz * P;
     q += bz * P;
-    sor += bz * 2 * P;
+    sor += bo * 2 * P;
     const std::size_t p = pidx(i, j, ii);
     T v = apply_offdiag<T, NINE>(so, P, q, p, ii, qf[p]);
     q[p] = v * sor[P + p];
@@ -479,15 +492,15 @@ __device__ inline double last_lane(double v) {
 template <typename T, bool NINE, int DIR>
 __global__ __launch_bounds__(256) void relax_lines_k(const T *__restrict__ so, const T *__restrict__ qf,
                                                      T *__restrict__ q, const T *__restrict__ sor,
-                                                     int ii, int jj, int first, int nlines) {
+                                                     int ii, int jj, int first, int nlines, int rep) {
     const int lane = threadIdx.x % WAVE;
     const int line = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     if (line >= nlines) return;  // whole wavefront leaves together; no block barrier below
-    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.y;
-    so += bz * (NINE ? 5 : 3) * P;
+    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.y, bo = bz / rep;
+    so += bo * (NINE ? 5 : 3) * P;
     qf += bz * P;
     q += bz * P;
-    sor += bz * 2 * P;
+    sor += bo * 2 * P;
     const int fixed = first + 2 * line;
     const int n = (DIR == 0) ? ii - 2 : jj - 2;
     const std::size_t stride = (DIR == 0) ? 1 : (std::size_t)ii;
@@ -766,7 +779,7 @@ __global__ __launch_bounds__(L *W * 64) void relax_xlines_seg_k(const T *__restr
                                                                T *__restrict__ q,
                                                                const T *__restrict__ sor, int ii,
                                                                int jj, int first, int nlines,
-                                                               int ntiles, int rev) {
+                                                               int ntiles, int rev, int rep) {
     __shared__ seg_map<T> xch[2][L][W > 1 ? W * CPW : 1];
     const int tile = xcd_tile(ntiles, rev);
     if (tile >= ntiles) return;  // uniform over the workgroup
@@ -775,11 +788,11 @@ __global__ __launch_bounds__(L *W * 64) void relax_xlines_seg_k(const T *__restr
     const int line = tile * L + slot;
     const bool valid = line < nlines;  // invalid wavefronts solve a copy of the last line
     const int j = first + 2 * min(line, nlines - 1);
-    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.y;
-    so += bz * (NINE ? 5 : 3) * P;
+    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.y, bo = bz / rep;
+    so += bo * (NINE ? 5 : 3) * P;
     qf += bz * P;
     q += bz * P;
-    sor += bz * 2 * P;
+    sor += bo * 2 * P;
     const int n = ii - 2;
     const std::size_t row = (std::size_t)j * ii;
     const T *D = sor + row + 1;      // D[k]
@@ -857,7 +870,8 @@ __global__ __launch_bounds__(NC *W * 64) void relax_ylines_seg_k(const T *__rest
                                                                 T *__restrict__ q,
                                                                 const T *__restrict__ sor,
                                                                 int ii, int jj, int first,
-                                                                int ncols, int ntiles, int rev) {
+                                                                int ncols, int ntiles, int rev,
+                                                                int rep) {
     static_assert(CPW % G == 0, "G must divide CPW");
     __shared__ T tile_s[G][W][NC][WAVE + 1];
     __shared__ seg_map<T> xch[2][NC][W > 1 ? W * CPW : 1];
@@ -865,11 +879,11 @@ __global__ __launch_bounds__(NC *W * 64) void relax_ylines_seg_k(const T *__rest
     if (tile >= ntiles) return;  // uniform over the workgroup
     const int wl = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int cs = wl / W, w = wl % W;
-    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.y;
-    so += bz * (NINE ? 5 : 3) * P;
+    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.y, bo = bz / rep;
+    so += bo * (NINE ? 5 : 3) * P;
     qf += bz * P;
     q += bz * P;
-    sor += bz * 2 * P;
+    sor += bo * 2 * P;
     const int n = jj - 2;
     const int col0 = tile * NC;
     const int nc = min(NC, ncols - col0);
@@ -971,15 +985,57 @@ __device__ inline T restrict_point(const T *__restrict__ ci, const T *__restrict
 template <typename T>
 __global__ __launch_bounds__(256) void restrict_k(const T *__restrict__ ci, const T *__restrict__ q,
                                                   T *__restrict__ qc, int iif, int jjf, int iic,
-                                                  int jjc) {
+                                                  int jjc, int rep) {
     const int ic = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
     const int jc = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
     if (ic > iic - 2 || jc > jjc - 2) return;
     const std::size_t Pc = (std::size_t)iic * jjc, bz = blockIdx.z;
-    ci += bz * 8 * Pc;
+    ci += (bz / rep) * 8 * Pc;
     q += bz * iif * (std::size_t)jjf;
     qc += bz * Pc;
     qc[pidx(ic, jc, iic)] = restrict_point<T>(ci, q, iif, iic, Pc, ic, jc);
+}
+
+// Restriction for a V-cycle step: qc = P^T q as restrict_k (same operations), and in the same
+// launch the coarse iterate xc is zeroed (ghosts included: the coarse cycle starts from 0) and,
+// where qcT is given, qc is also written transposed (qcT[ic * jjc + jc], for a coarse level whose
+// y sweeps run through the transposed grid).  Tiles of 64 (ic) x 16 (jc) coarse points over the
+// ghost-inclusive coarse grid, 256 threads.
+template <typename T>
+__global__ __launch_bounds__(256) void restrict_cycle_k(const T *__restrict__ ci,
+                                                        const T *__restrict__ q,
+                                                        T *__restrict__ qc, T *__restrict__ xc,
+                                                        T *__restrict__ qcT, int iif, int jjf,
+                                                        int iic, int jjc, int rep) {
+    __shared__ T tile[16][65];
+    const int ic0 = blockIdx.x * 64, jc0 = blockIdx.y * 16;
+    const std::size_t Pc = (std::size_t)iic * jjc, bz = blockIdx.z;
+    ci += (bz / rep) * 8 * Pc;
+    q += bz * iif * (std::size_t)jjf;
+    qc += bz * Pc;
+    xc += bz * Pc;
+    const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+    const int ic = ic0 + tx;
+    for (int k = ty; k < 16; k += 4) {
+        const int jc = jc0 + k;
+        if (ic >= iic || jc >= jjc) continue;
+        const std::size_t pc = pidx(ic, jc, iic);
+        xc[pc] = T(0);
+        if (ic >= 1 && ic <= iic - 2 && jc >= 1 && jc <= jjc - 2) {
+            const T v = restrict_point<T>(ci, q, iif, iic, Pc, ic, jc);
+            qc[pc] = v;
+            tile[k][tx] = v;
+        }
+    }
+    if (!qcT) return;  // uniform over the launch
+    qcT += bz * Pc;
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * 16; e += 256) {
+        const int a = e / 16, c = e % 16;  // a: ic offset, c: jc offset (contiguous in qcT)
+        const int i2 = ic0 + a, j2 = jc0 + c;
+        if (i2 >= 1 && i2 <= iic - 2 && j2 >= 1 && j2 <= jjc - 2)
+            qcT[(std::size_t)i2 * jjc + j2] = tile[c][a];
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1098,11 +1154,12 @@ __global__ __launch_bounds__(256) void interp_add_k(T *__restrict__ q, const T *
                                                     const T *__restrict__ res,
                                                     const T *__restrict__ so,
                                                     const T *__restrict__ ci, int iic, int jjc,
-                                                    int iif, int jjf, int ns) {
+                                                    int iif, int jjf, int ns, int rep) {
     const int m = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
     const int n = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
     const std::size_t Pc = (std::size_t)iic * jjc, Pf = (std::size_t)iif * jjf, bz = blockIdx.z;
-    interp_block<T>(q + bz * Pf, qc + bz * Pc, res + bz * Pf, so + bz * ns * Pf, ci + bz * 8 * Pc,
+    const std::size_t bo = bz / rep;
+    interp_block<T>(q + bz * Pf, qc + bz * Pc, res + bz * Pf, so + bo * ns * Pf, ci + bo * 8 * Pc,
                     iic, jjc, iif, jjf, m, n);
 }
 
@@ -1339,12 +1396,12 @@ __global__ __launch_bounds__(256) void setup_cg_k(const double *__restrict__ so,
 
 template <typename T>
 __global__ void solve_cg_k(T *__restrict__ q, const T *__restrict__ qf, const double *__restrict__ abd,
-                           double *__restrict__ x, int ii, int jj) {
+                           double *__restrict__ x, int ii, int jj, int rep) {
     if (threadIdx.x != 0) return;
     const int nx = ii - 2, ny = jj - 2, n = nx * ny, kd = ii - 1, ld = ii;
     q += (std::size_t)blockIdx.x * ii * jj;  // one workgroup per problem of a batch
     qf += (std::size_t)blockIdx.x * ii * jj;
-    abd += (std::size_t)blockIdx.x * ld * n;
+    abd += (std::size_t)(blockIdx.x / rep) * ld * n;
     x += (std::size_t)blockIdx.x * n;
     for (int kk = 0; kk < n; ++kk) x[kk] = (double)qf[pidx(kk % nx + 1, kk / nx + 1, ii)];
     band_solve_upper(abd, n, kd, ld, x);
@@ -1413,22 +1470,22 @@ static dim3 gridb(int nx, int ny, int nb) {
 
 template <typename T>
 void residual(const T *so, int nstncl, const T *qf, const T *q, T *res, int ii, int jj,
-              hipStream_t s, int nb) {
+              hipStream_t s, int nb, int rep) {
     const dim3 g = gridb(ii - 2, jj - 2, nb);
     if (nstncl == 5)
-        residual_k<T, true, false><<<g, 256, 0, s>>>(so, qf, q, res, ii, jj, nullptr);
+        residual_k<T, true, false><<<g, 256, 0, s>>>(so, qf, q, res, ii, jj, nullptr, rep);
     else
-        residual_k<T, false, false><<<g, 256, 0, s>>>(so, qf, q, res, ii, jj, nullptr);
+        residual_k<T, false, false><<<g, 256, 0, s>>>(so, qf, q, res, ii, jj, nullptr, rep);
 }
 
 template <typename T>
 void residual_norm(const T *so, int nstncl, const T *qf, const T *q, T *res, int ii, int jj,
-                   double *partials, double *sumsq_out, hipStream_t s, int nb) {
+                   double *partials, double *sumsq_out, hipStream_t s, int nb, int rep) {
     const dim3 g = gridb(ii - 2, jj - 2, nb);
     if (nstncl == 5)
-        residual_k<T, true, true><<<g, 256, 0, s>>>(so, qf, q, res, ii, jj, partials);
+        residual_k<T, true, true><<<g, 256, 0, s>>>(so, qf, q, res, ii, jj, partials, rep);
     else
-        residual_k<T, false, true><<<g, 256, 0, s>>>(so, qf, q, res, ii, jj, partials);
+        residual_k<T, false, true><<<g, 256, 0, s>>>(so, qf, q, res, ii, jj, partials, rep);
     finalize_sum_k<<<nb, 1024, 0, s>>>(partials, (std::size_t)g.x * g.y, sumsq_out);
 }
 
@@ -1441,24 +1498,24 @@ void sumsq(const T *x, int ii, int jj, double *partials, double *out, hipStream_
 
 void residual_ir(const double *so, int nstncl, const double *qf, const double *x, const float *e,
                  double *xnew, float *r32, int ii, int jj, double *partials, double *sumsq_out,
-                 hipStream_t s, int nb, const int *active) {
+                 hipStream_t s, int nb, const int *active, int rep, float *r32T) {
     const dim3 g((unsigned)((ii - 2 + 63) / 64), (unsigned)((jj - 2 + IR_ROWS - 1) / IR_ROWS),
                  (unsigned)nb);
     const unsigned nt = 64 * IR_ROWS;
     if (nstncl == 5) {
         if (e)
             residual_ir_k<true, true><<<g, nt, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, partials,
-                                                       active);
+                                                       active, rep, r32T);
         else
             residual_ir_k<true, false><<<g, nt, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, partials,
-                                                        active);
+                                                        active, rep, r32T);
     } else {
         if (e)
             residual_ir_k<false, true><<<g, nt, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, partials,
-                                                        active);
+                                                        active, rep, r32T);
         else
             residual_ir_k<false, false><<<g, nt, 0, s>>>(so, qf, x, e, xnew, r32, ii, jj, partials,
-                                                         active);
+                                                         active, rep, r32T);
     }
     finalize_sum_k<<<nb, 1024, 0, s>>>(partials, (std::size_t)g.x * g.y, sumsq_out);
 }
@@ -1470,7 +1527,7 @@ void setup_recip(const double *so, int nstncl, double *sor, int ii, int jj, hipS
 
 template <typename T>
 void relax_gs(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
-              int updown, hipStream_t s, int nb) {
+              int updown, hipStream_t s, int nb, int rep) {
     if (nstncl == 5) {
         // DOWN: (i odd, j odd), (even, odd), (odd, even), (even, even); UP: reverse
         static const int order[4][2] = {{1, 1}, {0, 1}, {1, 0}, {0, 0}};
@@ -1478,14 +1535,14 @@ void relax_gs(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, 
         for (int t = 0; t < 4; ++t) {
             const int c = updown == 0 ? t : 3 - t;
             gs_colour_k<T, true><<<g, 256, 0, s>>>(so, qf, q, sor, ii, jj, order[c][0], order[c][1],
-                                                   t & 1);
+                                                   rep, t & 1);
         }
     } else {
         // DOWN: (i + j) even first; UP: odd first
         const dim3 g = gridb(ii / 2 + 1, jj - 2, nb);
         for (int t = 0; t < 2; ++t) {
             const int parity = updown == 0 ? t : 1 - t;
-            gs_colour_k<T, false><<<g, 256, 0, s>>>(so, qf, q, sor, ii, jj, parity, 0, t & 1);
+            gs_colour_k<T, false><<<g, 256, 0, s>>>(so, qf, q, sor, ii, jj, parity, 0, rep, t & 1);
         }
     }
 }
@@ -1603,11 +1660,11 @@ template <typename T, bool NINE>
 __global__ __launch_bounds__(256) void residual_qT_k(const T *__restrict__ so,
                                                      const T *__restrict__ qf,
                                                      const T *__restrict__ qT,
-                                                     T *__restrict__ res, int ii, int jj) {
+                                                     T *__restrict__ res, int ii, int jj, int rep) {
     __shared__ T sq[kTJ + 2][kTI + 3];  // q(i0-1 .. i0+64, j0-1 .. j0+32) as [j][i]
     const int i0 = blockIdx.x * kTI + 1, j0 = blockIdx.y * kTJ + 1;
     const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.z;
-    so += bz * (NINE ? 5 : 3) * P;
+    so += (bz / rep) * (NINE ? 5 : 3) * P;
     qf += bz * P;
     qT += bz * P;
     res += bz * P;
@@ -1649,7 +1706,7 @@ __global__ __launch_bounds__(512) void interp_add_qT_k(T *__restrict__ qT, const
                                                        const T *__restrict__ res,
                                                        const T *__restrict__ so,
                                                        const T *__restrict__ ci, int iic, int jjc,
-                                                       int iif, int jjf, int ns) {
+                                                       int iif, int jjf, int ns, int rep) {
     __shared__ T t1[32][65], t2[32][65];  // [j - jt][i - it]
     const int m0 = blockIdx.x * 32 + 1, n0 = blockIdx.y * 16 + 1;
     const int it = 2 * m0 - 1, jt = 2 * n0 - 1;  // first fine point of the tile
@@ -1657,8 +1714,8 @@ __global__ __launch_bounds__(512) void interp_add_qT_k(T *__restrict__ qT, const
     qT += bz * Pf;
     qc += bz * Pc;
     res += bz * Pf;
-    so += bz * ns * Pf;
-    ci += bz * 8 * Pc;
+    so += (bz / rep) * ns * Pf;
+    ci += (bz / rep) * 8 * Pc;
     const int imax = iif - 2 + ((iif - 2) % 2 == 0 ? 1 : 0);
     const int jmax = jjf - 2 + ((jjf - 2) % 2 == 0 ? 1 : 0);
     {
@@ -1778,26 +1835,75 @@ __device__ inline void wave_line(const T *__restrict__ so, const T *__restrict__
     }
 }
 
+// LDS bytes the fused cycle needs to hold every array of its levels on chip (per level: the
+// operator, both relaxation factors, x, b, res and, below the first level, the interpolation),
+// each array rounded up to 16 bytes
+template <typename T>
+__host__ __device__ inline std::size_t fused_level_bytes(int ii, int jj, bool has_ci) {
+    const std::size_t plane = ((std::size_t)ii * jj * sizeof(T) + 15) / 16 * 16;
+    return plane * (5 + 2 + 2 + 3 + (has_ci ? 8 : 0));
+}
+
 template <typename T, int CH>
 __global__ __launch_bounds__(1024) void coarse_cycle_k(const coarse_level<T> *__restrict__ lv,
                                                        int nlev, int relax, int npre, int npost,
                                                        const double *__restrict__ abd,
-                                                       double *__restrict__ bbd) {
+                                                       double *__restrict__ bbd, int rep,
+                                                       int on_chip) {
     constexpr int NT = 1024, NWAVE = NT / WAVE;
+    extern __shared__ __align__(16) unsigned char lds[];
     const int tid = threadIdx.x, wave = tid / WAVE;
-    const std::size_t bz = blockIdx.x;
-    auto level = [&](int l) {  // the level's arrays of problem bz
+    const std::size_t bz = blockIdx.x, bo = bz / rep;
+    auto level_global = [&](int l) {  // the level's arrays of problem bz (operator bo)
         coarse_level<T> d = lv[l];
         const std::size_t P = (std::size_t)d.ii * d.jj;
-        d.so += bz * 5 * P;
+        d.so += bo * 5 * P;
         d.x += bz * P;
         d.b += bz * P;
         d.res += bz * P;
-        d.sor0 += bz * 2 * P;
-        d.sor1 += bz * 2 * P;
-        if (d.ci) d.ci += bz * 8 * P;
+        d.sor0 += bo * 2 * P;
+        d.sor1 += bo * 2 * P;
+        if (d.ci) d.ci += bo * 8 * P;
         return d;
     };
+    // on_chip: every array of every level lives in LDS for the whole cycle (the launcher checked
+    // that they fit); the first level's b comes in from memory and its x goes back at the end,
+    // the only values the caller exchanges with the fused levels
+    auto level_lds = [&](int l) {
+        std::size_t off = 0;
+        for (int m = 0; m < l; ++m) off += fused_level_bytes<T>(lv[m].ii, lv[m].jj, m > 0);
+        coarse_level<T> d = lv[l];
+        const std::size_t plane = ((std::size_t)d.ii * d.jj * sizeof(T) + 15) / 16 * 16;
+        unsigned char *base = lds + off;
+        d.so = reinterpret_cast<T *>(base);
+        d.sor0 = reinterpret_cast<T *>(base + 5 * plane);
+        d.sor1 = reinterpret_cast<T *>(base + 7 * plane);
+        d.x = reinterpret_cast<T *>(base + 9 * plane);
+        d.b = reinterpret_cast<T *>(base + 10 * plane);
+        d.res = reinterpret_cast<T *>(base + 11 * plane);
+        d.ci = l > 0 ? reinterpret_cast<T *>(base + 12 * plane) : nullptr;
+        return d;
+    };
+    auto level = [&](int l) { return on_chip ? level_lds(l) : level_global(l); };
+    if (on_chip) {
+        for (int l = 0; l < nlev; ++l) {
+            const coarse_level<T> g = level_global(l), d = level_lds(l);
+            const std::size_t P = (std::size_t)g.ii * g.jj;
+            for (std::size_t k = tid; k < 5 * P; k += NT) const_cast<T *>(d.so)[k] = g.so[k];
+            for (std::size_t k = tid; k < 2 * P; k += NT) {
+                const_cast<T *>(d.sor0)[k] = g.sor0[k];
+                const_cast<T *>(d.sor1)[k] = g.sor1[k];
+            }
+            if (l > 0)
+                for (std::size_t k = tid; k < 8 * P; k += NT) const_cast<T *>(d.ci)[k] = g.ci[k];
+            if (l == 0)
+                for (std::size_t k = tid; k < P; k += NT) {
+                    d.b[k] = g.b[k];
+                    d.x[k] = g.x[k];
+                }
+        }
+        __syncthreads();
+    }
     auto lines = [&](const coarse_level<T> &d, const T *sor, int dir, int updown) {
         const int nfix = dir == 0 ? d.jj - 2 : d.ii - 2;
         for (int t = 0; t < 2; ++t) {
@@ -1877,7 +1983,7 @@ __global__ __launch_bounds__(1024) void coarse_cycle_k(const coarse_level<T> *__
             const int nx = C.ii - 2, n = nx * (C.jj - 2), ld = C.ii;
             double *xs = bbd + bz * n;
             for (int kk = 0; kk < n; ++kk) xs[kk] = (double)C.b[pidx(kk % nx + 1, kk / nx + 1, C.ii)];
-            band_solve_upper(abd + bz * (std::size_t)ld * n, n, C.ii - 1, ld, xs);
+            band_solve_upper(abd + bo * (std::size_t)ld * n, n, C.ii - 1, ld, xs);
             for (int kk = 0; kk < n; ++kk) C.x[pidx(kk % nx + 1, kk / nx + 1, C.ii)] = (T)xs[kk];
         }
         __syncthreads();
@@ -1892,6 +1998,11 @@ __global__ __launch_bounds__(1024) void coarse_cycle_k(const coarse_level<T> *__
         __syncthreads();
         smooth(F, 1);
     }
+    if (on_chip) {  // the smoothing ends with a workgroup barrier
+        const coarse_level<T> g = level_global(0), d = level_lds(0);
+        const std::size_t P = (std::size_t)g.ii * g.jj;
+        for (std::size_t k = tid; k < P; k += NT) g.x[k] = d.x[k];
+    }
 }
 
 // ---- line kernel shapes and their autotuner ---------------------------------------------------
@@ -1904,20 +2015,20 @@ __global__ __launch_bounds__(1024) void coarse_cycle_k(const coarse_level<T> *__
 
 template <typename T, bool NINE, int CPW, int W, int L, bool TR>
 static void launch_x(const T *so, const T *qf, T *q, const T *sor, int ii, int jj, int first,
-                     int nlines, hipStream_t s, int nb, int rev) {
+                     int nlines, hipStream_t s, int nb, int rev, int rep) {
     const int ntiles = (nlines + L - 1) / L;
     const dim3 blocks((unsigned)(8 * ((ntiles + 7) / 8)), (unsigned)nb);
     relax_xlines_seg_k<T, NINE, CPW, W, L, TR><<<blocks, L * W * 64, 0, s>>>(
-        so, qf, q, sor, ii, jj, first, nlines, ntiles, rev);
+        so, qf, q, sor, ii, jj, first, nlines, ntiles, rev, rep);
 }
 
 template <typename T, bool NINE, int CPW, int W, int NC, int G>
 static void launch_y(const T *so, const T *qf, T *q, const T *sor, int ii, int jj, int first,
-                     int nlines, hipStream_t s, int nb, int rev) {
+                     int nlines, hipStream_t s, int nb, int rev, int rep) {
     const int ntiles = (nlines + NC - 1) / NC;
     const dim3 blocks((unsigned)(8 * ((ntiles + 7) / 8)), (unsigned)nb);
     relax_ylines_seg_k<T, NINE, CPW, W, NC, G><<<blocks, NC * W * 64, 0, s>>>(
-        so, qf, q, sor, ii, jj, first, nlines, ntiles, rev);
+        so, qf, q, sor, ii, jj, first, nlines, ntiles, rev, rep);
 }
 
 // Launches candidate c of chunk class NCH (lines of at most 64*NCH points); returns the number
@@ -1936,7 +2047,7 @@ static std::string shape_text(const char *k, int cpw, int w, const char *tn, int
 template <typename T, bool NINE, int DIR, int NCH>
 static int launch_shape(int c, const T *so, const T *qf, T *q, const T *sor, int ii, int jj,
                         int first, int nlines, hipStream_t s, int nb, std::string *name = nullptr,
-                        int rev = 0) {
+                        int rev = 0, int rep = 1) {
     constexpr bool TR = DIR == 2;
     constexpr bool F64 = sizeof(T) == 8;
 #define CDR_X(k, C, W, L)                                                                       \
@@ -1944,14 +2055,16 @@ static int launch_shape(int c, const T *so, const T *qf, T *q, const T *sor, int
         if (name)                                                                               \
             *name = shape_text("relax_xlines_seg_k", C, W, "L", L, 0);                          \
         else                                                                                    \
-            launch_x<T, NINE, C, W, L, TR>(so, qf, q, sor, ii, jj, first, nlines, s, nb, rev);  \
+            launch_x<T, NINE, C, W, L, TR>(so, qf, q, sor, ii, jj, first, nlines, s, nb, rev,   \
+                                           rep);                                                \
     }
 #define CDR_Y(k, C, W, NC, G)                                                                   \
     if (c == k) {                                                                               \
         if (name)                                                                               \
             *name = shape_text("relax_ylines_seg_k", C, W, "NC", NC, G);                        \
         else                                                                                    \
-            launch_y<T, NINE, C, W, NC, G>(so, qf, q, sor, ii, jj, first, nlines, s, nb, rev);  \
+            launch_y<T, NINE, C, W, NC, G>(so, qf, q, sor, ii, jj, first, nlines, s, nb, rev,   \
+                                           rep);                                                \
     }
     // x: (CPW, W, L); y: (CPW, W, NC, G).  At most 16 wavefronts per workgroup.
     if constexpr (DIR != 1) {
@@ -1966,13 +2079,15 @@ static int launch_shape(int c, const T *so, const T *qf, T *q, const T *sor, int
             CDR_X(0, 8, 1, 4);
             CDR_X(1, 4, 2, 2);
             CDR_X(2, 2, 4, 1);
-            return 3;
+            CDR_X(3, 1, 8, 1);  // one chunk per wavefront: the shortest chain per line
+            return 4;
         } else if constexpr (NCH == 16) {
             CDR_X(0, 16, 1, 4);
             CDR_X(1, 8, 2, 2);
             CDR_X(2, 4, 4, 1);
             CDR_X(3, 2, 8, 1);
-            return 4;
+            CDR_X(4, 1, 16, 1);
+            return 5;
         } else if constexpr (NCH == 32) {
             CDR_X(0, 16, 2, 2);
             CDR_X(1, 8, 4, 1);
@@ -1994,21 +2109,28 @@ static int launch_shape(int c, const T *so, const T *qf, T *q, const T *sor, int
     } else {
         // columns per workgroup for 64-byte (N1) and 128-byte (2 N1) row segments
         constexpr int N1 = F64 ? 4 : 8;
+        // small levels: fewer columns per workgroup put more workgroups on the GPU (their rows
+        // come from the caches at these sizes)
         if constexpr (NCH <= 2) {
             CDR_Y(0, NCH, 1, N1, NCH);
             CDR_Y(1, NCH, 1, 2 * N1, NCH);
-            return 2;
+            CDR_Y(2, NCH, 1, N1 / 4, NCH);
+            return 3;
         } else if constexpr (NCH == 4) {
             CDR_Y(0, 4, 1, N1, 4);
             CDR_Y(1, 4, 1, 2 * N1, 2);
             CDR_Y(2, 1, 4, N1 / 2, 1);
-            return 3;
+            CDR_Y(3, 4, 1, N1 / 4, 4);
+            CDR_Y(4, 2, 2, N1 / 4, 2);
+            return 5;
         } else if constexpr (NCH == 8) {
             CDR_Y(0, 8, 1, N1, 4);
             CDR_Y(1, 8, 1, 2 * N1, 2);
             CDR_Y(2, 4, 2, N1, 4);
             CDR_Y(3, 2, 4, N1 / 2, 2);
-            return 4;
+            CDR_Y(4, 8, 1, N1 / 4, 4);
+            CDR_Y(5, 4, 2, N1 / 4, 4);
+            return 6;
         } else if constexpr (NCH == 16) {
             CDR_Y(0, 16, 1, N1, 4);
             CDR_Y(1, 8, 2, N1, 4);
@@ -2044,12 +2166,12 @@ static int chunk_class(int n) {
 template <typename T, bool NINE, int DIR>
 static int launch_class(int c, const T *so, const T *qf, T *q, const T *sor, int ii, int jj,
                         int first, int nlines, hipStream_t s, int nb, std::string *name = nullptr,
-                        int rev = 0) {
+                        int rev = 0, int rep = 1) {
     const int n = (DIR != 1) ? ii - 2 : jj - 2;
 #define CDR_C(K)                                                                                  \
     case K:                                                                                       \
         return launch_shape<T, NINE, DIR, K>(c, so, qf, q, sor, ii, jj, first, nlines, s, nb, name, \
-                                             rev)
+                                             rev, rep)
     switch (chunk_class(n)) {
         CDR_C(1);
         CDR_C(2);
@@ -2065,7 +2187,7 @@ static int launch_class(int c, const T *so, const T *qf, T *q, const T *sor, int
             if (c == 0 && name) *name = "relax_lines_k";
             if (c == 0 && !name)
                 relax_lines_k<T, NINE, DIR><<<dim3((unsigned)((nlines + 3) / 4), (unsigned)nb), 256,
-                                              0, s>>>(so, qf, q, sor, ii, jj, first, nlines);
+                                              0, s>>>(so, qf, q, sor, ii, jj, first, nlines, rep);
             return 1;
         }
     }
@@ -2093,10 +2215,12 @@ static int tuned_shape(const tune_key &k) {
 template <typename T, int DIR>
 static int launch_colour(int c, const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii,
                          int jj, int first, int nlines, hipStream_t s, int nb,
-                         std::string *name = nullptr, int rev = 0) {
+                         std::string *name = nullptr, int rev = 0, int rep = 1) {
     if (nstncl == 5)
-        return launch_class<T, true, DIR>(c, so, qf, q, sor, ii, jj, first, nlines, s, nb, name, rev);
-    return launch_class<T, false, DIR>(c, so, qf, q, sor, ii, jj, first, nlines, s, nb, name, rev);
+        return launch_class<T, true, DIR>(c, so, qf, q, sor, ii, jj, first, nlines, s, nb, name, rev,
+                                          rep);
+    return launch_class<T, false, DIR>(c, so, qf, q, sor, ii, jj, first, nlines, s, nb, name, rev,
+                                       rep);
 }
 
 static std::string shape_name(const std::array<int, 6> &k, int c) {
@@ -2121,7 +2245,7 @@ static std::string shape_name(const std::array<int, 6> &k, int c) {
 
 template <typename T, int DIR>
 static void relax_lines(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
-                        int updown, hipStream_t s, int nb) {
+                        int updown, hipStream_t s, int nb, int rep = 1) {
     const int c = tuned_shape(key_of<T>(DIR, nstncl, ii, jj, nb));
     // lines of one colour: DOWN relaxes the lines at even array index (odd Cedar index) first,
     // UP the odd ones first (src/2d/ftn/BMG2_SymStd_relax_lines_x.f90:82-97)
@@ -2131,20 +2255,21 @@ static void relax_lines(const T *so, int nstncl, const T *qf, T *q, const T *sor
         if (first > nfix) continue;
         const int nlines = (nfix - first) / 2 + 1;
         // the second colour walks the lines in the opposite order (see xcd_tile)
-        launch_colour<T, DIR>(c, so, nstncl, qf, q, sor, ii, jj, first, nlines, s, nb, nullptr, t);
+        launch_colour<T, DIR>(c, so, nstncl, qf, q, sor, ii, jj, first, nlines, s, nb, nullptr, t,
+                              rep);
     }
 }
 
 template <typename T>
 void relax_lines_x(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
-                   int updown, hipStream_t s, int nb) {
-    relax_lines<T, 0>(so, nstncl, qf, q, sor, ii, jj, updown, s, nb);
+                   int updown, hipStream_t s, int nb, int rep) {
+    relax_lines<T, 0>(so, nstncl, qf, q, sor, ii, jj, updown, s, nb, rep);
 }
 
 template <typename T>
 void relax_lines_y(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
-                   int updown, hipStream_t s, int nb) {
-    relax_lines<T, 1>(so, nstncl, qf, q, sor, ii, jj, updown, s, nb);
+                   int updown, hipStream_t s, int nb, int rep) {
+    relax_lines<T, 1>(so, nstncl, qf, q, sor, ii, jj, updown, s, nb, rep);
 }
 
 int set_autotune(int on) { return g_autotune.exchange(on ? 1 : 0); }
@@ -2179,7 +2304,7 @@ void set_line_shape(int dir, int wbytes, int nstncl, int ii, int jj, int nb, int
 
 template <typename T>
 int tune_lines(int dir, const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
-               hipStream_t s, int nb) {
+               hipStream_t s, int nb, int rep) {
     const tune_key k = key_of<T>(dir, nstncl, ii, jj, nb);
     {
         std::lock_guard<std::mutex> g(g_tune_mu);
@@ -2187,9 +2312,9 @@ int tune_lines(int dir, const T *so, int nstncl, const T *qf, T *q, const T *sor
     }
     auto run = [&](int c) {
         return dir == 0 ? launch_colour<T, 0>(c, so, nstncl, qf, q, sor, ii, jj, 1,
-                                              (jj - 2 + 1) / 2, s, nb)
+                                              (jj - 2 + 1) / 2, s, nb, nullptr, 0, rep)
                         : launch_colour<T, 1>(c, so, nstncl, qf, q, sor, ii, jj, 1,
-                                              (ii - 2 + 1) / 2, s, nb);
+                                              (ii - 2 + 1) / 2, s, nb, nullptr, 0, rep);
     };
     const int ncand = run(-1);  // count only
     if (!g_autotune.load() || ncand < 2) return 0;
@@ -2241,7 +2366,8 @@ void transpose_op(const T *so, T *soT, int nstncl, int ii, int jj, hipStream_t s
 template <typename T>
 static void relax_lines_ytr(const T *soT, int nstncl, const T *qf, T *q, const T *sory, T *qT,
                             T *qfT, int ii, int jj, int updown, int shape, hipStream_t s, int nb,
-                            bool qfT_ready = false, bool q_in_T = false, bool keep_T = false) {
+                            bool qfT_ready = false, bool q_in_T = false, bool keep_T = false,
+                            int rep = 1) {
     if (!q_in_T) transpose<T>(q, qT, jj, ii, s, nb);
     if (!qfT_ready) transpose<T>(qf, qfT, jj, ii, s, nb);
     const int nfix = ii - 2;  // columns of the original grid = rows of the transposed one
@@ -2251,10 +2377,10 @@ static void relax_lines_ytr(const T *soT, int nstncl, const T *qf, T *q, const T
         const int nlines = (nfix - first) / 2 + 1;
         if (nstncl == 5)
             launch_class<T, true, 2>(shape, soT, qfT, qT, sory, jj, ii, first, nlines, s, nb,
-                                     nullptr, t);
+                                     nullptr, t, rep);
         else
             launch_class<T, false, 2>(shape, soT, qfT, qT, sory, jj, ii, first, nlines, s, nb,
-                                      nullptr, t);
+                                      nullptr, t, rep);
     }
     if (!keep_T) transpose<T>(qT, q, ii, jj, s, nb);
 }
@@ -2268,15 +2394,15 @@ static int tuned_transposed(const tune_key &k) {
 template <typename T>
 bool relax_lines_y_auto(const T *so, const T *soT, int nstncl, const T *qf, T *q, const T *sory,
                         T *qT, T *qfT, int ii, int jj, int updown, hipStream_t s, int nb,
-                        bool qfT_ready, bool q_in_T, bool keep_T) {
+                        bool qfT_ready, bool q_in_T, bool keep_T, int rep) {
     const int c = uses_transposed_y<T>(soT, nstncl, ii, jj, nb);
     if (c < 0) {
         if (q_in_T) transpose<T>(qT, q, ii, jj, s, nb);
-        relax_lines<T, 1>(so, nstncl, qf, q, sory, ii, jj, updown, s, nb);
+        relax_lines<T, 1>(so, nstncl, qf, q, sory, ii, jj, updown, s, nb, rep);
         return false;
     }
     relax_lines_ytr<T>(soT, nstncl, qf, q, sory, qT, qfT, ii, jj, updown, c, s, nb, qfT_ready,
-                       q_in_T, keep_T);
+                       q_in_T, keep_T, rep);
     return true;
 }
 
@@ -2314,7 +2440,7 @@ static float time_captured(hipStream_t cs, hipStream_t s, hipEvent_t e0, hipEven
 
 template <typename T>
 int tune_lines_ytr(const T *so, const T *soT, int nstncl, const T *qf, T *q, const T *sory,
-                   T *qT, T *qfT, int ii, int jj, hipStream_t s, int nb, int sweeps) {
+                   T *qT, T *qfT, int ii, int jj, hipStream_t s, int nb, int sweeps, int rep) {
     sweeps = sweeps < 1 ? 1 : sweeps;
     const tune_key k = key_of<T>(2, nstncl, ii, jj, nb);
     {
@@ -2322,7 +2448,7 @@ int tune_lines_ytr(const T *so, const T *soT, int nstncl, const T *qf, T *q, con
         if (g_tune.count(k)) return g_tune[k];
     }
     if (!g_autotune.load()) return -1;
-    if (tune_lines<T>(1, so, nstncl, qf, q, sory, ii, jj, s, nb) < 0) return -2;
+    if (tune_lines<T>(1, so, nstncl, qf, q, sory, ii, jj, s, nb, rep) < 0) return -2;
     const int ncand = nstncl == 5 ? launch_class<T, true, 2>(-1, soT, qf, q, sory, jj, ii, 1, 1, s, nb)
                                   : launch_class<T, false, 2>(-1, soT, qf, q, sory, jj, ii, 1, 1, s, nb);
     hipEvent_t e0, e1;
@@ -2347,7 +2473,8 @@ int tune_lines_ytr(const T *so, const T *soT, int nstncl, const T *qf, T *q, con
     };
     // `sweeps` y sweeps share one right-hand side (one transpose of qf)
     const float tdirect = timed([&](hipStream_t st) {
-        for (int k = 0; k < sweeps; ++k) relax_lines<T, 1>(so, nstncl, qf, q, sory, ii, jj, 0, st, nb);
+        for (int k = 0; k < sweeps; ++k)
+            relax_lines<T, 1>(so, nstncl, qf, q, sory, ii, jj, 0, st, nb, rep);
     });
     int best = -1;
     float tbest = tdirect;
@@ -2355,7 +2482,7 @@ int tune_lines_ytr(const T *so, const T *soT, int nstncl, const T *qf, T *q, con
         const float t = timed([&](hipStream_t st) {
             for (int k = 0; k < sweeps; ++k)
                 relax_lines_ytr<T>(soT, nstncl, qf, q, sory, qT, qfT, ii, jj, 0, c, st, nb, k > 0,
-                                   k > 0, k < sweeps - 1);
+                                   k > 0, k < sweeps - 1, rep);
         });
         if (t >= 0.f && t < tbest) {
             tbest = t;
@@ -2391,48 +2518,55 @@ std::string tune_report() {
 
 template <typename T>
 void relax_lines_xy(const T *so, int nstncl, const T *qf, T *q, const T *sorx, const T *sory,
-                    T *work, int ii, int jj, int updown, hipStream_t s, int nb) {
+                    T *work, int ii, int jj, int updown, hipStream_t s, int nb, int rep) {
     (void)work;
     if (updown == 0) {
-        relax_lines<T, 0>(so, nstncl, qf, q, sorx, ii, jj, updown, s, nb);
-        relax_lines<T, 1>(so, nstncl, qf, q, sory, ii, jj, updown, s, nb);
+        relax_lines<T, 0>(so, nstncl, qf, q, sorx, ii, jj, updown, s, nb, rep);
+        relax_lines<T, 1>(so, nstncl, qf, q, sory, ii, jj, updown, s, nb, rep);
     } else {
-        relax_lines<T, 1>(so, nstncl, qf, q, sory, ii, jj, updown, s, nb);
-        relax_lines<T, 0>(so, nstncl, qf, q, sorx, ii, jj, updown, s, nb);
+        relax_lines<T, 1>(so, nstncl, qf, q, sory, ii, jj, updown, s, nb, rep);
+        relax_lines<T, 0>(so, nstncl, qf, q, sorx, ii, jj, updown, s, nb, rep);
     }
 }
 
 template <typename T>
 void restrict_(const T *ci, const T *q, T *qc, int iif, int jjf, int iic, int jjc, hipStream_t s,
-               int nb) {
-    restrict_k<T><<<gridb(iic - 2, jjc - 2, nb), 256, 0, s>>>(ci, q, qc, iif, jjf, iic, jjc);
+               int nb, int rep) {
+    restrict_k<T><<<gridb(iic - 2, jjc - 2, nb), 256, 0, s>>>(ci, q, qc, iif, jjf, iic, jjc, rep);
+}
+
+template <typename T>
+void restrict_cycle(const T *ci, const T *q, T *qc, T *xc, T *qcT, int iif, int jjf, int iic,
+                    int jjc, hipStream_t s, int nb, int rep) {
+    const dim3 g((unsigned)((iic + 63) / 64), (unsigned)((jjc + 15) / 16), (unsigned)nb);
+    restrict_cycle_k<T><<<g, 256, 0, s>>>(ci, q, qc, xc, qcT, iif, jjf, iic, jjc, rep);
 }
 
 template <typename T>
 void interp_add(T *q, const T *qc, const T *res, const T *so, int nstncl, const T *ci, int iic,
-                int jjc, int iif, int jjf, hipStream_t s, int nb) {
+                int jjc, int iif, int jjf, hipStream_t s, int nb, int rep) {
     // one thread per 2 x 2 fine block (the range includes the ghost point of even sizes)
     interp_add_k<T><<<gridb(iif / 2, jjf / 2, nb), 256, 0, s>>>(q, qc, res, so, ci, iic, jjc, iif,
-                                                                jjf, nstncl);
+                                                                jjf, nstncl, rep);
 }
 
 template <typename T>
 void residual_qT(const T *so, int nstncl, const T *qf, const T *qT, T *res, int ii, int jj,
-                 hipStream_t s, int nb) {
+                 hipStream_t s, int nb, int rep) {
     const dim3 g((unsigned)((ii - 2 + kTI - 1) / kTI), (unsigned)((jj - 2 + kTJ - 1) / kTJ),
                  (unsigned)nb);
     if (nstncl == 5)
-        residual_qT_k<T, true><<<g, 256, 0, s>>>(so, qf, qT, res, ii, jj);
+        residual_qT_k<T, true><<<g, 256, 0, s>>>(so, qf, qT, res, ii, jj, rep);
     else
-        residual_qT_k<T, false><<<g, 256, 0, s>>>(so, qf, qT, res, ii, jj);
+        residual_qT_k<T, false><<<g, 256, 0, s>>>(so, qf, qT, res, ii, jj, rep);
 }
 
 template <typename T>
 void interp_add_qT(T *qT, const T *qc, const T *res, const T *so, int nstncl, const T *ci,
-                   int iic, int jjc, int iif, int jjf, hipStream_t s, int nb) {
+                   int iic, int jjc, int iif, int jjf, hipStream_t s, int nb, int rep) {
     // tiles of 32 x 16 blocks of 2 x 2 fine points over the blocks m <= iif/2, n <= jjf/2
     const dim3 g((unsigned)((iif / 2 + 31) / 32), (unsigned)((jjf / 2 + 15) / 16), (unsigned)nb);
-    interp_add_qT_k<T><<<g, 512, 0, s>>>(qT, qc, res, so, ci, iic, jjc, iif, jjf, nstncl);
+    interp_add_qT_k<T><<<g, 512, 0, s>>>(qT, qc, res, so, ci, iic, jjc, iif, jjf, nstncl, rep);
 }
 
 void setup_interp(const double *so, int nstncl, double *ci, int iif, int jjf, int iic, int jjc,
@@ -2467,17 +2601,36 @@ void setup_cg(const double *so, int nstncl, double *abd, int ii, int jj, int *in
 
 template <typename T>
 void solve_cg(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj, hipStream_t s,
-              int nb) {
-    solve_cg_k<T><<<nb, 64, 0, s>>>(q, qf, abd, bbd, ii, jj);
+              int nb, int rep) {
+    solve_cg_k<T><<<nb, 64, 0, s>>>(q, qf, abd, bbd, ii, jj, rep);
+}
+
+std::size_t fused_cycle_lds_bytes(int wbytes, const int *ii, const int *jj, int nlev) {
+    std::size_t b = 0;
+    for (int l = 0; l < nlev; ++l)
+        b += wbytes == 8 ? fused_level_bytes<double>(ii[l], jj[l], l > 0)
+                         : fused_level_bytes<float>(ii[l], jj[l], l > 0);
+    return b;
 }
 
 template <typename T>
 void coarse_cycle(const coarse_level<T> *lv, int nlev, int max_side, int relax, int npre,
-                  int npost, const double *abd, double *bbd, hipStream_t s, int nb) {
-    if (max_side <= 64)
-        coarse_cycle_k<T, 1><<<nb, 1024, 0, s>>>(lv, nlev, relax, npre, npost, abd, bbd);
-    else
-        coarse_cycle_k<T, 2><<<nb, 1024, 0, s>>>(lv, nlev, relax, npre, npost, abd, bbd);
+                  int npost, const double *abd, double *bbd, hipStream_t s, int nb, int rep,
+                  std::size_t lds_bytes) {
+    const int on_chip = lds_bytes > 0 ? 1 : 0;
+    if (max_side <= 64) {
+        if (on_chip)
+            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(coarse_cycle_k<T, 1>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+        coarse_cycle_k<T, 1><<<nb, 1024, lds_bytes, s>>>(lv, nlev, relax, npre, npost, abd, bbd,
+                                                         rep, on_chip);
+    } else {
+        if (on_chip)
+            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(coarse_cycle_k<T, 2>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+        coarse_cycle_k<T, 2><<<nb, 1024, lds_bytes, s>>>(lv, nlev, relax, npre, npost, abd, bbd,
+                                                         rep, on_chip);
+    }
 }
 
 template <typename Tin, typename Tout>
@@ -3136,14 +3289,14 @@ void relax_gs_per(const T *so, int nstncl, const T *qf, T *q, const T *sor, int 
         const dim3 g = gridb(ii / 2 + 1, jj / 2 + 1, nb);
         for (int t = 0; t < 4; ++t) {
             const int c = updown == 0 ? t : 3 - t;
-            gs_colour_k<T, true><<<g, 256, 0, s>>>(so, qf, q, sor, ii, jj, order[c][0], order[c][1]);
+            gs_colour_k<T, true><<<g, 256, 0, s>>>(so, qf, q, sor, ii, jj, order[c][0], order[c][1], 1);
             if (per_x(jpn)) ghost_x<T>(q, ii, jj, order[c][1] == 1 ? 1 : 2, jj - 2, 2, s, nb);
         }
     } else {
         const dim3 g = gridb(ii / 2 + 1, jj - 2, nb);
         for (int t = 0; t < 2; ++t) {
             const int parity = updown == 0 ? t : 1 - t;
-            gs_colour_k<T, false><<<g, 256, 0, s>>>(so, qf, q, sor, ii, jj, parity, 0);
+            gs_colour_k<T, false><<<g, 256, 0, s>>>(so, qf, q, sor, ii, jj, parity, 0, 1);
             if (per_x(jpn)) ghost_x<T>(q, ii, jj, 1, jj - 2, 1, s, nb);
         }
     }
@@ -3204,13 +3357,13 @@ void restrict_per(const T *ci, T *q, T *qc, int iif, int jjf, int iic, int jjc, 
     // when the fine extent is even
     if (per_y(jpn) && jjf / 2 + 1 == jjc) ghost_y<T>(q, iif, jjf, 0, iif - 1, s, nb);
     if (per_x(jpn) && iif / 2 + 1 == iic) ghost_x<T>(q, iif, jjf, 0, jjf - 1, 1, s, nb);
-    restrict_<T>(ci, q, qc, iif, jjf, iic, jjc, s, nb);
+    restrict_<T>(ci, q, qc, iif, jjf, iic, jjc, s, nb, 1);
 }
 
 template <typename T>
 void interp_add_per(T *q, const T *qc, const T *res, const T *so, int nstncl, const T *ci, int iic,
                     int jjc, int iif, int jjf, int jpn, hipStream_t s, int nb) {
-    interp_add<T>(q, qc, res, so, nstncl, ci, iic, jjc, iif, jjf, s, nb);
+    interp_add<T>(q, qc, res, so, nstncl, ci, iic, jjc, iif, jjf, s, nb, 1);
     // src/2d/ftn/BMG2_SymStd_interp_add.f90:145-156
     if (per_y(jpn)) ghost_y<T>(q, iif, jjf, 0, iif - 1, s, nb);
     if (per_x(jpn)) ghost_x<T>(q, iif, jjf, 0, jjf - 1, 1, s, nb);
@@ -3298,38 +3451,42 @@ void copy(const T *src, T *dst, std::size_t n, hipStream_t s) {
 // explicit instantiations
 #define CDR_INST(T)                                                                               \
     template void residual<T>(const T *, int, const T *, const T *, T *, int, int, hipStream_t,   \
-                              int);                                                               \
+                              int, int);                                                          \
     template void residual_norm<T>(const T *, int, const T *, const T *, T *, int, int, double *, \
-                                   double *, hipStream_t, int);                                   \
+                                   double *, hipStream_t, int, int);                              \
     template void sumsq<T>(const T *, int, int, double *, double *, hipStream_t, int);            \
     template void relax_gs<T>(const T *, int, const T *, T *, const T *, int, int, int,           \
-                              hipStream_t, int);                                                  \
+                              hipStream_t, int, int);                                             \
     template void relax_lines_x<T>(const T *, int, const T *, T *, const T *, int, int, int,      \
-                                   hipStream_t, int);                                             \
+                                   hipStream_t, int, int);                                        \
     template void relax_lines_y<T>(const T *, int, const T *, T *, const T *, int, int, int,      \
-                                   hipStream_t, int);                                             \
+                                   hipStream_t, int, int);                                        \
     template void relax_lines_xy<T>(const T *, int, const T *, T *, const T *, const T *, T *, int,\
-                                    int, int, hipStream_t, int);                                  \
+                                    int, int, hipStream_t, int, int);                             \
     template int tune_lines<T>(int, const T *, int, const T *, T *, const T *, int, int,          \
-                               hipStream_t, int);                                                 \
+                               hipStream_t, int, int);                                            \
     template void transpose<T>(const T *, T *, int, int, hipStream_t, int);                       \
     template void coarse_cycle<T>(const coarse_level<T> *, int, int, int, int, int, const double *, \
-                                  double *, hipStream_t, int);                                    \
+                                  double *, hipStream_t, int, int, std::size_t);                  \
     template void transpose_op<T>(const T *, T *, int, int, int, hipStream_t, int);               \
     template bool relax_lines_y_auto<T>(const T *, const T *, int, const T *, T *, const T *, T *, \
-                                        T *, int, int, int, hipStream_t, int, bool, bool, bool);  \
+                                        T *, int, int, int, hipStream_t, int, bool, bool, bool,   \
+                                        int);                                                     \
     template int uses_transposed_y<T>(const T *, int, int, int, int);                           \
     template void residual_qT<T>(const T *, int, const T *, const T *, T *, int, int, hipStream_t, \
-                                 int);                                                            \
+                                 int, int);                                                       \
     template void interp_add_qT<T>(T *, const T *, const T *, const T *, int, const T *, int, int, \
-                                   int, int, hipStream_t, int);                                   \
-    template int tune_lines_ytr<T>(const T *, const T *, int, const T *, T *, const T *, T *, T *, \
                                    int, int, hipStream_t, int, int);                              \
-    template void restrict_<T>(const T *, const T *, T *, int, int, int, int, hipStream_t, int);  \
+    template int tune_lines_ytr<T>(const T *, const T *, int, const T *, T *, const T *, T *, T *, \
+                                   int, int, hipStream_t, int, int, int);                         \
+    template void restrict_<T>(const T *, const T *, T *, int, int, int, int, hipStream_t, int,   \
+                               int);                                                              \
+    template void restrict_cycle<T>(const T *, const T *, T *, T *, T *, int, int, int, int,       \
+                                    hipStream_t, int, int);                                       \
     template void interp_add<T>(T *, const T *, const T *, const T *, int, const T *, int, int,   \
-                                int, int, hipStream_t, int);                                      \
+                                int, int, hipStream_t, int, int);                                 \
     template void solve_cg<T>(T *, const T *, const double *, double *, int, int, hipStream_t,    \
-                              int);                                                               \
+                              int, int);                                                          \
     template void fill<T>(T *, T, std::size_t, hipStream_t);                                     \
     template void relax_gs_per<T>(const T *, int, const T *, T *, const T *, int, int, int, int,  \
                                   hipStream_t, int);                                              \

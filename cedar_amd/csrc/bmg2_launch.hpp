@@ -6,7 +6,12 @@
 // synchronises; argument checking happens in the C ABI (capi2d.hip).
 //
 // The trailing `nb` runs the same call on nb problems of identical extents stored back to back
-// (each array of problem b starts at b times the size of one problem's array); per-problem
+// (each array of problem b starts at b times the size of one problem's array).  Where a launcher
+// also takes `rep`, consecutive groups of rep problems share one operator: the operator-side
+// arrays (stencil, interpolation, relaxation factors, coarse band factor) hold nb / rep entries
+// and problem b reads entry b / rep, so several right-hand sides of one operator are relaxed,
+// restricted and interpolated in the same launch while the operator is read once per group;
+// per-problem
 // outputs of the reductions (sumsq) are consecutive doubles, and `partials` holds
 // nb * norm_partials(ii, jj) values.  Factorisation info words are shared by the batch: the
 // kernels raise them with atomicMax, so they must start at 0 and end at 0 when every problem is
@@ -33,21 +38,24 @@ std::size_t norm_partials(int ii, int jj);
 
 template <typename T>
 void residual(const T *so, int nstncl, const T *qf, const T *q, T *res, int ii, int jj,
-              hipStream_t s, int nb = 1);
+              hipStream_t s, int nb = 1, int rep = 1);
 // residual + per-block partial sums of r^2 (FP64) + final reduction into *sumsq (device)
 template <typename T>
 void residual_norm(const T *so, int nstncl, const T *qf, const T *q, T *res, int ii, int jj,
-                   double *partials, double *sumsq, hipStream_t s, int nb = 1);
+                   double *partials, double *sumsq, hipStream_t s, int nb = 1, int rep = 1);
 // sum of squares of the interior of x into *sumsq (device), deterministic two-stage reduction
 template <typename T>
 void sumsq(const T *x, int ii, int jj, double *partials, double *out, hipStream_t s, int nb = 1);
 
 // Mixed-precision outer step: xnew = x + (double)e (e may be null: xnew = x),
 // r = b - A xnew in FP64, r32 = (float) r, partial sums of r^2 into *sumsq.  With `active`
-// (nb ints, may be null) a problem b with active[b] == 0 keeps xnew = x.
+// (nb ints, may be null) a problem b with active[b] == 0 keeps xnew = x.  With r32T non-null the
+// interior of r32 is also written transposed (ii rows of jj values), for a finest FP32 level
+// whose y sweeps run through the transposed grid.
 void residual_ir(const double *so, int nstncl, const double *qf, const double *x, const float *e,
                  double *xnew, float *r32, int ii, int jj, double *partials, double *sumsq,
-                 hipStream_t s, int nb = 1, const int *active = nullptr);
+                 hipStream_t s, int nb = 1, const int *active = nullptr, int rep = 1,
+                 float *r32T = nullptr);
 // out[b] = sum of partials[b*n .. b*n+n-1] in a fixed order (deterministic), b < nb
 void finalize_sum(const double *partials, std::size_t n, double *out, int nb, hipStream_t s);
 // x[k] = xprev[k] over every problem b (per values each) with active[b] == 0
@@ -58,7 +66,7 @@ void setup_recip(const double *so, int nstncl, double *sor, int ii, int jj, hipS
                  int nb = 1);
 template <typename T>
 void relax_gs(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
-              int updown, hipStream_t s, int nb = 1);
+              int updown, hipStream_t s, int nb = 1, int rep = 1);
 
 // LDL^T factors of every line (x: rows, y: columns, stored transposed); info (zero on entry)
 // is raised to the largest failing pivot index (1-based, line-local); it stays 0 when every
@@ -69,16 +77,16 @@ void setup_lines_y(const double *so, int nstncl, double *sor, int ii, int jj, in
                    hipStream_t s, int nb = 1);
 template <typename T>
 void relax_lines_x(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
-                   int updown, hipStream_t s, int nb = 1);
+                   int updown, hipStream_t s, int nb = 1, int rep = 1);
 template <typename T>
 void relax_lines_y(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
-                   int updown, hipStream_t s, int nb = 1);
+                   int updown, hipStream_t s, int nb = 1, int rep = 1);
 
 // One line-xy smoothing step (include/cedar/multilevel.h:175-178, 204-207): x-lines then y-lines
 // on DOWN, y then x on UP.  `work` is unused (kept for callers that size it).
 template <typename T>
 void relax_lines_xy(const T *so, int nstncl, const T *qf, T *q, const T *sorx, const T *sory,
-                    T *work, int ii, int jj, int updown, hipStream_t s, int nb = 1);
+                    T *work, int ii, int jj, int updown, hipStream_t s, int nb = 1, int rep = 1);
 
 // Line-kernel shape autotuner.  Every shape of the line kernels gives bitwise identical results;
 // tune_lines times each candidate shape for one direction (0: x, 1: y) once on the given data
@@ -88,7 +96,7 @@ void relax_lines_xy(const T *so, int nstncl, const T *qf, T *q, const T *sorx, c
 // previous setting) nothing is timed or recorded and the default shape 0 runs.
 template <typename T>
 int tune_lines(int dir, const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
-               hipStream_t s, int nb = 1);
+               hipStream_t s, int nb = 1, int rep = 1);
 int set_autotune(int on);
 bool autotune_enabled();
 
@@ -109,9 +117,15 @@ struct coarse_level {
 // the first level from its b (x zero on entry).  max_side: the largest nx or ny of the levels
 // (at most 128).  relax: CDR_RELAX_*; npre / npost: smoothing steps.  Bitwise equal to the
 // unfused V-cycle over the same levels.
+// lds_bytes > 0 (at least fused_cycle_lds_bytes of the levels): every array of the fused levels
+// is staged in LDS for the whole cycle (only the first level's b and x cross memory).
 template <typename T>
 void coarse_cycle(const coarse_level<T> *lv, int nlev, int max_side, int relax, int npre,
-                  int npost, const double *abd, double *bbd, hipStream_t s, int nb = 1);
+                  int npost, const double *abd, double *bbd, hipStream_t s, int nb = 1,
+                  int rep = 1, std::size_t lds_bytes = 0);
+// LDS bytes that on-chip fused levels of these extents (first level first) need at word size
+// wbytes
+std::size_t fused_cycle_lds_bytes(int wbytes, const int *ii, const int *jj, int nlev);
 
 // dst (cols x rows) = transpose of src (rows x cols), row-major, per problem of a batch
 template <typename T>
@@ -130,7 +144,8 @@ void transpose_op(const T *so, T *soT, int nstncl, int ii, int jj, hipStream_t s
 template <typename T>
 bool relax_lines_y_auto(const T *so, const T *soT, int nstncl, const T *qf, T *q, const T *sory,
                         T *qT, T *qfT, int ii, int jj, int updown, hipStream_t s, int nb = 1,
-                        bool qfT_ready = false, bool q_in_T = false, bool keep_T = false);
+                        bool qfT_ready = false, bool q_in_T = false, bool keep_T = false,
+                        int rep = 1);
 // The shape the transposed y path runs with at these extents, or -1 where the direct y-line
 // kernel runs (no transposed operator, or tuning chose it).
 template <typename T>
@@ -140,10 +155,10 @@ int uses_transposed_y(const T *soT, int nstncl, int ii, int jj, int nb);
 // untransposed iterate.  interp_add_qT updates qT in place.
 template <typename T>
 void residual_qT(const T *so, int nstncl, const T *qf, const T *qT, T *res, int ii, int jj,
-                 hipStream_t s, int nb = 1);
+                 hipStream_t s, int nb = 1, int rep = 1);
 template <typename T>
 void interp_add_qT(T *qT, const T *qc, const T *res, const T *so, int nstncl, const T *ci,
-                   int iic, int jjc, int iif, int jjf, hipStream_t s, int nb = 1);
+                   int iic, int jjc, int iif, int jjf, hipStream_t s, int nb = 1, int rep = 1);
 // Times the direct y-lines against the transposed path (each x-line shape) once and records the
 // choice (-1: direct); `sweeps` y sweeps are timed per right-hand side (one transpose of qf), as
 // a level visit of a V(nu1, nu2) cycle makes nu1 + nu2 of them.  Overwrites q, qT, qfT;
@@ -151,7 +166,7 @@ void interp_add_qT(T *qT, const T *qc, const T *res, const T *so, int nstncl, co
 // autotuning off.
 template <typename T>
 int tune_lines_ytr(const T *so, const T *soT, int nstncl, const T *qf, T *q, const T *sory, T *qT,
-                   T *qfT, int ii, int jj, hipStream_t s, int nb = 1, int sweeps = 2);
+                   T *qfT, int ii, int jj, hipStream_t s, int nb = 1, int sweeps = 2, int rep = 1);
 // number of candidate shapes for lines of the given direction, word size (4 or 8) and extents
 int line_shape_count(int dir, int wbytes, int nstncl, int ii, int jj);
 // records `shape` for (direction, word size, stencil, ii, jj, nb) (shape < 0: erases the entry)
@@ -160,10 +175,17 @@ void set_line_shape(int dir, int wbytes, int nstncl, int ii, int jj, int nb, int
 std::string tune_report();
 
 template <typename T>
-void restrict_(const T *ci, const T *q, T *qc, int iif, int jjf, int iic, int jjc, hipStream_t s, int nb = 1);
+void restrict_(const T *ci, const T *q, T *qc, int iif, int jjf, int iic, int jjc, hipStream_t s,
+               int nb = 1, int rep = 1);
 template <typename T>
 void interp_add(T *q, const T *qc, const T *res, const T *so, int nstncl, const T *ci, int iic,
-                int jjc, int iif, int jjf, hipStream_t s, int nb = 1);
+                int jjc, int iif, int jjf, hipStream_t s, int nb = 1, int rep = 1);
+// qc = P^T q (bitwise restrict_), xc = 0 over the whole coarse grid, and, with qcT non-null, qc
+// also written transposed (qcT: iic rows of jjc values, interior only): the restriction step of
+// a V-cycle in one launch
+template <typename T>
+void restrict_cycle(const T *ci, const T *q, T *qc, T *xc, T *qcT, int iif, int jjf, int iic,
+                    int jjc, hipStream_t s, int nb = 1, int rep = 1);
 
 void setup_interp(const double *so, int nstncl, double *ci, int iif, int jjf, int iic, int jjc,
                   hipStream_t s, int nb = 1);
@@ -174,7 +196,8 @@ void setup_galerkin(const double *so, int nstncl, const double *ci, double *soc,
 // the failing column when a pivot is not positive
 void setup_cg(const double *so, int nstncl, double *abd, int ii, int jj, int *info, hipStream_t s, int nb = 1);
 template <typename T>
-void solve_cg(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj, hipStream_t s, int nb = 1);
+void solve_cg(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj, hipStream_t s,
+              int nb = 1, int rep = 1);
 
 // Periodic boundaries: jpn = BMG_get_bc(per_mask) (2 periodic in x, 1 in y, 3 both; 0 runs the
 // Dirichlet launcher).  Each is the Dirichlet kernel plus the reference's periodic images in the

@@ -28,6 +28,8 @@ inline bool coarse_ok(int iif, int jjf, int iic, int jjc) {
 }
 
 inline bool nb_ok(int nb) { return nb >= 1 && nb <= 65535; }
+// rep problems per operator: a positive divisor of the batch
+inline bool rep_ok(int nb, int rep) { return rep >= 1 && nb % rep == 0; }
 
 // runs a factorisation kernel, then reads its device info word (setup-time synchronisation)
 template <typename F>
@@ -50,19 +52,20 @@ namespace api2 {
 
 template <typename T>
 int residual(const T *so, int ns, const T *qf, const T *q, T *res, int ii, int jj, hipStream_t s,
-             int nb) {
-    if (!so || !qf || !q || !res || !ext_ok(ii, jj) || !sten_ok(ns) || !nb_ok(nb))
+             int nb, int rep) {
+    if (!so || !qf || !q || !res || !ext_ok(ii, jj) || !sten_ok(ns) || !nb_ok(nb) ||
+        !rep_ok(nb, rep))
         return CDR_ERR_ARG;
-    k2::residual<T>(so, ns, qf, q, res, ii, jj, s, nb);
+    k2::residual<T>(so, ns, qf, q, res, ii, jj, s, nb, rep);
     return launched();
 }
 
 int residual_norm(const double *so, int ns, const double *qf, const double *q, double *res, int ii,
-                  int jj, double *work, double *sumsq_dev, hipStream_t s, int nb) {
+                  int jj, double *work, double *sumsq_dev, hipStream_t s, int nb, int rep) {
     if (!so || !qf || !q || !res || !work || !sumsq_dev || !ext_ok(ii, jj) || !sten_ok(ns) ||
-        !nb_ok(nb))
+        !nb_ok(nb) || !rep_ok(nb, rep))
         return CDR_ERR_ARG;
-    k2::residual_norm<double>(so, ns, qf, q, res, ii, jj, work, sumsq_dev, s, nb);
+    k2::residual_norm<double>(so, ns, qf, q, res, ii, jj, work, sumsq_dev, s, nb, rep);
     return launched();
 }
 
@@ -81,10 +84,11 @@ int setup_recip(const double *so, double *sor, int ii, int jj, int ns, hipStream
 
 template <typename T>
 int relax_gs(const T *so, int ns, const T *qf, T *q, const T *sor, int ii, int jj, int ud,
-             hipStream_t s, int nb) {
-    if (!so || !qf || !q || !sor || !ext_ok(ii, jj) || !sten_ok(ns) || !dir_ok(ud) || !nb_ok(nb))
+             hipStream_t s, int nb, int rep) {
+    if (!so || !qf || !q || !sor || !ext_ok(ii, jj) || !sten_ok(ns) || !dir_ok(ud) || !nb_ok(nb) ||
+        !rep_ok(nb, rep))
         return CDR_ERR_ARG;
-    k2::relax_gs<T>(so, ns, qf, q, sor, ii, jj, ud, s, nb);
+    k2::relax_gs<T>(so, ns, qf, q, sor, ii, jj, ud, s, nb, rep);
     return launched();
 }
 
@@ -102,42 +106,43 @@ int setup_lines(int dir, const double *so, double *sor, int ii, int jj, int ns, 
 
 template <typename T>
 int relax_lines(int dir, const T *so, int ns, const T *qf, T *q, const T *sor, int ii, int jj,
-                int ud, hipStream_t s, int nb) {
+                int ud, hipStream_t s, int nb, int rep) {
     if (!so || !qf || !q || !sor || !ext_ok(ii, jj) || !sten_ok(ns) || !dir_ok(ud) ||
-        !nb_ok(nb) || (dir != 0 && dir != 1))
+        !nb_ok(nb) || !rep_ok(nb, rep) || (dir != 0 && dir != 1))
         return CDR_ERR_ARG;
     if (dir == 0)
-        k2::relax_lines_x<T>(so, ns, qf, q, sor, ii, jj, ud, s, nb);
+        k2::relax_lines_x<T>(so, ns, qf, q, sor, ii, jj, ud, s, nb, rep);
     else
-        k2::relax_lines_y<T>(so, ns, qf, q, sor, ii, jj, ud, s, nb);
+        k2::relax_lines_y<T>(so, ns, qf, q, sor, ii, jj, ud, s, nb, rep);
     return launched();
 }
 
 template <typename T>
 int relax_lines_xy(const T *so, int ns, const T *qf, T *q, const T *sorx, const T *sory, T *work,
-                   int ii, int jj, int ud, hipStream_t s, int nb) {
+                   int ii, int jj, int ud, hipStream_t s, int nb, int rep) {
     if (!so || !qf || !q || !sorx || !sory || !ext_ok(ii, jj) || !sten_ok(ns) || !dir_ok(ud) ||
-        !nb_ok(nb))
+        !nb_ok(nb) || !rep_ok(nb, rep))
         return CDR_ERR_ARG;
-    k2::relax_lines_xy<T>(so, ns, qf, q, sorx, sory, work, ii, jj, ud, s, nb);
+    k2::relax_lines_xy<T>(so, ns, qf, q, sorx, sory, work, ii, jj, ud, s, nb, rep);
     return launched();
 }
 
 template <typename T>
 int restrict_(const T *ci, const T *q, T *qc, int iif, int jjf, int iic, int jjc, hipStream_t s,
-              int nb) {
-    if (!ci || !q || !qc || !coarse_ok(iif, jjf, iic, jjc) || !nb_ok(nb)) return CDR_ERR_ARG;
-    k2::restrict_<T>(ci, q, qc, iif, jjf, iic, jjc, s, nb);
+              int nb, int rep) {
+    if (!ci || !q || !qc || !coarse_ok(iif, jjf, iic, jjc) || !nb_ok(nb) || !rep_ok(nb, rep))
+        return CDR_ERR_ARG;
+    k2::restrict_<T>(ci, q, qc, iif, jjf, iic, jjc, s, nb, rep);
     return launched();
 }
 
 template <typename T>
 int interp_add(T *q, const T *qc, const T *res, const T *so, int ns, const T *ci, int iic, int jjc,
-               int iif, int jjf, hipStream_t s, int nb) {
+               int iif, int jjf, hipStream_t s, int nb, int rep) {
     if (!q || !qc || !res || !so || !ci || !sten_ok(ns) || !coarse_ok(iif, jjf, iic, jjc) ||
-        !nb_ok(nb))
+        !nb_ok(nb) || !rep_ok(nb, rep))
         return CDR_ERR_ARG;
-    k2::interp_add<T>(q, qc, res, so, ns, ci, iic, jjc, iif, jjf, s, nb);
+    k2::interp_add<T>(q, qc, res, so, ns, ci, iic, jjc, iif, jjf, s, nb, rep);
     return launched();
 }
 
@@ -165,9 +170,10 @@ int setup_cg(const double *so, int ns, double *abd, int ii, int jj, int *info_de
 
 template <typename T>
 int solve_cg(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj, hipStream_t s,
-             int nb) {
-    if (!q || !qf || !abd || !bbd || !ext_ok(ii, jj) || !nb_ok(nb)) return CDR_ERR_ARG;
-    k2::solve_cg<T>(q, qf, abd, bbd, ii, jj, s, nb);
+             int nb, int rep) {
+    if (!q || !qf || !abd || !bbd || !ext_ok(ii, jj) || !nb_ok(nb) || !rep_ok(nb, rep))
+        return CDR_ERR_ARG;
+    k2::solve_cg<T>(q, qf, abd, bbd, ii, jj, s, nb, rep);
     return launched();
 }
 
@@ -286,17 +292,19 @@ int solve_cg_per(T *q, const T *qf, const double *abd, double *bbd, int ii, int 
     template int relax_lines_per<T>(int, const T *, int, const T *, T *, const T *, T *, int, int, \
                                     int, int, hipStream_t, int);                                   \
     template int residual<T>(const T *, int, const T *, const T *, T *, int, int, hipStream_t,     \
-                             int);                                                                 \
+                             int, int);                                                            \
     template int relax_gs<T>(const T *, int, const T *, T *, const T *, int, int, int, hipStream_t, \
-                             int);                                                                 \
+                             int, int);                                                            \
     template int relax_lines<T>(int, const T *, int, const T *, T *, const T *, int, int, int,      \
-                                hipStream_t, int);                                                 \
+                                hipStream_t, int, int);                                            \
     template int relax_lines_xy<T>(const T *, int, const T *, T *, const T *, const T *, T *, int,  \
-                                   int, int, hipStream_t, int);                                    \
-    template int restrict_<T>(const T *, const T *, T *, int, int, int, int, hipStream_t, int);    \
+                                   int, int, hipStream_t, int, int);                               \
+    template int restrict_<T>(const T *, const T *, T *, int, int, int, int, hipStream_t, int,     \
+                              int);                                                                \
     template int interp_add<T>(T *, const T *, const T *, const T *, int, const T *, int, int, int, \
-                               int, hipStream_t, int);                                             \
-    template int solve_cg<T>(T *, const T *, const double *, double *, int, int, hipStream_t, int);
+                               int, hipStream_t, int, int);                                        \
+    template int solve_cg<T>(T *, const T *, const double *, double *, int, int, hipStream_t, int,  \
+                             int);
 CDR_API_INST(double)
 CDR_API_INST(float)
 #undef CDR_API_INST

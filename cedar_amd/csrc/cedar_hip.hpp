@@ -127,10 +127,13 @@ struct five_pt { static constexpr int ndirs = 3; };
 struct nine_pt { static constexpr int ndirs = 5; };
 
 // ghost-inclusive extents of a 2D grid quantity; nb problems of these extents are stored back
-// to back (a batch), each kernel runs on all of them
+// to back (a batch), each kernel runs on all of them.  Operator-side quantities (stencils,
+// interpolation, relaxation factors, coarse factor) may be shared by groups of rep consecutive
+// problems: they then store nb entries for nb * rep problems (rep = 1 for grid functions).
 struct extents {
     int ii = 0, jj = 0;
     int nb = 1;
+    int rep = 1;
     std::size_t points() const { return (std::size_t)ii * jj; }  // one problem
     std::size_t total() const { return points() * (std::size_t)nb; }
     int nx() const { return ii - 2; }

@@ -32,7 +32,7 @@ template <typename T>
 struct hip_residual : kernels::residual<T> {
     void run(const stencil_op<T> &so, const grid_func<T> &x, const grid_func<T> &b,
              grid_func<T> &r) override {
-        check(api2::residual<T>(so.p, so.nstncl, b.p, x.p, r.p, so.ext.ii, so.ext.jj, this->stream, so.ext.nb), "residual");
+        check(api2::residual<T>(so.p, so.nstncl, b.p, x.p, r.p, so.ext.ii, so.ext.jj, this->stream, x.ext.nb, so.ext.rep), "residual");
     }
 };
 
@@ -46,7 +46,7 @@ struct hip_rbgs : kernels::point_relax<T> {
         if (this->jpn)
             check(api2::relax_gs_per<T>(so.p, so.nstncl, b.p, x.p, sor.p, so.ext.ii, so.ext.jj, (int)d, this->jpn, this->stream, so.ext.nb), "relax_gs");
         else
-            check(api2::relax_gs<T>(so.p, so.nstncl, b.p, x.p, sor.p, so.ext.ii, so.ext.jj, (int)d, this->stream, so.ext.nb), "relax_gs");
+            check(api2::relax_gs<T>(so.p, so.nstncl, b.p, x.p, sor.p, so.ext.ii, so.ext.jj, (int)d, this->stream, x.ext.nb, so.ext.rep), "relax_gs");
     }
 };
 
@@ -67,7 +67,7 @@ struct hip_lines : kernels::line_relax<T, RD> {
         if (this->jpn)
             check(api2::relax_lines_per<T>(dir, so.p, so.nstncl, b.p, x.p, sor.p, res.p, so.ext.ii, so.ext.jj, (int)d, this->jpn, this->stream, so.ext.nb), "relax_lines");
         else
-            check(api2::relax_lines<T>(dir, so.p, so.nstncl, b.p, x.p, sor.p, so.ext.ii, so.ext.jj, (int)d, this->stream, so.ext.nb), "relax_lines");
+            check(api2::relax_lines<T>(dir, so.p, so.nstncl, b.p, x.p, sor.p, so.ext.ii, so.ext.jj, (int)d, this->stream, x.ext.nb, so.ext.rep), "relax_lines");
     }
 
 private:
@@ -103,7 +103,7 @@ struct hip_restrict : kernels::restriction<T> {
         if (this->jpn)
             check(api2::restrict_per<T>(R.p, fine.p, coarse.p, fine.ext.ii, fine.ext.jj, coarse.ext.ii, coarse.ext.jj, this->jpn, this->stream, fine.ext.nb), "restrict");
         else
-            check(api2::restrict_<T>(R.p, fine.p, coarse.p, fine.ext.ii, fine.ext.jj, coarse.ext.ii, coarse.ext.jj, this->stream, fine.ext.nb), "restrict");
+            check(api2::restrict_<T>(R.p, fine.p, coarse.p, fine.ext.ii, fine.ext.jj, coarse.ext.ii, coarse.ext.jj, this->stream, fine.ext.nb, R.ext.rep), "restrict");
     }
 };
 
@@ -115,7 +115,7 @@ struct hip_interp : kernels::interp_add<T> {
         if (this->jpn)
             check(api2::interp_add_per<T>(fine.p, coarse.p, residual.p, fop.p, fop.nstncl, P.p, coarse.ext.ii, coarse.ext.jj, fine.ext.ii, fine.ext.jj, this->jpn, this->stream, fine.ext.nb), "interp_add");
         else
-            check(api2::interp_add<T>(fine.p, coarse.p, residual.p, fop.p, fop.nstncl, P.p, coarse.ext.ii, coarse.ext.jj, fine.ext.ii, fine.ext.jj, this->stream, fine.ext.nb), "interp_add");
+            check(api2::interp_add<T>(fine.p, coarse.p, residual.p, fop.p, fop.nstncl, P.p, coarse.ext.ii, coarse.ext.jj, fine.ext.ii, fine.ext.jj, this->stream, fine.ext.nb, P.ext.rep), "interp_add");
     }
 };
 
@@ -134,7 +134,7 @@ struct hip_solve_cg : kernels::solve_cg<T> {
         if (this->jpn)
             check(api2::solve_cg_per<T>(x.p, b.p, abd.p, bbd, x.ext.ii, x.ext.jj, this->jpn, this->stream, x.ext.nb), "solve_cg");
         else
-            check(api2::solve_cg<T>(x.p, b.p, abd.p, bbd, x.ext.ii, x.ext.jj, this->stream, x.ext.nb), "solve_cg");
+            check(api2::solve_cg<T>(x.p, b.p, abd.p, bbd, x.ext.ii, x.ext.jj, this->stream, x.ext.nb, abd.ext.rep), "solve_cg");
     }
 
 private:
@@ -229,7 +229,14 @@ int cdr_hip_solver2d_create(const double *so_dev, int nstncl, int nx, int ny,
 int cdr_hip_solver2d_create_batched(const double *so_dev, int nstncl, int nx, int ny, int nb,
                                     const cdr_solver_settings *s, void *stream,
                                     cdr_hip_solver2d **out) {
-    if (!so_dev || !s || !out || nx < 1 || ny < 1 || nb < 1 || nb > 65535 ||
+    return cdr_hip_solver2d_create_shared(so_dev, nstncl, nx, ny, nb, 1, s, stream, out);
+}
+
+int cdr_hip_solver2d_create_shared(const double *so_dev, int nstncl, int nx, int ny, int nops,
+                                   int nrhs, const cdr_solver_settings *s, void *stream,
+                                   cdr_hip_solver2d **out) {
+    const long long nb = (long long)nops * nrhs;
+    if (!so_dev || !s || !out || nx < 1 || ny < 1 || nops < 1 || nrhs < 1 || nb > 65535 ||
         (nstncl != 3 && nstncl != 5) ||
         s->relaxation < CDR_RELAX_POINT || s->relaxation > CDR_RELAX_LINE_XY ||
         s->nrelax_pre < 0 || s->nrelax_post < 0 || s->maxiter < 0 || s->min_coarse < 2 ||
@@ -243,8 +250,9 @@ int cdr_hip_solver2d_create_batched(const double *so_dev, int nstncl, int nx, in
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CDR_ERR_NO_DEVICE;
     return guarded([&] {
         auto h = std::make_unique<cdr_hip_solver2d>();
-        h->impl = std::make_unique<cedar::hip::solver2d>(so_dev, nstncl, nx, ny, nb, *s,
-                                                         reinterpret_cast<hipStream_t>(stream));
+        h->impl = std::make_unique<cedar::hip::solver2d>(so_dev, nstncl, nx, ny, nops, *s,
+                                                         reinterpret_cast<hipStream_t>(stream),
+                                                         nrhs);
         *out = h.release();
         return (int)CDR_OK;
     });

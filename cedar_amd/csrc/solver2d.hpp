@@ -50,9 +50,10 @@ struct level_data {
 
 class solver2d {
 public:
-    // nb problems of the same extents: so_dev holds nb operators back to back
+    // nb operators of the same extents (so_dev holds them back to back), each with rep
+    // right-hand sides: nb * rep problems, problem b using operator b / rep
     solver2d(const double *so_dev, int nstncl, int nx, int ny, int nb,
-             const cdr_solver_settings &s, hipStream_t stream)
+             const cdr_solver_settings &s, hipStream_t stream, int rep = 1)
         : set_(s), stream_(stream), cur_(stream), km_(stream) {
         check_hip(hipEventCreate(&ev0_), "hipEventCreate");
         check_hip(hipEventCreate(&ev1_), "hipEventCreate");
@@ -63,7 +64,7 @@ public:
         if (s.periodic < 0 || s.periodic > 3) throw error(CDR_ERR_ARG, "invalid periodic mask");
         jpn_ = bc_of_mask[s.periodic];
         km_.set_bc(jpn_);
-        build(so_dev, nstncl, nx, ny, nb);
+        build(so_dev, nstncl, nx, ny, nb, rep);
     }
 
     ~solver2d() {
@@ -77,7 +78,8 @@ public:
     }
 
     int nlevels() const { return (int)L64_.size(); }
-    int nbatch() const { return L64_[0].ext.nb; }
+    int nbatch() const { return L64_[0].ext.nb; }  // problems (operators x right-hand sides)
+    int nrep() const { return rep_; }
     const level_data<double> &level(int l) const { return L64_[l]; }
     double setup_seconds() const { return setup_s_; }
     double solve_seconds() const { return solve_s_; }
@@ -159,8 +161,13 @@ public:
         };
         if (set_.precision == CDR_PREC_F64) {
             grid_func<double> x{x_dev, e}, b{const_cast<double *>(b_dev), e};
+            // b is fixed through the solve: its transpose for the finest level's y sweeps is made
+            // once here, and every cycle starts with it in place
+            double *bT0 = bT_target<double>(L64_[0]);
+            if (bT0) k2::transpose<double>(b.p, bT0, e.jj, e.ii, stream_, nb);
             check(api2::residual_norm(A.p, A.nstncl, b.p, x.p, L64_[0].res.p, e.ii, e.jj,
-                                      work_.data(), sumsq_.data(), stream_, nb), "residual_norm");
+                                      work_.data(), sumsq_.data(), stream_, nb, rep_),
+                  "residual_norm");
             read_norms();
             for (int p = 0; p < nb; ++p) res0[p] = std::sqrt(host_norm_[p]);
             for (int i = 0; i < set_.maxiter; ++i) {
@@ -172,10 +179,12 @@ public:
                                              hipMemcpyDeviceToDevice, stream_), "save x");
                     push_mask();
                 }
-                run_maybe_graph(g64_, x.p, b.p, [&] { run_cycle<double>(L64_, x, b); });
+                run_maybe_graph(g64_, x.p, b.p,
+                                [&] { run_cycle<double>(L64_, x, b, bT0 != nullptr); });
                 if (masked) k2::keep_inactive(x.p, xprev_.data(), active_.data(), e.points(), nb, stream_);
                 check(api2::residual_norm(A.p, A.nstncl, b.p, x.p, L64_[0].res.p, e.ii, e.jj,
-                                          work_.data(), sumsq_.data(), stream_, nb), "residual_norm");
+                                          work_.data(), sumsq_.data(), stream_, nb, rep_),
+                      "residual_norm");
                 read_norms();
                 if (!record(i)) break;
             }
@@ -184,8 +193,11 @@ public:
             //   r = b - A x (FP64), e = Vcycle32(r), x <- x + e, fused into one residual pass.
             double *xa = xa_.data(), *xb = xb_.data();
             level_data<float> &F = L32_[0];
+            // the outer step also writes the FP32 right-hand side transposed where the finest
+            // level's y sweeps run through the transposed grid
+            float *r32T = bT_target<float>(F);
             k2::residual_ir(A.p, A.nstncl, b_dev, x_dev, nullptr, xa, F.b.p, e.ii, e.jj,
-                            work_.data(), sumsq_.data(), stream_, nb);
+                            work_.data(), sumsq_.data(), stream_, nb, nullptr, rep_, r32T);
             check_hip(hipGetLastError(), "residual_ir");
             periodic_images(xa, e, nb);
             read_norms();
@@ -193,12 +205,12 @@ public:
             for (int i = 0; i < set_.maxiter; ++i) {
                 run_maybe_graph(g32_, F.x.p, F.b.p, [&] {
                     k2::fill<float>(F.x.p, 0.0f, e.total(), cur_);
-                    run_cycle<float>(L32_, F.x, F.b);
+                    run_cycle<float>(L32_, F.x, F.b, r32T != nullptr);
                 });
                 push_mask();
                 k2::residual_ir(A.p, A.nstncl, b_dev, xa, F.x.p, xb, F.b.p, e.ii, e.jj,
                                 work_.data(), sumsq_.data(), stream_, nb,
-                                nactive < nb ? active_.data() : nullptr);
+                                nactive < nb ? active_.data() : nullptr, rep_, r32T);
                 check_hip(hipGetLastError(), "residual_ir");
                 std::swap(xa, xb);
                 periodic_images(xa, e, nb);
@@ -231,6 +243,7 @@ private:
     }
 
     int jpn_ = 0;
+    int rep_ = 1;  // right-hand sides per operator
     cdr_solver_settings set_;
     hipStream_t stream_;
     hipStream_t cur_;                  // stream the cycle kernels go to (capture stream while capturing)
@@ -248,6 +261,7 @@ private:
     double *host_norm_ = nullptr;  // nb squared norms (pinned)
     // fused coarse-level cycle: first level (-1: none), largest side, device level descriptors
     int fuse_lvl_ = -1, fuse_side_ = 0;
+    std::size_t fuse_lds_ = 0;  // LDS bytes of the on-chip fused cycle (0: arrays in memory)
     dbuf<unsigned char> fuse64_, fuse32_;
     int *host_active_ = nullptr;   // nb flags (pinned)
     hipEvent_t ev0_{}, ev1_{};
@@ -302,17 +316,23 @@ private:
     }
 
     template <typename T>
+    // e: extents of the level's grid functions (e.nb problems); the operator-side arrays hold
+    // e.nb / rep entries, one per operator
     static void alloc_level(level_data<T> &L, extents e, int nstncl, bool own_op, bool need_ci,
-                            bool need_xb, hipStream_t s) {
+                            bool need_xb, hipStream_t s, int rep) {
         L.ext = e;
         const std::size_t np = e.total();
+        extents eo = e;
+        eo.nb = e.nb / rep;
+        eo.rep = rep;
+        const std::size_t npo = eo.total();
         if (own_op) {
-            L.so_buf = dbuf<T>(np * (std::size_t)nstncl, s);
-            L.A = stencil_op<T>{L.so_buf.data(), e, nstncl};
+            L.so_buf = dbuf<T>(npo * (std::size_t)nstncl, s);
+            L.A = stencil_op<T>{L.so_buf.data(), eo, nstncl};
         }
         if (need_ci) {
-            L.ci_buf = dbuf<T>(np * 8, s);
-            L.P = prolong_op<T>{L.ci_buf.data(), e, nullptr};
+            L.ci_buf = dbuf<T>(npo * 8, s);
+            L.P = prolong_op<T>{L.ci_buf.data(), eo, nullptr};
         }
         if (need_xb) {
             L.x_buf = dbuf<T>(np, s);
@@ -323,14 +343,19 @@ private:
         L.res_buf = dbuf<T>(np, s);
         L.res = grid_func<T>{L.res_buf.data(), e};
         for (int k = 0; k < 2; ++k) {
-            L.sor_buf[k] = dbuf<T>(np * 2, s);
-            L.SOR[k] = relax_stencil<T>{L.sor_buf[k].data(), e};
+            L.sor_buf[k] = dbuf<T>(npo * 2, s);
+            L.SOR[k] = relax_stencil<T>{L.sor_buf[k].data(), eo};
         }
     }
 
-    void build(const double *so_dev, int nstncl, int nx, int ny, int nb) {
-        if (!so_dev || (nstncl != 3 && nstncl != 5) || nx < 1 || ny < 1 || nb < 1 || nb > 65535)
+    void build(const double *so_dev, int nstncl, int nx, int ny, int nops, int rep) {
+        if (!so_dev || (nstncl != 3 && nstncl != 5) || nx < 1 || ny < 1 || nops < 1 || rep < 1 ||
+            (long long)nops * rep > 65535)
             throw error(CDR_ERR_ARG, "invalid fine operator");
+        if (rep > 1 && jpn_ != 0)
+            throw error(CDR_ERR_ARG, "shared operators are not supported with periodic boundaries");
+        rep_ = rep;
+        const int nb = nops * rep;  // problems
         int nl = compute_num_levels(nx, ny, set_.min_coarse);
         if (set_.num_levels > 0) {
             if (set_.num_levels > nl) throw error(CDR_ERR_LEVELS, "too many levels specified");
@@ -342,10 +367,11 @@ private:
         for (int l = 0; l < nl; ++l) {
             extents e{cx + 2, cy + 2, nb};
             if (l == 0) {
-                alloc_level(L64_[0], e, nstncl, false, false, false, stream_);
-                L64_[0].A = stencil_op<double>{const_cast<double *>(so_dev), e, nstncl};
+                alloc_level(L64_[0], e, nstncl, false, false, false, stream_, rep);
+                extents eo{cx + 2, cy + 2, nops, rep};
+                L64_[0].A = stencil_op<double>{const_cast<double *>(so_dev), eo, nstncl};
             } else {
-                alloc_level(L64_[l], e, 5, true, true, true, stream_);
+                alloc_level(L64_[l], e, 5, true, true, true, stream_, rep);
             }
             cx = (int)((cx - 1) / 2.0 + 1);
             cy = (int)((cy - 1) / 2.0 + 1);
@@ -354,7 +380,7 @@ private:
         // the coarse matrix: band ABD(ii, n), or dense ABD(n, n) when periodic
         // (include/cedar/2d/solver.h:116-121)
         const std::size_t ncg = (std::size_t)ec.nx() * ec.ny();
-        abd_ = dbuf<double>((jpn_ ? ncg : (std::size_t)ec.ii) * ncg * nb, stream_);
+        abd_ = dbuf<double>((jpn_ ? ncg : (std::size_t)ec.ii) * ncg * nops, stream_);
         bbd_ = dbuf<double>((std::size_t)ec.nx() * ec.ny() * nb, stream_);
         work_ = dbuf<double>(cdr_hip_norm_work2d(L64_[0].ext.ii, L64_[0].ext.jj) * nb, stream_);
         sumsq_ = dbuf<double>(nb, stream_);
@@ -396,23 +422,25 @@ private:
     grid_func<double> &abd_grid() {
         const extents ec = L64_.back().ext;
         const int ncg = ec.nx() * ec.ny();
-        abd_g_ = grid_func<double>{abd_.data(), extents{jpn_ ? ncg : ec.ii, ncg, ec.nb}};
+        abd_g_ = grid_func<double>{abd_.data(), extents{jpn_ ? ncg : ec.ii, ncg, ec.nb / rep_, rep_}};
         return abd_g_;
     }
 
     // The fused coarse-level cycle starts at the first level below the finest whose sides are at
-    // most 32, 64 or 128 points and covers it and every coarser level (it needs at least two of
-    // them).  One workgroup per problem is latency-bound on larger levels, where one launch per
-    // step over the whole GPU is faster; where the three starting levels differ, the cycle from
-    // the highest of them down is timed with each (captured graphs, replayed) and the fastest is
-    // kept.  The fused cycle is bitwise the unfused one, so the choice never changes a result.
+    // most 8, 16, 32, 64 or 128 points and covers it and every coarser level (it needs at least
+    // two of them).  One workgroup per problem is latency-bound on larger levels, where one launch
+    // per step over the whole GPU is faster (and a batch of fewer problems than the GPU has
+    // compute units leaves most of it idle); where the candidate starting levels differ, the
+    // cycle from the highest of them down is timed with each (captured graphs, replayed) and the
+    // fastest is kept.  The fused cycle is bitwise the unfused one, so the choice never changes a
+    // result.
     template <typename T>
     void setup_fused(std::vector<level_data<T>> &L, dbuf<unsigned char> &dst) {
         fuse_lvl_ = -1;
         if (!set_.fused_coarse || jpn_ != 0) return;  // the fused cycle has no periodic images
         const int nl = (int)L.size();
         std::vector<int> cand;
-        for (int side : {32, 64, 128}) {
+        for (int side : {8, 16, 32, 64, 128}) {
             for (int l = 1; l + 1 < nl; ++l)
                 if (std::max(L[l].ext.nx(), L[l].ext.ny()) <= side) {
                     if (std::find(cand.begin(), cand.end(), l) == cand.end()) cand.push_back(l);
@@ -454,7 +482,17 @@ private:
         check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
         fuse_lvl_ = lt;
         fuse_side_ = std::max(L[lt].ext.nx(), L[lt].ext.ny());
+        // the fused levels run out of LDS where all their arrays fit in one workgroup's share
+        std::vector<int> ii, jj;
+        for (int l = lt; l < nl; ++l) {
+            ii.push_back(L[l].ext.ii);
+            jj.push_back(L[l].ext.jj);
+        }
+        const std::size_t need = k2::fused_cycle_lds_bytes((int)sizeof(T), ii.data(), jj.data(),
+                                                           nl - lt);
+        fuse_lds_ = need <= kFusedLdsMax ? need : 0;
     }
+    static constexpr std::size_t kFusedLdsMax = 150 * 1024;  // of the 160 KiB per workgroup
 
     // Milliseconds per replay of what body() launches, captured as a graph on the capture stream
     // and replayed 3 times on the solver's stream after a warm-up; -1 if capture fails.
@@ -525,21 +563,22 @@ private:
                 if ((dir == 0 && r == CDR_RELAX_LINE_Y) || (dir == 1 && r == CDR_RELAX_LINE_X)) continue;
                 const relax_stencil<T> &S = D.SOR[r == CDR_RELAX_LINE_XY ? dir : 0];
                 if (k2::tune_lines<T>(dir, D.A.p, D.A.nstncl, f.data(), q.data(), S.p, D.ext.ii,
-                                      D.ext.jj, stream_, D.ext.nb) < 0)
+                                      D.ext.jj, stream_, D.ext.nb, rep_) < 0)
                     throw error(CDR_ERR_HIP, "line-kernel tuning failed");
             }
             if (r != CDR_RELAX_LINE_XY) continue;
             // y-lines through the transposed grid: kept only where it is faster
             const std::size_t nl = D.ext.total();
-            D.soT_buf = dbuf<T>(nl * (std::size_t)D.A.nstncl, stream_);
+            D.soT_buf = dbuf<T>(D.A.ext.total() * (std::size_t)D.A.nstncl, stream_);
             D.xT_buf = dbuf<T>(nl, stream_);
             D.bT_buf = dbuf<T>(nl, stream_);
             k2::transpose_op<T>(D.A.p, D.soT_buf.data(), D.A.nstncl, D.ext.ii, D.ext.jj, stream_,
-                                D.ext.nb);
+                                D.A.ext.nb);
             const int c = k2::tune_lines_ytr<T>(D.A.p, D.soT_buf.data(), D.A.nstncl, f.data(),
                                                 q.data(), D.SOR[1].p, D.xT_buf.data(),
                                                 D.bT_buf.data(), D.ext.ii, D.ext.jj, stream_,
-                                                D.ext.nb, set_.nrelax_pre + set_.nrelax_post);
+                                                D.ext.nb, set_.nrelax_pre + set_.nrelax_post,
+                                                rep_);
             if (c == -2) throw error(CDR_ERR_HIP, "line-kernel tuning failed");
             if (c < 0) {  // direct y-lines: the transposed workspace is not needed
                 D.soT_buf = dbuf<T>();
@@ -571,11 +610,11 @@ private:
         for (int l = 0; l < nl; ++l) {
             auto &D = L64_[l];
             auto &F = L32_[l];
-            alloc_level(F, D.ext, D.A.nstncl, true, l > 0, true, stream_);
-            const std::size_t np = D.ext.total();
-            k2::convert<double, float>(D.A.p, F.A.p, np * D.A.nstncl, stream_);
-            if (l > 0) k2::convert<double, float>(D.P.p, F.P.p, np * 8, stream_);
-            for (int k = 0; k < 2; ++k) k2::convert<double, float>(D.SOR[k].p, F.SOR[k].p, np * 2, stream_);
+            alloc_level(F, D.ext, D.A.nstncl, true, l > 0, true, stream_, rep_);
+            const std::size_t npo = D.A.ext.total();  // operator-side entries
+            k2::convert<double, float>(D.A.p, F.A.p, npo * D.A.nstncl, stream_);
+            if (l > 0) k2::convert<double, float>(D.P.p, F.P.p, npo * 8, stream_);
+            for (int k = 0; k < 2; ++k) k2::convert<double, float>(D.SOR[k].p, F.SOR[k].p, npo * 2, stream_);
         }
         for (int l = 1; l < nl; ++l) L32_[l].P.fine_op = &L32_[l - 1].A;
         check_hip(hipGetLastError(), "convert");
@@ -615,7 +654,8 @@ private:
                             *xT = false;
                         }
                         check(api2::relax_lines<T>(0, L.A.p, L.A.nstncl, b.p, x.p, L.SOR[0].p,
-                                                   L.ext.ii, L.ext.jj, (int)d, cur_, L.ext.nb),
+                                                   L.ext.ii, L.ext.jj, (int)d, cur_, L.ext.nb,
+                                                   rep_),
                               "relax_lines_x");
                     } else {
                         // the last sweep of a DOWN smoothing may leave the iterate transposed
@@ -625,7 +665,7 @@ private:
                         bT_ok = k2::relax_lines_y_auto<T>(
                             L.A.p, L.soT_buf.data(), L.A.nstncl, b.p, x.p, L.SOR[1].p,
                             L.xT_buf.data(), L.bT_buf.data(), L.ext.ii, L.ext.jj, (int)d, cur_,
-                            L.ext.nb, bT_ok, in_T, keep);
+                            L.ext.nb, bT_ok, in_T, keep, rep_);
                         if (xT) *xT = bT_ok && keep;
                         check_hip(hipGetLastError(), "relax_lines_y");
                     }
@@ -634,8 +674,10 @@ private:
         }
     }
 
+    // bT_ready: the finest level's transposed right-hand side (L[0].bT_buf) already holds b
     template <typename T>
-    void run_cycle(std::vector<level_data<T>> &L, grid_func<T> &x, const grid_func<T> &b) {
+    void run_cycle(std::vector<level_data<T>> &L, grid_func<T> &x, const grid_func<T> &b,
+                   bool bT_ready = false) {
         if (L.size() == 1) {
             km_.run<kernels::solve_cg<T>>(x, b, abd_grid(), bbd_.data());
             return;
@@ -643,7 +685,18 @@ private:
         if (set_.cycle == CDR_CYCLE_F)
             fmg<T>(L, 0, x, b);
         else
-            ncycle<T>(L, 0, x, b);
+            ncycle<T>(L, 0, x, b, bT_ready);
+    }
+
+    // the level's transposed right-hand side buffer when its y sweeps run through the transposed
+    // grid (so that a producer of b can write it there as well), else null
+    template <typename T>
+    T *bT_target(level_data<T> &D) {
+        if (jpn_ != 0 || set_.relaxation != CDR_RELAX_LINE_XY || !D.soT_buf.data()) return nullptr;
+        return k2::uses_transposed_y<T>(D.soT_buf.data(), D.A.nstncl, D.ext.ii, D.ext.jj,
+                                        D.ext.nb) >= 0
+                   ? D.bT_buf.data()
+                   : nullptr;
     }
 
     // fcycle::fmg_cycle (include/cedar/cycle/fcycle.h:49-83): restrict b itself, recurse, then
@@ -672,8 +725,10 @@ private:
         ncycle<T>(L, lvl, x, b);
     }
 
+    // bT_ready: L[lvl].bT_buf already holds the transpose of b (its producer wrote it)
     template <typename T>
-    void ncycle(std::vector<level_data<T>> &L, int lvl, grid_func<T> &x, const grid_func<T> &b) {
+    void ncycle(std::vector<level_data<T>> &L, int lvl, grid_func<T> &x, const grid_func<T> &b,
+                bool bT_ready = false) {
         if (lvl == fuse_lvl_ && !timing_ && x.p == L[lvl].x.p && b.p == L[lvl].b.p) {
             // this level and all coarser ones in one launch (bitwise the same V-cycle)
             const dbuf<unsigned char> &d = std::is_same<T, double>::value ? fuse64_ : fuse32_;
@@ -681,14 +736,14 @@ private:
                 k2::coarse_cycle<T>(reinterpret_cast<const k2::coarse_level<T> *>(d.data()),
                                     (int)L.size() - lvl, fuse_side_, set_.relaxation,
                                     set_.nrelax_pre, set_.nrelax_post, abd_.data(), bbd_.data(),
-                                    cur_, L[lvl].ext.nb);
+                                    cur_, L[lvl].ext.nb, rep_, fuse_lds_);
                 check_hip(hipGetLastError(), "coarse_cycle");
                 return;
             }
         }
         auto &F = L[lvl];
         auto &C = L[lvl + 1];
-        bool bT_ok = false;  // b is fixed for this visit of the level
+        bool bT_ok = bT_ready;  // b is fixed for this visit of the level
         // xT: between the smoothings the iterate may stay in the transposed layout of the y
         // sweeps (F.xT_buf), read there by the residual and updated there by interp_add
         bool xT = false;
@@ -697,22 +752,37 @@ private:
         timed(lvl, "residual", [&] {
             if (xT)
                 k2::residual_qT<T>(F.A.p, F.A.nstncl, b.p, F.xT_buf.data(), F.res.p, F.ext.ii,
-                                   F.ext.jj, cur_, F.ext.nb);
+                                   F.ext.jj, cur_, F.ext.nb, rep_);
             else
                 km_.run<kernels::residual<T>>(F.A, x, b, F.res);
         });
-        timed(lvl, "restrict", [&] { km_.run<kernels::restriction<T>>(C.P, F.res, C.b); });
-        k2::fill<T>(C.x.p, T(0), C.ext.total(), cur_);
+        // restriction, the coarse iterate's zero start and (where the coarse level's y sweeps run
+        // through the transposed grid) the transposed coarse right-hand side in one launch
+        bool cbT = false;
+        timed(lvl, "restrict", [&] {
+            if (jpn_ == 0) {
+                T *qcT = (lvl + 1 < (int)L.size() - 1 && lvl + 1 != fuse_lvl_) ? bT_target<T>(C)
+                                                                                : nullptr;
+                k2::restrict_cycle<T>(C.P.p, F.res.p, C.b.p, C.x.p, qcT, F.ext.ii, F.ext.jj,
+                                      C.ext.ii, C.ext.jj, cur_, F.ext.nb, rep_);
+                check_hip(hipGetLastError(), "restrict");
+                cbT = qcT != nullptr;
+            } else {
+                km_.run<kernels::restriction<T>>(C.P, F.res, C.b);
+                k2::fill<T>(C.x.p, T(0), C.ext.total(), cur_);
+            }
+        });
         if (lvl + 1 == (int)L.size() - 1)
             timed(lvl + 1, "coarse-solve", [&] {
                 km_.run<kernels::solve_cg<T>>(C.x, C.b, abd_grid(), bbd_.data());
             });
         else
-            ncycle<T>(L, lvl + 1, C.x, C.b);
+            ncycle<T>(L, lvl + 1, C.x, C.b, cbT);
         timed(lvl, "interp-add", [&] {
             if (xT)
                 k2::interp_add_qT<T>(F.xT_buf.data(), C.x.p, F.res.p, F.A.p, F.A.nstncl, C.P.p,
-                                     C.ext.ii, C.ext.jj, F.ext.ii, F.ext.jj, cur_, F.ext.nb);
+                                     C.ext.ii, C.ext.jj, F.ext.ii, F.ext.jj, cur_, F.ext.nb,
+                                     rep_);
             else
                 km_.run<kernels::interp_add<T>>(C.P, C.x, F.res, x);
         });

@@ -342,6 +342,17 @@ int cdr_hip_solver2d_create(const double *so_dev, int nstncl, int nx, int ny,
 int cdr_hip_solver2d_create_batched(const double *so_dev, int nstncl, int nx, int ny, int nb,
                                     const cdr_solver_settings *s, void *stream,
                                     cdr_hip_solver2d **out);
+/* Several right-hand sides per operator (no counterpart in the reference): nops operators of
+ * the same extents (so_dev as for the batched form), each solved for nrhs right-hand sides, so
+ * nops * nrhs problems (at most 65535).  b_dev / x_dev of solve and vcycle hold the problems back
+ * to back, operator-major: right-hand side r of operator k is problem k * nrhs + r.  The
+ * hierarchy is built once per operator; every cycle kernel covers all problems in one launch
+ * and reads each operator's stencil, interpolation and line factors once for its nrhs
+ * problems.  Each problem gets bitwise the iterate and history of its own single solve.  Not
+ * available with periodic boundaries (CDR_ERR_ARG). */
+int cdr_hip_solver2d_create_shared(const double *so_dev, int nstncl, int nx, int ny, int nops,
+                                   int nrhs, const cdr_solver_settings *s, void *stream,
+                                   cdr_hip_solver2d **out);
 /* solve(b, x) from the x given (x0 = 0 gives Cedar's solve(b)); rho_i = ||r_i|| / ||r_0|| go to
  * hist (host, up to maxhist per problem, problem b at hist[b * maxhist + i]), niter[b] gets the
  * cycles run for problem b.  A problem stops, keeping its iterate, once its rho < tol, so each

@@ -123,3 +123,42 @@ def test_batched_line_sweep_entry_points(C, direction, dtype, nine):
                                     sor.to(dtype).contiguous(), ud, direction)
         for k in range(3):
             assert torch.equal(got[k], want[k]), (k, ud)
+
+
+@pytest.mark.parametrize("relax,n,nine,precision", [
+    ("line-xy", 127, True, "mixed"), ("line-xy", 96, True, "f64"), ("point", 65, False, "f64")])
+def test_shared_operator_rhs_equal_individual_solves(C, relax, n, nine, precision):
+    """nrhs right-hand sides per operator (cdr_hip_solver2d_create_shared): the hierarchy is built
+    once per operator and every cycle kernel reads it once for the operator's right-hand sides;
+    each (operator, right-hand side) problem gets bitwise the iterate and history of its own
+    single solve, including problems that stop earlier than others."""
+    nops, nrhs = 2, 3
+    so, b0 = problems(n, nops, nine)
+    rng = np.random.default_rng(41)
+    rhs = np.zeros((nops, nrhs) + b0.shape[1:])
+    for k in range(nops):
+        for r in range(nrhs):
+            rhs[k, r, 1:-1, 1:-1] = rng.uniform(-1, 1, (n, n)) * (1.0 + 3.0 * r)
+    kw = dict(relaxation=relax, nrelax_pre=1, nrelax_post=1, maxiter=40, tol=1e-10,
+              precision=precision)
+    s = C.Solver(dev(so), nrhs=nrhs, **kw)
+    assert s.nb == nops * nrhs
+    x, h = s.solve(dev(rhs.reshape((nops * nrhs,) + b0.shape[1:])))
+    lens = set()
+    for k in range(nops):
+        single = C.Solver(dev(so[k]), **kw)
+        for r in range(nrhs):
+            xs, hs = single.solve(dev(rhs[k, r]))
+            assert np.array_equal(h[k * nrhs + r], hs), (k, r)
+            assert torch.equal(x[k * nrhs + r], xs), (k, r)
+            lens.add(len(hs))
+    assert min(lens) < 40
+
+
+def test_shared_operator_argument_errors(C):
+    so, b = problems(33, 2, True)
+    with pytest.raises(C.CedarHipError):
+        C.Solver(dev(so), nrhs=2, relaxation="point", periodic=(True, False))
+    s = C.Solver(dev(so), nrhs=2, relaxation="point")
+    with pytest.raises(ValueError):
+        s.solve(dev(b))  # 2 grids given, 4 problems

@@ -198,8 +198,9 @@ def test_fcycle_matches_oracle(C, relax, n, nine, precision):
         assert rel_l2(xg.cpu().numpy(), xo) <= 1e-10
     else:
         # inside iterative refinement the F-cycle solves for the correction, so the first cycle
-        # matches the plain F-cycle and the later ones keep reducing the residual
-        assert abs(hg[0] - ho[0]) <= 1e-5 * ho[0]
+        # matches the plain F-cycle up to the FP32 rounding of that correction (rho is relative to
+        # |b| here: a few units of 2^-24) and the later ones keep reducing the residual
+        assert abs(hg[0] - ho[0]) <= 2.0 ** -22
         assert hg[2] < hg[1] < hg[0]
         # run to the tolerance: the FP64 oracle residual of the returned x meets it
         g = C.Solver(dev(so), relaxation=relax, nrelax_pre=1, nrelax_post=1, maxiter=60,
