@@ -1896,11 +1896,14 @@ __global__ __launch_bounds__(1024) void coarse_cycle_k(const coarse_level<T> *__
             }
             if (l > 0)
                 for (std::size_t k = tid; k < 8 * P; k += NT) const_cast<T *>(d.ci)[k] = g.ci[k];
-            if (l == 0)
-                for (std::size_t k = tid; k < P; k += NT) {
-                    d.b[k] = g.b[k];
-                    d.x[k] = g.x[k];
-                }
+            // res and, below the first level, b start at zero as in memory: their ghosts are
+            // never written, and interp_add adds the ghost res of an even-sized level to the
+            // ghost of x
+            for (std::size_t k = tid; k < P; k += NT) {
+                d.res[k] = T(0);
+                d.b[k] = l == 0 ? g.b[k] : T(0);
+                if (l == 0) d.x[k] = g.x[k];
+            }
         }
         __syncthreads();
     }
