@@ -242,6 +242,48 @@ def poisson3(nx, ny, nz):
     return so
 
 
+def periodic_problem3(nx, ny, nz, periodic=(True, True, True)):
+    """Cedar's 3D periodic example (examples/basic-3d-mpi/periodic.cc:17-214) on one process:
+    7-point Poisson on nx x ny x nz points, periodic in the flagged directions (a periodic
+    direction of n points has period n, h = 1/n); the couplings are set up to the upper ghost
+    plane of a periodic direction, as the example does, and the lower ghost planes of the
+    operator and the ghosts of b hold the periodic images too.  b = 12 pi^2 sin(2 pi x)
+    sin(2 pi y) sin(2 pi z) hx hy hz; returns (so, b, exact) with exact = sin(2 pi x) sin(2 pi y)
+    sin(2 pi z) at x = i hx (i the array index)."""
+    px, py, pz = (bool(p) for p in periodic)
+    mx, my, mz = nx - px, ny - py, nz - pz
+    hx, hy, hz = 1.0 / (mx + 1), 1.0 / (my + 1), 1.0 / (mz + 1)
+    xh, yh, zh = hy * hz / hx, hx * hz / hy, hx * hy / hz
+    so = np.zeros((4, nz + 2, ny + 2, nx + 2))
+    ib, jb, kb = (1 if px else 2), (1 if py else 2), (1 if pz else 2)
+    ie, je, ke = nx + 1 + px, ny + 1 + py, nz + 1 + pz
+    so[PS3, 1:ke, jb:je, 1:ie] = 1.0 * yh
+    so[PW3, 1:ke, 1:je, ib:ie] = 1.0 * xh
+    so[B3, kb:ke, 1:je, 1:ie] = 1.0 * zh
+    so[P3, 1:nz + 1, 1:ny + 1, 1:nx + 1] = 2 * xh + 2 * yh + 2 * zh
+    def images(a):
+        if py:
+            a[..., 0, :] = a[..., ny, :]
+            a[..., ny + 1, :] = a[..., 1, :]
+        if px:
+            a[..., :, 0] = a[..., :, nx]
+            a[..., :, nx + 1] = a[..., :, 1]
+        if pz:
+            a[..., 0, :, :] = a[..., nz, :, :]
+            a[..., nz + 1, :, :] = a[..., 1, :, :]
+        return a
+
+    so = images(so)
+    Z, Y, X = np.meshgrid(np.arange(nz + 2) * hz, np.arange(ny + 2) * hy, np.arange(nx + 2) * hx,
+                          indexing="ij")
+    u = np.sin(2 * np.pi * X) * np.sin(2 * np.pi * Y) * np.sin(2 * np.pi * Z)
+    b = np.zeros((nz + 2, ny + 2, nx + 2))
+    b[1:-1, 1:-1, 1:-1] = (12 * np.pi ** 2 * u * (hx * hy * hz))[1:-1, 1:-1, 1:-1]
+    exact = np.zeros_like(b)
+    exact[1:-1, 1:-1, 1:-1] = u[1:-1, 1:-1, 1:-1]
+    return so, images(b), exact
+
+
 def fe3(nx, ny, nz):
     """All-ones 27-point stencil, centre 26 (src/3d/gallery.cc:104-187)."""
     so = np.zeros((14, nz + 2, ny + 2, nx + 2))

@@ -300,6 +300,366 @@ static double min4(double a, double b, double c, double d)
     return dmin2(dmin2(dmin2(a, b), c), d);
 }
 
+#define OI_PT_ARGS ORC3_OI_PT_ARGS
+
+/* lxyl, lxyr of coarse (ic,jc,kc): the fine x-edge point (i-1, j, k) */
+void orc3_oi_xl(OI_PT_ARGS)
+{
+    const double p0 = S3(i - 1, j, k, kp);
+    double a, b, c, ep;
+    if (nine) {
+        a = S3(i - 1, j + 1, k, kpnw) + S3(i - 1, j, k, kpw) + S3(i - 1, j, k, kpsw) +
+            S3(i - 1, j + 1, k, kbnw) + S3(i - 1, j, k, kbw) + S3(i - 1, j, k, kbsw) +
+            S3(i - 1, j + 1, k + 1, kbse) + S3(i - 1, j, k + 1, kbe) +
+            S3(i - 1, j, k + 1, kbne);
+        b = S3(i, j + 1, k, kpsw) + S3(i, j, k, kpw) + S3(i, j, k, kpnw) +
+            S3(i, j + 1, k, kbne) + S3(i, j, k, kbe) + S3(i, j, k, kbse) +
+            S3(i, j + 1, k + 1, kbsw) + S3(i, j, k + 1, kbw) + S3(i, j, k + 1, kbnw);
+        c = a + b + S3(i - 1, j, k, kps) + S3(i - 1, j + 1, k, kps) +
+            S3(i - 1, j + 1, k, kbn) + S3(i - 1, j, k, kb) + S3(i - 1, j, k, kbs) +
+            S3(i - 1, j + 1, k + 1, kbs) + S3(i - 1, j, k + 1, kb) +
+            S3(i - 1, j, k + 1, kbn);
+        ep = dmin2(fabs(a / p0), fabs(b / p0));
+    } else {
+        a = S3(i - 1, j, k, kpw);
+        b = S3(i, j, k, kpw);
+        ep = dmin2(fabs(a / p0), fabs(b) / p0);
+        c = a + b + S3(i - 1, j, k, kps) + S3(i - 1, j + 1, k, kps) + S3(i - 1, j, k, kb) +
+            S3(i - 1, j, k + 1, kb);
+    }
+    c = lump3(a, b, c, p0, ep);
+    CI3(ic, jc, kc, lxyl) = a / c;
+    CI3(ic, jc, kc, lxyr) = b / c;
+}
+
+/* lxya, lxyb of coarse (ic,jc,kc): the fine y-edge point (i, j-1, k) */
+void orc3_oi_ya(OI_PT_ARGS)
+{
+    const double p0 = S3(i, j - 1, k, kp);
+    double a, b, c;
+    if (nine) {
+        a = S3(i, j, k, kpnw) + S3(i, j, k, kps) + S3(i + 1, j, k, kpsw) +
+            S3(i, j, k, kbnw) + S3(i, j, k, kbn) + S3(i + 1, j, k, kbne) +
+            S3(i, j, k + 1, kbse) + S3(i, j, k + 1, kbs) + S3(i + 1, j, k + 1, kbsw);
+        b = S3(i, j - 1, k, kpsw) + S3(i, j - 1, k, kps) + S3(i + 1, j - 1, k, kpnw) +
+            S3(i, j - 1, k, kbsw) + S3(i, j - 1, k, kbs) + S3(i + 1, j - 1, k, kbse) +
+            S3(i, j - 1, k + 1, kbne) + S3(i, j - 1, k + 1, kbn) +
+            S3(i + 1, j - 1, k + 1, kbnw);
+        c = a + b + S3(i, j - 1, k, kpw) + S3(i + 1, j - 1, k, kpw) +
+            S3(i, j - 1, k, kbw) + S3(i, j - 1, k, kb) + S3(i + 1, j - 1, k, kbe) +
+            S3(i, j - 1, k + 1, kbe) + S3(i, j - 1, k + 1, kb) +
+            S3(i + 1, j - 1, k + 1, kbw);
+    } else {
+        a = S3(i, j, k, kps);
+        b = S3(i, j - 1, k, kps);
+        c = a + b + S3(i, j - 1, k, kpw) + S3(i + 1, j - 1, k, kpw) + S3(i, j - 1, k, kb) +
+            S3(i, j - 1, k + 1, kb);
+    }
+    const double ep = dmin2(fabs(a / p0), fabs(b / p0));
+    c = lump3(a, b, c, p0, ep);
+    CI3(ic, jc, kc, lxya) = a / c;
+    CI3(ic, jc, kc, lxyb) = b / c;
+}
+
+/* lxza, lxzb of coarse (ic,jc,kc): the fine z-edge point (i, j, k-1) */
+void orc3_oi_za(OI_PT_ARGS)
+{
+    const double p0 = S3(i, j, k - 1, kp);
+    double a, b, c;
+    if (nine) {
+        a = S3(i, j + 1, k, kbse) + S3(i, j + 1, k, kbs) + S3(i + 1, j + 1, k, kbsw) +
+            S3(i, j, k, kbe) + S3(i, j, k, kb) + S3(i + 1, j, k, kbw) +
+            S3(i, j, k, kbne) + S3(i, j, k, kbn) + S3(i + 1, j, k, kbnw);
+        b = S3(i, j + 1, k - 1, kbnw) + S3(i, j + 1, k - 1, kbn) +
+            S3(i + 1, j + 1, k - 1, kbne) + S3(i, j, k - 1, kbw) + S3(i, j, k - 1, kb) +
+            S3(i + 1, j, k - 1, kbe) + S3(i, j, k - 1, kbsw) + S3(i, j, k - 1, kbs) +
+            S3(i + 1, j, k - 1, kbse);
+        c = a + b + S3(i, j, k - 1, kpw) + S3(i + 1, j, k - 1, kpw) +
+            S3(i, j + 1, k - 1, kpnw) + S3(i, j + 1, k - 1, kps) +
+            S3(i + 1, j + 1, k - 1, kpsw) + S3(i, j, k - 1, kpsw) +
+            S3(i, j, k - 1, kps) + S3(i + 1, j, k - 1, kpnw);
+    } else {
+        a = S3(i, j, k, kb);
+        b = S3(i, j, k - 1, kb);
+        c = a + b + S3(i, j, k - 1, kpw) + S3(i + 1, j, k - 1, kpw) +
+            S3(i, j + 1, k - 1, kps) + S3(i, j, k - 1, kps);
+    }
+    const double ep = dmin2(fabs(a / p0), fabs(b / p0));
+    c = lump3(a, b, c, p0, ep);
+    CI3(ic, jc, kc, lxza) = a / c;
+    CI3(ic, jc, kc, lxzb) = b / c;
+}
+
+/* lxy corners of coarse (ic,jc,kc): the fine xy-face point (i-1, j-1, k) */
+void orc3_oi_xyf(OI_PT_ARGS)
+{
+    const double p0 = S3(i - 1, j - 1, k, kp);
+    double dnw, dn, dne, dw, de, dsw, ds, dse, ep;
+    if (nine) {
+        dnw = S3(i - 1, j, k, kpnw) + S3(i - 1, j, k, kbnw) + S3(i - 1, j, k + 1, kbse);
+        dn = S3(i - 1, j, k, kps) + S3(i - 1, j, k, kbn) + S3(i - 1, j, k + 1, kbs);
+        dne = S3(i, j, k, kpsw) + S3(i, j, k, kbne) + S3(i, j, k + 1, kbsw);
+        dw = S3(i - 1, j - 1, k, kpw) + S3(i - 1, j - 1, k, kbw) + S3(i - 1, j - 1, k + 1, kbe);
+        de = S3(i, j - 1, k, kpw) + S3(i, j - 1, k, kbe) + S3(i, j - 1, k + 1, kbw);
+        dsw = S3(i - 1, j - 1, k, kpsw) + S3(i - 1, j - 1, k, kbsw) +
+              S3(i - 1, j - 1, k + 1, kbne);
+        ds = S3(i - 1, j - 1, k, kps) + S3(i - 1, j - 1, k, kbs) + S3(i - 1, j - 1, k + 1, kbn);
+        dse = S3(i, j - 1, k, kpnw) + S3(i, j - 1, k, kbse) + S3(i, j - 1, k + 1, kbnw);
+        ep = min4(fabs((dsw + dw + dnw) / p0), fabs((dnw + dn + dne) / p0),
+                  fabs((dne + de + dse) / p0), fabs((dse + ds + dsw) / p0));
+    } else {
+        dn = S3(i - 1, j, k, kps);
+        dw = S3(i - 1, j - 1, k, kpw);
+        de = S3(i, j - 1, k, kpw);
+        ds = S3(i - 1, j - 1, k, kps);
+        dnw = dne = dsw = dse = 0.0;
+        ep = min4(fabs(dw / p0), fabs(dn / p0), fabs(de / p0), fabs(ds / p0));
+    }
+    double dp = nine ? dw + dnw + dn + dne + de + dse + ds + dsw : dw + dn + de + ds;
+    const double sum = p0 - S3(i - 1, j - 1, k, kb) - S3(i - 1, j - 1, k + 1, kb);
+    dp = 1.0 / lumpd(dp, sum, ep);
+    if (nine) {
+        CI3(ic, jc, kc, lxynw) = dp * (dnw + CI3(ic - 1, jc, kc, lxya) * dw + CI3(ic, jc, kc, lxyl) * dn);
+        CI3(ic, jc, kc, lxyne) = dp * (dne + CI3(ic, jc, kc, lxyr) * dn + CI3(ic, jc, kc, lxya) * de);
+        CI3(ic, jc, kc, lxyse) = dp * (dse + CI3(ic, jc, kc, lxyb) * de + CI3(ic, jc - 1, kc, lxyr) * ds);
+        CI3(ic, jc, kc, lxysw) = dp * (dsw + CI3(ic, jc - 1, kc, lxyl) * ds + CI3(ic - 1, jc, kc, lxyb) * dw);
+    } else {
+        CI3(ic, jc, kc, lxynw) = dp * (CI3(ic - 1, jc, kc, lxya) * dw + CI3(ic, jc, kc, lxyl) * dn);
+        CI3(ic, jc, kc, lxyne) = dp * (CI3(ic, jc, kc, lxyr) * dn + CI3(ic, jc, kc, lxya) * de);
+        CI3(ic, jc, kc, lxyse) = dp * (CI3(ic, jc, kc, lxyb) * de + CI3(ic, jc - 1, kc, lxyr) * ds);
+        CI3(ic, jc, kc, lxysw) = dp * (CI3(ic, jc - 1, kc, lxyl) * ds + CI3(ic - 1, jc, kc, lxyb) * dw);
+    }
+}
+
+/* lxz corners of coarse (ic,jc,kc): the fine xz-face point (i-1, j, k-1) */
+void orc3_oi_xzf(OI_PT_ARGS)
+{
+    const double p0 = S3(i - 1, j, k - 1, kp);
+    double dnw, dn, dne, dw, de, dsw, ds, dse, ep;
+    if (nine) {
+        dnw = S3(i - 1, j + 1, k, kbse) + S3(i - 1, j, k, kbe) + S3(i - 1, j, k, kbne);
+        dn = S3(i - 1, j + 1, k, kbs) + S3(i - 1, j, k, kb) + S3(i - 1, j, k, kbn);
+        dne = S3(i, j + 1, k, kbsw) + S3(i, j, k, kbw) + S3(i, j, k, kbnw);
+        dw = S3(i - 1, j + 1, k - 1, kpnw) + S3(i - 1, j, k - 1, kpw) + S3(i - 1, j, k - 1, kpsw);
+        de = S3(i, j + 1, k - 1, kpsw) + S3(i, j, k - 1, kpw) + S3(i, j, k - 1, kpnw);
+        dsw = S3(i - 1, j + 1, k - 1, kbnw) + S3(i - 1, j, k - 1, kbw) +
+              S3(i - 1, j, k - 1, kbsw);
+        ds = S3(i - 1, j + 1, k - 1, kbn) + S3(i - 1, j, k - 1, kb) + S3(i - 1, j, k - 1, kbs);
+        dse = S3(i, j + 1, k - 1, kbne) + S3(i, j, k - 1, kbe) + S3(i, j, k - 1, kbse);
+        ep = min4(fabs((dsw + dw + dnw) / p0), fabs((dnw + dn + dne) / p0),
+                  fabs((dne + de + dse) / p0), fabs((dse + ds + dsw) / p0));
+    } else {
+        dn = S3(i - 1, j, k, kb);
+        dw = S3(i - 1, j, k - 1, kpw);
+        de = S3(i, j, k - 1, kpw);
+        ds = S3(i - 1, j, k - 1, kb);
+        dnw = dne = dsw = dse = 0.0;
+        ep = min4(fabs(dw / p0), fabs(dn / p0), fabs(de / p0), fabs(ds / p0));
+    }
+    double dp = nine ? dw + dnw + dn + dne + de + dse + ds + dsw : dw + dn + de + ds;
+    const double sum = p0 - S3(i - 1, j + 1, k - 1, kps) - S3(i - 1, j, k - 1, kps);
+    dp = 1.0 / lumpd(dp, sum, ep);
+    if (nine) {
+        CI3(ic, jc, kc, lxznw) = dp * (dnw + CI3(ic - 1, jc, kc, lxza) * dw + CI3(ic, jc, kc, lxyl) * dn);
+        CI3(ic, jc, kc, lxzne) = dp * (dne + CI3(ic, jc, kc, lxyr) * dn + CI3(ic, jc, kc, lxza) * de);
+        CI3(ic, jc, kc, lxzse) = dp * (dse + CI3(ic, jc, kc, lxzb) * de + CI3(ic, jc, kc - 1, lxyr) * ds);
+        CI3(ic, jc, kc, lxzsw) = dp * (dsw + CI3(ic, jc, kc - 1, lxyl) * ds + CI3(ic - 1, jc, kc, lxzb) * dw);
+    } else {
+        CI3(ic, jc, kc, lxznw) = dp * (CI3(ic - 1, jc, kc, lxza) * dw + CI3(ic, jc, kc, lxyl) * dn);
+        CI3(ic, jc, kc, lxzne) = dp * (CI3(ic, jc, kc, lxyr) * dn + CI3(ic, jc, kc, lxza) * de);
+        CI3(ic, jc, kc, lxzse) = dp * (CI3(ic, jc, kc, lxzb) * de + CI3(ic, jc, kc - 1, lxyr) * ds);
+        CI3(ic, jc, kc, lxzsw) = dp * (CI3(ic, jc, kc - 1, lxyl) * ds + CI3(ic - 1, jc, kc, lxzb) * dw);
+    }
+}
+
+/* lyz corners of coarse (ic,jc,kc): the fine yz-face point (i, j-1, k-1) */
+void orc3_oi_yzf(OI_PT_ARGS)
+{
+    const double p0 = S3(i, j - 1, k - 1, kp);
+    double dnw, dn, dne, dw, de, dsw, ds, dse, ep;
+    if (nine) {
+        dnw = S3(i, j, k, kbse) + S3(i, j, k, kbs) + S3(i + 1, j, k, kbsw);
+        dn = S3(i, j - 1, k, kbe) + S3(i, j - 1, k, kb) + S3(i + 1, j - 1, k, kbw);
+        dne = S3(i, j - 1, k, kbne) + S3(i, j - 1, k, kbn) + S3(i + 1, j - 1, k, kbnw);
+        dw = S3(i, j, k - 1, kpnw) + S3(i, j, k - 1, kps) + S3(i + 1, j, k - 1, kpsw);
+        de = S3(i, j - 1, k - 1, kpsw) + S3(i, j - 1, k - 1, kps) + S3(i + 1, j - 1, k - 1, kpnw);
+        dsw = S3(i, j, k - 1, kbnw) + S3(i, j, k - 1, kbn) + S3(i + 1, j, k - 1, kbne);
+        ds = S3(i, j - 1, k - 1, kbw) + S3(i, j - 1, k - 1, kb) + S3(i + 1, j - 1, k - 1, kbe);
+        dse = S3(i, j - 1, k - 1, kbsw) + S3(i, j - 1, k - 1, kbs) +
+              S3(i + 1, j - 1, k - 1, kbse);
+        ep = min4(fabs((dsw + dw + dnw) / p0), fabs((dnw + dn + dne) / p0),
+                  fabs((dne + de + dse) / p0), fabs((dse + ds + dsw) / p0));
+    } else {
+        dn = S3(i, j - 1, k, kb);
+        dw = S3(i, j, k - 1, kps);
+        de = S3(i, j - 1, k - 1, kps);
+        ds = S3(i, j - 1, k - 1, kb);
+        dnw = dne = dsw = dse = 0.0;
+        ep = min4(fabs(dw / p0), fabs(dn / p0), fabs(de / p0), fabs(ds / p0));
+    }
+    double dp = nine ? dw + dnw + dn + dne + de + dse + ds + dsw : dw + dn + de + ds;
+    const double sum = p0 - S3(i, j - 1, k - 1, kpw) - S3(i + 1, j - 1, k - 1, kpw);
+    dp = 1.0 / lumpd(dp, sum, ep);
+    if (nine) {
+        CI3(ic, jc, kc, lyznw) = dp * (dnw + CI3(ic, jc, kc, lxza) * dw + CI3(ic, jc, kc, lxya) * dn);
+        CI3(ic, jc, kc, lyzne) = dp * (dne + CI3(ic, jc, kc, lxyb) * dn + CI3(ic, jc - 1, kc, lxza) * de);
+        CI3(ic, jc, kc, lyzse) = dp * (dse + CI3(ic, jc - 1, kc, lxzb) * de + CI3(ic, jc, kc - 1, lxyb) * ds);
+        CI3(ic, jc, kc, lyzsw) = dp * (dsw + CI3(ic, jc, kc - 1, lxya) * ds + CI3(ic, jc, kc, lxzb) * dw);
+    } else {
+        CI3(ic, jc, kc, lyznw) = dp * (CI3(ic, jc, kc, lxza) * dw + CI3(ic, jc, kc, lxya) * dn);
+        CI3(ic, jc, kc, lyzne) = dp * (CI3(ic, jc, kc, lxyb) * dn + CI3(ic, jc - 1, kc, lxza) * de);
+        CI3(ic, jc, kc, lyzse) = dp * (CI3(ic, jc - 1, kc, lxzb) * de + CI3(ic, jc, kc - 1, lxyb) * ds);
+        CI3(ic, jc, kc, lyzsw) = dp * (CI3(ic, jc, kc - 1, lxya) * ds + CI3(ic, jc, kc, lxzb) * dw);
+    }
+}
+
+/* lt*, lb* of coarse (ic,jc,kc): the fine cell-centre point (i-1, j-1, k-1) */
+void orc3_oi_cell(OI_PT_ARGS)
+{
+    const double p0 = S3(i - 1, j - 1, k - 1, kp);
+    if (nine) {
+        double yp = S3(i - 1, j - 1, k - 1, kpw) + S3(i - 1, j, k - 1, kpnw) +
+                    S3(i - 1, j, k - 1, kps) + S3(i, j, k - 1, kpsw) +
+                    S3(i, j - 1, k - 1, kpw) + S3(i, j - 1, k - 1, kpnw) +
+                    S3(i - 1, j - 1, k - 1, kps) + S3(i - 1, j - 1, k - 1, kpsw) +
+                    S3(i - 1, j - 1, k - 1, kb) + S3(i - 1, j - 1, k - 1, kbw) +
+                    S3(i - 1, j, k - 1, kbnw) + S3(i - 1, j, k - 1, kbn) +
+                    S3(i, j, k - 1, kbne) + S3(i, j - 1, k - 1, kbe) +
+                    S3(i, j - 1, k - 1, kbse) + S3(i - 1, j - 1, k - 1, kbs) +
+                    S3(i - 1, j - 1, k - 1, kbsw) + S3(i - 1, j - 1, k, kb) +
+                    S3(i - 1, j - 1, k, kbe) + S3(i - 1, j, k, kbse) +
+                    S3(i - 1, j, k, kbs) + S3(i, j, k, kbsw) + S3(i, j - 1, k, kbw) +
+                    S3(i, j - 1, k, kbnw) + S3(i - 1, j - 1, k, kbn) +
+                    S3(i - 1, j - 1, k, kbne);
+        /* as in the reference, the third and fifth sums are not divided by SO_p,
+           and kbse enters the fourth sum twice */
+        double ye = min4(
+            fabs(S3(i - 1, j - 1, k - 1, kpw) + S3(i - 1, j, k - 1, kpnw) +
+                 S3(i - 1, j, k, kbse) + S3(i - 1, j - 1, k, kbe) +
+                 S3(i - 1, j - 1, k, kbne) + S3(i - 1, j - 1, k - 1, kpsw) +
+                 S3(i - 1, j - 1, k - 1, kbsw) + S3(i - 1, j - 1, k - 1, kbw) +
+                 S3(i - 1, j, k - 1, kbnw)) / p0,
+            fabs(S3(i, j - 1, k - 1, kpw) + S3(i, j, k - 1, kpsw) + S3(i, j, k, kbsw) +
+                 S3(i, j - 1, k, kbw) + S3(i, j - 1, k, kbnw) + S3(i, j - 1, k - 1, kpnw) +
+                 S3(i, j - 1, k - 1, kbse) + S3(i, j - 1, k - 1, kbe) +
+                 S3(i, j, k - 1, kbne)) / p0,
+            fabs(S3(i - 1, j, k - 1, kps) + S3(i - 1, j, k - 1, kpnw) +
+                 S3(i - 1, j, k, kbse) + S3(i - 1, j, k, kbs) + S3(i, j, k, kbsw) +
+                 S3(i, j, k - 1, kpsw) + S3(i, j, k - 1, kbne) + S3(i - 1, j, k - 1, kbn) +
+                 S3(i - 1, j, k - 1, kbnw)),
+            fabs(S3(i - 1, j - 1, k - 1, kps) + S3(i - 1, j - 1, k - 1, kpsw) +
+                 S3(i - 1, j - 1, k, kbne) + S3(i - 1, j - 1, k, kbn) +
+                 S3(i, j - 1, k, kbnw) + S3(i, j - 1, k - 1, kpnw) +
+                 S3(i, j - 1, k - 1, kbse) + S3(i - 1, j - 1, k - 1, kbs) +
+                 S3(i, j - 1, k - 1, kbse)) / p0);
+        ye = dmin2(dmin2(ye,
+                         fabs(S3(i - 1, j - 1, k - 1, kb) + S3(i - 1, j - 1, k - 1, kbw) +
+                              S3(i - 1, j, k - 1, kbnw) + S3(i - 1, j, k - 1, kbn) +
+                              S3(i, j, k - 1, kbne) + S3(i, j - 1, k - 1, kbe) +
+                              S3(i, j - 1, k - 1, kbse) + S3(i - 1, j - 1, k - 1, kbs) +
+                              S3(i - 1, j - 1, k - 1, kbsw))),
+                   fabs(S3(i - 1, j - 1, k, kb) + S3(i - 1, j - 1, k, kbe) +
+                        S3(i - 1, j, k, kbse) + S3(i - 1, j, k, kbs) + S3(i, j, k, kbsw) +
+                        S3(i, j - 1, k, kbw) + S3(i, j - 1, k, kbnw) +
+                        S3(i - 1, j - 1, k, kbn) + S3(i - 1, j - 1, k, kbne)) / p0);
+        yp = lumpd(yp, p0, ye);
+        yp = 1.0 / yp;
+        CI3(ic, jc, kc, ltnw) = yp * (S3(i - 1, j, k, kbse) +
+            CI3(ic - 1, jc, kc, lyznw) * S3(i - 1, j - 1, k - 1, kpw) +
+            CI3(ic - 1, jc, kc, lxza) * S3(i - 1, j, k - 1, kpnw) +
+            CI3(ic, jc, kc, lxznw) * S3(i - 1, j, k - 1, kps) +
+            CI3(ic - 1, jc, kc, lxya) * S3(i - 1, j - 1, k, kbe) +
+            CI3(ic, jc, kc, lxyl) * S3(i - 1, j, k, kbs) +
+            CI3(ic, jc, kc, lxynw) * S3(i - 1, j - 1, k, kb));
+        CI3(ic, jc, kc, ltne) = yp * (S3(i, j, k, kbsw) +
+            CI3(ic, jc, kc, lxzne) * S3(i - 1, j, k - 1, kps) +
+            CI3(ic, jc, kc, lxza) * S3(i, j, k - 1, kpsw) +
+            CI3(ic, jc, kc, lyznw) * S3(i, j - 1, k - 1, kpw) +
+            CI3(ic, jc, kc, lxyr) * S3(i - 1, j, k, kbs) +
+            CI3(ic, jc, kc, lxya) * S3(i, j - 1, k, kbw) +
+            CI3(ic, jc, kc, lxyne) * S3(i - 1, j - 1, k, kb));
+        CI3(ic, jc, kc, lbnw) = yp * (S3(i - 1, j, k - 1, kbnw) +
+            CI3(ic - 1, jc, kc - 1, lxya) * S3(i - 1, j - 1, k - 1, kbw) +
+            CI3(ic, jc, kc - 1, lxyl) * S3(i - 1, j, k - 1, kbn) +
+            CI3(ic, jc, kc - 1, lxynw) * S3(i - 1, j - 1, k - 1, kb) +
+            CI3(ic - 1, jc, kc, lyzsw) * S3(i - 1, j - 1, k - 1, kpw) +
+            CI3(ic - 1, jc, kc, lxzb) * S3(i - 1, j, k - 1, kpnw) +
+            CI3(ic, jc, kc, lxzsw) * S3(i - 1, j, k - 1, kps));
+        CI3(ic, jc, kc, lbne) = yp * (S3(i, j, k - 1, kbne) +
+            CI3(ic, jc, kc - 1, lxyne) * S3(i - 1, j - 1, k - 1, kb) +
+            CI3(ic, jc, kc - 1, lxyr) * S3(i - 1, j, k - 1, kbn) +
+            CI3(ic, jc, kc - 1, lxya) * S3(i, j - 1, k - 1, kbe) +
+            CI3(ic, jc, kc, lxzse) * S3(i - 1, j, k - 1, kps) +
+            CI3(ic, jc, kc, lxzb) * S3(i, j, k - 1, kpsw) +
+            CI3(ic, jc, kc, lyzsw) * S3(i, j - 1, k - 1, kpw));
+        CI3(ic, jc, kc, lbsw) = yp * (S3(i - 1, j - 1, k - 1, kbsw) +
+            CI3(ic - 1, jc, kc - 1, lxyb) * S3(i - 1, j - 1, k - 1, kbw) +
+            CI3(ic, jc, kc - 1, lxysw) * S3(i - 1, j - 1, k - 1, kb) +
+            CI3(ic, jc - 1, kc - 1, lxyl) * S3(i - 1, j - 1, k - 1, kbs) +
+            CI3(ic - 1, jc, kc, lyzse) * S3(i - 1, j - 1, k - 1, kpw) +
+            CI3(ic, jc - 1, kc, lxzsw) * S3(i - 1, j - 1, k - 1, kps) +
+            CI3(ic - 1, jc - 1, kc, lxzb) * S3(i - 1, j - 1, k - 1, kpsw));
+        CI3(ic, jc, kc, ltsw) = yp * (S3(i - 1, j - 1, k, kbne) +
+            CI3(ic - 1, jc, kc, lxyb) * S3(i - 1, j - 1, k, kbe) +
+            CI3(ic, jc, kc, lxysw) * S3(i - 1, j - 1, k, kb) +
+            CI3(ic, jc - 1, kc, lxyl) * S3(i - 1, j - 1, k, kbn) +
+            CI3(ic - 1, jc, kc, lyzne) * S3(i - 1, j - 1, k - 1, kpw) +
+            CI3(ic, jc - 1, kc, lxznw) * S3(i - 1, j - 1, k - 1, kps) +
+            CI3(ic - 1, jc - 1, kc, lxza) * S3(i - 1, j - 1, k - 1, kpsw));
+        CI3(ic, jc, kc, ltse) = yp * (S3(i, j - 1, k, kbnw) +
+            CI3(ic, jc - 1, kc, lxyr) * S3(i - 1, j - 1, k, kbn) +
+            CI3(ic, jc, kc, lxyse) * S3(i - 1, j - 1, k, kb) +
+            CI3(ic, jc, kc, lxyb) * S3(i, j - 1, k, kbw) +
+            CI3(ic, jc - 1, kc, lxzne) * S3(i - 1, j - 1, k - 1, kps) +
+            CI3(ic, jc, kc, lyzne) * S3(i, j - 1, k - 1, kpw) +
+            CI3(ic, jc - 1, kc, lxza) * S3(i, j - 1, k - 1, kpnw));
+        CI3(ic, jc, kc, lbse) = yp * (S3(i, j - 1, k - 1, kbse) +
+            CI3(ic, jc - 1, kc - 1, lxyr) * S3(i - 1, j - 1, k - 1, kbs) +
+            CI3(ic, jc, kc - 1, lxyse) * S3(i - 1, j - 1, k - 1, kb) +
+            CI3(ic, jc, kc - 1, lxyb) * S3(i, j - 1, k - 1, kbe) +
+            CI3(ic, jc - 1, kc, lxzse) * S3(i - 1, j - 1, k - 1, kps) +
+            CI3(ic, jc, kc, lyzse) * S3(i, j - 1, k - 1, kpw) +
+            CI3(ic, jc - 1, kc, lxzb) * S3(i, j - 1, k - 1, kpnw));
+    } else {
+        double dp = S3(i - 1, j - 1, k - 1, kpw) + S3(i - 1, j, k - 1, kps) +
+                    S3(i, j - 1, k - 1, kpw) + S3(i - 1, j - 1, k - 1, kps) +
+                    S3(i - 1, j - 1, k - 1, kb) + S3(i - 1, j - 1, k, kb);
+        const double ep = dmin2(
+            dmin2(min4(fabs(S3(i - 1, j - 1, k - 1, kpw) / p0), fabs(S3(i - 1, j, k - 1, kps) / p0),
+                       fabs(S3(i, j - 1, k - 1, kpw) / p0), fabs(S3(i - 1, j - 1, k - 1, kps) / p0)),
+                  fabs(S3(i - 1, j - 1, k - 1, kb) / p0)),
+            fabs(S3(i - 1, j - 1, k, kb) / p0));
+        /* here the reference adds the correction before dp */
+        dp = (p0 - dp) * dmax2(p0 - (1.0 + ep) * dp, 0.0) /
+                 (fabs(p0 - (1.0 + ep) * dp) + EMACH) + dp;
+        dp = 1.0 / dp;
+        CI3(ic, jc, kc, ltnw) = dp * (CI3(ic - 1, jc, kc, lyznw) * S3(i - 1, j - 1, k - 1, kpw) +
+                                      CI3(ic, jc, kc, lxznw) * S3(i - 1, j, k - 1, kps) +
+                                      CI3(ic, jc, kc, lxynw) * S3(i - 1, j - 1, k, kb));
+        CI3(ic, jc, kc, ltne) = dp * (CI3(ic, jc, kc, lxzne) * S3(i - 1, j, k - 1, kps) +
+                                      CI3(ic, jc, kc, lyznw) * S3(i, j - 1, k - 1, kpw) +
+                                      CI3(ic, jc, kc, lxyne) * S3(i - 1, j - 1, k, kb));
+        CI3(ic, jc, kc, lbnw) = dp * (CI3(ic, jc, kc - 1, lxynw) * S3(i - 1, j - 1, k - 1, kb) +
+                                      CI3(ic - 1, jc, kc, lyzsw) * S3(i - 1, j - 1, k - 1, kpw) +
+                                      CI3(ic, jc, kc, lxzsw) * S3(i - 1, j, k - 1, kps));
+        CI3(ic, jc, kc, lbne) = dp * (CI3(ic, jc, kc - 1, lxyne) * S3(i - 1, j - 1, k - 1, kb) +
+                                      CI3(ic, jc, kc, lxzse) * S3(i - 1, j, k - 1, kps) +
+                                      CI3(ic, jc, kc, lyzsw) * S3(i, j - 1, k - 1, kpw));
+        CI3(ic, jc, kc, lbsw) = dp * (CI3(ic, jc, kc - 1, lxysw) * S3(i - 1, j - 1, k - 1, kb) +
+                                      CI3(ic - 1, jc, kc, lyzse) * S3(i - 1, j - 1, k - 1, kpw) +
+                                      CI3(ic, jc - 1, kc, lxzsw) * S3(i - 1, j - 1, k - 1, kps));
+        CI3(ic, jc, kc, ltsw) = dp * (CI3(ic, jc, kc, lxysw) * S3(i - 1, j - 1, k, kb) +
+                                      CI3(ic - 1, jc, kc, lyzne) * S3(i - 1, j - 1, k - 1, kpw) +
+                                      CI3(ic, jc - 1, kc, lxznw) * S3(i - 1, j - 1, k - 1, kps));
+        CI3(ic, jc, kc, ltse) = dp * (CI3(ic, jc, kc, lxyse) * S3(i - 1, j - 1, k, kb) +
+                                      CI3(ic, jc - 1, kc, lxzne) * S3(i - 1, j - 1, k - 1, kps) +
+                                      CI3(ic, jc, kc, lyzne) * S3(i, j - 1, k - 1, kpw));
+        CI3(ic, jc, kc, lbse) = dp * (CI3(ic, jc, kc - 1, lxyse) * S3(i - 1, j - 1, k - 1, kb) +
+                                      CI3(ic, jc - 1, kc, lxzse) * S3(i - 1, j - 1, k - 1, kps) +
+                                      CI3(ic, jc, kc, lyzse) * S3(i, j - 1, k - 1, kpw));
+    }
+}
+
 void orc3_setup_interp(const double *so, int nstncl, double *ci, int iif, int jjf, int kkf,
                        int iic, int jjc, int kkc)
 {
@@ -308,397 +668,66 @@ void orc3_setup_interp(const double *so, int nstncl, double *ci, int iif, int jj
     const int iic1 = iic - 1, jjc1 = jjc - 1, kkc1 = kkc - 1;
     const int iicf1 = (iif - 2) / 2 + 2, jjcf1 = (jjf - 2) / 2 + 2, kkcf1 = (kkf - 2) / 2 + 2;
     const int nine = nstncl == 14;
-
-    /* x-edges on coarse planes: fine (2ic-3, 2(jc-1), 2(kc-1)) */
     for (int kc = 2; kc <= kkc1; ++kc) {
         const int k = 2 * (kc - 1);
         for (int jc = 2; jc <= jjc1; ++jc) {
             const int j = 2 * (jc - 1);
             for (int ic = 3; ic <= iicf1; ++ic) {
-                const int i = 2 * (ic - 1);
-                const double p0 = S3(i - 1, j, k, kp);
-                double a, b, c, ep;
-                if (nine) {
-                    a = S3(i - 1, j + 1, k, kpnw) + S3(i - 1, j, k, kpw) + S3(i - 1, j, k, kpsw) +
-                        S3(i - 1, j + 1, k, kbnw) + S3(i - 1, j, k, kbw) + S3(i - 1, j, k, kbsw) +
-                        S3(i - 1, j + 1, k + 1, kbse) + S3(i - 1, j, k + 1, kbe) +
-                        S3(i - 1, j, k + 1, kbne);
-                    b = S3(i, j + 1, k, kpsw) + S3(i, j, k, kpw) + S3(i, j, k, kpnw) +
-                        S3(i, j + 1, k, kbne) + S3(i, j, k, kbe) + S3(i, j, k, kbse) +
-                        S3(i, j + 1, k + 1, kbsw) + S3(i, j, k + 1, kbw) + S3(i, j, k + 1, kbnw);
-                    c = a + b + S3(i - 1, j, k, kps) + S3(i - 1, j + 1, k, kps) +
-                        S3(i - 1, j + 1, k, kbn) + S3(i - 1, j, k, kb) + S3(i - 1, j, k, kbs) +
-                        S3(i - 1, j + 1, k + 1, kbs) + S3(i - 1, j, k + 1, kb) +
-                        S3(i - 1, j, k + 1, kbn);
-                    ep = dmin2(fabs(a / p0), fabs(b / p0));
-                } else {
-                    a = S3(i - 1, j, k, kpw);
-                    b = S3(i, j, k, kpw);
-                    ep = dmin2(fabs(a / p0), fabs(b) / p0);
-                    c = a + b + S3(i - 1, j, k, kps) + S3(i - 1, j + 1, k, kps) + S3(i - 1, j, k, kb) +
-                        S3(i - 1, j, k + 1, kb);
-                }
-                c = lump3(a, b, c, p0, ep);
-                CI3(ic, jc, kc, lxyl) = a / c;
-                CI3(ic, jc, kc, lxyr) = b / c;
+                orc3_oi_xl(so, nine, ci, ii, jj, P, iic, jjc, Pc, 2 * (ic - 1), j, k, ic, jc, kc);
             }
         }
     }
-    /* y-edges on coarse planes: fine (2(ic-1), 2jc-3, 2(kc-1)) */
     for (int kc = 2; kc <= kkc1; ++kc) {
         const int k = 2 * (kc - 1);
         for (int jc = 3; jc <= jjcf1; ++jc) {
             const int j = 2 * (jc - 1);
             for (int ic = 2; ic <= iic1; ++ic) {
-                const int i = 2 * (ic - 1);
-                const double p0 = S3(i, j - 1, k, kp);
-                double a, b, c;
-                if (nine) {
-                    a = S3(i, j, k, kpnw) + S3(i, j, k, kps) + S3(i + 1, j, k, kpsw) +
-                        S3(i, j, k, kbnw) + S3(i, j, k, kbn) + S3(i + 1, j, k, kbne) +
-                        S3(i, j, k + 1, kbse) + S3(i, j, k + 1, kbs) + S3(i + 1, j, k + 1, kbsw);
-                    b = S3(i, j - 1, k, kpsw) + S3(i, j - 1, k, kps) + S3(i + 1, j - 1, k, kpnw) +
-                        S3(i, j - 1, k, kbsw) + S3(i, j - 1, k, kbs) + S3(i + 1, j - 1, k, kbse) +
-                        S3(i, j - 1, k + 1, kbne) + S3(i, j - 1, k + 1, kbn) +
-                        S3(i + 1, j - 1, k + 1, kbnw);
-                    c = a + b + S3(i, j - 1, k, kpw) + S3(i + 1, j - 1, k, kpw) +
-                        S3(i, j - 1, k, kbw) + S3(i, j - 1, k, kb) + S3(i + 1, j - 1, k, kbe) +
-                        S3(i, j - 1, k + 1, kbe) + S3(i, j - 1, k + 1, kb) +
-                        S3(i + 1, j - 1, k + 1, kbw);
-                } else {
-                    a = S3(i, j, k, kps);
-                    b = S3(i, j - 1, k, kps);
-                    c = a + b + S3(i, j - 1, k, kpw) + S3(i + 1, j - 1, k, kpw) + S3(i, j - 1, k, kb) +
-                        S3(i, j - 1, k + 1, kb);
-                }
-                const double ep = dmin2(fabs(a / p0), fabs(b / p0));
-                c = lump3(a, b, c, p0, ep);
-                CI3(ic, jc, kc, lxya) = a / c;
-                CI3(ic, jc, kc, lxyb) = b / c;
+                orc3_oi_ya(so, nine, ci, ii, jj, P, iic, jjc, Pc, 2 * (ic - 1), j, k, ic, jc, kc);
             }
         }
     }
-    /* z-edges: fine (2(ic-1), 2(jc-1), 2kc-3) */
     for (int kc = 3; kc <= kkcf1; ++kc) {
         const int k = 2 * (kc - 1);
         for (int jc = 2; jc <= jjc1; ++jc) {
             const int j = 2 * (jc - 1);
             for (int ic = 2; ic <= iic1; ++ic) {
-                const int i = 2 * (ic - 1);
-                const double p0 = S3(i, j, k - 1, kp);
-                double a, b, c;
-                if (nine) {
-                    a = S3(i, j + 1, k, kbse) + S3(i, j + 1, k, kbs) + S3(i + 1, j + 1, k, kbsw) +
-                        S3(i, j, k, kbe) + S3(i, j, k, kb) + S3(i + 1, j, k, kbw) +
-                        S3(i, j, k, kbne) + S3(i, j, k, kbn) + S3(i + 1, j, k, kbnw);
-                    b = S3(i, j + 1, k - 1, kbnw) + S3(i, j + 1, k - 1, kbn) +
-                        S3(i + 1, j + 1, k - 1, kbne) + S3(i, j, k - 1, kbw) + S3(i, j, k - 1, kb) +
-                        S3(i + 1, j, k - 1, kbe) + S3(i, j, k - 1, kbsw) + S3(i, j, k - 1, kbs) +
-                        S3(i + 1, j, k - 1, kbse);
-                    c = a + b + S3(i, j, k - 1, kpw) + S3(i + 1, j, k - 1, kpw) +
-                        S3(i, j + 1, k - 1, kpnw) + S3(i, j + 1, k - 1, kps) +
-                        S3(i + 1, j + 1, k - 1, kpsw) + S3(i, j, k - 1, kpsw) +
-                        S3(i, j, k - 1, kps) + S3(i + 1, j, k - 1, kpnw);
-                } else {
-                    a = S3(i, j, k, kb);
-                    b = S3(i, j, k - 1, kb);
-                    c = a + b + S3(i, j, k - 1, kpw) + S3(i + 1, j, k - 1, kpw) +
-                        S3(i, j + 1, k - 1, kps) + S3(i, j, k - 1, kps);
-                }
-                const double ep = dmin2(fabs(a / p0), fabs(b / p0));
-                c = lump3(a, b, c, p0, ep);
-                CI3(ic, jc, kc, lxza) = a / c;
-                CI3(ic, jc, kc, lxzb) = b / c;
+                orc3_oi_za(so, nine, ci, ii, jj, P, iic, jjc, Pc, 2 * (ic - 1), j, k, ic, jc, kc);
             }
         }
     }
-    /* xy-faces on coarse planes: fine (2ic-3, 2jc-3, 2(kc-1)) */
     for (int kc = 2; kc <= kkc1; ++kc) {
         const int k = 2 * (kc - 1);
         for (int jc = 3; jc <= jjcf1; ++jc) {
             const int j = 2 * (jc - 1);
             for (int ic = 3; ic <= iicf1; ++ic) {
-                const int i = 2 * (ic - 1);
-                const double p0 = S3(i - 1, j - 1, k, kp);
-                double dnw, dn, dne, dw, de, dsw, ds, dse, ep;
-                if (nine) {
-                    dnw = S3(i - 1, j, k, kpnw) + S3(i - 1, j, k, kbnw) + S3(i - 1, j, k + 1, kbse);
-                    dn = S3(i - 1, j, k, kps) + S3(i - 1, j, k, kbn) + S3(i - 1, j, k + 1, kbs);
-                    dne = S3(i, j, k, kpsw) + S3(i, j, k, kbne) + S3(i, j, k + 1, kbsw);
-                    dw = S3(i - 1, j - 1, k, kpw) + S3(i - 1, j - 1, k, kbw) + S3(i - 1, j - 1, k + 1, kbe);
-                    de = S3(i, j - 1, k, kpw) + S3(i, j - 1, k, kbe) + S3(i, j - 1, k + 1, kbw);
-                    dsw = S3(i - 1, j - 1, k, kpsw) + S3(i - 1, j - 1, k, kbsw) +
-                          S3(i - 1, j - 1, k + 1, kbne);
-                    ds = S3(i - 1, j - 1, k, kps) + S3(i - 1, j - 1, k, kbs) + S3(i - 1, j - 1, k + 1, kbn);
-                    dse = S3(i, j - 1, k, kpnw) + S3(i, j - 1, k, kbse) + S3(i, j - 1, k + 1, kbnw);
-                    ep = min4(fabs((dsw + dw + dnw) / p0), fabs((dnw + dn + dne) / p0),
-                              fabs((dne + de + dse) / p0), fabs((dse + ds + dsw) / p0));
-                } else {
-                    dn = S3(i - 1, j, k, kps);
-                    dw = S3(i - 1, j - 1, k, kpw);
-                    de = S3(i, j - 1, k, kpw);
-                    ds = S3(i - 1, j - 1, k, kps);
-                    dnw = dne = dsw = dse = 0.0;
-                    ep = min4(fabs(dw / p0), fabs(dn / p0), fabs(de / p0), fabs(ds / p0));
-                }
-                double dp = nine ? dw + dnw + dn + dne + de + dse + ds + dsw : dw + dn + de + ds;
-                const double sum = p0 - S3(i - 1, j - 1, k, kb) - S3(i - 1, j - 1, k + 1, kb);
-                dp = 1.0 / lumpd(dp, sum, ep);
-                if (nine) {
-                    CI3(ic, jc, kc, lxynw) = dp * (dnw + CI3(ic - 1, jc, kc, lxya) * dw + CI3(ic, jc, kc, lxyl) * dn);
-                    CI3(ic, jc, kc, lxyne) = dp * (dne + CI3(ic, jc, kc, lxyr) * dn + CI3(ic, jc, kc, lxya) * de);
-                    CI3(ic, jc, kc, lxyse) = dp * (dse + CI3(ic, jc, kc, lxyb) * de + CI3(ic, jc - 1, kc, lxyr) * ds);
-                    CI3(ic, jc, kc, lxysw) = dp * (dsw + CI3(ic, jc - 1, kc, lxyl) * ds + CI3(ic - 1, jc, kc, lxyb) * dw);
-                } else {
-                    CI3(ic, jc, kc, lxynw) = dp * (CI3(ic - 1, jc, kc, lxya) * dw + CI3(ic, jc, kc, lxyl) * dn);
-                    CI3(ic, jc, kc, lxyne) = dp * (CI3(ic, jc, kc, lxyr) * dn + CI3(ic, jc, kc, lxya) * de);
-                    CI3(ic, jc, kc, lxyse) = dp * (CI3(ic, jc, kc, lxyb) * de + CI3(ic, jc - 1, kc, lxyr) * ds);
-                    CI3(ic, jc, kc, lxysw) = dp * (CI3(ic, jc - 1, kc, lxyl) * ds + CI3(ic - 1, jc, kc, lxyb) * dw);
-                }
+                orc3_oi_xyf(so, nine, ci, ii, jj, P, iic, jjc, Pc, 2 * (ic - 1), j, k, ic, jc, kc);
             }
         }
     }
-    /* xz-faces: fine (2ic-3, 2(jc-1), 2kc-3) */
     for (int kc = 3; kc <= kkcf1; ++kc) {
         const int k = 2 * (kc - 1);
         for (int jc = 2; jc <= jjc1; ++jc) {
             const int j = 2 * (jc - 1);
             for (int ic = 3; ic <= iicf1; ++ic) {
-                const int i = 2 * (ic - 1);
-                const double p0 = S3(i - 1, j, k - 1, kp);
-                double dnw, dn, dne, dw, de, dsw, ds, dse, ep;
-                if (nine) {
-                    dnw = S3(i - 1, j + 1, k, kbse) + S3(i - 1, j, k, kbe) + S3(i - 1, j, k, kbne);
-                    dn = S3(i - 1, j + 1, k, kbs) + S3(i - 1, j, k, kb) + S3(i - 1, j, k, kbn);
-                    dne = S3(i, j + 1, k, kbsw) + S3(i, j, k, kbw) + S3(i, j, k, kbnw);
-                    dw = S3(i - 1, j + 1, k - 1, kpnw) + S3(i - 1, j, k - 1, kpw) + S3(i - 1, j, k - 1, kpsw);
-                    de = S3(i, j + 1, k - 1, kpsw) + S3(i, j, k - 1, kpw) + S3(i, j, k - 1, kpnw);
-                    dsw = S3(i - 1, j + 1, k - 1, kbnw) + S3(i - 1, j, k - 1, kbw) +
-                          S3(i - 1, j, k - 1, kbsw);
-                    ds = S3(i - 1, j + 1, k - 1, kbn) + S3(i - 1, j, k - 1, kb) + S3(i - 1, j, k - 1, kbs);
-                    dse = S3(i, j + 1, k - 1, kbne) + S3(i, j, k - 1, kbe) + S3(i, j, k - 1, kbse);
-                    ep = min4(fabs((dsw + dw + dnw) / p0), fabs((dnw + dn + dne) / p0),
-                              fabs((dne + de + dse) / p0), fabs((dse + ds + dsw) / p0));
-                } else {
-                    dn = S3(i - 1, j, k, kb);
-                    dw = S3(i - 1, j, k - 1, kpw);
-                    de = S3(i, j, k - 1, kpw);
-                    ds = S3(i - 1, j, k - 1, kb);
-                    dnw = dne = dsw = dse = 0.0;
-                    ep = min4(fabs(dw / p0), fabs(dn / p0), fabs(de / p0), fabs(ds / p0));
-                }
-                double dp = nine ? dw + dnw + dn + dne + de + dse + ds + dsw : dw + dn + de + ds;
-                const double sum = p0 - S3(i - 1, j + 1, k - 1, kps) - S3(i - 1, j, k - 1, kps);
-                dp = 1.0 / lumpd(dp, sum, ep);
-                if (nine) {
-                    CI3(ic, jc, kc, lxznw) = dp * (dnw + CI3(ic - 1, jc, kc, lxza) * dw + CI3(ic, jc, kc, lxyl) * dn);
-                    CI3(ic, jc, kc, lxzne) = dp * (dne + CI3(ic, jc, kc, lxyr) * dn + CI3(ic, jc, kc, lxza) * de);
-                    CI3(ic, jc, kc, lxzse) = dp * (dse + CI3(ic, jc, kc, lxzb) * de + CI3(ic, jc, kc - 1, lxyr) * ds);
-                    CI3(ic, jc, kc, lxzsw) = dp * (dsw + CI3(ic, jc, kc - 1, lxyl) * ds + CI3(ic - 1, jc, kc, lxzb) * dw);
-                } else {
-                    CI3(ic, jc, kc, lxznw) = dp * (CI3(ic - 1, jc, kc, lxza) * dw + CI3(ic, jc, kc, lxyl) * dn);
-                    CI3(ic, jc, kc, lxzne) = dp * (CI3(ic, jc, kc, lxyr) * dn + CI3(ic, jc, kc, lxza) * de);
-                    CI3(ic, jc, kc, lxzse) = dp * (CI3(ic, jc, kc, lxzb) * de + CI3(ic, jc, kc - 1, lxyr) * ds);
-                    CI3(ic, jc, kc, lxzsw) = dp * (CI3(ic, jc, kc - 1, lxyl) * ds + CI3(ic - 1, jc, kc, lxzb) * dw);
-                }
+                orc3_oi_xzf(so, nine, ci, ii, jj, P, iic, jjc, Pc, 2 * (ic - 1), j, k, ic, jc, kc);
             }
         }
     }
-    /* yz-faces: fine (2(ic-1), 2jc-3, 2kc-3) */
     for (int kc = 3; kc <= kkcf1; ++kc) {
         const int k = 2 * (kc - 1);
         for (int jc = 3; jc <= jjcf1; ++jc) {
             const int j = 2 * (jc - 1);
             for (int ic = 2; ic <= iic1; ++ic) {
-                const int i = 2 * (ic - 1);
-                const double p0 = S3(i, j - 1, k - 1, kp);
-                double dnw, dn, dne, dw, de, dsw, ds, dse, ep;
-                if (nine) {
-                    dnw = S3(i, j, k, kbse) + S3(i, j, k, kbs) + S3(i + 1, j, k, kbsw);
-                    dn = S3(i, j - 1, k, kbe) + S3(i, j - 1, k, kb) + S3(i + 1, j - 1, k, kbw);
-                    dne = S3(i, j - 1, k, kbne) + S3(i, j - 1, k, kbn) + S3(i + 1, j - 1, k, kbnw);
-                    dw = S3(i, j, k - 1, kpnw) + S3(i, j, k - 1, kps) + S3(i + 1, j, k - 1, kpsw);
-                    de = S3(i, j - 1, k - 1, kpsw) + S3(i, j - 1, k - 1, kps) + S3(i + 1, j - 1, k - 1, kpnw);
-                    dsw = S3(i, j, k - 1, kbnw) + S3(i, j, k - 1, kbn) + S3(i + 1, j, k - 1, kbne);
-                    ds = S3(i, j - 1, k - 1, kbw) + S3(i, j - 1, k - 1, kb) + S3(i + 1, j - 1, k - 1, kbe);
-                    dse = S3(i, j - 1, k - 1, kbsw) + S3(i, j - 1, k - 1, kbs) +
-                          S3(i + 1, j - 1, k - 1, kbse);
-                    ep = min4(fabs((dsw + dw + dnw) / p0), fabs((dnw + dn + dne) / p0),
-                              fabs((dne + de + dse) / p0), fabs((dse + ds + dsw) / p0));
-                } else {
-                    dn = S3(i, j - 1, k, kb);
-                    dw = S3(i, j, k - 1, kps);
-                    de = S3(i, j - 1, k - 1, kps);
-                    ds = S3(i, j - 1, k - 1, kb);
-                    dnw = dne = dsw = dse = 0.0;
-                    ep = min4(fabs(dw / p0), fabs(dn / p0), fabs(de / p0), fabs(ds / p0));
-                }
-                double dp = nine ? dw + dnw + dn + dne + de + dse + ds + dsw : dw + dn + de + ds;
-                const double sum = p0 - S3(i, j - 1, k - 1, kpw) - S3(i + 1, j - 1, k - 1, kpw);
-                dp = 1.0 / lumpd(dp, sum, ep);
-                if (nine) {
-                    CI3(ic, jc, kc, lyznw) = dp * (dnw + CI3(ic, jc, kc, lxza) * dw + CI3(ic, jc, kc, lxya) * dn);
-                    CI3(ic, jc, kc, lyzne) = dp * (dne + CI3(ic, jc, kc, lxyb) * dn + CI3(ic, jc - 1, kc, lxza) * de);
-                    CI3(ic, jc, kc, lyzse) = dp * (dse + CI3(ic, jc - 1, kc, lxzb) * de + CI3(ic, jc, kc - 1, lxyb) * ds);
-                    CI3(ic, jc, kc, lyzsw) = dp * (dsw + CI3(ic, jc, kc - 1, lxya) * ds + CI3(ic, jc, kc, lxzb) * dw);
-                } else {
-                    CI3(ic, jc, kc, lyznw) = dp * (CI3(ic, jc, kc, lxza) * dw + CI3(ic, jc, kc, lxya) * dn);
-                    CI3(ic, jc, kc, lyzne) = dp * (CI3(ic, jc, kc, lxyb) * dn + CI3(ic, jc - 1, kc, lxza) * de);
-                    CI3(ic, jc, kc, lyzse) = dp * (CI3(ic, jc - 1, kc, lxzb) * de + CI3(ic, jc, kc - 1, lxyb) * ds);
-                    CI3(ic, jc, kc, lyzsw) = dp * (CI3(ic, jc, kc - 1, lxya) * ds + CI3(ic, jc, kc, lxzb) * dw);
-                }
+                orc3_oi_yzf(so, nine, ci, ii, jj, P, iic, jjc, Pc, 2 * (ic - 1), j, k, ic, jc, kc);
             }
         }
     }
-    /* cell centres: fine (2ic-3, 2jc-3, 2kc-3) */
     for (int kc = 3; kc <= kkcf1; ++kc) {
         const int k = 2 * (kc - 1);
         for (int jc = 3; jc <= jjcf1; ++jc) {
             const int j = 2 * (jc - 1);
             for (int ic = 3; ic <= iicf1; ++ic) {
-                const int i = 2 * (ic - 1);
-                const double p0 = S3(i - 1, j - 1, k - 1, kp);
-                if (nine) {
-                    double yp = S3(i - 1, j - 1, k - 1, kpw) + S3(i - 1, j, k - 1, kpnw) +
-                                S3(i - 1, j, k - 1, kps) + S3(i, j, k - 1, kpsw) +
-                                S3(i, j - 1, k - 1, kpw) + S3(i, j - 1, k - 1, kpnw) +
-                                S3(i - 1, j - 1, k - 1, kps) + S3(i - 1, j - 1, k - 1, kpsw) +
-                                S3(i - 1, j - 1, k - 1, kb) + S3(i - 1, j - 1, k - 1, kbw) +
-                                S3(i - 1, j, k - 1, kbnw) + S3(i - 1, j, k - 1, kbn) +
-                                S3(i, j, k - 1, kbne) + S3(i, j - 1, k - 1, kbe) +
-                                S3(i, j - 1, k - 1, kbse) + S3(i - 1, j - 1, k - 1, kbs) +
-                                S3(i - 1, j - 1, k - 1, kbsw) + S3(i - 1, j - 1, k, kb) +
-                                S3(i - 1, j - 1, k, kbe) + S3(i - 1, j, k, kbse) +
-                                S3(i - 1, j, k, kbs) + S3(i, j, k, kbsw) + S3(i, j - 1, k, kbw) +
-                                S3(i, j - 1, k, kbnw) + S3(i - 1, j - 1, k, kbn) +
-                                S3(i - 1, j - 1, k, kbne);
-                    /* as in the reference, the third and fifth sums are not divided by SO_p,
-                       and kbse enters the fourth sum twice */
-                    double ye = min4(
-                        fabs(S3(i - 1, j - 1, k - 1, kpw) + S3(i - 1, j, k - 1, kpnw) +
-                             S3(i - 1, j, k, kbse) + S3(i - 1, j - 1, k, kbe) +
-                             S3(i - 1, j - 1, k, kbne) + S3(i - 1, j - 1, k - 1, kpsw) +
-                             S3(i - 1, j - 1, k - 1, kbsw) + S3(i - 1, j - 1, k - 1, kbw) +
-                             S3(i - 1, j, k - 1, kbnw)) / p0,
-                        fabs(S3(i, j - 1, k - 1, kpw) + S3(i, j, k - 1, kpsw) + S3(i, j, k, kbsw) +
-                             S3(i, j - 1, k, kbw) + S3(i, j - 1, k, kbnw) + S3(i, j - 1, k - 1, kpnw) +
-                             S3(i, j - 1, k - 1, kbse) + S3(i, j - 1, k - 1, kbe) +
-                             S3(i, j, k - 1, kbne)) / p0,
-                        fabs(S3(i - 1, j, k - 1, kps) + S3(i - 1, j, k - 1, kpnw) +
-                             S3(i - 1, j, k, kbse) + S3(i - 1, j, k, kbs) + S3(i, j, k, kbsw) +
-                             S3(i, j, k - 1, kpsw) + S3(i, j, k - 1, kbne) + S3(i - 1, j, k - 1, kbn) +
-                             S3(i - 1, j, k - 1, kbnw)),
-                        fabs(S3(i - 1, j - 1, k - 1, kps) + S3(i - 1, j - 1, k - 1, kpsw) +
-                             S3(i - 1, j - 1, k, kbne) + S3(i - 1, j - 1, k, kbn) +
-                             S3(i, j - 1, k, kbnw) + S3(i, j - 1, k - 1, kpnw) +
-                             S3(i, j - 1, k - 1, kbse) + S3(i - 1, j - 1, k - 1, kbs) +
-                             S3(i, j - 1, k - 1, kbse)) / p0);
-                    ye = dmin2(dmin2(ye,
-                                     fabs(S3(i - 1, j - 1, k - 1, kb) + S3(i - 1, j - 1, k - 1, kbw) +
-                                          S3(i - 1, j, k - 1, kbnw) + S3(i - 1, j, k - 1, kbn) +
-                                          S3(i, j, k - 1, kbne) + S3(i, j - 1, k - 1, kbe) +
-                                          S3(i, j - 1, k - 1, kbse) + S3(i - 1, j - 1, k - 1, kbs) +
-                                          S3(i - 1, j - 1, k - 1, kbsw))),
-                               fabs(S3(i - 1, j - 1, k, kb) + S3(i - 1, j - 1, k, kbe) +
-                                    S3(i - 1, j, k, kbse) + S3(i - 1, j, k, kbs) + S3(i, j, k, kbsw) +
-                                    S3(i, j - 1, k, kbw) + S3(i, j - 1, k, kbnw) +
-                                    S3(i - 1, j - 1, k, kbn) + S3(i - 1, j - 1, k, kbne)) / p0);
-                    yp = lumpd(yp, p0, ye);
-                    yp = 1.0 / yp;
-                    CI3(ic, jc, kc, ltnw) = yp * (S3(i - 1, j, k, kbse) +
-                        CI3(ic - 1, jc, kc, lyznw) * S3(i - 1, j - 1, k - 1, kpw) +
-                        CI3(ic - 1, jc, kc, lxza) * S3(i - 1, j, k - 1, kpnw) +
-                        CI3(ic, jc, kc, lxznw) * S3(i - 1, j, k - 1, kps) +
-                        CI3(ic - 1, jc, kc, lxya) * S3(i - 1, j - 1, k, kbe) +
-                        CI3(ic, jc, kc, lxyl) * S3(i - 1, j, k, kbs) +
-                        CI3(ic, jc, kc, lxynw) * S3(i - 1, j - 1, k, kb));
-                    CI3(ic, jc, kc, ltne) = yp * (S3(i, j, k, kbsw) +
-                        CI3(ic, jc, kc, lxzne) * S3(i - 1, j, k - 1, kps) +
-                        CI3(ic, jc, kc, lxza) * S3(i, j, k - 1, kpsw) +
-                        CI3(ic, jc, kc, lyznw) * S3(i, j - 1, k - 1, kpw) +
-                        CI3(ic, jc, kc, lxyr) * S3(i - 1, j, k, kbs) +
-                        CI3(ic, jc, kc, lxya) * S3(i, j - 1, k, kbw) +
-                        CI3(ic, jc, kc, lxyne) * S3(i - 1, j - 1, k, kb));
-                    CI3(ic, jc, kc, lbnw) = yp * (S3(i - 1, j, k - 1, kbnw) +
-                        CI3(ic - 1, jc, kc - 1, lxya) * S3(i - 1, j - 1, k - 1, kbw) +
-                        CI3(ic, jc, kc - 1, lxyl) * S3(i - 1, j, k - 1, kbn) +
-                        CI3(ic, jc, kc - 1, lxynw) * S3(i - 1, j - 1, k - 1, kb) +
-                        CI3(ic - 1, jc, kc, lyzsw) * S3(i - 1, j - 1, k - 1, kpw) +
-                        CI3(ic - 1, jc, kc, lxzb) * S3(i - 1, j, k - 1, kpnw) +
-                        CI3(ic, jc, kc, lxzsw) * S3(i - 1, j, k - 1, kps));
-                    CI3(ic, jc, kc, lbne) = yp * (S3(i, j, k - 1, kbne) +
-                        CI3(ic, jc, kc - 1, lxyne) * S3(i - 1, j - 1, k - 1, kb) +
-                        CI3(ic, jc, kc - 1, lxyr) * S3(i - 1, j, k - 1, kbn) +
-                        CI3(ic, jc, kc - 1, lxya) * S3(i, j - 1, k - 1, kbe) +
-                        CI3(ic, jc, kc, lxzse) * S3(i - 1, j, k - 1, kps) +
-                        CI3(ic, jc, kc, lxzb) * S3(i, j, k - 1, kpsw) +
-                        CI3(ic, jc, kc, lyzsw) * S3(i, j - 1, k - 1, kpw));
-                    CI3(ic, jc, kc, lbsw) = yp * (S3(i - 1, j - 1, k - 1, kbsw) +
-                        CI3(ic - 1, jc, kc - 1, lxyb) * S3(i - 1, j - 1, k - 1, kbw) +
-                        CI3(ic, jc, kc - 1, lxysw) * S3(i - 1, j - 1, k - 1, kb) +
-                        CI3(ic, jc - 1, kc - 1, lxyl) * S3(i - 1, j - 1, k - 1, kbs) +
-                        CI3(ic - 1, jc, kc, lyzse) * S3(i - 1, j - 1, k - 1, kpw) +
-                        CI3(ic, jc - 1, kc, lxzsw) * S3(i - 1, j - 1, k - 1, kps) +
-                        CI3(ic - 1, jc - 1, kc, lxzb) * S3(i - 1, j - 1, k - 1, kpsw));
-                    CI3(ic, jc, kc, ltsw) = yp * (S3(i - 1, j - 1, k, kbne) +
-                        CI3(ic - 1, jc, kc, lxyb) * S3(i - 1, j - 1, k, kbe) +
-                        CI3(ic, jc, kc, lxysw) * S3(i - 1, j - 1, k, kb) +
-                        CI3(ic, jc - 1, kc, lxyl) * S3(i - 1, j - 1, k, kbn) +
-                        CI3(ic - 1, jc, kc, lyzne) * S3(i - 1, j - 1, k - 1, kpw) +
-                        CI3(ic, jc - 1, kc, lxznw) * S3(i - 1, j - 1, k - 1, kps) +
-                        CI3(ic - 1, jc - 1, kc, lxza) * S3(i - 1, j - 1, k - 1, kpsw));
-                    CI3(ic, jc, kc, ltse) = yp * (S3(i, j - 1, k, kbnw) +
-                        CI3(ic, jc - 1, kc, lxyr) * S3(i - 1, j - 1, k, kbn) +
-                        CI3(ic, jc, kc, lxyse) * S3(i - 1, j - 1, k, kb) +
-                        CI3(ic, jc, kc, lxyb) * S3(i, j - 1, k, kbw) +
-                        CI3(ic, jc - 1, kc, lxzne) * S3(i - 1, j - 1, k - 1, kps) +
-                        CI3(ic, jc, kc, lyzne) * S3(i, j - 1, k - 1, kpw) +
-                        CI3(ic, jc - 1, kc, lxza) * S3(i, j - 1, k - 1, kpnw));
-                    CI3(ic, jc, kc, lbse) = yp * (S3(i, j - 1, k - 1, kbse) +
-                        CI3(ic, jc - 1, kc - 1, lxyr) * S3(i - 1, j - 1, k - 1, kbs) +
-                        CI3(ic, jc, kc - 1, lxyse) * S3(i - 1, j - 1, k - 1, kb) +
-                        CI3(ic, jc, kc - 1, lxyb) * S3(i, j - 1, k - 1, kbe) +
-                        CI3(ic, jc - 1, kc, lxzse) * S3(i - 1, j - 1, k - 1, kps) +
-                        CI3(ic, jc, kc, lyzse) * S3(i, j - 1, k - 1, kpw) +
-                        CI3(ic, jc - 1, kc, lxzb) * S3(i, j - 1, k - 1, kpnw));
-                } else {
-                    double dp = S3(i - 1, j - 1, k - 1, kpw) + S3(i - 1, j, k - 1, kps) +
-                                S3(i, j - 1, k - 1, kpw) + S3(i - 1, j - 1, k - 1, kps) +
-                                S3(i - 1, j - 1, k - 1, kb) + S3(i - 1, j - 1, k, kb);
-                    const double ep = dmin2(
-                        dmin2(min4(fabs(S3(i - 1, j - 1, k - 1, kpw) / p0), fabs(S3(i - 1, j, k - 1, kps) / p0),
-                                   fabs(S3(i, j - 1, k - 1, kpw) / p0), fabs(S3(i - 1, j - 1, k - 1, kps) / p0)),
-                              fabs(S3(i - 1, j - 1, k - 1, kb) / p0)),
-                        fabs(S3(i - 1, j - 1, k, kb) / p0));
-                    /* here the reference adds the correction before dp */
-                    dp = (p0 - dp) * dmax2(p0 - (1.0 + ep) * dp, 0.0) /
-                             (fabs(p0 - (1.0 + ep) * dp) + EMACH) + dp;
-                    dp = 1.0 / dp;
-                    CI3(ic, jc, kc, ltnw) = dp * (CI3(ic - 1, jc, kc, lyznw) * S3(i - 1, j - 1, k - 1, kpw) +
-                                                  CI3(ic, jc, kc, lxznw) * S3(i - 1, j, k - 1, kps) +
-                                                  CI3(ic, jc, kc, lxynw) * S3(i - 1, j - 1, k, kb));
-                    CI3(ic, jc, kc, ltne) = dp * (CI3(ic, jc, kc, lxzne) * S3(i - 1, j, k - 1, kps) +
-                                                  CI3(ic, jc, kc, lyznw) * S3(i, j - 1, k - 1, kpw) +
-                                                  CI3(ic, jc, kc, lxyne) * S3(i - 1, j - 1, k, kb));
-                    CI3(ic, jc, kc, lbnw) = dp * (CI3(ic, jc, kc - 1, lxynw) * S3(i - 1, j - 1, k - 1, kb) +
-                                                  CI3(ic - 1, jc, kc, lyzsw) * S3(i - 1, j - 1, k - 1, kpw) +
-                                                  CI3(ic, jc, kc, lxzsw) * S3(i - 1, j, k - 1, kps));
-                    CI3(ic, jc, kc, lbne) = dp * (CI3(ic, jc, kc - 1, lxyne) * S3(i - 1, j - 1, k - 1, kb) +
-                                                  CI3(ic, jc, kc, lxzse) * S3(i - 1, j, k - 1, kps) +
-                                                  CI3(ic, jc, kc, lyzsw) * S3(i, j - 1, k - 1, kpw));
-                    CI3(ic, jc, kc, lbsw) = dp * (CI3(ic, jc, kc - 1, lxysw) * S3(i - 1, j - 1, k - 1, kb) +
-                                                  CI3(ic - 1, jc, kc, lyzse) * S3(i - 1, j - 1, k - 1, kpw) +
-                                                  CI3(ic, jc - 1, kc, lxzsw) * S3(i - 1, j - 1, k - 1, kps));
-                    CI3(ic, jc, kc, ltsw) = dp * (CI3(ic, jc, kc, lxysw) * S3(i - 1, j - 1, k, kb) +
-                                                  CI3(ic - 1, jc, kc, lyzne) * S3(i - 1, j - 1, k - 1, kpw) +
-                                                  CI3(ic, jc - 1, kc, lxznw) * S3(i - 1, j - 1, k - 1, kps));
-                    CI3(ic, jc, kc, ltse) = dp * (CI3(ic, jc, kc, lxyse) * S3(i - 1, j - 1, k, kb) +
-                                                  CI3(ic, jc - 1, kc, lxzne) * S3(i - 1, j - 1, k - 1, kps) +
-                                                  CI3(ic, jc, kc, lyzne) * S3(i, j - 1, k - 1, kpw));
-                    CI3(ic, jc, kc, lbse) = dp * (CI3(ic, jc, kc - 1, lxyse) * S3(i - 1, j - 1, k - 1, kb) +
-                                                  CI3(ic, jc - 1, kc, lxzse) * S3(i - 1, j - 1, k - 1, kps) +
-                                                  CI3(ic, jc, kc, lyzse) * S3(i, j - 1, k - 1, kpw));
-                }
+                orc3_oi_cell(so, nine, ci, ii, jj, P, iic, jjc, Pc, 2 * (ic - 1), j, k, ic, jc, kc);
             }
         }
     }
@@ -747,6 +776,62 @@ static double aentry3(const double *so, int nstncl, size_t P, int ii, int jj, in
         }
     }
     return 0.0;
+}
+
+/* index n of a periodic direction with interior 2..m+1, mapped into the interior */
+static int wrap_in(int n, int m) { return ((n - 2) % m + m) % m + 2; }
+
+/* (P^T A P)(X, Y) for coarse X = (icx,jcx,kcx), Y = X + (DX,DY,DZ) (Fortran indices).  In a
+ * periodic direction (per[d]) the fine points of the product are taken in the unwrapped
+ * geometry between X and Y (so each stored coupling is the one across its own side, as the
+ * reference's hand-expanded products form it) and only the array reads wrap: coarse indices
+ * into 2..n+1, fine points into the interior. */
+static double kpair3w(const double *so, int nstncl, const double *ci, int iif, int jjf, int kkf,
+                      int iic, int jjc, int kkc, int icx, int jcx, int kcx, int DX, int DY, int DZ,
+                      const int per[3])
+{
+    const int ii = iif, jj = jjf;
+    const size_t P = (size_t)iif * jjf * kkf, Pc = (size_t)iic * jjc * kkc;
+    (void)ii;
+    (void)jj;
+    const int mf[3] = {iif - 2, jjf - 2, kkf - 2}, mc[3] = {iic - 2, jjc - 2, kkc - 2};
+    if (per[0]) icx = wrap_in(icx, mc[0]);
+    if (per[1]) jcx = wrap_in(jcx, mc[1]);
+    if (per[2]) kcx = wrap_in(kcx, mc[2]);
+    int icy = icx + DX, jcy = jcx + DY, kcy = kcx + DZ;
+    if (per[0]) icy = wrap_in(icy, mc[0]);
+    if (per[1]) jcy = wrap_in(jcy, mc[1]);
+    if (per[2]) kcy = wrap_in(kcy, mc[2]);
+    const int X0 = 2 * (icx - 1), Y0 = 2 * (jcx - 1), Z0 = 2 * (kcx - 1);
+    double sum = 0.0;
+    for (int fz = -1; fz <= 1; ++fz)
+        for (int fy = -1; fy <= 1; ++fy)
+            for (int fx = -1; fx <= 1; ++fx) {
+                int x = X0 + fx, y = Y0 + fy, z = Z0 + fz;
+                if (x < 2 || x > iif - 1) { if (!per[0]) continue; x = wrap_in(x, mf[0]); }
+                if (y < 2 || y > jjf - 1) { if (!per[1]) continue; y = wrap_in(y, mf[1]); }
+                if (z < 2 || z > kkf - 1) { if (!per[2]) continue; z = wrap_in(z, mf[2]); }
+                double inner = 0.0;
+                int any = 0;
+                for (int oz = -1; oz <= 1; ++oz)
+                    for (int oy = -1; oy <= 1; ++oy)
+                        for (int ox = -1; ox <= 1; ++ox) {
+                            const int gx = fx + ox - 2 * DX, gy = fy + oy - 2 * DY,
+                                      gz = fz + oz - 2 * DZ;
+                            if (gx < -1 || gx > 1 || gy < -1 || gy > 1 || gz < -1 || gz > 1) continue;
+                            const int xg = x + ox, yg = y + oy, zg = z + oz;
+                            if ((!per[0] && (xg < 2 || xg > iif - 1)) ||
+                                (!per[1] && (yg < 2 || yg > jjf - 1)) ||
+                                (!per[2] && (zg < 2 || zg > kkf - 1)))
+                                continue;
+                            if (nstncl == 4 && (ox != 0) + (oy != 0) + (oz != 0) > 1) continue;
+                            inner += aentry3(so, nstncl, P, iif, jjf, x, y, z, ox, oy, oz) *
+                                     pweight3(ci, Pc, iic, jjc, icy, jcy, kcy, gx, gy, gz);
+                            any = 1;
+                        }
+                if (any) sum += pweight3(ci, Pc, iic, jjc, icx, jcx, kcx, fx, fy, fz) * inner;
+            }
+    return sum;
 }
 
 /* (P^T A P)(X, Y) for coarse X = (icx,jcx,kcx), Y = X + (DX,DY,DZ) (Fortran indices) */
@@ -815,6 +900,85 @@ void orc3_setup_galerkin(const double *so, int nstncl, const double *ci, double 
                     soc[(size_t)(c->d - 1) * Pc + pc] = v;
                 }
             }
+}
+
+/* Galerkin coarse operator with periodic directions: P^T A P with the wrapped geometry of
+ * kpair3w, then the images of the coarse operator copied as the reference's hand-expanded
+ * products end (src/3d/ftn/BMG3_SymStd_SETUP_ITLI07_ex.f90:1002-1113, identical in
+ * BMG3_SymStd_SETUP_ITLI27_ex.f90:1884-1995): y images, then x, then z, all 14 planes. */
+void orc3_setup_galerkin_bc(const double *so, int nstncl, const double *ci, double *soc, int iif,
+                            int jjf, int kkf, int iic, int jjc, int kkc, int jpn)
+{
+    const int per[3] = {orc3_per_x(jpn), orc3_per_y(jpn), orc3_per_z(jpn)};
+    if (!per[0] && !per[1] && !per[2]) {
+        orc3_setup_galerkin(so, nstncl, ci, soc, iif, jjf, kkf, iic, jjc, kkc);
+        return;
+    }
+    const size_t Pc = (size_t)iic * jjc * kkc;
+    for (int kc = 2; kc <= kkc - 1; ++kc)
+        for (int jc = 2; jc <= jjc - 1; ++jc)
+            for (int ic = 2; ic <= iic - 1; ++ic) {
+                const size_t pc = ix3(ic, jc, kc, iic, jjc);
+                soc[pc] = kpair3w(so, nstncl, ci, iif, jjf, kkf, iic, jjc, kkc, ic, jc, kc, 0, 0,
+                                  0, per);
+                for (int t = 0; t < 13; ++t) {
+                    const cpair3 *c = &CPAIRS[t];
+                    const int xa = ic + c->ax, ya = jc + c->ay, za = kc + c->az;
+                    const int xb = ic + c->bx, yb = jc + c->by, zb = kc + c->bz;
+                    double v = 0.0;
+                    if ((per[0] || (xa >= 2 && xb >= 2)) && (per[1] || (ya >= 2 && yb >= 2)) &&
+                        (per[2] || (za >= 2 && zb >= 2)))
+                        v = -kpair3w(so, nstncl, ci, iif, jjf, kkf, iic, jjc, kkc, xa, ya, za,
+                                     xb - xa, yb - ya, zb - za, per);
+                    soc[(size_t)(c->d - 1) * Pc + pc] = v;
+                }
+            }
+    const int iic1 = iic - 1, jjc1 = jjc - 1, kkc1 = kkc - 1;
+#define SOC(i, j, k, d) soc[(size_t)((d)-1) * Pc + ix3((i), (j), (k), iic, jjc)]
+    if (per[1])
+        for (int kc = 1; kc <= kkc; ++kc)
+            for (int ic = 1; ic <= iic; ++ic) {
+                for (int d = 1; d <= 14; ++d) SOC(ic, jjc, kc, d) = SOC(ic, 2, kc, d);
+                for (int d = 1; d <= 14; ++d) SOC(ic, 1, kc, d) = SOC(ic, jjc1, kc, d);
+            }
+    if (per[0])
+        for (int kc = 1; kc <= kkc; ++kc)
+            for (int jc = 1; jc <= jjc; ++jc) {
+                for (int d = 1; d <= 14; ++d) SOC(iic, jc, kc, d) = SOC(2, jc, kc, d);
+                for (int d = 1; d <= 14; ++d) SOC(1, jc, kc, d) = SOC(iic1, jc, kc, d);
+            }
+    if (per[2])
+        for (int jc = 1; jc <= jjc; ++jc)
+            for (int ic = 1; ic <= iic; ++ic) {
+                for (int d = 1; d <= 14; ++d) SOC(ic, jc, kkc, d) = SOC(ic, jc, 2, d);
+                for (int d = 1; d <= 14; ++d) SOC(ic, jc, 1, d) = SOC(ic, jc, kkc1, d);
+            }
+#undef SOC
+}
+
+/* qf + the off-diagonal terms of the Gauss-Seidel update at (i,j,k) (shared with the periodic
+ * branch) */
+double orc3_gs_point(const double *so, int nstncl, const double *qf, const double *q, int i, int j,
+                     int k, int ii, int jj, int kk)
+{
+    return offdiag3(so, nstncl, qf, q, i, j, k, ii, jj, (size_t)ii * jj * kk);
+}
+
+/* matrix entry A(r, r+o) of term t of the stencil at r = (i,j,k) in the residual's term order
+ * (t = -1: the diagonal, o = 0) */
+double orc3_stencil_term(const double *so, int nstncl, int ii, int jj, int kk, int i, int j, int k,
+                         int t, int *oi, int *oj, int *ok)
+{
+    const size_t P = (size_t)ii * jj * kk;
+    if (t < 0) {
+        *oi = *oj = *ok = 0;
+        return S3(i, j, k, kp);
+    }
+    const nbr3 *n = &NB27[nstncl == 14 ? t : NB7[t]];
+    *oi = n->oi;
+    *oj = n->oj;
+    *ok = n->ok;
+    return -S3(i + n->si, j + n->sj, k + n->sk, n->d);
 }
 
 /* ------------------------------------------------------------------------------------------
@@ -1136,6 +1300,7 @@ typedef struct {
 
 struct orc3_solver {
     orc3_settings s;
+    int jpn; /* BoxMG boundary id of s.periodic */
     int nlevels;
     orc3_level *lv;
     double *abd, *bbd;
@@ -1217,7 +1382,7 @@ static void smooth(orc3_solver *slv, orc3_level *L, double *x, const double *b, 
     for (int r = 0; r < nrep; ++r) {
         switch (slv->s.relax) {
         case ORC3_RELAX_POINT:
-            orc3_relax_gs(L->so, L->nstncl, b, x, L->sor, L->ii, L->jj, L->kk, updown);
+            orc3_relax_gs_bc(L->so, L->nstncl, b, x, L->sor, L->ii, L->jj, L->kk, updown, slv->jpn);
             break;
         case ORC3_RELAX_PLANE_XY: relax_planes(slv, L, ORC3_XY, x, b, updown); break;
         case ORC3_RELAX_PLANE_XZ: relax_planes(slv, L, ORC3_XZ, x, b, updown); break;
@@ -1245,7 +1410,7 @@ void orc3_solver_relax_planes(orc3_solver *slv, int lvl, int dir, double *x, con
 static void coarse_solve(orc3_solver *slv, double *x, const double *b)
 {
     orc3_level *C = &slv->lv[slv->nlevels - 1];
-    orc3_solve_cg(x, b, slv->abd, slv->ldab, slv->bbd, C->ii, C->jj, C->kk);
+    orc3_solve_cg_bc(x, b, slv->abd, slv->ldab, slv->bbd, C->ii, C->jj, C->kk, slv->jpn);
 }
 
 /* vcycle::ncycle_helper (include/cedar/cycle/vcycle.h:57-115) */
@@ -1254,14 +1419,14 @@ static void ncycle(orc3_solver *slv, int l, double *x, const double *b)
     orc3_level *F = &slv->lv[l], *C = &slv->lv[l + 1];
     smooth(slv, F, x, b, ORC_DOWN);
     orc3_residual(F->so, F->nstncl, b, x, F->res, F->ii, F->jj, F->kk);
-    orc3_restrict(C->ci, F->res, C->b, F->ii, F->jj, F->kk, C->ii, C->jj, C->kk);
+    orc3_restrict_bc(C->ci, F->res, C->b, F->ii, F->jj, F->kk, C->ii, C->jj, C->kk, slv->jpn);
     memset(C->x, 0, sizeof(double) * (size_t)C->ii * C->jj * C->kk);
     if (l + 1 == slv->nlevels - 1)
         coarse_solve(slv, C->x, C->b);
     else
         ncycle(slv, l + 1, C->x, C->b);
-    orc3_interp_add(x, C->x, F->res, F->so, F->nstncl, C->ci, C->ii, C->jj, C->kk, F->ii, F->jj,
-                    F->kk);
+    orc3_interp_add_bc(x, C->x, F->res, F->so, F->nstncl, C->ci, C->ii, C->jj, C->kk, F->ii,
+                       F->jj, F->kk, slv->jpn);
     smooth(slv, F, x, b, ORC_UP);
 }
 
@@ -1281,13 +1446,13 @@ static void fmg3(orc3_solver *slv, int l, double *x, const double *b)
         return;
     }
     orc3_level *F = &slv->lv[l], *C = &slv->lv[l + 1];
-    orc3_restrict(C->ci, b, C->b, F->ii, F->jj, F->kk, C->ii, C->jj, C->kk);
+    orc3_restrict_bc(C->ci, (double *)b, C->b, F->ii, F->jj, F->kk, C->ii, C->jj, C->kk, slv->jpn);
     fmg3(slv, l + 1, C->x, C->b);
     const size_t np = (size_t)F->ii * F->jj * F->kk;
     memset(x, 0, np * sizeof(double));
     memset(F->res, 0, np * sizeof(double));
-    orc3_interp_add(x, C->x, F->res, F->so, F->nstncl, C->ci, C->ii, C->jj, C->kk, F->ii, F->jj,
-                    F->kk);
+    orc3_interp_add_bc(x, C->x, F->res, F->so, F->nstncl, C->ci, C->ii, C->jj, C->kk, F->ii,
+                       F->jj, F->kk, slv->jpn);
     ncycle(slv, l, x, b);
 }
 
@@ -1306,6 +1471,7 @@ orc3_solver *orc3_solver_create(const double *so_fine, int nstncl, int nx, int n
 {
     orc3_solver *slv = (orc3_solver *)calloc(1, sizeof(orc3_solver));
     slv->s = *s;
+    slv->jpn = orc3_bc_from_mask(s->periodic);
     int nl = orc3_num_levels(nx, ny, nz, s->min_coarse);
     if (s->num_levels > 0 && s->num_levels <= nl) nl = s->num_levels;
     slv->nlevels = nl;
@@ -1334,15 +1500,17 @@ orc3_solver *orc3_solver_create(const double *so_fine, int nstncl, int nx, int n
     }
     orc3_level *C = &slv->lv[nl - 1];
     const int kd = orc3_cg_bandwidth(C->ii, C->jj);
-    slv->ldab = kd + 1;
     const size_t n = (size_t)C->nx * C->ny * C->nz;
+    /* periodic: a dense n x n factor (include/cedar/3d/solver.h:123-126) */
+    slv->ldab = slv->jpn != 0 ? (int)n : kd + 1;
     slv->abd = zeros((size_t)slv->ldab * n);
     slv->bbd = zeros(n);
     for (int l = 0; l < nl - 1; ++l) {
         orc3_level *F = &slv->lv[l], *Cn = &slv->lv[l + 1];
-        orc3_setup_interp(F->so, F->nstncl, Cn->ci, F->ii, F->jj, F->kk, Cn->ii, Cn->jj, Cn->kk);
-        orc3_setup_galerkin(F->so, F->nstncl, Cn->ci, Cn->so, F->ii, F->jj, F->kk, Cn->ii, Cn->jj,
-                            Cn->kk);
+        orc3_setup_interp_bc(F->so, F->nstncl, Cn->ci, F->ii, F->jj, F->kk, Cn->ii, Cn->jj, Cn->kk,
+                             slv->jpn);
+        orc3_setup_galerkin_bc(F->so, F->nstncl, Cn->ci, Cn->so, F->ii, F->jj, F->kk, Cn->ii,
+                               Cn->jj, Cn->kk, slv->jpn);
         if (s->relax == ORC3_RELAX_POINT) {
             orc3_setup_recip(F->so, F->sor, F->ii, F->jj, F->kk);
         } else if (plane_relax(s->relax)) {
@@ -1355,7 +1523,7 @@ orc3_solver *orc3_solver_create(const double *so_fine, int nstncl, int nx, int n
                 planes_setup(F, ORC3_YZ, &s->plane);
         }
     }
-    orc3_setup_cg(C->so, C->nstncl, slv->abd, slv->ldab, C->ii, C->jj, C->kk);
+    orc3_setup_cg_bc(C->so, C->nstncl, slv->abd, slv->ldab, C->ii, C->jj, C->kk, slv->jpn);
     return slv;
 }
 

@@ -18,6 +18,8 @@
 #ifndef CEDAR_ORACLE_BMG3_H
 #define CEDAR_ORACLE_BMG3_H
 
+#include <stddef.h>
+
 #include "bmg2_oracle.h"
 
 #ifdef __cplusplus
@@ -46,6 +48,19 @@ void orc3_interp_add(double *q, const double *qc, double *res, const double *so,
                      const double *ci, int iic, int jjc, int kkc, int iif, int jjf, int kkf);
 void orc3_setup_interp(const double *so, int nstncl, double *ci, int iif, int jjf, int kkf,
                        int iic, int jjc, int kkc);
+/* One loop body of the interpolation setup: the weights of coarse point (ic,jc,kc) tied to the
+ * fine anchor (i,j,k) (Dirichlet: i = 2(ic-1), ...), for the x-, y-, z-edge, xy-, xz-, yz-face
+ * and cell-centre fine points; nine = 27-point fine operator.  Shared by the periodic branch. */
+#define ORC3_OI_PT_ARGS                                                                      \
+    const double *so, int nine, double *ci, int ii, int jj, size_t P, int iic, int jjc,      \
+        size_t Pc, int i, int j, int k, int ic, int jc, int kc
+void orc3_oi_xl(ORC3_OI_PT_ARGS);
+void orc3_oi_ya(ORC3_OI_PT_ARGS);
+void orc3_oi_za(ORC3_OI_PT_ARGS);
+void orc3_oi_xyf(ORC3_OI_PT_ARGS);
+void orc3_oi_xzf(ORC3_OI_PT_ARGS);
+void orc3_oi_yzf(ORC3_OI_PT_ARGS);
+void orc3_oi_cell(ORC3_OI_PT_ARGS);
 /* Galerkin coarse operator P^T A P (14 planes) from the fine operator and CI */
 void orc3_setup_galerkin(const double *so, int nstncl, const double *ci, double *soc, int iif,
                          int jjf, int kkf, int iic, int jjc, int kkc);
@@ -58,6 +73,38 @@ double orc3_norm(const double *x, int ii, int jj, int kk);
 /* y = A q over the interior (ghosts of y untouched) */
 void orc3_matvec(const double *so, int nstncl, const double *q, double *y, int ii, int jj,
                  int kk);
+
+/* ---- periodic branches (bmg3_periodic.c): jpn is the BoxMG boundary id of the periodic
+ * mask (orc3_bc_from_mask); jpn == 0 gives the Dirichlet kernels above ---- */
+int  orc3_bc_from_mask(int mask);
+int  orc3_per_x(int jpn);
+int  orc3_per_y(int jpn);
+int  orc3_per_z(int jpn);
+void orc3_images(double *q, int ii, int jj, int kk, int jpn);
+void orc3_relax_gs_bc(const double *so, int nstncl, const double *qf, double *q, const double *sor,
+                      int ii, int jj, int kk, int updown, int jpn);
+/* refreshes the images of q (the fine residual) in place first */
+void orc3_restrict_bc(const double *ci, double *q, double *qc, int iif, int jjf, int kkf, int iic,
+                      int jjc, int kkc, int jpn);
+void orc3_interp_add_bc(double *q, const double *qc, double *res, const double *so, int nstncl,
+                        const double *ci, int iic, int jjc, int kkc, int iif, int jjf, int kkf,
+                        int jpn);
+void orc3_setup_interp_bc(const double *so, int nstncl, double *ci, int iif, int jjf, int kkf,
+                          int iic, int jjc, int kkc, int jpn);
+void orc3_setup_galerkin_bc(const double *so, int nstncl, const double *ci, double *soc, int iif,
+                            int jjf, int kkf, int iic, int jjc, int kkc, int jpn);
+/* periodic coarsest grid: dense n x n (n = nx ny nz) upper Cholesky factor; ldab is ignored */
+void orc3_assemble_cg_periodic(const double *so, int nstncl, double *abd, int ii, int jj, int kk,
+                               int jpn);
+int  orc3_setup_cg_bc(const double *so, int nstncl, double *abd, int ldab, int ii, int jj, int kk,
+                      int jpn);
+void orc3_solve_cg_bc(double *q, const double *qf, const double *abd, int ldab, double *bbd,
+                      int ii, int jj, int kk, int jpn);
+/* shared pieces of the kernels above */
+double orc3_gs_point(const double *so, int nstncl, const double *qf, const double *q, int i, int j,
+                     int k, int ii, int jj, int kk);
+double orc3_stencil_term(const double *so, int nstncl, int ii, int jj, int kk, int i, int j, int k,
+                         int t, int *oi, int *oj, int *ok);
 
 /* ---- plane relaxation pieces (include/cedar/3d/relax_planes.h, src/3d/relax_planes.cc) ---- */
 /* 2D operator of plane ipl (1..n of the direction normal to the plane) */
@@ -79,6 +126,7 @@ typedef struct {
     int min_coarse;   /* default 3 */
     orc2_settings plane; /* nested plane solves: line-xy, max-iter 1 (src/kernel_params.cc:83-89) */
     int cycle;        /* 0 = V (default), 1 = F */
+    int periodic;     /* per_mask: bits x = 1, y = 2, z = 4 (point relaxation) */
 } orc3_settings;
 
 typedef struct orc3_solver orc3_solver;

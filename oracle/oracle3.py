@@ -25,7 +25,7 @@ _p = _o._ptr
 class _Settings3(ctypes.Structure):
     _fields_ = [("relax", _i), ("nrelax_pre", _i), ("nrelax_post", _i), ("num_levels", _i),
                 ("maxiter", _i), ("tol", ctypes.c_double), ("min_coarse", _i),
-                ("plane", _o._Settings), ("cycle", _i)]
+                ("plane", _o._Settings), ("cycle", _i), ("periodic", _i)]
 
 
 _init = False
@@ -146,6 +146,94 @@ def solve_cg(abd, qf):
     return q
 
 
+# ---- periodic branches (bmg3_periodic.c); jpn is the BoxMG boundary id of the mask ----
+def bc_from_mask(mask):
+    """BMG_get_bc for the 3D periodic mask (bits x = 1, y = 2, z = 4)."""
+    return lib().orc3_bc_from_mask(_i(mask))
+
+
+def images(q, jpn):
+    kk, jj, ii = q.shape
+    q = q.copy()
+    lib().orc3_images(_p(q), _i(ii), _i(jj), _i(kk), _i(jpn))
+    return q
+
+
+def relax_gs_bc(so, qf, q, sor, updown, jpn):
+    kk, jj, ii = q.shape
+    q = q.copy()
+    lib().orc3_relax_gs_bc(_p(so), _i(so.shape[0]), _p(qf), _p(q), _p(sor), _i(ii), _i(jj),
+                           _i(kk), _i(updown), _i(jpn))
+    return q
+
+
+def restrict_bc(ci, q, jpn):
+    """Returns (qc, q with its images refreshed, as the reference leaves it)."""
+    kkf, jjf, iif = q.shape
+    _, kkc, jjc, iic = ci.shape
+    q = q.copy()
+    qc = np.zeros((kkc, jjc, iic))
+    lib().orc3_restrict_bc(_p(ci), _p(q), _p(qc), _i(iif), _i(jjf), _i(kkf), _i(iic), _i(jjc),
+                           _i(kkc), _i(jpn))
+    return qc, q
+
+
+def interp_add_bc(q, qc, res, so, ci, jpn):
+    kkf, jjf, iif = q.shape
+    kkc, jjc, iic = qc.shape
+    q = q.copy()
+    res = res.copy()
+    lib().orc3_interp_add_bc(_p(q), _p(qc), _p(res), _p(so), _i(so.shape[0]), _p(ci), _i(iic),
+                             _i(jjc), _i(kkc), _i(iif), _i(jjf), _i(kkf), _i(jpn))
+    return q, res
+
+
+def setup_interp_bc(so, jpn, cdims=None):
+    ns, kkf, jjf, iif = so.shape
+    iic, jjc, kkc = cdims or coarse_dims3(iif, jjf, kkf)
+    ci = np.zeros((26, kkc, jjc, iic))
+    lib().orc3_setup_interp_bc(_p(so), _i(ns), _p(ci), _i(iif), _i(jjf), _i(kkf), _i(iic),
+                               _i(jjc), _i(kkc), _i(jpn))
+    return ci
+
+
+def setup_galerkin_bc(so, ci, jpn):
+    ns, kkf, jjf, iif = so.shape
+    _, kkc, jjc, iic = ci.shape
+    soc = np.zeros((14, kkc, jjc, iic))
+    lib().orc3_setup_galerkin_bc(_p(so), _i(ns), _p(ci), _p(soc), _i(iif), _i(jjf), _i(kkf),
+                                 _i(iic), _i(jjc), _i(kkc), _i(jpn))
+    return soc
+
+
+def assemble_cg_periodic(so, jpn):
+    """Dense n x n periodic matrix, upper triangle (A[r, c] for r <= c), C order [c, r]."""
+    ns, kk, jj, ii = so.shape
+    n = (ii - 2) * (jj - 2) * (kk - 2)
+    a = np.zeros((n, n))
+    lib().orc3_assemble_cg_periodic(_p(so), _i(ns), _p(a), _i(ii), _i(jj), _i(kk), _i(jpn))
+    return a
+
+
+def setup_cg_bc(so, jpn):
+    """Periodic: (dense upper Cholesky factor, C order [c, r] = ABD(r, c), info)."""
+    ns, kk, jj, ii = so.shape
+    n = (ii - 2) * (jj - 2) * (kk - 2)
+    abd = np.zeros((n, n))
+    info = lib().orc3_setup_cg_bc(_p(so), _i(ns), _p(abd), _i(n), _i(ii), _i(jj), _i(kk),
+                                  _i(jpn))
+    return abd, info
+
+
+def solve_cg_bc(abd, qf, jpn):
+    kk, jj, ii = qf.shape
+    q = np.zeros_like(qf)
+    bbd = np.zeros(abd.shape[0])
+    lib().orc3_solve_cg_bc(_p(q), _p(qf), _p(abd), _i(abd.shape[1]), _p(bbd), _i(ii), _i(jj),
+                           _i(kk), _i(jpn))
+    return q
+
+
 def plane_dims(direction, ii, jj, kk):
     """(ii2, jj2, number of planes) of the planes of a direction."""
     d = PLANE[direction]
@@ -198,12 +286,16 @@ class Solver3:
     """3D FP64 oracle solver: cdr3::solver<fsten>(so, conf) + solve(b, x)."""
 
     def __init__(self, so, relaxation="point", nrelax_pre=2, nrelax_post=1, num_levels=-1,
-                 maxiter=10, tol=1e-8, min_coarse=3, plane=None, cycle="v"):
+                 maxiter=10, tol=1e-8, min_coarse=3, plane=None, cycle="v", periodic=0):
+        """periodic: per_mask (bits x = 1, y = 2, z = 4), point relaxation only."""
         self.so = np.ascontiguousarray(so, dtype=np.float64)
         ns, kk, jj, ii = self.so.shape
         self.shape = (kk, jj, ii)
+        if periodic and relaxation != "point":
+            raise ValueError("periodic 3D problems take point relaxation")
         s = _Settings3(RELAX3[relaxation], nrelax_pre, nrelax_post, num_levels, maxiter, tol,
-                       min_coarse, plane or plane_settings(), {"v": 0, "f": 1}[cycle])
+                       min_coarse, plane or plane_settings(), {"v": 0, "f": 1}[cycle],
+                       int(periodic))
         self.maxiter = maxiter
         self._h = ctypes.c_void_p(lib().orc3_solver_create(_p(self.so), _i(ns), _i(ii - 2),
                                                            _i(jj - 2), _i(kk - 2),

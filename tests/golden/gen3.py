@@ -30,3 +30,36 @@ def rand_so3(rng, nx, ny, nz, ns, lo=0.1, hi=1.0):
         so[d] = np.where(coupling_mask(nx, ny, nz, d), rng.uniform(lo, hi, so[d].shape), 0.0)
     so[0, 1:-1, 1:-1, 1:-1] = 2.0 * (ns - 1) + rng.uniform(0.0, 1.0, (nz, ny, nx))
     return so
+
+
+def images3(a, mask):
+    """Periodic images of a (..., kk, jj, ii) in place for the mask (bits x = 1, y = 2, z = 4),
+    y first, then x, then z, over the full extents."""
+    if mask & 2:
+        a[..., 0, :] = a[..., -2, :]
+        a[..., -1, :] = a[..., 1, :]
+    if mask & 1:
+        a[..., :, 0] = a[..., :, -2]
+        a[..., :, -1] = a[..., :, 1]
+    if mask & 4:
+        a[..., 0, :, :] = a[..., -2, :, :]
+        a[..., -1, :, :] = a[..., 1, :, :]
+    return a
+
+
+def rand_so3_periodic(rng, nx, ny, nz, ns, mask, lo=0.1, hi=1.0):
+    """Diagonally dominant M-matrix stencil that couples across the boundaries of the periodic
+    directions (mask bits x = 1, y = 2, z = 4), with every plane's periodic images filled."""
+    so = np.zeros((ns, nz + 2, ny + 2, nx + 2))
+    k, j, i = np.meshgrid(np.arange(nz + 2), np.arange(ny + 2), np.arange(nx + 2), indexing="ij")
+    interior = (i >= 1) & (i <= nx) & (j >= 1) & (j <= ny) & (k >= 1) & (k <= nz)
+    for d in range(1, ns):
+        ok = interior.copy()
+        for off in PAIRS3[d]:
+            ii_, jj_, kk_ = i + off[0], j + off[1], k + off[2]
+            ok &= ((ii_ >= 1) & (ii_ <= nx)) | bool(mask & 1)
+            ok &= ((jj_ >= 1) & (jj_ <= ny)) | bool(mask & 2)
+            ok &= ((kk_ >= 1) & (kk_ <= nz)) | bool(mask & 4)
+        so[d] = np.where(ok, rng.uniform(lo, hi, so[d].shape), 0.0)
+    so[0, 1:-1, 1:-1, 1:-1] = 2.0 * (ns - 1) + rng.uniform(0.0, 1.0, (nz, ny, nx))
+    return images3(so, mask)

@@ -1,0 +1,178 @@
+"""The periodic branches of the 3D oracle (oracle/bmg3_periodic.c) pinned on the CPU.
+
+Gauss-Seidel, restriction and the operator-induced interpolation are bitwise equal to the
+reference's own Fortran kernels compiled from source (oracle/_ref) and called with the BoxMG
+boundary id of each periodic mask, for 7- and 27-point operators of odd, even and unequal sizes.
+The Galerkin operator (P^T A P over the wrapped geometry) is checked against the reference's
+hand-expanded products (ITLI07/ITLI27 with their image copies) to rounding, and against a sparse
+triple product of the prolongation that interp_add applies with the periodic matrix.  The
+periodic coarse matrix equals the operator probed column by column through the matvec with
+refreshed images; its factor and solve equal dense LAPACK ones with the mean removed.
+
+interp_add: the reference's image loops (src/3d/ftn/BMG3_SymStd_interp_add.f90:237-280) test an
+unassigned IPN and index with left-over loop variables, so they are not run; the oracle's
+interp_add is the definite loops (bitwise against the reference called with jpn = 0) followed
+by a full image refresh, which is what is checked here.
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from oracle import oracle3 as O3
+from tests.golden.gen3 import images3, rand_so3_periodic
+
+R3 = pytest.importorskip("oracle.refkernels3")
+
+MASKS = [1, 2, 3, 4, 5, 6, 7]
+SIZES = [(6, 6, 6), (8, 6, 10), (7, 9, 5), (6, 10, 6)]
+
+
+def have_ref():
+    try:
+        from oracle import oracle as O
+        O.ref()
+        return True
+    except Exception:
+        return False
+
+
+needs_ref = pytest.mark.skipif(not have_ref(), reason="reference kernels not built (oracle/_ref)")
+
+
+def case(seed, n, ns, mask):
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = n
+    so = rand_so3_periodic(rng, nx, ny, nz, ns, mask)
+    shp = (nz + 2, ny + 2, nx + 2)
+    return rng, so, rng.standard_normal(shp), rng.standard_normal(shp)
+
+
+def test_bc_ids():
+    # BMG_get_bc (src/2d/ftn/BMG_get_bc.f90) with BMG_parameters_c.h:193-204
+    assert [O3.bc_from_mask(m) for m in range(8)] == [0, 2, 1, 3, 5, 6, 7, 8]
+
+
+@needs_ref
+@pytest.mark.parametrize("ns", [4, 14])
+@pytest.mark.parametrize("mask", MASKS)
+@pytest.mark.parametrize("n", SIZES)
+def test_gs_matches_reference(n, mask, ns):
+    _, so, q, qf = case(11 + mask, n, ns, mask)
+    jpn = O3.bc_from_mask(mask)
+    sor = O3.setup_recip(so)
+    for updown in (O3.DOWN, O3.UP):
+        want = R3.relax_gs(so, qf, q, sor, updown, jpn)
+        got = O3.relax_gs_bc(so, qf, q, sor, updown, jpn)
+        assert np.array_equal(got, want), updown
+
+
+@needs_ref
+@pytest.mark.parametrize("ns", [4, 14])
+@pytest.mark.parametrize("mask", MASKS)
+@pytest.mark.parametrize("n", SIZES)
+def test_interp_setup_and_restriction_match_reference(n, mask, ns):
+    _, so, q, _ = case(31 + mask, n, ns, mask)
+    jpn = O3.bc_from_mask(mask)
+    ci = O3.setup_interp_bc(so, jpn)
+    # equal_nan: with a 7-point operator the reference fills some image entries from 0 / 0
+    assert np.array_equal(ci, R3.setup_interp(so, jpn=jpn), equal_nan=True)
+    qc, qi = O3.restrict_bc(ci, q, jpn)
+    qc_r, qi_r = R3.restrict(ci, q, jpn=jpn, with_q=True)
+    assert np.array_equal(qc, qc_r)
+    assert np.array_equal(qi, qi_r)
+
+
+def test_interp_add_refreshes_images():
+    rng, so, q, res = case(5, (8, 6, 10), 14, 7)
+    ci = O3.setup_interp_bc(so, 8)
+    qc = rng.standard_normal(ci.shape[1:])
+    got, res_got = O3.interp_add_bc(q, qc, res, so, ci, 8)
+    want, res_want = O3.interp_add(q, qc, res, so, ci)
+    assert np.array_equal(res_got, res_want)
+    assert np.array_equal(got[1:-1, 1:-1, 1:-1], want[1:-1, 1:-1, 1:-1])
+    assert np.array_equal(got, images3(want.copy(), 7))
+
+
+def prolongation(ci, fshape, mask):
+    """Sparse P (fine interior x coarse interior) of interp_add with periodic coarse images."""
+    import scipy.sparse as sp
+    kkf, jjf, iif = fshape
+    _, kkc, jjc, iic = ci.shape
+    nc = (iic - 2) * (jjc - 2) * (kkc - 2)
+    cols = []
+    for c in range(nc):
+        qc = np.zeros((kkc, jjc, iic))
+        qc[1:-1, 1:-1, 1:-1].flat[c] = 1.0
+        qc = images3(qc, mask)
+        x, _ = O3.interp_add(np.zeros(fshape), qc, np.zeros(fshape), np.ones((14,) + fshape), ci)
+        cols.append(sp.csc_matrix(x[1:-1, 1:-1, 1:-1].reshape(-1, 1)))
+    return sp.hstack(cols).tocsr()
+
+
+def periodic_matrix(so, mask):
+    import scipy.sparse as sp
+    kk, jj, ii = so.shape[1:]
+    n = (ii - 2) * (jj - 2) * (kk - 2)
+    cols = []
+    for c in range(n):
+        e = np.zeros((kk, jj, ii))
+        e[1:-1, 1:-1, 1:-1].flat[c] = 1.0
+        e = images3(e, mask)
+        y = O3.matvec(so, e)
+        cols.append(sp.csc_matrix(y[1:-1, 1:-1, 1:-1].reshape(-1, 1)))
+    return sp.hstack(cols).tocsr()
+
+
+@needs_ref
+@pytest.mark.parametrize("ns", [4, 14])
+@pytest.mark.parametrize("mask", MASKS)
+def test_galerkin_matches_triple_product_and_reference(mask, ns):
+    n = (8, 6, 10)
+    _, so, _, _ = case(71 + mask, n, ns, mask)
+    jpn = O3.bc_from_mask(mask)
+    ci = O3.setup_interp_bc(so, jpn)
+    soc = O3.setup_galerkin_bc(so, ci, jpn)
+    # the reference's hand-expanded products agree to rounding
+    ref = R3.setup_galerkin(so, ci, jpn=jpn)
+    assert np.abs(soc - ref).max() <= 1e-13 * np.abs(ref).max()
+    # where the interpolation weights of the image rows are consistent, both are the triple
+    # product P^T A P of the periodic operator and the prolongation interp_add applies.  With y
+    # (7-point) or y and z (27-point) periodic they are not: the 7-point branch leaves lxza/lxzb
+    # of the image row jjc unset beyond column jjc-1 (SETUP_interp_OI.f90:2359 loops to JJC1),
+    # and the 27-point branch copies x images instead of y images on the plane kkc (:1057-1065),
+    # so interp_add and the Galerkin products read different weights for the same fine points.
+    if (mask, ns) not in {(2, 4), (3, 4), (6, 4), (7, 4), (6, 14), (7, 14)}:
+        Pm = prolongation(ci, so.shape[1:], mask)
+        want = (Pm.T @ periodic_matrix(so, mask) @ Pm).toarray()
+        got = periodic_matrix(soc, mask).toarray()
+        assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("ns", [4, 14])
+@pytest.mark.parametrize("mask", MASKS)
+def test_periodic_coarse_solve(mask, ns):
+    rng, so, _, b = case(91 + mask, (4, 3, 4), ns, mask)
+    jpn = O3.bc_from_mask(mask)
+    a = O3.assemble_cg_periodic(so, jpn)          # [c, r] = A(r, c), r <= c
+    dense = periodic_matrix(so, mask).toarray()
+    assert np.array_equal(np.triu(a.T), np.triu(dense))
+    fac, info = O3.setup_cg_bc(so, jpn)
+    assert info == 0
+    u = np.triu(fac.T)
+    assert np.abs(u - sla.cholesky(dense, lower=False)).max() <= 1e-13 * np.abs(u).max()
+    x = O3.solve_cg_bc(fac, b, jpn)
+    xs = np.linalg.solve(dense, b[1:-1, 1:-1, 1:-1].ravel())
+    xs = xs - xs.mean()
+    assert np.abs(x[1:-1, 1:-1, 1:-1].ravel() - xs).max() <= 1e-12 * np.abs(xs).max()
+    xi = x.copy()
+    # images refreshed in the order x, y, z
+    if mask & 1:
+        xi[:, :, 0] = xi[:, :, -2]
+        xi[:, :, -1] = xi[:, :, 1]
+    if mask & 2:
+        xi[:, 0, :] = xi[:, -2, :]
+        xi[:, -1, :] = xi[:, 1, :]
+    if mask & 4:
+        xi[0] = xi[-2]
+        xi[-1] = xi[1]
+    assert np.array_equal(x, xi)
