@@ -35,7 +35,7 @@ class Settings3(ctypes.Structure):
     """cdr_solver3_settings"""
     _fields_ = [("relaxation", _i), ("nrelax_pre", _i), ("nrelax_post", _i), ("num_levels", _i),
                 ("maxiter", _i), ("tol", _d), ("min_coarse", _i), ("plane", Settings),
-                ("cycle", _i), ("precision", _i), ("stall_cycles", _i)]
+                ("cycle", _i), ("precision", _i), ("stall_cycles", _i), ("periodic", _i)]
 
 
 # name -> (restype, argtypes); pointers to device memory are passed as c_void_p
@@ -117,6 +117,19 @@ PROTOTYPES = {
     "cdr_hip_solve_cg3d_f32": (_i, [_P, _P, _P, _i, _P, _i, _i, _i, _vp]),
     # 3D solver
     "cdr_hip_default_settings3": (None, [ctypes.POINTER(Settings3)]),
+    "cdr_hip_relax_gs3d_per_f64": (_i, [_P, _i, _P, _P, _P, _i, _i, _i, _i, _i, _vp]),
+    "cdr_hip_relax_gs3d_per_f32": (_i, [_P, _i, _P, _P, _P, _i, _i, _i, _i, _i, _vp]),
+    "cdr_hip_restrict3d_per_f64": (_i, [_P, _P, _P, _i, _i, _i, _i, _i, _i, _i, _vp]),
+    "cdr_hip_restrict3d_per_f32": (_i, [_P, _P, _P, _i, _i, _i, _i, _i, _i, _i, _vp]),
+    "cdr_hip_interp_add3d_per_f64": (_i, [_P, _P, _P, _P, _i, _P, _i, _i, _i, _i, _i, _i, _i,
+                                          _vp]),
+    "cdr_hip_interp_add3d_per_f32": (_i, [_P, _P, _P, _P, _i, _P, _i, _i, _i, _i, _i, _i, _i,
+                                          _vp]),
+    "cdr_hip_setup_interp3d_per_f64": (_i, [_P, _i, _P, _i, _i, _i, _i, _i, _i, _i, _vp]),
+    "cdr_hip_setup_galerkin3d_per_f64": (_i, [_P, _i, _P, _P, _i, _i, _i, _i, _i, _i, _i, _vp]),
+    "cdr_hip_setup_cg3d_per_f64": (_i, [_P, _i, _P, _i, _i, _i, _i, _P, _vp]),
+    "cdr_hip_solve_cg3d_per_f64": (_i, [_P, _P, _P, _P, _i, _i, _i, _i, _vp]),
+    "cdr_hip_solve_cg3d_per_f32": (_i, [_P, _P, _P, _P, _i, _i, _i, _i, _vp]),
     "cdr_hip_solver3d_create": (_i, [_P, _i, _i, _i, _i, ctypes.POINTER(Settings3), _vp,
                                      ctypes.POINTER(_vp)]),
     "cdr_hip_solver3d_solve": (_i, [_vp, _P, _P, ctypes.POINTER(_d), _i, ctypes.POINTER(_i)]),

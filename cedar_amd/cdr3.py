@@ -112,6 +112,66 @@ def solve_cg(q, qf, abd):
     return q
 
 
+# ---- periodic boundaries (the reference's JPN != 0 branches) --------------------------------
+def bc_from_mask(mask):
+    """BMG_get_bc for the 3D periodic mask (bits x = 1, y = 2, z = 4): 2, 1, 3, 5, 6, 7, 8."""
+    return (0, 2, 1, 3, 5, 6, 7, 8)[int(mask)]
+
+
+def relax_gs_per(so, qf, q, sor, updown, jpn):
+    _abi.call(f"cdr_hip_relax_gs3d_per_{_sfx(q)}", _p(so), so.shape[0], _p(qf), _p(q), _p(sor),
+              *_dims(q), int(updown), int(jpn), _stream())
+    return q
+
+
+def restrict_per(ci, q, qc, jpn):
+    """Restriction; refreshes the images of q in place first."""
+    _abi.call(f"cdr_hip_restrict3d_per_{_sfx(q)}", _p(ci), _p(q), _p(qc), *_dims(q), *_dims(qc),
+              int(jpn), _stream())
+    return qc
+
+
+def interp_add_per(q, qc, res, so, ci, jpn):
+    _abi.call(f"cdr_hip_interp_add3d_per_{_sfx(q)}", _p(q), _p(qc), _p(res), _p(so), so.shape[0],
+              _p(ci), *_dims(qc), *_dims(q), int(jpn), _stream())
+    return q
+
+
+def setup_interp_per(so, ci, jpn):
+    """ci must be zero on entry."""
+    _, kkf, jjf, iif = so.shape
+    _, kkc, jjc, iic = ci.shape
+    _abi.call("cdr_hip_setup_interp3d_per_f64", _p(so), so.shape[0], _p(ci), iif, jjf, kkf, iic,
+              jjc, kkc, int(jpn), _stream())
+    return ci
+
+
+def setup_galerkin_per(so, ci, soc, jpn):
+    _, kkf, jjf, iif = so.shape
+    _, kkc, jjc, iic = ci.shape
+    _abi.call("cdr_hip_setup_galerkin3d_per_f64", _p(so), so.shape[0], _p(ci), _p(soc), iif, jjf,
+              kkf, iic, jjc, kkc, int(jpn), _stream())
+    return soc
+
+
+def setup_cg_per(so, jpn):
+    """Dense upper Cholesky factor (n, n) tensor, [c, r] = ABD(r, c) (synchronises)."""
+    _, kk, jj, ii = so.shape
+    n = (ii - 2) * (jj - 2) * (kk - 2)
+    abd = torch.zeros((n, n), dtype=torch.float64, device=so.device)
+    info = torch.zeros(1, dtype=torch.int32, device=so.device)
+    _abi.call("cdr_hip_setup_cg3d_per_f64", _p(so), so.shape[0], _p(abd), ii, jj, kk, int(jpn),
+              _p(info), _stream())
+    return abd
+
+
+def solve_cg_per(q, qf, abd, jpn):
+    bbd = torch.zeros(abd.shape[0], dtype=torch.float64, device=q.device)
+    _abi.call(f"cdr_hip_solve_cg3d_per_{_sfx(q)}", _p(q), _p(qf), _p(abd), _p(bbd), *_dims(q),
+              int(jpn), _stream())
+    return q
+
+
 def _registry(fn, h):
     import json
     n = fn(h, None, 0)
@@ -136,19 +196,21 @@ class Solver3:
     Settings use Cedar's keys and defaults: relaxation ('point' | 'plane-xy' | 'plane-xz' |
     'plane-yz' | 'plane-xyz'), nrelax_pre=2, nrelax_post=1, num_levels=-1, maxiter=10, tol=1e-8,
     min_coarse=3, plane = plane_settings() (line-xy, one V(2,1) cycle per plane solve);
-    stall_cycles as for the 2D Solver.
+    stall_cycles as for the 2D Solver.  periodic: Cedar's grid.periodic as a per_mask (bits
+    x = 1, y = 2, z = 4) with point relaxation; the operator, b and x carry the periodic images
+    in their ghosts, and periodic extents must be even on every coarsened level.
     """
 
     def __init__(self, so, relaxation="point", nrelax_pre=2, nrelax_post=1, num_levels=-1,
                  maxiter=10, tol=1e-8, min_coarse=3, plane=None, cycle="v", precision="f64",
-                 stall_cycles=0):
+                 stall_cycles=0, periodic=0):
         so = so if isinstance(so, torch.Tensor) else torch.as_tensor(np.ascontiguousarray(so))
         self.so = so.to(device="cuda", dtype=torch.float64).contiguous()  # held by reference
         ns, kk, jj, ii = self.so.shape
         self.maxiter = maxiter
         s = _abi.Settings3(_abi.RELAX[relaxation], nrelax_pre, nrelax_post, num_levels, maxiter, tol,
                            min_coarse, plane or plane_settings(), _abi.CYCLE[cycle],
-                           _abi.PRECISION[precision], int(stall_cycles))
+                           _abi.PRECISION[precision], int(stall_cycles), int(periodic))
         h = ctypes.c_void_p()
         _abi.call("cdr_hip_solver3d_create", _p(self.so), ns, ii - 2, jj - 2, kk - 2,
                   ctypes.byref(s), _stream(), ctypes.byref(h))

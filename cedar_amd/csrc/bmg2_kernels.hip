@@ -1002,11 +1002,13 @@ __global__ __launch_bounds__(256) void restrict_k(const T *__restrict__ ci, cons
 // y sweeps run through the transposed grid).  Tiles of 64 (ic) x 16 (jc) coarse points over the
 // ghost-inclusive coarse grid, 256 threads.
 template <typename T>
-__global__ __launch_bounds__(256) void restrict_cycle_k(const T *__restrict__ ci,
-                                                        const T *__restrict__ q,
-                                                        T *__restrict__ qc, T *__restrict__ xc,
-                                                        T *__restrict__ qcT, int iif, int jjf,
-                                                        int iic, int jjc, int rep) {
+__global__ __launch_bounds__(1024) void restrict_cycle_k(const T *__restrict__ ci,
+                                                         const T *__restrict__ q,
+                                                         T *__restrict__ qc, T *__restrict__ xc,
+                                                         T *__restrict__ qcT, int iif, int jjf,
+                                                         int iic, int jjc, int rep) {
+    // one thread per coarse point of the 64 x 16 tile: every point's loads are in flight at
+    // once (with a loop over rows the FP64 build kept one row's loads in flight at a time)
     __shared__ T tile[16][65];
     const int ic0 = blockIdx.x * 64, jc0 = blockIdx.y * 16;
     const std::size_t Pc = (std::size_t)iic * jjc, bz = blockIdx.z;
@@ -1014,11 +1016,9 @@ __global__ __launch_bounds__(256) void restrict_cycle_k(const T *__restrict__ ci
     q += bz * iif * (std::size_t)jjf;
     qc += bz * Pc;
     xc += bz * Pc;
-    const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
-    const int ic = ic0 + tx;
-    for (int k = ty; k < 16; k += 4) {
-        const int jc = jc0 + k;
-        if (ic >= iic || jc >= jjc) continue;
+    const int tx = threadIdx.x % 64, k = threadIdx.x / 64;
+    const int ic = ic0 + tx, jc = jc0 + k;
+    if (ic < iic && jc < jjc) {
         const std::size_t pc = pidx(ic, jc, iic);
         xc[pc] = T(0);
         if (ic >= 1 && ic <= iic - 2 && jc >= 1 && jc <= jjc - 2) {
@@ -1030,12 +1030,10 @@ __global__ __launch_bounds__(256) void restrict_cycle_k(const T *__restrict__ ci
     if (!qcT) return;  // uniform over the launch
     qcT += bz * Pc;
     __syncthreads();
-    for (int e = threadIdx.x; e < 64 * 16; e += 256) {
-        const int a = e / 16, c = e % 16;  // a: ic offset, c: jc offset (contiguous in qcT)
-        const int i2 = ic0 + a, j2 = jc0 + c;
-        if (i2 >= 1 && i2 <= iic - 2 && j2 >= 1 && j2 <= jjc - 2)
-            qcT[(std::size_t)i2 * jjc + j2] = tile[c][a];
-    }
+    const int a = threadIdx.x / 16, c = threadIdx.x % 16;  // a: ic offset, c: jc offset
+    const int i2 = ic0 + a, j2 = jc0 + c;                  // (c contiguous in qcT)
+    if (i2 >= 1 && i2 <= iic - 2 && j2 >= 1 && j2 <= jjc - 2)
+        qcT[(std::size_t)i2 * jjc + j2] = tile[c][a];
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2542,7 +2540,7 @@ template <typename T>
 void restrict_cycle(const T *ci, const T *q, T *qc, T *xc, T *qcT, int iif, int jjf, int iic,
                     int jjc, hipStream_t s, int nb, int rep) {
     const dim3 g((unsigned)((iic + 63) / 64), (unsigned)((jjc + 15) / 16), (unsigned)nb);
-    restrict_cycle_k<T><<<g, 256, 0, s>>>(ci, q, qc, xc, qcT, iif, jjf, iic, jjc, rep);
+    restrict_cycle_k<T><<<g, 1024, 0, s>>>(ci, q, qc, xc, qcT, iif, jjf, iic, jjc, rep);
 }
 
 template <typename T>

@@ -339,18 +339,17 @@ __global__ __launch_bounds__(256) void interp3_k(T *__restrict__ q, const T *__r
 // operator-induced interpolation (src/3d/ftn/BMG3_SymStd_SETUP_interp_OI.f90:86-808), one
 // thread per coarse point in three launches: edge weights, then face weights (they read edge
 // weights of neighbouring coarse points), then cell-centre weights (they read both).  Each
-// device function below is one loop nest of the reference (27-point branch when NINE, 7-point
-// branch otherwise) for the fine point tied to coarse (ic, jc, kc); i, j, k = 2 (ic-1), ...
+// device function below is one loop body of the reference (27-point branch when NINE, 7-point
+// branch otherwise) for the fine point tied to coarse (ic, jc, kc) through the fine anchor
+// (i, j, k): 2 (ic-1), ... in the definite branch, wrapped indices in the periodic one.
 // ------------------------------------------------------------------------------------------
 #define OI_ARGS const double *__restrict__ so, double *__restrict__ ci, int ii, int jj, std::size_t P, \
-                int iic, int jjc, std::size_t Pc, int ic, int jc, int kc
-#define OI_PASS so, ci, ii, jj, P, iic, jjc, Pc, ic, jc, kc
-#define OI_IJK const int i = 2 * (ic - 1), j = 2 * (jc - 1), k = 2 * (kc - 1); (void)i; (void)j; (void)k;
+                int iic, int jjc, std::size_t Pc, int ic, int jc, int kc, int i, int j, int k
+#define OI_PASS so, ci, ii, jj, P, iic, jjc, Pc, ic, jc, kc, 2 * (ic - 1), 2 * (jc - 1), 2 * (kc - 1)
 
 template <bool NINE>
 __device__ inline void oi_xedge(OI_ARGS) {
     constexpr bool nine = NINE;
-    OI_IJK
     const double p0 = S3(i - 1, j, k, kp);
     double a, b, c, ep;
     if (nine) {
@@ -381,7 +380,6 @@ __device__ inline void oi_xedge(OI_ARGS) {
 template <bool NINE>
 __device__ inline void oi_yedge(OI_ARGS) {
     constexpr bool nine = NINE;
-    OI_IJK
     const double p0 = S3(i, j - 1, k, kp);
     double a, b, c;
     if (nine) {
@@ -411,7 +409,6 @@ __device__ inline void oi_yedge(OI_ARGS) {
 template <bool NINE>
 __device__ inline void oi_zedge(OI_ARGS) {
     constexpr bool nine = NINE;
-    OI_IJK
     const double p0 = S3(i, j, k - 1, kp);
     double a, b, c;
     if (nine) {
@@ -441,7 +438,6 @@ __device__ inline void oi_zedge(OI_ARGS) {
 template <bool NINE>
 __device__ inline void oi_xyface(OI_ARGS) {
     constexpr bool nine = NINE;
-    OI_IJK
     const double p0 = S3(i - 1, j - 1, k, kp);
     double dnw, dn, dne, dw, de, dsw, ds, dse, ep;
     if (nine) {
@@ -483,7 +479,6 @@ __device__ inline void oi_xyface(OI_ARGS) {
 template <bool NINE>
 __device__ inline void oi_xzface(OI_ARGS) {
     constexpr bool nine = NINE;
-    OI_IJK
     const double p0 = S3(i - 1, j, k - 1, kp);
     double dnw, dn, dne, dw, de, dsw, ds, dse, ep;
     if (nine) {
@@ -525,7 +520,6 @@ __device__ inline void oi_xzface(OI_ARGS) {
 template <bool NINE>
 __device__ inline void oi_yzface(OI_ARGS) {
     constexpr bool nine = NINE;
-    OI_IJK
     const double p0 = S3(i, j - 1, k - 1, kp);
     double dnw, dn, dne, dw, de, dsw, ds, dse, ep;
     if (nine) {
@@ -567,7 +561,6 @@ __device__ inline void oi_yzface(OI_ARGS) {
 template <bool NINE>
 __device__ inline void oi_cube(OI_ARGS) {
     constexpr bool nine = NINE;
-    OI_IJK
     const double p0 = S3(i - 1, j - 1, k - 1, kp);
     if (nine) {
         double yp = S3(i - 1, j - 1, k - 1, kpw) + S3(i - 1, j, k - 1, kpnw) +
@@ -913,6 +906,340 @@ __global__ void solve_cg3_k(T *__restrict__ q, const T *__restrict__ qf, const d
 }
 
 // ------------------------------------------------------------------------------------------
+// Periodic boundaries: the JPN != 0 branches of the 3D kernels.  jpn is the BoxMG boundary id
+// BMG_get_bc returns for the periodic mask (bits x = 1, y = 2, z = 4): 2 x, 1 y, 3 xy, 5 z,
+// 6 xz, 7 yz, 8 xyz (include/cedar/2d/ftn/BMG_parameters_c.h:193-204).  The images of a
+// periodic direction live in its ghost layer: q(1) = q(n-1), q(n) = q(2).
+// ------------------------------------------------------------------------------------------
+// one axis' images, q(1) = q(n-1) then q(n) = q(2) along `axis`, on the lines a = a0 + astep t
+// (t < na), b = b0 + bstep u (u < nb) of the other two axes (x: a = j, b = k; y: a = i, b = k;
+// z: a = i, b = j); lines are independent
+template <typename T>
+__global__ __launch_bounds__(256) void images3_k(T *__restrict__ q, int ii, int jj, int kk,
+                                                 int axis, int a0, int astep, int na, int b0,
+                                                 int bstep, int nb) {
+    const int t = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (t >= na * nb) return;
+    const int a = a0 + astep * (t % na), b = b0 + bstep * (t / na);
+    if (axis == 0) {
+        Q3(q, 1, a, b) = Q3(q, ii - 1, a, b);
+        Q3(q, ii, a, b) = Q3(q, 2, a, b);
+    } else if (axis == 1) {
+        Q3(q, a, 1, b) = Q3(q, a, jj - 1, b);
+        Q3(q, a, jj, b) = Q3(q, a, 2, b);
+    } else {
+        Q3(q, a, b, 1) = Q3(q, a, b, kk - 1);
+        Q3(q, a, b, kk) = Q3(q, a, b, 2);
+    }
+}
+
+// One loop nest of the periodic interpolation setup (src/3d/ftn/BMG3_SymStd_SETUP_interp_OI.f90
+// :809-2784): per axis, the coarse index runs c0..c1 with the fine index 2 (c-1) (mode 0) or
+// advanced by I = MAX(MOD(I+2, M), MIN(I+2, 3)) from f0 (mode 1), or is the single pair
+// (c0, f0) (mode 2).  The loop bodies are the definite branch's (oi_* above); a nest writes
+// only its own weight planes and reads others, so its points are independent.
+struct oi_axis {
+    int mode, c0, c1, f0, m;
+};
+__device__ inline int oi_wnext(int prev, int n) {
+    const int a = (prev + 2) % n, b = prev + 2 < 3 ? prev + 2 : 3;
+    return a > b ? a : b;
+}
+__device__ inline void oi_at(const oi_axis &a, int t, int &c, int &f) {
+    if (a.mode == 2) {
+        c = a.c0;
+        f = a.f0;
+        return;
+    }
+    c = a.c0 + t;
+    if (a.mode == 0) {
+        f = 2 * (c - 1);
+        return;
+    }
+    f = a.f0;
+    for (int u = 0; u <= t; ++u) f = oi_wnext(f, a.m);
+}
+
+template <bool NINE, int TYPE>
+__global__ __launch_bounds__(256) void oi_nest_k(const double *__restrict__ so, double *__restrict__ ci,
+                                                 int iif, int jjf, int kkf, int iic, int jjc, int kkc,
+                                                 oi_axis ax, oi_axis ay, oi_axis az, int nx, int ny,
+                                                 int nz) {
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (long long)nx * ny * nz) return;
+    int ic, i, jc, j, kc, k;
+    oi_at(ax, (int)(t % nx), ic, i);
+    oi_at(ay, (int)((t / nx) % ny), jc, j);
+    oi_at(az, (int)(t / ((long long)nx * ny)), kc, k);
+    const int ii = iif, jj = jjf;
+    const std::size_t P = (std::size_t)iif * jjf * kkf, Pc = (std::size_t)iic * jjc * kkc;
+    if (TYPE == 0) oi_xedge<NINE>(so, ci, ii, jj, P, iic, jjc, Pc, ic, jc, kc, i, j, k);
+    if (TYPE == 1) oi_yedge<NINE>(so, ci, ii, jj, P, iic, jjc, Pc, ic, jc, kc, i, j, k);
+    if (TYPE == 2) oi_zedge<NINE>(so, ci, ii, jj, P, iic, jjc, Pc, ic, jc, kc, i, j, k);
+    if (TYPE == 3) oi_xyface<NINE>(so, ci, ii, jj, P, iic, jjc, Pc, ic, jc, kc, i, j, k);
+    if (TYPE == 4) oi_xzface<NINE>(so, ci, ii, jj, P, iic, jjc, Pc, ic, jc, kc, i, j, k);
+    if (TYPE == 5) oi_yzface<NINE>(so, ci, ii, jj, P, iic, jjc, Pc, ic, jc, kc, i, j, k);
+    if (TYPE == 6) oi_cube<NINE>(so, ci, ii, jj, P, iic, jjc, Pc, ic, jc, kc, i, j, k);
+}
+
+// A group of up to four weight copies CI(dst) = CI(src) of the periodic interpolation setup,
+// executed in order for each value v = lo..hi of a loop variable; a coordinate of -1 stands
+// for v.  The groups copy within one row, column or plane per v, so values of v are
+// independent.
+struct ci_copy {
+    int n, lo, hi;
+    int dst[4][4], src[4][4];  // (ic, jc, kc, plane)
+};
+__global__ __launch_bounds__(256) void ci_copy_k(double *__restrict__ ci, int iic, int jjc, int kkc,
+                                                 ci_copy cp) {
+    const int v = cp.lo + (int)(blockIdx.x * 256 + threadIdx.x);
+    if (v > cp.hi) return;
+    const std::size_t Pc = (std::size_t)iic * jjc * kkc;
+    for (int s = 0; s < cp.n; ++s) {
+        int d[4], r[4];
+        for (int a = 0; a < 4; ++a) {
+            d[a] = cp.dst[s][a] == -1 ? v : cp.dst[s][a];
+            r[a] = cp.src[s][a] == -1 ? v : cp.src[s][a];
+        }
+        CI3(d[0], d[1], d[2], d[3]) = CI3(r[0], r[1], r[2], r[3]);
+    }
+}
+
+// Galerkin product with periodic directions (bit 0 x, 1 y, 2 z of pm): the fine points of the
+// product are taken in the unwrapped geometry between X and Y, so each stored coupling is the
+// one across its own side, as the reference's hand-expanded products form it; only the array
+// reads wrap (coarse indices into 2..n+1, fine points into the interior).  Summation order as
+// in kpair3.
+__device__ inline int wrap_in(int n, int m) { return ((n - 2) % m + m) % m + 2; }
+
+template <bool NINE>
+__device__ double kpair3w(const double *__restrict__ so, const double *__restrict__ ci, int iif,
+                          int jjf, int kkf, int iic, int jjc, int kkc, int icx, int jcx, int kcx,
+                          int DX, int DY, int DZ, int pm) {
+    const int ii = iif, jj = jjf;
+    const std::size_t P = (std::size_t)iif * jjf * kkf, Pc = (std::size_t)iic * jjc * kkc;
+    const bool px = pm & 1, py = pm & 2, pz = pm & 4;
+    if (px) icx = wrap_in(icx, iic - 2);
+    if (py) jcx = wrap_in(jcx, jjc - 2);
+    if (pz) kcx = wrap_in(kcx, kkc - 2);
+    int icy = icx + DX, jcy = jcx + DY, kcy = kcx + DZ;
+    if (px) icy = wrap_in(icy, iic - 2);
+    if (py) jcy = wrap_in(jcy, jjc - 2);
+    if (pz) kcy = wrap_in(kcy, kkc - 2);
+    const int X0 = 2 * (icx - 1), Y0 = 2 * (jcx - 1), Z0 = 2 * (kcx - 1);
+    double sum = 0.0;
+#pragma unroll 1
+    for (int fz = -1; fz <= 1; ++fz)
+#pragma unroll 1
+        for (int fy = -1; fy <= 1; ++fy)
+#pragma unroll 1
+            for (int fx = -1; fx <= 1; ++fx) {
+                int x = X0 + fx, y = Y0 + fy, z = Z0 + fz;
+                if (x < 2 || x > iif - 1) {
+                    if (!px) continue;
+                    x = wrap_in(x, iif - 2);
+                }
+                if (y < 2 || y > jjf - 1) {
+                    if (!py) continue;
+                    y = wrap_in(y, jjf - 2);
+                }
+                if (z < 2 || z > kkf - 1) {
+                    if (!pz) continue;
+                    z = wrap_in(z, kkf - 2);
+                }
+                double inner = 0.0;
+                bool any = false;
+#pragma unroll 1
+                for (int oz = -1; oz <= 1; ++oz)
+#pragma unroll 1
+                    for (int oy = -1; oy <= 1; ++oy)
+#pragma unroll 1
+                        for (int ox = -1; ox <= 1; ++ox) {
+                            const int gx = fx + ox - 2 * DX, gy = fy + oy - 2 * DY, gz = fz + oz - 2 * DZ;
+                            if (gx < -1 || gx > 1 || gy < -1 || gy > 1 || gz < -1 || gz > 1) continue;
+                            const int xg = x + ox, yg = y + oy, zg = z + oz;
+                            if ((!px && (xg < 2 || xg > iif - 1)) || (!py && (yg < 2 || yg > jjf - 1)) ||
+                                (!pz && (zg < 2 || zg > kkf - 1)))
+                                continue;
+                            if (!NINE && (ox != 0) + (oy != 0) + (oz != 0) > 1) continue;
+                            inner += aentry3<NINE>(so, P, ii, jj, x, y, z, ox, oy, oz) *
+                                     pweight3(ci, Pc, iic, jjc, icy, jcy, kcy, gx, gy, gz);
+                            any = true;
+                        }
+                if (any) sum += pweight3(ci, Pc, iic, jjc, icx, jcx, kcx, fx, fy, fz) * inner;
+            }
+    return sum;
+}
+
+template <bool NINE>
+__global__ __launch_bounds__(256) void galerkin3_per_k(const double *__restrict__ so,
+                                                       const double *__restrict__ ci,
+                                                       double *__restrict__ soc, int iif, int jjf,
+                                                       int kkf, int iic, int jjc, int kkc, int pm) {
+    const int ic = blockIdx.x * 64 + (threadIdx.x % 64) + 2;
+    const int jc = blockIdx.y * 4 + (threadIdx.x / 64) + 2;
+    const int kc = blockIdx.z + 2;
+    if (ic > iic - 1 || jc > jjc - 1) return;
+    const std::size_t Pc = (std::size_t)iic * jjc * kkc, pc = ix3(ic, jc, kc, iic, jjc);
+    soc[pc] = kpair3w<NINE>(so, ci, iif, jjf, kkf, iic, jjc, kkc, ic, jc, kc, 0, 0, 0, pm);
+#pragma unroll 1
+    for (int t = 0; t < 13; ++t) {
+        const int xa = ic + CP_A[t][0], ya = jc + CP_A[t][1], za = kc + CP_A[t][2];
+        const int xb = ic + CP_B[t][0], yb = jc + CP_B[t][1], zb = kc + CP_B[t][2];
+        double v = 0.0;
+        if (((pm & 1) || (xa >= 2 && xb >= 2)) && ((pm & 2) || (ya >= 2 && yb >= 2)) &&
+            ((pm & 4) || (za >= 2 && zb >= 2)))
+            v = -kpair3w<NINE>(so, ci, iif, jjf, kkf, iic, jjc, kkc, xa, ya, za, xb - xa, yb - ya,
+                               zb - za, pm);
+        soc[(std::size_t)(CP_D[t] - 1) * Pc + pc] = v;
+    }
+}
+
+// images of the coarse operator, all 14 planes, as the hand-expanded products end
+// (src/3d/ftn/BMG3_SymStd_SETUP_ITLI07_ex.f90:1002-1113): per line of `axis`, the upper image
+// from 2 first, then the lower image from n-1
+__global__ __launch_bounds__(256) void soc_images_k(double *__restrict__ soc, int ii, int jj, int kk,
+                                                    int axis) {
+    const int na = axis == 0 ? jj : ii, nb = axis == 2 ? jj : kk;
+    const int t = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (t >= na * nb) return;
+    const int a = t % na + 1, b = t / na + 1;
+    const std::size_t P = (std::size_t)ii * jj * kk;
+    auto at = [&](int u, int d) -> double & {
+        const std::size_t p = axis == 0 ? ix3(u, a, b, ii, jj) : axis == 1 ? ix3(a, u, b, ii, jj)
+                                                                           : ix3(a, b, u, ii, jj);
+        return soc[(std::size_t)(d - 1) * P + p];
+    };
+    const int n = axis == 0 ? ii : axis == 1 ? jj : kk;
+    for (int d = 1; d <= 14; ++d) at(n, d) = at(2, d);
+    for (int d = 1; d <= 14; ++d) at(1, d) = at(n - 1, d);
+}
+
+// coarsest grid with periodic directions: the dense matrix of the operator with its periodic
+// couplings (row r's terms in the residual's order, a term whose neighbour wraps onto c added
+// at (r, c), r <= c kept), then a Cholesky in DPOTF2's order (src/3d/ftn/
+// BMG3_SymStd_SETUP_cg_LU.f90:190-612 writes the same matrix term by term and calls DPOTRF)
+__constant__ signed char NB_SI[26] = {0, 0, 0, 1, 1, 1, 0, 0, 0, 0, 0, 0, 1, 1, 1, 0, 0, 0, 0, 0, 0, 1, 1, 1, 0, 0};
+__constant__ signed char NB_SJ[26] = {0, 1, 1, 1, 0, 0, 0, 0, 0, 0, 1, 1, 1, 0, 0, 0, 0, 0, 0, 1, 1, 1, 0, 0, 0, 0};
+__constant__ signed char NB_SK[26] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+__constant__ signed char NB_D[26] = {kpw, kpnw, kps, kpsw, kpw, kpnw, kps, kpsw, kb, kbw, kbnw, kbn, kbne,
+                                     kbe, kbse, kbs, kbsw, kb, kbe, kbse, kbs, kbsw, kbw, kbnw, kbn, kbne};
+__constant__ signed char NB_OI[26] = {-1, -1, 0, 1, 1, 1, 0, -1, 0, -1, -1, 0, 1, 1, 1, 0, -1, 0, -1, -1, 0, 1, 1, 1, 0, -1};
+__constant__ signed char NB_OJ[26] = {0, 1, 1, 1, 0, -1, -1, -1, 0, 0, 1, 1, 1, 0, -1, -1, -1, 0, 0, 1, 1, 1, 0, -1, -1, -1};
+__constant__ signed char NB_OK[26] = {0, 0, 0, 0, 0, 0, 0, 0, -1, -1, -1, -1, -1, -1, -1, -1, -1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+__constant__ signed char NB7_T[6] = {0, 2, 4, 6, 8, 17};
+
+__global__ __launch_bounds__(256) void setup_cg3_per_k(const double *__restrict__ so, int nstncl,
+                                                       double *__restrict__ abd, int ii, int jj,
+                                                       int kk, int pm, int *__restrict__ info) {
+    const int nx = ii - 2, ny = jj - 2, nz = kk - 2, n = nx * ny * nz;
+    const std::size_t P = (std::size_t)ii * jj * kk;
+    for (std::size_t t = threadIdx.x; t < (std::size_t)n * n; t += blockDim.x) abd[t] = 0.0;
+    __syncthreads();
+    auto A = [&](int r, int c) -> double & { return abd[(std::size_t)c * n + r]; };  // 0-based
+    const int nt = nstncl == 14 ? 26 : 6;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {  // row r is written by its thread only
+        const int i = r % nx + 2, j = (r / nx) % ny + 2, k = r / (nx * ny) + 2;
+        A(r, r) += S3(i, j, k, kp);
+        for (int u = 0; u < nt; ++u) {
+            const int t = nstncl == 14 ? u : NB7_T[u];
+            const double a = -S3(i + NB_SI[t], j + NB_SJ[t], k + NB_SK[t], NB_D[t]);
+            int ci_ = i + NB_OI[t], cj = j + NB_OJ[t], ck = k + NB_OK[t];
+            if (ci_ < 2 || ci_ > ii - 1) {
+                if (!(pm & 1)) continue;
+                ci_ = ci_ < 2 ? ci_ + nx : ci_ - nx;
+            }
+            if (cj < 2 || cj > jj - 1) {
+                if (!(pm & 2)) continue;
+                cj = cj < 2 ? cj + ny : cj - ny;
+            }
+            if (ck < 2 || ck > kk - 1) {
+                if (!(pm & 4)) continue;
+                ck = ck < 2 ? ck + nz : ck - nz;
+            }
+            const int c = (ci_ - 2) + nx * ((cj - 2) + ny * (ck - 2));
+            if (r <= c) A(r, c) += a;
+        }
+    }
+    __syncthreads();
+    __shared__ double piv;
+    __shared__ int bad;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    for (int j = 0; j < n; ++j) {
+        if (threadIdx.x == 0) {
+            double ajj = A(j, j);
+            for (int k = 0; k < j; ++k) ajj -= A(k, j) * A(k, j);
+            if (!(ajj > 0.0)) {
+                A(j, j) = ajj;
+                bad = j + 1;
+            } else {
+                ajj = sqrt(ajj);
+                A(j, j) = ajj;
+            }
+            piv = ajj;
+        }
+        __syncthreads();
+        if (bad) break;
+        for (int c = j + 1 + (int)threadIdx.x; c < n; c += (int)blockDim.x) {
+            double s = A(j, c);
+            for (int k = 0; k < j; ++k) s -= A(k, j) * A(k, c);
+            A(j, c) = s / piv;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && bad) atomicMax(info, bad);
+}
+
+// DPOTRS on the dense factor, then the mean removal and the images in the order x, y, z
+// (src/3d/ftn/BMG3_SymStd_SOLVE_cg.f90:103-196); one lane (the sums are sequential)
+template <typename T>
+__global__ void solve_cg3_per_k(T *__restrict__ q, const T *__restrict__ qf,
+                                const double *__restrict__ abd, double *__restrict__ x, int ii,
+                                int jj, int kk, int pm) {
+    if (threadIdx.x != 0) return;
+    const int nx = ii - 2, ny = jj - 2, n = nx * ny * (kk - 2);
+    auto A = [&](int r, int c) { return abd[(std::size_t)c * n + r]; };
+    for (int t = 0; t < n; ++t) x[t] = (double)Q3(qf, t % nx + 2, (t / nx) % ny + 2, t / (nx * ny) + 2);
+    for (int j = 0; j < n; ++j) {
+        double s = x[j];
+        for (int k = 0; k < j; ++k) s -= A(k, j) * x[k];
+        x[j] = s / A(j, j);
+    }
+    for (int j = n - 1; j >= 0; --j) {
+        double s = x[j];
+        for (int k = j + 1; k < n; ++k) s -= A(j, k) * x[k];
+        x[j] = s / A(j, j);
+    }
+    // the mean of the FP64 solution, removed before q takes it (rounded once in FP32)
+    double cint = 0.0, qint = 0.0;
+    for (int t = 0; t < n; ++t) {
+        qint = qint + x[t];
+        cint = cint + 1.0;
+    }
+    const double c = -qint / cint;
+    for (int t = 0; t < n; ++t) Q3(q, t % nx + 2, (t / nx) % ny + 2, t / (nx * ny) + 2) = (T)(x[t] + c);
+    if (pm & 1)
+        for (int k = 1; k <= kk; ++k)
+            for (int j = 1; j <= jj; ++j) {
+                Q3(q, 1, j, k) = Q3(q, ii - 1, j, k);
+                Q3(q, ii, j, k) = Q3(q, 2, j, k);
+            }
+    if (pm & 2)
+        for (int k = 1; k <= kk; ++k)
+            for (int i = 1; i <= ii; ++i) {
+                Q3(q, i, 1, k) = Q3(q, i, jj - 1, k);
+                Q3(q, i, jj, k) = Q3(q, i, 2, k);
+            }
+    if (pm & 4)
+        for (int j = 1; j <= jj; ++j)
+            for (int i = 1; i <= ii; ++i) {
+                Q3(q, i, j, 1) = Q3(q, i, j, kk - 1);
+                Q3(q, i, j, kk) = Q3(q, i, j, 2);
+            }
+}
+
+// ------------------------------------------------------------------------------------------
 // plane relaxation data movement (include/cedar/3d/relax_planes.h:75-163,
 // src/3d/relax_planes.cc:25-238); C++ indices here (ghost = 0), grid z = batch entry b,
 // plane ipl = first + 2b.  Plane extents: XY (ii, jj), XZ (ii, kk), YZ (jj, kk).
@@ -1186,6 +1513,290 @@ void solve_cg(T *q, const T *qf, const double *abd, int ldab, double *bbd, int i
     solve_cg3_k<T><<<1, 64, 0, s>>>(q, qf, abd, ldab, bbd, ii, jj, kk);
 }
 
+// ---- periodic boundaries -------------------------------------------------------------------
+int per_mask3(int jpn) {
+    const int p = jpn < 0 ? -jpn : jpn;
+    int m = 0;
+    if (p == 2 || p == 3 || p == 6 || p == 8) m |= 1;
+    if (p == 1 || p == 3 || p == 7 || p == 8) m |= 2;
+    if (p == 5 || p == 6 || p == 7 || p == 8) m |= 4;
+    return m;
+}
+
+int bc_from_mask3(int mask) {
+    static const int map[8] = {0, 2, 1, 3, 5, 6, 7, 8};  // BMG_get_bc
+    return mask >= 0 && mask < 8 ? map[mask] : -1;
+}
+
+namespace {
+template <typename T>
+void images_axis(T *q, int ii, int jj, int kk, int axis, int a0, int astep, int na, int b0,
+                 int bstep, int nb, hipStream_t s) {
+    if (na <= 0 || nb <= 0) return;
+    images3_k<T><<<(unsigned)((na * nb + 255) / 256), 256, 0, s>>>(q, ii, jj, kk, axis, a0, astep, na,
+                                                                  b0, bstep, nb);
+}
+}  // namespace
+
+template <typename T>
+void images(T *q, int ii, int jj, int kk, int jpn, int order_xyz, hipStream_t s) {
+    const int m = per_mask3(jpn);
+    const int seq[3] = {order_xyz ? 0 : 1, order_xyz ? 1 : 0, 2};
+    for (int axis : seq) {
+        if (!(m & (1 << axis))) continue;
+        if (axis == 0) images_axis<T>(q, ii, jj, kk, 0, 1, 1, jj, 1, 1, kk, s);
+        if (axis == 1) images_axis<T>(q, ii, jj, kk, 1, 1, 1, ii, 1, 1, kk, s);
+        if (axis == 2) images_axis<T>(q, ii, jj, kk, 2, 1, 1, ii, 1, 1, jj, s);
+    }
+}
+
+// src/3d/ftn/BMG3_SymStd_relax_GS.f90:189-352: after each colour, the x images of the rows and
+// the y images of the planes that the colour visited; the z images after all colours
+template <typename T>
+void relax_gs_per(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
+                  int kk, int updown, int jpn, hipStream_t s) {
+    const int m = per_mask3(jpn);
+    if (nstncl == 14) {
+        for (int t = 0; t < 8; ++t) {
+            const int pts = updown == 0 ? 8 - t : 1 + t;
+            const int pi = 2 + (pts - 1) % 2, pj = 2 + ((pts - 1) / 2) % 2, pk = 2 + ((pts - 1) / 4) % 2;
+            auto count = [](int first, int last) { return last >= first ? (last - first) / 2 + 1 : 0; };
+            const int ni = count(pi, ii - 1), nj = count(pj, jj - 1), nk = count(pk, kk - 1);
+            if (ni > 0 && nj > 0 && nk > 0)
+                gs27_colour_k<T><<<grid3(ni, nj, nk), 256, 0, s>>>(so, qf, q, sor, ii, jj, kk, pi, pj,
+                                                                   pk, t & 1);
+            if (m & 1) images_axis<T>(q, ii, jj, kk, 0, pj, 2, nj, pk, 2, nk, s);
+            if (m & 2) images_axis<T>(q, ii, jj, kk, 1, 1, 1, ii, pk, 2, nk, s);
+        }
+    } else {
+        for (int t = 0; t < 2; ++t) {
+            const int pts = updown == 0 ? 1 - t : t;
+            gs7_colour_k<T><<<grid3((ii - 1) / 2, jj - 2, kk - 2), 256, 0, s>>>(so, qf, q, sor, ii, jj, kk,
+                                                                                pts, t & 1);
+            if (m & 1) images_axis<T>(q, ii, jj, kk, 0, 2, 1, jj - 2, 2, 1, kk - 2, s);
+            if (m & 2) images_axis<T>(q, ii, jj, kk, 1, 1, 1, ii, 2, 1, kk - 2, s);
+        }
+    }
+    if (m & 4) images_axis<T>(q, ii, jj, kk, 2, 1, 1, ii, 1, 1, jj, s);
+}
+
+// src/3d/ftn/BMG3_SymStd_restrict.f90:79-141: the images of q first (y, x, z), in place
+template <typename T>
+void restrict_per(const T *ci, T *q, T *qc, int iif, int jjf, int kkf, int iic, int jjc, int kkc,
+                  int jpn, hipStream_t s) {
+    images<T>(q, iif, jjf, kkf, jpn, 0, s);
+    restrict3_k<T><<<grid3(iic - 2, jjc - 2, kkc - 2), 256, 0, s>>>(ci, q, qc, iif, jjf, kkf, iic, jjc, kkc);
+}
+
+// interp_add, then every image of the updated q (the reference's own image loops read an
+// unassigned IPN and left-over loop indices, src/3d/ftn/BMG3_SymStd_interp_add.f90:237-280)
+template <typename T>
+void interp_add_per(T *q, const T *qc, const T *res, const T *so, int nstncl, const T *ci,
+                    int iic, int jjc, int kkc, int iif, int jjf, int kkf, int jpn, hipStream_t s) {
+    interp_add<T>(q, qc, res, so, nstncl, ci, iic, jjc, kkc, iif, jjf, kkf, s);
+    images<T>(q, iif, jjf, kkf, jpn, 0, s);
+}
+
+// src/3d/ftn/BMG3_SymStd_SETUP_interp_OI.f90:809-2784, loop nest by loop nest (see oi_nest_k);
+// the 27- and 7-point branches differ in three statements (:1057-1065 / :2111-2119,
+// :1274 / :2269, :1391 / :2359), followed as written
+void setup_interp_per(const double *so, int nstncl, double *ci, int iif, int jjf, int kkf, int iic,
+                      int jjc, int kkc, int jpn, hipStream_t s) {
+    const bool nine = nstncl == 14;
+    const int iic1 = iic - 1, jjc1 = jjc - 1, kkc1 = kkc - 1;
+    const int iif1 = iif - 1, jjf1 = jjf - 1, kkf1 = kkf - 1;
+    const int iicf = (iif - 2) / 2 + 3, jjcf = (jjf - 2) / 2 + 3, kkcf = (kkf - 2) / 2 + 3;
+    const int iifc = 2 * (iicf - 2) + 2, jjfc = 2 * (jjcf - 2) + 2, kkfc = 2 * (kkcf - 2) + 2;
+    const int m = per_mask3(jpn);
+    const bool px = m & 1, py = m & 2, pz = m & 4;
+    int ibeg = 2, ibegc = 3, ibeg_x = 2, iend_x = iif - 2, iendc = iicf - 1;
+    int jbeg = 2, jbegc = 3, jbeg_y = 2, jend_y = jjf - 2, jendc = jjcf - 1;
+    int kbeg = 2, kbegc = 3, kbeg_z = 2, kend_z = kkf - 2, kendc = kkcf - 1;
+    if (px) {
+        ibeg = 0;
+        ibegc = 2;
+        if (iic == iicf) { iendc = iicf; ibeg_x = 3; iend_x = iif1; }
+    }
+    if (py) {
+        jbeg = 0;
+        jbegc = 2;
+        if (jjc == jjcf) { jendc = jjcf; jbeg_y = 3; jend_y = jjf1; }
+    }
+    if (pz) {
+        kbeg = 0;
+        kbegc = 2;
+        if (kkc == kkcf) { kendc = kkcf; kbeg_z = 3; kend_z = kkf1; }
+    }
+    auto REG = [](int c0, int c1) { return oi_axis{0, c0, c1, 0, 1}; };
+    auto WRP = [](int c0, int c1, int f0, int mod) { return oi_axis{1, c0, c1, f0, mod}; };
+    auto FIX = [](int c, int f) { return oi_axis{2, c, c, f, 1}; };
+    const oi_axis WX = WRP(ibegc, iendc, ibeg, iifc), WY = WRP(jbegc, jendc, jbeg, jjfc);
+    const oi_axis WZ = WRP(kbegc, kendc, kbeg, kkfc);
+    auto nest = [&](int type, oi_axis ax, oi_axis ay, oi_axis az) {
+        auto cnt = [](const oi_axis &a) { return a.mode == 2 ? 1 : a.c1 - a.c0 + 1; };
+        const int nx = cnt(ax), ny = cnt(ay), nz = cnt(az);
+        if (nx <= 0 || ny <= 0 || nz <= 0) return;
+        const unsigned g = (unsigned)(((long long)nx * ny * nz + 255) / 256);
+#define OI_NEST(N, TY)                                                                              \
+    oi_nest_k<N, TY><<<g, 256, 0, s>>>(so, ci, iif, jjf, kkf, iic, jjc, kkc, ax, ay, az, nx, ny, nz)
+#define OI_NEST2(TY)          \
+    if (nine)                 \
+        OI_NEST(true, TY);    \
+    else                      \
+        OI_NEST(false, TY);
+        switch (type) {
+        case 0: OI_NEST2(0) break;
+        case 1: OI_NEST2(1) break;
+        case 2: OI_NEST2(2) break;
+        case 3: OI_NEST2(3) break;
+        case 4: OI_NEST2(4) break;
+        case 5: OI_NEST2(5) break;
+        default: OI_NEST2(6)
+        }
+#undef OI_NEST2
+#undef OI_NEST
+    };
+    constexpr int V = -1;  // the loop variable
+    struct stmt {
+        int d[4], r[4];
+    };
+    auto copy = [&](int lo, int hi, std::initializer_list<stmt> st) {
+        if (hi < lo) return;
+        ci_copy cp{};
+        cp.n = 0;
+        cp.lo = lo;
+        cp.hi = hi;
+        for (const stmt &x : st) {
+            for (int a = 0; a < 4; ++a) {
+                cp.dst[cp.n][a] = x.d[a];
+                cp.src[cp.n][a] = x.r[a];
+            }
+            ++cp.n;
+        }
+        ci_copy_k<<<(unsigned)((hi - lo + 1 + 255) / 256), 256, 0, s>>>(ci, iic, jjc, kkc, cp);
+    };
+
+    // fine points on coarse k-planes and fine-only x-lines (lxyl, lxyr): :930-1106
+    nest(0, WX, REG(2, jjc1), REG(2, kkc1));
+    if (py) {
+        nest(0, WX, FIX(jjc, jbeg_y), REG(2, kkc1));
+        nest(0, WX, FIX(1, jend_y), REG(2, kkc1));
+    }
+    if (pz) {
+        nest(0, WX, REG(2, jjc1), FIX(kkc, kbeg_z));
+        if (py) {
+            if (nine) {  // jc is left at jjc by the loop before; one pass of the repeated copies
+                if (ibegc <= iendc)
+                    copy(0, 0, {{{1, jjc, kkc, lxyl}, {iic1, jjc, kkc, lxyl}},
+                                {{1, jjc, kkc, lxyr}, {iic1, jjc, kkc, lxyr}},
+                                {{iic, jjc, kkc, lxyl}, {2, jjc, kkc, lxyl}},
+                                {{iic, jjc, kkc, lxyr}, {2, jjc, kkc, lxyr}}});
+            } else {
+                copy(ibegc, iendc, {{{V, 1, kkc, lxyl}, {iic, jjc1, kkc, lxyl}},
+                                    {{V, 1, kkc, lxyr}, {iic, jjc1, kkc, lxyr}},
+                                    {{V, jjc, kkc, lxyl}, {V, 2, kkc, lxyl}},
+                                    {{V, jjc, kkc, lxyr}, {V, 2, kkc, lxyr}}});
+            }
+        }
+        nest(0, WX, REG(2, jjc1), FIX(1, kend_z));
+        if (py)
+            copy(ibegc, iendc, {{{V, 1, 1, lxyl}, {V, jjc1, 1, lxyl}},
+                                {{V, 1, 1, lxyr}, {V, jjc1, 1, lxyr}},
+                                {{V, jjc, 1, lxyl}, {V, 2, 1, lxyl}},
+                                {{V, jjc, 1, lxyr}, {V, 2, 1, lxyr}}});
+    }
+
+    // fine points on coarse k-planes and fine-only y-lines (lxya, lxyb): :1109-1277
+    nest(1, REG(2, iic1), WY, REG(2, kkc1));
+    if (px) {
+        nest(1, FIX(iic, ibeg_x), WY, REG(2, kkc1));
+        nest(1, FIX(1, iend_x), WY, REG(2, kkc1));
+    }
+    if (pz) {
+        nest(1, REG(2, iic1), WY, FIX(kkc, kbeg_z));
+        if (px)
+            copy(jbegc, jendc, {{{1, V, kkc, lxya}, {iic1, V, kkc, lxya}},
+                                {{1, V, kkc, lxyb}, {iic1, V, kkc, lxyb}},
+                                {{iic, V, kkc, lxya}, {2, V, kkc, lxya}},
+                                {{iic, V, kkc, lxyb}, {2, V, kkc, lxyb}}});
+        nest(1, REG(2, iic1), WY, FIX(1, kend_z));
+        if (px)
+            copy(jbegc, jendc, {{{1, V, 1, lxya}, {iic1, V, 1, lxya}},
+                                {{1, V, 1, lxyb}, {iic1, V, 1, lxyb}},
+                                {{iic, V, 1, lxya}, {2, V, 1, lxya}},
+                                {{iic, V, 1, lxyb}, {nine ? 1 : 2, V, 1, lxyb}}});
+    }
+
+    // fine points on coarse j-planes and fine-only z-lines (lxza, lxzb): :1279-1442
+    nest(2, REG(2, iic1), REG(2, jjc1), WZ);
+    if (px) {
+        for (int side = 0; side < 2; ++side) {
+            const int icx = side == 0 ? iic : 1;
+            nest(2, FIX(icx, side == 0 ? ibeg_x : iend_x), REG(2, jjc1), WZ);
+            if (py)
+                copy(kbegc, kendc, {{{icx, 1, V, lxza}, {icx, jjc1, V, lxza}},
+                                    {{icx, 1, V, lxzb}, {icx, jjc1, V, lxzb}},
+                                    {{icx, jjc, V, lxza}, {icx, 2, V, lxza}},
+                                    {{icx, jjc, V, lxzb}, {icx, 2, V, lxzb}}});
+        }
+    }
+    if (py) {
+        nest(2, REG(2, nine ? iic1 : jjc1), FIX(jjc, jbeg_y), WZ);
+        nest(2, REG(2, iic1), FIX(1, jend_y), WZ);
+    }
+
+    // fine points on coarse k-planes, fine-only x- and y-lines: :1444-1583
+    nest(3, WX, WY, REG(2, kkc1));
+    if (pz) {
+        nest(3, WX, WY, FIX(kkc, kbeg_z));
+        nest(3, WX, WY, FIX(1, kend_z));
+    }
+    // fine points on coarse j-planes, fine-only x- and z-lines: :1587-1720
+    nest(4, WX, REG(2, jjc1), WZ);
+    if (py) {
+        nest(4, WX, FIX(jjc, jbeg_y), WZ);
+        nest(4, WX, FIX(1, jend_y), WZ);
+    }
+    // fine points on coarse i-planes, fine-only y- and z-lines, J advanced with the z
+    // modulus: :1723-1859
+    const oi_axis WYZ = WRP(jbegc, jendc, jbeg, kkfc);
+    nest(5, REG(2, iic1), WYZ, WZ);
+    if (px) {
+        nest(5, FIX(iic, ibeg_x), WYZ, WZ);
+        nest(5, FIX(1, iend_x), WYZ, WZ);
+    }
+    // cell centres: :1862-2013
+    nest(6, WX, WY, WZ);
+}
+
+// Galerkin product over the wrapped geometry, then the coarse images (y, x, z) as
+// src/3d/ftn/BMG3_SymStd_SETUP_ITLI07_ex.f90:1002-1113 copies them
+void setup_galerkin_per(const double *so, int nstncl, const double *ci, double *soc, int iif,
+                        int jjf, int kkf, int iic, int jjc, int kkc, int jpn, hipStream_t s) {
+    const int m = per_mask3(jpn);
+    const dim3 g = grid3(iic - 2, jjc - 2, kkc - 2);
+    if (nstncl == 14)
+        galerkin3_per_k<true><<<g, 256, 0, s>>>(so, ci, soc, iif, jjf, kkf, iic, jjc, kkc, m);
+    else
+        galerkin3_per_k<false><<<g, 256, 0, s>>>(so, ci, soc, iif, jjf, kkf, iic, jjc, kkc, m);
+    for (int axis : {1, 0, 2}) {
+        if (!(m & (1 << axis))) continue;
+        const int na = axis == 0 ? jjc : iic, nb = axis == 2 ? jjc : kkc;
+        soc_images_k<<<(unsigned)((na * nb + 255) / 256), 256, 0, s>>>(soc, iic, jjc, kkc, axis);
+    }
+}
+
+void setup_cg_per(const double *so, int nstncl, double *abd, int ii, int jj, int kk, int jpn,
+                  int *info, hipStream_t s) {
+    setup_cg3_per_k<<<1, 256, 0, s>>>(so, nstncl, abd, ii, jj, kk, per_mask3(jpn), info);
+}
+
+template <typename T>
+void solve_cg_per(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj, int kk,
+                  int jpn, hipStream_t s) {
+    solve_cg3_per_k<T><<<1, 64, 0, s>>>(q, qf, abd, bbd, ii, jj, kk, per_mask3(jpn));
+}
+
 static dim3 plane_grid(int dir, int ii, int jj, int kk, int npl, bool interior) {
     const int n1 = dir == YZ ? jj : ii, n2 = dir == XY ? jj : kk;
     dim3 g = interior ? grid3(n1 - 2, n2 - 2, 1) : grid3(n1, n2, 1);
@@ -1232,7 +1843,15 @@ void copy_rhs(int dir, const T *so3, int nstncl3, const T *x, const T *b, T *b2,
     template void copy32<T>(int, const T *, T *, int, int, int, int, int, hipStream_t);             \
     template void copy23<T>(int, const T *, T *, int, int, int, int, int, hipStream_t);             \
     template void copy_rhs<T>(int, const T *, int, const T *, const T *, T *, int, int, int, int, int, \
-                              hipStream_t);
+                              hipStream_t);                                                         \
+    template void images<T>(T *, int, int, int, int, int, hipStream_t);                             \
+    template void relax_gs_per<T>(const T *, int, const T *, T *, const T *, int, int, int, int, int, \
+                                  hipStream_t);                                                     \
+    template void restrict_per<T>(const T *, T *, T *, int, int, int, int, int, int, int, hipStream_t); \
+    template void interp_add_per<T>(T *, const T *, const T *, const T *, int, const T *, int, int,  \
+                                    int, int, int, int, int, hipStream_t);                           \
+    template void solve_cg_per<T>(T *, const T *, const double *, double *, int, int, int, int,      \
+                                  hipStream_t);
 CDR3_INST(double)
 CDR3_INST(float)
 #undef CDR3_INST

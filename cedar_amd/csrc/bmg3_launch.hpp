@@ -60,6 +60,35 @@ template <typename T>
 void solve_cg(T *q, const T *qf, const double *abd, int ldab, double *bbd, int ii, int jj, int kk,
               hipStream_t s);
 
+// Periodic boundaries.  jpn is the BoxMG boundary id of the periodic mask (bc_from_mask3:
+// bits x = 1, y = 2, z = 4 -> 2, 1, 3, 5, 6, 7, 8); per_mask3 maps it back.  Images live in the
+// ghost layer of a periodic direction.  order_xyz: images refreshed x, y, z (else y, x, z).
+int per_mask3(int jpn);
+int bc_from_mask3(int mask);
+template <typename T>
+void images(T *q, int ii, int jj, int kk, int jpn, int order_xyz, hipStream_t s);
+template <typename T>
+void relax_gs_per(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
+                  int kk, int updown, int jpn, hipStream_t s);
+// refreshes the images of q in place first
+template <typename T>
+void restrict_per(const T *ci, T *q, T *qc, int iif, int jjf, int kkf, int iic, int jjc, int kkc,
+                  int jpn, hipStream_t s);
+template <typename T>
+void interp_add_per(T *q, const T *qc, const T *res, const T *so, int nstncl, const T *ci,
+                    int iic, int jjc, int kkc, int iif, int jjf, int kkf, int jpn, hipStream_t s);
+// ci must be zero on entry (the periodic loops write only the entries the reference writes)
+void setup_interp_per(const double *so, int nstncl, double *ci, int iif, int jjf, int kkf, int iic,
+                      int jjc, int kkc, int jpn, hipStream_t s);
+void setup_galerkin_per(const double *so, int nstncl, const double *ci, double *soc, int iif,
+                        int jjf, int kkf, int iic, int jjc, int kkc, int jpn, hipStream_t s);
+// dense n x n factor (n = nx ny nz, column-major, upper triangle); *info as for setup_cg
+void setup_cg_per(const double *so, int nstncl, double *abd, int ii, int jj, int kk, int jpn,
+                  int *info, hipStream_t s);
+template <typename T>
+void solve_cg_per(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj, int kk,
+                  int jpn, hipStream_t s);
+
 // Plane relaxation pieces for the planes first, first+2, ... (npl of them) of family dir; the
 // npl 2D arrays of the batch are stored back to back, plane b holding plane first + 2b.
 // Plane extents: XY (ii, jj), XZ (ii, kk), YZ (jj, kk).

@@ -67,6 +67,55 @@ int solve_cg3_t(T *q, const T *qf, const double *abd, int ldab, double *bbd, int
     return launched();
 }
 
+// a periodic boundary id (BMG_get_bc of a nonzero mask, or its indefinite negative)
+inline bool jpn3_ok(int jpn) { return jpn != 0 && k3::per_mask3(jpn) != 0; }
+// the periodic coarsening pairs n fine points with n/2 coarse ones: even periodic extents
+inline bool even_per3(int jpn, int iif, int jjf, int kkf) {
+    const int m = k3::per_mask3(jpn);
+    return (!(m & 1) || (iif - 2) % 2 == 0) && (!(m & 2) || (jjf - 2) % 2 == 0) &&
+           (!(m & 4) || (kkf - 2) % 2 == 0);
+}
+constexpr int kDenseMax3 = 2048;  // coarsest unknowns of the periodic dense factor
+
+template <typename T>
+int gs3_per_t(const T *so, int ns, const T *qf, T *q, const T *sor, int ii, int jj, int kk, int ud,
+              int jpn, void *st) {
+    if (!so || !qf || !q || !sor || !ext3_ok(ii, jj, kk) || !sten3_ok(ns) || !dir_ok(ud) ||
+        !jpn3_ok(jpn))
+        return CDR_ERR_ARG;
+    k3::relax_gs_per<T>(so, ns, qf, q, sor, ii, jj, kk, ud, jpn, S(st));
+    return launched();
+}
+
+template <typename T>
+int restrict3_per_t(const T *ci, T *q, T *qc, int iif, int jjf, int kkf, int iic, int jjc, int kkc,
+                    int jpn, void *st) {
+    if (!ci || !q || !qc || !coarse3_ok(iif, jjf, kkf, iic, jjc, kkc) || !jpn3_ok(jpn))
+        return CDR_ERR_ARG;
+    k3::restrict_per<T>(ci, q, qc, iif, jjf, kkf, iic, jjc, kkc, jpn, S(st));
+    return launched();
+}
+
+template <typename T>
+int interp3_per_t(T *q, const T *qc, const T *res, const T *so, int ns, const T *ci, int iic,
+                  int jjc, int kkc, int iif, int jjf, int kkf, int jpn, void *st) {
+    if (!q || !qc || !res || !so || !ci || !sten3_ok(ns) ||
+        !coarse3_ok(iif, jjf, kkf, iic, jjc, kkc) || !jpn3_ok(jpn))
+        return CDR_ERR_ARG;
+    k3::interp_add_per<T>(q, qc, res, so, ns, ci, iic, jjc, kkc, iif, jjf, kkf, jpn, S(st));
+    return launched();
+}
+
+template <typename T>
+int solve_cg3_per_t(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj, int kk,
+                    int jpn, void *st) {
+    if (!q || !qf || !abd || !bbd || !ext3_ok(ii, jj, kk) || !jpn3_ok(jpn) ||
+        (long long)(ii - 2) * (jj - 2) * (kk - 2) > kDenseMax3)
+        return CDR_ERR_ARG;
+    k3::solve_cg_per<T>(q, qf, abd, bbd, ii, jj, kk, jpn, S(st));
+    return launched();
+}
+
 }  // namespace
 
 extern "C" {
@@ -169,6 +218,80 @@ int cdr_hip_solve_cg3d_f64(double *q, const double *qf, const double *abd, int l
 int cdr_hip_solve_cg3d_f32(float *q, const float *qf, const double *abd, int ldab, double *bbd,
                            int ii, int jj, int kk, void *st) {
     return solve_cg3_t<float>(q, qf, abd, ldab, bbd, ii, jj, kk, st);
+}
+
+
+int cdr_hip_relax_gs3d_per_f64(const double *so, int ns, const double *qf, double *q,
+                               const double *sor, int ii, int jj, int kk, int ud, int jpn,
+                               void *st) {
+    return gs3_per_t<double>(so, ns, qf, q, sor, ii, jj, kk, ud, jpn, st);
+}
+int cdr_hip_relax_gs3d_per_f32(const float *so, int ns, const float *qf, float *q,
+                               const float *sor, int ii, int jj, int kk, int ud, int jpn,
+                               void *st) {
+    return gs3_per_t<float>(so, ns, qf, q, sor, ii, jj, kk, ud, jpn, st);
+}
+
+int cdr_hip_restrict3d_per_f64(const double *ci, double *q, double *qc, int iif, int jjf, int kkf,
+                               int iic, int jjc, int kkc, int jpn, void *st) {
+    return restrict3_per_t<double>(ci, q, qc, iif, jjf, kkf, iic, jjc, kkc, jpn, st);
+}
+int cdr_hip_restrict3d_per_f32(const float *ci, float *q, float *qc, int iif, int jjf, int kkf,
+                               int iic, int jjc, int kkc, int jpn, void *st) {
+    return restrict3_per_t<float>(ci, q, qc, iif, jjf, kkf, iic, jjc, kkc, jpn, st);
+}
+
+int cdr_hip_interp_add3d_per_f64(double *q, const double *qc, const double *res, const double *so,
+                                 int ns, const double *ci, int iic, int jjc, int kkc, int iif,
+                                 int jjf, int kkf, int jpn, void *st) {
+    return interp3_per_t<double>(q, qc, res, so, ns, ci, iic, jjc, kkc, iif, jjf, kkf, jpn, st);
+}
+int cdr_hip_interp_add3d_per_f32(float *q, const float *qc, const float *res, const float *so,
+                                 int ns, const float *ci, int iic, int jjc, int kkc, int iif,
+                                 int jjf, int kkf, int jpn, void *st) {
+    return interp3_per_t<float>(q, qc, res, so, ns, ci, iic, jjc, kkc, iif, jjf, kkf, jpn, st);
+}
+
+int cdr_hip_setup_interp3d_per_f64(const double *so, int ns, double *ci, int iif, int jjf,
+                                   int kkf, int iic, int jjc, int kkc, int jpn, void *st) {
+    if (!so || !ci || !sten3_ok(ns) || !coarse3_ok(iif, jjf, kkf, iic, jjc, kkc) || !jpn3_ok(jpn))
+        return CDR_ERR_ARG;
+    k3::setup_interp_per(so, ns, ci, iif, jjf, kkf, iic, jjc, kkc, jpn, S(st));
+    return launched();
+}
+
+int cdr_hip_setup_galerkin3d_per_f64(const double *so, int ns, const double *ci, double *soc,
+                                     int iif, int jjf, int kkf, int iic, int jjc, int kkc, int jpn,
+                                     void *st) {
+    if (!so || !ci || !soc || !sten3_ok(ns) || !coarse3_ok(iif, jjf, kkf, iic, jjc, kkc) ||
+        !jpn3_ok(jpn) || !even_per3(jpn, iif, jjf, kkf))
+        return CDR_ERR_ARG;
+    k3::setup_galerkin_per(so, ns, ci, soc, iif, jjf, kkf, iic, jjc, kkc, jpn, S(st));
+    return launched();
+}
+
+int cdr_hip_setup_cg3d_per_f64(const double *so, int ns, double *abd, int ii, int jj, int kk,
+                               int jpn, int *info_dev, void *st) {
+    if (!so || !abd || !info_dev || !ext3_ok(ii, jj, kk) || !sten3_ok(ns) || !jpn3_ok(jpn) ||
+        (long long)(ii - 2) * (jj - 2) * (kk - 2) > kDenseMax3)
+        return CDR_ERR_ARG;
+    k2::fill<int>(info_dev, 0, 1, S(st));
+    k3::setup_cg_per(so, ns, abd, ii, jj, kk, jpn, info_dev, S(st));
+    if (hipGetLastError() != hipSuccess) return CDR_ERR_HIP;
+    int info = 0;
+    if (hipMemcpyAsync(&info, info_dev, sizeof(int), hipMemcpyDeviceToHost, S(st)) != hipSuccess)
+        return CDR_ERR_HIP;
+    if (hipStreamSynchronize(S(st)) != hipSuccess) return CDR_ERR_HIP;
+    return info != 0 ? CDR_ERR_NOT_SPD : CDR_OK;
+}
+
+int cdr_hip_solve_cg3d_per_f64(double *q, const double *qf, const double *abd, double *bbd, int ii,
+                               int jj, int kk, int jpn, void *st) {
+    return solve_cg3_per_t<double>(q, qf, abd, bbd, ii, jj, kk, jpn, st);
+}
+int cdr_hip_solve_cg3d_per_f32(float *q, const float *qf, const double *abd, double *bbd, int ii,
+                               int jj, int kk, int jpn, void *st) {
+    return solve_cg3_per_t<float>(q, qf, abd, bbd, ii, jj, kk, jpn, st);
 }
 
 }  // extern "C"

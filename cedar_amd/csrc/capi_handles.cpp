@@ -388,7 +388,7 @@ struct mat_obj {
 struct solver_obj {
     int nd;
     int maxiter;
-    int pmask = 0;   // periodic directions (2D)
+    int pmask = 0;   // periodic directions (per_mask)
     cdr_hip_solver2d *s2 = nullptr;
     cdr_hip_solver3d *s3 = nullptr;
     std::size_t npts;
@@ -591,6 +591,47 @@ void vector_images(double *q, std::size_t ii, std::size_t jj, int mask) {
             q[j * ii] = q[j * ii + nx];
             q[j * ii + nx + 1] = q[j * ii + 1];
         }
+}
+
+// 3D: the images of every periodic direction over the full extents, y, x, then z (what a
+// periodic halo exchange on one process leaves in the ghost layer)
+void images3(double *a, std::size_t ii, std::size_t jj, std::size_t kk, int mask) {
+    auto at = [&](std::size_t i, std::size_t j, std::size_t k) -> double & {
+        return a[i + ii * (j + jj * k)];
+    };
+    if (mask & 2)
+        for (std::size_t k = 0; k < kk; ++k)
+            for (std::size_t i = 0; i < ii; ++i) {
+                at(i, 0, k) = at(i, jj - 2, k);
+                at(i, jj - 1, k) = at(i, 1, k);
+            }
+    if (mask & 1)
+        for (std::size_t k = 0; k < kk; ++k)
+            for (std::size_t j = 0; j < jj; ++j) {
+                at(0, j, k) = at(ii - 2, j, k);
+                at(ii - 1, j, k) = at(1, j, k);
+            }
+    if (mask & 4)
+        for (std::size_t j = 0; j < jj; ++j)
+            for (std::size_t i = 0; i < ii; ++i) {
+                at(i, j, 0) = at(i, j, kk - 2);
+                at(i, j, kk - 1) = at(i, j, 1);
+            }
+}
+// the periodic images of an operator (every plane) or a grid function of either dimension
+void fill_operator_images(double *so, int planes, const std::size_t len[3], int nd, int mask) {
+    if (nd == 2) {
+        operator_images(so, planes, len[0], len[1], mask);
+        return;
+    }
+    const std::size_t P = len[0] * len[1] * len[2];
+    for (int d = 0; d < planes; ++d) images3(so + (std::size_t)d * P, len[0], len[1], len[2], mask);
+}
+void fill_vector_images(double *q, const std::size_t len[3], int nd, int mask) {
+    if (nd == 2)
+        vector_images(q, len[0], len[1], mask);
+    else
+        images3(q, len[0], len[1], len[2], mask);
 }
 
 bool upload(double *dst, const double *src, std::size_t n) {
@@ -891,13 +932,13 @@ int cedar_matvec(cedar_mat mat, cedar_vec x, cedar_vec y) {
         return CEDAR_ERR_OTHER;
     // the reference exchanges the halo of x first (src/2d/interface/c/mat.cc:172-183); on one
     // grid that fills the periodic images
-    const int pmask = m->nd == 2 ? (periodic_mask(m->cfg->root) & 3) : 0;
+    const int pmask = periodic_mask(m->cfg->root) & (m->nd == 2 ? 3 : 7);
     if (m->dirty) {
         const double *src = m->so.data();
         std::vector<double> tmp;
         if (pmask) {
             tmp = m->so;
-            operator_images(tmp.data(), m->nstncl, m->len[0], m->len[1], pmask);
+            fill_operator_images(tmp.data(), m->nstncl, m->len, m->nd, pmask);
             src = tmp.data();
         }
         if (!upload(m->dso.p, src, m->so.size())) return CEDAR_ERR_OTHER;
@@ -907,7 +948,7 @@ int cedar_matvec(cedar_mat mat, cedar_vec x, cedar_vec y) {
     std::vector<double> xtmp;
     if (pmask) {
         xtmp = xv->data;
-        vector_images(xtmp.data(), m->len[0], m->len[1], pmask);
+        fill_vector_images(xtmp.data(), m->len, m->nd, pmask);
         xs = xtmp.data();
     }
     if (!upload(m->dx.p, xs, n) || !upload(m->dy.p, yv->data.data(), n))
@@ -978,8 +1019,12 @@ int cedar_solver_create(cedar_mat mat, cedar_solver *solver) {
         return CEDAR_ERR_CONFIG;
     }
     const int pmask = periodic_mask(root);
-    if (pmask != 0 && (m->nd != 2 || (pmask & 4))) {
-        log_line(LOG_ERROR, "grid.periodic: periodic boundaries are built for 2D problems");
+    if (m->nd == 2 && (pmask & 4)) {
+        log_line(LOG_ERROR, "grid.periodic: a 2D problem has no z direction");
+        return CEDAR_ERR_CONFIG;
+    }
+    if (pmask != 0 && m->nd == 3 && k.relaxation != CDR_RELAX_POINT) {
+        log_line(LOG_ERROR, "grid.periodic: 3D periodic problems take point relaxation");
         return CEDAR_ERR_CONFIG;
     }
     if (get_or(root, "gpu.autotune", false, as_bool)) cdr_hip_autotune(1);
@@ -995,7 +1040,7 @@ int cedar_solver_create(cedar_mat mat, cedar_solver *solver) {
     s->pmask = pmask;
     {
         std::vector<double> op = m->so;
-        if (pmask) operator_images(op.data(), m->nstncl, m->len[0], m->len[1], pmask);
+        if (pmask) fill_operator_images(op.data(), m->nstncl, m->len, m->nd, pmask);
         if (!upload(s->dso.p, op.data(), op.size())) return CEDAR_ERR_OTHER;
     }
     int st;
@@ -1027,6 +1072,7 @@ int cedar_solver_create(cedar_mat mat, cedar_solver *solver) {
         cs.cycle = k.cycle;
         cs.precision = prec;
         cs.stall_cycles = std::max(0, get_or(root, "gpu.stall-cycles", 0, as_int));
+        cs.periodic = pmask;
         // plane-config: its own ml settings when present, else line-xy with one cycle
         // (src/kernel_params.cc:79-85)
         const jvalue *pc = root.member("plane-config");
@@ -1069,8 +1115,8 @@ int cedar_solver_run(cedar_solver solver, cedar_vec x, cedar_vec b) {
     if (s->pmask) {  // images of x and b, as the reference's halo exchange provides them
         xtmp = xv->data;
         btmp = bv->data;
-        vector_images(xtmp.data(), s->len[0], s->len[1], s->pmask);
-        vector_images(btmp.data(), s->len[0], s->len[1], s->pmask);
+        fill_vector_images(xtmp.data(), s->len, s->nd, s->pmask);
+        fill_vector_images(btmp.data(), s->len, s->nd, s->pmask);
         xs = xtmp.data();
         bs = btmp.data();
     }

@@ -120,9 +120,10 @@ struct solve_cg : kernel_base {
 }  // namespace kernels3
 
 // registers the "hip" implementations of the 3D interfaces (both precisions); plane solves use
-// `plane` (the plane-config) with the given precision
+// `plane` (the plane-config) with the given precision; jpn is the BoxMG boundary id of the
+// periodic mask (0: Dirichlet)
 void register_hip_kernels3(kernel_manager &km, int *info_dev, const cdr_solver_settings &plane,
-                           int precision);
+                           int precision, int jpn = 0);
 
 }  // namespace hip
 }  // namespace cedar

@@ -139,8 +139,12 @@ struct hip_residual3 : kernels3::residual<T> {
     }
 };
 
+// jpn: the BoxMG boundary id of the periodic mask (0: Dirichlet); periodic problems run the
+// reference's JPN != 0 branches
 template <typename T>
 struct hip_rbgs3 : kernels3::point_relax<T> {
+    explicit hip_rbgs3(int j = 0) : jpn(j) {}
+    int jpn;
     void setup(const stencil_op3<double> &so, relax_stencil3<double> &sor) override {
         const extents3 e = so.ext;
         k3::setup_recip(so.p, so.nstncl, sor.p, e.ii, e.jj, e.kk, this->stream);
@@ -149,7 +153,11 @@ struct hip_rbgs3 : kernels3::point_relax<T> {
     void run(const stencil_op3<T> &so, grid_func3<T> &x, const grid_func3<T> &b,
              const relax_stencil3<T> &sor, Dir d) override {
         const extents3 e = so.ext;
-        k3::relax_gs<T>(so.p, so.nstncl, b.p, x.p, sor.p, e.ii, e.jj, e.kk, (int)d, this->stream);
+        if (jpn)
+            k3::relax_gs_per<T>(so.p, so.nstncl, b.p, x.p, sor.p, e.ii, e.jj, e.kk, (int)d, jpn,
+                                this->stream);
+        else
+            k3::relax_gs<T>(so.p, so.nstncl, b.p, x.p, sor.p, e.ii, e.jj, e.kk, (int)d, this->stream);
         launched3("relax_gs3");
     }
 };
@@ -184,55 +192,85 @@ struct hip_planes : kernels3::plane_relax<T, PD> {
 };
 
 struct hip_setup_interp3 : kernels3::setup_interp {
+    explicit hip_setup_interp3(int j = 0) : jpn(j) {}
+    int jpn;
     void run(const stencil_op3<double> &fop, const stencil_op3<double> &cop,
              prolong_op3<double> &P) override {
         const extents3 f = fop.ext, c = cop.ext;
-        k3::setup_interp(fop.p, fop.nstncl, P.p, f.ii, f.jj, f.kk, c.ii, c.jj, c.kk,
-                         this->stream);
+        if (jpn)  // the periodic loops write only what the reference writes: start from zero
+            k2::fill<double>(P.p, 0.0, (std::size_t)26 * c.ii * c.jj * c.kk, this->stream);
+        if (jpn)
+            k3::setup_interp_per(fop.p, fop.nstncl, P.p, f.ii, f.jj, f.kk, c.ii, c.jj, c.kk, jpn,
+                                 this->stream);
+        else
+            k3::setup_interp(fop.p, fop.nstncl, P.p, f.ii, f.jj, f.kk, c.ii, c.jj, c.kk,
+                             this->stream);
         P.fine_op = &fop;
         launched3("setup_interp3");
     }
 };
 
 struct hip_galerkin3 : kernels3::coarsen_op {
+    explicit hip_galerkin3(int j = 0) : jpn(j) {}
+    int jpn;
     void run(const prolong_op3<double> &P, const stencil_op3<double> &fop,
              stencil_op3<double> &cop) override {
         const extents3 f = fop.ext, c = cop.ext;
-        k3::setup_galerkin(fop.p, fop.nstncl, P.p, cop.p, f.ii, f.jj, f.kk, c.ii, c.jj, c.kk,
-                           this->stream);
+        if (jpn)
+            k3::setup_galerkin_per(fop.p, fop.nstncl, P.p, cop.p, f.ii, f.jj, f.kk, c.ii, c.jj,
+                                   c.kk, jpn, this->stream);
+        else
+            k3::setup_galerkin(fop.p, fop.nstncl, P.p, cop.p, f.ii, f.jj, f.kk, c.ii, c.jj, c.kk,
+                               this->stream);
         launched3("setup_galerkin3");
     }
 };
 
 template <typename T>
 struct hip_restrict3 : kernels3::restriction<T> {
+    explicit hip_restrict3(int j = 0) : jpn(j) {}
+    int jpn;
     void run(const prolong_op3<T> &R, const grid_func3<T> &fine, grid_func3<T> &coarse) override {
         const extents3 f = fine.ext, c = coarse.ext;
-        k3::restrict_<T>(R.p, fine.p, coarse.p, f.ii, f.jj, f.kk, c.ii, c.jj, c.kk, this->stream);
+        if (jpn)  // refreshes the images of the fine input in place, as the reference does
+            k3::restrict_per<T>(R.p, fine.p, coarse.p, f.ii, f.jj, f.kk, c.ii, c.jj, c.kk, jpn,
+                                this->stream);
+        else
+            k3::restrict_<T>(R.p, fine.p, coarse.p, f.ii, f.jj, f.kk, c.ii, c.jj, c.kk, this->stream);
         launched3("restrict3");
     }
 };
 
 template <typename T>
 struct hip_interp3 : kernels3::interp_add<T> {
+    explicit hip_interp3(int j = 0) : jpn(j) {}
+    int jpn;
     void run(const prolong_op3<T> &P, const grid_func3<T> &coarse, const grid_func3<T> &residual,
              grid_func3<T> &fine) override {
         if (!P.fine_op) throw error(CDR_ERR_ARG, "interp_add: prolongation without fine operator");
         const extents3 f = fine.ext, c = coarse.ext;
-        k3::interp_add<T>(fine.p, coarse.p, residual.p, P.fine_op->p, P.fine_op->nstncl, P.p, c.ii,
-                          c.jj, c.kk, f.ii, f.jj, f.kk, this->stream);
+        if (jpn)
+            k3::interp_add_per<T>(fine.p, coarse.p, residual.p, P.fine_op->p, P.fine_op->nstncl,
+                                  P.p, c.ii, c.jj, c.kk, f.ii, f.jj, f.kk, jpn, this->stream);
+        else
+            k3::interp_add<T>(fine.p, coarse.p, residual.p, P.fine_op->p, P.fine_op->nstncl, P.p,
+                              c.ii, c.jj, c.kk, f.ii, f.jj, f.kk, this->stream);
         launched3("interp_add3");
     }
 };
 
 template <typename T>
 struct hip_solve_cg3 : kernels3::solve_cg<T> {
-    explicit hip_solve_cg3(int *info) : info_dev(info) {}
+    explicit hip_solve_cg3(int *info, int j = 0) : info_dev(info), jpn(j) {}
     int *info_dev;
+    int jpn;  // periodic: a dense factor (ldab = n), include/cedar/3d/solver.h:123-126
     void setup(const stencil_op3<double> &so, double *abd, int ldab) override {
         const extents3 e = so.ext;
         k2::fill<int>(info_dev, 0, 1, this->stream);
-        k3::setup_cg(so.p, so.nstncl, abd, ldab, e.ii, e.jj, e.kk, info_dev, this->stream);
+        if (jpn)
+            k3::setup_cg_per(so.p, so.nstncl, abd, e.ii, e.jj, e.kk, jpn, info_dev, this->stream);
+        else
+            k3::setup_cg(so.p, so.nstncl, abd, ldab, e.ii, e.jj, e.kk, info_dev, this->stream);
         launched3("setup_cg3");
         int info = 0;
         check_hip(hipMemcpyAsync(&info, info_dev, sizeof(int), hipMemcpyDeviceToHost,
@@ -243,7 +281,10 @@ struct hip_solve_cg3 : kernels3::solve_cg<T> {
     void run(grid_func3<T> &x, const grid_func3<T> &b, const double *abd, int ldab,
              double *bbd) override {
         const extents3 e = x.ext;
-        k3::solve_cg<T>(x.p, b.p, abd, ldab, bbd, e.ii, e.jj, e.kk, this->stream);
+        if (jpn)
+            k3::solve_cg_per<T>(x.p, b.p, abd, bbd, e.ii, e.jj, e.kk, jpn, this->stream);
+        else
+            k3::solve_cg<T>(x.p, b.p, abd, ldab, bbd, e.ii, e.jj, e.kk, this->stream);
         launched3("solve_cg3");
     }
 };
@@ -259,25 +300,25 @@ void add_planes(kernel_manager &km, const cdr_solver_settings &s2) {
 }  // namespace
 
 void register_hip_kernels3(kernel_manager &km, int *info_dev, const cdr_solver_settings &plane,
-                           int precision) {
+                           int precision, int jpn) {
     km.add<kernels3::residual<double>, hip_residual3<double>>("hip");
     km.add<kernels3::residual<float>, hip_residual3<float>>("hip");
-    km.add<kernels3::point_relax<double>, hip_rbgs3<double>>("hip");
-    km.add<kernels3::point_relax<float>, hip_rbgs3<float>>("hip");
+    km.add<kernels3::point_relax<double>, hip_rbgs3<double>>("hip", jpn);
+    km.add<kernels3::point_relax<float>, hip_rbgs3<float>>("hip", jpn);
     cdr_solver_settings s2 = plane;
     s2.precision = precision;
     s2.graph = 0;
     add_planes<plane_dir::xy>(km, s2);
     add_planes<plane_dir::xz>(km, s2);
     add_planes<plane_dir::yz>(km, s2);
-    km.add<kernels3::setup_interp, hip_setup_interp3>("hip");
-    km.add<kernels3::coarsen_op, hip_galerkin3>("hip");
-    km.add<kernels3::restriction<double>, hip_restrict3<double>>("hip");
-    km.add<kernels3::restriction<float>, hip_restrict3<float>>("hip");
-    km.add<kernels3::interp_add<double>, hip_interp3<double>>("hip");
-    km.add<kernels3::interp_add<float>, hip_interp3<float>>("hip");
-    km.add<kernels3::solve_cg<double>, hip_solve_cg3<double>>("hip", info_dev);
-    km.add<kernels3::solve_cg<float>, hip_solve_cg3<float>>("hip", info_dev);
+    km.add<kernels3::setup_interp, hip_setup_interp3>("hip", jpn);
+    km.add<kernels3::coarsen_op, hip_galerkin3>("hip", jpn);
+    km.add<kernels3::restriction<double>, hip_restrict3<double>>("hip", jpn);
+    km.add<kernels3::restriction<float>, hip_restrict3<float>>("hip", jpn);
+    km.add<kernels3::interp_add<double>, hip_interp3<double>>("hip", jpn);
+    km.add<kernels3::interp_add<float>, hip_interp3<float>>("hip", jpn);
+    km.add<kernels3::solve_cg<double>, hip_solve_cg3<double>>("hip", info_dev, jpn);
+    km.add<kernels3::solve_cg<float>, hip_solve_cg3<float>>("hip", info_dev, jpn);
 }
 
 // one precision's views of a level (device arrays owned by level3)
@@ -315,7 +356,8 @@ public:
         check_hip(hipEventCreate(&ev0_), "hipEventCreate");
         check_hip(hipEventCreate(&ev1_), "hipEventCreate");
         info_ = dbuf<int>(1, stream_);
-        register_hip_kernels3(km_, info_.data(), set_.plane, set_.precision);
+        jpn_ = k3::bc_from_mask3(set_.periodic);
+        register_hip_kernels3(km_, info_.data(), set_.plane, set_.precision, jpn_);
         build(so_dev, ns, nx, ny, nz);
     }
     ~solver3d() {
@@ -379,6 +421,8 @@ public:
             xb = xb_.data();
             k3::residual_ir(F.d.A.p, F.ns, b_dev, x_dev, nullptr, xa, F.f.b.p, e.ii, e.jj, e.kk,
                             work_.data(), sumsq_.data(), stream_);
+            // the outer iterate keeps its periodic images (residual_ir writes the interior)
+            if (jpn_) k3::images<double>(xa, e.ii, e.jj, e.kk, jpn_, 0, stream_);
         } else {
             k3::residual_norm(F.d.A.p, F.ns, b_dev, x_dev, F.d.res.p, e.ii, e.jj, e.kk,
                               work_.data(), sumsq_.data(), stream_);
@@ -393,6 +437,7 @@ public:
                 cycle<float>(F.f.x.p, F.f.b.p);
                 k3::residual_ir(F.d.A.p, F.ns, b_dev, xa, F.f.x.p, xb, F.f.b.p, e.ii, e.jj, e.kk,
                                 work_.data(), sumsq_.data(), stream_);
+                if (jpn_) k3::images<double>(xb, e.ii, e.jj, e.kk, jpn_, 0, stream_);
                 std::swap(xa, xb);
             } else {
                 cycle<double>(x_dev, b_dev);
@@ -425,6 +470,7 @@ private:
     cdr_solver3_settings set_;
     hipStream_t stream_;
     kernel_manager km_;
+    int jpn_ = 0;  // BoxMG boundary id of set_.periodic (BMG_get_bc)
     std::vector<level3> L_;
     dbuf<double> abd_, bbd_, work_, sumsq_, xa_, xb_;
     dbuf<int> info_;
@@ -460,6 +506,21 @@ private:
             if (set_.num_levels > nl) throw error(CDR_ERR_LEVELS, "too many levels specified");
             nl = set_.num_levels;
         }
+        if (jpn_) {
+            // a periodic direction of n points coarsens to n/2: every coarsened level needs even
+            // periodic extents (odd ones put two C-points side by side across the seam)
+            const int m = k3::per_mask3(jpn_);
+            int cx = nx, cy = ny, cz = nz;
+            for (int l = 0; l + 1 < nl; ++l) {
+                if (((m & 1) && cx % 2) || ((m & 2) && cy % 2) || ((m & 4) && cz % 2))
+                    throw error(CDR_ERR_ARG, "periodic extents must be even on every coarsened level");
+                cx = (int)((cx - 1) / 2.0 + 1);
+                cy = (int)((cy - 1) / 2.0 + 1);
+                cz = (int)((cz - 1) / 2.0 + 1);
+            }
+            if ((long long)cx * cy * cz > 2048)
+                throw error(CDR_ERR_ARG, "periodic coarsest grid above 2048 points (dense factor)");
+        }
         // setup_space (include/cedar/3d/solver.h:80-128)
         L_.resize(nl);
         int cx = nx, cy = ny, cz = nz;
@@ -490,8 +551,8 @@ private:
         for (int l = 1; l < nl; ++l) L_[l].d.P.fine_op = &L_[l - 1].d.A;
         const ext3 ec = L_[nl - 1].e;
         const int kd = k3::cg_bandwidth(ec.ii, ec.jj);
-        ldab_ = kd + 1;
         const std::size_t n = (std::size_t)(ec.ii - 2) * (ec.jj - 2) * (ec.kk - 2);
+        ldab_ = jpn_ ? (int)n : kd + 1;  // periodic: dense n x n (include/cedar/3d/solver.h:124)
         abd_ = dbuf<double>((std::size_t)ldab_ * n, stream_);
         bbd_ = dbuf<double>(n, stream_);
         work_ = dbuf<double>(k3::norm_partials(L_[0].e.ii, L_[0].e.jj, L_[0].e.kk), stream_);
@@ -620,7 +681,13 @@ private:
         level3 &C = L_[l + 1];
         lvl3v<T> &fv = F.v<T>();
         lvl3v<T> &cv = C.v<T>();
-        km_.run<kernels3::restriction<T>>(cv.P, b, cv.b);
+        if (jpn_) {  // the periodic restriction refreshes its input's images: restrict a copy
+            check_hip(hipMemcpyAsync(fv.res.p, b.p, F.e.points() * sizeof(T),
+                                     hipMemcpyDeviceToDevice, stream_), "copy b");
+            km_.run<kernels3::restriction<T>>(cv.P, fv.res, cv.b);
+        } else {
+            km_.run<kernels3::restriction<T>>(cv.P, b, cv.b);
+        }
         fmg<T>(l + 1, cv.x, cv.b);
         k2::fill<T>(x.p, T(0), F.e.points(), stream_);
         k2::fill<T>(fv.res.p, T(0), F.e.points(), stream_);
@@ -687,6 +754,7 @@ void cdr_hip_default_settings3(cdr_solver3_settings *s) {
     s->cycle = CDR_CYCLE_V;
     s->precision = CDR_PREC_F64;
     s->stall_cycles = 0;
+    s->periodic = 0;
     // plane-config defaults (src/kernel_params.cc:83-89): line-xy, max-iter 1
     cdr_hip_default_settings(&s->plane);
     s->plane.relaxation = CDR_RELAX_LINE_XY;
@@ -703,7 +771,8 @@ int cdr_hip_solver3d_create(const double *so_dev, int nstncl, int nx, int ny, in
         s->plane.maxiter < 0 || s->plane.nrelax_pre < 0 || s->plane.nrelax_post < 0 ||
         s->plane.min_coarse < 2 || (s->cycle != CDR_CYCLE_V && s->cycle != CDR_CYCLE_F) ||
         (s->plane.fused_coarse != 0 && s->plane.fused_coarse != 1) ||
-        (s->precision != CDR_PREC_F64 && s->precision != CDR_PREC_MIXED) || s->stall_cycles < 0)
+        (s->precision != CDR_PREC_F64 && s->precision != CDR_PREC_MIXED) || s->stall_cycles < 0 ||
+        s->periodic < 0 || s->periodic > 7 || (s->periodic != 0 && s->relaxation != CDR_RELAX_POINT))
         return CDR_ERR_ARG;
     *out = nullptr;
     int ndev = 0;

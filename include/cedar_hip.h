@@ -458,6 +458,61 @@ int cdr_hip_solve_cg3d_f32(float *q, const float *qf, const double *abd, int lda
                            int ii, int jj, int kk, void *stream);
 
 /* ---------------------------------------------------------------------------------------
+ * 3D periodic boundaries (grid.periodic, include/cedar/kernel_params.h:35-42).  jpn is the
+ * BoxMG boundary id BMG_get_bc returns for the periodic mask (bits x = 1, y = 2, z = 4;
+ * src/2d/ftn/BMG_get_bc.f90:1-24): 2 x, 1 y, 3 xy, 5 z, 6 xz, 7 yz, 8 xyz
+ * (include/cedar/2d/ftn/BMG_parameters_c.h:193-204); the Dirichlet id 0 is rejected here (use
+ * the kernels above).  The images of a periodic direction live in its ghost layer: q(1) =
+ * q(n-1), q(n) = q(2) for the operator, q and the right-hand side.  Each entry point is the
+ * reference's JPN != 0 branch:
+ * ------------------------------------------------------------------------------------- */
+/* src/3d/ftn/BMG3_SymStd_relax_GS.f90:189-352 (include/cedar/3d/relax.h:94-98): x images of
+ * each row and y images of each plane after its colour pass, z images after the sweep */
+int cdr_hip_relax_gs3d_per_f64(const double *so, int nstncl, const double *qf, double *q,
+                               const double *sor, int ii, int jj, int kk, int updown, int jpn,
+                               void *stream);
+int cdr_hip_relax_gs3d_per_f32(const float *so, int nstncl, const float *qf, float *q,
+                               const float *sor, int ii, int jj, int kk, int updown, int jpn,
+                               void *stream);
+/* src/3d/ftn/BMG3_SymStd_restrict.f90:79-141 (src/3d/restrict.cc:27): the images of q are
+ * refreshed in place (y, x, z) before restricting (q is written) */
+int cdr_hip_restrict3d_per_f64(const double *ci, double *q, double *qc, int iif, int jjf, int kkf,
+                               int iic, int jjc, int kkc, int jpn, void *stream);
+int cdr_hip_restrict3d_per_f32(const float *ci, float *q, float *qc, int iif, int jjf, int kkf,
+                               int iic, int jjc, int kkc, int jpn, void *stream);
+/* src/3d/ftn/BMG3_SymStd_interp_add.f90:72-212 (src/3d/interp.cc:39), then every image of q
+ * (y, x, z).  The reference's own image loops (:237-280) test IPN before assigning it and
+ * index with left-over loop variables; this refresh is the well-defined one restriction uses. */
+int cdr_hip_interp_add3d_per_f64(double *q, const double *qc, const double *res, const double *so,
+                                 int nstncl, const double *ci, int iic, int jjc, int kkc, int iif,
+                                 int jjf, int kkf, int jpn, void *stream);
+int cdr_hip_interp_add3d_per_f32(float *q, const float *qc, const float *res, const float *so,
+                                 int nstncl, const float *ci, int iic, int jjc, int kkc, int iif,
+                                 int jjf, int kkf, int jpn, void *stream);
+/* src/3d/ftn/BMG3_SymStd_SETUP_interp_OI.f90:809-2784 (include/cedar/3d/interp.h:65-73),
+ * including the image weights it writes; ci must be zero on entry */
+int cdr_hip_setup_interp3d_per_f64(const double *so, int nstncl, double *ci, int iif, int jjf,
+                                   int kkf, int iic, int jjc, int kkc, int jpn, void *stream);
+/* src/3d/ftn/BMG3_SymStd_SETUP_ITLI07_ex.f90, _ITLI27_ex.f90 (include/cedar/3d/coarsen.h:
+ * 41-61) with IPN: the product P^T A P over the periodic geometry (equal to the reference's
+ * hand-expanded products to rounding), then the images of the coarse operator (y, x, z).
+ * Periodic extents must be even (n fine points coarsen to n/2). */
+int cdr_hip_setup_galerkin3d_per_f64(const double *so, int nstncl, const double *ci, double *soc,
+                                     int iif, int jjf, int kkf, int iic, int jjc, int kkc, int jpn,
+                                     void *stream);
+/* src/3d/ftn/BMG3_SymStd_SETUP_cg_LU.f90:190-612 and BMG3_SymStd_SOLVE_cg.f90:103-196
+ * (include/cedar/3d/solver.h:123-126): the dense matrix abd(n, n), n = nx*ny*nz <= 2048, of the
+ * operator with its periodic couplings, factored by an upper Cholesky (DPOTF2 order;
+ * CDR_ERR_NOT_SPD for a non-positive pivot); the solve removes the mean of the coarse solution
+ * and refreshes its images (x, y, z).  bbd holds n doubles.  Synchronises (setup). */
+int cdr_hip_setup_cg3d_per_f64(const double *so, int nstncl, double *abd, int ii, int jj, int kk,
+                               int jpn, int *info_dev, void *stream);
+int cdr_hip_solve_cg3d_per_f64(double *q, const double *qf, const double *abd, double *bbd, int ii,
+                               int jj, int kk, int jpn, void *stream);
+int cdr_hip_solve_cg3d_per_f32(float *q, const float *qf, const double *abd, double *bbd, int ii,
+                               int jj, int kk, int jpn, void *stream);
+
+/* ---------------------------------------------------------------------------------------
  * 3D solver (cdr3::solver<seven_pt|xxvii_pt>), FP64, point or plane relaxation
  * ------------------------------------------------------------------------------------- */
 typedef struct {
@@ -476,6 +531,10 @@ typedef struct {
     int precision;    /* CDR_PREC_F64 (default) or CDR_PREC_MIXED: FP32 3D cycle and FP32 plane
                          solves inside FP64 iterative refinement */
     int stall_cycles; /* as in cdr_solver_settings (default 0: no stall test) */
+    int periodic;     /* grid.periodic as Cedar's per_mask (include/cedar/kernel_params.h:35-42):
+                         bit 0 x, bit 1 y, bit 2 z; default 0 (Dirichlet).  Point relaxation
+                         only; the operator, b and x carry the periodic images in their ghosts,
+                         and the periodic extents must be even on every coarsened level. */
 } cdr_solver3_settings;
 
 typedef struct cdr_hip_solver3d cdr_hip_solver3d;

@@ -63,3 +63,12 @@ def rand_so3_periodic(rng, nx, ny, nz, ns, mask, lo=0.1, hi=1.0):
         so[d] = np.where(ok, rng.uniform(lo, hi, so[d].shape), 0.0)
     so[0, 1:-1, 1:-1, 1:-1] = 2.0 * (ns - 1) + rng.uniform(0.0, 1.0, (nz, ny, nx))
     return images3(so, mask)
+
+
+def periodic_case(n, ns, mask):
+    """(so, q, qf) of a periodic kernel case: seeded by its extents, stencil and mask."""
+    nx, ny, nz = n
+    rng = np.random.default_rng(1000 * mask + 100 * ns + nx * 7 + ny * 5 + nz * 3)
+    so = rand_so3_periodic(rng, nx, ny, nz, ns, mask)
+    shp = (nz + 2, ny + 2, nx + 2)
+    return so, rng.standard_normal(shp), rng.standard_normal(shp)
