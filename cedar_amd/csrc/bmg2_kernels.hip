@@ -70,6 +70,36 @@ __device__ inline void block_sum_store(double v, double *dst) {
 // (blockIdx.z for tiled kernels, blockIdx.y for line kernels, blockIdx.x for one-workgroup
 // kernels); the problems of a batch are stored back to back, each with the layout above.
 //
+// Problems that share an operator come in groups of rep right-hand sides (problem b uses
+// operator b / rep).  Such launches renumber their workgroups so that the rep workgroups of one
+// tile of one operator run one after another on one XCD (workgroups are dealt round-robin over
+// the 8 XCDs in launch order): the operator's tile comes from HBM once and from that XCD's L2
+// for the other right-hand sides.  t (the workgroup's tile, tiles per problem) and b (its
+// problem) are replaced by the logical pair; the map is a permutation of the launch, so every
+// tile of every problem is still done once.  Launches whose tiles per problem are not a
+// multiple of 8 keep the launch order.  Measured on 29 operators of 2049^2 with 4 right-hand
+// sides each (mixed V(1,1) line-xy solve): 688 against 720 ms.
+__device__ inline void shared_op_order(int tiles, int rep, int &t, int &b) {
+    if (rep <= 1 || tiles % 8 != 0) return;
+    const int op = b / rep;
+    const int off = (b - op * rep) * tiles + t;  // position in the operator's range, which
+    const int k = off / 8;                       // starts on an XCD-0 workgroup
+    t = (k / rep) * 8 + off % 8;
+    b = op * rep + k % rep;
+}
+
+// logical tile (x, y) and problem z of a tiled launch (problem index in blockIdx.z)
+struct wg_pos {
+    int x, y;
+    std::size_t z;
+};
+__device__ inline wg_pos tiled_wg(int rep) {
+    if (rep <= 1) return {(int)blockIdx.x, (int)blockIdx.y, (std::size_t)blockIdx.z};
+    int t = (int)(blockIdx.y * gridDim.x + blockIdx.x), b = (int)blockIdx.z;
+    shared_op_order((int)(gridDim.x * gridDim.y), rep, t, b);
+    return {t % (int)gridDim.x, t / (int)gridDim.x, (std::size_t)b};
+}
+
 // one block of 1024 threads: sum of n partials in a fixed order -> *out.  Thread t adds
 // partials t, t+1024, t+2048, ... in that order; loads are batched so they overlap.
 __global__ __launch_bounds__(1024) void finalize_sum_k(const double *__restrict__ partials,
@@ -115,9 +145,10 @@ __global__ __launch_bounds__(256) void residual_k(const T *__restrict__ so, cons
                                                   const T *__restrict__ q, T *__restrict__ res,
                                                   int ii, int jj, double *__restrict__ partials,
                                                   int rep) {
-    const int i = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
-    const int j = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
-    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.z;
+    const wg_pos g = tiled_wg(rep);
+    const int i = g.x * 64 + (threadIdx.x % 64) + 1;
+    const int j = g.y * 4 + (threadIdx.x / 64) + 1;
+    const std::size_t P = (std::size_t)ii * jj, bz = g.z;
     so += (bz / rep) * (NINE ? 5 : 3) * P;
     qf += bz * P;
     q += bz * P;
@@ -131,7 +162,7 @@ __global__ __launch_bounds__(256) void residual_k(const T *__restrict__ so, cons
         res[p] = r;
         if (NORM) sq = (double)r * (double)r;
     }
-    if (NORM) block_sum_store(sq, &partials[blockIdx.y * gridDim.x + blockIdx.x]);
+    if (NORM) block_sum_store(sq, &partials[g.y * gridDim.x + g.x]);
 }
 
 template <typename T>
@@ -167,9 +198,10 @@ __global__ __launch_bounds__(64 * IR_ROWS) void residual_ir_k(const double *__re
     __shared__ double xs[IR_ROWS + 2][66];
     __shared__ float rt[IR_ROWS][65];  // r32 of the block, for the transposed copy
     const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
-    const int i0 = blockIdx.x * 64, j0 = blockIdx.y * IR_ROWS;  // halo corner of the block
+    const wg_pos g = tiled_wg(rep);
+    const int i0 = g.x * 64, j0 = g.y * IR_ROWS;  // halo corner of the block
     const int i = i0 + tx + 1, j = j0 + ty + 1;
-    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.z;
+    const std::size_t P = (std::size_t)ii * jj, bz = g.z;
     so += (bz / rep) * (NINE ? 5 : 3) * P;
     qf += bz * P;
     x += bz * P;
@@ -211,7 +243,7 @@ __global__ __launch_bounds__(64 * IR_ROWS) void residual_ir_k(const double *__re
         rt[ty][tx] = (float)r;
         sq = r * r;
     }
-    block_sum_store<IR_ROWS>(sq, &partials[blockIdx.y * gridDim.x + blockIdx.x]);
+    block_sum_store<IR_ROWS>(sq, &partials[g.y * gridDim.x + g.x]);
     if (r32T) {  // uniform over the launch; block_sum_store ends with a workgroup barrier
         r32T += bz * P;
         __syncthreads();
@@ -246,8 +278,9 @@ __global__ __launch_bounds__(256) void gs_colour_k(const T *__restrict__ so, con
                                                    T *__restrict__ q, const T *__restrict__ sor,
                                                    int ii, int jj, int pi, int pj, int rep,
                                                    int rev = 0) {
-    const int bx = rev ? (int)(gridDim.x - 1 - blockIdx.x) : (int)blockIdx.x;
-    const int by = rev ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.y;
+    const wg_pos g = tiled_wg(rep);
+    const int bx = rev ? (int)gridDim.x - 1 - g.x : g.x;
+    const int by = rev ? (int)gridDim.y - 1 - g.y : g.y;
     const int k = bx * 64 + (threadIdx.x % 64);
     const int m = by * 4 + (threadIdx.x / 64);
     int i, j;
@@ -259,7 +292,7 @@ __global__ __launch_bounds__(256) void gs_colour_k(const T *__restrict__ so, con
         i = 2 * k + ((pi + j) & 1);  // (i + j) % 2 == pi
     }
     if (i < 1 || i > ii - 2 || j < 1 || j > jj - 2) return;
-    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.z, bo = bz / rep;
+    const std::size_t P = (std::size_t)ii * jj, bz = g.z, bo = bz / rep;
     so += bo * (NINE ? 5 : 3) * P;
     qf += bz * P;
     q += bz * P;
@@ -494,9 +527,11 @@ __global__ __launch_bounds__(256) void relax_lines_k(const T *__restrict__ so, c
                                                      T *__restrict__ q, const T *__restrict__ sor,
                                                      int ii, int jj, int first, int nlines, int rep) {
     const int lane = threadIdx.x % WAVE;
-    const int line = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    int bx = (int)blockIdx.x, prob = (int)blockIdx.y;
+    shared_op_order((int)gridDim.x, rep, bx, prob);
+    const int line = bx * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     if (line >= nlines) return;  // whole wavefront leaves together; no block barrier below
-    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.y, bo = bz / rep;
+    const std::size_t P = (std::size_t)ii * jj, bz = prob, bo = bz / rep;
     so += bo * (NINE ? 5 : 3) * P;
     qf += bz * P;
     q += bz * P;
@@ -559,10 +594,10 @@ __global__ __launch_bounds__(256) void relax_lines_k(const T *__restrict__ so, c
 // operator, which then stay in one XCD's L2.  Launch 8 * ceil(ntiles / 8) workgroups.
 // rev: each XCD walks its range of tiles from the end, so that a launch starts on the lines the
 // previous launch touched last (still in the memory-side cache); the work is unchanged.
-__device__ inline int xcd_tile(int ntiles, int rev = 0) {
+__device__ inline int xcd_tile(int bx, int ntiles, int rev = 0) {
     const int per = (ntiles + 7) / 8;
-    const int k = (int)(blockIdx.x / 8);
-    return (int)(blockIdx.x % 8) * per + (rev ? per - 1 - k : k);
+    const int k = bx / 8;
+    return (bx % 8) * per + (rev ? per - 1 - k : k);
 }
 
 // raw operands of the right-hand side of one point of a line
@@ -781,14 +816,16 @@ __global__ __launch_bounds__(L *W * 64) void relax_xlines_seg_k(const T *__restr
                                                                int jj, int first, int nlines,
                                                                int ntiles, int rev, int rep) {
     __shared__ seg_map<T> xch[2][L][W > 1 ? W * CPW : 1];
-    const int tile = xcd_tile(ntiles, rev);
+    int bx = (int)blockIdx.x, prob = (int)blockIdx.y;
+    shared_op_order((int)gridDim.x, rep, bx, prob);  // gridDim.x is a multiple of 8
+    const int tile = xcd_tile(bx, ntiles, rev);
     if (tile >= ntiles) return;  // uniform over the workgroup
     const int wl = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int slot = wl / W, w = wl % W;
     const int line = tile * L + slot;
     const bool valid = line < nlines;  // invalid wavefronts solve a copy of the last line
     const int j = first + 2 * min(line, nlines - 1);
-    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.y, bo = bz / rep;
+    const std::size_t P = (std::size_t)ii * jj, bz = prob, bo = bz / rep;
     so += bo * (NINE ? 5 : 3) * P;
     qf += bz * P;
     q += bz * P;
@@ -875,11 +912,13 @@ __global__ __launch_bounds__(NC *W * 64) void relax_ylines_seg_k(const T *__rest
     static_assert(CPW % G == 0, "G must divide CPW");
     __shared__ T tile_s[G][W][NC][WAVE + 1];
     __shared__ seg_map<T> xch[2][NC][W > 1 ? W * CPW : 1];
-    const int tile = xcd_tile(ntiles, rev);
+    int bx = (int)blockIdx.x, prob = (int)blockIdx.y;
+    shared_op_order((int)gridDim.x, rep, bx, prob);  // gridDim.x is a multiple of 8
+    const int tile = xcd_tile(bx, ntiles, rev);
     if (tile >= ntiles) return;  // uniform over the workgroup
     const int wl = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int cs = wl / W, w = wl % W;
-    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.y, bo = bz / rep;
+    const std::size_t P = (std::size_t)ii * jj, bz = prob, bo = bz / rep;
     so += bo * (NINE ? 5 : 3) * P;
     qf += bz * P;
     q += bz * P;
@@ -986,10 +1025,11 @@ template <typename T>
 __global__ __launch_bounds__(256) void restrict_k(const T *__restrict__ ci, const T *__restrict__ q,
                                                   T *__restrict__ qc, int iif, int jjf, int iic,
                                                   int jjc, int rep) {
-    const int ic = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
-    const int jc = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
+    const wg_pos g = tiled_wg(rep);
+    const int ic = g.x * 64 + (threadIdx.x % 64) + 1;
+    const int jc = g.y * 4 + (threadIdx.x / 64) + 1;
     if (ic > iic - 2 || jc > jjc - 2) return;
-    const std::size_t Pc = (std::size_t)iic * jjc, bz = blockIdx.z;
+    const std::size_t Pc = (std::size_t)iic * jjc, bz = g.z;
     ci += (bz / rep) * 8 * Pc;
     q += bz * iif * (std::size_t)jjf;
     qc += bz * Pc;
@@ -1000,25 +1040,34 @@ __global__ __launch_bounds__(256) void restrict_k(const T *__restrict__ ci, cons
 // launch the coarse iterate xc is zeroed (ghosts included: the coarse cycle starts from 0) and,
 // where qcT is given, qc is also written transposed (qcT[ic * jjc + jc], for a coarse level whose
 // y sweeps run through the transposed grid).  Tiles of 64 (ic) x 16 (jc) coarse points over the
-// ghost-inclusive coarse grid, 256 threads.
+// ghost-inclusive coarse grid, 64 * rc_rows<T> threads.
+// FP64: one thread per coarse point of the tile, so that every point's loads are in flight at
+// once (with a loop over rows the FP64 build kept one row's loads in flight at a time).  FP32: 4
+// rows per thread (one point per thread was 3.8x slower over the levels of a batch of 116
+// problems of 2049^2, and no faster on 4095^2).
 template <typename T>
-__global__ __launch_bounds__(1024) void restrict_cycle_k(const T *__restrict__ ci,
-                                                         const T *__restrict__ q,
-                                                         T *__restrict__ qc, T *__restrict__ xc,
-                                                         T *__restrict__ qcT, int iif, int jjf,
-                                                         int iic, int jjc, int rep) {
-    // one thread per coarse point of the 64 x 16 tile: every point's loads are in flight at
-    // once (with a loop over rows the FP64 build kept one row's loads in flight at a time)
+constexpr int rc_rows = sizeof(T) == 8 ? 16 : 4;
+template <typename T>
+__global__ __launch_bounds__(64 * rc_rows<T>) void restrict_cycle_k(const T *__restrict__ ci,
+                                                                    const T *__restrict__ q,
+                                                                    T *__restrict__ qc,
+                                                                    T *__restrict__ xc,
+                                                                    T *__restrict__ qcT, int iif,
+                                                                    int jjf, int iic, int jjc,
+                                                                    int rep) {
+    constexpr int NT = 64 * rc_rows<T>;
     __shared__ T tile[16][65];
-    const int ic0 = blockIdx.x * 64, jc0 = blockIdx.y * 16;
-    const std::size_t Pc = (std::size_t)iic * jjc, bz = blockIdx.z;
+    const wg_pos g = tiled_wg(rep);
+    const int ic0 = g.x * 64, jc0 = g.y * 16;
+    const std::size_t Pc = (std::size_t)iic * jjc, bz = g.z;
     ci += (bz / rep) * 8 * Pc;
     q += bz * iif * (std::size_t)jjf;
     qc += bz * Pc;
     xc += bz * Pc;
-    const int tx = threadIdx.x % 64, k = threadIdx.x / 64;
-    const int ic = ic0 + tx, jc = jc0 + k;
-    if (ic < iic && jc < jjc) {
+    const int tx = threadIdx.x % 64, ic = ic0 + tx;
+    for (int k = threadIdx.x / 64; k < 16; k += rc_rows<T>) {
+        const int jc = jc0 + k;
+        if (ic >= iic || jc >= jjc) continue;
         const std::size_t pc = pidx(ic, jc, iic);
         xc[pc] = T(0);
         if (ic >= 1 && ic <= iic - 2 && jc >= 1 && jc <= jjc - 2) {
@@ -1030,10 +1079,12 @@ __global__ __launch_bounds__(1024) void restrict_cycle_k(const T *__restrict__ c
     if (!qcT) return;  // uniform over the launch
     qcT += bz * Pc;
     __syncthreads();
-    const int a = threadIdx.x / 16, c = threadIdx.x % 16;  // a: ic offset, c: jc offset
-    const int i2 = ic0 + a, j2 = jc0 + c;                  // (c contiguous in qcT)
-    if (i2 >= 1 && i2 <= iic - 2 && j2 >= 1 && j2 <= jjc - 2)
-        qcT[(std::size_t)i2 * jjc + j2] = tile[c][a];
+    for (int e = threadIdx.x; e < 64 * 16; e += NT) {
+        const int a = e / 16, c = e % 16;  // a: ic offset, c: jc offset (contiguous in qcT)
+        const int i2 = ic0 + a, j2 = jc0 + c;
+        if (i2 >= 1 && i2 <= iic - 2 && j2 >= 1 && j2 <= jjc - 2)
+            qcT[(std::size_t)i2 * jjc + j2] = tile[c][a];
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1153,9 +1204,10 @@ __global__ __launch_bounds__(256) void interp_add_k(T *__restrict__ q, const T *
                                                     const T *__restrict__ so,
                                                     const T *__restrict__ ci, int iic, int jjc,
                                                     int iif, int jjf, int ns, int rep) {
-    const int m = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
-    const int n = blockIdx.y * 4 + (threadIdx.x / 64) + 1;
-    const std::size_t Pc = (std::size_t)iic * jjc, Pf = (std::size_t)iif * jjf, bz = blockIdx.z;
+    const wg_pos g = tiled_wg(rep);
+    const int m = g.x * 64 + (threadIdx.x % 64) + 1;
+    const int n = g.y * 4 + (threadIdx.x / 64) + 1;
+    const std::size_t Pc = (std::size_t)iic * jjc, Pf = (std::size_t)iif * jjf, bz = g.z;
     const std::size_t bo = bz / rep;
     interp_block<T>(q + bz * Pf, qc + bz * Pc, res + bz * Pf, so + bo * ns * Pf, ci + bo * 8 * Pc,
                     iic, jjc, iif, jjf, m, n);
@@ -1660,8 +1712,9 @@ __global__ __launch_bounds__(256) void residual_qT_k(const T *__restrict__ so,
                                                      const T *__restrict__ qT,
                                                      T *__restrict__ res, int ii, int jj, int rep) {
     __shared__ T sq[kTJ + 2][kTI + 3];  // q(i0-1 .. i0+64, j0-1 .. j0+32) as [j][i]
-    const int i0 = blockIdx.x * kTI + 1, j0 = blockIdx.y * kTJ + 1;
-    const std::size_t P = (std::size_t)ii * jj, bz = blockIdx.z;
+    const wg_pos g = tiled_wg(rep);
+    const int i0 = g.x * kTI + 1, j0 = g.y * kTJ + 1;
+    const std::size_t P = (std::size_t)ii * jj, bz = g.z;
     so += (bz / rep) * (NINE ? 5 : 3) * P;
     qf += bz * P;
     qT += bz * P;
@@ -1706,9 +1759,10 @@ __global__ __launch_bounds__(512) void interp_add_qT_k(T *__restrict__ qT, const
                                                        const T *__restrict__ ci, int iic, int jjc,
                                                        int iif, int jjf, int ns, int rep) {
     __shared__ T t1[32][65], t2[32][65];  // [j - jt][i - it]
-    const int m0 = blockIdx.x * 32 + 1, n0 = blockIdx.y * 16 + 1;
+    const wg_pos g = tiled_wg(rep);
+    const int m0 = g.x * 32 + 1, n0 = g.y * 16 + 1;
     const int it = 2 * m0 - 1, jt = 2 * n0 - 1;  // first fine point of the tile
-    const std::size_t Pc = (std::size_t)iic * jjc, Pf = (std::size_t)iif * jjf, bz = blockIdx.z;
+    const std::size_t Pc = (std::size_t)iic * jjc, Pf = (std::size_t)iif * jjf, bz = g.z;
     qT += bz * Pf;
     qc += bz * Pc;
     res += bz * Pf;
@@ -2098,6 +2152,35 @@ static int launch_shape(int c, const T *so, const T *qf, T *q, const T *sor, int
             CDR_X(4, 8, 4, 2);
             CDR_X(5, 4, 8, 2);
             return 6;
+        } else if constexpr (NCH == 3) {  // classes of 2^k + 1 chunks: W * CPW >= NCH
+            CDR_X(0, 3, 1, 4);
+            CDR_X(1, 1, 3, 2);
+            return 2;
+        } else if constexpr (NCH == 5) {
+            CDR_X(0, 5, 1, 4);
+            CDR_X(1, 3, 2, 2);
+            CDR_X(2, 1, 5, 2);
+            return 3;
+        } else if constexpr (NCH == 9) {
+            CDR_X(0, 9, 1, 4);
+            CDR_X(1, 5, 2, 2);
+            CDR_X(2, 3, 3, 2);
+            CDR_X(3, 1, 9, 1);
+            return 4;
+        } else if constexpr (NCH == 17) {
+            CDR_X(0, 17, 1, 4);
+            CDR_X(1, 9, 2, 2);
+            CDR_X(2, 6, 3, 1);
+            CDR_X(3, 6, 3, 2);
+            CDR_X(4, 3, 6, 1);
+            return 5;
+        } else if constexpr (NCH == 33) {
+            CDR_X(0, 11, 3, 1);
+            CDR_X(1, 11, 3, 2);
+            CDR_X(2, 6, 6, 1);
+            CDR_X(3, 3, 11, 1);
+            CDR_X(4, 17, 2, 2);
+            return 5;
         } else {
             CDR_X(0, 16, 4, 1);
             CDR_X(1, 8, 8, 1);
@@ -2156,9 +2239,13 @@ static int launch_shape(int c, const T *so, const T *qf, T *q, const T *sor, int
 #undef CDR_Y
 }
 
-// chunk class of a line of n points: 1, 2, 4, ..., 64, or 0 beyond 4096 points
-static int chunk_class(int n) {
+// chunk class of a line of n points: 1, 2, 4, ..., 64, or 0 beyond 4096 points.  With odd set
+// (x-line shapes), lines of 3, 5, 9, 17 or 33 chunks get a class of their own: the lines of the
+// 2^k + 1 grids (129 ... 2049 points) would otherwise leave almost half of the lanes of a
+// power-of-two class idle.
+static int chunk_class(int n, bool odd) {
     const int nch = (n + WAVE - 1) / WAVE;
+    if (odd && (nch == 3 || nch == 5 || nch == 9 || nch == 17 || nch == 33)) return nch;
     int c = 1;
     while (c < nch && c < 64) c *= 2;
     return nch > 64 ? 0 : c;
@@ -2173,13 +2260,25 @@ static int launch_class(int c, const T *so, const T *qf, T *q, const T *sor, int
     case K:                                                                                       \
         return launch_shape<T, NINE, DIR, K>(c, so, qf, q, sor, ii, jj, first, nlines, s, nb, name, \
                                              rev, rep)
-    switch (chunk_class(n)) {
+#define CDR_O(K)                                                                                  \
+    case K:                                                                                       \
+        if constexpr (DIR != 1)                                                                   \
+            return launch_shape<T, NINE, DIR, K>(c, so, qf, q, sor, ii, jj, first, nlines, s, nb,  \
+                                                 name, rev, rep);                                 \
+        else                                                                                      \
+            return 0;  /* not reached: y-lines use the power-of-two classes */
+    switch (chunk_class(n, DIR != 1)) {
         CDR_C(1);
         CDR_C(2);
+        CDR_O(3);
         CDR_C(4);
+        CDR_O(5);
         CDR_C(8);
+        CDR_O(9);
         CDR_C(16);
+        CDR_O(17);
         CDR_C(32);
+        CDR_O(33);
         CDR_C(64);
     default:  // lines longer than 4096: stage the forward results in q itself
         if constexpr (DIR == 2) {
@@ -2193,6 +2292,7 @@ static int launch_class(int c, const T *so, const T *qf, T *q, const T *sor, int
         }
     }
 #undef CDR_C
+#undef CDR_O
 }
 
 namespace {
@@ -2540,7 +2640,7 @@ template <typename T>
 void restrict_cycle(const T *ci, const T *q, T *qc, T *xc, T *qcT, int iif, int jjf, int iic,
                     int jjc, hipStream_t s, int nb, int rep) {
     const dim3 g((unsigned)((iic + 63) / 64), (unsigned)((jjc + 15) / 16), (unsigned)nb);
-    restrict_cycle_k<T><<<g, 1024, 0, s>>>(ci, q, qc, xc, qcT, iif, jjf, iic, jjc, rep);
+    restrict_cycle_k<T><<<g, 64 * rc_rows<T>, 0, s>>>(ci, q, qc, xc, qcT, iif, jjf, iic, jjc, rep);
 }
 
 template <typename T>

@@ -276,9 +276,10 @@ def test_line_xy_equals_sequential_sweeps(C, nx, ny, ns, dtype):
         assert torch.equal(q1[0], q2[0]) and torch.equal(q1[:, -1], f(q)[:, -1])
 
 
-# line lengths covering every chunk class (1..64 chunks of 64) and partial last segments
-SHAPE_SIZES = [(5, 9), (100, 9), (200, 9), (500, 9), (1000, 9), (1500, 9), (2047, 9), (3000, 9),
-               (4095, 9)]
+# line lengths covering every chunk class (1..64 chunks of 64, and the x-line classes of 3, 5, 9,
+# 17 and 33 chunks of the 2^k + 1 lines) and partial last segments
+SHAPE_SIZES = [(5, 9), (100, 9), (129, 9), (200, 9), (257, 9), (500, 9), (513, 9), (1000, 9),
+               (1025, 9), (1500, 9), (2047, 9), (2049, 9), (3000, 9), (4095, 9)]
 
 
 @pytest.mark.parametrize("n,m", SHAPE_SIZES)
