@@ -52,13 +52,24 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
-template <int NWAVES = 4>
+// barrier that orders LDS only: prefetched global loads stay in flight across it
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// LDS_ONLY: the barrier orders LDS alone (global loads issued before it stay in flight)
+template <int NWAVES = 4, bool LDS_ONLY = false>
 __device__ inline void block_sum_store(double v, double *dst) {
     __shared__ double part[NWAVES];
     v = wave_sum(v);
     int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
     if (lane == 0) part[wid] = v;
-    __syncthreads();
+    if (LDS_ONLY)
+        lds_barrier();
+    else
+        __syncthreads();
     if (threadIdx.x == 0) {
         double t = part[0];
         for (int w = 1; w < NWAVES; ++w) t += part[w];
@@ -182,7 +193,14 @@ __global__ __launch_bounds__(256) void sumsq_k(const T *__restrict__ x, int ii, 
 }
 
 // mixed precision outer step: xnew = x + e, r = f - A xnew (FP64), r32 = (float) r; blocks of
-// 64 (i) x IR_ROWS (j) points, 64 * IR_ROWS threads
+// 64 (i) x IR_ROWS (j) points, 64 * IR_ROWS threads.  One workgroup per tile of one operator
+// (blockIdx.z) does the rep right-hand sides of that operator (problems z * rep .. z * rep +
+// rep - 1) one after another: the operator's couplings of a point are loaded once into
+// registers for all of them (with one workgroup per problem, each re-read the 40 bytes per point
+// of the operator).  f is loaded with the staged iterate, and the barriers order LDS only, so
+// that stores of one right-hand side do not hold up the next.  Measured on the mixed solve of 29
+// operators of 2049^2 with 4 right-hand sides each: 658 ms, against 686 ms with one workgroup
+// per problem and 702 ms with the next right-hand side's loads software-pipelined.
 constexpr int IR_ROWS = 8;
 template <bool NINE, bool HAS_E>
 __global__ __launch_bounds__(64 * IR_ROWS) void residual_ir_k(const double *__restrict__ so,
@@ -198,59 +216,75 @@ __global__ __launch_bounds__(64 * IR_ROWS) void residual_ir_k(const double *__re
     __shared__ double xs[IR_ROWS + 2][66];
     __shared__ float rt[IR_ROWS][65];  // r32 of the block, for the transposed copy
     const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
-    const wg_pos g = tiled_wg(rep);
-    const int i0 = g.x * 64, j0 = g.y * IR_ROWS;  // halo corner of the block
+    const int i0 = blockIdx.x * 64, j0 = blockIdx.y * IR_ROWS;  // halo corner of the block
     const int i = i0 + tx + 1, j = j0 + ty + 1;
-    const std::size_t P = (std::size_t)ii * jj, bz = g.z;
-    so += (bz / rep) * (NINE ? 5 : 3) * P;
-    qf += bz * P;
-    x += bz * P;
-    if (HAS_E) e += bz * P;
-    xnew += bz * P;
-    r32 += bz * P;
-    partials += bz * gridDim.x * gridDim.y;
-    // a problem of a batch that has already converged keeps its iterate (active[b] == 0)
-    const bool upd = HAS_E && (active == nullptr || active[bz] != 0);
-    for (int t = threadIdx.x; t < (IR_ROWS + 2) * 66; t += 64 * IR_ROWS) {
-        const int ti = i0 + t % 66, tj = j0 + t / 66;
-        double v = 0.0;
-        if (ti < ii && tj < jj) {
-            const std::size_t k = pidx(ti, tj, ii);
-            v = upd ? x[k] + (double)e[k] : x[k];
-        }
-        xs[t / 66][t % 66] = v;
-    }
-    __syncthreads();
-    auto X = [&](int di, int dj) { return xs[ty + 1 + dj][tx + 1 + di]; };
-    double sq = 0.0;
-    if (i <= ii - 2 && j <= jj - 2) {
-        const std::size_t p = pidx(i, j, ii);
-        double r = qf[p];
-        r = r + so[P * KW + p] * X(-1, 0);
-        r = r + so[P * KW + p + 1] * X(1, 0);
-        r = r + so[P * KS + p] * X(0, -1);
-        r = r + so[P * KS + p + ii] * X(0, 1);
+    const std::size_t P = (std::size_t)ii * jj;
+    const bool in = i <= ii - 2 && j <= jj - 2;
+    const std::size_t p = in ? pidx(i, j, ii) : 0;
+    so += (std::size_t)blockIdx.z * (NINE ? 5 : 3) * P;
+    double cw0 = 0.0, cw1 = 0.0, cs0 = 0.0, cs1 = 0.0, co = 0.0;
+    double csw0 = 0.0, cnw0 = 0.0, cnw1 = 0.0, csw1 = 0.0;
+    if (in) {
+        cw0 = so[P * KW + p];
+        cw1 = so[P * KW + p + 1];
+        cs0 = so[P * KS + p];
+        cs1 = so[P * KS + p + ii];
         if (NINE) {
-            r = r + so[P * KSW + p] * X(-1, -1);
-            r = r + so[P * KNW + p + 1] * X(1, -1);
-            r = r + so[P * KNW + p + ii] * X(-1, 1);
-            r = r + so[P * KSW + p + ii + 1] * X(1, 1);
+            csw0 = so[P * KSW + p];
+            cnw0 = so[P * KNW + p + 1];
+            cnw1 = so[P * KNW + p + ii];
+            csw1 = so[P * KSW + p + ii + 1];
         }
-        const double xc = X(0, 0);
-        r = r - so[p] * xc;
-        xnew[p] = xc;
-        r32[p] = (float)r;
-        rt[ty][tx] = (float)r;
-        sq = r * r;
+        co = so[p];
     }
-    block_sum_store<IR_ROWS>(sq, &partials[g.y * gridDim.x + g.x]);
-    if (r32T) {  // uniform over the launch; block_sum_store ends with a workgroup barrier
-        r32T += bz * P;
-        __syncthreads();
-        // consecutive threads write consecutive j of one row of the transposed grid
-        const int a = threadIdx.x / IR_ROWS, c = threadIdx.x % IR_ROWS;
-        const int i2 = i0 + a + 1, j2 = j0 + c + 1;
-        if (i2 <= ii - 2 && j2 <= jj - 2) r32T[(std::size_t)i2 * jj + j2] = rt[c][a];
+    auto X = [&](int di, int dj) { return xs[ty + 1 + dj][tx + 1 + di]; };
+    for (int rh = 0; rh < rep; ++rh) {
+        // xs, rt and the block sum's slots of the previous right-hand side are free: every
+        // thread has passed the barrier of its block_sum_store (and, with r32T, the one before
+        // its transposed copy), and thread 0 the staging barrier below
+        const std::size_t bz = (std::size_t)blockIdx.z * rep + rh, off = bz * P;
+        // a problem of a batch that has already converged keeps its iterate (active[b] == 0)
+        const bool upd = HAS_E && (active == nullptr || active[bz] != 0);
+        const double f = in ? qf[off + p] : 0.0;
+        for (int t = threadIdx.x; t < (IR_ROWS + 2) * 66; t += 64 * IR_ROWS) {
+            const int ti = i0 + t % 66, tj = j0 + t / 66;
+            double v = 0.0;
+            if (ti < ii && tj < jj) {
+                const std::size_t k = off + pidx(ti, tj, ii);
+                v = upd ? x[k] + (double)e[k] : x[k];
+            }
+            xs[t / 66][t % 66] = v;
+        }
+        lds_barrier();
+        double sq = 0.0;
+        if (in) {
+            double r = f;
+            r = r + cw0 * X(-1, 0);
+            r = r + cw1 * X(1, 0);
+            r = r + cs0 * X(0, -1);
+            r = r + cs1 * X(0, 1);
+            if (NINE) {
+                r = r + csw0 * X(-1, -1);
+                r = r + cnw0 * X(1, -1);
+                r = r + cnw1 * X(-1, 1);
+                r = r + csw1 * X(1, 1);
+            }
+            const double xc = X(0, 0);
+            r = r - co * xc;
+            xnew[off + p] = xc;
+            r32[off + p] = (float)r;
+            rt[ty][tx] = (float)r;
+            sq = r * r;
+        }
+        block_sum_store<IR_ROWS, true>(
+            sq, &partials[bz * gridDim.x * gridDim.y + blockIdx.y * gridDim.x + blockIdx.x]);
+        if (r32T) {  // uniform over the launch
+            lds_barrier();
+            // consecutive threads write consecutive j of one row of the transposed grid
+            const int a = threadIdx.x / IR_ROWS, c = threadIdx.x % IR_ROWS;
+            const int i2 = i0 + a + 1, j2 = j0 + c + 1;
+            if (i2 <= ii - 2 && j2 <= jj - 2) r32T[off + (std::size_t)i2 * jj + j2] = rt[c][a];
+        }
     }
 }
 
@@ -655,13 +689,6 @@ __device__ inline void line_load(line_raw<T> &r, const T *__restrict__ so, const
         r.sw1 = so[P * KSW + p + ii + 1];
         r.q1p = q[p + ii + 1];
     }
-}
-
-// barrier that orders LDS only: prefetched global loads stay in flight across it
-__device__ inline void lds_barrier() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1549,8 +1576,9 @@ void sumsq(const T *x, int ii, int jj, double *partials, double *out, hipStream_
 void residual_ir(const double *so, int nstncl, const double *qf, const double *x, const float *e,
                  double *xnew, float *r32, int ii, int jj, double *partials, double *sumsq_out,
                  hipStream_t s, int nb, const int *active, int rep, float *r32T) {
+    // one workgroup per tile of an operator; nb = operators * rep
     const dim3 g((unsigned)((ii - 2 + 63) / 64), (unsigned)((jj - 2 + IR_ROWS - 1) / IR_ROWS),
-                 (unsigned)nb);
+                 (unsigned)(nb / rep));
     const unsigned nt = 64 * IR_ROWS;
     if (nstncl == 5) {
         if (e)
