@@ -1471,18 +1471,42 @@ __global__ __launch_bounds__(256) void setup_cg_k(const double *__restrict__ so,
     if (threadIdx.x == 0 && bad) atomicMax(info, bad);
 }
 
+// The coarsest level's band solve: one thread runs the two triangular solves, each step of which
+// reads what the previous one wrote.  Where the factor (ld * n) and the iterate (n) fit in
+// CG_LDS doubles, the workgroup first stages both into LDS, so that those steps wait on LDS and
+// not on global-memory round trips (4x4 coarsest grid of the 4095^2 hierarchy: 25 us per solve
+// through memory); otherwise the iterate lives in the global scratch x.  Same operations either
+// way.  tid / nt: this thread and the workgroup's thread count; sh: the LDS staging area.
+constexpr int CG_LDS = 1024;
 template <typename T>
-__global__ void solve_cg_k(T *__restrict__ q, const T *__restrict__ qf, const double *__restrict__ abd,
-                           double *__restrict__ x, int ii, int jj, int rep) {
-    if (threadIdx.x != 0) return;
+__device__ inline void coarse_band_solve(T *q, const T *qf, const double *abd, double *x, int ii,
+                                         int jj, int tid, int nt, double *sh) {
     const int nx = ii - 2, ny = jj - 2, n = nx * ny, kd = ii - 1, ld = ii;
+    const bool lds = (std::size_t)n * (ld + 1) <= (std::size_t)CG_LDS;
+    double *xx = lds ? sh : x;
+    const double *ab = abd;
+    if (lds) {
+        for (int k = tid; k < ld * n; k += nt) sh[n + k] = abd[k];
+        ab = sh + n;
+    }
+    for (int kk = tid; kk < n; kk += nt) xx[kk] = (double)qf[pidx(kk % nx + 1, kk / nx + 1, ii)];
+    __syncthreads();
+    if (tid == 0) band_solve_upper(ab, n, kd, ld, xx);
+    __syncthreads();
+    for (int kk = tid; kk < n; kk += nt) q[pidx(kk % nx + 1, kk / nx + 1, ii)] = (T)xx[kk];
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void solve_cg_k(T *__restrict__ q, const T *__restrict__ qf,
+                                                 const double *__restrict__ abd,
+                                                 double *__restrict__ x, int ii, int jj, int rep) {
+    __shared__ double sh[CG_LDS];
+    const int n = (ii - 2) * (jj - 2);
     q += (std::size_t)blockIdx.x * ii * jj;  // one workgroup per problem of a batch
     qf += (std::size_t)blockIdx.x * ii * jj;
-    abd += (std::size_t)(blockIdx.x / rep) * ld * n;
+    abd += (std::size_t)(blockIdx.x / rep) * ii * n;
     x += (std::size_t)blockIdx.x * n;
-    for (int kk = 0; kk < n; ++kk) x[kk] = (double)qf[pidx(kk % nx + 1, kk / nx + 1, ii)];
-    band_solve_upper(abd, n, kd, ld, x);
-    for (int kk = 0; kk < n; ++kk) q[pidx(kk % nx + 1, kk / nx + 1, ii)] = (T)x[kk];
+    coarse_band_solve<T>(q, qf, abd, x, ii, jj, threadIdx.x, blockDim.x, sh);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2059,16 +2083,13 @@ __global__ __launch_bounds__(1024) void coarse_cycle_k(const coarse_level<T> *__
         for (std::size_t k = tid; k < Pc; k += NT) C.x[k] = T(0);
         __syncthreads();
     }
-    // coarsest level: band Cholesky solve (solve_cg_k)
+    // coarsest level: band Cholesky solve (as solve_cg_k)
     {
+        __shared__ double cg_sh[CG_LDS];
         const coarse_level<T> C = level(nlev - 1);
-        if (tid == 0) {
-            const int nx = C.ii - 2, n = nx * (C.jj - 2), ld = C.ii;
-            double *xs = bbd + bz * n;
-            for (int kk = 0; kk < n; ++kk) xs[kk] = (double)C.b[pidx(kk % nx + 1, kk / nx + 1, C.ii)];
-            band_solve_upper(abd + bo * (std::size_t)ld * n, n, C.ii - 1, ld, xs);
-            for (int kk = 0; kk < n; ++kk) C.x[pidx(kk % nx + 1, kk / nx + 1, C.ii)] = (T)xs[kk];
-        }
+        const int n = (C.ii - 2) * (C.jj - 2);
+        coarse_band_solve<T>(C.x, C.b, abd + bo * (std::size_t)C.ii * n, bbd + bz * n, C.ii, C.jj,
+                             tid, NT, cg_sh);
         __syncthreads();
     }
     // up: interpolate and add, smooth

@@ -893,16 +893,34 @@ __global__ __launch_bounds__(256) void setup_cg3_k(const double *__restrict__ so
     if (threadIdx.x == 0 && bad) atomicMax(info, bad);
 }
 
+// One thread runs the two triangular solves, each step of which reads what the previous one
+// wrote.  Where the factor (ld * n) and the iterate (n) fit in CG3_LDS doubles, the workgroup
+// first stages both into LDS, so that those steps wait on LDS and not on global-memory round
+// trips (4^3 coarsest grid of the 255^3 hierarchy: about 300 us per solve through memory);
+// otherwise the iterate lives in the global scratch x.  Same operations either way.
+constexpr int CG3_LDS = 6144;
 template <typename T>
-__global__ void solve_cg3_k(T *__restrict__ q, const T *__restrict__ qf, const double *__restrict__ abd,
-                            int ld, double *__restrict__ x, int ii, int jj, int kk) {
-    if (threadIdx.x != 0) return;
+__global__ __launch_bounds__(256) void solve_cg3_k(T *__restrict__ q, const T *__restrict__ qf,
+                                                   const double *__restrict__ abd, int ld,
+                                                   double *__restrict__ x, int ii, int jj, int kk) {
+    __shared__ double sh[CG3_LDS];
     const int nx = ii - 2, ny = jj - 2, n = nx * ny * (kk - 2);
     const int kd = (ii - 2) * (jj - 1) + 1;
-    for (int t = 0; t < n; ++t)
-        x[t] = (double)Q3(qf, t % nx + 2, (t / nx) % ny + 2, t / (nx * ny) + 2);
-    band_solve_upper(abd, n, kd, ld, x);
-    for (int t = 0; t < n; ++t) Q3(q, t % nx + 2, (t / nx) % ny + 2, t / (nx * ny) + 2) = (T)x[t];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const bool lds = (std::size_t)n * (ld + 1) <= (std::size_t)CG3_LDS;
+    double *xx = lds ? sh : x;
+    const double *ab = abd;
+    if (lds) {
+        for (int t = tid; t < ld * n; t += nt) sh[n + t] = abd[t];
+        ab = sh + n;
+    }
+    for (int t = tid; t < n; t += nt)
+        xx[t] = (double)Q3(qf, t % nx + 2, (t / nx) % ny + 2, t / (nx * ny) + 2);
+    __syncthreads();
+    if (tid == 0) band_solve_upper(ab, n, kd, ld, xx);
+    __syncthreads();
+    for (int t = tid; t < n; t += nt)
+        Q3(q, t % nx + 2, (t / nx) % ny + 2, t / (nx * ny) + 2) = (T)xx[t];
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1510,7 +1528,7 @@ void setup_cg(const double *so, int nstncl, double *abd, int ldab, int ii, int j
 template <typename T>
 void solve_cg(T *q, const T *qf, const double *abd, int ldab, double *bbd, int ii, int jj, int kk,
               hipStream_t s) {
-    solve_cg3_k<T><<<1, 64, 0, s>>>(q, qf, abd, ldab, bbd, ii, jj, kk);
+    solve_cg3_k<T><<<1, 256, 0, s>>>(q, qf, abd, ldab, bbd, ii, jj, kk);
 }
 
 // ---- periodic boundaries -------------------------------------------------------------------

@@ -492,7 +492,9 @@ private:
                                                            nl - lt);
         fuse_lds_ = need <= kFusedLdsMax ? need : 0;
     }
-    static constexpr std::size_t kFusedLdsMax = 150 * 1024;  // of the 160 KiB per workgroup
+    // of the 160 KiB per workgroup; the kernel's static 8 KiB staging area of the coarsest
+    // solve comes on top
+    static constexpr std::size_t kFusedLdsMax = 150 * 1024;
 
     // Milliseconds per replay of what body() launches, captured as a graph on the capture stream
     // and replayed 3 times on the solver's stream after a warm-up; -1 if capture fails.
