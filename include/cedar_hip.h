@@ -346,9 +346,10 @@ int cdr_hip_solver2d_create_batched(const double *so_dev, int nstncl, int nx, in
  * the same extents (so_dev as for the batched form), each solved for nrhs right-hand sides, so
  * nops * nrhs problems (at most 65535).  b_dev / x_dev of solve and vcycle hold the problems back
  * to back, operator-major: right-hand side r of operator k is problem k * nrhs + r.  The
- * hierarchy is built once per operator; every cycle kernel covers all problems in one launch
- * and reads each operator's stencil, interpolation and line factors once for its nrhs
- * problems.  Each problem gets bitwise the iterate and history of its own single solve.  Not
+ * hierarchy is built once per operator and stores each operator's stencil, interpolation and
+ * line factors once for its nrhs problems; every cycle kernel covers all problems in one launch
+ * (the mixed outer step reads an operator point once for all its right-hand sides).  Each
+ * problem gets bitwise the iterate and history of its own single solve.  Not
  * available with periodic boundaries (CDR_ERR_ARG). */
 int cdr_hip_solver2d_create_shared(const double *so_dev, int nstncl, int nx, int ny, int nops,
                                    int nrhs, const cdr_solver_settings *s, void *stream,
