@@ -1092,7 +1092,9 @@ __global__ __launch_bounds__(64 * rc_rows<T>) void restrict_cycle_k(const T *__r
     qc += bz * Pc;
     xc += bz * Pc;
     const int tx = threadIdx.x % 64, ic = ic0 + tx;
-    for (int k = threadIdx.x / 64; k < 16; k += rc_rows<T>) {
+#pragma unroll
+    for (int u = 0; u < 16 / rc_rows<T>; ++u) {
+        const int k = threadIdx.x / 64 + u * rc_rows<T>;
         const int jc = jc0 + k;
         if (ic >= iic || jc >= jjc) continue;
         const std::size_t pc = pidx(ic, jc, iic);
@@ -1781,7 +1783,9 @@ __global__ __launch_bounds__(256) void residual_qT_k(const T *__restrict__ so,
     const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
     const int i = i0 + tx;
     if (i > ii - 2) return;
-    for (int k = ty; k < kTJ; k += 4) {
+#pragma unroll
+    for (int u = 0; u < kTJ / 4; ++u) {
+        const int k = ty + 4 * u;
         const int j = j0 + k;
         if (j > jj - 2) break;
         const std::size_t p = pidx(i, j, ii);
@@ -1855,15 +1859,25 @@ __global__ __launch_bounds__(512) void interp_add_qT_k(T *__restrict__ qT, const
         }
     }
     __syncthreads();
-    // phase 2: thread e -> (i offset e / 32, j offset e % 32): j-contiguous runs of 32 in qT
-    for (int e = threadIdx.x; e < 64 * 32; e += 512) {
-        const int li = e / 32, lj = e % 32;
+    // phase 2: thread e -> (i offset e / 32, j offset e % 32): j-contiguous runs of 32 in qT.
+    // The thread's 4 points are distinct, so all 4 loads are issued before the first store
+    constexpr int NP = 64 * 32 / 512;
+    T v[NP];
+    bool ok[NP];
+#pragma unroll
+    for (int u = 0; u < NP; ++u) {
+        const int e = threadIdx.x + u * 512, li = e / 32, lj = e % 32;
         const int i = it + li, j = jt + lj;
-        if (i > imax || j > jmax) continue;
-        T *p = qT + (std::size_t)i * jjf + j;
-        T v = *p + t1[lj][li];
-        if (!((li % 2 == 0) && (lj % 2 == 0))) v = v + t2[lj][li];
-        *p = v;
+        ok[u] = i <= imax && j <= jmax;
+        v[u] = ok[u] ? qT[(std::size_t)i * jjf + j] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < NP; ++u) {
+        const int e = threadIdx.x + u * 512, li = e / 32, lj = e % 32;
+        if (!ok[u]) continue;
+        T w = v[u] + t1[lj][li];
+        if (!((li % 2 == 0) && (lj % 2 == 0))) w = w + t2[lj][li];
+        qT[(std::size_t)(it + li) * jjf + (jt + lj)] = w;
     }
 }
 
