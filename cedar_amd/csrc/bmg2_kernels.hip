@@ -1805,6 +1805,68 @@ __global__ __launch_bounds__(256) void residual_qT_k(const T *__restrict__ so,
     }
 }
 
+// residual_qT_k for the nr (2..rq_maxr) right-hand sides of one operator in one workgroup
+// (blockIdx.z: the operator): the iterate tiles of all of them are staged (36 KB of LDS), then
+// each point's couplings are loaded once and applied to every right-hand side, in
+// residual_qT_k's order.
+template <typename T>
+constexpr int rq_maxr = sizeof(T) == 8 ? 2 : 4;
+template <typename T, bool NINE>
+__global__ __launch_bounds__(256) void residual_qT_rhs_k(const T *__restrict__ so,
+                                                         const T *__restrict__ qf,
+                                                         const T *__restrict__ qT,
+                                                         T *__restrict__ res, int ii, int jj,
+                                                         int nr) {
+    __shared__ T sq[rq_maxr<T>][kTJ + 2][kTI + 3];  // per right-hand side, as in residual_qT_k
+    const int i0 = blockIdx.x * kTI + 1, j0 = blockIdx.y * kTJ + 1;
+    const std::size_t P = (std::size_t)ii * jj, b0 = (std::size_t)blockIdx.z * nr;
+    so += (std::size_t)blockIdx.z * (NINE ? 5 : 3) * P;
+    for (int rh = 0; rh < nr; ++rh) {
+        const T *q = qT + (b0 + rh) * P;
+        for (int e = threadIdx.x; e < (kTI + 2) * (kTJ + 2); e += 256) {
+            const int a = e / (kTJ + 2), b = e % (kTJ + 2);
+            const int i = i0 - 1 + a, j = j0 - 1 + b;
+            sq[rh][b][a] = (i < ii && j < jj) ? q[(std::size_t)i * jj + j] : T(0);
+        }
+    }
+    __syncthreads();
+    const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+    const int i = i0 + tx;
+    if (i > ii - 2) return;
+    for (int u = 0; u < kTJ / 4; ++u) {
+        const int k = ty + 4 * u;
+        const int j = j0 + k;
+        if (j > jj - 2) break;
+        const std::size_t p = pidx(i, j, ii);
+        const T cw0 = so[P * KW + p], cw1 = so[P * KW + p + 1];
+        const T cs0 = so[P * KS + p], cs1 = so[P * KS + p + ii];
+        T csw0 = T(0), cnw0 = T(0), cnw1 = T(0), csw1 = T(0);
+        if (NINE) {
+            csw0 = so[P * KSW + p];
+            cnw0 = so[P * KNW + p + 1];
+            cnw1 = so[P * KNW + p + ii];
+            csw1 = so[P * KSW + p + ii + 1];
+        }
+        const T co = so[p];
+        for (int rh = 0; rh < nr; ++rh) {
+            auto Q = [&](int di, int dj) { return sq[rh][k + 1 + dj][tx + 1 + di]; };
+            const std::size_t pb = (b0 + rh) * P + p;
+            T acc = qf[pb];
+            acc = acc + cw0 * Q(-1, 0);
+            acc = acc + cw1 * Q(1, 0);
+            acc = acc + cs0 * Q(0, -1);
+            acc = acc + cs1 * Q(0, 1);
+            if (NINE) {
+                acc = acc + csw0 * Q(-1, -1);
+                acc = acc + cnw0 * Q(1, -1);
+                acc = acc + cnw1 * Q(-1, 1);
+                acc = acc + csw1 * Q(1, 1);
+            }
+            res[pb] = acc - co * Q(0, 0);
+        }
+    }
+}
+
 // interp_add with the fine iterate transposed: phase 1 forms, per fine point of a 64 x 32 tile
 // (32 x 16 blocks of 2 x 2 points, one thread each), the one (coarse point) or two (others) terms
 // that interp_add_k adds, in its order; phase 2 adds them to qT with j-contiguous accesses.
@@ -1814,70 +1876,105 @@ __global__ __launch_bounds__(512) void interp_add_qT_k(T *__restrict__ qT, const
                                                        const T *__restrict__ so,
                                                        const T *__restrict__ ci, int iic, int jjc,
                                                        int iif, int jjf, int ns, int rep) {
+    // one workgroup per tile of one operator (blockIdx.z) does the rep right-hand sides of that
+    // operator (problems z * rep ..) one after another, with the block's interpolation weights
+    // and operator diagonal loaded once into registers
     __shared__ T t1[32][65], t2[32][65];  // [j - jt][i - it]
-    const wg_pos g = tiled_wg(rep);
-    const int m0 = g.x * 32 + 1, n0 = g.y * 16 + 1;
+    const int m0 = blockIdx.x * 32 + 1, n0 = blockIdx.y * 16 + 1;
     const int it = 2 * m0 - 1, jt = 2 * n0 - 1;  // first fine point of the tile
-    const std::size_t Pc = (std::size_t)iic * jjc, Pf = (std::size_t)iif * jjf, bz = g.z;
-    qT += bz * Pf;
-    qc += bz * Pc;
-    res += bz * Pf;
-    so += (bz / rep) * ns * Pf;
-    ci += (bz / rep) * 8 * Pc;
+    const std::size_t Pc = (std::size_t)iic * jjc, Pf = (std::size_t)iif * jjf;
+    const std::size_t b0 = (std::size_t)blockIdx.z * rep;
+    so += (std::size_t)blockIdx.z * ns * Pf;
+    ci += (std::size_t)blockIdx.z * 8 * Pc;
     const int imax = iif - 2 + ((iif - 2) % 2 == 0 ? 1 : 0);
     const int jmax = jjf - 2 + ((jjf - 2) % 2 == 0 ? 1 : 0);
-    {
-        const int m = m0 + threadIdx.x % 32, n = n0 + threadIdx.x / 32;
-        const int i0 = 2 * m - 1, j0 = 2 * n - 1;
-        if (i0 <= imax && j0 <= jmax) {
-            const bool xe = i0 + 1 <= imax, ye = j0 + 1 <= jmax;
-            auto CI = [&](int a, int b, int k) { return ci[Pc * k + pidx(a, b, iic)]; };
-            auto QC = [&](int a, int b) { return qc[pidx(a, b, iic)]; };
-            auto R = [&](int i, int j) {
-                const std::size_t p = pidx(i, j, iif);
-                return (i > iif - 2 || j > jjf - 2) ? res[p] : res[p] / so[p];
+    const int m = m0 + threadIdx.x % 32, n = n0 + threadIdx.x / 32;
+    const int i0 = 2 * m - 1, j0 = 2 * n - 1;
+    const bool act = i0 <= imax && j0 <= jmax;
+    const bool xe = act && i0 + 1 <= imax, ye = act && j0 + 1 <= jmax;
+    const int li = i0 - it, lj = j0 - jt;
+    auto CI = [&](int a, int b, int k) { return ci[Pc * k + pidx(a, b, iic)]; };
+    // res / c at the block's F-points (the raw res beyond the interior, as interp_add_k)
+    auto fpt = [&](int i, int j, T &c) {
+        const bool raw = i > iif - 2 || j > jjf - 2;
+        c = raw ? T(1) : so[pidx(i, j, iif)];
+        return raw;
+    };
+    T cLR = T(0), cLL = T(0), cLA = T(0), cLB = T(0);
+    T cSW = T(0), cNW = T(0), cNE = T(0), cSE = T(0);
+    T c10 = T(1), c01 = T(1), c11 = T(1);
+    bool r10 = true, r01 = true, r11 = true;
+    if (xe) {
+        cLR = CI(m + 1, n, LR);
+        cLL = CI(m + 1, n, LL);
+        r10 = fpt(i0 + 1, j0, c10);
+    }
+    if (ye) {
+        cLA = CI(m, n + 1, LA);
+        cLB = CI(m, n + 1, LB);
+        r01 = fpt(i0, j0 + 1, c01);
+    }
+    if (xe && ye) {
+        cSW = CI(m + 1, n + 1, LSW);
+        cNW = CI(m + 1, n + 1, LNW);
+        cNE = CI(m + 1, n + 1, LNE);
+        cSE = CI(m + 1, n + 1, LSE);
+        r11 = fpt(i0 + 1, j0 + 1, c11);
+    }
+    for (int rh = 0; rh < rep; ++rh) {
+        const std::size_t bz = b0 + rh;
+        const T *qcb = qc + bz * Pc;
+        const T *rsb = res + bz * Pf;
+        T *qTb = qT + bz * Pf;
+        if (rh > 0) __syncthreads();  // phase 2 of the previous right-hand side is done
+        // phase 1: the one (coarse point) or two (others) terms interp_add_k adds, in its order
+        if (act) {
+            auto QC = [&](int a, int b) { return qcb[pidx(a, b, iic)]; };
+            auto R = [&](int i, int j, bool raw, T c) {
+                const T v = rsb[pidx(i, j, iif)];
+                return raw ? v : v / c;
             };
-            const int li = i0 - it, lj = j0 - jt;
             const T q00 = QC(m, n);
             t1[lj][li] = q00;
             if (xe) {
-                t1[lj][li + 1] = CI(m + 1, n, LR) * QC(m + 1, n) + CI(m + 1, n, LL) * q00;
-                t2[lj][li + 1] = R(i0 + 1, j0);
+                t1[lj][li + 1] = cLR * QC(m + 1, n) + cLL * q00;
+                t2[lj][li + 1] = R(i0 + 1, j0, r10, c10);
             }
             if (ye) {
-                t1[lj + 1][li] = CI(m, n + 1, LA) * QC(m, n + 1) + CI(m, n + 1, LB) * q00;
-                t2[lj + 1][li] = R(i0, j0 + 1);
+                t1[lj + 1][li] = cLA * QC(m, n + 1) + cLB * q00;
+                t2[lj + 1][li] = R(i0, j0 + 1, r01, c01);
             }
             if (xe && ye) {
-                T a = CI(m + 1, n + 1, LSW) * q00;
-                a = a + CI(m + 1, n + 1, LNW) * QC(m, n + 1);
-                a = a + CI(m + 1, n + 1, LNE) * QC(m + 1, n + 1);
-                a = a + CI(m + 1, n + 1, LSE) * QC(m + 1, n);
+                T a = cSW * q00;
+                a = a + cNW * QC(m, n + 1);
+                a = a + cNE * QC(m + 1, n + 1);
+                a = a + cSE * QC(m + 1, n);
                 t1[lj + 1][li + 1] = a;
-                t2[lj + 1][li + 1] = R(i0 + 1, j0 + 1);
+                t2[lj + 1][li + 1] = R(i0 + 1, j0 + 1, r11, c11);
             }
         }
-    }
-    __syncthreads();
-    // phase 2: thread e -> (i offset e / 32, j offset e % 32): j-contiguous runs of 32 in qT.
-    // The thread's 4 points are distinct, so all 4 loads are issued before the first store
-    constexpr int NP = 64 * 32 / 512;
-    T v[NP];
-    bool ok[NP];
+        __syncthreads();
+        // phase 2: thread e -> (i offset e / 32, j offset e % 32): j-contiguous runs of 32 in
+        // qT.  The thread's 4 points are distinct, so all 4 loads are issued before the first
+        // store
+        constexpr int NP = 64 * 32 / 512;
+        T v[NP];
+        bool ok[NP];
 #pragma unroll
-    for (int u = 0; u < NP; ++u) {
-        const int e = threadIdx.x + u * 512, li = e / 32, lj = e % 32;
-        const int i = it + li, j = jt + lj;
-        ok[u] = i <= imax && j <= jmax;
-        v[u] = ok[u] ? qT[(std::size_t)i * jjf + j] : T(0);
-    }
+        for (int u = 0; u < NP; ++u) {
+            const int e = threadIdx.x + u * 512, pi = e / 32, pj = e % 32;
+            const int i = it + pi, j = jt + pj;
+            ok[u] = i <= imax && j <= jmax;
+            v[u] = ok[u] ? qTb[(std::size_t)i * jjf + j] : T(0);
+        }
 #pragma unroll
-    for (int u = 0; u < NP; ++u) {
-        const int e = threadIdx.x + u * 512, li = e / 32, lj = e % 32;
-        if (!ok[u]) continue;
-        T w = v[u] + t1[lj][li];
-        if (!((li % 2 == 0) && (lj % 2 == 0))) w = w + t2[lj][li];
-        qT[(std::size_t)(it + li) * jjf + (jt + lj)] = w;
+        for (int u = 0; u < NP; ++u) {
+            const int e = threadIdx.x + u * 512, pi = e / 32, pj = e % 32;
+            if (!ok[u]) continue;
+            T w = v[u] + t1[pj][pi];
+            if (!((pi % 2 == 0) && (pj % 2 == 0))) w = w + t2[pj][pi];
+            qTb[(std::size_t)(it + pi) * jjf + (jt + pj)] = w;
+        }
     }
 }
 
@@ -2717,6 +2814,15 @@ void interp_add(T *q, const T *qc, const T *res, const T *so, int nstncl, const 
 template <typename T>
 void residual_qT(const T *so, int nstncl, const T *qf, const T *qT, T *res, int ii, int jj,
                  hipStream_t s, int nb, int rep) {
+    if (rep >= 2 && rep <= rq_maxr<T>) {  // one workgroup per operator tile, all its rhs
+        const dim3 g((unsigned)((ii - 2 + kTI - 1) / kTI), (unsigned)((jj - 2 + kTJ - 1) / kTJ),
+                     (unsigned)(nb / rep));
+        if (nstncl == 5)
+            residual_qT_rhs_k<T, true><<<g, 256, 0, s>>>(so, qf, qT, res, ii, jj, rep);
+        else
+            residual_qT_rhs_k<T, false><<<g, 256, 0, s>>>(so, qf, qT, res, ii, jj, rep);
+        return;
+    }
     const dim3 g((unsigned)((ii - 2 + kTI - 1) / kTI), (unsigned)((jj - 2 + kTJ - 1) / kTJ),
                  (unsigned)nb);
     if (nstncl == 5)
@@ -2729,7 +2835,9 @@ template <typename T>
 void interp_add_qT(T *qT, const T *qc, const T *res, const T *so, int nstncl, const T *ci,
                    int iic, int jjc, int iif, int jjf, hipStream_t s, int nb, int rep) {
     // tiles of 32 x 16 blocks of 2 x 2 fine points over the blocks m <= iif/2, n <= jjf/2
-    const dim3 g((unsigned)((iif / 2 + 31) / 32), (unsigned)((jjf / 2 + 15) / 16), (unsigned)nb);
+    // (one workgroup per tile of an operator; nb = operators * rep)
+    const dim3 g((unsigned)((iif / 2 + 31) / 32), (unsigned)((jjf / 2 + 15) / 16),
+                 (unsigned)(nb / rep));
     interp_add_qT_k<T><<<g, 512, 0, s>>>(qT, qc, res, so, ci, iic, jjc, iif, jjf, nstncl, rep);
 }
 
