@@ -2050,12 +2050,16 @@ __device__ inline void wave_line(const T *__restrict__ so, const T *__restrict__
     }
 }
 
-// LDS bytes the fused cycle needs to hold every array of its levels on chip (per level: the
+// LDS bytes the fused cycle needs to hold the arrays of one level on chip (per level: the
 // operator, both relaxation factors, x, b, res and, below the first level, the interpolation),
-// each array rounded up to 16 bytes
+// each array rounded up to 16 bytes.  first_mode (the first level of lds_mode 2 or 3): x, b
+// and res (2) or x alone (3).
 template <typename T>
-__host__ __device__ inline std::size_t fused_level_bytes(int ii, int jj, bool has_ci) {
+__host__ __device__ inline std::size_t fused_level_bytes(int ii, int jj, bool has_ci,
+                                                        int first_mode = 1) {
     const std::size_t plane = ((std::size_t)ii * jj * sizeof(T) + 15) / 16 * 16;
+    if (first_mode == 2) return 3 * plane;
+    if (first_mode == 3) return plane;
     return plane * (5 + 2 + 2 + 3 + (has_ci ? 8 : 0));
 }
 
@@ -2064,7 +2068,7 @@ __global__ __launch_bounds__(1024) void coarse_cycle_k(const coarse_level<T> *__
                                                        int nlev, int relax, int npre, int npost,
                                                        const double *__restrict__ abd,
                                                        double *__restrict__ bbd, int rep,
-                                                       int on_chip) {
+                                                       int lds_mode) {
     constexpr int NT = 1024, NWAVE = NT / WAVE;
     extern __shared__ __align__(16) unsigned char lds[];
     const int tid = threadIdx.x, wave = tid / WAVE;
@@ -2081,15 +2085,30 @@ __global__ __launch_bounds__(1024) void coarse_cycle_k(const coarse_level<T> *__
         if (d.ci) d.ci += bo * 8 * P;
         return d;
     };
-    // on_chip: every array of every level lives in LDS for the whole cycle (the launcher checked
-    // that they fit); the first level's b comes in from memory and its x goes back at the end,
-    // the only values the caller exchanges with the fused levels
+    // lds_mode 1: every array of every level lives in LDS for the whole cycle (the launcher
+    // checked that they fit); the first level's b comes in from memory and its x goes back at the
+    // end, the only values the caller exchanges with the fused levels.  For a first level too
+    // large to hold whole, lds_mode 2 keeps only its x, b and res in LDS and lds_mode 3 its x
+    // alone; its other arrays stay in memory, where the sweeps only read them, so that a sweep no
+    // longer waits on memory for the previous sweep's writes.
+    const bool on_chip = lds_mode != 0;
+    const int m0 = lds_mode >= 2 ? lds_mode : 1;  // fused_level_bytes mode of the first level
     auto level_lds = [&](int l) {
         std::size_t off = 0;
-        for (int m = 0; m < l; ++m) off += fused_level_bytes<T>(lv[m].ii, lv[m].jj, m > 0);
+        for (int m = 0; m < l; ++m)
+            off += fused_level_bytes<T>(lv[m].ii, lv[m].jj, m > 0, m == 0 ? m0 : 1);
         coarse_level<T> d = lv[l];
         const std::size_t plane = ((std::size_t)d.ii * d.jj * sizeof(T) + 15) / 16 * 16;
         unsigned char *base = lds + off;
+        if (l == 0 && m0 >= 2) {
+            d = level_global(0);
+            d.x = reinterpret_cast<T *>(base);
+            if (m0 == 2) {
+                d.b = reinterpret_cast<T *>(base + plane);
+                d.res = reinterpret_cast<T *>(base + 2 * plane);
+            }
+            return d;
+        }
         d.so = reinterpret_cast<T *>(base);
         d.sor0 = reinterpret_cast<T *>(base + 5 * plane);
         d.sor1 = reinterpret_cast<T *>(base + 7 * plane);
@@ -2104,13 +2123,20 @@ __global__ __launch_bounds__(1024) void coarse_cycle_k(const coarse_level<T> *__
         for (int l = 0; l < nlev; ++l) {
             const coarse_level<T> g = level_global(l), d = level_lds(l);
             const std::size_t P = (std::size_t)g.ii * g.jj;
-            for (std::size_t k = tid; k < 5 * P; k += NT) const_cast<T *>(d.so)[k] = g.so[k];
-            for (std::size_t k = tid; k < 2 * P; k += NT) {
-                const_cast<T *>(d.sor0)[k] = g.sor0[k];
-                const_cast<T *>(d.sor1)[k] = g.sor1[k];
+            if (l == 0 && m0 == 3) {  // the first level's x alone
+                for (std::size_t k = tid; k < P; k += NT) d.x[k] = g.x[k];
+                continue;
             }
-            if (l > 0)
-                for (std::size_t k = tid; k < 8 * P; k += NT) const_cast<T *>(d.ci)[k] = g.ci[k];
+            if (!(l == 0 && m0 == 2)) {
+                for (std::size_t k = tid; k < 5 * P; k += NT) const_cast<T *>(d.so)[k] = g.so[k];
+                for (std::size_t k = tid; k < 2 * P; k += NT) {
+                    const_cast<T *>(d.sor0)[k] = g.sor0[k];
+                    const_cast<T *>(d.sor1)[k] = g.sor1[k];
+                }
+                if (l > 0)
+                    for (std::size_t k = tid; k < 8 * P; k += NT)
+                        const_cast<T *>(d.ci)[k] = g.ci[k];
+            }
             // res and, below the first level, b start at zero as in memory: their ghosts are
             // never written, and interp_add adds the ghost res of an even-sized level to the
             // ghost of x
@@ -2877,31 +2903,34 @@ void solve_cg(T *q, const T *qf, const double *abd, double *bbd, int ii, int jj,
     solve_cg_k<T><<<nb, 64, 0, s>>>(q, qf, abd, bbd, ii, jj, rep);
 }
 
-std::size_t fused_cycle_lds_bytes(int wbytes, const int *ii, const int *jj, int nlev) {
+std::size_t fused_cycle_lds_bytes(int wbytes, const int *ii, const int *jj, int nlev,
+                                  int lds_mode) {
     std::size_t b = 0;
-    for (int l = 0; l < nlev; ++l)
-        b += wbytes == 8 ? fused_level_bytes<double>(ii[l], jj[l], l > 0)
-                         : fused_level_bytes<float>(ii[l], jj[l], l > 0);
+    for (int l = 0; l < nlev; ++l) {
+        const int m = l == 0 ? lds_mode : 1;
+        b += wbytes == 8 ? fused_level_bytes<double>(ii[l], jj[l], l > 0, m)
+                         : fused_level_bytes<float>(ii[l], jj[l], l > 0, m);
+    }
     return b;
 }
 
 template <typename T>
 void coarse_cycle(const coarse_level<T> *lv, int nlev, int max_side, int relax, int npre,
                   int npost, const double *abd, double *bbd, hipStream_t s, int nb, int rep,
-                  std::size_t lds_bytes) {
-    const int on_chip = lds_bytes > 0 ? 1 : 0;
+                  std::size_t lds_bytes, int lds_mode) {
+    const int mode = lds_bytes > 0 ? lds_mode : 0;
     if (max_side <= 64) {
-        if (on_chip)
+        if (mode)
             (void)hipFuncSetAttribute(reinterpret_cast<const void *>(coarse_cycle_k<T, 1>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
         coarse_cycle_k<T, 1><<<nb, 1024, lds_bytes, s>>>(lv, nlev, relax, npre, npost, abd, bbd,
-                                                         rep, on_chip);
+                                                         rep, mode);
     } else {
-        if (on_chip)
+        if (mode)
             (void)hipFuncSetAttribute(reinterpret_cast<const void *>(coarse_cycle_k<T, 2>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
         coarse_cycle_k<T, 2><<<nb, 1024, lds_bytes, s>>>(lv, nlev, relax, npre, npost, abd, bbd,
-                                                         rep, on_chip);
+                                                         rep, mode);
     }
 }
 
@@ -3739,7 +3768,7 @@ void copy(const T *src, T *dst, std::size_t n, hipStream_t s) {
                                hipStream_t, int, int);                                            \
     template void transpose<T>(const T *, T *, int, int, hipStream_t, int);                       \
     template void coarse_cycle<T>(const coarse_level<T> *, int, int, int, int, int, const double *, \
-                                  double *, hipStream_t, int, int, std::size_t);                  \
+                                  double *, hipStream_t, int, int, std::size_t, int);             \
     template void transpose_op<T>(const T *, T *, int, int, int, hipStream_t, int);               \
     template bool relax_lines_y_auto<T>(const T *, const T *, int, const T *, T *, const T *, T *, \
                                         T *, int, int, int, hipStream_t, int, bool, bool, bool,   \

@@ -117,15 +117,18 @@ struct coarse_level {
 // the first level from its b (x zero on entry).  max_side: the largest nx or ny of the levels
 // (at most 128).  relax: CDR_RELAX_*; npre / npost: smoothing steps.  Bitwise equal to the
 // unfused V-cycle over the same levels.
-// lds_bytes > 0 (at least fused_cycle_lds_bytes of the levels): every array of the fused levels
-// is staged in LDS for the whole cycle (only the first level's b and x cross memory).
+// lds_bytes > 0 (at least fused_cycle_lds_bytes of the levels for lds_mode): lds_mode 1, every
+// array of the fused levels is staged in LDS for the whole cycle (only the first level's b and x
+// cross memory); lds_mode 2 or 3, the same except that of the first level only x, b and res (2)
+// or x alone (3) are in LDS, its other arrays stay in memory.
 template <typename T>
 void coarse_cycle(const coarse_level<T> *lv, int nlev, int max_side, int relax, int npre,
                   int npost, const double *abd, double *bbd, hipStream_t s, int nb = 1,
-                  int rep = 1, std::size_t lds_bytes = 0);
-// LDS bytes that on-chip fused levels of these extents (first level first) need at word size
-// wbytes
-std::size_t fused_cycle_lds_bytes(int wbytes, const int *ii, const int *jj, int nlev);
+                  int rep = 1, std::size_t lds_bytes = 0, int lds_mode = 1);
+// LDS bytes that fused levels of these extents (first level first) need at word size wbytes
+// with lds_mode 1, 2 or 3
+std::size_t fused_cycle_lds_bytes(int wbytes, const int *ii, const int *jj, int nlev,
+                                  int lds_mode = 1);
 
 // dst (cols x rows) = transpose of src (rows x cols), row-major, per problem of a batch
 template <typename T>

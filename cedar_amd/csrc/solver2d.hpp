@@ -262,6 +262,7 @@ private:
     // fused coarse-level cycle: first level (-1: none), largest side, device level descriptors
     int fuse_lvl_ = -1, fuse_side_ = 0;
     std::size_t fuse_lds_ = 0;  // LDS bytes of the on-chip fused cycle (0: arrays in memory)
+    int fuse_lds_mode_ = 0;     // its lds_mode (k2::coarse_cycle)
     dbuf<unsigned char> fuse64_, fuse32_;
     int *host_active_ = nullptr;   // nb flags (pinned)
     hipEvent_t ev0_{}, ev1_{};
@@ -488,9 +489,18 @@ private:
             ii.push_back(L[l].ext.ii);
             jj.push_back(L[l].ext.jj);
         }
-        const std::size_t need = k2::fused_cycle_lds_bytes((int)sizeof(T), ii.data(), jj.data(),
-                                                           nl - lt);
-        fuse_lds_ = need <= kFusedLdsMax ? need : 0;
+        // every array on chip where they fit, else all but the first level's read-only arrays,
+        // else all but everything of the first level except its x, else none
+        fuse_lds_mode_ = 0;
+        fuse_lds_ = 0;
+        for (int mode = 1; mode <= 3 && !fuse_lds_mode_; ++mode) {
+            const std::size_t need = k2::fused_cycle_lds_bytes((int)sizeof(T), ii.data(),
+                                                               jj.data(), nl - lt, mode);
+            if (need <= kFusedLdsMax) {
+                fuse_lds_mode_ = mode;
+                fuse_lds_ = need;
+            }
+        }
     }
     // of the 160 KiB per workgroup; the kernel's static 8 KiB staging area of the coarsest
     // solve comes on top
@@ -738,7 +748,7 @@ private:
                 k2::coarse_cycle<T>(reinterpret_cast<const k2::coarse_level<T> *>(d.data()),
                                     (int)L.size() - lvl, fuse_side_, set_.relaxation,
                                     set_.nrelax_pre, set_.nrelax_post, abd_.data(), bbd_.data(),
-                                    cur_, L[lvl].ext.nb, rep_, fuse_lds_);
+                                    cur_, L[lvl].ext.nb, rep_, fuse_lds_, fuse_lds_mode_);
                 check_hip(hipGetLastError(), "coarse_cycle");
                 return;
             }
