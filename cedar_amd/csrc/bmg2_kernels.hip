@@ -2392,6 +2392,13 @@ static int launch_shape(int c, const T *so, const T *qf, T *q, const T *sor, int
             CDR_Y(2, 1, 4, N1 / 2, 1);
             CDR_Y(3, 4, 1, N1 / 4, 4);
             CDR_Y(4, 2, 2, N1 / 4, 2);
+            // FP32: one column per workgroup as well (FP64 has it as N1 / 4): the levels of a few
+            // hundred columns are latency-bound, where more, smaller workgroups fill more CUs
+            if constexpr (!F64) {
+                CDR_Y(5, 2, 2, 1, 2);
+                CDR_Y(6, 1, 4, 1, 1);
+                return 7;
+            }
             return 5;
         } else if constexpr (NCH == 8) {
             CDR_Y(0, 8, 1, N1, 4);
@@ -2400,12 +2407,22 @@ static int launch_shape(int c, const T *so, const T *qf, T *q, const T *sor, int
             CDR_Y(3, 2, 4, N1 / 2, 2);
             CDR_Y(4, 8, 1, N1 / 4, 4);
             CDR_Y(5, 4, 2, N1 / 4, 4);
+            if constexpr (!F64) {
+                CDR_Y(6, 4, 2, 1, 4);
+                CDR_Y(7, 2, 4, 1, 2);
+                return 8;
+            }
             return 6;
         } else if constexpr (NCH == 16) {
             CDR_Y(0, 16, 1, N1, 4);
             CDR_Y(1, 8, 2, N1, 4);
             CDR_Y(2, 16, 1, 2 * N1, 2);
             CDR_Y(3, 4, 4, N1 / 2, 4);
+            if constexpr (!F64) {
+                CDR_Y(4, 4, 4, 1, 4);
+                CDR_Y(5, 8, 2, 1, 4);
+                return 6;
+            }
             return 4;
         } else if constexpr (NCH == 32) {
             CDR_Y(0, 16, 2, N1, 2);
