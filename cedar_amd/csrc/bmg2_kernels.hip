@@ -1529,6 +1529,27 @@ __global__ __launch_bounds__(256) void fill_k(T *__restrict__ x, T v, std::size_
         x[k] = v;
 }
 
+// fill of a 16-byte aligned array: 16-byte stores (a quarter of the store instructions of
+// fill_k; the zero start of the FP32 correction at 4095^2: 11.8 against 18.5 us), the tail in
+// scalars
+template <typename T>
+__global__ __launch_bounds__(256) void fill16_k(T *__restrict__ x, T v, std::size_t n) {
+    constexpr int W = 16 / sizeof(T);
+    struct __attribute__((aligned(16))) vec {
+        T e[W];
+    };
+    vec w;
+#pragma unroll
+    for (int u = 0; u < W; ++u) w.e[u] = v;
+    vec *xv = reinterpret_cast<vec *>(x);
+    const std::size_t nv = n / W;
+    for (std::size_t k = (std::size_t)blockIdx.x * blockDim.x + threadIdx.x; k < nv;
+         k += (std::size_t)gridDim.x * blockDim.x)
+        xv[k] = w;
+    const std::size_t t = (std::size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n - nv * W) x[nv * W + t] = v;
+}
+
 __global__ __launch_bounds__(256) void add_correction_k(double *__restrict__ x, const float *__restrict__ e,
                                                         int ii, int jj) {
     const int i = blockIdx.x * 64 + (threadIdx.x % 64) + 1;
@@ -2958,7 +2979,10 @@ void convert(const Tin *in, Tout *out, std::size_t n, hipStream_t s) {
 
 template <typename T>
 void fill(T *x, T v, std::size_t n, hipStream_t s) {
-    fill_k<T><<<grid1(n), 256, 0, s>>>(x, v, n);
+    if (reinterpret_cast<std::uintptr_t>(x) % 16 == 0 && (16 / sizeof(T)) > 1)
+        fill16_k<T><<<grid1(n / (16 / sizeof(T)) + 1), 256, 0, s>>>(x, v, n);
+    else
+        fill_k<T><<<grid1(n), 256, 0, s>>>(x, v, n);
 }
 
 void finalize_sum(const double *partials, std::size_t n, double *out, int nb, hipStream_t s) {
