@@ -81,4 +81,33 @@ __device__ inline void band_solve_upper(const double *__restrict__ abd, int n, i
     }
 }
 
+// The two triangular sweeps of band_solve_upper by one wavefront (lane 0..63, kd <= 63), with x
+// in LDS: as soon as an unknown is final, lanes t = 1..kd apply its column to the kd unknowns
+// after it (U^T sweep) or before it (U sweep).  Each unknown receives the same terms in the same
+// order as in band_solve_upper: the U^T sweep's row sums subtract U(i, c) y_i for increasing i,
+// exactly the order in which the column updates arrive, and its U sweep is column-oriented
+// already; so the result is bitwise that of band_solve_upper.  The dependency chain is one
+// division per unknown instead of a row sum.  LDS accesses of one wavefront complete in issue
+// order, which orders each step's reads after the previous step's writes.
+__device__ inline void band_solve_upper_wave(const double *__restrict__ abd, int n, int kd, int ld,
+                                             double *__restrict__ x, int lane) {
+    auto AB = [&](int r, int c) { return abd[(std::size_t)(c - 1) * ld + (r - 1)]; };
+    const int kp1 = kd + 1;
+    for (int j = 1; j <= n; ++j) {  // U^T y = x
+        const double y = x[j - 1] / AB(kp1, j);
+        const int c = j + lane;
+        if (lane == 0) x[j - 1] = y;
+        if (lane >= 1 && lane <= kd && c <= n) x[c - 1] = x[c - 1] - AB(kp1 - lane, c) * y;
+    }
+    for (int j = n; j >= 1; --j) {  // U x = y
+        const double xj = x[j - 1];
+        if (xj != 0.0) {
+            const double t = xj / AB(kp1, j);
+            const int i = j - lane;
+            if (lane == 0) x[j - 1] = t;
+            if (lane >= 1 && lane <= kd && i >= 1) x[i - 1] = x[i - 1] - t * AB(kp1 - lane, j);
+        }
+    }
+}
+
 }  // namespace cdr

@@ -1493,7 +1493,11 @@ __device__ inline void coarse_band_solve(T *q, const T *qf, const double *abd, d
     }
     for (int kk = tid; kk < n; kk += nt) xx[kk] = (double)qf[pidx(kk % nx + 1, kk / nx + 1, ii)];
     __syncthreads();
-    if (tid == 0) band_solve_upper(ab, n, kd, ld, xx);
+    if (lds && kd < WAVE && nt >= WAVE) {  // bitwise the same sweeps, one wavefront
+        if (tid < WAVE) band_solve_upper_wave(ab, n, kd, ld, xx, tid);
+    } else if (tid == 0) {
+        band_solve_upper(ab, n, kd, ld, xx);
+    }
     __syncthreads();
     for (int kk = tid; kk < n; kk += nt) q[pidx(kk % nx + 1, kk / nx + 1, ii)] = (T)xx[kk];
 }

@@ -917,7 +917,11 @@ __global__ __launch_bounds__(256) void solve_cg3_k(T *__restrict__ q, const T *_
     for (int t = tid; t < n; t += nt)
         xx[t] = (double)Q3(qf, t % nx + 2, (t / nx) % ny + 2, t / (nx * ny) + 2);
     __syncthreads();
-    if (tid == 0) band_solve_upper(ab, n, kd, ld, xx);
+    if (lds && kd < 64 && nt >= 64) {  // bitwise the same sweeps, one wavefront
+        if (tid < 64) band_solve_upper_wave(ab, n, kd, ld, xx, tid);
+    } else if (tid == 0) {
+        band_solve_upper(ab, n, kd, ld, xx);
+    }
     __syncthreads();
     for (int t = tid; t < n; t += nt)
         Q3(q, t % nx + 2, (t / nx) % ny + 2, t / (nx * ny) + 2) = (T)xx[t];
