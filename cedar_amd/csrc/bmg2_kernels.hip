@@ -1746,17 +1746,26 @@ __global__ __launch_bounds__(512) void transpose_pairs_k(const float *__restrict
     const int c0 = blockIdx.x * 128, r0 = blockIdx.y * 64;
     const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
     const int c = c0 + 2 * tx;  // this lane's pair of source columns
-    for (int k = ty; k < 64; k += 8) {
-        const int r = r0 + k;
-        if (r >= rows) continue;
+    // all 8 loads of a thread in flight before the first LDS store
+    float_pair v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int k = ty + 8 * u, r = r0 + k;
         const float *p = src + (std::size_t)r * cols + c;
-        if (c + 1 < cols) {
-            float_pair v;
-            __builtin_memcpy(&v, p, sizeof(v));
-            tile[k][2 * tx] = v.a;
-            tile[k][2 * tx + 1] = v.b;
-        } else if (c < cols) {
-            tile[k][2 * tx] = p[0];
+        v[u] = float_pair{0.f, 0.f};
+        if (r < rows) {
+            if (c + 1 < cols)
+                __builtin_memcpy(&v[u], p, sizeof(float_pair));
+            else if (c < cols)
+                v[u].a = p[0];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int k = ty + 8 * u;
+        if (r0 + k < rows && c < cols) {
+            tile[k][2 * tx] = v[u].a;
+            if (c + 1 < cols) tile[k][2 * tx + 1] = v[u].b;
         }
     }
     __syncthreads();
