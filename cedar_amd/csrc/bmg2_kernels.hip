@@ -1807,11 +1807,21 @@ __global__ __launch_bounds__(256) void residual_qT_k(const T *__restrict__ so,
     qf += bz * P;
     qT += bz * P;
     res += bz * P;
-    // stage: consecutive threads read consecutive j of one row of qT (a column of q)
-    for (int e = threadIdx.x; e < (kTI + 2) * (kTJ + 2); e += 256) {
+    // stage: consecutive threads read consecutive j of one row of qT (a column of q); all loads
+    // of a thread in flight before its LDS stores
+    constexpr int NS = (kTI + 2) * (kTJ + 2), NU = (NS + 255) / 256;
+    T v[NU];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+        const int e = threadIdx.x + 256 * u;
         const int a = e / (kTJ + 2), b = e % (kTJ + 2);
         const int i = i0 - 1 + a, j = j0 - 1 + b;
-        sq[b][a] = (i < ii && j < jj) ? qT[(std::size_t)i * jj + j] : T(0);
+        v[u] = (e < NS && i < ii && j < jj) ? qT[(std::size_t)i * jj + j] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+        const int e = threadIdx.x + 256 * u;
+        if (e < NS) sq[e % (kTJ + 2)][e / (kTJ + 2)] = v[u];
     }
     __syncthreads();
     const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
