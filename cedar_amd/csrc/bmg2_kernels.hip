@@ -2810,7 +2810,7 @@ int tune_lines_ytr(const T *so, const T *soT, int nstncl, const T *qf, T *q, con
     }
     hipStream_t cs = nullptr;
     if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) cs = nullptr;
-    auto timed = [&](auto &&run) {  // run(stream) launches one timed unit on that stream
+    auto timed1 = [&](auto &&run) {  // run(stream) launches one timed unit on that stream
         const float tc = time_captured(cs, s, e0, e1, run);
         if (tc >= 0.f) return tc;
         (void)hipGetLastError();  // eager timing when capture is not available
@@ -2821,6 +2821,10 @@ int tune_lines_ytr(const T *so, const T *soT, int nstncl, const T *qf, T *q, con
         float ms = -1.f;
         if (hipEventSynchronize(e1) == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
         return ms;
+    };
+    auto timed = [&](auto &&run) {  // the faster of two timings
+        const float a = timed1(run), b = timed1(run);
+        return a < 0.f ? b : (b < 0.f ? a : (b < a ? b : a));
     };
     // `sweeps` y sweeps share one right-hand side (one transpose of qf)
     const float tdirect = timed([&](hipStream_t st) {

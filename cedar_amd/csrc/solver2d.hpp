@@ -456,7 +456,13 @@ private:
             for (int c : cand) {
                 build_fused<T>(L, c, dst);
                 k2::fill<T>(L[top].b.p, T(0), L[top].ext.total(), stream_);
-                const float t = time_graph([&] { ncycle<T>(L, top, L[top].x, L[top].b); });
+                // the faster of two timings (one timing alone let the noise pick among
+                // near-equal candidates)
+                float t = -1.f;
+                for (int round = 0; round < 2; ++round) {
+                    const float tr = time_graph([&] { ncycle<T>(L, top, L[top].x, L[top].b); });
+                    if (tr >= 0.f && (t < 0.f || tr < t)) t = tr;
+                }
                 if (t >= 0.f && (tbest < 0.f || t < tbest)) {
                     tbest = t;
                     lt = c;
