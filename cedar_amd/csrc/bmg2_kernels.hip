@@ -667,28 +667,29 @@ __device__ inline T line_rhs(const line_raw<T> &r) {
 // loads the operands of point p (array index) of an x-line (DIR 0) or y-line (DIR 1)
 template <typename T, bool NINE, int DIR>
 __device__ inline void line_load(line_raw<T> &r, const T *__restrict__ so, const T *__restrict__ qf,
-                                 const T *__restrict__ q, std::size_t P, int ii, std::size_t p) {
+                                 const T *__restrict__ q, std::size_t P, int ii, std::size_t p,
+                                 bool zq = false) {
     r.qf = qf[p];
     if (DIR == 0) {
         r.s0 = so[P * KS + p];
-        r.q0 = q[p - ii];
+        r.q0 = zq ? T(0) : q[p - ii];
         r.s1 = so[P * KS + p + ii];
-        r.q1 = q[p + ii];
+        r.q1 = zq ? T(0) : q[p + ii];
     } else {
         r.s0 = so[P * KW + p];
-        r.q0 = q[p - 1];
+        r.q0 = zq ? T(0) : q[p - 1];
         r.s1 = so[P * KW + p + 1];
-        r.q1 = q[p + 1];
+        r.q1 = zq ? T(0) : q[p + 1];
     }
     if (NINE) {
         r.sw0 = so[P * KSW + p];
-        r.q0m = q[p - ii - 1];
+        r.q0m = zq ? T(0) : q[p - ii - 1];
         r.nw0 = so[P * KNW + p + 1];
-        r.q0p = q[p - ii + 1];
+        r.q0p = zq ? T(0) : q[p - ii + 1];
         r.nw1 = so[P * KNW + p + ii];
-        r.q1m = q[p + ii - 1];
+        r.q1m = zq ? T(0) : q[p + ii - 1];
         r.sw1 = so[P * KSW + p + ii + 1];
-        r.q1p = q[p + ii + 1];
+        r.q1p = zq ? T(0) : q[p + ii + 1];
     }
 }
 
@@ -836,6 +837,9 @@ __device__ inline void seg_backward_ab(T (&a)[CPW], T (&b)[CPW], seg_map<T> *xch
 
 // x-lines of one colour, in place: L lines per workgroup, W wavefronts per line.  TR: the grid
 // is the transpose of a y-line problem (right-hand side terms in the y-line order).
+// rev bit 0: the tile walk of xcd_tile; bit 1 (zero start): q holds zeros outside the lines of
+// this colour (a sweep that starts from a zero iterate), so the neighbour rows are not read and
+// literal zeros enter the same arithmetic.
 template <typename T, bool NINE, int CPW, int W, int L, bool TR = false>
 __global__ __launch_bounds__(L *W * 64) void relax_xlines_seg_k(const T *__restrict__ so,
                                                                const T *__restrict__ qf,
@@ -846,8 +850,9 @@ __global__ __launch_bounds__(L *W * 64) void relax_xlines_seg_k(const T *__restr
     __shared__ seg_map<T> xch[2][L][W > 1 ? W * CPW : 1];
     int bx = (int)blockIdx.x, prob = (int)blockIdx.y;
     shared_op_order((int)gridDim.x, rep, bx, prob);  // gridDim.x is a multiple of 8
-    const int tile = xcd_tile(bx, ntiles, rev);
+    const int tile = xcd_tile(bx, ntiles, rev & 1);
     if (tile >= ntiles) return;  // uniform over the workgroup
+    const bool zq = (rev & 2) != 0;
     const int wl = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int slot = wl / W, w = wl % W;
     const int line = tile * L + slot;
@@ -880,7 +885,7 @@ __global__ __launch_bounds__(L *W * 64) void relax_xlines_seg_k(const T *__restr
             const int k = u * WAVE + lane;
             const int kc = min(k, n - 1);
             line_raw<T> r;
-            line_load<T, NINE, 0>(r, so, qf, q, P, ii, row + kc + 1);
+            line_load<T, NINE, 0>(r, so, qf, q, P, ii, row + kc + 1, zq);
             T bu = line_rhs<T, NINE, TR>(r);
             T au = (k > 0) ? -E[max(kc - 1, 0)] : T(0);
             scan_affine_dpp(au, bu);
@@ -905,7 +910,7 @@ __global__ __launch_bounds__(L *W * 64) void relax_xlines_seg_k(const T *__restr
             const int k = (w * CPW + u) * WAVE + lane;
             const int kc = min(k, n - 1);
             line_raw<T> r;
-            line_load<T, NINE, 0>(r, so, qf, q, P, ii, row + kc + 1);
+            line_load<T, NINE, 0>(r, so, qf, q, P, ii, row + kc + 1, zq);
             b[u] = line_rhs<T, NINE, TR>(r);
             a[u] = (k > 0) ? -E[max(kc - 1, 0)] : T(0);
         }
@@ -2592,9 +2597,10 @@ static std::string shape_name(const std::array<int, 6> &k, int c) {
     return k[0] == 2 ? "transposed, " + name : name;
 }
 
+// zq (x-lines of at most 4096 points only): the sweep starts from q = 0 (see relax_xlines_seg_k)
 template <typename T, int DIR>
 static void relax_lines(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
-                        int updown, hipStream_t s, int nb, int rep = 1) {
+                        int updown, hipStream_t s, int nb, int rep = 1, int zq = 0) {
     const int c = tuned_shape(key_of<T>(DIR, nstncl, ii, jj, nb));
     // lines of one colour: DOWN relaxes the lines at even array index (odd Cedar index) first,
     // UP the odd ones first (src/2d/ftn/BMG2_SymStd_relax_lines_x.f90:82-97)
@@ -2604,16 +2610,20 @@ static void relax_lines(const T *so, int nstncl, const T *qf, T *q, const T *sor
         if (first > nfix) continue;
         const int nlines = (nfix - first) / 2 + 1;
         // the second colour walks the lines in the opposite order (see xcd_tile)
-        launch_colour<T, DIR>(c, so, nstncl, qf, q, sor, ii, jj, first, nlines, s, nb, nullptr, t,
-                              rep);
+        const int z = (DIR == 0 && zq && t == 0) ? 2 : 0;
+        launch_colour<T, DIR>(c, so, nstncl, qf, q, sor, ii, jj, first, nlines, s, nb, nullptr,
+                              t | z, rep);
     }
 }
 
 template <typename T>
 void relax_lines_x(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
-                   int updown, hipStream_t s, int nb, int rep) {
-    relax_lines<T, 0>(so, nstncl, qf, q, sor, ii, jj, updown, s, nb, rep);
+                   int updown, hipStream_t s, int nb, int rep, int zero_start) {
+    relax_lines<T, 0>(so, nstncl, qf, q, sor, ii, jj, updown, s, nb, rep,
+                      zero_start && chunk_class(ii - 2, true) != 0);
 }
+
+bool lines_zero_start(int ii) { return chunk_class(ii - 2, true) != 0; }
 
 template <typename T>
 void relax_lines_y(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
@@ -3830,7 +3840,7 @@ void copy(const T *src, T *dst, std::size_t n, hipStream_t s) {
     template void relax_gs<T>(const T *, int, const T *, T *, const T *, int, int, int,           \
                               hipStream_t, int, int);                                             \
     template void relax_lines_x<T>(const T *, int, const T *, T *, const T *, int, int, int,      \
-                                   hipStream_t, int, int);                                        \
+                                   hipStream_t, int, int, int);                                   \
     template void relax_lines_y<T>(const T *, int, const T *, T *, const T *, int, int, int,      \
                                    hipStream_t, int, int);                                        \
     template void relax_lines_xy<T>(const T *, int, const T *, T *, const T *, const T *, T *, int,\

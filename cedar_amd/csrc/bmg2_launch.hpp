@@ -75,9 +75,12 @@ void setup_lines_x(const double *so, int nstncl, double *sor, int ii, int jj, in
                    hipStream_t s, int nb = 1);
 void setup_lines_y(const double *so, int nstncl, double *sor, int ii, int jj, int *info,
                    hipStream_t s, int nb = 1);
+// zero_start: q is zero on entry (not read by the first colour; its zeros are the operands);
+// honoured where lines_zero_start(ii), ignored otherwise
 template <typename T>
 void relax_lines_x(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
-                   int updown, hipStream_t s, int nb = 1, int rep = 1);
+                   int updown, hipStream_t s, int nb = 1, int rep = 1, int zero_start = 0);
+bool lines_zero_start(int ii);
 template <typename T>
 void relax_lines_y(const T *so, int nstncl, const T *qf, T *q, const T *sor, int ii, int jj,
                    int updown, hipStream_t s, int nb = 1, int rep = 1);

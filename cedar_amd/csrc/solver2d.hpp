@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <fstream>
 #include <map>
@@ -202,10 +203,11 @@ public:
             periodic_images(xa, e, nb);
             read_norms();
             for (int p = 0; p < nb; ++p) res0[p] = std::sqrt(host_norm_[p]);
+            const bool zs = zero_start_ok();
             for (int i = 0; i < set_.maxiter; ++i) {
                 run_maybe_graph(g32_, F.x.p, F.b.p, [&] {
-                    k2::fill<float>(F.x.p, 0.0f, e.total(), cur_);
-                    run_cycle<float>(L32_, F.x, F.b, r32T != nullptr);
+                    if (!zs) k2::fill<float>(F.x.p, 0.0f, e.total(), cur_);
+                    run_cycle<float>(L32_, F.x, F.b, r32T != nullptr, zs);
                 });
                 push_mask();
                 k2::residual_ir(A.p, A.nstncl, b_dev, xa, F.x.p, xb, F.b.p, e.ii, e.jj,
@@ -261,6 +263,12 @@ private:
     double *host_norm_ = nullptr;  // nb squared norms (pinned)
     // fused coarse-level cycle: first level (-1: none), largest side, device level descriptors
     int fuse_lvl_ = -1, fuse_side_ = 0;
+    // CEDAR_AMD_ZERO_START=0 at construction turns the zero start of the FP32 cycles off (the
+    // correction is then zero-filled before each cycle; the results are the same bit for bit)
+    bool zs_env_ = [] {
+        const char *v = std::getenv("CEDAR_AMD_ZERO_START");
+        return !(v && v[0] == '0');
+    }();
     std::size_t fuse_lds_ = 0;  // LDS bytes of the on-chip fused cycle (0: arrays in memory)
     int fuse_lds_mode_ = 0;     // its lds_mode (k2::coarse_cycle)
     dbuf<unsigned char> fuse64_, fuse32_;
@@ -646,9 +654,11 @@ private:
     // smoothing whose last sweep is such a y sweep leaves the iterate there (the residual and
     // interp_add of the visit read it in that layout) and an UP smoothing always ends with x
     // current.  Without xT every sweep leaves x current.
+    // zero_start: x is zero on entry to this (DOWN) smoothing, and its first sweep is an x sweep
+    // that reads x only there (see zero_start_ok)
     template <typename T>
     void smooth(level_data<T> &L, grid_func<T> &x, const grid_func<T> &b, Dir d, bool &bT_ok,
-                bool *xT = nullptr) {
+                bool *xT = nullptr, bool zero_start = false) {
         const int nrep = d == Dir::DOWN ? set_.nrelax_pre : set_.nrelax_post;
         for (int r = 0; r < nrep; ++r) {
             switch (set_.relaxation) {
@@ -671,10 +681,17 @@ private:
                                              L.ext.nb);
                             *xT = false;
                         }
-                        check(api2::relax_lines<T>(0, L.A.p, L.A.nstncl, b.p, x.p, L.SOR[0].p,
-                                                   L.ext.ii, L.ext.jj, (int)d, cur_, L.ext.nb,
-                                                   rep_),
-                              "relax_lines_x");
+                        if (zero_start && r == 0 && half == 0) {
+                            k2::relax_lines_x<T>(L.A.p, L.A.nstncl, b.p, x.p, L.SOR[0].p,
+                                                 L.ext.ii, L.ext.jj, (int)d, cur_, L.ext.nb, rep_,
+                                                 1);
+                            check_hip(hipGetLastError(), "relax_lines_x");
+                        } else {
+                            check(api2::relax_lines<T>(0, L.A.p, L.A.nstncl, b.p, x.p,
+                                                       L.SOR[0].p, L.ext.ii, L.ext.jj, (int)d,
+                                                       cur_, L.ext.nb, rep_),
+                                  "relax_lines_x");
+                        }
                     } else {
                         // the last sweep of a DOWN smoothing may leave the iterate transposed
                         const bool last = r == nrep - 1 && half == 1;
@@ -692,10 +709,28 @@ private:
         }
     }
 
+    // The FP32 V-cycle of the mixed solve starts from a zero correction.  Instead of a fill of
+    // the finest level's iterate before each cycle, the first colour of its first x sweep takes
+    // the zeros as literal operands (bitwise the same sweep).  That covers every read of the old
+    // iterate when: line-xy relaxation (the first sweep is an x sweep, red-black by lines, and
+    // the first colour reads only lines of the other colour and the boundary rows), at least one
+    // pre-smoothing sweep, a V-cycle, non-periodic, the finest level not inside the fused coarse
+    // cycle, and odd extents (interp_add then writes no boundary point, so the boundary of the
+    // iterate keeps the zeros of its allocation; every sweep writes interior points only).
+    bool zero_start_ok() const {
+        if (!zs_env_ || set_.relaxation != CDR_RELAX_LINE_XY || jpn_ != 0 ||
+            set_.cycle == CDR_CYCLE_F || set_.nrelax_pre < 1 || L32_.size() < 2)
+            return false;
+        if (fuse_lvl_ == 0 && !timing_ && fuse32_.data()) return false;
+        const extents &e = L32_[0].ext;
+        return (e.ii - 2) % 2 == 1 && (e.jj - 2) % 2 == 1 && k2::lines_zero_start(e.ii);
+    }
+
     // bT_ready: the finest level's transposed right-hand side (L[0].bT_buf) already holds b
+    // zero_start: x is zero on entry (zero_start_ok() holds; the caller skips the fill)
     template <typename T>
     void run_cycle(std::vector<level_data<T>> &L, grid_func<T> &x, const grid_func<T> &b,
-                   bool bT_ready = false) {
+                   bool bT_ready = false, bool zero_start = false) {
         if (L.size() == 1) {
             km_.run<kernels::solve_cg<T>>(x, b, abd_grid(), bbd_.data());
             return;
@@ -703,7 +738,7 @@ private:
         if (set_.cycle == CDR_CYCLE_F)
             fmg<T>(L, 0, x, b);
         else
-            ncycle<T>(L, 0, x, b, bT_ready);
+            ncycle<T>(L, 0, x, b, bT_ready, zero_start);
     }
 
     // the level's transposed right-hand side buffer when its y sweeps run through the transposed
@@ -746,7 +781,7 @@ private:
     // bT_ready: L[lvl].bT_buf already holds the transpose of b (its producer wrote it)
     template <typename T>
     void ncycle(std::vector<level_data<T>> &L, int lvl, grid_func<T> &x, const grid_func<T> &b,
-                bool bT_ready = false) {
+                bool bT_ready = false, bool zero_start = false) {
         if (lvl == fuse_lvl_ && !timing_ && x.p == L[lvl].x.p && b.p == L[lvl].b.p) {
             // this level and all coarser ones in one launch (bitwise the same V-cycle)
             const dbuf<unsigned char> &d = std::is_same<T, double>::value ? fuse64_ : fuse32_;
@@ -766,7 +801,7 @@ private:
         // sweeps (F.xT_buf), read there by the residual and updated there by interp_add
         bool xT = false;
         // timer labels and levels as in include/cedar/cycle/vcycle.h:61-109
-        timed(lvl, "relaxation", [&] { smooth<T>(F, x, b, Dir::DOWN, bT_ok, &xT); });
+        timed(lvl, "relaxation", [&] { smooth<T>(F, x, b, Dir::DOWN, bT_ok, &xT, zero_start); });
         timed(lvl, "residual", [&] {
             if (xT)
                 k2::residual_qT<T>(F.A.p, F.A.nstncl, b.p, F.xT_buf.data(), F.res.p, F.ext.ii,

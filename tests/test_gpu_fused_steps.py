@@ -72,3 +72,50 @@ def test_fused_cycle_steps_equal_unfused(nx, ny, nine, relax, precision, nops, n
         assert np.array_equal(a, c)
         assert a[-1] < 1e-10
     assert torch.equal(x0, x1)
+
+
+ZS_CASES = [
+    # (nx, ny, 9-point, pre-smoothing sweeps, problems, right-hand sides per operator)
+    (255, 255, True, 1, 1, 1),
+    (127, 255, False, 1, 1, 1),
+    (1023, 1023, True, 2, 1, 1),
+    (129, 129, True, 1, 2, 2),
+]
+
+
+@pytest.mark.parametrize("nx,ny,nine,npre,nops,nrhs", ZS_CASES)
+def test_zero_start_equals_zero_filled_start(monkeypatch, nx, ny, nine, npre, nops, nrhs):
+    """The mixed solve's FP32 cycle starts from a zero correction: by default the first colour of
+    the finest level's first x sweep takes the zeros as operands instead of a fill of the
+    correction before each cycle (CEDAR_AMD_ZERO_START=0 at construction keeps the fill).  Both
+    must give the same residual histories and iterates, bit for bit, in graph replays and with
+    the per-level timers on (eager)."""
+    import cedar_amd
+    print(f"zero start {nx}x{ny} nine={nine} npre={npre} nops={nops} nrhs={nrhs}", flush=True)
+    ops, rhs = [], []
+    for k in range(nops):
+        so, b = problem(nx, ny, nine, 21 + k)
+        ops.append(so)
+        for r in range(nrhs):
+            rhs.append(b * (1.0 + 0.5 * r) - 0.02 * r)
+    so = np.stack(ops) if nops > 1 else ops[0]
+    b = np.stack(rhs) if len(rhs) > 1 else rhs[0]
+    kw = dict(relaxation="line-xy", nrelax_pre=npre, nrelax_post=1, maxiter=40, tol=1e-10,
+              precision="mixed")
+    if nrhs > 1:
+        kw["nrhs"] = nrhs
+    out = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("CEDAR_AMD_ZERO_START", flag)
+        for timers in (False, True):
+            s = cedar_amd.Solver(dev(so), **kw)
+            s.timers(timers)
+            x, h = s.solve(dev(b))
+            out.append((x, h if isinstance(h, list) else [h]))
+    x0, h0 = out[0]
+    assert len(h0) == nops * nrhs
+    for x, h in out[1:]:
+        for a, c in zip(h0, h):
+            assert np.array_equal(a, c)
+        assert torch.equal(x0, x)
+    assert all(a[-1] < 1e-9 for a in h0)
