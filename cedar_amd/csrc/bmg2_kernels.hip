@@ -837,10 +837,11 @@ __device__ inline void seg_backward_ab(T (&a)[CPW], T (&b)[CPW], seg_map<T> *xch
 
 // x-lines of one colour, in place: L lines per workgroup, W wavefronts per line.  TR: the grid
 // is the transpose of a y-line problem (right-hand side terms in the y-line order).
-// rev bit 0: the tile walk of xcd_tile; bit 1 (zero start): q holds zeros outside the lines of
-// this colour (a sweep that starts from a zero iterate), so the neighbour rows are not read and
-// literal zeros enter the same arithmetic.
-template <typename T, bool NINE, int CPW, int W, int L, bool TR = false>
+// rev: the tile walk of xcd_tile.  ZQ (zero start): q holds zeros outside the lines of this
+// colour (a sweep that starts from a zero iterate), so the neighbour rows are not read and
+// literal zeros enter the same arithmetic (a compile-time switch: a run-time select in front of
+// the loads slowed the kernel by a quarter).
+template <typename T, bool NINE, int CPW, int W, int L, bool TR = false, bool ZQ = false>
 __global__ __launch_bounds__(L *W * 64) void relax_xlines_seg_k(const T *__restrict__ so,
                                                                const T *__restrict__ qf,
                                                                T *__restrict__ q,
@@ -850,9 +851,8 @@ __global__ __launch_bounds__(L *W * 64) void relax_xlines_seg_k(const T *__restr
     __shared__ seg_map<T> xch[2][L][W > 1 ? W * CPW : 1];
     int bx = (int)blockIdx.x, prob = (int)blockIdx.y;
     shared_op_order((int)gridDim.x, rep, bx, prob);  // gridDim.x is a multiple of 8
-    const int tile = xcd_tile(bx, ntiles, rev & 1);
+    const int tile = xcd_tile(bx, ntiles, rev);
     if (tile >= ntiles) return;  // uniform over the workgroup
-    const bool zq = (rev & 2) != 0;
     const int wl = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int slot = wl / W, w = wl % W;
     const int line = tile * L + slot;
@@ -885,7 +885,7 @@ __global__ __launch_bounds__(L *W * 64) void relax_xlines_seg_k(const T *__restr
             const int k = u * WAVE + lane;
             const int kc = min(k, n - 1);
             line_raw<T> r;
-            line_load<T, NINE, 0>(r, so, qf, q, P, ii, row + kc + 1, zq);
+            line_load<T, NINE, 0>(r, so, qf, q, P, ii, row + kc + 1, ZQ);
             T bu = line_rhs<T, NINE, TR>(r);
             T au = (k > 0) ? -E[max(kc - 1, 0)] : T(0);
             scan_affine_dpp(au, bu);
@@ -910,7 +910,7 @@ __global__ __launch_bounds__(L *W * 64) void relax_xlines_seg_k(const T *__restr
             const int k = (w * CPW + u) * WAVE + lane;
             const int kc = min(k, n - 1);
             line_raw<T> r;
-            line_load<T, NINE, 0>(r, so, qf, q, P, ii, row + kc + 1, zq);
+            line_load<T, NINE, 0>(r, so, qf, q, P, ii, row + kc + 1, ZQ);
             b[u] = line_rhs<T, NINE, TR>(r);
             a[u] = (k > 0) ? -E[max(kc - 1, 0)] : T(0);
         }
@@ -2309,8 +2309,13 @@ static void launch_x(const T *so, const T *qf, T *q, const T *sor, int ii, int j
                      int nlines, hipStream_t s, int nb, int rev, int rep) {
     const int ntiles = (nlines + L - 1) / L;
     const dim3 blocks((unsigned)(8 * ((ntiles + 7) / 8)), (unsigned)nb);
-    relax_xlines_seg_k<T, NINE, CPW, W, L, TR><<<blocks, L * W * 64, 0, s>>>(
-        so, qf, q, sor, ii, jj, first, nlines, ntiles, rev, rep);
+    // rev bit 1: zero start (x-lines of the grid itself only)
+    if (!TR && (rev & 2))
+        relax_xlines_seg_k<T, NINE, CPW, W, L, TR, !TR><<<blocks, L * W * 64, 0, s>>>(
+            so, qf, q, sor, ii, jj, first, nlines, ntiles, rev & 1, rep);
+    else
+        relax_xlines_seg_k<T, NINE, CPW, W, L, TR, false><<<blocks, L * W * 64, 0, s>>>(
+            so, qf, q, sor, ii, jj, first, nlines, ntiles, rev & 1, rep);
 }
 
 template <typename T, bool NINE, int CPW, int W, int NC, int G>
